@@ -1,0 +1,181 @@
+"""Benchmark of the batched cylinder-fitting hot path on one MI355X.
+
+Workload: 2000 synthetic trunk clusters (n = floor(exp(U(ln 50, ln 5000))),
+radius U(0.05, 0.6) m, tilt <= 5 deg, sigma 2 cm, 20 % outliers), 1024 seeded
+RANSAC hypotheses per cluster, 5 cm inlier threshold, LO refit.  A step is one
+batch: seed + score + refit kernels.  Inputs are resident in HBM before the
+timed region.  Prints one JSON line.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cylinder fits/sec + keyframe latency p50/p99 (ms) at ≤1 cm radius err vs plain"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--clusters", type=int, default=2000)
+    ap.add_argument("--hyp", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--hyp-chunk", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(the batch would shard by clusters with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from sloam_amd import synth
+    from sloam_amd.gpu import Context, CylinderFitPlan
+
+    thr, max_radius, seed = 0.05, 1.0, 0x5EED
+    d = synth.make_clusters(args.clusters, seed=1)
+    n_points = int(d["offsets"][-1])
+    ctx = Context(0)
+    plan = CylinderFitPlan(ctx, d["offsets"], n_hyp=args.hyp, thr=thr, max_radius=max_radius,
+                           seed=seed, hyp_chunk=args.hyp_chunk, tile_points=args.tile)
+    dev = torch.device("cuda", 0)
+    xyz = torch.as_tensor(d["xyz"]).to(dev)
+    off = torch.as_tensor(d["offsets"]).to(dev)
+    out = plan.alloc_outputs(counts=False, mask=True, best=True)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        plan.exec(xyz, off, out)
+    ctx.sync()
+
+    # timed region: HIP events on the context stream + per-kernel events
+    stream = ctx.torch_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        plan.exec(xyz, off, out)
+    ev1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.enable_timing(False)
+    gpu_ms = ev0.elapsed_time(ev1)
+    wall_ms = (t1 - t0) * 1e3
+    ms_per_step = gpu_ms / args.steps
+    fits_per_s = args.clusters * args.steps / (gpu_ms / 1e3)
+
+    kern = {}
+    for name in ("cyl_seed", "cyl_score", "cyl_refit"):
+        ms, n = ctx.kernel_time(name)
+        kern[name] = ms / max(n, 1)
+
+    # dominant kernel: cyl_score.  Algorithmic work per (point, hypothesis)
+    # pair: 2 FMA + 2 compares = 6 FP32 ops; bytes: points once (12 B) +
+    # counts (4 B per hypothesis).
+    pairs = n_points * args.hyp
+    score_s = kern["cyl_score"] / 1e3
+    ops = 6.0 * pairs
+    score_bytes = 12.0 * n_points + 4.0 * args.clusters * args.hyp
+    achieved_tflops = ops / score_s / 1e12
+    achieved_gbs = score_bytes / score_s / 1e9
+
+    # accuracy against the generator's truth (informational)
+    ctx.sync()
+    model = out["model"].cpu().numpy()
+    rerr = np.abs(model[:, 6] - d["radius"])
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        # bounded sample: the first 600 clusters (~30 % of the points), one thread
+        ns = min(600, args.clusters)
+        sub_off = d["offsets"][:ns + 1]
+        sub_xyz = d["xyz"][:sub_off[-1]]
+        oracle.load()
+        c0 = time.perf_counter()
+        oracle.cyl_fit_batch(sub_xyz, sub_off, args.hyp, thr=thr, max_radius=max_radius,
+                             seed=seed, counts=False, mask=True, threads=args.cpu_threads)
+        c1 = time.perf_counter()
+        cpu = {
+            "value": ns / (c1 - c0),
+            "unit": "cylinder fits/s",
+            "cores": args.cpu_threads,
+            "kind": "port",
+            "sample": f"first {ns} of the {args.clusters} clusters ({int(sub_off[-1])} points), "
+                      f"{args.hyp} hypotheses, same seed, oracle C++ -O3, "
+                      f"{args.cpu_threads} thread(s); {c1 - c0:.2f} s",
+        }
+
+    line = {
+        "metric": METRIC,
+        "value": fits_per_s,
+        "unit": "cylinder fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched cylinder fitting: {args.clusters} clusters of "
+                        f"log-U[50,5000] points ({n_points} total), {args.hyp} RANSAC "
+                        f"hypotheses each, 5 cm threshold, LO refit",
+            "clusters": args.clusters,
+            "hypotheses": args.hyp,
+            "points": n_points,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "cyl_score",
+            "achieved": achieved_tflops,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "work_per_launch": f"{pairs} point-hypothesis pairs x 6 FP32 ops",
+            "avg_launch_ms": kern["cyl_score"],
+            "hbm_achieved_gbs": achieved_gbs,
+            "hbm_frac": achieved_gbs / HBM_PEAK_GBS,
+        },
+        "kernel_ms": kern,
+        "wall_ms": wall_ms,
+        "radius_err_vs_truth_cm": {"median": float(np.median(rerr) * 100),
+                                   "p90": float(np.percentile(rerr, 90) * 100)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+/*
+ * sloam_gpu.h -- C ABI of the MI355X (gfx950) implementation of SLOAM's
+ * geometric hot path.
+ *
+ * The reference (KumarRobotics/sloam) exposes no FFI for this path: its
+ * boundary is the C++ class API (sloam::sloam::RunSloam, Cylinder, Plane,
+ * seg::Segmentation).  The C++ drop-in classes in sloam_amd/host/ keep those
+ * names and call the entry points below; any other host language binds them
+ * directly (ctypes / cgo / JNI stubs are shown in INTEGRATION.md).
+ *
+ * Conventions
+ *   - every function returns int status: 0 = OK, < 0 = error (sloam_strerror);
+ *     no C++ exception crosses this boundary;
+ *   - "d_" pointers are device (HBM) pointers owned by the caller, "h_" pointers
+ *     are host pointers owned by the caller; scratch memory is owned by the
+ *     context (or by a plan created from it);
+ *   - all work is enqueued on the context's HIP stream and is asynchronous until
+ *     sloam_sync(); a context is not thread-safe (one context per host thread);
+ *   - points are float32 xyz, 12 bytes each, array-of-structs.
+ */
+#ifndef SLOAM_GPU_H_
+#define SLOAM_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SLOAM_OK 0
+#define SLOAM_ERR_INVALID_ARG (-1)
+#define SLOAM_ERR_HIP (-2)
+#define SLOAM_ERR_NO_DEVICE (-3)
+#define SLOAM_ERR_TOO_LARGE (-4)
+#define SLOAM_ERR_ALLOC (-5)
+#define SLOAM_ERR_INTERNAL (-6)
+
+typedef struct sloam_ctx sloam_ctx;
+
+/* ---- context ------------------------------------------------------------
+ * A context owns one HIP stream (default priority) on one device, the
+ * scratch buffers of the entry points below and optional per-kernel timers.
+ * Replaces: the per-process state of sloam::sloam (sloam/include/core/sloam.h:98-106)
+ * as far as device resources go; there is no device state in the reference. */
+int sloam_ctx_create(int device, sloam_ctx** out);
+void sloam_ctx_destroy(sloam_ctx* ctx);
+int sloam_sync(sloam_ctx* ctx);
+const char* sloam_strerror(int status);
+/* Last HIP error string recorded by the context (empty if none). */
+const char* sloam_ctx_last_error(const sloam_ctx* ctx);
+/* The context's hipStream_t, as an opaque pointer (for event timing). */
+void* sloam_ctx_stream(sloam_ctx* ctx);
+/* Per-kernel timing with HIP events on the context stream.  When enabled,
+ * every launch of a named kernel is bracketed by events; after sloam_sync()
+ * sloam_ctx_kernel_time() returns the summed milliseconds and launch count. */
+int sloam_ctx_enable_timing(sloam_ctx* ctx, int enable);
+int sloam_ctx_kernel_time(sloam_ctx* ctx, const char* kernel, double* ms_total, int64_t* launches);
+int sloam_ctx_reset_timing(sloam_ctx* ctx);
+
+/* ---- batched cylinder RANSAC fitting (point-level, fixed hypotheses) ------
+ * For each cluster c (points d_xyz[offsets[c] .. offsets[c+1])):
+ *   1. axis seed: largest-eigenvector of the cluster's point covariance
+ *      (integer-exact sums, FP64 3x3 Jacobi eigen-solve);
+ *   2. n_hyp hypotheses; hypothesis h draws 3 distinct point indices from
+ *      Philox4x32-10(counter = {h, c, stream_tag, 0}, key = seed), projects them
+ *      on the plane orthogonal to the axis and takes their circumcircle
+ *      (centre on the axis, radius r); hypotheses with a degenerate triangle or
+ *      r > max_radius score 0;
+ *   3. inlier predicate | ||(p - c) x a|| - r | < thr, evaluated for every
+ *      (point, hypothesis) pair in FP32 as L < u*A + v*B + s < U (squared
+ *      form, see sloam_amd/csrc/cylfit.hip); counts per hypothesis;
+ *   4. best hypothesis = max count, lowest index on ties; inlier bit mask;
+ *   5. FP64 LO-RANSAC refit: refit_rounds x (refit_iters Gauss-Newton steps
+ *      of the 5-parameter geometric cylinder fit over the current inliers,
+ *      then re-selection |d| < thr under the new model); the first round
+ *      starts from the best hypothesis and its FP32 inliers.
+ * The per-point predicate is the reference's point-to-cylinder distance
+ * (Cylinder::distance(const PointT&), sloam/src/objects/cylinder.cpp:196-203)
+ * with a unit ray; the hypothesis generator has no counterpart in the
+ * reference, whose Cylinder::computeModel (cylinder.cpp:75-173) fits a line to
+ * trellis vertex medians instead.                                            */
+typedef struct sloam_cyl_fit_cfg {
+  int32_t n_hyp;          /* hypotheses per cluster (e.g. 200, 1024)          */
+  float thr;              /* inlier threshold, metres (e.g. 0.05)             */
+  float max_radius;       /* hypotheses with r > max_radius are rejected      */
+  int32_t refit_iters;    /* Gauss-Newton steps per LO refit round            */
+  uint64_t seed;          /* Philox key                                       */
+  int32_t stream_tag;     /* Philox counter word 2 (separates sample streams) */
+  int32_t hyp_chunk;      /* hypotheses per workgroup (0 = autotuned)         */
+  int32_t tile_points;    /* points per LDS tile (0 = autotuned)              */
+  int32_t refit_rounds;   /* LO refit rounds (Gauss-Newton + re-selection)    */
+} sloam_cyl_fit_cfg;
+
+/* Output model record, one per cluster (8 doubles):
+ *   [0..2] root (axis point at the height of the final inlier centroid)
+ *   [3..5] ray  (unit axis; largest |component| positive)
+ *   [6]    radius (-1 if the fit is invalid: fewer than 3 inliers)
+ *   [7]    number of inliers of the refitted model (FP64 |d| < thr)       */
+#define SLOAM_CYL_MODEL_STRIDE 8
+
+typedef struct sloam_cyl_plan sloam_cyl_plan;
+
+/* Plan a batch: uploads the work list (clusters ordered largest first and
+ * split into hypothesis chunks) once; h_offsets has n_clusters+1 entries.  */
+int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
+                          const sloam_cyl_fit_cfg* cfg, sloam_cyl_plan** out);
+void sloam_cyl_plan_destroy(sloam_cyl_plan* plan);
+/* Run a planned batch.  d_offsets must hold the same offsets as h_offsets.
+ * Optional outputs may be NULL:
+ *   d_counts  [n_clusters * n_hyp] int32 inlier count per hypothesis
+ *   d_samples [n_clusters * n_hyp * 3] int32 sampled point indices (cluster-local)
+ *   d_mask    [(n_points + 31) / 32] uint32 inlier bits of the best hypothesis,
+ *             bit i of word w = global point 32*w + i
+ *   d_best    [n_clusters] int32 best hypothesis index (-1 if n < 3)
+ * Required: d_model [n_clusters * SLOAM_CYL_MODEL_STRIDE] double.            */
+int sloam_cyl_plan_exec(sloam_cyl_plan* plan, const float* d_xyz, const int32_t* d_offsets,
+                        int32_t* d_counts, int32_t* d_samples, uint32_t* d_mask,
+                        int32_t* d_best, double* d_model);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLOAM_GPU_H_ */

@@ -1,0 +1,128 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes binding of the CPU oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker / CPU baseline.  The product code under
+sloam_amd/ never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-j8", "-s"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(lib):
+    P = C.c_void_p
+    lib.oracle_philox4x32_10.argtypes = [P, P, P]
+    lib.oracle_draw3.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P]
+    lib.oracle_eig3.argtypes = [P, P, P]
+    lib.oracle_cyl_seed.argtypes = [P, C.c_int, P, P]
+    lib.oracle_cyl_hypothesis.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                          C.c_uint64, C.c_int, P, P, P]
+    lib.oracle_cyl_fit_batch.argtypes = [P, P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                         C.c_int, C.c_uint64, C.c_int, P, P, P, P, P, C.c_int]
+    lib.oracle_cyl_distances_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float,
+                                              C.c_float, C.c_uint64, C.c_int, P]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def philox(ctr, key):
+    lib = load()
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib.oracle_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def draw3(x0, x1, x2, n):
+    out = np.zeros(3, np.int32)
+    st = load().oracle_draw3(x0, x1, x2, n, _p(out))
+    if st:
+        raise ValueError("n < 3")
+    return out
+
+
+def eig3(a6):
+    a = np.ascontiguousarray(a6, np.float64)
+    w = np.zeros(3)
+    V = np.zeros(9)
+    load().oracle_eig3(_p(a), _p(w), _p(V))
+    return w, V.reshape(3, 3)
+
+
+def cyl_seed(xyz):
+    x = np.ascontiguousarray(xyz, np.float32)
+    fr = np.zeros(9, np.float32)
+    ax = np.zeros(3)
+    load().oracle_cyl_seed(_p(x), len(x), _p(fr), _p(ax))
+    return fr, ax
+
+
+def cyl_hypothesis(xyz, cluster, h, thr, max_radius, seed, tag=0):
+    x = np.ascontiguousarray(xyz, np.float32)
+    idx = np.zeros(3, np.int32)
+    circ = np.zeros(4, np.float32)
+    prm = np.zeros(4, np.float32)
+    st = load().oracle_cyl_hypothesis(_p(x), len(x), cluster, h, thr, max_radius, seed, tag,
+                                      _p(idx), _p(circ), _p(prm))
+    if st:
+        raise ValueError("cluster too small")
+    return idx, circ, prm
+
+
+def cyl_fit_batch(xyz, offsets, n_hyp, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=3,
+                  seed=0x5EED,
+                  tag=0, counts=True, samples=False, mask=True, threads=1):
+    x = np.ascontiguousarray(xyz, np.float32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    C_ = len(off) - 1
+    N = int(off[-1])
+    out = {
+        "model": np.zeros((C_, 8)),
+        "best": np.zeros(C_, np.int32),
+    }
+    if counts:
+        out["counts"] = np.zeros((C_, n_hyp), np.int32)
+    if samples:
+        out["samples"] = np.zeros((C_, n_hyp, 3), np.int32)
+    if mask:
+        out["mask"] = np.zeros((N + 31) // 32, np.uint32)
+    st = load().oracle_cyl_fit_batch(_p(x), _p(off), C_, n_hyp, thr, max_radius, refit_rounds,
+                                     refit_iters,
+                                     seed, tag, _p(out.get("counts")), _p(out.get("samples")),
+                                     _p(out.get("mask")), _p(out["best"]), _p(out["model"]),
+                                     threads)
+    if st:
+        raise ValueError("oracle_cyl_fit_batch failed")
+    return out
+
+
+def cyl_distances_fp64(xyz, cluster, h, thr, max_radius, seed, tag=0):
+    x = np.ascontiguousarray(xyz, np.float32)
+    d = np.zeros(len(x))
+    st = load().oracle_cyl_distances_fp64(_p(x), len(x), cluster, h, thr, max_radius, seed, tag,
+                                          _p(d))
+    return (d if st == 0 else None)

@@ -1,0 +1,84 @@
+"""ctypes binding of libsloam_gpu.so (the C ABI declared in include/sloam_gpu.h).
+
+The library is required: importing this module raises if it has not been built
+(python -m sloam_amd.build).  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsloam_gpu.so")
+
+# mirrors of the C structs --------------------------------------------------
+
+
+class CylFitCfg(C.Structure):
+    _fields_ = [
+        ("n_hyp", C.c_int32),
+        ("thr", C.c_float),
+        ("max_radius", C.c_float),
+        ("refit_iters", C.c_int32),
+        ("seed", C.c_uint64),
+        ("stream_tag", C.c_int32),
+        ("hyp_chunk", C.c_int32),
+        ("tile_points", C.c_int32),
+        ("refit_rounds", C.c_int32),
+    ]
+
+
+CYL_MODEL_STRIDE = 8
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+
+# name -> (restype, argtypes); every exported symbol of include/sloam_gpu.h
+PROTOTYPES = {
+    "sloam_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "sloam_ctx_destroy": (None, [_P]),
+    "sloam_sync": (C.c_int, [_P]),
+    "sloam_strerror": (C.c_char_p, [C.c_int]),
+    "sloam_ctx_last_error": (C.c_char_p, [_P]),
+    "sloam_ctx_stream": (_P, [_P]),
+    "sloam_ctx_enable_timing": (C.c_int, [_P, C.c_int]),
+    "sloam_ctx_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_I64)]),
+    "sloam_ctx_reset_timing": (C.c_int, [_P]),
+    "sloam_cyl_plan_create": (C.c_int, [_P, _P, _I32, C.POINTER(CylFitCfg), C.POINTER(_P)]),
+    "sloam_cyl_plan_destroy": (None, [_P]),
+    "sloam_cyl_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads the library once and declares every prototype."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m sloam_amd.build` "
+            "(the HIP path has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SloamError(RuntimeError):
+    pass
+
+
+def check(status, ctx=None):
+    if status != 0:
+        lib = load()
+        msg = lib.sloam_strerror(status).decode()
+        if ctx is not None:
+            detail = lib.sloam_ctx_last_error(ctx)
+            if detail:
+                msg += f" ({detail.decode()})"
+        raise SloamError(f"sloam status {status}: {msg}")

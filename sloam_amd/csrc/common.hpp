@@ -1,0 +1,254 @@
+// Shared declarations of the gfx950 kernels and the C-ABI layer.
+//
+// Everything here is compiled with -ffp-contract=off: the FP32 hypothesis and
+// inlier arithmetic is written operation by operation (explicit fmaf where a
+// fused multiply-add is meant) so that results do not depend on the compiler's
+// contraction choices.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "sloam_gpu.h"
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct SloamTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct sloam_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  bool timing = false;
+  std::map<std::string, SloamTimer> timers;
+  std::vector<hipEvent_t> event_pool;
+};
+
+// Records a HIP failure in the context and returns SLOAM_ERR_HIP.
+int sloam_record_hip_error(sloam_ctx* ctx, hipError_t e, const char* what);
+
+#define SLOAM_HIP(ctx, call)                                            \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess) return sloam_record_hip_error((ctx), e_, #call); \
+  } while (0)
+
+// Brackets one kernel launch with events when timing is enabled.
+struct SloamKernelScope {
+  sloam_ctx* ctx;
+  const char* name;
+  hipEvent_t start = nullptr;
+  SloamKernelScope(sloam_ctx* c, const char* n);
+  ~SloamKernelScope();
+};
+
+// Device allocation helpers that report through the context.
+int sloam_dev_alloc(sloam_ctx* ctx, void** p, size_t bytes);
+void sloam_dev_free(void* p);
+
+// Fill kernel (used instead of hipMemsetAsync so that a captured graph holds
+// kernel nodes only).
+int sloam_fill_u32(sloam_ctx* ctx, uint32_t* d, size_t n, uint32_t value);
+
+// ---------------------------------------------------------------------------
+// counter-based random numbers: Philox4x32-10
+// J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11.  Round multipliers and Weyl key
+// increments are the published constants of the 4x32 variant.
+// ---------------------------------------------------------------------------
+struct SlU4 {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline void sl_mulhilo(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  *lo = (uint32_t)p;
+}
+
+__host__ __device__ inline SlU4 sl_philox4x32_10(SlU4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    sl_mulhilo(0xD2511F53u, c.x, &hi0, &lo0);
+    sl_mulhilo(0xCD9E8D57u, c.z, &hi1, &lo1);
+    SlU4 o;
+    o.x = hi1 ^ c.y ^ k0;
+    o.y = lo1;
+    o.z = hi0 ^ c.w ^ k1;
+    o.w = lo0;
+    c = o;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Uniform integer in [0, m) from a 32-bit word: floor(x * m / 2^32).
+__host__ __device__ inline uint32_t sl_bounded(uint32_t x, uint32_t m) {
+  return (uint32_t)(((uint64_t)x * (uint64_t)m) >> 32);
+}
+
+// Three distinct indices in [0, n), n >= 3, from three random words.
+__host__ __device__ inline void sl_draw3(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t n,
+                                         int32_t* i0, int32_t* i1, int32_t* i2) {
+  uint32_t a = sl_bounded(x0, n);
+  uint32_t b = sl_bounded(x1, n - 1u);
+  if (b >= a) b += 1u;
+  uint32_t lo = a < b ? a : b;
+  uint32_t hi = a < b ? b : a;
+  uint32_t c = sl_bounded(x2, n - 2u);
+  if (c >= lo) c += 1u;
+  if (c >= hi) c += 1u;
+  *i0 = (int32_t)a;
+  *i1 = (int32_t)b;
+  *i2 = (int32_t)c;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 symmetric 3x3 eigen-solver (cyclic Jacobi).  Only +,-,*,/,sqrt, which
+// are correctly rounded on gfx950 and on the host, so the result is the same
+// on both for the same input.
+// a = {a00, a01, a02, a11, a12, a22}; on return w = eigenvalues and the
+// columns of V (row-major V[3*r + c]) the eigenvectors.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void sl_jacobi_rotate(double A[3][3], double V[3][3], int p, int q) {
+  const double apq = A[p][q];
+  if (apq == 0.0) return;
+  const double theta = 0.5 * (A[q][q] - A[p][p]) / apq;
+  double t;
+  const double at = theta < 0.0 ? -theta : theta;
+  if (at > 1.0e100) {
+    t = 0.5 / theta;
+  } else {
+    t = 1.0 / (at + sqrt(1.0 + theta * theta));
+    if (theta < 0.0) t = -t;
+  }
+  const double c = 1.0 / sqrt(1.0 + t * t);
+  const double s = t * c;
+  const double tau = s / (1.0 + c);
+  const double h = t * apq;
+  A[p][p] = A[p][p] - h;
+  A[q][q] = A[q][q] + h;
+  A[p][q] = 0.0;
+  A[q][p] = 0.0;
+  for (int r = 0; r < 3; ++r) {
+    if (r == p || r == q) continue;
+    const double g = A[r][p];
+    const double hh = A[r][q];
+    const double np = g - s * (hh + g * tau);
+    const double nq = hh + s * (g - hh * tau);
+    A[r][p] = np;
+    A[p][r] = np;
+    A[r][q] = nq;
+    A[q][r] = nq;
+  }
+  for (int r = 0; r < 3; ++r) {
+    const double g = V[r][p];
+    const double hh = V[r][q];
+    V[r][p] = g - s * (hh + g * tau);
+    V[r][q] = hh + s * (g - hh * tau);
+  }
+}
+
+__host__ __device__ inline double sl_fabs(double x) { return x < 0.0 ? -x : x; }
+
+__host__ __device__ inline void sl_eig3(const double a[6], double w[3], double V[3][3]) {
+  double A[3][3] = {{a[0], a[1], a[2]}, {a[1], a[3], a[4]}, {a[2], a[4], a[5]}};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) V[r][c] = (r == c) ? 1.0 : 0.0;
+  const double scale = sl_fabs(a[0]) + sl_fabs(a[3]) + sl_fabs(a[5]) +
+                       2.0 * (sl_fabs(a[1]) + sl_fabs(a[2]) + sl_fabs(a[4]));
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = sl_fabs(A[0][1]) + sl_fabs(A[0][2]) + sl_fabs(A[1][2]);
+    if (!(off > 1.0e-17 * scale)) break;
+    sl_jacobi_rotate(A, V, 0, 1);
+    sl_jacobi_rotate(A, V, 0, 2);
+    sl_jacobi_rotate(A, V, 1, 2);
+  }
+  w[0] = A[0][0];
+  w[1] = A[1][1];
+  w[2] = A[2][2];
+}
+
+// Eigenvector column k of V, sign-normalised so that its largest-magnitude
+// component is positive (lowest index on ties).
+__host__ __device__ inline void sl_eigvec(const double V[3][3], int k, double out[3]) {
+  out[0] = V[0][k];
+  out[1] = V[1][k];
+  out[2] = V[2][k];
+  int im = 0;
+  double best = sl_fabs(out[0]);
+  if (sl_fabs(out[1]) > best) { im = 1; best = sl_fabs(out[1]); }
+  if (sl_fabs(out[2]) > best) { im = 2; }
+  if (out[im] < 0.0) {
+    out[0] = -out[0];
+    out[1] = -out[1];
+    out[2] = -out[2];
+  }
+}
+
+__host__ __device__ inline int sl_argmax3(const double w[3]) {
+  int k = 0;
+  if (w[1] > w[k]) k = 1;
+  if (w[2] > w[k]) k = 2;
+  return k;
+}
+
+__host__ __device__ inline int sl_argmin3(const double w[3]) {
+  int k = 0;
+  if (w[1] < w[k]) k = 1;
+  if (w[2] < w[k]) k = 2;
+  return k;
+}
+
+// Orthonormal basis (e1, e2) of the plane orthogonal to the unit vector a:
+// e1 = normalise(r - (r.a) a) with r the coordinate axis least aligned with a
+// (lowest index on ties), e2 = a x e1.
+__host__ __device__ inline void sl_basis(const double a[3], double e1[3], double e2[3]) {
+  int k = 0;
+  double best = sl_fabs(a[0]);
+  if (sl_fabs(a[1]) < best) { k = 1; best = sl_fabs(a[1]); }
+  if (sl_fabs(a[2]) < best) { k = 2; }
+  double r[3] = {0.0, 0.0, 0.0};
+  r[k] = 1.0;
+  const double d = a[k];
+  double x = r[0] - d * a[0];
+  double y = r[1] - d * a[1];
+  double z = r[2] - d * a[2];
+  const double nrm = sqrt(x * x + y * y + z * z);
+  x = x / nrm;
+  y = y / nrm;
+  z = z / nrm;
+  e1[0] = x;
+  e1[1] = y;
+  e1[2] = z;
+  e2[0] = a[1] * z - a[2] * y;
+  e2[1] = a[2] * x - a[0] * z;
+  e2[2] = a[0] * y - a[1] * x;
+}
+
+// ---------------------------------------------------------------------------
+// wave / block reductions
+// ---------------------------------------------------------------------------
+__device__ inline double sl_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline long long sl_wave_sum_i64(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}

@@ -1,0 +1,179 @@
+// Context, error reporting, timers and small utility kernels of the C ABI.
+#include "common.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+int sloam_record_hip_error(sloam_ctx* ctx, hipError_t e, const char* what) {
+  if (ctx) {
+    ctx->last_error = std::string(what) + ": " + hipGetErrorString(e);
+  }
+  return SLOAM_ERR_HIP;
+}
+
+static hipEvent_t pool_get(sloam_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+SloamKernelScope::SloamKernelScope(sloam_ctx* c, const char* n) : ctx(c), name(n) {
+  if (ctx && ctx->timing) {
+    start = pool_get(ctx);
+    if (start) hipEventRecord(start, ctx->stream);
+  }
+}
+
+SloamKernelScope::~SloamKernelScope() {
+  if (!start) return;
+  hipEvent_t stop = pool_get(ctx);
+  if (!stop) return;
+  hipEventRecord(stop, ctx->stream);
+  ctx->timers[name].pending.emplace_back(start, stop);
+}
+
+int sloam_dev_alloc(sloam_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return SLOAM_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    sloam_record_hip_error(ctx, e, "hipMalloc");
+    return SLOAM_ERR_ALLOC;
+  }
+  return SLOAM_OK;
+}
+
+void sloam_dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+__global__ void sl_fill_u32_kernel(uint32_t* __restrict__ d, size_t n, uint32_t value) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] = value;
+}
+
+int sloam_fill_u32(sloam_ctx* ctx, uint32_t* d, size_t n, uint32_t value) {
+  if (n == 0) return SLOAM_OK;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(sl_fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d, n,
+                     value);
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
+extern "C" {
+
+int sloam_ctx_create(int device, sloam_ctx** out) {
+  if (!out) return SLOAM_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SLOAM_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return SLOAM_ERR_INVALID_ARG;
+  sloam_ctx* ctx = new (std::nothrow) sloam_ctx();
+  if (!ctx) return SLOAM_ERR_ALLOC;
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return SLOAM_ERR_HIP;
+  }
+  *out = ctx;
+  return SLOAM_OK;
+}
+
+void sloam_ctx_destroy(sloam_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->timers)
+    for (auto& pr : kv.second.pending) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int sloam_sync(sloam_ctx* ctx) {
+  if (!ctx) return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SLOAM_OK;
+}
+
+const char* sloam_strerror(int status) {
+  switch (status) {
+    case SLOAM_OK: return "ok";
+    case SLOAM_ERR_INVALID_ARG: return "invalid argument";
+    case SLOAM_ERR_HIP: return "HIP runtime error";
+    case SLOAM_ERR_NO_DEVICE: return "no HIP device";
+    case SLOAM_ERR_TOO_LARGE: return "input exceeds a supported maximum";
+    case SLOAM_ERR_ALLOC: return "device allocation failed";
+    case SLOAM_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+const char* sloam_ctx_last_error(const sloam_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : "";
+}
+
+void* sloam_ctx_stream(sloam_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int sloam_ctx_enable_timing(sloam_ctx* ctx, int enable) {
+  if (!ctx) return SLOAM_ERR_INVALID_ARG;
+  ctx->timing = enable != 0;
+  return SLOAM_OK;
+}
+
+static int drain_timer(sloam_ctx* ctx, SloamTimer& t) {
+  for (auto& pr : t.pending) {
+    SLOAM_HIP(ctx, hipEventSynchronize(pr.second));
+    float ms = 0.f;
+    SLOAM_HIP(ctx, hipEventElapsedTime(&ms, pr.first, pr.second));
+    t.ms += ms;
+    t.launches += 1;
+    ctx->event_pool.push_back(pr.first);
+    ctx->event_pool.push_back(pr.second);
+  }
+  t.pending.clear();
+  return SLOAM_OK;
+}
+
+int sloam_ctx_kernel_time(sloam_ctx* ctx, const char* kernel, double* ms_total, int64_t* launches) {
+  if (!ctx || !kernel) return SLOAM_ERR_INVALID_ARG;
+  auto it = ctx->timers.find(kernel);
+  if (it == ctx->timers.end()) {
+    if (ms_total) *ms_total = 0.0;
+    if (launches) *launches = 0;
+    return SLOAM_OK;
+  }
+  int st = drain_timer(ctx, it->second);
+  if (st) return st;
+  if (ms_total) *ms_total = it->second.ms;
+  if (launches) *launches = it->second.launches;
+  return SLOAM_OK;
+}
+
+int sloam_ctx_reset_timing(sloam_ctx* ctx) {
+  if (!ctx) return SLOAM_ERR_INVALID_ARG;
+  for (auto& kv : ctx->timers) {
+    int st = drain_timer(ctx, kv.second);
+    if (st) return st;
+    kv.second.ms = 0.0;
+    kv.second.launches = 0;
+  }
+  return SLOAM_OK;
+}
+
+}  // extern "C"

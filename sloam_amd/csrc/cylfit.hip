@@ -1,0 +1,781 @@
+// Batched point-level cylinder RANSAC for gfx950.
+//
+// Three kernels per batch (all on the context stream):
+//   cyl_seed    one workgroup per cluster: bounding-box centre m, integer-exact
+//               covariance sums, FP64 Jacobi eigen-solve -> seed axis and an
+//               FP32 basis (e1, e2) of the plane orthogonal to it.
+//   cyl_score   one workgroup per (cluster, hypothesis chunk), clusters largest
+//               first: builds the chunk's hypotheses into LDS, streams the
+//               cluster's points through LDS tiles as (u, v, s = u^2 + v^2) and
+//               counts inliers with a wavefront ballot + popcount, 64 points per
+//               compare, eight hypotheses per pass over a tile.
+//   cyl_refit   one workgroup per cluster: best hypothesis (max count, lowest
+//               index), inlier bit mask, FP64 refit (PCA axis, algebraic circle,
+//               Gauss-Newton geometric circle).
+//
+// The inlier predicate is the reference's point-to-cylinder distance
+// (sloam/src/objects/cylinder.cpp:196-203, Cylinder::distance(const PointT&)):
+// | ||(p - c) x a|| - r | < thr for a unit axis a.  With (u, v) the coordinates
+// of p in the plane orthogonal to a and (cu, cv) the circle centre,
+// ||(p - c) x a||^2 = (u - cu)^2 + (v - cv)^2 = s + A u + B v + K with
+// A = -2 cu, B = -2 cv, K = cu^2 + cv^2, so the test becomes
+// L < fma(u, A, fma(v, B, s)) < U with L = (r - thr)^2 - K (or -inf when
+// r <= thr) and U = (r + thr)^2 - K: two FMAs and two compares per pair.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <numeric>
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kHB = 8;  // hypotheses held in registers per pass over a tile
+constexpr int kMaxLds = 65536;  // LDS bytes per workgroup this kernel asks for
+
+struct CylFrame {
+  float m[3];
+  float e1[3];
+  float e2[3];
+  int32_t n;
+  double axis[3];
+  double reserved;
+};
+
+struct HypOut {
+  float A, B, L, U;  // predicate parameters
+  float cu, cv, r;   // circle in the local frame
+  int valid;
+};
+
+__device__ inline void local_uvs(const CylFrame& f, float x, float y, float z, float* u, float* v,
+                                 float* s) {
+  const float dx = x - f.m[0];
+  const float dy = y - f.m[1];
+  const float dz = z - f.m[2];
+  const float uu = (dx * f.e1[0] + dy * f.e1[1]) + dz * f.e1[2];
+  const float vv = (dx * f.e2[0] + dy * f.e2[1]) + dz * f.e2[2];
+  *u = uu;
+  *v = vv;
+  *s = uu * uu + vv * vv;
+}
+
+// Circumcircle of three points of the local plane and the predicate parameters.
+__device__ inline HypOut make_hyp(float ua, float va, float ub, float vb, float uc, float vc,
+                                  float thr, float max_radius) {
+  HypOut h;
+  const float bx = ub - ua;
+  const float by = vb - va;
+  const float cx = uc - ua;
+  const float cy = vc - va;
+  const float d = 2.0f * (bx * cy - by * cx);
+  const float b2 = bx * bx + by * by;
+  const float c2 = cx * cx + cy * cy;
+  const float ox = (cy * b2 - by * c2) / d;
+  const float oy = (bx * c2 - cx * b2) / d;
+  const float r = sqrtf(ox * ox + oy * oy);
+  h.cu = ua + ox;
+  h.cv = va + oy;
+  h.r = r;
+  h.valid = (r <= max_radius) ? 1 : 0;  // false for NaN / inf (degenerate triangle)
+  if (h.valid) {
+    h.A = -2.0f * h.cu;
+    h.B = -2.0f * h.cv;
+    const float K = h.cu * h.cu + h.cv * h.cv;
+    const float hi = r + thr;
+    h.U = hi * hi - K;
+    const float lo = r - thr;
+    h.L = lo > 0.0f ? lo * lo - K : -INFINITY;
+  } else {
+    h.A = 0.0f;
+    h.B = 0.0f;
+    h.L = INFINITY;
+    h.U = -INFINITY;
+  }
+  return h;
+}
+
+__device__ inline HypOut hyp_for(const float* __restrict__ P, int n, int c, int h,
+                                 const CylFrame& f, float thr, float max_radius, uint32_t k0,
+                                 uint32_t k1, uint32_t tag, int32_t idx[3]) {
+  SlU4 ctr = {(uint32_t)h, (uint32_t)c, tag, 0u};
+  const SlU4 x = sl_philox4x32_10(ctr, k0, k1);
+  sl_draw3(x.x, x.y, x.z, (uint32_t)n, &idx[0], &idx[1], &idx[2]);
+  float u[3], v[3], s;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float* q = P + 3 * (size_t)idx[k];
+    local_uvs(f, q[0], q[1], q[2], &u[k], &v[k], &s);
+  }
+  return make_hyp(u[0], v[0], u[1], v[1], u[2], v[2], thr, max_radius);
+}
+
+__device__ inline bool is_inlier(float u, float v, float s, const HypOut& h) {
+  const float t = __builtin_fmaf(u, h.A, __builtin_fmaf(v, h.B, s));
+  return (t > h.L) && (t < h.U);
+}
+
+// Sums K doubles over the block; every thread receives the totals.  The
+// cross-wave order is fixed (wave 0..3), lanes are combined by xor-shuffles.
+template <int K>
+__device__ inline void block_sum(double (&v)[K], double* lds) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = sl_wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double t = lds[k];
+    for (int w = 1; w < kWaves; ++w) t += lds[w * K + k];
+    v[k] = t;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// seed: bounding-box centre, integer covariance, FP64 Jacobi, basis
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void cyl_seed_kernel(const float* __restrict__ xyz,
+                                                          const int32_t* __restrict__ offsets,
+                                                          int n_clusters,
+                                                          CylFrame* __restrict__ frames) {
+  __shared__ float red_f[kWaves][6];
+  __shared__ long long red_i[kWaves][9];
+  const int c = blockIdx.x;
+  if (c >= n_clusters) return;
+  const int off = offsets[c];
+  const int n = offsets[c + 1] - off;
+  const float* P = xyz + 3 * (size_t)off;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float q = P[3 * (size_t)i + k];
+      mn[k] = fminf(mn[k], q);
+      mx[k] = fmaxf(mx[k], q);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+    }
+  }
+  if (lane == 0) {
+    for (int k = 0; k < 3; ++k) {
+      red_f[wave][k] = mn[k];
+      red_f[wave][3 + k] = mx[k];
+    }
+  }
+  __syncthreads();
+  float m[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float a = red_f[0][k], b = red_f[0][3 + k];
+    for (int w = 1; w < kWaves; ++w) {
+      a = fminf(a, red_f[w][k]);
+      b = fmaxf(b, red_f[w][3 + k]);
+    }
+    m[k] = n > 0 ? 0.5f * (a + b) : 0.0f;
+  }
+
+  // integer-exact sums of q = rint((p - m) * 2^16), |q| < 2^23
+  long long s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    long long q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double d = ((double)P[3 * (size_t)i + k] - (double)m[k]) * 65536.0;
+      d = rint(d);
+      if (d > 8388607.0) d = 8388607.0;
+      if (d < -8388607.0) d = -8388607.0;
+      q[k] = (long long)d;
+    }
+    s[0] += q[0];
+    s[1] += q[1];
+    s[2] += q[2];
+    s[3] += q[0] * q[0];
+    s[4] += q[0] * q[1];
+    s[5] += q[0] * q[2];
+    s[6] += q[1] * q[1];
+    s[7] += q[1] * q[2];
+    s[8] += q[2] * q[2];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) s[k] = sl_wave_sum_i64(s[k]);
+  if (lane == 0)
+    for (int k = 0; k < 9; ++k) red_i[wave][k] = s[k];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  long long t[9];
+  for (int k = 0; k < 9; ++k) {
+    t[k] = red_i[0][k];
+    for (int w = 1; w < kWaves; ++w) t[k] += red_i[w][k];
+  }
+  CylFrame f;
+  f.m[0] = m[0];
+  f.m[1] = m[1];
+  f.m[2] = m[2];
+  f.n = n;
+  f.reserved = 0.0;
+  double axis[3] = {0.0, 0.0, 1.0};
+  if (n >= 2) {
+    const double dn = (double)n;
+    const double S0 = (double)t[0], S1 = (double)t[1], S2 = (double)t[2];
+    double a6[6];
+    a6[0] = (double)t[3] - S0 * S0 / dn;
+    a6[1] = (double)t[4] - S0 * S1 / dn;
+    a6[2] = (double)t[5] - S0 * S2 / dn;
+    a6[3] = (double)t[6] - S1 * S1 / dn;
+    a6[4] = (double)t[7] - S1 * S2 / dn;
+    a6[5] = (double)t[8] - S2 * S2 / dn;
+    double w[3], V[3][3];
+    sl_eig3(a6, w, V);
+    sl_eigvec(V, sl_argmax3(w), axis);
+  }
+  double e1[3], e2[3];
+  sl_basis(axis, e1, e2);
+  for (int k = 0; k < 3; ++k) {
+    f.axis[k] = axis[k];
+    f.e1[k] = (float)e1[k];
+    f.e2[k] = (float)e2[k];
+  }
+  frames[c] = f;
+}
+
+// ---------------------------------------------------------------------------
+// score: hypotheses x points, ballot/popcount inlier counts
+// work item = {cluster, first hypothesis, hypothesis count, unused}
+// LDS: hyp params [hcap] float4 | counts [hcap] int | u,v,s [tile] float
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void cyl_score_kernel(
+    const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
+    const CylFrame* __restrict__ frames, const int4* __restrict__ work, int n_hyp, int hcap,
+    int tile, float thr, float max_radius, uint32_t k0, uint32_t k1, uint32_t tag,
+    int32_t* __restrict__ counts, int32_t* __restrict__ samples) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float4* hp = reinterpret_cast<float4*>(smem_raw);
+  int* cnt = reinterpret_cast<int*>(hp + hcap);
+  float* su = reinterpret_cast<float*>(cnt + hcap);
+  float* sv = su + tile;
+  float* ss = sv + tile;
+
+  const int4 w = work[blockIdx.x];
+  const int c = w.x, h0 = w.y, hc = w.z;
+  const int off = offsets[c];
+  const int n = offsets[c + 1] - off;
+  const float* P = xyz + 3 * (size_t)off;
+  const CylFrame f = frames[c];
+
+  for (int t = threadIdx.x; t < hcap; t += kBlock) {
+    float4 prm = make_float4(0.f, 0.f, INFINITY, -INFINITY);
+    if (t < hc && n >= 3) {
+      int32_t idx[3];
+      const HypOut h = hyp_for(P, n, c, h0 + t, f, thr, max_radius, k0, k1, tag, idx);
+      prm = make_float4(h.A, h.B, h.L, h.U);
+      if (samples) {
+        int32_t* o = samples + 3 * ((size_t)c * n_hyp + h0 + t);
+        o[0] = idx[0];
+        o[1] = idx[1];
+        o[2] = idx[2];
+      }
+    } else if (t < hc && samples) {
+      int32_t* o = samples + 3 * ((size_t)c * n_hyp + h0 + t);
+      o[0] = -1;
+      o[1] = -1;
+      o[2] = -1;
+    }
+    hp[t] = prm;
+    cnt[t] = 0;
+  }
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int hpw = hcap / kWaves;  // hypotheses per wave (multiple of kHB)
+
+  for (int t0 = 0; t0 < n; t0 += tile) {
+    const int tn = min(tile, n - t0);
+    const int tn64 = (tn + 63) & ~63;
+    __syncthreads();
+    for (int i = threadIdx.x; i < tn64; i += kBlock) {
+      float u = 0.f, v = 0.f, s = NAN;
+      if (i < tn) {
+        const float* q = P + 3 * (size_t)(t0 + i);
+        local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
+      }
+      su[i] = u;
+      sv[i] = v;
+      ss[i] = s;
+    }
+    __syncthreads();
+    for (int hb = 0; hb < hpw; hb += kHB) {
+      const int hbase = wave * hpw + hb;
+      if (hbase >= hc) break;
+      float A[kHB], B[kHB], L[kHB], U[kHB];
+      int cc[kHB];
+#pragma unroll
+      for (int j = 0; j < kHB; ++j) {
+        const float4 q = hp[hbase + j];
+        A[j] = q.x;
+        B[j] = q.y;
+        L[j] = q.z;
+        U[j] = q.w;
+        cc[j] = 0;
+      }
+      // uniform trip count: i0 lives in a scalar register, the two compares
+      // write scalar lane masks that are and-ed and popcounted on the SALU
+      for (int i0 = 0; i0 < tn64; i0 += 64) {
+        const int i = i0 + lane;
+        const float u = su[i];
+        const float v = sv[i];
+        const float s = ss[i];
+#pragma unroll
+        for (int j = 0; j < kHB; ++j) {
+          const float t = __builtin_fmaf(u, A[j], __builtin_fmaf(v, B[j], s));
+          const uint64_t gt = __builtin_amdgcn_ballot_w64(t > L[j]);
+          const uint64_t lt = __builtin_amdgcn_ballot_w64(t < U[j]);
+          cc[j] += __builtin_popcountll(gt & lt);
+        }
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < kHB; ++j) cnt[hbase + j] += cc[j];
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < hc; t += kBlock) counts[(size_t)c * n_hyp + h0 + t] = cnt[t];
+}
+
+struct CylModel {
+  double a[3], c[3], r;
+};
+
+__device__ inline double cyl_dist(const CylModel& M, const double p[3], double w[3], double* h) {
+  const double q[3] = {p[0] - M.c[0], p[1] - M.c[1], p[2] - M.c[2]};
+  const double hh = q[0] * M.a[0] + q[1] * M.a[1] + q[2] * M.a[2];
+  w[0] = q[0] - hh * M.a[0];
+  w[1] = q[1] - hh * M.a[1];
+  w[2] = q[2] - hh * M.a[2];
+  *h = hh;
+  return sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) - M.r;
+}
+
+// Inlier membership during the refit: the FP32 hypothesis predicate in the
+// first round, the FP64 distance to the previous round's model afterwards.
+__device__ inline bool member(bool fp32_sel, const CylFrame& f, const HypOut& h,
+                              const CylModel& sel, double thr, const float* q,
+                              const double x[3]) {
+  if (fp32_sel) {
+    float u, v, s;
+    local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
+    return is_inlier(u, v, s, h);
+  }
+  double w[3], hh;
+  const double d = cyl_dist(sel, x, w, &hh);
+  return d < thr && d > -thr;
+}
+
+__device__ inline void write_invalid(double* out, double k) {
+  for (int i = 0; i < 6; ++i) out[i] = 0.0;
+  out[5] = 1.0;
+  out[6] = -1.0;
+  out[7] = k;
+}
+
+// n x n solve (row-major M, destroyed), Gaussian elimination, partial pivoting.
+__device__ inline bool solve_n(double* M, double* b, double* x, int n) {
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < n; ++r)
+      if (fabs(M[r * n + col]) > fabs(M[piv * n + col])) piv = r;
+    if (!(fabs(M[piv * n + col]) > 0.0)) return false;
+    if (piv != col) {
+      for (int k = 0; k < n; ++k) {
+        const double t = M[col * n + k];
+        M[col * n + k] = M[piv * n + k];
+        M[piv * n + k] = t;
+      }
+      const double t = b[col];
+      b[col] = b[piv];
+      b[piv] = t;
+    }
+    for (int r = col + 1; r < n; ++r) {
+      const double fct = M[r * n + col] / M[col * n + col];
+      for (int k = col; k < n; ++k) M[r * n + k] = M[r * n + k] - fct * M[col * n + k];
+      b[r] = b[r] - fct * b[col];
+    }
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double t = b[r];
+    for (int k = r + 1; k < n; ++k) t = t - M[r * n + k] * x[k];
+    x[r] = t / M[r * n + r];
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// refit: best hypothesis, mask, FP64 LO-RANSAC cylinder refit
+// model record: root[3], ray[3], radius, inliers
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void cyl_refit_kernel(
+    const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
+    const CylFrame* __restrict__ frames, const int32_t* __restrict__ counts, int n_hyp,
+    float thr, float max_radius, uint32_t k0, uint32_t k1, uint32_t tag, int refit_rounds,
+    int refit_iters, uint32_t* __restrict__ mask, int32_t* __restrict__ best_out, double* __restrict__ model) {
+  __shared__ double red[kWaves * 20];
+  __shared__ long long redk[kWaves];
+  __shared__ HypOut best_h;
+  const int c = blockIdx.x;
+  const int off = offsets[c];
+  const int n = offsets[c + 1] - off;
+  const float* P = xyz + 3 * (size_t)off;
+  const CylFrame f = frames[c];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  double* out = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
+
+  // 1. best hypothesis: key = count * 2^32 + (2^32 - 1 - index), max wins
+  long long key = -1;
+  for (int h = threadIdx.x; h < n_hyp; h += kBlock) {
+    const long long k = ((long long)counts[(size_t)c * n_hyp + h] << 32) |
+                        (long long)(0xFFFFFFFFu - (uint32_t)h);
+    key = k > key ? k : key;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const long long other = __shfl_xor(key, o, 64);
+    key = other > key ? other : key;
+  }
+  if (lane == 0) redk[wave] = key;
+  __syncthreads();
+  key = redk[0];
+  for (int w = 1; w < kWaves; ++w) key = redk[w] > key ? redk[w] : key;
+  const int best = (n >= 3 && n_hyp > 0) ? (int)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFll)) : -1;
+  if (threadIdx.x == 0) {
+    if (best >= 0) {
+      int32_t idx[3];
+      best_h = hyp_for(P, n, c, best, f, thr, max_radius, k0, k1, tag, idx);
+    } else {
+      best_h.valid = 0;
+      best_h.A = best_h.B = 0.f;
+      best_h.L = INFINITY;
+      best_h.U = -INFINITY;
+    }
+    if (best_out) best_out[c] = best;
+  }
+  __syncthreads();
+  const HypOut h = best_h;
+
+  // 2. mask of the best hypothesis (FP32 predicate) + inlier centroid
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int gbeg = off & ~63;
+  const int gend = off + n;
+  for (int g = gbeg + (int)threadIdx.x; g - (int)threadIdx.x < gend; g += kBlock) {
+    bool in = false;
+    double d[3] = {0.0, 0.0, 0.0};
+    if (g >= off && g < gend) {
+      const float* q = xyz + 3 * (size_t)g;
+      float u, v, s;
+      local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
+      in = is_inlier(u, v, s, h);
+      if (in) {
+        d[0] = (double)q[0] - (double)f.m[0];
+        d[1] = (double)q[1] - (double)f.m[1];
+        d[2] = (double)q[2] - (double)f.m[2];
+      }
+    }
+    const unsigned long long bits = __ballot(in);
+    if (mask && lane == 0 && bits) {
+      const int w0 = (g - lane) >> 5;
+      const uint32_t lo = (uint32_t)bits;
+      const uint32_t hi = (uint32_t)(bits >> 32);
+      if (lo) atomicOr(mask + w0, lo);
+      if (hi) atomicOr(mask + w0 + 1, hi);
+    }
+    if (in) {
+      acc[0] += d[0];
+      acc[1] += d[1];
+      acc[2] += d[2];
+      acc[3] += 1.0;
+    }
+  }
+  block_sum<4>(acc, red);
+  double k = acc[3];
+  if (k < 3.0 || !h.valid) {
+    if (threadIdx.x == 0) write_invalid(out, k);
+    return;
+  }
+
+  // 3. LO-RANSAC refit: rounds of Gauss-Newton on the 5-parameter geometric
+  //    cylinder residual over the current inliers, then FP64 re-selection.
+  CylModel M;
+  for (int i = 0; i < 3; ++i) {
+    M.a[i] = f.axis[i];
+    M.c[i] = (double)h.cu * (double)f.e1[i] + (double)h.cv * (double)f.e2[i];
+  }
+  M.r = (double)h.r;
+  CylModel sel = M;   // membership model of rounds >= 1
+  bool fp32_sel = true;
+  double mu[3] = {acc[0] / k, acc[1] / k, acc[2] / k};
+  const double dthr = (double)thr;
+  for (int round = 0; round < refit_rounds; ++round) {
+    for (int it = 0; it < refit_iters; ++it) {
+      if (k < 5.0) break;
+      const double ht = (mu[0] - M.c[0]) * M.a[0] + (mu[1] - M.c[1]) * M.a[1] +
+                        (mu[2] - M.c[2]) * M.a[2];
+      for (int i = 0; i < 3; ++i) M.c[i] += ht * M.a[i];
+      double e1[3], e2[3];
+      sl_basis(M.a, e1, e2);
+      double g[20];
+#pragma unroll
+      for (int i = 0; i < 20; ++i) g[i] = 0.0;
+      for (int i = threadIdx.x; i < n; i += kBlock) {
+        const float* q = P + 3 * (size_t)i;
+        const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
+                             (double)q[2] - (double)f.m[2]};
+        if (!member(fp32_sel, f, h, sel, dthr, q, x)) continue;
+        double w[3], hh;
+        const double dd = cyl_dist(M, x, w, &hh);
+        const double rho = dd + M.r;
+        if (!(rho > 0.0)) continue;
+        const double n1 = (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]) / rho;
+        const double n2 = (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]) / rho;
+        const double J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0};
+        int t = 0;
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+#pragma unroll
+          for (int b = a; b < 5; ++b) g[t++] += J[a] * J[b];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) g[15 + a] += J[a] * dd;
+      }
+      block_sum<20>(g, red);
+      double N[25], rhs[5], dl[5];
+      {
+        int t = 0;
+        for (int a = 0; a < 5; ++a)
+          for (int b = a; b < 5; ++b) {
+            N[a * 5 + b] = g[t];
+            N[b * 5 + a] = g[t];
+            ++t;
+          }
+        for (int a = 0; a < 5; ++a) rhs[a] = -g[15 + a];
+      }
+      if (!solve_n(N, rhs, dl, 5)) break;
+      double na[3];
+      for (int i = 0; i < 3; ++i) na[i] = M.a[i] + dl[0] * e1[i] + dl[1] * e2[i];
+      const double nl = sqrt(na[0] * na[0] + na[1] * na[1] + na[2] * na[2]);
+      for (int i = 0; i < 3; ++i) {
+        M.a[i] = na[i] / nl;
+        M.c[i] = M.c[i] + dl[2] * e1[i] + dl[3] * e2[i];
+      }
+      M.r = M.r + dl[4];
+      if (!(M.r > 0.0)) break;
+    }
+    if (!(M.r > 0.0)) break;
+    // re-selection under the new model: count + centroid
+    double cacc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+      const float* q = P + 3 * (size_t)i;
+      const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
+                           (double)q[2] - (double)f.m[2]};
+      double w[3], hh;
+      const double dd = cyl_dist(M, x, w, &hh);
+      if (dd < dthr && dd > -dthr) {
+        cacc[0] += x[0];
+        cacc[1] += x[1];
+        cacc[2] += x[2];
+        cacc[3] += 1.0;
+      }
+    }
+    block_sum<4>(cacc, red);
+    k = cacc[3];
+    sel = M;
+    fp32_sel = false;
+    if (k < 3.0) break;
+    mu[0] = cacc[0] / k;
+    mu[1] = cacc[1] / k;
+    mu[2] = cacc[2] / k;
+  }
+  if (!(M.r > 0.0) || k < 3.0) {
+    if (threadIdx.x == 0) write_invalid(out, k);
+    return;
+  }
+  if (threadIdx.x == 0) {
+    // root: axis point at the height of the final inlier centroid
+    const double ht = (mu[0] - M.c[0]) * M.a[0] + (mu[1] - M.c[1]) * M.a[1] +
+                      (mu[2] - M.c[2]) * M.a[2];
+    double ray[3] = {M.a[0], M.a[1], M.a[2]};
+    int big = 0;
+    for (int i = 1; i < 3; ++i)
+      if (fabs(ray[i]) > fabs(ray[big])) big = i;
+    if (ray[big] < 0.0)
+      for (int i = 0; i < 3; ++i) ray[i] = -ray[i];
+    for (int i = 0; i < 3; ++i) {
+      out[i] = (double)f.m[i] + M.c[i] + ht * M.a[i];
+      out[3 + i] = ray[i];
+    }
+    out[6] = M.r;
+    out[7] = k;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+struct sloam_cyl_plan {
+  sloam_ctx* ctx = nullptr;
+  sloam_cyl_fit_cfg cfg{};
+  int32_t n_clusters = 0;
+  int64_t n_points = 0;
+  int hcap = 0;
+  int tile = 0;
+  int n_work = 0;
+  int4* d_work = nullptr;
+  CylFrame* d_frames = nullptr;
+  int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
+};
+
+static int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+extern "C" {
+
+int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
+                          const sloam_cyl_fit_cfg* cfg, sloam_cyl_plan** out) {
+  if (!ctx || !h_offsets || !cfg || !out || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (cfg->n_hyp < 1 || cfg->n_hyp > (1 << 20) || !(cfg->thr > 0.0f) ||
+      !(cfg->max_radius > 0.0f) || cfg->refit_iters < 0 || cfg->refit_rounds < 0 || cfg->hyp_chunk < 0 ||
+      cfg->tile_points < 0)
+    return SLOAM_ERR_INVALID_ARG;
+  if (h_offsets[0] != 0) return SLOAM_ERR_INVALID_ARG;
+  for (int i = 0; i < n_clusters; ++i) {
+    const int64_t len = (int64_t)h_offsets[i + 1] - h_offsets[i];
+    if (len < 0) return SLOAM_ERR_INVALID_ARG;
+    if (len > (1 << 17)) return SLOAM_ERR_TOO_LARGE;  // integer-exact covariance bound
+  }
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  sloam_cyl_plan* p = new (std::nothrow) sloam_cyl_plan();
+  if (!p) return SLOAM_ERR_ALLOC;
+  p->ctx = ctx;
+  p->cfg = *cfg;
+  p->n_clusters = n_clusters;
+  p->n_points = h_offsets[n_clusters];
+
+  // hypotheses per workgroup: a multiple of kWaves * kHB
+  int chunk = cfg->hyp_chunk > 0 ? cfg->hyp_chunk : 256;
+  chunk = std::min(chunk, round_up(cfg->n_hyp, kWaves * kHB));
+  p->hcap = round_up(chunk, kWaves * kHB);
+  p->tile = cfg->tile_points > 0 ? round_up(cfg->tile_points, 64) : 2048;
+  // keep the workgroup's LDS within 64 KiB: shrink the point tile if needed
+  const int hyp_bytes = p->hcap * (int)(sizeof(float4) + sizeof(int));
+  if (hyp_bytes > kMaxLds - 64 * 12) {
+    delete p;
+    return SLOAM_ERR_TOO_LARGE;
+  }
+  p->tile = std::min(p->tile, ((kMaxLds - hyp_bytes) / 12) / 64 * 64);
+
+  // work list: clusters by decreasing size (ties: lower index), chunks in order
+  std::vector<int> order(n_clusters);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (h_offsets[a + 1] - h_offsets[a]) > (h_offsets[b + 1] - h_offsets[b]);
+  });
+  std::vector<int4> work;
+  work.reserve((size_t)n_clusters * ((cfg->n_hyp + p->hcap - 1) / p->hcap));
+  for (int c : order) {
+    for (int h0 = 0; h0 < cfg->n_hyp; h0 += p->hcap) {
+      const int hc = std::min(p->hcap, cfg->n_hyp - h0);
+      work.push_back(make_int4(c, h0, hc, h_offsets[c + 1] - h_offsets[c]));
+    }
+  }
+  p->n_work = (int)work.size();
+  int st = SLOAM_OK;
+  if (p->n_work > 0) {
+    st = sloam_dev_alloc(ctx, (void**)&p->d_work, sizeof(int4) * work.size());
+    if (!st) {
+      hipError_t e = hipMemcpy(p->d_work, work.data(), sizeof(int4) * work.size(),
+                               hipMemcpyHostToDevice);
+      if (e != hipSuccess) st = sloam_record_hip_error(ctx, e, "hipMemcpy(work)");
+    }
+  }
+  if (!st && n_clusters > 0)
+    st = sloam_dev_alloc(ctx, (void**)&p->d_frames, sizeof(CylFrame) * n_clusters);
+  if (!st && n_clusters > 0)
+    st = sloam_dev_alloc(ctx, (void**)&p->d_counts,
+                         sizeof(int32_t) * (size_t)n_clusters * cfg->n_hyp);
+  if (st) {
+    sloam_cyl_plan_destroy(p);
+    return st;
+  }
+  *out = p;
+  return SLOAM_OK;
+}
+
+void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
+  if (!p) return;
+  sloam_dev_free(p->d_work);
+  sloam_dev_free(p->d_frames);
+  sloam_dev_free(p->d_counts);
+  delete p;
+}
+
+int sloam_cyl_plan_exec(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_offsets,
+                        int32_t* d_counts, int32_t* d_samples, uint32_t* d_mask, int32_t* d_best,
+                        double* d_model) {
+  if (!p || !d_offsets || !d_model || (p->n_points > 0 && !d_xyz)) return SLOAM_ERR_INVALID_ARG;
+  sloam_ctx* ctx = p->ctx;
+  if (p->n_clusters == 0) return SLOAM_OK;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  const sloam_cyl_fit_cfg& cfg = p->cfg;
+  const uint32_t k0 = (uint32_t)cfg.seed;
+  const uint32_t k1 = (uint32_t)(cfg.seed >> 32);
+  const uint32_t tag = (uint32_t)cfg.stream_tag;
+  int32_t* counts = d_counts ? d_counts : p->d_counts;
+  if (d_mask) {
+    int st = sloam_fill_u32(ctx, d_mask, (size_t)((p->n_points + 31) / 32), 0u);
+    if (st) return st;
+  }
+  {
+    SloamKernelScope ks(ctx, "cyl_seed");
+    hipLaunchKernelGGL(cyl_seed_kernel, dim3(p->n_clusters), dim3(kBlock), 0, ctx->stream, d_xyz,
+                       d_offsets, p->n_clusters, p->d_frames);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  if (p->n_work > 0) {
+    const size_t lds = (size_t)p->hcap * (sizeof(float4) + sizeof(int)) + 3 * sizeof(float) * p->tile;
+    SloamKernelScope ks(ctx, "cyl_score");
+    hipLaunchKernelGGL(cyl_score_kernel, dim3(p->n_work), dim3(kBlock), lds, ctx->stream, d_xyz,
+                       d_offsets, p->d_frames, p->d_work, cfg.n_hyp, p->hcap, p->tile, cfg.thr,
+                       cfg.max_radius, k0, k1, tag, counts, d_samples);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  {
+    SloamKernelScope ks(ctx, "cyl_refit");
+    hipLaunchKernelGGL(cyl_refit_kernel, dim3(p->n_clusters), dim3(kBlock), 0, ctx->stream, d_xyz,
+                       d_offsets, p->d_frames, counts, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1,
+                       tag, cfg.refit_rounds, cfg.refit_iters, d_mask, d_best, d_model);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
+}  // extern "C"

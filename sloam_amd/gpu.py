@@ -1,0 +1,147 @@
+"""Python front end of the C ABI: contexts and batched fitting plans.
+
+Device buffers are torch CUDA (HIP) tensors; torch is used for device memory
+only, every computation runs in the library's own kernels.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import CYL_MODEL_STRIDE, CylFitCfg, check
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device tensor expected")
+    if not t.is_contiguous():
+        raise ValueError("contiguous tensor expected")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One HIP stream + scratch on one device (sloam_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        check(self.lib.sloam_ctx_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            self.lib.sloam_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(self.lib.sloam_sync(self.handle), self.handle)
+
+    @property
+    def stream_ptr(self):
+        return self.lib.sloam_ctx_stream(self.handle)
+
+    def torch_stream(self):
+        """The context's stream as a torch ExternalStream (for events)."""
+        return torch.cuda.ExternalStream(self.stream_ptr, device=torch.device("cuda", self.device))
+
+    def enable_timing(self, on=True):
+        check(self.lib.sloam_ctx_enable_timing(self.handle, 1 if on else 0), self.handle)
+
+    def reset_timing(self):
+        check(self.lib.sloam_ctx_reset_timing(self.handle), self.handle)
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_int64()
+        check(self.lib.sloam_ctx_kernel_time(self.handle, name.encode(), C.byref(ms), C.byref(n)),
+              self.handle)
+        return ms.value, n.value
+
+
+class CylinderFitPlan:
+    """Batched point-level cylinder RANSAC over clusters (sloam_cyl_plan)."""
+
+    def __init__(self, ctx, offsets, n_hyp=1024, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=3,
+                 seed=0x5EED, stream_tag=0, hyp_chunk=0, tile_points=0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.offsets_host = off
+        self.n_clusters = len(off) - 1
+        self.n_points = int(off[-1])
+        self.cfg = CylFitCfg(n_hyp=n_hyp, thr=thr, max_radius=max_radius,
+                             refit_iters=refit_iters, seed=seed, stream_tag=stream_tag,
+                             hyp_chunk=hyp_chunk, tile_points=tile_points,
+                             refit_rounds=refit_rounds)
+        h = C.c_void_p()
+        check(self.lib.sloam_cyl_plan_create(ctx.handle, off.ctypes.data_as(C.c_void_p),
+                                             self.n_clusters, C.byref(self.cfg), C.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.sloam_cyl_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc_outputs(self, counts=False, samples=False, mask=True, best=True):
+        dev = torch.device("cuda", self.ctx.device)
+        out = {"model": torch.empty((self.n_clusters, CYL_MODEL_STRIDE), dtype=torch.float64,
+                                    device=dev)}
+        H = self.cfg.n_hyp
+        if counts:
+            out["counts"] = torch.empty((self.n_clusters, H), dtype=torch.int32, device=dev)
+        if samples:
+            out["samples"] = torch.empty((self.n_clusters, H, 3), dtype=torch.int32, device=dev)
+        if mask:
+            out["mask"] = torch.empty(((self.n_points + 31) // 32,), dtype=torch.int32,
+                                      device=dev)
+        if best:
+            out["best"] = torch.empty((self.n_clusters,), dtype=torch.int32, device=dev)
+        return out
+
+    def exec(self, xyz, offsets, out):
+        """Enqueues one batch on the context stream (asynchronous)."""
+        if xyz.dtype != torch.float32 or xyz.shape != (self.n_points, 3):
+            raise ValueError("xyz must be float32 [n_points, 3]")
+        if offsets.dtype != torch.int32 or offsets.numel() != self.n_clusters + 1:
+            raise ValueError("offsets must be int32 [n_clusters + 1]")
+        check(self.lib.sloam_cyl_plan_exec(
+            self.handle, _ptr(xyz), _ptr(offsets), _ptr(out.get("counts")),
+            _ptr(out.get("samples")), _ptr(out.get("mask")), _ptr(out.get("best")),
+            _ptr(out["model"])), self.ctx.handle)
+        return out
+
+
+def fit_cylinders(ctx, xyz, offsets_host, **kw):
+    """One-shot convenience: plan, run, sync; returns numpy outputs."""
+    want = {k: kw.pop(k) for k in ("counts", "samples", "mask", "best") if k in kw}
+    plan = CylinderFitPlan(ctx, offsets_host, **kw)
+    dev = torch.device("cuda", ctx.device)
+    xyz_d = torch.as_tensor(np.ascontiguousarray(xyz, dtype=np.float32)).to(dev)
+    off_d = torch.as_tensor(np.ascontiguousarray(offsets_host, dtype=np.int32)).to(dev)
+    out = plan.alloc_outputs(**want)
+    torch.cuda.synchronize()
+    plan.exec(xyz_d, off_d, out)
+    ctx.sync()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    if "mask" in res:
+        res["mask"] = res["mask"].view(np.uint32)
+    plan.close()
+    return res

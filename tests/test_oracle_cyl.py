@@ -1,0 +1,194 @@
+"""CPU tests that pin the cylinder-fitting oracle to facts it does not itself
+define: published Philox known answers, exhaustive / statistical properties of
+the sampler, numpy's eigensolver, closed-form noise-free cylinders, the FP64
+reference distance (sloam/src/objects/cylinder.cpp:196-203) and accuracy
+against the generator's ground truth."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from sloam_amd import synth
+
+# Random123 known-answer vectors for philox4x32 with 10 rounds
+# (counter[4], key[2]) -> output[4]
+PHILOX_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, expect):
+    assert list(oracle.philox(ctr, key)) == expect
+
+
+def test_draw3_exhaustive_small_n():
+    # every (x0, x1, x2) on a grid of the 32-bit range maps to 3 distinct
+    # in-range indices, and the map hits every ordered triple of [0, n)
+    for n in (3, 4, 5, 7):
+        seen = set()
+        grid = [int(i * (2**32 - 1) / 15) for i in range(16)]
+        for x0, x1, x2 in itertools.product(grid, grid, grid):
+            t = tuple(oracle.draw3(x0, x1, x2, n))
+            assert len(set(t)) == 3 and all(0 <= i < n for i in t)
+            seen.add(t)
+        assert len(seen) == n * (n - 1) * (n - 2)
+
+
+def test_draw3_uniform():
+    rng = np.random.default_rng(0)
+    n = 6
+    counts = {}
+    for _ in range(30000):
+        x = rng.integers(0, 2**32, 3, dtype=np.uint64)
+        t = tuple(oracle.draw3(int(x[0]), int(x[1]), int(x[2]), n))
+        counts[t] = counts.get(t, 0) + 1
+    obs = np.array(list(counts.values()), float)
+    assert len(obs) == 120
+    exp = 30000 / 120
+    chi2 = ((obs - exp) ** 2 / exp).sum()
+    assert chi2 < 200  # 119 dof, p ~ 1e-5
+
+
+def test_eig3_matches_numpy():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        B = rng.normal(size=(3, 3)) * rng.uniform(0.01, 100)
+        S = B @ B.T
+        a6 = [S[0, 0], S[0, 1], S[0, 2], S[1, 1], S[1, 2], S[2, 2]]
+        w, V = oracle.eig3(a6)
+        wn, Vn = np.linalg.eigh(S)
+        assert np.allclose(np.sort(w), wn, rtol=1e-10, atol=1e-12 * abs(wn).max())
+        # V S V^T diagonal, V orthonormal
+        assert np.allclose(V.T @ V, np.eye(3), atol=1e-12)
+        assert np.allclose(S @ V, V * w[None, :], atol=1e-9 * abs(wn).max())
+
+
+def _grid_cylinder(center, radius, height, n_h=40, n_a=36):
+    """Noise-free points on a vertical cylinder, regular grid (symmetric)."""
+    hs = np.linspace(0, height, n_h)
+    th = np.linspace(0, 2 * math.pi, n_a, endpoint=False)
+    H, T = np.meshgrid(hs, th, indexing="ij")
+    pts = np.stack([center[0] + radius * np.cos(T), center[1] + radius * np.sin(T),
+                    center[2] + H], axis=-1).reshape(-1, 3)
+    return pts.astype(np.float32)
+
+
+def test_seed_axis_closed_form():
+    pts = _grid_cylinder((3.0, -2.0, 0.5), 0.3, 6.0)
+    fr, ax = oracle.cyl_seed(pts)
+    assert abs(ax[2]) > 1 - 1e-9
+    # frame centre = bounding-box centre
+    assert np.allclose(fr[:3], [3.0, -2.0, 3.5], atol=1e-5)
+    # basis orthonormal and orthogonal to the axis
+    e1, e2 = fr[3:6].astype(float), fr[6:9].astype(float)
+    assert abs(e1 @ e2) < 1e-6 and abs(e1 @ ax) < 1e-6 and abs(np.linalg.norm(e1) - 1) < 1e-6
+
+
+def test_hypothesis_circle_closed_form():
+    pts = _grid_cylinder((1.0, 1.0, 0.0), 0.25, 5.0)
+    fr, ax = oracle.cyl_seed(pts)
+    m = fr[:3].astype(float)
+    e1, e2 = fr[3:6].astype(float), fr[6:9].astype(float)
+    for h in range(50):
+        idx, circ, prm = oracle.cyl_hypothesis(pts, 0, h, 0.05, 1.0, 7)
+        if len({tuple(pts[i][:2]) for i in idx}) < 3:
+            assert circ[3] == 0.0  # three points on one generatrix: degenerate
+            continue
+        assert circ[3] == 1.0
+        assert abs(circ[2] - 0.25) < 2e-6
+        c3 = m + circ[0] * e1 + circ[1] * e2
+        assert np.allclose(c3[:2], [1.0, 1.0], atol=2e-6)
+
+
+def test_predicate_vs_fp64_reference_distance():
+    """Inlier membership of the FP32 squared-form predicate equals the FP64
+    reference distance test |d| < thr except within 1e-5 m of the threshold."""
+    d = synth.make_clusters(12, seed=5)
+    thr = 0.05
+    off = d["offsets"]
+    worst = 0.0
+    n_cmp = 0
+    for c in range(12):
+        P = d["xyz"][off[c]:off[c + 1]]
+        single = np.array([0, len(P)], np.int32)
+        res = oracle.cyl_fit_batch(P, single, 64, thr=thr, counts=True, mask=False)
+        for h in range(64):
+            dist = oracle.cyl_distances_fp64(P, 0, h, thr, 1.0, 0x5EED)
+            if dist is None:
+                assert res["counts"][0, h] == 0
+                continue
+            fp64 = np.abs(dist) < thr
+            # the FP32 count must lie within the count band of near-threshold points
+            near = np.abs(np.abs(dist) - thr) < 1e-5
+            lo = int((fp64 & ~near).sum())
+            hi = int((fp64 | near).sum())
+            cnt = int(res["counts"][0, h])
+            assert lo <= cnt <= hi, (c, h, cnt, lo, hi)
+            n_cmp += 1
+            worst = max(worst, abs(cnt - int(fp64.sum())))
+    assert n_cmp > 500
+    assert worst <= 2
+
+
+def test_fit_noise_free_cylinder_exact():
+    pts = _grid_cylinder((5.0, 4.0, -1.0), 0.2, 4.0)
+    off = np.array([0, len(pts)], np.int32)
+    res = oracle.cyl_fit_batch(pts, off, 64, thr=0.05)
+    m = res["model"][0]
+    assert abs(m[6] - 0.2) < 1e-6
+    assert abs(abs(m[5]) - 1.0) < 1e-9
+    assert np.allclose(m[:2], [5.0, 4.0], atol=1e-6)
+    assert m[7] == len(pts)
+
+
+def test_fit_tilted_noisy_cylinder_accuracy():
+    rng = np.random.default_rng(3)
+    a = np.array([math.sin(0.06), 0.0, math.cos(0.06)])
+    base = np.array([8.0, 3.0, 0.0])
+    pts, _ = synth.trunk_points(rng, 3000, base, a, 0.3, 6.0, 0.01, 0.2)
+    off = np.array([0, len(pts)], np.int32)
+    res = oracle.cyl_fit_batch(pts.astype(np.float32), off, 512, thr=0.05)
+    m = res["model"][0]
+    assert abs(m[6] - 0.3) < 0.005
+    ang = math.degrees(math.acos(min(1.0, abs(m[3:6] @ a))))
+    assert ang < 0.3
+    # root lies on the true axis
+    q = m[:3] - base
+    assert np.linalg.norm(q - (q @ a) * a) < 0.01
+
+
+def test_batch_accuracy_vs_truth():
+    d = synth.make_clusters(150, seed=11)
+    res = oracle.cyl_fit_batch(d["xyz"], d["offsets"], 1024, thr=0.05, threads=8)
+    err = np.abs(res["model"][:, 6] - d["radius"])
+    assert np.median(err) < 0.005
+    assert (err > 0.05).mean() < 0.10
+
+
+def test_batch_threads_do_not_change_results():
+    d = synth.make_clusters(40, seed=2)
+    a = oracle.cyl_fit_batch(d["xyz"], d["offsets"], 128, threads=1, samples=True)
+    b = oracle.cyl_fit_batch(d["xyz"], d["offsets"], 128, threads=4, samples=True)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_edge_sizes():
+    # clusters of 0, 1, 2, 3 points and a degenerate (collinear) one
+    pts = [np.zeros((0, 3)), np.ones((1, 3)), np.ones((2, 3)),
+           np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float),
+           np.array([[0, 0, i] for i in range(10)], float)]
+    xyz = np.concatenate(pts).astype(np.float32)
+    off = np.cumsum([0] + [len(p) for p in pts]).astype(np.int32)
+    res = oracle.cyl_fit_batch(xyz, off, 32, samples=True)
+    assert list(res["best"][:3]) == [-1, -1, -1]
+    assert np.all(res["counts"][:3] == 0)
+    assert np.all(res["samples"][:3] == -1)
+    assert np.all(res["model"][:3, 6] == -1)
+    assert np.all(res["model"][4, 6] == -1)  # collinear: no circle
