@@ -93,7 +93,7 @@ def cyl_hypothesis(xyz, cluster, h, thr, max_radius, seed, tag=0):
     return idx, circ, prm
 
 
-def cyl_fit_batch(xyz, offsets, n_hyp, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=3,
+def cyl_fit_batch(xyz, offsets, n_hyp, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=2,
                   seed=0x5EED,
                   tag=0, counts=True, samples=False, mask=True, threads=1):
     x = np.ascontiguousarray(xyz, np.float32)

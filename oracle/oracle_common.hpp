@@ -71,5 +71,9 @@ void basis(const double a[3], double e1[3], double e2[3]);
 bool solve3(double M[3][3], double b[3], double x[3]);
 // n x n linear solve (row-major M, destroyed), partial pivoting.
 bool solve_n(double* M, double* b, double* x, int n);
+// Cholesky solve of a symmetric positive definite 5x5 (or 6x6) system;
+// false if a pivot is not positive.
+bool cholesky_solve5(const double A[5][5], const double b[5], double x[5]);
+bool cholesky_solve6(const double A[6][6], const double b[6], double x[6]);
 
 }  // namespace oracle

@@ -146,14 +146,16 @@ inline bool inlier(float u, float v, float s, const Hyp& h) {
 }
 
 // LO-RANSAC refit (FP64) of the best hypothesis h.
-// Round 0 takes the FP32 inliers of h; every round runs `iters` Gauss-Newton
-// steps of the 5-parameter geometric cylinder fit
+// `rounds` rounds of `iters` Gauss-Newton steps of the 5-parameter geometric
+// cylinder fit
 //   d_i = ||(p_i - c) - ((p_i - c).a) a|| - r
-// over the current inliers (axis a perturbed in its orthogonal plane, axis
-// point c moved in that plane, radius r), then re-selects the inliers as
-// |d_i| < thr under the new model.  Output: root (axis point at the height of
-// the inlier centroid), ray (unit, largest |component| positive), radius and
-// the final inlier count.
+// (axis a perturbed in its orthogonal plane, axis point c moved in that plane,
+// radius r).  The inliers of round 0 are the FP32 inliers of h; the inliers of
+// round j > 0 are |d_i| < thr under the model at the start of round j.  A round
+// stops early when fewer than 5 points are inliers or the normal matrix is not
+// positive definite.  Output: root (axis point at the height of the final
+// inlier centroid), ray (unit, largest |component| positive), radius and the
+// final inlier count (|d_i| < thr under the final model).
 struct CylModel {
   double a[3], c[3], r;
 };
@@ -170,25 +172,25 @@ inline double cyl_dist(const CylModel& M, const double p[3], double w[3], double
 
 void refit(const float* P, int n, const Frame& f, const Hyp& h, int rounds, int iters, float thr,
            double out[8]) {
-  std::vector<double> X(3 * (size_t)n);
-  for (int i = 0; i < n; ++i)
-    for (int c = 0; c < 3; ++c) X[3 * i + c] = (double)P[3 * i + c] - (double)f.m[c];
-  std::vector<char> in((size_t)n, 0);
-  int k = 0;
-  for (int i = 0; i < n; ++i) {
-    float u, v, s;
-    to_local(f, P + 3 * i, u, v, s);
-    in[i] = inlier(u, v, s, h) ? 1 : 0;
-    k += in[i];
-  }
   auto invalid = [&](double cnt) {
     for (int i = 0; i < 6; ++i) out[i] = 0.0;
     out[5] = 1.0;
     out[6] = -1.0;
     out[7] = cnt;
   };
-  if (k < 3 || !h.valid) {
-    invalid((double)k);
+  std::vector<double> X(3 * (size_t)n);
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) X[3 * i + c] = (double)P[3 * i + c] - (double)f.m[c];
+  std::vector<char> in((size_t)n, 0);
+  int k0 = 0;
+  for (int i = 0; i < n; ++i) {
+    float u, v, s;
+    to_local(f, P + 3 * i, u, v, s);
+    in[i] = inlier(u, v, s, h) ? 1 : 0;
+    k0 += in[i];
+  }
+  if (k0 < 3 || !h.valid) {
+    invalid((double)k0);
     return;
   }
   CylModel M;
@@ -198,41 +200,46 @@ void refit(const float* P, int n, const Frame& f, const Hyp& h, int rounds, int 
   }
   M.r = (double)h.r;
   const double dthr = (double)thr;
-  for (int round = 0; round < rounds; ++round) {
-    for (int it = 0; it < iters; ++it) {
-      // re-centre c at the height of the inlier centroid
-      double mu[3] = {0, 0, 0};
-      double cnt = 0;
+  bool stop = false;
+  for (int round = 0; round < rounds && !stop; ++round) {
+    if (round > 0) {
       for (int i = 0; i < n; ++i) {
-        if (!in[i]) continue;
-        for (int c = 0; c < 3; ++c) mu[c] += X[3 * i + c];
-        cnt += 1.0;
+        double w[3], hh;
+        const double d = cyl_dist(M, &X[3 * i], w, &hh);
+        in[i] = (d < dthr && d > -dthr) ? 1 : 0;
       }
-      if (cnt < 5.0) break;
-      const double ht = ((mu[0] / cnt - M.c[0]) * M.a[0] + (mu[1] / cnt - M.c[1]) * M.a[1] +
-                         (mu[2] / cnt - M.c[2]) * M.a[2]);
-      for (int c = 0; c < 3; ++c) M.c[c] += ht * M.a[c];
+    }
+    for (int it = 0; it < iters; ++it) {
       double e1[3], e2[3];
       basis(M.a, e1, e2);
-      double JtJ[5][5] = {};
-      double Jtd[5] = {};
+      double A[5][5] = {};
+      double g[5] = {};
+      int cnt = 0;
       for (int i = 0; i < n; ++i) {
         if (!in[i]) continue;
+        ++cnt;
         double w[3], hh;
         const double d = cyl_dist(M, &X[3 * i], w, &hh);
         const double rho = d + M.r;
         if (!(rho > 0.0)) continue;
-        const double n1 = (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]) / rho;
-        const double n2 = (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]) / rho;
+        const double inv = 1.0 / rho;
+        const double n1 = (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]) * inv;
+        const double n2 = (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]) * inv;
         const double J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0};
         for (int a = 0; a < 5; ++a) {
-          for (int b = 0; b < 5; ++b) JtJ[a][b] += J[a] * J[b];
-          Jtd[a] += J[a] * d;
+          for (int b = a; b < 5; ++b) A[a][b] += J[a] * J[b];
+          g[a] += J[a] * d;
         }
       }
-      double rhs[5] = {-Jtd[0], -Jtd[1], -Jtd[2], -Jtd[3], -Jtd[4]};
+      if (cnt < 5) {
+        stop = true;
+        break;
+      }
+      for (int a = 0; a < 5; ++a)
+        for (int b = 0; b < a; ++b) A[a][b] = A[b][a];
+      const double rhs[5] = {-g[0], -g[1], -g[2], -g[3], -g[4]};
       double dl[5];
-      if (!solve_n(&JtJ[0][0], rhs, dl, 5)) break;
+      if (!cholesky_solve5(A, rhs, dl)) break;
       double na[3];
       for (int c = 0; c < 3; ++c) na[c] = M.a[c] + dl[0] * e1[c] + dl[1] * e2[c];
       const double nl = std::sqrt(na[0] * na[0] + na[1] * na[1] + na[2] * na[2]);
@@ -241,27 +248,29 @@ void refit(const float* P, int n, const Frame& f, const Hyp& h, int rounds, int 
         M.c[c] = M.c[c] + dl[2] * e1[c] + dl[3] * e2[c];
       }
       M.r = M.r + dl[4];
-      if (!(M.r > 0.0)) break;
+      if (!(M.r > 0.0)) {
+        stop = true;
+        break;
+      }
     }
-    if (!(M.r > 0.0)) break;
-    k = 0;
+  }
+  // final inliers under the final model
+  double mu[3] = {0, 0, 0};
+  int k = 0;
+  if (M.r > 0.0) {
     for (int i = 0; i < n; ++i) {
       double w[3], hh;
       const double d = cyl_dist(M, &X[3 * i], w, &hh);
-      in[i] = (d < dthr && d > -dthr) ? 1 : 0;
-      k += in[i];
+      if (d < dthr && d > -dthr) {
+        for (int c = 0; c < 3; ++c) mu[c] += X[3 * i + c];
+        ++k;
+      }
     }
-    if (k < 3) break;
   }
   if (!(M.r > 0.0) || k < 3) {
     invalid((double)k);
     return;
   }
-  // root: axis point at the height of the final inlier centroid
-  double mu[3] = {0, 0, 0};
-  for (int i = 0; i < n; ++i)
-    if (in[i])
-      for (int c = 0; c < 3; ++c) mu[c] += X[3 * i + c];
   const double ht = ((mu[0] / k - M.c[0]) * M.a[0] + (mu[1] / k - M.c[1]) * M.a[1] +
                      (mu[2] / k - M.c[2]) * M.a[2]);
   double ray[3] = {M.a[0], M.a[1], M.a[2]};

@@ -167,4 +167,41 @@ bool solve_n(double* M, double* b, double* x, int n) {
   return true;
 }
 
+template <int N>
+static bool cholesky_solve(const double A[N][N], const double b[N], double x[N]) {
+  double L[N][N] = {};
+  for (int j = 0; j < N; ++j) {
+    double s = A[j][j];
+    for (int k = 0; k < j; ++k) s = s - L[j][k] * L[j][k];
+    if (!(s > 0.0)) return false;
+    const double d = std::sqrt(s);
+    L[j][j] = d;
+    for (int i = j + 1; i < N; ++i) {
+      double t = A[i][j];
+      for (int k = 0; k < j; ++k) t = t - L[i][k] * L[j][k];
+      L[i][j] = t / d;
+    }
+  }
+  double y[N];
+  for (int i = 0; i < N; ++i) {
+    double t = b[i];
+    for (int k = 0; k < i; ++k) t = t - L[i][k] * y[k];
+    y[i] = t / L[i][i];
+  }
+  for (int i = N - 1; i >= 0; --i) {
+    double t = y[i];
+    for (int k = i + 1; k < N; ++k) t = t - L[k][i] * x[k];
+    x[i] = t / L[i][i];
+  }
+  return true;
+}
+
+bool cholesky_solve5(const double A[5][5], const double b[5], double x[5]) {
+  return cholesky_solve<5>(A, b, x);
+}
+
+bool cholesky_solve6(const double A[6][6], const double b[6], double x[6]) {
+  return cholesky_solve<6>(A, b, x);
+}
+
 }  // namespace oracle

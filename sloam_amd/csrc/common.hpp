@@ -181,21 +181,27 @@ __host__ __device__ inline void sl_eig3(const double a[6], double w[3], double V
   w[2] = A[2][2];
 }
 
-// Eigenvector column k of V, sign-normalised so that its largest-magnitude
-// component is positive (lowest index on ties).
-__host__ __device__ inline void sl_eigvec(const double V[3][3], int k, double out[3]) {
-  out[0] = V[0][k];
-  out[1] = V[1][k];
-  out[2] = V[2][k];
-  int im = 0;
-  double best = sl_fabs(out[0]);
-  if (sl_fabs(out[1]) > best) { im = 1; best = sl_fabs(out[1]); }
-  if (sl_fabs(out[2]) > best) { im = 2; }
-  if (out[im] < 0.0) {
-    out[0] = -out[0];
-    out[1] = -out[1];
-    out[2] = -out[2];
+// Flips v so that its largest-magnitude component is positive (lowest index
+// on ties).  Written with selects only: no dynamically indexed arrays.
+__host__ __device__ inline void sl_sign_normalise(double v[3]) {
+  const double a0 = sl_fabs(v[0]), a1 = sl_fabs(v[1]), a2 = sl_fabs(v[2]);
+  const bool pick1 = a1 > a0;
+  const double best01 = pick1 ? a1 : a0;
+  const double val01 = pick1 ? v[1] : v[0];
+  const double lead = (a2 > best01) ? v[2] : val01;
+  if (lead < 0.0) {
+    v[0] = -v[0];
+    v[1] = -v[1];
+    v[2] = -v[2];
   }
+}
+
+// Eigenvector column k of V, sign-normalised (see sl_sign_normalise).
+__host__ __device__ inline void sl_eigvec(const double V[3][3], int k, double out[3]) {
+  out[0] = k == 0 ? V[0][0] : (k == 1 ? V[0][1] : V[0][2]);
+  out[1] = k == 0 ? V[1][0] : (k == 1 ? V[1][1] : V[1][2]);
+  out[2] = k == 0 ? V[2][0] : (k == 1 ? V[2][1] : V[2][2]);
+  sl_sign_normalise(out);
 }
 
 __host__ __device__ inline int sl_argmax3(const double w[3]) {
@@ -214,18 +220,17 @@ __host__ __device__ inline int sl_argmin3(const double w[3]) {
 
 // Orthonormal basis (e1, e2) of the plane orthogonal to the unit vector a:
 // e1 = normalise(r - (r.a) a) with r the coordinate axis least aligned with a
-// (lowest index on ties), e2 = a x e1.
+// (lowest index on ties), e2 = a x e1.  Selects only (no dynamic indexing).
 __host__ __device__ inline void sl_basis(const double a[3], double e1[3], double e2[3]) {
-  int k = 0;
-  double best = sl_fabs(a[0]);
-  if (sl_fabs(a[1]) < best) { k = 1; best = sl_fabs(a[1]); }
-  if (sl_fabs(a[2]) < best) { k = 2; }
-  double r[3] = {0.0, 0.0, 0.0};
-  r[k] = 1.0;
-  const double d = a[k];
-  double x = r[0] - d * a[0];
-  double y = r[1] - d * a[1];
-  double z = r[2] - d * a[2];
+  const double a0 = sl_fabs(a[0]), a1 = sl_fabs(a[1]), a2 = sl_fabs(a[2]);
+  const bool k1 = a1 < a0;
+  const double best01 = k1 ? a1 : a0;
+  const bool k2 = a2 < best01;
+  const int k = k2 ? 2 : (k1 ? 1 : 0);
+  const double d = k == 0 ? a[0] : (k == 1 ? a[1] : a[2]);
+  double x = (k == 0 ? 1.0 : 0.0) - d * a[0];
+  double y = (k == 1 ? 1.0 : 0.0) - d * a[1];
+  double z = (k == 2 ? 1.0 : 0.0) - d * a[2];
   const double nrm = sqrt(x * x + y * y + z * z);
   x = x / nrm;
   y = y / nrm;

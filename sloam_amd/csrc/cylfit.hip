@@ -9,9 +9,10 @@
 //               cluster's points through LDS tiles as (u, v, s = u^2 + v^2) and
 //               counts inliers with a wavefront ballot + popcount, 64 points per
 //               compare, eight hypotheses per pass over a tile.
-//   cyl_refit   one workgroup per cluster: best hypothesis (max count, lowest
-//               index), inlier bit mask, FP64 refit (PCA axis, algebraic circle,
-//               Gauss-Newton geometric circle).
+//   cyl_refit   one workgroup per cluster (one wave for small clusters): best
+//               hypothesis (max count, lowest index), inlier bit mask, FP64
+//               LO-RANSAC refit (Gauss-Newton on the 5-parameter geometric
+//               cylinder residual with inlier re-selection).
 //
 // The inlier predicate is the reference's point-to-cylinder distance
 // (sloam/src/objects/cylinder.cpp:196-203, Cylinder::distance(const PointT&)):
@@ -35,6 +36,7 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kHB = 8;  // hypotheses held in registers per pass over a tile
 constexpr int kMaxLds = 65536;  // LDS bytes per workgroup this kernel asks for
+constexpr int kSmallRefit = 768;  // clusters up to this size are refitted by one wave
 
 struct CylFrame {
   float m[3];
@@ -116,29 +118,6 @@ __device__ inline HypOut hyp_for(const float* __restrict__ P, int n, int c, int 
 __device__ inline bool is_inlier(float u, float v, float s, const HypOut& h) {
   const float t = __builtin_fmaf(u, h.A, __builtin_fmaf(v, h.B, s));
   return (t > h.L) && (t < h.U);
-}
-
-// Sums K doubles over the block; every thread receives the totals.  The
-// cross-wave order is fixed (wave 0..3), lanes are combined by xor-shuffles.
-template <int K>
-__device__ inline void block_sum(double (&v)[K], double* lds) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = sl_wave_sum(v[k]);
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double t = lds[k];
-    for (int w = 1; w < kWaves; ++w) t += lds[w * K + k];
-    v[k] = t;
-  }
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -376,21 +355,6 @@ __device__ inline double cyl_dist(const CylModel& M, const double p[3], double w
   return sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) - M.r;
 }
 
-// Inlier membership during the refit: the FP32 hypothesis predicate in the
-// first round, the FP64 distance to the previous round's model afterwards.
-__device__ inline bool member(bool fp32_sel, const CylFrame& f, const HypOut& h,
-                              const CylModel& sel, double thr, const float* q,
-                              const double x[3]) {
-  if (fp32_sel) {
-    float u, v, s;
-    local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
-    return is_inlier(u, v, s, h);
-  }
-  double w[3], hh;
-  const double d = cyl_dist(sel, x, w, &hh);
-  return d < thr && d > -thr;
-}
-
 __device__ inline void write_invalid(double* out, double k) {
   for (int i = 0; i < 6; ++i) out[i] = 0.0;
   out[5] = 1.0;
@@ -398,50 +362,94 @@ __device__ inline void write_invalid(double* out, double k) {
   out[7] = k;
 }
 
-// n x n solve (row-major M, destroyed), Gaussian elimination, partial pivoting.
-__device__ inline bool solve_n(double* M, double* b, double* x, int n) {
-  for (int col = 0; col < n; ++col) {
-    int piv = col;
-    for (int r = col + 1; r < n; ++r)
-      if (fabs(M[r * n + col]) > fabs(M[piv * n + col])) piv = r;
-    if (!(fabs(M[piv * n + col]) > 0.0)) return false;
-    if (piv != col) {
-      for (int k = 0; k < n; ++k) {
-        const double t = M[col * n + k];
-        M[col * n + k] = M[piv * n + k];
-        M[piv * n + k] = t;
-      }
-      const double t = b[col];
-      b[col] = b[piv];
-      b[piv] = t;
-    }
-    for (int r = col + 1; r < n; ++r) {
-      const double fct = M[r * n + col] / M[col * n + col];
-      for (int k = col; k < n; ++k) M[r * n + k] = M[r * n + k] - fct * M[col * n + k];
-      b[r] = b[r] - fct * b[col];
+// Cholesky solve of a symmetric positive definite N x N system, fully
+// unrolled so that every index is static (no private memory).
+template <int N>
+__device__ inline bool cholesky_solve(const double (&A)[N][N], const double (&b)[N],
+                                      double (&x)[N]) {
+  double L[N][N];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double s = A[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s = s - L[j][k] * L[j][k];
+    ok = ok && (s > 0.0);
+    const double d = sqrt(s);
+    L[j][j] = d;
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      double t = A[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t = t - L[i][k] * L[j][k];
+      L[i][j] = t / d;
     }
   }
-  for (int r = n - 1; r >= 0; --r) {
-    double t = b[r];
-    for (int k = r + 1; k < n; ++k) t = t - M[r * n + k] * x[k];
-    x[r] = t / M[r * n + r];
+  double y[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double t = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) t = t - L[i][k] * y[k];
+    y[i] = t / L[i][i];
   }
-  return true;
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    double t = y[i];
+#pragma unroll
+    for (int k = i + 1; k < N; ++k) t = t - L[k][i] * x[k];
+    x[i] = t / L[i][i];
+  }
+  return ok;
+}
+
+// Sums K doubles over a block of BLOCK threads; every thread gets the totals.
+template <int BLOCK, int K>
+__device__ inline void bsum(double (&v)[K], double* lds) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = sl_wave_sum(v[k]);
+  if (BLOCK == 64) return;
+  constexpr int W = BLOCK / 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double t = lds[k];
+#pragma unroll
+    for (int w = 1; w < W; ++w) t += lds[w * K + k];
+    v[k] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// refit: best hypothesis, mask, FP64 LO-RANSAC cylinder refit
-// model record: root[3], ray[3], radius, inliers
+// refit: best hypothesis, mask, FP64 LO-RANSAC cylinder refit.
+// One workgroup per cluster (BLOCK = 64 for small clusters, 256 for large).
+// Rounds of Gauss-Newton steps on the 5-parameter geometric cylinder residual
+//   d_i = ||(p_i - c) - ((p_i - c).a) a|| - r
+// over the round's inliers: the FP32 inliers of the best hypothesis in round
+// 0, |d_i| < thr under the model at the start of the round afterwards.  The
+// membership of a round is computed in its first pass and kept as a bit mask
+// in LDS.  Output record: root[3], ray[3], radius, inliers.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void cyl_refit_kernel(
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
     const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
-    const CylFrame* __restrict__ frames, const int32_t* __restrict__ counts, int n_hyp,
-    float thr, float max_radius, uint32_t k0, uint32_t k1, uint32_t tag, int refit_rounds,
-    int refit_iters, uint32_t* __restrict__ mask, int32_t* __restrict__ best_out, double* __restrict__ model) {
-  __shared__ double red[kWaves * 20];
-  __shared__ long long redk[kWaves];
+    const CylFrame* __restrict__ frames, const int32_t* __restrict__ counts,
+    const int32_t* __restrict__ list, int n_hyp, float thr, float max_radius, uint32_t k0,
+    uint32_t k1, uint32_t tag, int refit_rounds, int refit_iters, uint32_t* __restrict__ mask,
+    int32_t* __restrict__ best_out, double* __restrict__ model) {
+  constexpr int W = BLOCK / 64;
+  __shared__ double red[W * 20];
+  __shared__ long long redk[W];
   __shared__ HypOut best_h;
-  const int c = blockIdx.x;
+  extern __shared__ uint32_t bits[];  // membership of the current round
+  const int c = list[blockIdx.x];
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
   const float* P = xyz + 3 * (size_t)off;
@@ -452,20 +460,23 @@ __global__ __launch_bounds__(kBlock) void cyl_refit_kernel(
 
   // 1. best hypothesis: key = count * 2^32 + (2^32 - 1 - index), max wins
   long long key = -1;
-  for (int h = threadIdx.x; h < n_hyp; h += kBlock) {
-    const long long k = ((long long)counts[(size_t)c * n_hyp + h] << 32) |
-                        (long long)(0xFFFFFFFFu - (uint32_t)h);
-    key = k > key ? k : key;
+  for (int h = threadIdx.x; h < n_hyp; h += BLOCK) {
+    const long long kk = ((long long)counts[(size_t)c * n_hyp + h] << 32) |
+                         (long long)(0xFFFFFFFFu - (uint32_t)h);
+    key = kk > key ? kk : key;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const long long other = __shfl_xor(key, o, 64);
     key = other > key ? other : key;
   }
-  if (lane == 0) redk[wave] = key;
-  __syncthreads();
-  key = redk[0];
-  for (int w = 1; w < kWaves; ++w) key = redk[w] > key ? redk[w] : key;
+  if (BLOCK > 64) {
+    if (lane == 0) redk[wave] = key;
+    __syncthreads();
+    key = redk[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) key = redk[w] > key ? redk[w] : key;
+  }
   const int best = (n >= 3 && n_hyp > 0) ? (int)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFll)) : -1;
   if (threadIdx.x == 0) {
     if (best >= 0) {
@@ -476,164 +487,193 @@ __global__ __launch_bounds__(kBlock) void cyl_refit_kernel(
       best_h.A = best_h.B = 0.f;
       best_h.L = INFINITY;
       best_h.U = -INFINITY;
+      best_h.cu = best_h.cv = best_h.r = 0.f;
     }
     if (best_out) best_out[c] = best;
   }
   __syncthreads();
   const HypOut h = best_h;
 
-  // 2. mask of the best hypothesis (FP32 predicate) + inlier centroid
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int gbeg = off & ~63;
-  const int gend = off + n;
-  for (int g = gbeg + (int)threadIdx.x; g - (int)threadIdx.x < gend; g += kBlock) {
+  // 2. round-0 membership = FP32 inliers of the best hypothesis: global mask
+  //    output and LDS bits (64 consecutive points per wave, 64-aligned)
+  double k0cnt[1] = {0.0};
+  for (int base = 0; base < n; base += BLOCK) {
+    const int i = base + (int)threadIdx.x;
     bool in = false;
-    double d[3] = {0.0, 0.0, 0.0};
-    if (g >= off && g < gend) {
-      const float* q = xyz + 3 * (size_t)g;
+    if (i < n) {
+      const float* q = P + 3 * (size_t)i;
       float u, v, s;
       local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
       in = is_inlier(u, v, s, h);
-      if (in) {
-        d[0] = (double)q[0] - (double)f.m[0];
-        d[1] = (double)q[1] - (double)f.m[1];
-        d[2] = (double)q[2] - (double)f.m[2];
+    }
+    const uint64_t b = __builtin_amdgcn_ballot_w64(in);
+    if (lane == 0) {
+      const int wbase = (base + wave * 64) >> 5;
+      bits[wbase] = (uint32_t)b;
+      bits[wbase + 1] = (uint32_t)(b >> 32);
+    }
+    if (in) k0cnt[0] += 1.0;
+  }
+  if (mask) {
+    // global mask: iterate 64-aligned global indices so that each wave owns
+    // two whole 32-bit words (or-ed atomically: neighbouring clusters share
+    // boundary words)
+    const int gend = off + n;
+    for (int gb = (off & ~63); gb < gend; gb += BLOCK) {
+      const int g = gb + (int)threadIdx.x;
+      bool in = false;
+      if (g >= off && g < gend) {
+        const float* q = xyz + 3 * (size_t)g;
+        float u, v, s;
+        local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
+        in = is_inlier(u, v, s, h);
+      }
+      const uint64_t b = __builtin_amdgcn_ballot_w64(in);
+      if (lane == 0 && b) {
+        const int w0 = (gb + wave * 64) >> 5;
+        if ((uint32_t)b) atomicOr(mask + w0, (uint32_t)b);
+        if ((uint32_t)(b >> 32)) atomicOr(mask + w0 + 1, (uint32_t)(b >> 32));
       }
     }
-    const unsigned long long bits = __ballot(in);
-    if (mask && lane == 0 && bits) {
-      const int w0 = (g - lane) >> 5;
-      const uint32_t lo = (uint32_t)bits;
-      const uint32_t hi = (uint32_t)(bits >> 32);
-      if (lo) atomicOr(mask + w0, lo);
-      if (hi) atomicOr(mask + w0 + 1, hi);
-    }
-    if (in) {
-      acc[0] += d[0];
-      acc[1] += d[1];
-      acc[2] += d[2];
-      acc[3] += 1.0;
-    }
   }
-  block_sum<4>(acc, red);
-  double k = acc[3];
-  if (k < 3.0 || !h.valid) {
-    if (threadIdx.x == 0) write_invalid(out, k);
+  bsum<BLOCK, 1>(k0cnt, red);
+  if (k0cnt[0] < 3.0 || !h.valid) {
+    if (threadIdx.x == 0) write_invalid(out, k0cnt[0]);
     return;
   }
 
-  // 3. LO-RANSAC refit: rounds of Gauss-Newton on the 5-parameter geometric
-  //    cylinder residual over the current inliers, then FP64 re-selection.
+  // 3. LO refit
   CylModel M;
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     M.a[i] = f.axis[i];
     M.c[i] = (double)h.cu * (double)f.e1[i] + (double)h.cv * (double)f.e2[i];
   }
   M.r = (double)h.r;
-  CylModel sel = M;   // membership model of rounds >= 1
-  bool fp32_sel = true;
-  double mu[3] = {acc[0] / k, acc[1] / k, acc[2] / k};
   const double dthr = (double)thr;
-  for (int round = 0; round < refit_rounds; ++round) {
+  bool stop = false;
+  for (int round = 0; round < refit_rounds && !stop; ++round) {
     for (int it = 0; it < refit_iters; ++it) {
-      if (k < 5.0) break;
-      const double ht = (mu[0] - M.c[0]) * M.a[0] + (mu[1] - M.c[1]) * M.a[1] +
-                        (mu[2] - M.c[2]) * M.a[2];
-      for (int i = 0; i < 3; ++i) M.c[i] += ht * M.a[i];
       double e1[3], e2[3];
       sl_basis(M.a, e1, e2);
-      double g[20];
+      double g[21];
 #pragma unroll
-      for (int i = 0; i < 20; ++i) g[i] = 0.0;
-      for (int i = threadIdx.x; i < n; i += kBlock) {
-        const float* q = P + 3 * (size_t)i;
-        const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
-                             (double)q[2] - (double)f.m[2]};
-        if (!member(fp32_sel, f, h, sel, dthr, q, x)) continue;
-        double w[3], hh;
-        const double dd = cyl_dist(M, x, w, &hh);
-        const double rho = dd + M.r;
-        if (!(rho > 0.0)) continue;
-        const double n1 = (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]) / rho;
-        const double n2 = (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]) / rho;
-        const double J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0};
-        int t = 0;
+      for (int t = 0; t < 21; ++t) g[t] = 0.0;
+      const bool reselect = (round > 0 && it == 0);
+      for (int base = 0; base < n; base += BLOCK) {
+        const int i = base + (int)threadIdx.x;
+        bool in = false;
+        double dd = 0.0, hh = 0.0, w[3] = {0.0, 0.0, 0.0};
+        if (i < n) {
+          const float* q = P + 3 * (size_t)i;
+          const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
+                               (double)q[2] - (double)f.m[2]};
+          if (reselect) {
+            dd = cyl_dist(M, x, w, &hh);
+            in = dd < dthr && dd > -dthr;
+          } else {
+            in = (bits[i >> 5] >> (i & 31)) & 1u;
+            if (in) dd = cyl_dist(M, x, w, &hh);
+          }
+        }
+        if (reselect) {
+          // bits of point i are written and read by the same wave only
+          const uint64_t b = __builtin_amdgcn_ballot_w64(in);
+          if (lane == 0) {
+            const int wbase = (base + wave * 64) >> 5;
+            bits[wbase] = (uint32_t)b;
+            bits[wbase + 1] = (uint32_t)(b >> 32);
+          }
+        }
+        if (in) {
+          g[20] += 1.0;
+          const double rho = dd + M.r;
+          if (rho > 0.0) {
+            const double inv = 1.0 / rho;
+            const double n1 = (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]) * inv;
+            const double n2 = (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]) * inv;
+            const double J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0};
+            int t = 0;
 #pragma unroll
-        for (int a = 0; a < 5; ++a)
+            for (int a = 0; a < 5; ++a)
 #pragma unroll
-          for (int b = a; b < 5; ++b) g[t++] += J[a] * J[b];
+              for (int bb = a; bb < 5; ++bb) g[t++] += J[a] * J[bb];
 #pragma unroll
-        for (int a = 0; a < 5; ++a) g[15 + a] += J[a] * dd;
+            for (int a = 0; a < 5; ++a) g[15 + a] += J[a] * dd;
+          }
+        }
       }
-      block_sum<20>(g, red);
-      double N[25], rhs[5], dl[5];
+      bsum<BLOCK, 21>(g, red);
+      if (g[20] < 5.0) {
+        stop = true;
+        break;
+      }
+      double A[5][5], rhs[5], dl[5];
       {
         int t = 0;
+#pragma unroll
         for (int a = 0; a < 5; ++a)
-          for (int b = a; b < 5; ++b) {
-            N[a * 5 + b] = g[t];
-            N[b * 5 + a] = g[t];
+#pragma unroll
+          for (int bb = a; bb < 5; ++bb) {
+            A[a][bb] = g[t];
+            A[bb][a] = g[t];
             ++t;
           }
+#pragma unroll
         for (int a = 0; a < 5; ++a) rhs[a] = -g[15 + a];
       }
-      if (!solve_n(N, rhs, dl, 5)) break;
+      if (!cholesky_solve<5>(A, rhs, dl)) break;
       double na[3];
+#pragma unroll
       for (int i = 0; i < 3; ++i) na[i] = M.a[i] + dl[0] * e1[i] + dl[1] * e2[i];
       const double nl = sqrt(na[0] * na[0] + na[1] * na[1] + na[2] * na[2]);
+#pragma unroll
       for (int i = 0; i < 3; ++i) {
         M.a[i] = na[i] / nl;
         M.c[i] = M.c[i] + dl[2] * e1[i] + dl[3] * e2[i];
       }
       M.r = M.r + dl[4];
-      if (!(M.r > 0.0)) break;
+      if (!(M.r > 0.0)) {
+        stop = true;
+        break;
+      }
     }
-    if (!(M.r > 0.0)) break;
-    // re-selection under the new model: count + centroid
-    double cacc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += kBlock) {
+  }
+
+  // 4. final inliers under the final model: count + centroid
+  double fin[4] = {0.0, 0.0, 0.0, 0.0};
+  if (M.r > 0.0) {
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
       const float* q = P + 3 * (size_t)i;
       const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
                            (double)q[2] - (double)f.m[2]};
       double w[3], hh;
       const double dd = cyl_dist(M, x, w, &hh);
       if (dd < dthr && dd > -dthr) {
-        cacc[0] += x[0];
-        cacc[1] += x[1];
-        cacc[2] += x[2];
-        cacc[3] += 1.0;
+        fin[0] += x[0];
+        fin[1] += x[1];
+        fin[2] += x[2];
+        fin[3] += 1.0;
       }
     }
-    block_sum<4>(cacc, red);
-    k = cacc[3];
-    sel = M;
-    fp32_sel = false;
-    if (k < 3.0) break;
-    mu[0] = cacc[0] / k;
-    mu[1] = cacc[1] / k;
-    mu[2] = cacc[2] / k;
   }
+  bsum<BLOCK, 4>(fin, red);
+  const double k = fin[3];
+  if (threadIdx.x != 0) return;
   if (!(M.r > 0.0) || k < 3.0) {
-    if (threadIdx.x == 0) write_invalid(out, k);
+    write_invalid(out, k);
     return;
   }
-  if (threadIdx.x == 0) {
-    // root: axis point at the height of the final inlier centroid
-    const double ht = (mu[0] - M.c[0]) * M.a[0] + (mu[1] - M.c[1]) * M.a[1] +
-                      (mu[2] - M.c[2]) * M.a[2];
-    double ray[3] = {M.a[0], M.a[1], M.a[2]};
-    int big = 0;
-    for (int i = 1; i < 3; ++i)
-      if (fabs(ray[i]) > fabs(ray[big])) big = i;
-    if (ray[big] < 0.0)
-      for (int i = 0; i < 3; ++i) ray[i] = -ray[i];
-    for (int i = 0; i < 3; ++i) {
-      out[i] = (double)f.m[i] + M.c[i] + ht * M.a[i];
-      out[3 + i] = ray[i];
-    }
-    out[6] = M.r;
-    out[7] = k;
+  const double ht = (fin[0] / k - M.c[0]) * M.a[0] + (fin[1] / k - M.c[1]) * M.a[1] +
+                    (fin[2] / k - M.c[2]) * M.a[2];
+  double ray[3] = {M.a[0], M.a[1], M.a[2]};
+  sl_sign_normalise(ray);
+  for (int i = 0; i < 3; ++i) {
+    out[i] = (double)f.m[i] + M.c[i] + ht * M.a[i];
+    out[3 + i] = ray[i];
   }
+  out[6] = M.r;
+  out[7] = k;
 }
 
 }  // namespace
@@ -650,6 +690,9 @@ struct sloam_cyl_plan {
   int tile = 0;
   int n_work = 0;
   int4* d_work = nullptr;
+  int32_t* d_lists = nullptr;   // small-cluster list then large-cluster list
+  int n_small = 0, n_large = 0;
+  int bits_small = 0, bits_large = 0;  // LDS bytes of the membership bits
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
 };
@@ -708,8 +751,36 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
     }
   }
   p->n_work = (int)work.size();
+  // refit lists: clusters up to kSmallRefit points get one wave, larger ones
+  // a 256-thread workgroup; both in decreasing size
+  std::vector<int32_t> small, large;
+  int max_small = 0, max_large = 0;
+  for (int c : order) {
+    const int len = h_offsets[c + 1] - h_offsets[c];
+    if (len <= kSmallRefit) {
+      small.push_back(c);
+      max_small = std::max(max_small, len);
+    } else {
+      large.push_back(c);
+      max_large = std::max(max_large, len);
+    }
+  }
+  p->n_small = (int)small.size();
+  p->n_large = (int)large.size();
+  p->bits_small = round_up(std::max(max_small, 1), 64) / 8;
+  p->bits_large = round_up(std::max(max_large, 1), 256) / 8;
+  std::vector<int32_t> lists(small);
+  lists.insert(lists.end(), large.begin(), large.end());
   int st = SLOAM_OK;
-  if (p->n_work > 0) {
+  if (!lists.empty()) {
+    st = sloam_dev_alloc(ctx, (void**)&p->d_lists, sizeof(int32_t) * lists.size());
+    if (!st) {
+      hipError_t e = hipMemcpy(p->d_lists, lists.data(), sizeof(int32_t) * lists.size(),
+                               hipMemcpyHostToDevice);
+      if (e != hipSuccess) st = sloam_record_hip_error(ctx, e, "hipMemcpy(lists)");
+    }
+  }
+  if (!st && p->n_work > 0) {
     st = sloam_dev_alloc(ctx, (void**)&p->d_work, sizeof(int4) * work.size());
     if (!st) {
       hipError_t e = hipMemcpy(p->d_work, work.data(), sizeof(int4) * work.size(),
@@ -733,6 +804,7 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
 void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
   if (!p) return;
   sloam_dev_free(p->d_work);
+  sloam_dev_free(p->d_lists);
   sloam_dev_free(p->d_frames);
   sloam_dev_free(p->d_counts);
   delete p;
@@ -770,9 +842,16 @@ int sloam_cyl_plan_exec(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
   SLOAM_HIP(ctx, hipGetLastError());
   {
     SloamKernelScope ks(ctx, "cyl_refit");
-    hipLaunchKernelGGL(cyl_refit_kernel, dim3(p->n_clusters), dim3(kBlock), 0, ctx->stream, d_xyz,
-                       d_offsets, p->d_frames, counts, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1,
-                       tag, cfg.refit_rounds, cfg.refit_iters, d_mask, d_best, d_model);
+    if (p->n_small > 0)
+      hipLaunchKernelGGL(cyl_refit_kernel<64>, dim3(p->n_small), dim3(64), p->bits_small,
+                         ctx->stream, d_xyz, d_offsets, p->d_frames, counts, p->d_lists,
+                         cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, cfg.refit_rounds,
+                         cfg.refit_iters, d_mask, d_best, d_model);
+    if (p->n_large > 0)
+      hipLaunchKernelGGL(cyl_refit_kernel<256>, dim3(p->n_large), dim3(256), p->bits_large,
+                         ctx->stream, d_xyz, d_offsets, p->d_frames, counts,
+                         p->d_lists + p->n_small, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1,
+                         tag, cfg.refit_rounds, cfg.refit_iters, d_mask, d_best, d_model);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;

@@ -71,7 +71,7 @@ class Context:
 class CylinderFitPlan:
     """Batched point-level cylinder RANSAC over clusters (sloam_cyl_plan)."""
 
-    def __init__(self, ctx, offsets, n_hyp=1024, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=3,
+    def __init__(self, ctx, offsets, n_hyp=1024, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=2,
                  seed=0x5EED, stream_tag=0, hyp_chunk=0, tile_points=0):
         self.ctx = ctx
         self.lib = ctx.lib
