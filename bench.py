@@ -33,7 +33,71 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--hyp-chunk", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--no-pose", action="store_true", help="skip the batched pose leg")
+    ap.add_argument("--pose-steps", type=int, default=10)
     return ap.parse_args()
+
+
+def bench_pose(ctx, args):
+    """Batched pose estimation: 64 problems x (500 cylinders + 20 ground
+    patches), ~2M points, 10 LM iterations each; one step = one solve of all
+    64 problems.  Reports LM iterations/s and the residual kernel's HBM rate."""
+    import time as _t
+
+    import numpy as np
+    import torch
+
+    from sloam_amd import synth
+    from sloam_amd.gpu import PosePlan
+
+    prob = synth.make_pose_problems(64, seed=2)
+    dev = torch.device("cuda", 0)
+    iters = 10
+    plan = PosePlan(ctx, prob["res_offsets"], len(prob["cyl"]), len(prob["planes"]),
+                    max_iters=iters)
+    t = {k: torch.as_tensor(np.ascontiguousarray(prob[k])).to(dev)
+         for k in ("points", "cyl", "cyl_offsets", "planes", "plane_offsets", "pose_init")}
+    out = plan.alloc_outputs()
+    torch.cuda.synchronize()
+    for _ in range(2):
+        plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+                  t["pose_init"], out)
+    ctx.sync()
+    stream = ctx.torch_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    e0.record(stream)
+    for _ in range(args.pose_steps):
+        plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+                  t["pose_init"], out)
+    e1.record(stream)
+    ctx.sync()
+    ctx.enable_timing(False)
+    ms = e0.elapsed_time(e1) / args.pose_steps
+    ev_ms, ev_n = ctx.kernel_time("pose_eval")
+    sv_ms, sv_n = ctx.kernel_time("pose_solve")
+    n_res = int(prob["res_offsets"][-1])
+    eval_ms = ev_ms / max(ev_n, 1)
+    # algorithmic bytes per evaluation: 16 B per residual (xyz + landmark id)
+    # + the FP32 landmark tables (32 B per cylinder, 16 B per plane)
+    eval_bytes = 16.0 * n_res + 32.0 * len(prob["cyl"]) + 16.0 * len(prob["planes"])
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    dt, dr = synth.pose_error(res["pose"], prob["pose_true"])
+    plan.close()
+    return {
+        "workload": f"64 problems x (500 cylinders + 20 ground patches), {n_res} residuals, "
+                    f"{iters} LM iterations",
+        "ms_per_solve": ms,
+        "lm_iters_per_s": 64 * iters / (ms / 1e3),
+        "residuals_per_s": n_res * (iters + 1) / (ms / 1e3),
+        "eval_kernel_ms": eval_ms,
+        "solve_kernel_ms": sv_ms / max(sv_n, 1),
+        "eval_hbm_gbs": eval_bytes / (eval_ms / 1e3) / 1e9,
+        "eval_hbm_frac": eval_bytes / (eval_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "pose_err_vs_truth": {"max_m": float(dt.max()), "max_deg": float(np.degrees(dr.max()))},
+    }
 
 
 def main():
@@ -109,6 +173,10 @@ def main():
     model = out["model"].cpu().numpy()
     rerr = np.abs(model[:, 6] - d["radius"])
 
+    pose = None
+    if not args.no_pose:
+        pose = bench_pose(ctx, args)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -171,6 +239,7 @@ def main():
         "radius_err_vs_truth_cm": {"median": float(np.median(rerr) * 100),
                                    "p90": float(np.percentile(rerr, 90) * 100)},
         "cpu_baseline": cpu,
+        "pose": pose,
     }
     print(json.dumps(line), flush=True)
     plan.close()

@@ -119,6 +119,48 @@ int sloam_cyl_plan_exec(sloam_cyl_plan* plan, const float* d_xyz, const int32_t*
                         int32_t* d_counts, int32_t* d_samples, uint32_t* d_mask,
                         int32_t* d_best, double* d_model);
 
+/* ---- batched pose estimation (Levenberg-Marquardt, Huber loss) ------------
+ * Replaces: sloam::sloam::OptimizePose (sloam/src/core/sloam.cpp:174-255) and
+ * its residual functors CylinderCost (sloam/include/objects/cylinder.h:90-132)
+ * and PlaneCost (sloam/include/objects/plane.h:76-116), solved there by Ceres
+ * (LM, DENSE_QR, HuberLoss(0.1), max 50 iterations).
+ * Each problem k owns residuals [res_offsets[k], res_offsets[k+1]) of
+ * d_points, cylinders [cyl_offsets[k], ...) and planes [plane_offsets[k], ...).
+ * d_points: 4 floats per residual: x, y, z (sensor frame) and the landmark
+ *   reference as int32 bits: id >= 0 -> cylinder id, id < 0 -> plane -(id+1)
+ *   (global indices into d_cyl / d_planes).
+ * d_cyl: 8 doubles per cylinder (map frame): root[3], ray[3], radius, unused.
+ * d_planes: 4 doubles per plane (map frame): a, b, c, d of a x + b y + c z + d.
+ * d_pose_in / d_pose_out: 7 doubles per problem: t[3], q[4] (x, y, z, w),
+ *   the map <- sensor transform (lp = q * p + t).
+ * d_status (nullable): 2 int32 per problem: termination (0 CONVERGENCE,
+ *   1 NO_CONVERGENCE) and iterations; d_cost (nullable): 2 doubles per problem:
+ *   initial and final cost 1/2 sum rho(r^2).
+ * Residuals and Jacobians are FP32 (relative to the initial translation), the
+ * normal equations are summed with error-free FP32 transformations and solved
+ * in FP64.                                                                  */
+typedef struct sloam_pose_cfg {
+  int32_t max_iters;             /* LM iterations (reference: 50)             */
+  int32_t stop_on_convergence;   /* 1: stop at convergence; 0: run max_iters  */
+  double huber_delta;            /* Huber loss scale (reference: 0.1)         */
+  double function_tolerance;     /* |dF| / F (Ceres default 1e-6)             */
+  double gradient_tolerance;     /* ||g||_inf (Ceres default 1e-10)           */
+  double parameter_tolerance;    /* ||dx|| / (||x|| + tol) (1e-8)             */
+  double initial_radius;         /* trust region (Ceres default 1e4)          */
+  double min_relative_decrease;  /* step acceptance (Ceres default 1e-3)      */
+} sloam_pose_cfg;
+
+typedef struct sloam_pose_plan sloam_pose_plan;
+
+int sloam_pose_plan_create(sloam_ctx* ctx, const int32_t* h_res_offsets, int32_t n_problems,
+                           int32_t n_cylinders, int32_t n_planes, const sloam_pose_cfg* cfg,
+                           sloam_pose_plan** out);
+void sloam_pose_plan_destroy(sloam_pose_plan* plan);
+int sloam_pose_plan_exec(sloam_pose_plan* plan, const float* d_points, const double* d_cyl,
+                         const int32_t* d_cyl_offsets, const double* d_planes,
+                         const int32_t* d_plane_offsets, const double* d_pose_in,
+                         double* d_pose_out, int32_t* d_status, double* d_cost);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,8 @@ def _declare(lib):
                                          C.c_int, C.c_uint64, C.c_int, P, P, P, P, P, C.c_int]
     lib.oracle_cyl_distances_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float,
                                               C.c_float, C.c_uint64, C.c_int, P]
+    lib.oracle_pose_residuals.argtypes = [P, C.c_int, P, P, P, P, P]
+    lib.oracle_pose_lm_batch.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P, C.c_int]
 
 
 def _p(a):
@@ -126,3 +128,39 @@ def cyl_distances_fp64(xyz, cluster, h, thr, max_radius, seed, tag=0):
     st = load().oracle_cyl_distances_fp64(_p(x), len(x), cluster, h, thr, max_radius, seed, tag,
                                           _p(d))
     return (d if st == 0 else None)
+
+
+POSE_DEFAULTS = dict(max_iters=10, stop_on_convergence=False, huber_delta=0.1,
+                     function_tolerance=1e-6, gradient_tolerance=1e-10,
+                     parameter_tolerance=1e-8, initial_radius=1e4, min_relative_decrease=1e-3)
+
+
+def pose_residuals(points, cyl, planes, pose):
+    pts = np.ascontiguousarray(points, np.float32)
+    n = len(pts)
+    r = np.zeros(n)
+    J = np.zeros((n, 6))
+    load().oracle_pose_residuals(_p(pts), n, _p(np.ascontiguousarray(cyl, np.float64)),
+                                 _p(np.ascontiguousarray(planes, np.float64)),
+                                 _p(np.ascontiguousarray(pose, np.float64)), _p(r), _p(J))
+    return r, J
+
+
+def pose_lm_batch(prob, threads=1, **kw):
+    c = dict(POSE_DEFAULTS)
+    c.update(kw)
+    cfg_i = np.array([c["max_iters"], int(c["stop_on_convergence"])], np.int32)
+    cfg_d = np.array([c["huber_delta"], c["function_tolerance"], c["gradient_tolerance"],
+                      c["parameter_tolerance"], c["initial_radius"],
+                      c["min_relative_decrease"]], np.float64)
+    P = len(prob["res_offsets"]) - 1
+    pose = np.zeros((P, 7))
+    status = np.zeros((P, 2), np.int32)
+    cost = np.zeros((P, 2))
+    pts = np.ascontiguousarray(prob["points"], np.float32)
+    load().oracle_pose_lm_batch(_p(pts), _p(np.ascontiguousarray(prob["res_offsets"], np.int32)),
+                                P, _p(np.ascontiguousarray(prob["cyl"], np.float64)),
+                                _p(np.ascontiguousarray(prob["planes"], np.float64)),
+                                _p(np.ascontiguousarray(prob["pose_init"], np.float64)),
+                                _p(cfg_i), _p(cfg_d), _p(pose), _p(status), _p(cost), threads)
+    return {"pose": pose, "status": status, "cost": cost}

@@ -28,6 +28,19 @@ class CylFitCfg(C.Structure):
 
 CYL_MODEL_STRIDE = 8
 
+
+class PoseCfg(C.Structure):
+    _fields_ = [
+        ("max_iters", C.c_int32),
+        ("stop_on_convergence", C.c_int32),
+        ("huber_delta", C.c_double),
+        ("function_tolerance", C.c_double),
+        ("gradient_tolerance", C.c_double),
+        ("parameter_tolerance", C.c_double),
+        ("initial_radius", C.c_double),
+        ("min_relative_decrease", C.c_double),
+    ]
+
 _P = C.c_void_p
 _I32 = C.c_int32
 _I64 = C.c_int64
@@ -46,6 +59,10 @@ PROTOTYPES = {
     "sloam_cyl_plan_create": (C.c_int, [_P, _P, _I32, C.POINTER(CylFitCfg), C.POINTER(_P)]),
     "sloam_cyl_plan_destroy": (None, [_P]),
     "sloam_cyl_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "sloam_pose_plan_create": (C.c_int, [_P, _P, _I32, _I32, _I32, C.POINTER(PoseCfg),
+                                         C.POINTER(_P)]),
+    "sloam_pose_plan_destroy": (None, [_P]),
+    "sloam_pose_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

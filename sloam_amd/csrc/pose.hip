@@ -1,0 +1,593 @@
+// Batched pose estimation (Levenberg-Marquardt over point-to-cylinder and
+// point-to-plane residuals) for gfx950.
+//
+// Reference: sloam::sloam::OptimizePose (sloam/src/core/sloam.cpp:174-255) with
+// CylinderCost (sloam/include/objects/cylinder.h:90-132) and PlaneCost
+// (sloam/include/objects/plane.h:76-116), HuberLoss(0.1), Ceres LM.
+//
+// Per LM iteration two kernels run on the context stream:
+//   pose_eval   one workgroup per chunk of 256 * kRpt residuals of one problem:
+//               FP32 residual r, 1x6 Jacobian [d/dt, d/dtheta] (left
+//               perturbation), Huber IRLS weight w = min(1, delta/|r|); sums
+//               J'WJ (21), J'Wr (6) and the cost (1) per thread in FP32, then an
+//               error-free (TwoSum) tree over the workgroup; the workgroup's
+//               28 (sum, compensation) pairs go to HBM as FP64.
+//   pose_solve  one wave per problem: FP64 sum of the chunk partials in chunk
+//               order, then the LM trust-region logic and the FP64 6x6
+//               Cholesky solve of (J'WJ + D/radius) dx = -J'Wr; writes the next
+//               pose to evaluate.
+// Coordinates are evaluated relative to a per-problem anchor (the initial
+// translation) so that FP32 residuals keep sub-micrometre resolution.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+namespace {
+
+constexpr int kPBlock = 256;
+constexpr int kRpt = 8;  // residuals per thread per workgroup
+constexpr int kSums = 28;
+
+struct LMState {
+  double x[7];   // accepted pose: t, q (x, y, z, w)
+  double xe[7];  // pose to evaluate next
+  double anchor[3];
+  double F0, F;
+  double A[21];
+  double g[6];
+  double radius, factor;
+  double model, dn, xn;  // of the pending candidate
+  int32_t iter, term, done, phase;  // phase 0: next evaluation is the initial one
+};
+
+struct PoseCfgDev {
+  int32_t max_iters, stop_conv;
+  double delta, ftol, gtol, ptol, radius0, min_rel;
+};
+
+// --------------------------------------------------------------------------
+// small FP64 helpers (one thread)
+// --------------------------------------------------------------------------
+__device__ inline void quat_to_R(const double q[4], double R[3][3]) {
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  R[0][0] = 1 - 2 * (y * y + z * z);
+  R[0][1] = 2 * (x * y - z * w);
+  R[0][2] = 2 * (x * z + y * w);
+  R[1][0] = 2 * (x * y + z * w);
+  R[1][1] = 1 - 2 * (x * x + z * z);
+  R[1][2] = 2 * (y * z - x * w);
+  R[2][0] = 2 * (x * z - y * w);
+  R[2][1] = 2 * (y * z + x * w);
+  R[2][2] = 1 - 2 * (x * x + y * y);
+}
+
+__device__ inline void quat_left_update(const double q[4], const double dth[3], double out[4]) {
+  const double ang = sqrt(dth[0] * dth[0] + dth[1] * dth[1] + dth[2] * dth[2]);
+  double d[4];
+  if (ang > 1e-300) {
+    const double s = sin(0.5 * ang) / ang;
+    d[0] = dth[0] * s;
+    d[1] = dth[1] * s;
+    d[2] = dth[2] * s;
+    d[3] = cos(0.5 * ang);
+  } else {
+    d[0] = 0.5 * dth[0];
+    d[1] = 0.5 * dth[1];
+    d[2] = 0.5 * dth[2];
+    d[3] = 1.0;
+  }
+  const double x = d[3] * q[0] + d[0] * q[3] + d[1] * q[2] - d[2] * q[1];
+  const double y = d[3] * q[1] - d[0] * q[2] + d[1] * q[3] + d[2] * q[0];
+  const double z = d[3] * q[2] + d[0] * q[1] - d[1] * q[0] + d[2] * q[3];
+  const double w = d[3] * q[3] - d[0] * q[0] - d[1] * q[1] - d[2] * q[2];
+  const double n = sqrt(x * x + y * y + z * z + w * w);
+  out[0] = x / n;
+  out[1] = y / n;
+  out[2] = z / n;
+  out[3] = w / n;
+}
+
+__device__ inline bool chol6(const double A[6][6], const double b[6], double x[6]) {
+  double L[6][6];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double s = A[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s = s - L[j][k] * L[j][k];
+    ok = ok && (s > 0.0);
+    const double d = sqrt(s);
+    L[j][j] = d;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      double t = A[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t = t - L[i][k] * L[j][k];
+      L[i][j] = t / d;
+    }
+  }
+  double y[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double t = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) t = t - L[i][k] * y[k];
+    y[i] = t / L[i][i];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double t = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) t = t - L[k][i] * x[k];
+    x[i] = t / L[i][i];
+  }
+  return ok;
+}
+
+// error-free transformation: a + b = s + e exactly
+__device__ inline void two_sum(float a, float b, float* s, float* e) {
+  const float ss = a + b;
+  const float bp = ss - a;
+  *e = (a - (ss - bp)) + (b - bp);
+  *s = ss;
+}
+
+// --------------------------------------------------------------------------
+// prep: LM state init + FP32 landmark tables relative to the anchor
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
+    int n_problems, const double* __restrict__ pose_in, const double* __restrict__ cyl,
+    const int32_t* __restrict__ cyl_off, const double* __restrict__ pl,
+    const int32_t* __restrict__ pl_off, PoseCfgDev cfg, LMState* __restrict__ st,
+    float4* __restrict__ cylf, float4* __restrict__ plf) {
+  const int k = blockIdx.x;
+  if (k >= n_problems) return;
+  const double* x0 = pose_in + 7 * (size_t)k;
+  const double ax = x0[0], ay = x0[1], az = x0[2];
+  for (int i = cyl_off[k] + (int)threadIdx.x; i < cyl_off[k + 1]; i += kPBlock) {
+    const double* c = cyl + 8 * (size_t)i;
+    const double rn = sqrt(c[3] * c[3] + c[4] * c[4] + c[5] * c[5]);
+    cylf[2 * (size_t)i] = make_float4((float)(c[0] - ax), (float)(c[1] - ay), (float)(c[2] - az),
+                                      (float)c[6]);
+    cylf[2 * (size_t)i + 1] =
+        make_float4((float)(c[3] / rn), (float)(c[4] / rn), (float)(c[5] / rn), 0.f);
+  }
+  for (int i = pl_off[k] + (int)threadIdx.x; i < pl_off[k + 1]; i += kPBlock) {
+    const double* q = pl + 4 * (size_t)i;
+    const double nn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    const double dr = (q[3] + q[0] * ax + q[1] * ay + q[2] * az) / nn;
+    plf[i] = make_float4((float)(q[0] / nn), (float)(q[1] / nn), (float)(q[2] / nn), (float)dr);
+  }
+  if (threadIdx.x == 0) {
+    LMState s;
+    for (int i = 0; i < 7; ++i) {
+      s.x[i] = x0[i];
+      s.xe[i] = x0[i];
+    }
+    s.anchor[0] = ax;
+    s.anchor[1] = ay;
+    s.anchor[2] = az;
+    s.F0 = s.F = 0.0;
+    for (int i = 0; i < 21; ++i) s.A[i] = 0.0;
+    for (int i = 0; i < 6; ++i) s.g[i] = 0.0;
+    s.radius = cfg.radius0;
+    s.factor = 2.0;
+    s.model = s.dn = s.xn = 0.0;
+    s.iter = 0;
+    s.term = 1;
+    s.done = 0;
+    s.phase = 0;
+    st[k] = s;
+  }
+}
+
+// --------------------------------------------------------------------------
+// eval: residuals, Jacobians, compensated partial sums
+// work item = {problem, first residual, end residual, unused}
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
+    const float4* __restrict__ pts, const int4* __restrict__ work, const LMState* __restrict__ st,
+    const float4* __restrict__ cylf, const float4* __restrict__ plf, float delta,
+    double* __restrict__ partials) {
+  __shared__ float red_s[kPBlock / 64][kSums];
+  __shared__ float red_c[kPBlock / 64][kSums];
+  const int4 w = work[blockIdx.x];
+  const int k = w.x;
+  if (st[k].done) return;
+  // pose to evaluate, relative to the anchor, in FP32
+  float R[3][3], t[3];
+  {
+    double q[4] = {st[k].xe[3], st[k].xe[4], st[k].xe[5], st[k].xe[6]};
+    double Rd[3][3];
+    quat_to_R(q, Rd);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) R[i][j] = (float)Rd[i][j];
+      t[i] = (float)(st[k].xe[i] - st[k].anchor[i]);
+    }
+  }
+  const float d2 = delta * delta;
+  float acc[kSums];
+#pragma unroll
+  for (int i = 0; i < kSums; ++i) acc[i] = 0.f;
+  for (int j = 0; j < kRpt; ++j) {
+    const int i = w.y + j * kPBlock + (int)threadIdx.x;
+    if (i >= w.z) break;
+    const float4 p = pts[i];
+    const int id = __float_as_int(p.w);
+    float rp[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      rp[a] = __builtin_fmaf(R[a][0], p.x, __builtin_fmaf(R[a][1], p.y, R[a][2] * p.z));
+    const float lp[3] = {rp[0] + t[0], rp[1] + t[1], rp[2] + t[2]};
+    float dir[3], r;
+    if (id >= 0) {
+      const float4 L0 = cylf[2 * (size_t)id];
+      const float4 L1 = cylf[2 * (size_t)id + 1];
+      const float v0 = lp[0] - L0.x, v1 = lp[1] - L0.y, v2 = lp[2] - L0.z;
+      const float h = __builtin_fmaf(v0, L1.x, __builtin_fmaf(v1, L1.y, v2 * L1.z));
+      const float w0 = __builtin_fmaf(-h, L1.x, v0);
+      const float w1 = __builtin_fmaf(-h, L1.y, v1);
+      const float w2 = __builtin_fmaf(-h, L1.z, v2);
+      const float rho = sqrtf(__builtin_fmaf(w0, w0, __builtin_fmaf(w1, w1, w2 * w2)));
+      r = rho - L0.w;
+      const float inv = rho > 0.f ? 1.0f / rho : 0.f;
+      dir[0] = w0 * inv;
+      dir[1] = w1 * inv;
+      dir[2] = w2 * inv;
+    } else {
+      const float4 L = plf[-id - 1];
+      const float s = __builtin_fmaf(L.x, lp[0], __builtin_fmaf(L.y, lp[1],
+                                                               __builtin_fmaf(L.z, lp[2], L.w)));
+      r = fabsf(s);
+      const float sg = s < 0.f ? -1.f : 1.f;
+      dir[0] = sg * L.x;
+      dir[1] = sg * L.y;
+      dir[2] = sg * L.z;
+    }
+    float J[6];
+    J[0] = dir[0];
+    J[1] = dir[1];
+    J[2] = dir[2];
+    J[3] = __builtin_fmaf(rp[1], dir[2], -rp[2] * dir[1]);
+    J[4] = __builtin_fmaf(rp[2], dir[0], -rp[0] * dir[2]);
+    J[5] = __builtin_fmaf(rp[0], dir[1], -rp[1] * dir[0]);
+    const float s2 = r * r;
+    float wgt, rho;
+    if (s2 <= d2) {
+      wgt = 1.f;
+      rho = s2;
+    } else {
+      const float ar = fabsf(r);
+      wgt = delta / ar;
+      rho = __builtin_fmaf(2.f * delta, ar, -d2);
+    }
+    float wJ[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) wJ[a] = wgt * J[a];
+    int tt = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = a; b < 6; ++b) {
+        acc[tt] = __builtin_fmaf(wJ[a], J[b], acc[tt]);
+        ++tt;
+      }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) acc[21 + a] = __builtin_fmaf(wJ[a], r, acc[21 + a]);
+    acc[27] = __builtin_fmaf(0.5f, rho, acc[27]);
+  }
+  // error-free tree reduction: (sum, compensation) pairs
+  float cmp[kSums];
+#pragma unroll
+  for (int i = 0; i < kSums; ++i) cmp[i] = 0.f;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int i = 0; i < kSums; ++i) {
+      const float os = __shfl_xor(acc[i], o, 64);
+      const float oc = __shfl_xor(cmp[i], o, 64);
+      float s, e;
+      two_sum(acc[i], os, &s, &e);
+      acc[i] = s;
+      cmp[i] = cmp[i] + oc + e;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < kSums; ++i) {
+      red_s[wave][i] = acc[i];
+      red_c[wave][i] = cmp[i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kSums) {
+    const int i = threadIdx.x;
+    float s = red_s[0][i], c = red_c[0][i];
+    for (int v = 1; v < kPBlock / 64; ++v) {
+      float ns, e;
+      two_sum(s, red_s[v][i], &ns, &e);
+      s = ns;
+      c = c + red_c[v][i] + e;
+    }
+    partials[(size_t)blockIdx.x * kSums + i] = (double)s + (double)c;
+  }
+}
+
+// --------------------------------------------------------------------------
+// solve: FP64 chunk sums + LM trust-region logic, one wave per problem
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void pose_solve_kernel(
+    int n_problems, const int32_t* __restrict__ first_work, const double* __restrict__ partials,
+    PoseCfgDev cfg, LMState* __restrict__ st, double* __restrict__ pose_out,
+    int32_t* __restrict__ status, double* __restrict__ cost) {
+  __shared__ double tot[kSums];
+  const int k = blockIdx.x;
+  if (k >= n_problems) return;
+  if (st[k].done) return;
+  if (threadIdx.x < kSums) {
+    double s = 0.0;
+    for (int wk = first_work[k]; wk < first_work[k + 1]; ++wk)
+      s += partials[(size_t)wk * kSums + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  LMState S = st[k];
+  // evaluation result of S.xe
+  double A[6][6], g[6];
+  {
+    int tt = 0;
+    for (int a = 0; a < 6; ++a)
+      for (int b = a; b < 6; ++b) {
+        A[a][b] = tot[tt];
+        A[b][a] = tot[tt];
+        ++tt;
+      }
+    for (int a = 0; a < 6; ++a) g[a] = tot[21 + a];
+  }
+  const double Fe = tot[27];
+  auto store_eval = [&]() {
+    int tt = 0;
+    for (int a = 0; a < 6; ++a)
+      for (int b = a; b < 6; ++b) S.A[tt++] = A[a][b];
+    for (int a = 0; a < 6; ++a) S.g[a] = g[a];
+    S.F = Fe;
+  };
+  bool done = false;
+  if (S.phase == 0) {
+    store_eval();
+    S.F0 = Fe;
+    double gmax = 0.0;
+    for (int a = 0; a < 6; ++a) gmax = fmax(gmax, fabs(g[a]));
+    if (gmax <= cfg.gtol) {
+      S.term = 0;
+      if (cfg.stop_conv) done = true;
+    }
+  } else {
+    const double change = S.F - Fe;
+    const double q = S.model > 0.0 ? change / S.model : -1.0;
+    if (isfinite(Fe) && q > cfg.min_rel) {
+      for (int i = 0; i < 7; ++i) S.x[i] = S.xe[i];
+      const double Fprev = S.F;
+      store_eval();
+      const double c3 = 2.0 * q - 1.0;
+      S.radius = S.radius / fmax(1.0 / 3.0, 1.0 - c3 * c3 * c3);
+      S.radius = fmin(S.radius, 1e16);
+      S.factor = 2.0;
+      double gmax = 0.0;
+      for (int a = 0; a < 6; ++a) gmax = fmax(gmax, fabs(S.g[a]));
+      if (fabs(change) <= cfg.ftol * Fprev || gmax <= cfg.gtol ||
+          S.dn <= cfg.ptol * (S.xn + cfg.ptol)) {
+        S.term = 0;
+        if (cfg.stop_conv) done = true;
+      }
+    } else {
+      S.radius = S.radius / S.factor;
+      S.factor = S.factor * 2.0;
+      if (S.dn <= cfg.ptol * (S.xn + cfg.ptol)) {
+        S.term = 0;
+        if (cfg.stop_conv) done = true;
+      }
+    }
+  }
+  // next candidate
+  bool have_candidate = false;
+  while (!done && S.iter < cfg.max_iters) {
+    S.iter += 1;
+    double Acur[6][6], Aug[6][6], rhs[6], dx[6];
+    {
+      int tt = 0;
+      for (int a = 0; a < 6; ++a)
+        for (int b = a; b < 6; ++b) {
+          Acur[a][b] = S.A[tt];
+          Acur[b][a] = S.A[tt];
+          ++tt;
+        }
+    }
+    for (int a = 0; a < 6; ++a) {
+      for (int b = 0; b < 6; ++b) Aug[a][b] = Acur[a][b];
+      double dg = Acur[a][a];
+      dg = dg < 1e-6 ? 1e-6 : (dg > 1e32 ? 1e32 : dg);
+      Aug[a][a] += dg / S.radius;
+      rhs[a] = -S.g[a];
+    }
+    if (!chol6(Aug, rhs, dx)) {
+      S.radius = S.radius / S.factor;
+      S.factor = S.factor * 2.0;
+      continue;
+    }
+    double gdx = 0.0, dAd = 0.0;
+    for (int a = 0; a < 6; ++a) {
+      gdx += S.g[a] * dx[a];
+      double tt = 0.0;
+      for (int b = 0; b < 6; ++b) tt += Acur[a][b] * dx[b];
+      dAd += dx[a] * tt;
+    }
+    S.model = -(gdx + 0.5 * dAd);
+    double xn = 0.0, dn = 0.0;
+    for (int a = 0; a < 7; ++a) xn += S.x[a] * S.x[a];
+    for (int a = 0; a < 6; ++a) dn += dx[a] * dx[a];
+    S.xn = sqrt(xn);
+    S.dn = sqrt(dn);
+    for (int a = 0; a < 3; ++a) S.xe[a] = S.x[a] + dx[a];
+    double qn[4];
+    quat_left_update(S.x + 3, dx + 3, qn);
+    for (int a = 0; a < 4; ++a) S.xe[3 + a] = qn[a];
+    have_candidate = true;
+    break;
+  }
+  S.phase = 1;
+  if (!have_candidate) S.done = 1;
+  st[k] = S;
+  for (int i = 0; i < 7; ++i) pose_out[7 * (size_t)k + i] = S.x[i];
+  if (status) {
+    status[2 * k] = S.term;
+    status[2 * k + 1] = S.iter;
+  }
+  if (cost) {
+    cost[2 * k] = S.F0;
+    cost[2 * k + 1] = S.F;
+  }
+}
+
+}  // namespace
+
+struct sloam_pose_plan {
+  sloam_ctx* ctx = nullptr;
+  sloam_pose_cfg cfg{};
+  int32_t n_problems = 0;
+  int64_t n_res = 0;
+  int32_t n_cyl = 0, n_pl = 0;
+  int n_work = 0;
+  int4* d_work = nullptr;
+  int32_t* d_first = nullptr;
+  LMState* d_state = nullptr;
+  double* d_partials = nullptr;
+  float4* d_cylf = nullptr;
+  float4* d_plf = nullptr;
+};
+
+extern "C" {
+
+int sloam_pose_plan_create(sloam_ctx* ctx, const int32_t* h_res_offsets, int32_t n_problems,
+                           int32_t n_cylinders, int32_t n_planes, const sloam_pose_cfg* cfg,
+                           sloam_pose_plan** out) {
+  if (!ctx || !h_res_offsets || !cfg || !out || n_problems < 0 || n_cylinders < 0 ||
+      n_planes < 0)
+    return SLOAM_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (cfg->max_iters < 0 || !(cfg->huber_delta > 0.0) || !(cfg->initial_radius > 0.0))
+    return SLOAM_ERR_INVALID_ARG;
+  if (h_res_offsets[0] != 0) return SLOAM_ERR_INVALID_ARG;
+  for (int k = 0; k < n_problems; ++k)
+    if (h_res_offsets[k + 1] < h_res_offsets[k]) return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  sloam_pose_plan* p = new (std::nothrow) sloam_pose_plan();
+  if (!p) return SLOAM_ERR_ALLOC;
+  p->ctx = ctx;
+  p->cfg = *cfg;
+  p->n_problems = n_problems;
+  p->n_res = h_res_offsets[n_problems];
+  p->n_cyl = n_cylinders;
+  p->n_pl = n_planes;
+  const int chunk = kPBlock * kRpt;
+  std::vector<int4> work;
+  std::vector<int32_t> first(n_problems + 1, 0);
+  for (int k = 0; k < n_problems; ++k) {
+    first[k] = (int32_t)work.size();
+    const int b = h_res_offsets[k], e = h_res_offsets[k + 1];
+    if (e == b) work.push_back(make_int4(k, b, e, 0));  // keeps one (empty) chunk
+    for (int s = b; s < e; s += chunk) work.push_back(make_int4(k, s, std::min(e, s + chunk), 0));
+  }
+  first[n_problems] = (int32_t)work.size();
+  p->n_work = (int)work.size();
+  int st = SLOAM_OK;
+  auto upload = [&](void** dst, const void* src, size_t bytes) {
+    if (st || bytes == 0) return;
+    st = sloam_dev_alloc(ctx, dst, bytes);
+    if (st) return;
+    hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = sloam_record_hip_error(ctx, e, "hipMemcpy(pose plan)");
+  };
+  upload((void**)&p->d_work, work.data(), sizeof(int4) * work.size());
+  upload((void**)&p->d_first, first.data(), sizeof(int32_t) * first.size());
+  if (!st && n_problems > 0) st = sloam_dev_alloc(ctx, (void**)&p->d_state, sizeof(LMState) * n_problems);
+  if (!st && p->n_work > 0)
+    st = sloam_dev_alloc(ctx, (void**)&p->d_partials, sizeof(double) * kSums * p->n_work);
+  if (!st && n_cylinders > 0)
+    st = sloam_dev_alloc(ctx, (void**)&p->d_cylf, sizeof(float4) * 2 * (size_t)n_cylinders);
+  if (!st && n_planes > 0)
+    st = sloam_dev_alloc(ctx, (void**)&p->d_plf, sizeof(float4) * (size_t)n_planes);
+  if (st) {
+    sloam_pose_plan_destroy(p);
+    return st;
+  }
+  *out = p;
+  return SLOAM_OK;
+}
+
+void sloam_pose_plan_destroy(sloam_pose_plan* p) {
+  if (!p) return;
+  sloam_dev_free(p->d_work);
+  sloam_dev_free(p->d_first);
+  sloam_dev_free(p->d_state);
+  sloam_dev_free(p->d_partials);
+  sloam_dev_free(p->d_cylf);
+  sloam_dev_free(p->d_plf);
+  delete p;
+}
+
+int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double* d_cyl,
+                         const int32_t* d_cyl_offsets, const double* d_planes,
+                         const int32_t* d_plane_offsets, const double* d_pose_in,
+                         double* d_pose_out, int32_t* d_status, double* d_cost) {
+  if (!p || !d_cyl_offsets || !d_plane_offsets || !d_pose_in || !d_pose_out)
+    return SLOAM_ERR_INVALID_ARG;
+  if ((p->n_res > 0 && !d_points) || (p->n_cyl > 0 && !d_cyl) || (p->n_pl > 0 && !d_planes))
+    return SLOAM_ERR_INVALID_ARG;
+  sloam_ctx* ctx = p->ctx;
+  if (p->n_problems == 0) return SLOAM_OK;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  const sloam_pose_cfg& c = p->cfg;
+  PoseCfgDev dc;
+  dc.max_iters = c.max_iters;
+  dc.stop_conv = c.stop_on_convergence;
+  dc.delta = c.huber_delta;
+  dc.ftol = c.function_tolerance;
+  dc.gtol = c.gradient_tolerance;
+  dc.ptol = c.parameter_tolerance;
+  dc.radius0 = c.initial_radius;
+  dc.min_rel = c.min_relative_decrease;
+  {
+    SloamKernelScope ks(ctx, "pose_prep");
+    hipLaunchKernelGGL(pose_prep_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
+                       p->n_problems, d_pose_in, d_cyl, d_cyl_offsets, d_planes, d_plane_offsets,
+                       dc, p->d_state, p->d_cylf, p->d_plf);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  for (int it = 0; it <= c.max_iters; ++it) {
+    {
+      SloamKernelScope ks(ctx, "pose_eval");
+      hipLaunchKernelGGL(pose_eval_kernel, dim3(p->n_work), dim3(kPBlock), 0, ctx->stream,
+                         reinterpret_cast<const float4*>(d_points), p->d_work, p->d_state,
+                         p->d_cylf, p->d_plf, (float)c.huber_delta, p->d_partials);
+    }
+    SLOAM_HIP(ctx, hipGetLastError());
+    {
+      SloamKernelScope ks(ctx, "pose_solve");
+      hipLaunchKernelGGL(pose_solve_kernel, dim3(p->n_problems), dim3(64), 0, ctx->stream,
+                         p->n_problems, p->d_first, p->d_partials, dc, p->d_state, d_pose_out,
+                         d_status, d_cost);
+    }
+    SLOAM_HIP(ctx, hipGetLastError());
+  }
+  return SLOAM_OK;
+}
+
+}  // extern "C"

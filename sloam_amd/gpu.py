@@ -9,7 +9,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import CYL_MODEL_STRIDE, CylFitCfg, check
+from ._lib import CYL_MODEL_STRIDE, CylFitCfg, PoseCfg, check
 
 
 def _ptr(t):
@@ -143,5 +143,72 @@ def fit_cylinders(ctx, xyz, offsets_host, **kw):
     res = {k: v.cpu().numpy() for k, v in out.items()}
     if "mask" in res:
         res["mask"] = res["mask"].view(np.uint32)
+    plan.close()
+    return res
+
+
+class PosePlan:
+    """Batched LM pose estimation over independent problems (sloam_pose_plan)."""
+
+    def __init__(self, ctx, res_offsets, n_cylinders, n_planes, max_iters=10,
+                 stop_on_convergence=False, huber_delta=0.1, function_tolerance=1e-6,
+                 gradient_tolerance=1e-10, parameter_tolerance=1e-8, initial_radius=1e4,
+                 min_relative_decrease=1e-3):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        off = np.ascontiguousarray(res_offsets, dtype=np.int32)
+        self.n_problems = len(off) - 1
+        self.n_res = int(off[-1])
+        self.cfg = PoseCfg(max_iters=max_iters, stop_on_convergence=int(stop_on_convergence),
+                           huber_delta=huber_delta, function_tolerance=function_tolerance,
+                           gradient_tolerance=gradient_tolerance,
+                           parameter_tolerance=parameter_tolerance,
+                           initial_radius=initial_radius,
+                           min_relative_decrease=min_relative_decrease)
+        h = C.c_void_p()
+        check(self.lib.sloam_pose_plan_create(ctx.handle, off.ctypes.data_as(C.c_void_p),
+                                              self.n_problems, int(n_cylinders), int(n_planes),
+                                              C.byref(self.cfg), C.byref(h)), ctx.handle)
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.sloam_pose_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc_outputs(self):
+        dev = torch.device("cuda", self.ctx.device)
+        return {
+            "pose": torch.empty((self.n_problems, 7), dtype=torch.float64, device=dev),
+            "status": torch.empty((self.n_problems, 2), dtype=torch.int32, device=dev),
+            "cost": torch.empty((self.n_problems, 2), dtype=torch.float64, device=dev),
+        }
+
+    def exec(self, points, cyl, cyl_off, planes, plane_off, pose_in, out):
+        check(self.lib.sloam_pose_plan_exec(
+            self.handle, _ptr(points), _ptr(cyl), _ptr(cyl_off), _ptr(planes), _ptr(plane_off),
+            _ptr(pose_in), _ptr(out["pose"]), _ptr(out.get("status")), _ptr(out.get("cost"))),
+            self.ctx.handle)
+        return out
+
+
+def solve_poses(ctx, prob, **kw):
+    """One-shot convenience over a dict from synth.make_pose_problems."""
+    dev = torch.device("cuda", ctx.device)
+    plan = PosePlan(ctx, prob["res_offsets"], len(prob["cyl"]), len(prob["planes"]), **kw)
+    t = {k: torch.as_tensor(np.ascontiguousarray(prob[k])).to(dev)
+         for k in ("points", "cyl", "cyl_offsets", "planes", "plane_offsets", "pose_init")}
+    out = plan.alloc_outputs()
+    torch.cuda.synchronize()
+    plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+              t["pose_init"], out)
+    ctx.sync()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
     plan.close()
     return res

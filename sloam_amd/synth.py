@@ -92,3 +92,133 @@ def make_clusters(n_clusters=2000, seed=1, n_min=50, n_max=5000, r_min=0.05, r_m
         "radius": radius,
         "inlier": lab,
     }
+
+
+def _rotvec_to_R(v):
+    th = np.linalg.norm(v)
+    if th < 1e-15:
+        return np.eye(3)
+    k = v / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(th) * K + (1 - math.cos(th)) * K @ K
+
+
+def R_to_quat(R):
+    """Rotation matrix -> unit quaternion (x, y, z, w), w >= 0."""
+    tr = R[0, 0] + R[1, 1] + R[2, 2]
+    if tr > 0:
+        s = math.sqrt(tr + 1.0) * 2
+        w, x, y, z = 0.25 * s, (R[2, 1] - R[1, 2]) / s, (R[0, 2] - R[2, 0]) / s, (R[1, 0] - R[0, 1]) / s
+    elif R[0, 0] > R[1, 1] and R[0, 0] > R[2, 2]:
+        s = math.sqrt(1.0 + R[0, 0] - R[1, 1] - R[2, 2]) * 2
+        w, x, y, z = (R[2, 1] - R[1, 2]) / s, 0.25 * s, (R[0, 1] + R[1, 0]) / s, (R[0, 2] + R[2, 0]) / s
+    elif R[1, 1] > R[2, 2]:
+        s = math.sqrt(1.0 + R[1, 1] - R[0, 0] - R[2, 2]) * 2
+        w, x, y, z = (R[0, 2] - R[2, 0]) / s, (R[0, 1] + R[1, 0]) / s, 0.25 * s, (R[1, 2] + R[2, 1]) / s
+    else:
+        s = math.sqrt(1.0 + R[2, 2] - R[0, 0] - R[1, 1]) * 2
+        w, x, y, z = (R[1, 0] - R[0, 1]) / s, (R[0, 2] + R[2, 0]) / s, (R[1, 2] + R[2, 1]) / s, 0.25 * s
+    q = np.array([x, y, z, w])
+    q /= np.linalg.norm(q)
+    return q if q[3] >= 0 else -q
+
+
+def quat_to_R(q):
+    x, y, z, w = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def make_pose_problems(n_problems=64, n_cyl=500, n_planes=20, pts_per_cyl=60, pts_per_plane=50,
+                       seed=2, t_err=0.5, r_err_deg=5.0, cyl_noise=0.02, plane_noise=0.03,
+                       map_radius=30.0):
+    """Batched pose workload: per problem, a map of n_cyl vertical-ish trunk
+    cylinders and n_planes ground patches around a true sensor pose, features
+    observed in the sensor frame with noise, and an initial guess perturbed by
+    U(0, t_err) m and U(0, r_err_deg) degrees.  Landmark ids are global."""
+    rng = np.random.default_rng(seed)
+    pts, cyl, planes, pose_true, pose_init = [], [], [], [], []
+    res_off, cyl_off, pl_off = [0], [0], [0]
+    for k in range(n_problems):
+        t_true = np.array([rng.uniform(-100, 100), rng.uniform(-100, 100), rng.uniform(1, 3)])
+        ypr = [rng.uniform(-math.pi, math.pi), math.radians(rng.uniform(-3, 3)),
+               math.radians(rng.uniform(-3, 3))]
+        Rz = _rotvec_to_R(np.array([0, 0, ypr[0]]))
+        Ry = _rotvec_to_R(np.array([0, ypr[1], 0]))
+        Rx = _rotvec_to_R(np.array([ypr[2], 0, 0]))
+        R_true = Rz @ Ry @ Rx
+        ground_z = t_true[2] - rng.uniform(1.2, 2.0)
+        base_id = cyl_off[-1]
+        feats = []
+        for j in range(n_cyl):
+            rr = map_radius * math.sqrt(rng.uniform(0.01, 1.0))
+            bearing = rng.uniform(-math.pi, math.pi)
+            root = np.array([t_true[0] + rr * math.cos(bearing), t_true[1] + rr * math.sin(bearing),
+                             ground_z])
+            tilt = math.radians(rng.uniform(0, 5))
+            az = rng.uniform(-math.pi, math.pi)
+            ray = np.array([math.sin(tilt) * math.cos(az), math.sin(tilt) * math.sin(az),
+                            math.cos(tilt)])
+            rad = rng.uniform(0.1, 0.4)
+            cyl.append(np.concatenate([root, ray, [rad, 0.0]]))
+            p, _ = trunk_points(rng, pts_per_cyl, root - t_true + np.array([0, 0, 0]), ray, rad,
+                                6.0, cyl_noise, 0.0)
+            p = p + t_true  # trunk_points takes the base relative to the sensor
+            ids = np.full(pts_per_cyl, base_id + j, np.int32)
+            feats.append((p, ids))
+        pl_base = pl_off[-1]
+        for j in range(n_planes):
+            slope = math.radians(rng.uniform(0, 5))
+            az = rng.uniform(-math.pi, math.pi)
+            n = np.array([math.sin(slope) * math.cos(az), math.sin(slope) * math.sin(az),
+                          math.cos(slope)])
+            rr = map_radius * math.sqrt(rng.uniform(0.02, 1.0))
+            bearing = rng.uniform(-math.pi, math.pi)
+            c = np.array([t_true[0] + rr * math.cos(bearing), t_true[1] + rr * math.sin(bearing),
+                          ground_z])
+            d = -n @ c
+            planes.append(np.concatenate([n, [d]]))
+            e1, e2 = _perp_basis(n)
+            a = rng.uniform(0, 2 * math.pi, pts_per_plane)
+            rad = 2.0 * np.sqrt(rng.uniform(0, 1, pts_per_plane))
+            p = (c[None, :] + (rad * np.cos(a))[:, None] * e1 + (rad * np.sin(a))[:, None] * e2 +
+                 rng.normal(0, plane_noise, pts_per_plane)[:, None] * n)
+            ids = np.full(pts_per_plane, -(pl_base + j) - 1, np.int32)
+            feats.append((p, ids))
+        P = np.concatenate([f[0] for f in feats])
+        ids = np.concatenate([f[1] for f in feats])
+        ps = (P - t_true) @ R_true  # R^T (p - t)
+        rec = np.empty((len(ps), 4), np.float32)
+        rec[:, :3] = ps
+        rec[:, 3] = ids.view(np.float32)
+        pts.append(rec)
+        res_off.append(res_off[-1] + len(ps))
+        cyl_off.append(cyl_off[-1] + n_cyl)
+        pl_off.append(pl_off[-1] + n_planes)
+        # initial guess
+        dt = rng.normal(size=3)
+        dt *= rng.uniform(0, t_err) / np.linalg.norm(dt)
+        ax = rng.normal(size=3)
+        ax *= math.radians(rng.uniform(0, r_err_deg)) / np.linalg.norm(ax)
+        R_init = _rotvec_to_R(ax) @ R_true
+        pose_true.append(np.concatenate([t_true, R_to_quat(R_true)]))
+        pose_init.append(np.concatenate([t_true + dt, R_to_quat(R_init)]))
+    return {
+        "points": np.concatenate(pts),
+        "res_offsets": np.array(res_off, np.int32),
+        "cyl": np.array(cyl, np.float64).reshape(-1, 8),
+        "cyl_offsets": np.array(cyl_off, np.int32),
+        "planes": np.array(planes, np.float64).reshape(-1, 4),
+        "plane_offsets": np.array(pl_off, np.int32),
+        "pose_true": np.array(pose_true),
+        "pose_init": np.array(pose_init),
+    }
+
+
+def pose_error(pa, pb):
+    """Translation (m) and rotation (rad) distance between poses t,q(xyzw)."""
+    dt = np.linalg.norm(pa[..., :3] - pb[..., :3], axis=-1)
+    dot = np.abs(np.sum(pa[..., 3:] * pb[..., 3:], axis=-1)).clip(0, 1)
+    return dt, 2 * np.arccos(dot)

@@ -1,0 +1,84 @@
+"""GPU batched pose estimation (C ABI sloam_pose_plan_*) against the FP64 CPU
+oracle.  The GPU evaluates FP32 residuals / Jacobians with error-free FP32
+summation and an FP64 solve; the stated bar is 1e-4 m / rad against the FP64
+oracle.  The tests assert that bar on the maximum, 1e-6 on the median, and
+print both.  (Measured on the 64-problem workload: median ~3e-7 m; a few
+problems reach ~2e-5 m when an LM accept/reject decision within the fixed 10
+iterations differs between the FP32-compensated and the FP64 cost.)"""
+import numpy as np
+import pytest
+
+import oracle
+from sloam_amd import synth
+from sloam_amd.gpu import solve_poses
+
+pytestmark = pytest.mark.gpu
+
+TOL_T = 1e-4  # m
+TOL_R = 1e-4  # rad
+MED_T = 1e-6
+MED_R = 1e-6
+
+
+def _check(prob, g, o, label):
+    dt, dr = synth.pose_error(g["pose"], o["pose"])
+    print(f"[pose] {label}: |dt| max {dt.max():.3e} median {np.median(dt):.3e} m, "
+          f"|dr| max {dr.max():.3e} median {np.median(dr):.3e} rad vs FP64 oracle", flush=True)
+    assert dt.max() < TOL_T and dr.max() < TOL_R
+    assert np.median(dt) < MED_T and np.median(dr) < MED_R
+    assert np.allclose(g["cost"][:, 0], o["cost"][:, 0], rtol=1e-5)
+    assert np.allclose(g["cost"][:, 1], o["cost"][:, 1], rtol=1e-4)
+
+
+def test_small_batch_fixed_iterations(ctx):
+    prob = synth.make_pose_problems(8, n_cyl=100, n_planes=20, seed=21)
+    print(f"[pose] 8 problems, {prob['res_offsets'][-1]} residuals", flush=True)
+    g = solve_poses(ctx, prob, max_iters=10)
+    o = oracle.pose_lm_batch(prob, threads=8, max_iters=10)
+    _check(prob, g, o, "small fixed")
+    assert np.array_equal(g["status"][:, 1], o["status"][:, 1])
+
+
+def test_stop_on_convergence_reference_settings(ctx):
+    # the reference's solver settings: up to 50 iterations, stop at convergence
+    prob = synth.make_pose_problems(6, n_cyl=60, n_planes=15, seed=22)
+    g = solve_poses(ctx, prob, max_iters=50, stop_on_convergence=True)
+    o = oracle.pose_lm_batch(prob, threads=6, max_iters=50, stop_on_convergence=True)
+    _check(prob, g, o, "converge")
+    assert np.array_equal(g["status"][:, 0], o["status"][:, 0])
+
+
+def test_full_batch_64_problems(ctx):
+    """Batched pose workload: 64 problems x (500 cylinders + 20 planes), ~2M points."""
+    prob = synth.make_pose_problems(64, seed=2)
+    print(f"[pose] 64 problems, {prob['res_offsets'][-1]} residuals", flush=True)
+    g = solve_poses(ctx, prob, max_iters=10)
+    o = oracle.pose_lm_batch(prob, threads=16, max_iters=10)
+    _check(prob, g, o, "full")
+    dt, dr = synth.pose_error(g["pose"], prob["pose_true"])
+    assert dt.max() < 0.01 and np.degrees(dr.max()) < 0.1
+
+
+def test_empty_and_tiny_problems(ctx):
+    prob = synth.make_pose_problems(3, n_cyl=5, n_planes=2, pts_per_cyl=3, pts_per_plane=3,
+                                    seed=23)
+    # make problem 1 empty: drop its residuals
+    off = prob["res_offsets"]
+    keep = np.r_[0:off[1], off[2]:off[3]]
+    prob["points"] = prob["points"][keep]
+    n1 = off[2] - off[1]
+    prob["res_offsets"] = np.array([0, off[1], off[1], off[3] - n1], np.int32)
+    g = solve_poses(ctx, prob, max_iters=10)
+    o = oracle.pose_lm_batch(prob, max_iters=10)
+    assert np.allclose(g["pose"][1], prob["pose_init"][1])
+    assert np.allclose(o["pose"][1], prob["pose_init"][1])
+    dt, dr = synth.pose_error(g["pose"], o["pose"])
+    assert dt.max() < 1e-4 and dr.max() < 1e-4
+
+
+def test_deterministic_repeat(ctx):
+    prob = synth.make_pose_problems(4, n_cyl=80, n_planes=10, seed=24)
+    a = solve_poses(ctx, prob, max_iters=10)
+    b = solve_poses(ctx, prob, max_iters=10)
+    for k in a:
+        assert np.array_equal(a[k], b[k])

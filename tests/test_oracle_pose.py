@@ -1,0 +1,140 @@
+"""CPU tests that pin the pose-estimation oracle: analytic Jacobians against
+central differences, closed-form zero-noise solutions, rigid-motion
+equivariance, and the optimum against scipy.optimize.least_squares with the
+same Huber loss (an independent solver of the same cost:
+1/2 sum rho(r^2), rho(s) = s for s <= 0.01 else 0.2 sqrt(s) - 0.01,
+sloam/src/core/sloam.cpp:69 HuberLoss(0.1))."""
+import math
+
+import numpy as np
+import pytest
+from scipy.optimize import least_squares
+
+import oracle
+from sloam_amd import synth
+
+
+def _small(seed=4, **kw):
+    args = dict(n_problems=3, n_cyl=40, n_planes=10, pts_per_cyl=20, pts_per_plane=20,
+                seed=seed)
+    args.update(kw)
+    return synth.make_pose_problems(**args)
+
+
+def _apply(pose, d):
+    """Left perturbation: t + d[:3], exp(d[3:]) R."""
+    R = synth.quat_to_R(pose[3:])
+    Rn = synth._rotvec_to_R(d[3:]) @ R
+    return np.concatenate([pose[:3] + d[:3], synth.R_to_quat(Rn)])
+
+
+def test_jacobian_central_differences():
+    prob = _small()
+    pts = prob["points"][prob["res_offsets"][0]:prob["res_offsets"][1]]
+    pose = prob["pose_init"][0]
+    r0, J = oracle.pose_residuals(pts, prob["cyl"], prob["planes"], pose)
+    eps = 1e-6
+    Jn = np.zeros_like(J)
+    for a in range(6):
+        d = np.zeros(6)
+        d[a] = eps
+        rp, _ = oracle.pose_residuals(pts, prob["cyl"], prob["planes"], _apply(pose, d))
+        rm, _ = oracle.pose_residuals(pts, prob["cyl"], prob["planes"], _apply(pose, -d))
+        Jn[:, a] = (rp - rm) / (2 * eps)
+    # plane residuals are |s|: skip features within eps of the kink
+    ok = np.abs(r0) > 1e-4
+    assert np.allclose(J[ok], Jn[ok], atol=2e-5, rtol=1e-5)
+
+
+def test_residuals_reference_formula():
+    """Cylinder residual ||pp - lp|| - radius with pp the projection of lp on
+    the (unnormalised) ray, plane residual |pi.[lp;1]| / ||pi_xyz||, exactly as
+    CylinderCost / PlaneCost write them (cylinder.h:118-122, plane.h:102-107)."""
+    prob = _small()
+    pts = prob["points"][:2000].astype(np.float64)
+    pose = prob["pose_init"][0]
+    R = synth.quat_to_R(pose[3:])
+    cyl = prob["cyl"].copy()
+    cyl[:, 3:6] *= 3.7  # an unnormalised ray must give the same residual
+    planes = prob["planes"] * 2.5
+    r, _ = oracle.pose_residuals(prob["points"][:2000], cyl, planes, pose)
+    ids = prob["points"][:2000, 3].view(np.int32)
+    for i in range(0, 2000, 37):
+        lp = R @ pts[i, :3] + pose[:3]
+        if ids[i] >= 0:
+            root, ray, rad = cyl[ids[i], :3], cyl[ids[i], 3:6], cyl[ids[i], 6]
+            pp = root + ((lp - root) @ ray / (ray @ ray)) * ray
+            ref = np.linalg.norm(pp - lp) - rad
+        else:
+            pi = planes[-ids[i] - 1]
+            ref = abs(pi[:3] @ lp + pi[3]) / np.linalg.norm(pi[:3])
+        assert abs(r[i] - ref) < 1e-9
+
+
+def test_zero_noise_recovers_truth():
+    prob = _small(cyl_noise=0.0, plane_noise=0.0)
+    res = oracle.pose_lm_batch(prob, max_iters=50, stop_on_convergence=True)
+    dt, dr = synth.pose_error(res["pose"], prob["pose_true"])
+    assert dt.max() < 1e-6 and dr.max() < 1e-7
+    assert np.all(res["status"][:, 0] == 0)
+    # starting at the truth: zero cost, converged at iteration 0
+    prob["pose_init"] = prob["pose_true"].copy()
+    res = oracle.pose_lm_batch(prob, max_iters=50, stop_on_convergence=True)
+    assert np.all(res["cost"][:, 0] < 1e-9)
+
+
+def test_rigid_motion_equivariance():
+    prob = _small(seed=8)
+    res = oracle.pose_lm_batch(prob, max_iters=30, stop_on_convergence=True)
+    G = synth._rotvec_to_R(np.array([0.1, -0.2, 0.7]))
+    gt = np.array([5.0, -3.0, 1.0])
+    moved = dict(prob)
+    cyl = prob["cyl"].copy()
+    cyl[:, :3] = cyl[:, :3] @ G.T + gt
+    cyl[:, 3:6] = cyl[:, 3:6] @ G.T
+    moved["cyl"] = cyl
+    pl = prob["planes"].copy()
+    n2 = pl[:, :3] @ G.T
+    pl[:, 3] = pl[:, 3] - n2 @ gt
+    pl[:, :3] = n2
+    moved["planes"] = pl
+    init = []
+    for p in prob["pose_init"]:
+        R = G @ synth.quat_to_R(p[3:])
+        init.append(np.concatenate([G @ p[:3] + gt, synth.R_to_quat(R)]))
+    moved["pose_init"] = np.array(init)
+    res2 = oracle.pose_lm_batch(moved, max_iters=30, stop_on_convergence=True)
+    for a, b in zip(res["pose"], res2["pose"]):
+        R = G @ synth.quat_to_R(a[3:])
+        expect = np.concatenate([G @ a[:3] + gt, synth.R_to_quat(R)])
+        dt, dr = synth.pose_error(expect, b)
+        assert dt < 1e-6 and dr < 1e-7
+
+
+@pytest.mark.parametrize("seed", [3, 17])
+def test_optimum_matches_scipy_huber(seed):
+    prob = _small(seed=seed, n_problems=2)
+    res = oracle.pose_lm_batch(prob, max_iters=50, stop_on_convergence=True)
+    for k in range(2):
+        pts = prob["points"][prob["res_offsets"][k]:prob["res_offsets"][k + 1]]
+        p0 = prob["pose_init"][k]
+
+        def fun(d):
+            r, _ = oracle.pose_residuals(pts, prob["cyl"], prob["planes"], _apply(p0, d))
+            return r
+
+        sol = least_squares(fun, np.zeros(6), loss="huber", f_scale=0.1, method="trf",
+                            xtol=1e-14, ftol=1e-14, gtol=1e-14, max_nfev=500)
+        ref = _apply(p0, sol.x)
+        dt, dr = synth.pose_error(ref, res["pose"][k])
+        assert dt < 1e-6 and dr < 1e-7, (dt, dr)
+        # cost convention: 1/2 sum rho
+        assert abs(sol.cost - res["cost"][k, 1]) < 1e-8 * max(1.0, sol.cost)
+
+
+def test_threads_do_not_change_results():
+    prob = _small(seed=5, n_problems=6)
+    a = oracle.pose_lm_batch(prob, threads=1)
+    b = oracle.pose_lm_batch(prob, threads=6)
+    for key in a:
+        assert np.array_equal(a[key], b[key])
