@@ -161,6 +161,35 @@ int sloam_pose_plan_exec(sloam_pose_plan* plan, const float* d_points, const dou
                          const int32_t* d_plane_offsets, const double* d_pose_in,
                          double* d_pose_out, int32_t* d_status, double* d_cost);
 
+/* ---- range projection + ground / tree partition ---------------------------
+ * Replaces: seg::Segmentation::_doProjection / _makeTensor / _mask / maskCloud /
+ * _destaggerCloud (sloam/src/segmentation/inference.cpp:80-300,
+ * sloam/include/segmentation/inference.h:57-79) with the per-point labels given
+ * (0 other, 1 ground, 2 tree) instead of the ONNX network's argmax.
+ * d_xyzi: n points of 4 floats (x, y, z, intensity) in input order; d_label: n
+ * bytes.  Outputs (H = height, W = width):
+ *   d_proj   (nullable) n x 2 int32: column, row (-1, -1 for non-finite points)
+ *   d_range  (nullable) H*W float range image (nearest point; 0 = empty)
+ *   d_winner (nullable) H*W int32 index of the pixel's nearest point (-1 empty)
+ *   d_mask   H*W uint8: 255 tree, 1 ground, 0 other / invalid (range < 1 m)
+ *   d_ground capacity n x 4 floats: points whose pixel is ground, input order;
+ *            the count goes to d_n_ground (one int32, device)
+ *   d_tree   (nullable) n x 4 floats: dense cloud, the point where its pixel
+ *            is tree, NaN elsewhere; destagger shifts even rows by +32 columns
+ *            (requires n == H*W).                                            */
+typedef struct sloam_proj_cfg {
+  int32_t height;        /* range image rows (64)                           */
+  int32_t width;         /* range image columns (2048)                      */
+  float fov_up_deg;      /* 22.5                                            */
+  float fov_down_deg;    /* -22.5                                           */
+  int32_t destagger;     /* 0 / 1                                           */
+} sloam_proj_cfg;
+
+int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* d_label,
+                            int64_t n, const sloam_proj_cfg* cfg, int32_t* d_proj, float* d_range,
+                            int32_t* d_winner, uint8_t* d_mask, float* d_ground,
+                            int32_t* d_n_ground, float* d_tree);
+
 #ifdef __cplusplus
 }
 #endif

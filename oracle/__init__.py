@@ -43,6 +43,8 @@ def _declare(lib):
     lib.oracle_cyl_distances_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float,
                                               C.c_float, C.c_uint64, C.c_int, P]
     lib.oracle_pose_residuals.argtypes = [P, C.c_int, P, P, P, P, P]
+    lib.oracle_project_partition.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_float,
+                                             C.c_float, C.c_int, P, P, P, P, P, P, P]
     lib.oracle_pose_lm_batch.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P, C.c_int]
 
 
@@ -164,3 +166,24 @@ def pose_lm_batch(prob, threads=1, **kw):
                                 _p(np.ascontiguousarray(prob["pose_init"], np.float64)),
                                 _p(cfg_i), _p(cfg_d), _p(pose), _p(status), _p(cost), threads)
     return {"pose": pose, "status": status, "cost": cost}
+
+
+def project_partition(xyzi, label, H=64, W=2048, fov_up=22.5, fov_down=-22.5, destagger=False):
+    x = np.ascontiguousarray(xyzi, np.float32)
+    lab = np.ascontiguousarray(label, np.uint8)
+    n = len(x)
+    out = {
+        "proj": np.zeros((n, 2), np.int32),
+        "range": np.zeros((H, W), np.float32),
+        "winner": np.zeros((H, W), np.int32),
+        "mask": np.zeros((H, W), np.uint8),
+        "ground": np.zeros((n, 4), np.float32),
+        "tree": np.zeros((n, 4), np.float32),
+    }
+    ng = np.zeros(1, np.int32)
+    load().oracle_project_partition(_p(x), _p(lab), n, H, W, fov_up, fov_down, int(destagger),
+                                    _p(out["proj"]), _p(out["range"]), _p(out["winner"]),
+                                    _p(out["mask"]), _p(out["ground"]), _p(ng), _p(out["tree"]))
+    out["ground"] = out["ground"][:int(ng[0])]
+    out["n_ground"] = ng
+    return out

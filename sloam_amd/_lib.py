@@ -29,6 +29,16 @@ class CylFitCfg(C.Structure):
 CYL_MODEL_STRIDE = 8
 
 
+class ProjCfg(C.Structure):
+    _fields_ = [
+        ("height", C.c_int32),
+        ("width", C.c_int32),
+        ("fov_up_deg", C.c_float),
+        ("fov_down_deg", C.c_float),
+        ("destagger", C.c_int32),
+    ]
+
+
 class PoseCfg(C.Structure):
     _fields_ = [
         ("max_iters", C.c_int32),
@@ -63,6 +73,8 @@ PROTOTYPES = {
                                          C.POINTER(_P)]),
     "sloam_pose_plan_destroy": (None, [_P]),
     "sloam_pose_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sloam_project_partition": (C.c_int, [_P, _P, _P, _I64, C.POINTER(ProjCfg), _P, _P, _P, _P,
+                                          _P, _P, _P]),
 }
 
 _lib = None

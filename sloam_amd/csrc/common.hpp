@@ -32,7 +32,14 @@ struct sloam_ctx {
   bool timing = false;
   std::map<std::string, SloamTimer> timers;
   std::vector<hipEvent_t> event_pool;
+  // grow-on-demand scratch buffers, one per slot
+  std::vector<std::pair<void*, size_t>> scratch;
 };
+
+// Scratch buffer `slot` of at least `bytes` bytes (reallocated when too small;
+// the previous contents are not kept).  Callers own a slot for the duration of
+// one entry point.
+int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out);
 
 // Records a HIP failure in the context and returns SLOAM_ERR_HIP.
 int sloam_record_hip_error(sloam_ctx* ctx, hipError_t e, const char* what);

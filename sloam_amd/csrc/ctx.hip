@@ -54,6 +54,26 @@ void sloam_dev_free(void* p) {
   if (p) (void)hipFree(p);
 }
 
+int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out) {
+  *out = nullptr;
+  if (slot < 0) return SLOAM_ERR_INVALID_ARG;
+  if ((int)ctx->scratch.size() <= slot) ctx->scratch.resize(slot + 1, {nullptr, 0});
+  auto& s = ctx->scratch[slot];
+  if (s.second < bytes) {
+    // the stream may still use the old buffer
+    SLOAM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    sloam_dev_free(s.first);
+    s.first = nullptr;
+    s.second = 0;
+    const size_t sz = bytes < 4096 ? 4096 : bytes + bytes / 4;
+    int st = sloam_dev_alloc(ctx, &s.first, sz);
+    if (st) return st;
+    s.second = sz;
+  }
+  *out = s.first;
+  return SLOAM_OK;
+}
+
 __global__ void sl_fill_u32_kernel(uint32_t* __restrict__ d, size_t n, uint32_t value) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -101,6 +121,7 @@ void sloam_ctx_destroy(sloam_ctx* ctx) {
       (void)hipEventDestroy(pr.second);
     }
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (auto& s : ctx->scratch) sloam_dev_free(s.first);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -9,7 +9,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import CYL_MODEL_STRIDE, CylFitCfg, PoseCfg, check
+from ._lib import CYL_MODEL_STRIDE, CylFitCfg, PoseCfg, ProjCfg, check
 
 
 def _ptr(t):
@@ -211,4 +211,42 @@ def solve_poses(ctx, prob, **kw):
     ctx.sync()
     res = {k: v.cpu().numpy() for k, v in out.items()}
     plan.close()
+    return res
+
+
+def project_partition_dev(ctx, xyzi, label, H=64, W=2048, fov_up=22.5, fov_down=-22.5,
+                          destagger=False, out=None):
+    """Enqueues the projection + partition of device tensors xyzi [n, 4] float32
+    and label [n] uint8; returns the dict of device outputs."""
+    n = xyzi.shape[0]
+    dev = xyzi.device
+    if out is None:
+        out = {
+            "proj": torch.empty((n, 2), dtype=torch.int32, device=dev),
+            "range": torch.empty((H, W), dtype=torch.float32, device=dev),
+            "winner": torch.empty((H, W), dtype=torch.int32, device=dev),
+            "mask": torch.empty((H, W), dtype=torch.uint8, device=dev),
+            "ground": torch.empty((n, 4), dtype=torch.float32, device=dev),
+            "n_ground": torch.empty((1,), dtype=torch.int32, device=dev),
+            "tree": torch.empty((n, 4), dtype=torch.float32, device=dev),
+        }
+    cfg = ProjCfg(height=H, width=W, fov_up_deg=fov_up, fov_down_deg=fov_down,
+                  destagger=int(destagger))
+    check(ctx.lib.sloam_project_partition(
+        ctx.handle, _ptr(xyzi), _ptr(label), n, C.byref(cfg), _ptr(out.get("proj")),
+        _ptr(out.get("range")), _ptr(out.get("winner")), _ptr(out["mask"]), _ptr(out["ground"]),
+        _ptr(out["n_ground"]), _ptr(out.get("tree"))), ctx.handle)
+    return out
+
+
+def project_partition(ctx, xyzi, label, **kw):
+    """One-shot convenience on host arrays; returns numpy outputs."""
+    dev = torch.device("cuda", ctx.device)
+    x = torch.as_tensor(np.ascontiguousarray(xyzi, np.float32)).to(dev)
+    lab = torch.as_tensor(np.ascontiguousarray(label, np.uint8)).to(dev)
+    torch.cuda.synchronize()
+    out = project_partition_dev(ctx, x, lab, **kw)
+    ctx.sync()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    res["ground"] = res["ground"][:int(res["n_ground"][0])]
     return res

@@ -222,3 +222,91 @@ def pose_error(pa, pb):
     dt = np.linalg.norm(pa[..., :3] - pb[..., :3], axis=-1)
     dot = np.abs(np.sum(pa[..., 3:] * pb[..., 3:], axis=-1)).clip(0, 1)
     return dt, 2 * np.arccos(dot)
+
+
+def make_scan(seed=0, H=64, W=2048, fov_deg=22.5, n_trees=40, r_min=0.1, r_max=0.4,
+              h_min=2.0, h_max=8.0, tilt_max_deg=5.0, sensor_height=1.5, slope_max_deg=5.0,
+              clutter_frac=0.3, tree_noise=0.02, ground_noise=0.03, range_min=1.0,
+              range_max=40.0):
+    """One organised H x W lidar scan of a synthetic forest (ray cast).
+
+    Beam (row, col) points at the centre of range-image pixel (row, col) of
+    the spherical projection (row 0 = +fov, col from yaw = -atan2(y, x)).
+    Hits: the nearest of n_trees tilted cylinders and a sloped ground plane;
+    a clutter_frac share of returns is replaced by understory clutter at a
+    random range along the beam.  No return -> NaN.
+    Returns xyzi float32 [H*W, 4], label uint8 [H*W] (0 other, 1 ground,
+    2 tree), instance int32 [H*W] (tree index or -1) and the trees."""
+    rng = np.random.default_rng(seed)
+    fov = math.radians(fov_deg)
+    rows = np.arange(H)
+    cols = np.arange(W)
+    pitch = fov - (rows + 0.5) / H * (2 * fov)
+    yaw = (cols + 0.5) / W * 2 * math.pi - math.pi
+    az = -yaw  # atan2(y, x)
+    P, A = np.meshgrid(pitch, az, indexing="ij")
+    d = np.stack([np.cos(P) * np.cos(A), np.cos(P) * np.sin(A), np.sin(P)], -1).reshape(-1, 3)
+    n = H * W
+    best = np.full(n, np.inf)
+    label = np.zeros(n, np.uint8)
+    inst = np.full(n, -1, np.int32)
+    # ground plane n.x + d0 = 0 below the sensor
+    slope = math.radians(rng.uniform(0, slope_max_deg))
+    gaz = rng.uniform(-math.pi, math.pi)
+    gn = np.array([math.sin(slope) * math.cos(gaz), math.sin(slope) * math.sin(gaz),
+                   math.cos(slope)])
+    gd = sensor_height * gn[2]
+    den = d @ gn
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tg = -gd / den
+    ok = (den < -1e-6) & (tg > 0) & (tg < range_max)
+    best[ok] = tg[ok]
+    label[ok] = 1
+    trees = []
+    for k in range(n_trees):
+        rr = rng.uniform(3.0, 0.8 * range_max)
+        b = rng.uniform(-math.pi, math.pi)
+        base_xy = np.array([rr * math.cos(b), rr * math.sin(b)])
+        bz = -(gn[0] * base_xy[0] + gn[1] * base_xy[1] + gd) / gn[2]
+        base = np.array([base_xy[0], base_xy[1], bz])
+        tilt = math.radians(rng.uniform(0, tilt_max_deg))
+        ta = rng.uniform(-math.pi, math.pi)
+        ax = np.array([math.sin(tilt) * math.cos(ta), math.sin(tilt) * math.sin(ta),
+                       math.cos(tilt)])
+        rad = rng.uniform(r_min, r_max)
+        hgt = rng.uniform(h_min, h_max)
+        trees.append({"base": base, "axis": ax, "radius": rad, "height": hgt})
+        # ray / infinite cylinder: |(t d - base) x ax| = rad
+        w = d - (d @ ax)[:, None] * ax[None, :]
+        c0 = -base - (-base @ ax) * ax
+        qa = (w * w).sum(1)
+        qb = 2 * (w @ c0)
+        qc = c0 @ c0 - rad * rad
+        disc = qb * qb - 4 * qa * qc
+        hit = disc > 0
+        t = np.full(n, np.inf)
+        t[hit] = (-qb[hit] - np.sqrt(disc[hit])) / (2 * qa[hit])
+        with np.errstate(invalid="ignore"):
+            hz = (np.where(hit, t, 0.0)[:, None] * d - base) @ ax
+        hit &= (t > 0) & (hz >= 0) & (hz <= hgt)
+        closer = hit & (t < best)
+        best[closer] = t[closer]
+        label[closer] = 2
+        inst[closer] = k
+    noise = np.where(label == 2, tree_noise, ground_noise)
+    hitm = np.isfinite(best) & (best >= range_min)
+    rng_vals = best + rng.normal(0, 1, n) * noise
+    # understory clutter replaces a share of the returns
+    cl = hitm & (rng.uniform(0, 1, n) < clutter_frac)
+    rng_vals[cl] = rng.uniform(range_min, np.minimum(best[cl], range_max))
+    label[cl] = 0
+    inst[cl] = -1
+    xyz = rng_vals[:, None] * d
+    xyz[~hitm] = np.nan
+    label[~hitm] = 0
+    inst[~hitm] = -1
+    xyzi = np.empty((n, 4), np.float32)
+    xyzi[:, :3] = xyz
+    xyzi[:, 3] = rng.uniform(0, 255, n)
+    return {"xyzi": xyzi, "label": label, "instance": inst, "trees": trees,
+            "ground": np.concatenate([gn, [gd]]), "H": H, "W": W, "fov_deg": fov_deg}
