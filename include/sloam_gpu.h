@@ -190,6 +190,85 @@ int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* 
                             int32_t* d_winner, uint8_t* d_mask, float* d_ground,
                             int32_t* d_n_ground, float* d_tree);
 
+/* ---- ground planes: binning, bottom-k selection, per-cell plane fit -------
+ * Replaces: sloam::sloam::binGroundPoints (sloam/src/core/sloam.cpp:330-386)
+ * and Plane::Plane / Plane::computeModel (sloam/src/objects/plane.cpp:3-17,
+ * 96-128) as called by computeModels (sloam.cpp:391-400).
+ * d_points: capacity points of 4 floats (x, y, z, i); the first *d_count are
+ * used (d_count NULL: all capacity).  Cells are radii_bins x theta_bins,
+ * r-major (cell = r * theta_bins + t), as computeModels iterates them.
+ * Per cell: d_cell_info 4 int32 {valid, kept points, binned points, 0};
+ * d_cell_model 8 doubles {a, b, c, d (unit normal, z >= 0), centroid x, y, z,
+ * 0}; d_features num_features x 4 floats: the cell's first num_features kept
+ * points (bottom points by increasing z when the cell was reduced, input order
+ * otherwise).  A cell is valid iff it keeps >= num_features (and > 0) points. */
+typedef struct sloam_ground_cfg {
+  int32_t radii_bins;       /* groundRadiiBins                               */
+  int32_t theta_bins;       /* groundThetaBins                               */
+  double min_dist;          /* minGroundLidarDist                            */
+  double max_dist;          /* maxGroundLidarDist                            */
+  double retain_thresh;     /* groundRetainThresh (fraction kept per cell)   */
+  int32_t num_features;     /* numGroundFeatures                             */
+  float origin[3];          /* binning origin (the reference bins at SE3())  */
+} sloam_ground_cfg;
+
+int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
+                        const int32_t* d_count, const sloam_ground_cfg* cfg, int32_t* d_cell_info,
+                        double* d_cell_model, float* d_features);
+
+/* ---- batched ground-plane RANSAC (point-level, fixed hypotheses) ----------
+ * No counterpart in the reference (its planes are per-cell SVD fits); the
+ * inlier predicate is Plane::distance(const PointT&) < thr
+ * (sloam/src/objects/plane.cpp:136-151) for a unit normal.  Set s holds points
+ * d_xyz[offsets[s] .. offsets[s+1]) (3 floats each).  Hypothesis h of set s:
+ * Philox4x32-10(counter {h, s, stream_tag, 0}, key seed) draws 3 distinct
+ * points, the plane through them in FP32 (relative to the set's bounding-box
+ * centre); counts of |n.p + d| < thr; best = max count, lowest index; FP64
+ * PCA refit over its inliers.  Outputs as sloam_cyl_plan_exec, with the model
+ * record {a, b, c, d (unit normal, z >= 0), centroid[3], refitted inliers}. */
+typedef struct sloam_plane_fit_cfg {
+  int32_t n_hyp;
+  float thr;
+  uint64_t seed;
+  int32_t stream_tag;
+  int32_t reserved;
+} sloam_plane_fit_cfg;
+
+typedef struct sloam_plane_plan sloam_plane_plan;
+
+int sloam_plane_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_sets,
+                            const sloam_plane_fit_cfg* cfg, sloam_plane_plan** out);
+void sloam_plane_plan_destroy(sloam_plane_plan* plan);
+int sloam_plane_plan_exec(sloam_plane_plan* plan, const float* d_xyz, const int32_t* d_offsets,
+                          int32_t* d_counts, int32_t* d_samples, uint32_t* d_mask,
+                          int32_t* d_best, double* d_model);
+
+/* ---- batched RANSAC line fit (cylinder axis from trellis vertices) --------
+ * Replaces: the PCL SACSegmentation call of Cylinder::computeModel
+ * (sloam/src/objects/cylinder.cpp:116-139: SACMODEL_LINE, SAC_RANSAC,
+ * threshold 0.25, optimizeCoefficients) for a batch of trees.  Set s holds
+ * points d_xyz[offsets[s] .. offsets[s+1]) (3 floats each: the vertex medians).
+ * Sample-consensus semantics of the library: MT19937 seeded `seed` (12345),
+ * swap-shuffled index array persisting across draws, samples redrawn until
+ * all three coordinates of the two points differ (up to 1000 draws), adaptive
+ * bound k = log(1 - probability) / log(1 - w^2), at most max_iterations
+ * (50); optional refinement through the inliers' centroid along their
+ * principal axis.  Outputs per set: d_coeff 6 floats (point, unit direction),
+ * d_stats 4 int32 {ok, inliers, iterations, skipped}; d_inliers (nullable)
+ * one byte per point.  Sets hold at most 4096 points.                      */
+typedef struct sloam_line_fit_cfg {
+  double threshold;        /* 0.25                                           */
+  double probability;      /* 0.99                                           */
+  int32_t max_iterations;  /* 50                                             */
+  int32_t optimize;        /* 1                                              */
+  uint32_t seed;           /* 12345                                          */
+  int32_t reserved;
+} sloam_line_fit_cfg;
+
+int sloam_fit_lines(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets, int32_t n_sets,
+                    const sloam_line_fit_cfg* cfg, float* d_coeff, int32_t* d_stats,
+                    uint8_t* d_inliers);
+
 #ifdef __cplusplus
 }
 #endif

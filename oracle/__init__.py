@@ -43,6 +43,13 @@ def _declare(lib):
     lib.oracle_cyl_distances_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float,
                                               C.c_float, C.c_uint64, C.c_int, P]
     lib.oracle_pose_residuals.argtypes = [P, C.c_int, P, P, P, P, P]
+    lib.oracle_mt19937.argtypes = [C.c_uint32, C.c_int, P]
+    lib.oracle_line_ransac.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint32,
+                                       C.c_int, P, P, P]
+    lib.oracle_ground_planes.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, P, P, P, P]
+    lib.oracle_plane_fit_batch.argtypes = [P, P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_int,
+                                           P, P, P, P, P, C.c_int]
     lib.oracle_project_partition.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_float,
                                              C.c_float, C.c_int, P, P, P, P, P, P, P]
     lib.oracle_pose_lm_batch.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P, C.c_int]
@@ -187,3 +194,62 @@ def project_partition(xyzi, label, H=64, W=2048, fov_up=22.5, fov_down=-22.5, de
     out["ground"] = out["ground"][:int(ng[0])]
     out["n_ground"] = ng
     return out
+
+
+def ground_planes(points, radii_bins=1, theta_bins=18, min_dist=0.0, max_dist=30.0,
+                  retain_thresh=0.05, num_features=60, origin=(0.0, 0.0, 0.0)):
+    x = np.ascontiguousarray(points, np.float32)
+    nc = radii_bins * theta_bins
+    info = np.zeros((nc, 4), np.int32)
+    model = np.zeros((nc, 8))
+    feats = np.zeros((nc, max(num_features, 1), 4), np.float32)
+    org = np.array(origin, np.float32)
+    load().oracle_ground_planes(_p(x), len(x), radii_bins, theta_bins, min_dist, max_dist,
+                                retain_thresh, num_features, _p(org), _p(info), _p(model),
+                                _p(feats))
+    return {"info": info, "model": model, "features": feats}
+
+
+def plane_fit_batch(xyz, offsets, n_hyp, thr=0.05, seed=0x5EED, tag=1, counts=True, samples=False,
+                    mask=True, threads=1):
+    x = np.ascontiguousarray(xyz, np.float32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    S = len(off) - 1
+    N = int(off[-1])
+    out = {"model": np.zeros((S, 8)), "best": np.zeros(S, np.int32)}
+    if counts:
+        out["counts"] = np.zeros((S, n_hyp), np.int32)
+    if samples:
+        out["samples"] = np.zeros((S, n_hyp, 3), np.int32)
+    if mask:
+        out["mask"] = np.zeros((N + 31) // 32, np.uint32)
+    load().oracle_plane_fit_batch(_p(x), _p(off), S, n_hyp, thr, seed, tag, _p(out.get("counts")),
+                                  _p(out.get("samples")), _p(out.get("mask")), _p(out["best"]),
+                                  _p(out["model"]), threads)
+    return out
+
+
+def mt19937(seed, n):
+    out = np.zeros(n, np.uint32)
+    load().oracle_mt19937(seed, n, _p(out))
+    return out
+
+
+def line_ransac(points, threshold=0.25, max_iterations=50, probability=0.99, seed=12345,
+                optimize=True):
+    P = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = len(P)
+    coeff = np.zeros(6, np.float32)
+    inl = np.zeros(max(n, 1), np.uint8)
+    stats = np.zeros(4, np.int32)
+    load().oracle_line_ransac(_p(P), n, threshold, max_iterations, probability, seed,
+                              int(optimize), _p(coeff), _p(inl), _p(stats))
+    return {"coeff": coeff, "inliers": inl[:n], "stats": stats}
+
+
+def line_ransac_batch(xyz, offsets, **kw):
+    off = np.asarray(offsets)
+    res = [line_ransac(xyz[off[s]:off[s + 1]], **kw) for s in range(len(off) - 1)]
+    return {"coeff": np.array([r["coeff"] for r in res]).reshape(-1, 6),
+            "stats": np.array([r["stats"] for r in res]).reshape(-1, 4),
+            "inliers": np.concatenate([r["inliers"] for r in res]) if res else np.zeros(0, np.uint8)}

@@ -29,6 +29,39 @@ class CylFitCfg(C.Structure):
 CYL_MODEL_STRIDE = 8
 
 
+class GroundCfg(C.Structure):
+    _fields_ = [
+        ("radii_bins", C.c_int32),
+        ("theta_bins", C.c_int32),
+        ("min_dist", C.c_double),
+        ("max_dist", C.c_double),
+        ("retain_thresh", C.c_double),
+        ("num_features", C.c_int32),
+        ("origin", C.c_float * 3),
+    ]
+
+
+class PlaneFitCfg(C.Structure):
+    _fields_ = [
+        ("n_hyp", C.c_int32),
+        ("thr", C.c_float),
+        ("seed", C.c_uint64),
+        ("stream_tag", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class LineFitCfg(C.Structure):
+    _fields_ = [
+        ("threshold", C.c_double),
+        ("probability", C.c_double),
+        ("max_iterations", C.c_int32),
+        ("optimize", C.c_int32),
+        ("seed", C.c_uint32),
+        ("reserved", C.c_int32),
+    ]
+
+
 class ProjCfg(C.Structure):
     _fields_ = [
         ("height", C.c_int32),
@@ -75,6 +108,11 @@ PROTOTYPES = {
     "sloam_pose_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sloam_project_partition": (C.c_int, [_P, _P, _P, _I64, C.POINTER(ProjCfg), _P, _P, _P, _P,
                                           _P, _P, _P]),
+    "sloam_ground_planes": (C.c_int, [_P, _P, _I32, _P, C.POINTER(GroundCfg), _P, _P, _P]),
+    "sloam_plane_plan_create": (C.c_int, [_P, _P, _I32, C.POINTER(PlaneFitCfg), C.POINTER(_P)]),
+    "sloam_plane_plan_destroy": (None, [_P]),
+    "sloam_fit_lines": (C.c_int, [_P, _P, _P, _I32, C.POINTER(LineFitCfg), _P, _P, _P]),
+    "sloam_plane_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -9,7 +9,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import CYL_MODEL_STRIDE, CylFitCfg, PoseCfg, ProjCfg, check
+from ._lib import (CYL_MODEL_STRIDE, CylFitCfg, GroundCfg, LineFitCfg, PlaneFitCfg, PoseCfg,
+                   ProjCfg, check)
 
 
 def _ptr(t):
@@ -250,3 +251,124 @@ def project_partition(ctx, xyzi, label, **kw):
     res = {k: v.cpu().numpy() for k, v in out.items()}
     res["ground"] = res["ground"][:int(res["n_ground"][0])]
     return res
+
+
+def ground_planes_dev(ctx, points, count=None, radii_bins=1, theta_bins=18, min_dist=0.0,
+                      max_dist=30.0, retain_thresh=0.05, num_features=60, origin=(0.0, 0.0, 0.0),
+                      out=None):
+    """Binned per-cell ground plane fits of device points [cap, 4] (optional
+    device int32 count); returns device outputs."""
+    cap = points.shape[0]
+    dev = points.device
+    n_cells = radii_bins * theta_bins
+    if out is None:
+        out = {
+            "info": torch.empty((n_cells, 4), dtype=torch.int32, device=dev),
+            "model": torch.empty((n_cells, 8), dtype=torch.float64, device=dev),
+            "features": torch.empty((n_cells, max(num_features, 1), 4), dtype=torch.float32,
+                                    device=dev),
+        }
+    cfg = GroundCfg(radii_bins=radii_bins, theta_bins=theta_bins, min_dist=min_dist,
+                    max_dist=max_dist, retain_thresh=retain_thresh, num_features=num_features)
+    cfg.origin[0], cfg.origin[1], cfg.origin[2] = origin
+    check(ctx.lib.sloam_ground_planes(ctx.handle, _ptr(points), cap, _ptr(count), C.byref(cfg),
+                                      _ptr(out["info"]), _ptr(out["model"]),
+                                      _ptr(out["features"])), ctx.handle)
+    return out
+
+
+def ground_planes(ctx, points, **kw):
+    dev = torch.device("cuda", ctx.device)
+    x = torch.as_tensor(np.ascontiguousarray(points, np.float32)).to(dev)
+    torch.cuda.synchronize()
+    out = ground_planes_dev(ctx, x, **kw)
+    ctx.sync()
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+class PlaneFitPlan:
+    """Batched point-level plane RANSAC over point sets (sloam_plane_plan)."""
+
+    def __init__(self, ctx, offsets, n_hyp=200, thr=0.05, seed=0x5EED, stream_tag=1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.n_sets = len(off) - 1
+        self.n_points = int(off[-1])
+        self.cfg = PlaneFitCfg(n_hyp=n_hyp, thr=thr, seed=seed, stream_tag=stream_tag, reserved=0)
+        h = C.c_void_p()
+        check(self.lib.sloam_plane_plan_create(ctx.handle, off.ctypes.data_as(C.c_void_p),
+                                               self.n_sets, C.byref(self.cfg), C.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.sloam_plane_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc_outputs(self, counts=False, samples=False, mask=True, best=True):
+        dev = torch.device("cuda", self.ctx.device)
+        out = {"model": torch.empty((self.n_sets, 8), dtype=torch.float64, device=dev)}
+        H = self.cfg.n_hyp
+        if counts:
+            out["counts"] = torch.empty((self.n_sets, H), dtype=torch.int32, device=dev)
+        if samples:
+            out["samples"] = torch.empty((self.n_sets, H, 3), dtype=torch.int32, device=dev)
+        if mask:
+            out["mask"] = torch.empty(((self.n_points + 31) // 32,), dtype=torch.int32, device=dev)
+        if best:
+            out["best"] = torch.empty((self.n_sets,), dtype=torch.int32, device=dev)
+        return out
+
+    def exec(self, xyz, offsets, out):
+        check(self.lib.sloam_plane_plan_exec(
+            self.handle, _ptr(xyz), _ptr(offsets), _ptr(out.get("counts")),
+            _ptr(out.get("samples")), _ptr(out.get("mask")), _ptr(out.get("best")),
+            _ptr(out["model"])), self.ctx.handle)
+        return out
+
+
+def fit_planes(ctx, xyz, offsets_host, **kw):
+    want = {k: kw.pop(k) for k in ("counts", "samples", "mask", "best") if k in kw}
+    plan = PlaneFitPlan(ctx, offsets_host, **kw)
+    dev = torch.device("cuda", ctx.device)
+    x = torch.as_tensor(np.ascontiguousarray(xyz, np.float32)).to(dev)
+    o = torch.as_tensor(np.ascontiguousarray(offsets_host, np.int32)).to(dev)
+    out = plan.alloc_outputs(**want)
+    torch.cuda.synchronize()
+    plan.exec(x, o, out)
+    ctx.sync()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    if "mask" in res:
+        res["mask"] = res["mask"].view(np.uint32)
+    plan.close()
+    return res
+
+
+def fit_lines(ctx, xyz, offsets_host, threshold=0.25, probability=0.99, max_iterations=50,
+              optimize=True, seed=12345):
+    """Batched PCL-semantics RANSAC line fits; returns numpy coeff / stats / inliers."""
+    dev = torch.device("cuda", ctx.device)
+    off = np.ascontiguousarray(offsets_host, np.int32)
+    S = len(off) - 1
+    N = int(off[-1])
+    x = torch.as_tensor(np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)).to(dev)
+    o = torch.as_tensor(off).to(dev)
+    coeff = torch.empty((max(S, 1), 6), dtype=torch.float32, device=dev)
+    stats = torch.empty((max(S, 1), 4), dtype=torch.int32, device=dev)
+    inl = torch.empty((max(N, 1),), dtype=torch.uint8, device=dev)
+    cfg = LineFitCfg(threshold=threshold, probability=probability, max_iterations=max_iterations,
+                     optimize=int(optimize), seed=seed, reserved=0)
+    torch.cuda.synchronize()
+    check(ctx.lib.sloam_fit_lines(ctx.handle, _ptr(x), _ptr(o), S, C.byref(cfg), _ptr(coeff),
+                                  _ptr(stats), _ptr(inl)), ctx.handle)
+    ctx.sync()
+    return {"coeff": coeff.cpu().numpy()[:S], "stats": stats.cpu().numpy()[:S],
+            "inliers": inl.cpu().numpy()[:N]}
