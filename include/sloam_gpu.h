@@ -138,7 +138,18 @@ int sloam_cyl_plan_exec(sloam_cyl_plan* plan, const float* d_xyz, const int32_t*
  *   initial and final cost 1/2 sum rho(r^2).
  * Residuals and Jacobians are FP32 (relative to the initial translation), the
  * normal equations are summed with error-free FP32 transformations and solved
- * in FP64.                                                                  */
+ * in FP64.
+ * parametrization 0 (OptimizePose, sloam.cpp:174-255): t + unit quaternion,
+ *   rotation updated by left multiplication (Ceres' quaternion local
+ *   parameterization); all six parameters free.
+ * parametrization 1 (OptimizeXYYaw / OptimizeZRollPitch, sloam.cpp:55-172):
+ *   the 6-vector [t, angle-axis w] (ceres::QuaternionToAngleAxis of the
+ *   initial rotation) updated additively; d_free_mask (nullable: all free)
+ *   holds per problem a 6-bit mask of the optimised entries, as Ceres'
+ *   SubsetParameterization fixes the others ({2,3,4} fixed for XYYaw, {0,1,5}
+ *   for ZRollPitch).
+ * d_params_out (nullable): 6 doubles per problem, [t, angle-axis w] of the
+ *   result.                                                                  */
 typedef struct sloam_pose_cfg {
   int32_t max_iters;             /* LM iterations (reference: 50)             */
   int32_t stop_on_convergence;   /* 1: stop at convergence; 0: run max_iters  */
@@ -148,6 +159,8 @@ typedef struct sloam_pose_cfg {
   double parameter_tolerance;    /* ||dx|| / (||x|| + tol) (1e-8)             */
   double initial_radius;         /* trust region (Ceres default 1e4)          */
   double min_relative_decrease;  /* step acceptance (Ceres default 1e-3)      */
+  int32_t parametrization;       /* 0 quaternion / left, 1 angle-axis subset  */
+  int32_t reserved;
 } sloam_pose_cfg;
 
 typedef struct sloam_pose_plan sloam_pose_plan;
@@ -159,7 +172,8 @@ void sloam_pose_plan_destroy(sloam_pose_plan* plan);
 int sloam_pose_plan_exec(sloam_pose_plan* plan, const float* d_points, const double* d_cyl,
                          const int32_t* d_cyl_offsets, const double* d_planes,
                          const int32_t* d_plane_offsets, const double* d_pose_in,
-                         double* d_pose_out, int32_t* d_status, double* d_cost);
+                         const int32_t* d_free_mask, double* d_pose_out, double* d_params_out,
+                         int32_t* d_status, double* d_cost);
 
 /* ---- range projection + ground / tree partition ---------------------------
  * Replaces: seg::Segmentation::_doProjection / _makeTensor / _mask / maskCloud /

@@ -52,7 +52,9 @@ def _declare(lib):
                                            P, P, P, P, P, C.c_int]
     lib.oracle_project_partition.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_float,
                                              C.c_float, C.c_int, P, P, P, P, P, P, P]
-    lib.oracle_pose_lm_batch.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P, C.c_int]
+    lib.oracle_pose_lm_batch.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P, P, P, C.c_int]
+    lib.oracle_left_jacobian.argtypes = [P, P]
+    lib.oracle_aa_quat.argtypes = [P, P, P]
 
 
 def _p(a):
@@ -141,7 +143,8 @@ def cyl_distances_fp64(xyz, cluster, h, thr, max_radius, seed, tag=0):
 
 POSE_DEFAULTS = dict(max_iters=10, stop_on_convergence=False, huber_delta=0.1,
                      function_tolerance=1e-6, gradient_tolerance=1e-10,
-                     parameter_tolerance=1e-8, initial_radius=1e4, min_relative_decrease=1e-3)
+                     parameter_tolerance=1e-8, initial_radius=1e4, min_relative_decrease=1e-3,
+                     parametrization=0)
 
 
 def pose_residuals(points, cyl, planes, pose):
@@ -158,21 +161,39 @@ def pose_residuals(points, cyl, planes, pose):
 def pose_lm_batch(prob, threads=1, **kw):
     c = dict(POSE_DEFAULTS)
     c.update(kw)
-    cfg_i = np.array([c["max_iters"], int(c["stop_on_convergence"])], np.int32)
+    cfg_i = np.array([c["max_iters"], int(c["stop_on_convergence"]), c["parametrization"]],
+                     np.int32)
     cfg_d = np.array([c["huber_delta"], c["function_tolerance"], c["gradient_tolerance"],
                       c["parameter_tolerance"], c["initial_radius"],
                       c["min_relative_decrease"]], np.float64)
     P = len(prob["res_offsets"]) - 1
     pose = np.zeros((P, 7))
+    params = np.zeros((P, 6))
     status = np.zeros((P, 2), np.int32)
     cost = np.zeros((P, 2))
     pts = np.ascontiguousarray(prob["points"], np.float32)
+    fm = prob.get("free_mask")
+    fm = None if fm is None else np.ascontiguousarray(fm, np.int32)
     load().oracle_pose_lm_batch(_p(pts), _p(np.ascontiguousarray(prob["res_offsets"], np.int32)),
                                 P, _p(np.ascontiguousarray(prob["cyl"], np.float64)),
                                 _p(np.ascontiguousarray(prob["planes"], np.float64)),
-                                _p(np.ascontiguousarray(prob["pose_init"], np.float64)),
-                                _p(cfg_i), _p(cfg_d), _p(pose), _p(status), _p(cost), threads)
-    return {"pose": pose, "status": status, "cost": cost}
+                                _p(np.ascontiguousarray(prob["pose_init"], np.float64)), _p(fm),
+                                _p(cfg_i), _p(cfg_d), _p(pose), _p(params), _p(status), _p(cost),
+                                threads)
+    return {"pose": pose, "params": params, "status": status, "cost": cost}
+
+
+def left_jacobian(w):
+    J = np.zeros(9)
+    load().oracle_left_jacobian(_p(np.ascontiguousarray(w, np.float64)), _p(J))
+    return J.reshape(3, 3)
+
+
+def aa_quat(aa):
+    q = np.zeros(4)
+    back = np.zeros(3)
+    load().oracle_aa_quat(_p(np.ascontiguousarray(aa, np.float64)), _p(q), _p(back))
+    return q, back
 
 
 def project_partition(xyzi, label, H=64, W=2048, fov_up=22.5, fov_down=-22.5, destagger=False):

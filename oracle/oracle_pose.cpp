@@ -115,16 +115,68 @@ double residual(const double R[3][3], const double t[3], const float* p, int32_t
   return r;
 }
 
+// ceres::QuaternionToAngleAxis / AngleAxisToQuaternion (q stored x, y, z, w)
+void quat_to_aa(const double q[4], double aa[3]) {
+  const double s2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  if (s2 > 0.0) {
+    const double st = std::sqrt(s2);
+    const double c = q[3];
+    const double two_theta = 2.0 * ((c < 0.0) ? std::atan2(-st, -c) : std::atan2(st, c));
+    const double k = two_theta / st;
+    for (int i = 0; i < 3; ++i) aa[i] = q[i] * k;
+  } else {
+    for (int i = 0; i < 3; ++i) aa[i] = 2.0 * q[i];
+  }
+}
+
+void aa_to_quat(const double aa[3], double q[4]) {
+  const double t2 = aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2];
+  if (t2 > 0.0) {
+    const double th = std::sqrt(t2);
+    const double k = std::sin(0.5 * th) / th;
+    for (int i = 0; i < 3; ++i) q[i] = aa[i] * k;
+    q[3] = std::cos(0.5 * th);
+  } else {
+    for (int i = 0; i < 3; ++i) q[i] = 0.5 * aa[i];
+    q[3] = 1.0;
+  }
+}
+
+// left Jacobian of SO(3): Jl(w) = I + (1 - cos t)/t^2 [w]x + (t - sin t)/t^3 [w]x^2
+void left_jacobian(const double w[3], double J[3][3]) {
+  const double t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  double a, b;
+  if (t2 < 1e-12) {
+    a = 0.5 - t2 / 24.0;
+    b = 1.0 / 6.0 - t2 / 120.0;
+  } else {
+    const double t = std::sqrt(t2);
+    a = (1.0 - std::cos(t)) / t2;
+    b = (t - std::sin(t)) / (t2 * t);
+  }
+  const double K[3][3] = {{0.0, -w[2], w[1]}, {w[2], 0.0, -w[0]}, {-w[1], w[0], 0.0}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double k2 = 0.0;
+      for (int m = 0; m < 3; ++m) k2 += K[i][m] * K[m][j];
+      J[i][j] = (i == j ? 1.0 : 0.0) + a * K[i][j] + b * k2;
+    }
+}
+
 struct Eval {
   double F;
   double A[6][6];
   double g[6];
 };
 
-void evaluate(const Pose& x, const float* pts, int n, const double* cyl, const double* pl,
-              double delta, Eval* e) {
+// param 0: Jacobian w.r.t. [dt, left rotation]; param 1: w.r.t. [t, w]
+// additive (the rotation columns mapped by Jl(w)), masked by free_mask
+void evaluate(const Pose& x, const double* aa, int param, int free_mask, const float* pts, int n,
+              const double* cyl, const double* pl, double delta, Eval* e) {
   double R[3][3];
   quat_to_R(x.q, R);
+  double Jl[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  if (param == 1) left_jacobian(aa + 3, Jl);
   std::memset(e, 0, sizeof(Eval));
   const double d2 = delta * delta;
   for (int i = 0; i < n; ++i) {
@@ -133,6 +185,12 @@ void evaluate(const Pose& x, const float* pts, int n, const double* cyl, const d
     std::memcpy(&id, p + 3, 4);
     double J[6];
     const double r = residual(R, x.t, p, id, cyl, pl, J);
+    if (param == 1) {
+      const double jt[3] = {J[3], J[4], J[5]};
+      for (int c = 0; c < 3; ++c) J[3 + c] = jt[0] * Jl[0][c] + jt[1] * Jl[1][c] + jt[2] * Jl[2][c];
+    }
+    for (int a = 0; a < 6; ++a)
+      if (!((free_mask >> a) & 1)) J[a] = 0.0;
     const double s = r * r;
     double w, rho;
     if (s <= d2) {
@@ -191,11 +249,12 @@ int oracle_pose_residuals(const float* pts, int n, const double* cyl, const doub
 // (termination: 0 CONVERGENCE, 1 NO_CONVERGENCE; iterations); cost: 2 doubles
 // (initial, final).
 int oracle_pose_lm_batch(const float* pts, const int32_t* off, int n_problems, const double* cyl,
-                         const double* pl, const double* pose_in, const int32_t cfg_i[2],
-                         const double cfg_d[6], double* pose_out, int32_t* status, double* cost,
-                         int n_threads) {
+                         const double* pl, const double* pose_in, const int32_t* free_masks,
+                         const int32_t cfg_i[3], const double cfg_d[6], double* pose_out,
+                         double* params_out, int32_t* status, double* cost, int n_threads) {
   const int max_iters = cfg_i[0];
   const bool stop_conv = cfg_i[1] != 0;
+  const int param = cfg_i[2];
   const double delta = cfg_d[0], ftol = cfg_d[1], gtol = cfg_d[2], ptol = cfg_d[3];
   const double radius0 = cfg_d[4], min_rel = cfg_d[5];
 #pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads > 0 ? n_threads : 1)
@@ -205,8 +264,11 @@ int oracle_pose_lm_batch(const float* pts, const int32_t* off, int n_problems, c
     Pose x;
     std::memcpy(x.t, pose_in + 7 * k, 3 * sizeof(double));
     std::memcpy(x.q, pose_in + 7 * k + 3, 4 * sizeof(double));
+    const int fm = free_masks ? free_masks[k] : 0x3f;
+    double aa[6] = {x.t[0], x.t[1], x.t[2], 0, 0, 0};
+    quat_to_aa(x.q, aa + 3);
     Eval cur;
-    evaluate(x, P, n, cyl, pl, delta, &cur);
+    evaluate(x, aa, param, fm, P, n, cyl, pl, delta, &cur);
     const double F0 = cur.F;
     double radius = radius0, factor = 2.0;
     int term = 1, it = 0;
@@ -246,18 +308,33 @@ int oracle_pose_lm_batch(const float* pts, const int32_t* off, int n_problems, c
       }
       const double model = -(gdx + 0.5 * dAd);
       double xn = 0, dn = 0;
-      for (int a = 0; a < 3; ++a) xn += x.t[a] * x.t[a];
-      for (int a = 0; a < 4; ++a) xn += x.q[a] * x.q[a];
+      if (param == 1) {
+        for (int a = 0; a < 6; ++a) xn += aa[a] * aa[a];
+      } else {
+        for (int a = 0; a < 3; ++a) xn += x.t[a] * x.t[a];
+        for (int a = 0; a < 4; ++a) xn += x.q[a] * x.q[a];
+      }
       for (int a = 0; a < 6; ++a) dn += dx[a] * dx[a];
       xn = std::sqrt(xn);
       dn = std::sqrt(dn);
-      const Pose xc = apply_step(x, dx);
+      Pose xc;
+      double aac[6];
+      if (param == 1) {
+        for (int a = 0; a < 6; ++a) aac[a] = aa[a] + dx[a];
+        for (int a = 0; a < 3; ++a) xc.t[a] = aac[a];
+        aa_to_quat(aac + 3, xc.q);
+      } else {
+        xc = apply_step(x, dx);
+        for (int a = 0; a < 6; ++a) aac[a] = aa[a];
+      }
       Eval cand;
-      evaluate(xc, P, n, cyl, pl, delta, &cand);
+      evaluate(xc, aac, param, fm, P, n, cyl, pl, delta, &cand);
       const double change = cur.F - cand.F;
       const double q = model > 0 ? change / model : -1.0;
       if (std::isfinite(cand.F) && q > min_rel) {
         x = xc;
+        if (param == 1)
+          for (int a = 0; a < 6; ++a) aa[a] = aac[a];
         const double Fprev = cur.F;
         cur = cand;
         const double c3 = 2.0 * q - 1.0;
@@ -281,11 +358,33 @@ int oracle_pose_lm_batch(const float* pts, const int32_t* off, int n_problems, c
     }
     std::memcpy(pose_out + 7 * k, x.t, 3 * sizeof(double));
     std::memcpy(pose_out + 7 * k + 3, x.q, 4 * sizeof(double));
+    if (params_out) {
+      if (param == 1) {
+        std::memcpy(params_out + 6 * k, aa, 6 * sizeof(double));
+      } else {
+        std::memcpy(params_out + 6 * k, x.t, 3 * sizeof(double));
+        quat_to_aa(x.q, params_out + 6 * k + 3);
+      }
+    }
     status[2 * k] = term;
     status[2 * k + 1] = it;
     cost[2 * k] = F0;
     cost[2 * k + 1] = cur.F;
   }
+  return 0;
+}
+
+int oracle_left_jacobian(const double w[3], double J[9]) {
+  double JJ[3][3];
+  left_jacobian(w, JJ);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) J[3 * i + j] = JJ[i][j];
+  return 0;
+}
+
+int oracle_aa_quat(const double aa[3], double q[4], double back[3]) {
+  aa_to_quat(aa, q);
+  quat_to_aa(q, back);
   return 0;
 }
 

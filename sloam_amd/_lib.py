@@ -82,6 +82,8 @@ class PoseCfg(C.Structure):
         ("parameter_tolerance", C.c_double),
         ("initial_radius", C.c_double),
         ("min_relative_decrease", C.c_double),
+        ("parametrization", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 _P = C.c_void_p
@@ -105,7 +107,7 @@ PROTOTYPES = {
     "sloam_pose_plan_create": (C.c_int, [_P, _P, _I32, _I32, _I32, C.POINTER(PoseCfg),
                                          C.POINTER(_P)]),
     "sloam_pose_plan_destroy": (None, [_P]),
-    "sloam_pose_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sloam_pose_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sloam_project_partition": (C.c_int, [_P, _P, _P, _I64, C.POINTER(ProjCfg), _P, _P, _P, _P,
                                           _P, _P, _P]),
     "sloam_ground_planes": (C.c_int, [_P, _P, _I32, _P, C.POINTER(GroundCfg), _P, _P, _P]),

@@ -40,11 +40,14 @@ struct LMState {
   double g[6];
   double radius, factor;
   double model, dn, xn;  // of the pending candidate
+  double aa[6], aae[6];   // angle-axis parametrization: accepted / to evaluate [t, w]
   int32_t iter, term, done, phase;  // phase 0: next evaluation is the initial one
+  int32_t free_mask;                // bit i: parameter i is optimised
+  int32_t pad;
 };
 
 struct PoseCfgDev {
-  int32_t max_iters, stop_conv;
+  int32_t max_iters, stop_conv, param;  // param 0: quaternion / left perturbation, 1: angle-axis
   double delta, ftol, gtol, ptol, radius0, min_rel;
 };
 
@@ -88,6 +91,63 @@ __device__ inline void quat_left_update(const double q[4], const double dth[3], 
   out[1] = y / n;
   out[2] = z / n;
   out[3] = w / n;
+}
+
+// angle-axis <-> quaternion (w, x, y, z order internally)
+__device__ inline void quat_to_aa(const double qxyzw[4], double aa[3]) {
+  const double q1 = qxyzw[0], q2 = qxyzw[1], q3 = qxyzw[2], c = qxyzw[3];
+  const double s2 = q1 * q1 + q2 * q2 + q3 * q3;
+  if (s2 > 0.0) {
+    const double st = sqrt(s2);
+    const double two_theta = 2.0 * ((c < 0.0) ? atan2(-st, -c) : atan2(st, c));
+    const double k = two_theta / st;
+    aa[0] = q1 * k;
+    aa[1] = q2 * k;
+    aa[2] = q3 * k;
+  } else {
+    aa[0] = 2.0 * q1;
+    aa[1] = 2.0 * q2;
+    aa[2] = 2.0 * q3;
+  }
+}
+
+__device__ inline void aa_to_quat(const double aa[3], double qxyzw[4]) {
+  const double t2 = aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2];
+  if (t2 > 0.0) {
+    const double th = sqrt(t2);
+    const double h = 0.5 * th;
+    const double k = sin(h) / th;
+    qxyzw[0] = aa[0] * k;
+    qxyzw[1] = aa[1] * k;
+    qxyzw[2] = aa[2] * k;
+    qxyzw[3] = cos(h);
+  } else {
+    qxyzw[0] = 0.5 * aa[0];
+    qxyzw[1] = 0.5 * aa[1];
+    qxyzw[2] = 0.5 * aa[2];
+    qxyzw[3] = 1.0;
+  }
+}
+
+// left Jacobian of SO(3): exp(w + dw) ~ exp(Jl(w) dw) exp(w)
+__device__ inline void left_jacobian(const double w[3], double J[3][3]) {
+  const double t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  double a, b;
+  if (t2 < 1e-12) {
+    a = 0.5 - t2 / 24.0;
+    b = 1.0 / 6.0 - t2 / 120.0;
+  } else {
+    const double t = sqrt(t2);
+    a = (1.0 - cos(t)) / t2;
+    b = (t - sin(t)) / (t2 * t);
+  }
+  const double K[3][3] = {{0.0, -w[2], w[1]}, {w[2], 0.0, -w[0]}, {-w[1], w[0], 0.0}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double k2 = 0.0;
+      for (int m = 0; m < 3; ++m) k2 += K[i][m] * K[m][j];
+      J[i][j] = (i == j ? 1.0 : 0.0) + a * K[i][j] + b * k2;
+    }
 }
 
 __device__ inline bool chol6(const double A[6][6], const double b[6], double x[6]) {
@@ -139,7 +199,8 @@ __device__ inline void two_sum(float a, float b, float* s, float* e) {
 // prep: LM state init + FP32 landmark tables relative to the anchor
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
-    int n_problems, const double* __restrict__ pose_in, const double* __restrict__ cyl,
+    int n_problems, const double* __restrict__ pose_in, const int32_t* __restrict__ free_mask,
+    const double* __restrict__ cyl,
     const int32_t* __restrict__ cyl_off, const double* __restrict__ pl,
     const int32_t* __restrict__ pl_off, PoseCfgDev cfg, LMState* __restrict__ st,
     float4* __restrict__ cylf, float4* __restrict__ plf) {
@@ -180,6 +241,16 @@ __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
     s.term = 1;
     s.done = 0;
     s.phase = 0;
+    s.free_mask = free_mask ? free_mask[k] : 0x3f;
+    s.pad = 0;
+    s.aa[0] = s.aae[0] = x0[0];
+    s.aa[1] = s.aae[1] = x0[1];
+    s.aa[2] = s.aae[2] = x0[2];
+    {
+      double w[3];
+      quat_to_aa(x0 + 3, w);
+      for (int i = 0; i < 3; ++i) s.aa[3 + i] = s.aae[3 + i] = w[i];
+    }
     st[k] = s;
   }
 }
@@ -190,25 +261,40 @@ __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
     const float4* __restrict__ pts, const int4* __restrict__ work, const LMState* __restrict__ st,
-    const float4* __restrict__ cylf, const float4* __restrict__ plf, float delta,
+    const float4* __restrict__ cylf, const float4* __restrict__ plf, float delta, int param,
     double* __restrict__ partials) {
   __shared__ float red_s[kPBlock / 64][kSums];
   __shared__ float red_c[kPBlock / 64][kSums];
   const int4 w = work[blockIdx.x];
   const int k = w.x;
   if (st[k].done) return;
-  // pose to evaluate, relative to the anchor, in FP32
-  float R[3][3], t[3];
+  // pose to evaluate, relative to the anchor, in FP32; for the angle-axis
+  // parametrization the rotation columns of the Jacobian are mapped by the
+  // left Jacobian Jl(w) and fixed parameters are masked
+  float R[3][3], t[3], Jl[3][3], msk[6];
   {
     double q[4] = {st[k].xe[3], st[k].xe[4], st[k].xe[5], st[k].xe[6]};
+    double Jd[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    if (param == 1) {
+      const double w[3] = {st[k].aae[3], st[k].aae[4], st[k].aae[5]};
+      aa_to_quat(w, q);
+      left_jacobian(w, Jd);
+    }
     double Rd[3][3];
     quat_to_R(q, Rd);
+    const double* tt = param == 1 ? st[k].aae : st[k].xe;
+    const int fm = st[k].free_mask;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) R[i][j] = (float)Rd[i][j];
-      t[i] = (float)(st[k].xe[i] - st[k].anchor[i]);
+      for (int j = 0; j < 3; ++j) {
+        R[i][j] = (float)Rd[i][j];
+        Jl[i][j] = (float)Jd[i][j];
+      }
+      t[i] = (float)(tt[i] - st[k].anchor[i]);
     }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) msk[i] = ((fm >> i) & 1) ? 1.0f : 0.0f;
   }
   const float d2 = delta * delta;
   float acc[kSums];
@@ -250,12 +336,18 @@ __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
       dir[2] = sg * L.z;
     }
     float J[6];
-    J[0] = dir[0];
-    J[1] = dir[1];
-    J[2] = dir[2];
-    J[3] = __builtin_fmaf(rp[1], dir[2], -rp[2] * dir[1]);
-    J[4] = __builtin_fmaf(rp[2], dir[0], -rp[0] * dir[2]);
-    J[5] = __builtin_fmaf(rp[0], dir[1], -rp[1] * dir[0]);
+    J[0] = dir[0] * msk[0];
+    J[1] = dir[1] * msk[1];
+    J[2] = dir[2] * msk[2];
+    {
+      const float jt0 = __builtin_fmaf(rp[1], dir[2], -rp[2] * dir[1]);
+      const float jt1 = __builtin_fmaf(rp[2], dir[0], -rp[0] * dir[2]);
+      const float jt2 = __builtin_fmaf(rp[0], dir[1], -rp[1] * dir[0]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        J[3 + c] = __builtin_fmaf(jt0, Jl[0][c], __builtin_fmaf(jt1, Jl[1][c], jt2 * Jl[2][c])) *
+                   msk[3 + c];
+    }
     const float s2 = r * r;
     float wgt, rho;
     if (s2 <= d2) {
@@ -325,7 +417,7 @@ __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
 __global__ __launch_bounds__(64) void pose_solve_kernel(
     int n_problems, const int32_t* __restrict__ first_work, const double* __restrict__ partials,
     PoseCfgDev cfg, LMState* __restrict__ st, double* __restrict__ pose_out,
-    int32_t* __restrict__ status, double* __restrict__ cost) {
+    int32_t* __restrict__ status, double* __restrict__ cost, double* __restrict__ params_out) {
   __shared__ double tot[kSums];
   const int k = blockIdx.x;
   if (k >= n_problems) return;
@@ -374,6 +466,7 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
     const double q = S.model > 0.0 ? change / S.model : -1.0;
     if (isfinite(Fe) && q > cfg.min_rel) {
       for (int i = 0; i < 7; ++i) S.x[i] = S.xe[i];
+      for (int i = 0; i < 6; ++i) S.aa[i] = S.aae[i];
       const double Fprev = S.F;
       store_eval();
       const double c3 = 2.0 * q - 1.0;
@@ -431,14 +524,26 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
     }
     S.model = -(gdx + 0.5 * dAd);
     double xn = 0.0, dn = 0.0;
-    for (int a = 0; a < 7; ++a) xn += S.x[a] * S.x[a];
+    if (cfg.param == 1) {
+      for (int a = 0; a < 6; ++a) xn += S.aa[a] * S.aa[a];
+    } else {
+      for (int a = 0; a < 7; ++a) xn += S.x[a] * S.x[a];
+    }
     for (int a = 0; a < 6; ++a) dn += dx[a] * dx[a];
     S.xn = sqrt(xn);
     S.dn = sqrt(dn);
-    for (int a = 0; a < 3; ++a) S.xe[a] = S.x[a] + dx[a];
-    double qn[4];
-    quat_left_update(S.x + 3, dx + 3, qn);
-    for (int a = 0; a < 4; ++a) S.xe[3 + a] = qn[a];
+    if (cfg.param == 1) {
+      for (int a = 0; a < 6; ++a) S.aae[a] = S.aa[a] + dx[a];
+      double qn[4];
+      aa_to_quat(S.aae + 3, qn);
+      for (int a = 0; a < 3; ++a) S.xe[a] = S.aae[a];
+      for (int a = 0; a < 4; ++a) S.xe[3 + a] = qn[a];
+    } else {
+      for (int a = 0; a < 3; ++a) S.xe[a] = S.x[a] + dx[a];
+      double qn[4];
+      quat_left_update(S.x + 3, dx + 3, qn);
+      for (int a = 0; a < 4; ++a) S.xe[3 + a] = qn[a];
+    }
     have_candidate = true;
     break;
   }
@@ -446,6 +551,18 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
   if (!have_candidate) S.done = 1;
   st[k] = S;
   for (int i = 0; i < 7; ++i) pose_out[7 * (size_t)k + i] = S.x[i];
+  if (params_out) {
+    if (cfg.param == 1) {
+      for (int i = 0; i < 6; ++i) params_out[6 * (size_t)k + i] = S.aa[i];
+    } else {
+      double w[3];
+      quat_to_aa(S.x + 3, w);
+      for (int i = 0; i < 3; ++i) {
+        params_out[6 * (size_t)k + i] = S.x[i];
+        params_out[6 * (size_t)k + 3 + i] = w[i];
+      }
+    }
+  }
   if (status) {
     status[2 * k] = S.term;
     status[2 * k + 1] = S.iter;
@@ -482,7 +599,8 @@ int sloam_pose_plan_create(sloam_ctx* ctx, const int32_t* h_res_offsets, int32_t
       n_planes < 0)
     return SLOAM_ERR_INVALID_ARG;
   *out = nullptr;
-  if (cfg->max_iters < 0 || !(cfg->huber_delta > 0.0) || !(cfg->initial_radius > 0.0))
+  if (cfg->max_iters < 0 || !(cfg->huber_delta > 0.0) || !(cfg->initial_radius > 0.0) ||
+      cfg->parametrization < 0 || cfg->parametrization > 1)
     return SLOAM_ERR_INVALID_ARG;
   if (h_res_offsets[0] != 0) return SLOAM_ERR_INVALID_ARG;
   for (int k = 0; k < n_problems; ++k)
@@ -546,7 +664,8 @@ void sloam_pose_plan_destroy(sloam_pose_plan* p) {
 int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double* d_cyl,
                          const int32_t* d_cyl_offsets, const double* d_planes,
                          const int32_t* d_plane_offsets, const double* d_pose_in,
-                         double* d_pose_out, int32_t* d_status, double* d_cost) {
+                         const int32_t* d_free_mask, double* d_pose_out, double* d_params_out,
+                         int32_t* d_status, double* d_cost) {
   if (!p || !d_cyl_offsets || !d_plane_offsets || !d_pose_in || !d_pose_out)
     return SLOAM_ERR_INVALID_ARG;
   if ((p->n_res > 0 && !d_points) || (p->n_cyl > 0 && !d_cyl) || (p->n_pl > 0 && !d_planes))
@@ -558,6 +677,7 @@ int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double
   PoseCfgDev dc;
   dc.max_iters = c.max_iters;
   dc.stop_conv = c.stop_on_convergence;
+  dc.param = c.parametrization;
   dc.delta = c.huber_delta;
   dc.ftol = c.function_tolerance;
   dc.gtol = c.gradient_tolerance;
@@ -567,7 +687,8 @@ int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double
   {
     SloamKernelScope ks(ctx, "pose_prep");
     hipLaunchKernelGGL(pose_prep_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
-                       p->n_problems, d_pose_in, d_cyl, d_cyl_offsets, d_planes, d_plane_offsets,
+                       p->n_problems, d_pose_in, d_free_mask, d_cyl, d_cyl_offsets, d_planes,
+                       d_plane_offsets,
                        dc, p->d_state, p->d_cylf, p->d_plf);
   }
   SLOAM_HIP(ctx, hipGetLastError());
@@ -576,14 +697,15 @@ int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double
       SloamKernelScope ks(ctx, "pose_eval");
       hipLaunchKernelGGL(pose_eval_kernel, dim3(p->n_work), dim3(kPBlock), 0, ctx->stream,
                          reinterpret_cast<const float4*>(d_points), p->d_work, p->d_state,
-                         p->d_cylf, p->d_plf, (float)c.huber_delta, p->d_partials);
+                         p->d_cylf, p->d_plf, (float)c.huber_delta, c.parametrization,
+                         p->d_partials);
     }
     SLOAM_HIP(ctx, hipGetLastError());
     {
       SloamKernelScope ks(ctx, "pose_solve");
       hipLaunchKernelGGL(pose_solve_kernel, dim3(p->n_problems), dim3(64), 0, ctx->stream,
                          p->n_problems, p->d_first, p->d_partials, dc, p->d_state, d_pose_out,
-                         d_status, d_cost);
+                         d_status, d_cost, d_params_out);
     }
     SLOAM_HIP(ctx, hipGetLastError());
   }

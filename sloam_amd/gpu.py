@@ -154,7 +154,7 @@ class PosePlan:
     def __init__(self, ctx, res_offsets, n_cylinders, n_planes, max_iters=10,
                  stop_on_convergence=False, huber_delta=0.1, function_tolerance=1e-6,
                  gradient_tolerance=1e-10, parameter_tolerance=1e-8, initial_radius=1e4,
-                 min_relative_decrease=1e-3):
+                 min_relative_decrease=1e-3, parametrization=0):
         self.ctx = ctx
         self.lib = ctx.lib
         off = np.ascontiguousarray(res_offsets, dtype=np.int32)
@@ -165,7 +165,8 @@ class PosePlan:
                            gradient_tolerance=gradient_tolerance,
                            parameter_tolerance=parameter_tolerance,
                            initial_radius=initial_radius,
-                           min_relative_decrease=min_relative_decrease)
+                           min_relative_decrease=min_relative_decrease,
+                           parametrization=parametrization, reserved=0)
         h = C.c_void_p()
         check(self.lib.sloam_pose_plan_create(ctx.handle, off.ctypes.data_as(C.c_void_p),
                                               self.n_problems, int(n_cylinders), int(n_planes),
@@ -187,15 +188,16 @@ class PosePlan:
         dev = torch.device("cuda", self.ctx.device)
         return {
             "pose": torch.empty((self.n_problems, 7), dtype=torch.float64, device=dev),
+            "params": torch.empty((self.n_problems, 6), dtype=torch.float64, device=dev),
             "status": torch.empty((self.n_problems, 2), dtype=torch.int32, device=dev),
             "cost": torch.empty((self.n_problems, 2), dtype=torch.float64, device=dev),
         }
 
-    def exec(self, points, cyl, cyl_off, planes, plane_off, pose_in, out):
+    def exec(self, points, cyl, cyl_off, planes, plane_off, pose_in, out, free_mask=None):
         check(self.lib.sloam_pose_plan_exec(
             self.handle, _ptr(points), _ptr(cyl), _ptr(cyl_off), _ptr(planes), _ptr(plane_off),
-            _ptr(pose_in), _ptr(out["pose"]), _ptr(out.get("status")), _ptr(out.get("cost"))),
-            self.ctx.handle)
+            _ptr(pose_in), _ptr(free_mask), _ptr(out["pose"]), _ptr(out.get("params")),
+            _ptr(out.get("status")), _ptr(out.get("cost"))), self.ctx.handle)
         return out
 
 
@@ -205,10 +207,13 @@ def solve_poses(ctx, prob, **kw):
     plan = PosePlan(ctx, prob["res_offsets"], len(prob["cyl"]), len(prob["planes"]), **kw)
     t = {k: torch.as_tensor(np.ascontiguousarray(prob[k])).to(dev)
          for k in ("points", "cyl", "cyl_offsets", "planes", "plane_offsets", "pose_init")}
+    fm = None
+    if "free_mask" in prob:
+        fm = torch.as_tensor(np.ascontiguousarray(prob["free_mask"], np.int32)).to(dev)
     out = plan.alloc_outputs()
     torch.cuda.synchronize()
     plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
-              t["pose_init"], out)
+              t["pose_init"], out, free_mask=fm)
     ctx.sync()
     res = {k: v.cpu().numpy() for k, v in out.items()}
     plan.close()

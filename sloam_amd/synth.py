@@ -310,3 +310,44 @@ def make_scan(seed=0, H=64, W=2048, fov_deg=22.5, n_trees=40, r_min=0.1, r_max=0
     xyzi[:, 3] = rng.uniform(0, 255, n)
     return {"xyzi": xyzi, "label": label, "instance": inst, "trees": trees,
             "ground": np.concatenate([gn, [gd]]), "H": H, "W": W, "fov_deg": fov_deg}
+
+
+XYYAW_MASK = 0b100011  # t_x, t_y, w_z free (sloam.cpp:75-77 fixes {2, 3, 4})
+ZROLLPITCH_MASK = 0b011100  # t_z, w_x, w_y free (sloam.cpp:136-138 fixes {0, 1, 5})
+
+
+def split_two_step(prob):
+    """Two-step pose workload from a joint one: per problem, an XY-yaw problem
+    over its cylinder features and a Z-roll-pitch problem over its plane
+    features, both from the same initial pose (TwoStepOptimizePose,
+    sloam/src/core/sloam.cpp:33-53)."""
+    pts, off, masks, init = [], [0], [], []
+    P = prob["points"]
+    ro = prob["res_offsets"]
+    for k in range(len(ro) - 1):
+        seg = P[ro[k]:ro[k + 1]]
+        ids = seg[:, 3].view(np.int32)
+        for sel, m in ((ids >= 0, XYYAW_MASK), (ids < 0, ZROLLPITCH_MASK)):
+            pts.append(seg[sel])
+            off.append(off[-1] + int(sel.sum()))
+            masks.append(m)
+            init.append(prob["pose_init"][k])
+    out = dict(prob)
+    out["points"] = np.concatenate(pts) if pts else np.zeros((0, 4), np.float32)
+    out["res_offsets"] = np.array(off, np.int32)
+    out["free_mask"] = np.array(masks, np.int32)
+    out["pose_init"] = np.array(init)
+    # each derived problem keeps the landmark tables of its source problem
+    n = len(ro) - 1
+    out["cyl_offsets"] = np.repeat(prob["cyl_offsets"][:-1], 2).tolist() + [prob["cyl_offsets"][-1]]
+    out["cyl_offsets"] = np.array(out["cyl_offsets"], np.int32)
+    co = prob["cyl_offsets"]
+    po = prob["plane_offsets"]
+    cyl_off, pl_off = [0], [0]
+    for k in range(n):
+        # XY-yaw problem owns the cylinders, Z-roll-pitch the planes
+        cyl_off += [co[k + 1], co[k + 1]]
+        pl_off += [po[k], po[k + 1]]
+    out["cyl_offsets"] = np.array(cyl_off, np.int32)
+    out["plane_offsets"] = np.array(pl_off, np.int32)
+    return out

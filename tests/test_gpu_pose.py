@@ -82,3 +82,19 @@ def test_deterministic_repeat(ctx):
     b = solve_poses(ctx, prob, max_iters=10)
     for k in a:
         assert np.array_equal(a[k], b[k])
+
+
+def test_two_step_angle_axis_subsets(ctx):
+    """TwoStepOptimizePose's two solves (XY-yaw over cylinders, Z-roll-pitch
+    over planes, angle-axis parameters with subsets fixed) as one batch."""
+    prob = synth.split_two_step(synth.make_pose_problems(8, n_cyl=100, n_planes=20, seed=25))
+    g = solve_poses(ctx, prob, max_iters=50, stop_on_convergence=True, parametrization=1)
+    o = oracle.pose_lm_batch(prob, threads=8, max_iters=50, stop_on_convergence=True,
+                             parametrization=1)
+    _check(prob, g, o, "two-step")
+    for k, m in enumerate(prob["free_mask"]):
+        fixed = [i for i in range(6) if not (m >> i) & 1]
+        # fixed entries keep their initial values (the initial angle-axis
+        # conversion uses atan2, whose last bit may differ between libms)
+        assert np.allclose(g["params"][k, fixed], o["params"][k, fixed], atol=1e-12, rtol=0)
+    assert np.allclose(g["params"], o["params"], atol=1e-4)

@@ -138,3 +138,78 @@ def test_threads_do_not_change_results():
     b = oracle.pose_lm_batch(prob, threads=6)
     for key in a:
         assert np.array_equal(a[key], b[key])
+
+
+def test_left_jacobian_finite_differences():
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        w = rng.normal(size=3) * rng.uniform(0.01, 2.0)
+        J = oracle.left_jacobian(w)
+        R0 = synth._rotvec_to_R(w)
+        eps = 1e-6
+        for i in range(3):
+            d = np.zeros(3)
+            d[i] = eps
+            Rp = synth._rotvec_to_R(w + d) @ R0.T
+            Rm = synth._rotvec_to_R(w - d) @ R0.T
+            # log of a near-identity rotation: vee of the skew part
+            col = (np.array([Rp[2, 1] - Rp[1, 2], Rp[0, 2] - Rp[2, 0], Rp[1, 0] - Rp[0, 1]]) -
+                   np.array([Rm[2, 1] - Rm[1, 2], Rm[0, 2] - Rm[2, 0], Rm[1, 0] - Rm[0, 1]])) / (
+                       4 * eps)
+            assert np.allclose(J[:, i], col, atol=1e-7)
+
+
+def test_angle_axis_round_trip():
+    rng = np.random.default_rng(10)
+    for _ in range(50):
+        # canonical range |w| < pi (the inverse returns angles in [-pi, pi])
+        w = rng.normal(size=3)
+        w *= rng.uniform(0, 3.1) / np.linalg.norm(w)
+        q, back = oracle.aa_quat(w)
+        assert abs(np.linalg.norm(q) - 1) < 1e-12
+        assert np.allclose(back, w, atol=1e-12)
+        assert np.allclose(synth.quat_to_R(q), synth._rotvec_to_R(w), atol=1e-12)
+
+
+def test_two_step_subsets_fixed_and_optimum():
+    """XY-yaw over cylinders with z, roll, pitch fixed; Z-roll-pitch over
+    planes with x, y, yaw fixed (sloam.cpp:55-172).  Fixed entries of the
+    angle-axis parameter vector stay bit-identical; the free ones reach the
+    optimum scipy finds over the same three parameters."""
+    prob = synth.split_two_step(_small(seed=12, n_problems=2))
+    # tight tolerances: compare optima, not the default stopping point
+    res = oracle.pose_lm_batch(prob, max_iters=200, stop_on_convergence=True, parametrization=1,
+                               function_tolerance=1e-15, parameter_tolerance=1e-15,
+                               gradient_tolerance=1e-15)
+    for k in range(len(prob["res_offsets"]) - 1):
+        p0 = prob["pose_init"][k]
+        aa0 = np.concatenate([p0[:3], _aa(p0[3:])])
+        m = prob["free_mask"][k]
+        free = [i for i in range(6) if (m >> i) & 1]
+        fixed = [i for i in range(6) if not (m >> i) & 1]
+        # fixed entries: the initial angle-axis values, untouched
+        assert np.allclose(res["params"][k, fixed], aa0[fixed], atol=1e-12)
+        pts = prob["points"][prob["res_offsets"][k]:prob["res_offsets"][k + 1]]
+
+        def fun(z):
+            aa = aa0.copy()
+            aa[free] = z
+            q, _ = oracle.aa_quat(aa[3:])
+            r, _ = oracle.pose_residuals(pts, prob["cyl"], prob["planes"],
+                                         np.concatenate([aa[:3], q]))
+            return r
+
+        sol = least_squares(fun, aa0[free], loss="huber", f_scale=0.1, method="trf",
+                            xtol=1e-15, ftol=1e-15, gtol=1e-15, max_nfev=2000)
+        assert np.allclose(sol.x, res["params"][k, free], atol=1e-6)
+        assert res["cost"][k, 1] <= sol.cost * (1 + 1e-9) + 1e-12
+
+
+def _aa(q):
+    # independent quaternion -> rotation vector (scipy-free): via the matrix log
+    R = synth.quat_to_R(q)
+    th = math.acos(max(-1.0, min(1.0, (np.trace(R) - 1) / 2)))
+    if th < 1e-12:
+        return np.zeros(3)
+    v = np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    return v * th / (2 * math.sin(th))
