@@ -215,7 +215,12 @@ int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* 
  * d_cell_model 8 doubles {a, b, c, d (unit normal, z >= 0), centroid x, y, z,
  * 0}; d_features num_features x 4 floats: the cell's first num_features kept
  * points (bottom points by increasing z when the cell was reduced, input order
- * otherwise).  A cell is valid iff it keeps >= num_features (and > 0) points. */
+ * otherwise).  A cell is valid iff it keeps >= num_features (and > 0) points.
+ * Optional (both NULL or both set): d_kept_offsets [cells + 1] and
+ * d_kept_index [capacity]: cell c's binned points occupy slots
+ * [kept_offsets[c], kept_offsets[c+1]); its kept points' input indices come
+ * first, in the order above, and the remaining slots hold -1 (the cells of
+ * binGroundPoints, sloam.cpp:330-386).                                       */
 typedef struct sloam_ground_cfg {
   int32_t radii_bins;       /* groundRadiiBins                               */
   int32_t theta_bins;       /* groundThetaBins                               */
@@ -228,7 +233,8 @@ typedef struct sloam_ground_cfg {
 
 int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
                         const int32_t* d_count, const sloam_ground_cfg* cfg, int32_t* d_cell_info,
-                        double* d_cell_model, float* d_features);
+                        double* d_cell_model, float* d_features, int32_t* d_kept_offsets,
+                        int32_t* d_kept_index);
 
 /* ---- batched ground-plane RANSAC (point-level, fixed hypotheses) ----------
  * No counterpart in the reference (its planes are per-cell SVD fits); the

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsloam_gpu.so")
@@ -52,7 +53,8 @@ def _compile(src):
         cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
     else:
         # host-only C++ (drop-in classes); hipcc for the HIP runtime headers
-        cmd = [HIPCC] + COMMON + ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-c", src, "-o", obj]
+        cmd = [HIPCC] + COMMON + ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-isystem",
+                                  os.path.join(ROCM, "include"), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -136,7 +136,8 @@ __device__ inline void gsum(double (&v)[K], double* lds) {
 __global__ __launch_bounds__(kGB) void ground_fit_kernel(
     const float4* __restrict__ pts, const int32_t* __restrict__ offsets,
     const int32_t* __restrict__ by_idx, const int32_t* __restrict__ by_z, GroundCfgDev cfg,
-    int4* __restrict__ info, double* __restrict__ model, float4* __restrict__ features) {
+    int4* __restrict__ info, double* __restrict__ model, float4* __restrict__ features,
+    int32_t* __restrict__ kept_index, int32_t* __restrict__ kept_offsets) {
   __shared__ float tile[3][kCentroidTile];
   __shared__ double red[(kGB / 64) * 6];
   __shared__ float cen[3];
@@ -152,6 +153,14 @@ __global__ __launch_bounds__(kGB) void ground_fit_kernel(
   }
   const bool valid = k >= cfg.nf && k > 0;
   double* out = model + 8 * (size_t)c;
+  if (kept_index) {
+    // the cell's kept points, in the order used (slots past k hold -1)
+    for (int j = threadIdx.x; j < s; j += kGB) kept_index[off + j] = j < k ? list[j] : -1;
+    if (threadIdx.x == 0) {
+      kept_offsets[c] = off;
+      if (c == cfg.n_cells - 1) kept_offsets[cfg.n_cells] = off + s;
+    }
+  }
   if (!valid) {
     if (threadIdx.x == 0) {
       info[c] = make_int4(0, k, s, 0);
@@ -606,7 +615,8 @@ extern "C" {
 
 int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
                         const int32_t* d_count, const sloam_ground_cfg* cfg, int32_t* d_cell_info,
-                        double* d_cell_model, float* d_features) {
+                        double* d_cell_model, float* d_features, int32_t* d_kept_offsets,
+                        int32_t* d_kept_index) {
   if (!ctx || !cfg || !d_cell_info || !d_cell_model || capacity < 0) return SLOAM_ERR_INVALID_ARG;
   if (cfg->radii_bins < 1 || cfg->theta_bins < 1 || !(cfg->retain_thresh > 0.0) ||
       cfg->num_features < 0)
@@ -615,6 +625,7 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
   if (n_cells > 65535) return SLOAM_ERR_TOO_LARGE;
   if (cfg->num_features > 0 && !d_features) return SLOAM_ERR_INVALID_ARG;
   if (capacity > 0 && !d_points) return SLOAM_ERR_INVALID_ARG;
+  if ((d_kept_index != nullptr) != (d_kept_offsets != nullptr)) return SLOAM_ERR_INVALID_ARG;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
   GroundCfgDev dc;
   dc.R = cfg->radii_bins;
@@ -689,7 +700,7 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
     hipLaunchKernelGGL(ground_fit_kernel, dim3(n_cells), dim3(kGB), 0, ctx->stream,
                        reinterpret_cast<const float4*>(d_points), offs, vA, vB, dc,
                        reinterpret_cast<int4*>(d_cell_info), d_cell_model,
-                       reinterpret_cast<float4*>(d_features));
+                       reinterpret_cast<float4*>(d_features), d_kept_index, d_kept_offsets);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;

@@ -272,13 +272,16 @@ def ground_planes_dev(ctx, points, count=None, radii_bins=1, theta_bins=18, min_
             "model": torch.empty((n_cells, 8), dtype=torch.float64, device=dev),
             "features": torch.empty((n_cells, max(num_features, 1), 4), dtype=torch.float32,
                                     device=dev),
+            "kept_offsets": torch.empty((n_cells + 1,), dtype=torch.int32, device=dev),
+            "kept_index": torch.empty((max(cap, 1),), dtype=torch.int32, device=dev),
         }
     cfg = GroundCfg(radii_bins=radii_bins, theta_bins=theta_bins, min_dist=min_dist,
                     max_dist=max_dist, retain_thresh=retain_thresh, num_features=num_features)
     cfg.origin[0], cfg.origin[1], cfg.origin[2] = origin
     check(ctx.lib.sloam_ground_planes(ctx.handle, _ptr(points), cap, _ptr(count), C.byref(cfg),
                                       _ptr(out["info"]), _ptr(out["model"]),
-                                      _ptr(out["features"])), ctx.handle)
+                                      _ptr(out["features"]), _ptr(out.get("kept_offsets")),
+                                      _ptr(out.get("kept_index"))), ctx.handle)
     return out
 
 

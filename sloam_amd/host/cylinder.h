@@ -1,0 +1,58 @@
+// Tree trunk landmark: the interface of sloam/include/objects/cylinder.h:14-40.
+// The axis is fitted on the GPU (sloam_fit_lines, PCL sample-consensus
+// semantics); computeModels fits all trees of a scan in one batch.
+#pragma once
+
+#include <vector>
+
+#include "definitions.h"
+#include "plane.h"
+#include "semanticObject.h"
+#include "sloam_gpu.h"
+#include "utils.h"
+
+struct CylinderParameters {
+  Vector3 root;  // a point on the axis
+  Vector3 ray;   // axis direction
+  std::vector<double> radii;
+  std::vector<TreeVertex> vertices;
+  double radius = 0.0;
+  double lambda = 0.0;
+};
+
+// Result of the GPU line fit of one tree (coefficients point + direction).
+struct LineFit {
+  bool ok = false;
+  float coeff[6] = {0, 0, 0, 0, 0, 0};
+  int inliers = 0;
+};
+
+class Cylinder : public SemanticObject<CylinderParameters> {
+ public:
+  explicit Cylinder(const std::vector<TreeVertex> vertices, const Plane& gplane,
+                    const FeatureModelParams& fmParams);
+  // Construction from a batched GPU line fit (`fit` of vertices' medians);
+  // `fit` is ignored when the tree is out of range or too short.
+  Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
+           const FeatureModelParams& fmParams, const LineFit& fit);
+  Scalar distance(const CylinderParameters& model) const override;
+  Scalar distance(const PointT& point) const override;
+  void project(const SE3& tf) override;
+  PointT asPoint(const Scalar height);
+
+  // cylinder.cpp:3-13: vertex [1] within maxLidarDist (2-D) of the plane centroid
+  static bool withinRange(const std::vector<TreeVertex>& vertices, const Plane& gplane,
+                          const FeatureModelParams& fmParams);
+  // cylinder.cpp:101-113: the line fit is needed (tree tall enough, has vertices)
+  static bool needsLineFit(const std::vector<TreeVertex>& vertices);
+  // the sample-consensus line settings of computeModel (cylinder.cpp:116-124)
+  static sloam_line_fit_cfg lineFitCfg();
+
+ private:
+  void build(const std::vector<TreeVertex>& vertices, const Plane& gplane,
+             const FeatureModelParams& fmParams, const LineFit* fit);
+  bool filter(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta, const Plane& gplane);
+  void computeModel(const std::vector<TreeVertex>& landmarkVtxs, const Scalar defaultTreeRadius,
+                    const Scalar featuresPerTree, const LineFit& fit);
+  bool groundBasedRoot(const Plane& gplane);
+};

@@ -1,0 +1,249 @@
+// Host-side bridge from the drop-in classes to the C ABI.
+#include "gpu_bridge.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+namespace sloam_gpu {
+namespace {
+
+void fail(const char* what, int st, sloam_ctx* ctx) {
+  std::string msg = std::string(what) + ": " + sloam_strerror(st);
+  if (ctx) {
+    const char* d = sloam_ctx_last_error(ctx);
+    if (d && *d) msg += std::string(" (") + d + ")";
+  }
+  throw std::runtime_error(msg);
+}
+
+void check(int st, const char* what) {
+  if (st != SLOAM_OK) fail(what, st, context());
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct CtxHolder {
+  sloam_ctx* ctx = nullptr;
+  ~CtxHolder() {
+    if (ctx) sloam_ctx_destroy(ctx);
+  }
+};
+
+// RAII device buffer
+struct Dev {
+  void* p = nullptr;
+  explicit Dev(size_t bytes) {
+    if (bytes) hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  }
+  ~Dev() {
+    if (p) (void)hipFree(p);
+  }
+  Dev(const Dev&) = delete;
+  Dev& operator=(const Dev&) = delete;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <typename T>
+void up(Dev& d, const std::vector<T>& v) {
+  if (!v.empty())
+    hip_check(hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
+                             (hipStream_t)sloam_ctx_stream(context())),
+              "hipMemcpyAsync(H2D)");
+}
+
+template <typename T>
+void down(std::vector<T>& v, const Dev& d) {
+  if (!v.empty())
+    hip_check(hipMemcpyAsync(v.data(), d.p, sizeof(T) * v.size(), hipMemcpyDeviceToHost,
+                             (hipStream_t)sloam_ctx_stream(context())),
+              "hipMemcpyAsync(D2H)");
+}
+
+}  // namespace
+
+sloam_ctx* context() {
+  thread_local CtxHolder holder;
+  if (!holder.ctx) {
+    int dev = 0;
+    if (const char* e = std::getenv("SLOAM_DEVICE")) dev = std::atoi(e);
+    const int st = sloam_ctx_create(dev, &holder.ctx);
+    if (st != SLOAM_OK) fail("sloam_ctx_create", st, nullptr);
+  }
+  return holder.ctx;
+}
+
+std::vector<CellFit> groundPlanes(const VectorType& points, const sloam_ground_cfg& cfg) {
+  sloam_ctx* ctx = context();
+  const int n_cells = cfg.radii_bins * cfg.theta_bins;
+  const int nf = cfg.num_features;
+  std::vector<float> pts(4 * points.size());
+  for (size_t i = 0; i < points.size(); ++i) {
+    pts[4 * i] = points[i].x;
+    pts[4 * i + 1] = points[i].y;
+    pts[4 * i + 2] = points[i].z;
+    pts[4 * i + 3] = points[i].intensity;
+  }
+  Dev d_pts(sizeof(float) * pts.size());
+  Dev d_info(sizeof(int32_t) * 4 * n_cells);
+  Dev d_model(sizeof(double) * 8 * n_cells);
+  Dev d_feat(sizeof(float) * 4 * (size_t)n_cells * (size_t)(nf > 0 ? nf : 1));
+  up(d_pts, pts);
+  Dev d_ko(sizeof(int32_t) * (n_cells + 1));
+  Dev d_ki(sizeof(int32_t) * (points.empty() ? 1 : points.size()));
+  check(sloam_ground_planes(ctx, d_pts.as<float>(), (int32_t)points.size(), nullptr, &cfg,
+                            d_info.as<int32_t>(), d_model.as<double>(), d_feat.as<float>(),
+                            d_ko.as<int32_t>(), d_ki.as<int32_t>()),
+        "sloam_ground_planes");
+  std::vector<int32_t> ko(n_cells + 1), ki(points.size());
+  down(ko, d_ko);
+  down(ki, d_ki);
+  std::vector<int32_t> info(4 * n_cells);
+  std::vector<double> model(8 * n_cells);
+  std::vector<float> feat(4 * (size_t)n_cells * (size_t)(nf > 0 ? nf : 1));
+  down(info, d_info);
+  down(model, d_model);
+  down(feat, d_feat);
+  check(sloam_sync(ctx), "sloam_sync");
+  std::vector<CellFit> out(n_cells);
+  for (int c = 0; c < n_cells; ++c) {
+    CellFit& f = out[c];
+    f.valid = info[4 * c] != 0;
+    f.kept = info[4 * c + 1];
+    for (int i = 0; i < 4; ++i) f.plane[i] = model[8 * c + i];
+    for (int i = 0; i < 3; ++i) f.centroid[i] = model[8 * c + 4 + i];
+    if (f.valid)
+      for (int j = 0; j < nf; ++j) {
+        const float* q = &feat[4 * ((size_t)c * nf + j)];
+        f.features.push_back(PointT{q[0], q[1], q[2], q[3]});
+      }
+    if (!points.empty())
+      for (int j = ko[c]; j < ko[c + 1] && j - ko[c] < f.kept; ++j) f.kept_index.push_back(ki[j]);
+  }
+  return out;
+}
+
+std::vector<LineResult> fitLines(const std::vector<VectorType>& sets,
+                                 const sloam_line_fit_cfg& cfg) {
+  std::vector<LineResult> out(sets.size());
+  if (sets.empty()) return out;
+  sloam_ctx* ctx = context();
+  std::vector<float> xyz;
+  std::vector<int32_t> off(1, 0);
+  for (const auto& s : sets) {
+    for (const auto& p : s) {
+      xyz.push_back(p.x);
+      xyz.push_back(p.y);
+      xyz.push_back(p.z);
+    }
+    off.push_back((int32_t)(xyz.size() / 3));
+  }
+  const int S = (int)sets.size();
+  Dev d_xyz(sizeof(float) * (xyz.empty() ? 3 : xyz.size()));
+  Dev d_off(sizeof(int32_t) * off.size());
+  Dev d_coeff(sizeof(float) * 6 * S);
+  Dev d_stats(sizeof(int32_t) * 4 * S);
+  up(d_xyz, xyz);
+  up(d_off, off);
+  check(sloam_fit_lines(ctx, d_xyz.as<float>(), d_off.as<int32_t>(), S, &cfg, d_coeff.as<float>(),
+                        d_stats.as<int32_t>(), nullptr),
+        "sloam_fit_lines");
+  std::vector<float> coeff(6 * S);
+  std::vector<int32_t> stats(4 * S);
+  down(coeff, d_coeff);
+  down(stats, d_stats);
+  check(sloam_sync(ctx), "sloam_sync");
+  for (int s = 0; s < S; ++s) {
+    out[s].ok = stats[4 * s] != 0;
+    out[s].inliers = stats[4 * s + 1];
+    for (int i = 0; i < 6; ++i) out[s].coeff[i] = coeff[6 * s + i];
+  }
+  return out;
+}
+
+std::vector<PoseResult> solvePoses(const std::vector<PoseProblem>& problems,
+                                   const sloam_pose_cfg& cfg) {
+  std::vector<PoseResult> out(problems.size());
+  if (problems.empty()) return out;
+  sloam_ctx* ctx = context();
+  std::vector<float> pts;
+  std::vector<double> cyl, pl, pose;
+  std::vector<int32_t> res_off(1, 0), cyl_off(1, 0), pl_off(1, 0), masks;
+  for (const auto& pr : problems) {
+    // landmark ids are global: shift this problem's local ids
+    const int32_t cb = cyl_off.back(), pb = pl_off.back();
+    for (size_t i = 0; i < pr.points.size(); i += 4) {
+      pts.push_back(pr.points[i]);
+      pts.push_back(pr.points[i + 1]);
+      pts.push_back(pr.points[i + 2]);
+      int32_t id;
+      std::memcpy(&id, &pr.points[i + 3], 4);
+      id = id >= 0 ? id + cb : id - pb;
+      float f;
+      std::memcpy(&f, &id, 4);
+      pts.push_back(f);
+    }
+    cyl.insert(cyl.end(), pr.cyl.begin(), pr.cyl.end());
+    pl.insert(pl.end(), pr.planes.begin(), pr.planes.end());
+    res_off.push_back((int32_t)(pts.size() / 4));
+    cyl_off.push_back((int32_t)(cyl.size() / 8));
+    pl_off.push_back((int32_t)(pl.size() / 4));
+    pose.insert(pose.end(), pr.pose, pr.pose + 7);
+    masks.push_back(pr.free_mask);
+  }
+  const int P = (int)problems.size();
+  sloam_pose_plan* plan = nullptr;
+  check(sloam_pose_plan_create(ctx, res_off.data(), P, cyl_off.back(), pl_off.back(), &cfg, &plan),
+        "sloam_pose_plan_create");
+  std::unique_ptr<sloam_pose_plan, void (*)(sloam_pose_plan*)> guard(plan, sloam_pose_plan_destroy);
+  Dev d_pts(sizeof(float) * (pts.empty() ? 4 : pts.size()));
+  Dev d_cyl(sizeof(double) * (cyl.empty() ? 8 : cyl.size()));
+  Dev d_pl(sizeof(double) * (pl.empty() ? 4 : pl.size()));
+  Dev d_co(sizeof(int32_t) * cyl_off.size());
+  Dev d_po(sizeof(int32_t) * pl_off.size());
+  Dev d_pose(sizeof(double) * pose.size());
+  Dev d_mask(sizeof(int32_t) * masks.size());
+  Dev d_out(sizeof(double) * 7 * P);
+  Dev d_par(sizeof(double) * 6 * P);
+  Dev d_st(sizeof(int32_t) * 2 * P);
+  Dev d_cost(sizeof(double) * 2 * P);
+  up(d_pts, pts);
+  up(d_cyl, cyl);
+  up(d_pl, pl);
+  up(d_co, cyl_off);
+  up(d_po, pl_off);
+  up(d_pose, pose);
+  up(d_mask, masks);
+  check(sloam_pose_plan_exec(plan, d_pts.as<float>(), d_cyl.as<double>(), d_co.as<int32_t>(),
+                             d_pl.as<double>(), d_po.as<int32_t>(), d_pose.as<double>(),
+                             d_mask.as<int32_t>(), d_out.as<double>(), d_par.as<double>(),
+                             d_st.as<int32_t>(), d_cost.as<double>()),
+        "sloam_pose_plan_exec");
+  std::vector<double> o(7 * P), par(6 * P), cost(2 * P);
+  std::vector<int32_t> st(2 * P);
+  down(o, d_out);
+  down(par, d_par);
+  down(st, d_st);
+  down(cost, d_cost);
+  check(sloam_sync(ctx), "sloam_sync");
+  for (int k = 0; k < P; ++k) {
+    std::memcpy(out[k].pose, &o[7 * k], 7 * sizeof(double));
+    std::memcpy(out[k].params, &par[6 * k], 6 * sizeof(double));
+    out[k].termination = st[2 * k];
+    out[k].iterations = st[2 * k + 1];
+    out[k].cost[0] = cost[2 * k];
+    out[k].cost[1] = cost[2 * k + 1];
+  }
+  return out;
+}
+
+}  // namespace sloam_gpu

@@ -1,0 +1,426 @@
+// Per-keyframe SLOAM estimator (sloam/src/core/sloam.cpp) over the GPU
+// library: every ground cell of a scan is fitted in one sloam_ground_planes
+// call, every tree axis in one sloam_fit_lines call, and the pose solves
+// (joint, or the two steps together) in one sloam_pose_plan batch.
+#include "sloam.h"
+
+#include <cmath>
+#include <cstring>
+
+#include "gpu_bridge.h"
+
+namespace sloam {
+
+namespace {
+
+// Ceres settings of the reference's solves (sloam.cpp:88-97, 219-226):
+// LM, HuberLoss(0.1), 50 iterations, Ceres' default tolerances.
+sloam_pose_cfg reference_pose_cfg(int parametrization) {
+  sloam_pose_cfg c{};
+  c.max_iters = 50;
+  c.stop_on_convergence = 1;
+  c.huber_delta = 0.1;
+  c.function_tolerance = 1e-6;
+  c.gradient_tolerance = 1e-10;
+  c.parameter_tolerance = 1e-8;
+  c.initial_radius = 1e4;
+  c.min_relative_decrease = 1e-3;
+  c.parametrization = parametrization;
+  return c;
+}
+
+constexpr int32_t kXYYawFree = (1 << 0) | (1 << 1) | (1 << 5);       // x, y, w_z
+constexpr int32_t kZRollPitchFree = (1 << 2) | (1 << 3) | (1 << 4);  // z, w_x, w_y
+
+void push_point(sloam_gpu::PoseProblem& pr, const Vector3& f, int32_t id) {
+  pr.points.push_back((float)f[0]);
+  pr.points.push_back((float)f[1]);
+  pr.points.push_back((float)f[2]);
+  float bits;
+  std::memcpy(&bits, &id, 4);
+  pr.points.push_back(bits);
+}
+
+bool same_cylinder(const double* a, const CylinderParameters& m) {
+  return a[0] == m.root[0] && a[1] == m.root[1] && a[2] == m.root[2] && a[3] == m.ray[0] &&
+         a[4] == m.ray[1] && a[5] == m.ray[2] && a[6] == m.radius;
+}
+
+bool same_plane(const double* a, const PlaneParameters& m) {
+  return a[0] == m.plane[0] && a[1] == m.plane[1] && a[2] == m.plane[2] && a[3] == m.plane[3];
+}
+
+// Each ObjectMatch carries its own copy of the landmark; consecutive matches
+// against the same landmark share one table entry (the residuals are the
+// same as with one entry per match).
+void add_tree_matches(const std::vector<ObjectMatch<Cylinder>>& matches,
+                      sloam_gpu::PoseProblem& pr) {
+  for (const auto& m : matches) {
+    const CylinderParameters& o = m.object.model;
+    const size_t n = pr.cyl.size() / 8;
+    if (n == 0 || !same_cylinder(&pr.cyl[8 * (n - 1)], o)) {
+      const double row[8] = {o.root[0], o.root[1], o.root[2], o.ray[0],
+                             o.ray[1],  o.ray[2],  o.radius,  0.0};
+      pr.cyl.insert(pr.cyl.end(), row, row + 8);
+    }
+    push_point(pr, m.feature, (int32_t)(pr.cyl.size() / 8 - 1));
+  }
+}
+
+void add_ground_matches(const std::vector<ObjectMatch<Plane>>& matches,
+                        sloam_gpu::PoseProblem& pr) {
+  for (const auto& m : matches) {
+    const PlaneParameters& o = m.object.model;
+    const size_t n = pr.planes.size() / 4;
+    if (n == 0 || !same_plane(&pr.planes[4 * (n - 1)], o))
+      for (int i = 0; i < 4; ++i) pr.planes.push_back(o.plane[i]);
+    push_point(pr, m.feature, -(int32_t)(pr.planes.size() / 4));
+  }
+}
+
+void set_initial_pose(const SE3& pose, sloam_gpu::PoseProblem& pr) {
+  const Vector3& t = pose.translation();
+  const Quat& q = pose.unit_quaternion();
+  const double x[7] = {t[0], t[1], t[2], q.x, q.y, q.z, q.w};
+  std::memcpy(pr.pose, x, sizeof(x));
+}
+
+// [t, angle-axis] of a pose, as the reference forms its 6-vector
+// (sloam.cpp:59-64)
+void pose_params(const SE3& pose, double params[6]) {
+  const Vector3& t = pose.translation();
+  const Quat& q = pose.unit_quaternion();
+  const double quat[4] = {q.w, q.x, q.y, q.z};
+  params[0] = t[0];
+  params[1] = t[1];
+  params[2] = t[2];
+  quaternionToAngleAxis(quat, params + 3);
+}
+
+// Runs the XY-yaw (trees) and Z-roll-pitch (ground) solves of a pose as one
+// batch; a step that is not run or does not converge keeps the initial
+// values (sloam.cpp:99-110, 151-161).
+void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround,
+              const std::vector<ObjectMatch<Cylinder>>& allTMatch,
+              const std::vector<ObjectMatch<Plane>>& allGMatch, double treeOut[3],
+              double groundOut[3]) {
+  double p0[6];
+  pose_params(poseEstimate, p0);
+  treeOut[0] = p0[0];
+  treeOut[1] = p0[1];
+  treeOut[2] = p0[5];
+  groundOut[0] = p0[2];
+  groundOut[1] = p0[3];
+  groundOut[2] = p0[4];
+  std::vector<sloam_gpu::PoseProblem> probs;
+  int tree_k = -1, ground_k = -1;
+  // a solve without residuals has nothing to move (Ceres converges at once)
+  if (optimTrees && !allTMatch.empty()) {
+    sloam_gpu::PoseProblem pr;
+    set_initial_pose(poseEstimate, pr);
+    pr.free_mask = kXYYawFree;
+    add_tree_matches(allTMatch, pr);
+    tree_k = (int)probs.size();
+    probs.push_back(std::move(pr));
+  }
+  if (optimGround && !allGMatch.empty()) {
+    sloam_gpu::PoseProblem pr;
+    set_initial_pose(poseEstimate, pr);
+    pr.free_mask = kZRollPitchFree;
+    add_ground_matches(allGMatch, pr);
+    ground_k = (int)probs.size();
+    probs.push_back(std::move(pr));
+  }
+  if (probs.empty()) return;
+  const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(1));
+  if (tree_k >= 0 && res[tree_k].termination == 0) {
+    treeOut[0] = res[tree_k].params[0];
+    treeOut[1] = res[tree_k].params[1];
+    treeOut[2] = res[tree_k].params[5];
+  }
+  if (ground_k >= 0 && res[ground_k].termination == 0) {
+    groundOut[0] = res[ground_k].params[2];
+    groundOut[1] = res[ground_k].params[3];
+    groundOut[2] = res[ground_k].params[4];
+  }
+}
+
+}  // namespace
+
+sloam::sloam() {
+  T_Map_Anchor_ = SE3();
+  firstScan_ = true;
+  minPlanes_ = (fmParams_.groundRadiiBins * fmParams_.groundThetaBins) * 0.1;
+}
+
+std::vector<Plane> sloam::getPrevGroundModel() { return prevGPlanes_; }
+
+CloudT sloam::getPrevGroundFeatures() {
+  CloudT pc;
+  for (const auto& ground : prevGPlanes_)
+    for (const auto& pt : ground.features) pc.points.push_back(pt);
+  pc.height = 1;
+  pc.width = (uint32_t)pc.points.size();
+  return pc;
+}
+
+bool sloam::TwoStepOptimizePose(const SE3& poseEstimate, const bool optimTrees,
+                                const bool optimGround,
+                                const std::vector<ObjectMatch<Cylinder>>& allTMatch,
+                                const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
+  double treeOut[3], groundOut[3];
+  two_step(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, treeOut, groundOut);
+  // roll, pitch from the ground step, yaw from the tree step (sloam.cpp:41-51)
+  const double angleAxis[3] = {groundOut[1], groundOut[2], treeOut[2]};
+  double q[4];
+  angleAxisToQuaternion(angleAxis, q);
+  tf.setQuaternion(Quat(q[0], q[1], q[2], q[3]));
+  tf.translation()[0] = treeOut[0];
+  tf.translation()[1] = treeOut[1];
+  tf.translation()[2] = groundOut[0];
+  return true;
+}
+
+void sloam::OptimizeXYYaw(const SE3& poseEstimate, const bool optimize,
+                          const std::vector<ObjectMatch<Cylinder>>& allTMatch, double* out) {
+  double ground[3];
+  two_step(poseEstimate, optimize, false, allTMatch, {}, out, ground);
+}
+
+void sloam::OptimizeZRollPitch(const SE3& poseEstimate, const bool optimize,
+                               const std::vector<ObjectMatch<Plane>>& allGMatch, double* out) {
+  double tree[3];
+  two_step(poseEstimate, false, optimize, {}, allGMatch, tree, out);
+}
+
+bool sloam::OptimizePose(const SE3& poseEstimate,
+                         const std::vector<ObjectMatch<Cylinder>>& allTMatch,
+                         const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
+  std::vector<sloam_gpu::PoseProblem> probs(1);
+  set_initial_pose(poseEstimate, probs[0]);
+  add_tree_matches(allTMatch, probs[0]);
+  add_ground_matches(allGMatch, probs[0]);
+  bool success;
+  double x[7];
+  if (probs[0].points.empty()) {
+    // no residuals: the solve converges at the initial pose
+    success = true;
+    std::memcpy(x, probs[0].pose, sizeof(x));
+  } else {
+    const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(0));
+    success = res[0].termination == 0;
+    std::memcpy(x, res[0].pose, sizeof(x));
+  }
+  // the output is written only on convergence (sloam.cpp:238-244)
+  if (success) {
+    tf.setQuaternion(Quat(x[6], x[3], x[4], x[5]));
+    tf.translation() = Vector3(x[0], x[1], x[2]);
+  }
+  return success;
+}
+
+template <typename T>
+std::vector<ObjectMatch<T>> sloam::matchFeatures(const SE3 tf, const std::vector<T>& currObjects,
+                                                 const std::vector<T>& mapObjects,
+                                                 const Scalar distThresh) {
+  std::vector<ObjectMatch<T>> matches;
+  if (mapObjects.empty()) return matches;  // the reference reads mapObjects[0]
+  for (const auto& co : currObjects) {
+    T proj_obj = co;
+    proj_obj.project(tf);
+    // closest map model; ties keep the first (strict <)
+    Scalar bestDist = distThresh + 100;
+    size_t best = 0;
+    for (size_t i = 0; i < mapObjects.size(); ++i) {
+      const Scalar d = mapObjects[i].distance(proj_obj.getModel());
+      if (d < bestDist) {
+        bestDist = d;
+        best = i;
+      }
+    }
+    if (bestDist < distThresh) addFeatureMatches(co.features, mapObjects[best], bestDist, matches);
+  }
+  return matches;
+}
+
+template <typename T>
+void sloam::addFeatureMatches(const VectorType& features, const T& object, const double dist,
+                              std::vector<ObjectMatch<T>>& matches) {
+  for (const auto& feature : features) matches.push_back(ObjectMatch<T>(feature, object, dist));
+}
+
+template <typename T>
+void sloam::matchModels(const std::vector<T>& currObjects, const std::vector<T>& mapObjects,
+                        std::vector<int>& matchIndices) {
+  if (mapObjects.empty()) return;
+  size_t counter = 0;
+  for (const auto& co : currObjects) {
+    Scalar bestDist = fmParams_.treeMatchThresh + 100;
+    size_t bestKey = 0;
+    for (size_t key = 0; key < mapObjects.size(); ++key) {
+      const Scalar d = mapObjects[key].distance(co.getModel());
+      if (d < bestDist) {
+        bestDist = d;
+        bestKey = key;
+      }
+    }
+    if (bestDist < fmParams_.AddNewTreeThreshDist) matchIndices[counter] = (int)bestKey;
+    counter++;
+  }
+}
+
+template std::vector<ObjectMatch<Cylinder>> sloam::matchFeatures<Cylinder>(
+    const SE3, const std::vector<Cylinder>&, const std::vector<Cylinder>&, const Scalar);
+template std::vector<ObjectMatch<Plane>> sloam::matchFeatures<Plane>(
+    const SE3, const std::vector<Plane>&, const std::vector<Plane>&, const Scalar);
+template void sloam::matchModels<Cylinder>(const std::vector<Cylinder>&,
+                                           const std::vector<Cylinder>&, std::vector<int>&);
+
+sloam_ground_cfg sloam::groundCfg(const SE3& pose) const {
+  sloam_ground_cfg cfg{};
+  cfg.radii_bins = fmParams_.groundRadiiBins;
+  cfg.theta_bins = fmParams_.groundThetaBins;
+  cfg.min_dist = fmParams_.minGroundLidarDist;
+  cfg.max_dist = fmParams_.maxGroundLidarDist;
+  cfg.retain_thresh = fmParams_.groundRetainThresh;
+  cfg.num_features = fmParams_.numGroundFeatures;
+  // the binning origin is a PointT (sloam.cpp:333-336)
+  cfg.origin[0] = (float)pose.translation()[0];
+  cfg.origin[1] = (float)pose.translation()[1];
+  cfg.origin[2] = (float)pose.translation()[2];
+  return cfg;
+}
+
+void sloam::binGroundPoints(const SE3 pose, const VectorType& points, GroundGrid& scgf) {
+  const auto cells = sloam_gpu::groundPlanes(points, groundCfg(pose));
+  scgf.assign(fmParams_.groundRadiiBins, std::vector<VectorType>(fmParams_.groundThetaBins));
+  for (int r = 0; r < fmParams_.groundRadiiBins; ++r)
+    for (int t = 0; t < fmParams_.groundThetaBins; ++t)
+      for (const int32_t i : cells[(size_t)r * fmParams_.groundThetaBins + t].kept_index)
+        scgf[r][t].push_back(points[i]);
+}
+
+void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
+                          std::vector<Plane>& planes) {
+  // ground: all cells of the scan in one call (binned at the origin,
+  // sloam.cpp:390-393)
+  const auto cells = sloam_gpu::groundPlanes(in.groundCloud->points, groundCfg(SE3()));
+  const Vector3 src_vec(0, 0, 1);
+  const Matrix4 tfm = in.poseEstimate.inverse().matrix().transpose();
+  for (const auto& cell : cells) {
+    PlaneParameters fitted;
+    for (int i = 0; i < 4; ++i) fitted.plane(i) = cell.plane[i];
+    for (int i = 0; i < 3; ++i) fitted.centroid(i) = cell.centroid[i];
+    Plane ground(fitted, cell.features, cell.valid);
+    if (!ground.isValid) continue;
+    // roughly horizontal in the map frame (sloam.cpp:401-406)
+    const Vector4 normal = tfm * ground.model.plane;
+    const Quat q_rot = Quat::FromTwoVectors(src_vec, normal.head3());
+    double R[3][3];
+    q_rot.toRotationMatrix(R);
+    const Vector3 angles = eulerAngles012(R);
+    const bool angleCheck =
+        (angles[0] < 0.1 && angles[1] < 0.1 && angles[2] < 0.1) ||
+        (M_PI - std::fabs(angles[0]) < 0.1 && M_PI - std::fabs(angles[1]) < 0.1 &&
+         M_PI - std::fabs(angles[2]) < 0.1);
+    // under the robot (sloam.cpp:408)
+    const bool heightCheck =
+        (in.poseEstimate * ground.model.centroid)[2] < in.poseEstimate.translation()[2];
+    if (angleCheck && heightCheck) planes.push_back(ground);
+  }
+  if (planes.empty()) return;
+
+  // trees: nearest plane to vertex [1] of each, then one batched line fit
+  // for the trees in range and tall enough (sloam.cpp:416-435)
+  std::vector<size_t> best(in.landmarks.size(), 0);
+  std::vector<VectorType> sets;
+  std::vector<int> set_of(in.landmarks.size(), -1);
+  for (size_t k = 0; k < in.landmarks.size(); ++k) {
+    const auto& t = in.landmarks[k];
+    if (t.size() < 3) continue;  // the reference indexes vertices [1] and [2]
+    Scalar bestDist = 100000;
+    for (size_t i = 0; i < planes.size(); ++i) {
+      const Scalar d = planes[i].distance(t[1].coords);
+      if (d < bestDist) {
+        bestDist = d;
+        best[k] = i;
+      }
+    }
+    if (Cylinder::withinRange(t, planes[best[k]], fmParams_) && Cylinder::needsLineFit(t)) {
+      VectorType medians;
+      for (const auto& v : t) medians.push_back(v.coords);
+      set_of[k] = (int)sets.size();
+      sets.push_back(std::move(medians));
+    }
+  }
+  const auto fits = sloam_gpu::fitLines(sets, Cylinder::lineFitCfg());
+  for (size_t k = 0; k < in.landmarks.size(); ++k) {
+    if (in.landmarks[k].size() < 3) continue;
+    LineFit fit;
+    if (set_of[k] >= 0) {
+      const auto& r = fits[set_of[k]];
+      fit.ok = r.ok;
+      fit.inliers = r.inliers;
+      std::memcpy(fit.coeff, r.coeff, sizeof(fit.coeff));
+    }
+    Cylinder c(in.landmarks[k], planes[best[k]], fmParams_, fit);
+    if (c.isValid) landmarks.push_back(c);
+  }
+}
+
+void sloam::projectModels(const SE3& tf, std::vector<Cylinder>& landmarks,
+                          std::vector<Plane>& planes) {
+  for (auto& p : planes) p.project(tf);
+  for (auto& l : landmarks) l.project(tf);
+}
+
+bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
+  std::vector<Plane> planes;
+  std::vector<Cylinder> landmarks;
+  computeModels(in, landmarks, planes);
+
+  std::vector<int> matchIndices(landmarks.size(), -1);
+  bool success = true;
+  if (firstScan_) {
+    // the first scan's pose is the odometry's; its models start the map
+    projectModels(in.poseEstimate, landmarks, planes);
+    out.T_Map_Curr = in.poseEstimate;
+    out.matches = matchIndices;
+    out.tm = landmarks;
+    prevGPlanes_ = planes;
+    firstScan_ = false;
+    return success;
+  }
+  if (in.mapModels.empty()) return false;                // empty map
+  if (planes.empty() || landmarks.empty()) return false;  // no models in this scan
+
+  const auto treeMatches =
+      matchFeatures(in.poseEstimate, landmarks, in.mapModels, fmParams_.treeMatchThresh);
+  // the ground association threshold is fixed at 1.0 (sloam.cpp:482)
+  const auto planeMatches = matchFeatures(in.poseEstimate, planes, prevGPlanes_, 1.0);
+
+  SE3 T_Delta = SE3();
+  SE3 currPose = in.poseEstimate;
+  const bool treeCheck = landmarks.size() > fmParams_.minTreeModels &&
+                         treeMatches.size() > 5 * fmParams_.featuresPerTree;
+  const bool groundCheck =
+      planeMatches.size() > fmParams_.minGroundModels && planes.size() > minPlanes_;
+  if (fmParams_.twoStepOptim) {
+    success = TwoStepOptimizePose(in.poseEstimate, treeCheck, groundCheck, treeMatches,
+                                  planeMatches, currPose);
+  } else if (treeCheck && groundCheck) {
+    success = OptimizePose(in.poseEstimate, treeMatches, planeMatches, T_Delta);
+    if (success) currPose = T_Delta;
+  }
+
+  projectModels(currPose, landmarks, planes);
+  matchModels(landmarks, in.mapModels, matchIndices);
+  prevGPlanes_ = planes;
+  out.matches = matchIndices;
+  out.T_Map_Curr = currPose;
+  out.T_Delta = T_Delta;
+  out.tm = landmarks;
+  return success;
+}
+
+}  // namespace sloam

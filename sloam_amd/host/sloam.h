@@ -1,0 +1,97 @@
+// Per-keyframe SLOAM estimator: the interface of sloam/include/core/sloam.h.
+// computeModels fits every ground cell and every tree of a scan in batched GPU
+// calls; the pose solves run on the GPU LM kernels.
+#pragma once
+
+#include <vector>
+
+#include "cylinder.h"
+#include "definitions.h"
+#include "plane.h"
+#include "sloam_gpu.h"
+#include "utils.h"
+
+template <typename T>
+struct ObjectMatch {
+  ObjectMatch(PointT ft, T obj, Scalar d) : feature(ft.x, ft.y, ft.z), object(obj), dist(d) {}
+  Vector3 feature;
+  T object;
+  double dist;
+};
+
+struct SloamInput {
+  SloamInput() : groundCloud(new CloudT()) {}
+  SE3 poseEstimate;
+  Scalar distance = 0.0;
+  CloudT::Ptr groundCloud;
+  std::vector<Cylinder> mapModels;
+  std::vector<std::vector<TreeVertex>> landmarks;
+};
+
+struct SloamOutput {
+  SloamOutput() {}
+  std::vector<int> matches;
+  std::vector<Cylinder> tm;
+  SE3 T_Map_Curr;
+  SE3 T_Delta;
+};
+
+// ground cells [radii bin][theta bin] (the reference's boost::multi_array)
+using GroundGrid = std::vector<std::vector<VectorType>>;
+
+namespace sloam {
+
+class sloam {
+ public:
+  explicit sloam();
+
+  const FeatureModelParams& fmParams() const { return fmParams_; }
+  void setFmParams(const FeatureModelParams& fmParams) {
+    fmParams_ = fmParams;
+    minPlanes_ = (fmParams_.groundRadiiBins * fmParams_.groundThetaBins) * 0.1;
+  }
+  bool RunSloam(SloamInput& in, SloamOutput& out);
+
+  // Pose optimization
+  void OptimizeXYYaw(const SE3& poseEstimate, const bool optimize,
+                     const std::vector<ObjectMatch<Cylinder>>& allTMatch, double* out);
+  void OptimizeZRollPitch(const SE3& poseEstimate, const bool optimize,
+                          const std::vector<ObjectMatch<Plane>>& allGMatch, double* out);
+  bool TwoStepOptimizePose(const SE3& poseEstimate, const bool optimTrees, const bool optimGround,
+                           const std::vector<ObjectMatch<Cylinder>>& allTMatch,
+                           const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf);
+  bool OptimizePose(const SE3& poseEstimate, const std::vector<ObjectMatch<Cylinder>>& allTMatch,
+                    const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf);
+  // Model estimation
+  void projectModels(const SE3& tf, std::vector<Cylinder>& landmarks, std::vector<Plane>& planes);
+  void computeModels(SloamInput& in, std::vector<Cylinder>& landmarks, std::vector<Plane>& planes);
+  void binGroundPoints(const SE3 pose, const VectorType& points, GroundGrid& scgf);
+  // Data association
+  template <typename T>
+  void matchModels(const std::vector<T>& currObjects, const std::vector<T>& mapObjects,
+                   std::vector<int>& matchIndices);
+  template <typename T>
+  std::vector<ObjectMatch<T>> matchFeatures(const SE3 tf, const std::vector<T>& currObjects,
+                                            const std::vector<T>& mapObjects,
+                                            const Scalar distThresh);
+  template <typename T>
+  void addFeatureMatches(const VectorType& features, const T& object, const double dist,
+                         std::vector<ObjectMatch<T>>& matches);
+
+  std::vector<Plane> getPrevGroundModel();
+  CloudT getPrevGroundFeatures();
+
+ private:
+  // ground binning / fit settings of fmParams_, binned around `pose`
+  sloam_ground_cfg groundCfg(const SE3& pose) const;
+
+  SE3 T_Map_Anchor_;
+  std::vector<Plane> prevGPlanes_;
+  int numMapTrees_ = 0;
+  double totalDistance_ = 0.0;
+  FeatureModelParams fmParams_;
+  bool firstScan_ = true;
+  double minPlanes_ = 0.0;
+};
+
+}  // namespace sloam

@@ -1,0 +1,570 @@
+// Tests of the C++ drop-in classes (Plane, Cylinder, sloam::sloam), the
+// cases of the reference's gtests (sloam/src/tests/plane_test.cpp,
+// cylinder_test.cpp, core_test.cpp) over the same data, plus host-only checks
+// of the rotation helpers.  gtest is not part of this toolchain: a small
+// harness with the same EXPECT_* names.
+//
+//   dropin_test <fixture dir> [host|gpu|dump <out file>]
+//
+// The fixture directory holds still_landmarks_t{0,1}.lm and
+// still_ground_t{0,1}.xyz, written from tests/golden/ by the pytest wrapper
+// (tests/test_dropin.py):
+//   .lm : int32 n_trees; per tree int32 n_vertices; per vertex int32 treeId,
+//         beam, prevVertexSize, isValid, float64 radius, float32 coords[3],
+//         int32 n_points, float32 points[3 n_points]
+//   .xyz: int32 n; float32 xyz[3 n]
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cylinder.h"
+#include "definitions.h"
+#include "plane.h"
+#include "sloam.h"
+
+namespace {
+
+int g_failures = 0;
+const char* g_test = "";
+
+#define EXPECT_TRUE(c)                                                                    \
+  do {                                                                                    \
+    if (!(c)) {                                                                           \
+      std::fprintf(stderr, "[%s] %s:%d: EXPECT_TRUE(%s) failed\n", g_test, __FILE__,      \
+                   __LINE__, #c);                                                         \
+      ++g_failures;                                                                       \
+    }                                                                                     \
+  } while (0)
+#define EXPECT_EQ(a, b)                                                                   \
+  do {                                                                                    \
+    const double va_ = (a), vb_ = (b);                                                    \
+    if (!(va_ == vb_)) {                                                                  \
+      std::fprintf(stderr, "[%s] %s:%d: EXPECT_EQ(%s, %s): %.17g vs %.17g\n", g_test,     \
+                   __FILE__, __LINE__, #a, #b, va_, vb_);                                 \
+      ++g_failures;                                                                       \
+    }                                                                                     \
+  } while (0)
+#define EXPECT_NEAR(a, b, tol)                                                            \
+  do {                                                                                    \
+    const double va_ = (a), vb_ = (b);                                                    \
+    if (!(std::fabs(va_ - vb_) <= (tol))) {                                               \
+      std::fprintf(stderr, "[%s] %s:%d: EXPECT_NEAR(%s, %s, %s): %.17g vs %.17g\n",       \
+                   g_test, __FILE__, __LINE__, #a, #b, #tol, va_, vb_);                   \
+      ++g_failures;                                                                       \
+    }                                                                                     \
+  } while (0)
+
+std::string g_dir;
+
+using Landmarks = std::vector<std::vector<TreeVertex>>;
+
+template <typename T>
+void rd(FILE* f, T* p, size_t n, const std::string& path) {
+  if (n && std::fread(p, sizeof(T), n, f) != n) throw std::runtime_error("short read: " + path);
+}
+
+Landmarks read_landmarks(const std::string& name) {
+  const std::string path = g_dir + "/" + name + ".lm";
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + path);
+  Landmarks out;
+  int32_t nt;
+  rd(f, &nt, 1, path);
+  out.resize(nt);
+  for (auto& tree : out) {
+    int32_t nv;
+    rd(f, &nv, 1, path);
+    tree.resize(nv);
+    for (auto& v : tree) {
+      int32_t vi[4];
+      rd(f, vi, 4, path);
+      v.treeId = vi[0];
+      v.beam = vi[1];
+      v.prevVertexSize = vi[2];
+      v.isValid = vi[3] != 0;
+      rd(f, &v.radius, 1, path);
+      float c[3];
+      rd(f, c, 3, path);
+      v.coords.x = c[0];
+      v.coords.y = c[1];
+      v.coords.z = c[2];
+      int32_t np;
+      rd(f, &np, 1, path);
+      std::vector<float> p(3 * (size_t)np);
+      rd(f, p.data(), p.size(), path);
+      for (int i = 0; i < np; ++i) v.points.push_back(PointT{p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.f});
+    }
+  }
+  std::fclose(f);
+  return out;
+}
+
+CloudT::Ptr read_cloud(const std::string& name) {
+  const std::string path = g_dir + "/" + name + ".xyz";
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + path);
+  int32_t n;
+  rd(f, &n, 1, path);
+  std::vector<float> p(3 * (size_t)n);
+  rd(f, p.data(), p.size(), path);
+  std::fclose(f);
+  CloudT::Ptr c(new CloudT);
+  for (int i = 0; i < n; ++i) c->push_back(PointT{p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.f});
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// host-only: rotation helpers against their defining identities
+// ---------------------------------------------------------------------------
+void rot_x(double a, double R[3][3]) {
+  const double c = std::cos(a), s = std::sin(a);
+  const double M[3][3] = {{1, 0, 0}, {0, c, -s}, {0, s, c}};
+  std::memcpy(R, M, sizeof(M));
+}
+void rot_y(double a, double R[3][3]) {
+  const double c = std::cos(a), s = std::sin(a);
+  const double M[3][3] = {{c, 0, s}, {0, 1, 0}, {-s, 0, c}};
+  std::memcpy(R, M, sizeof(M));
+}
+void rot_z(double a, double R[3][3]) {
+  const double c = std::cos(a), s = std::sin(a);
+  const double M[3][3] = {{c, -s, 0}, {s, c, 0}, {0, 0, 1}};
+  std::memcpy(R, M, sizeof(M));
+}
+void mul(const double A[3][3], const double B[3][3], double C[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
+}
+
+void test_euler_angles() {
+  std::mt19937 g(7);
+  std::uniform_real_distribution<double> ua(0.01, M_PI - 0.01), ub(-1.5, 1.5), uc(-3.1, 3.1);
+  for (int it = 0; it < 200; ++it) {
+    const double a = ua(g), b = ub(g), c = uc(g);
+    double X[3][3], Y[3][3], Z[3][3], XY[3][3], R[3][3];
+    rot_x(a, X);
+    rot_y(b, Y);
+    rot_z(c, Z);
+    mul(X, Y, XY);
+    mul(XY, Z, R);
+    const Vector3 e = eulerAngles012(R);
+    // the decomposition is unique for a in (0, pi), |b| < pi / 2
+    EXPECT_NEAR(e[0], a, 1e-9);
+    EXPECT_NEAR(e[1], b, 1e-9);
+    EXPECT_NEAR(e[2], c, 1e-9);
+  }
+  // identity: all zero (the first angle is folded to +0, not pi)
+  const double I[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const Vector3 e = eulerAngles012(I);
+  EXPECT_EQ(std::fabs(e[0]) + std::fabs(e[1]) + std::fabs(e[2]), 0.0);
+}
+
+void test_from_two_vectors() {
+  std::mt19937 g(8);
+  std::normal_distribution<double> n(0.0, 1.0);
+  for (int it = 0; it < 200; ++it) {
+    const Vector3 a(n(g), n(g), n(g)), b(n(g), n(g), n(g));
+    const Quat q = Quat::FromTwoVectors(a, b);
+    EXPECT_NEAR(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z, 1.0, 1e-12);
+    const Vector3 r = q * a.normalized();
+    const Vector3 bn = b.normalized();
+    for (int i = 0; i < 3; ++i) EXPECT_NEAR(r[i], bn[i], 1e-12);
+    // about the common normal: the axis is orthogonal to both
+    EXPECT_NEAR(q.x * a[0] + q.y * a[1] + q.z * a[2], 0.0, 1e-12 * a.norm());
+  }
+  // parallel and antiparallel
+  const Quat q0 = Quat::FromTwoVectors(Vector3(0, 0, 1), Vector3(0, 0, 2));
+  EXPECT_EQ(q0.w, 1.0);
+  const Quat q1 = Quat::FromTwoVectors(Vector3(0, 0, 1), Vector3(0, 0, -1));
+  const Vector3 r1 = q1 * Vector3(0, 0, 1);
+  EXPECT_NEAR(r1[2], -1.0, 1e-12);
+}
+
+void test_angle_axis() {
+  std::mt19937 g(9);
+  std::normal_distribution<double> n(0.0, 1.0);
+  std::uniform_real_distribution<double> angle(0.0, 3.1);
+  for (int it = 0; it < 200; ++it) {
+    // angles in the canonical range [0, pi) that the inverse returns
+    double w[3] = {n(g), n(g), n(g)};
+    const double th = angle(g);
+    const double s = th / std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (double& x : w) x *= s;
+    double q[4], back[3];
+    angleAxisToQuaternion(w, q);
+    quaternionToAngleAxis(q, back);
+    for (int i = 0; i < 3; ++i) EXPECT_NEAR(back[i], w[i], 1e-12);
+    // the quaternion of the rotation by th about w / th
+    EXPECT_NEAR(q[0], std::cos(th / 2), 1e-15);
+  }
+  const double z[3] = {0, 0, 0};
+  double q[4];
+  angleAxisToQuaternion(z, q);
+  EXPECT_EQ(q[0], 1.0);
+}
+
+void test_se3() {
+  const Quat q = Quat(0.9, 0.1, -0.3, 0.2).normalized();
+  const SE3 T(q, Vector3(1, 2, 3));
+  const SE3 I = T * T.inverse();
+  EXPECT_NEAR(I.translation().norm(), 0.0, 1e-15);
+  EXPECT_NEAR(std::fabs(I.unit_quaternion().w), 1.0, 1e-15);
+  const Matrix4 M = T.matrix(), Mi = M.inverse(), P = M * Mi;
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) EXPECT_NEAR(P(r, c), r == c ? 1.0 : 0.0, 1e-14);
+}
+
+// ---------------------------------------------------------------------------
+// plane_test.cpp
+// ---------------------------------------------------------------------------
+FeatureModelParams plane_params() {
+  FeatureModelParams p;
+  p.scansPerSweep = 1;
+  p.maxLidarDist = 15.0;
+  p.maxGroundLidarDist = 30.0;
+  p.minGroundLidarDist = 0.0;
+  p.groundRadiiBins = 1;
+  p.groundThetaBins = 18;
+  p.groundMatchThresh = 2.0;
+  p.groundRetainThresh = 0.1;
+  p.numGroundFeatures = 1;
+  p.maxTreeRadius = 0.3;
+  p.maxAxisTheta = 10;
+  p.maxFocusOutlierDistance = 0.5;
+  p.roughTreeMatchThresh = 3.0;
+  p.treeMatchThresh = 1.0;
+  p.AddNewTreeThreshDist = 2.0;
+  p.featuresPerTree = 2;
+  p.defaultTreeRadius = 0.1;
+  return p;
+}
+
+VectorType three_points() {
+  PointT a, b, c;
+  b.y = 1;
+  c.x = 1;
+  return {a, b, c};
+}
+
+void plane_initializes() {
+  Plane plane(three_points(), plane_params());
+  EXPECT_TRUE(plane.isValid);
+}
+
+void plane_distance_to_feature() {
+  const VectorType g = three_points();
+  Plane plane(g, plane_params());
+  const float d = plane.distance(g[0]);
+  EXPECT_NEAR(0.0, d, 0.1);
+}
+
+void plane_translate_model() {
+  Plane plane(three_points(), plane_params());
+  SE3 tf;
+  tf.translation()[2] = 1;
+  const Vector3 centroid = plane.model.centroid;
+  plane.project(tf);
+  EXPECT_EQ(centroid[2] + 1, plane.model.centroid[2]);
+}
+
+// ---------------------------------------------------------------------------
+// cylinder_test.cpp
+// ---------------------------------------------------------------------------
+FeatureModelParams cylinder_params() {
+  FeatureModelParams p = plane_params();
+  p.maxLidarDist = 30.0;
+  p.groundThetaBins = 1;
+  p.numGroundFeatures = 3;
+  return p;
+}
+
+std::vector<Cylinder> valid_cylinders() {
+  const FeatureModelParams params = cylinder_params();
+  VectorType g = three_points();
+  PointT d;
+  d.x = 1;
+  d.y = 1;
+  g.push_back(d);
+  const Plane plane(g, params);
+  EXPECT_TRUE(plane.isValid);
+  std::vector<Cylinder> out;
+  for (const auto& l : read_landmarks("still_landmarks_t0")) {
+    Cylinder c(l, plane, params);
+    if (c.isValid) out.push_back(c);
+  }
+  return out;
+}
+
+void cylinder_distance_to_model() {
+  auto cyl = valid_cylinders();
+  EXPECT_TRUE(!cyl.empty());
+  if (cyl.empty()) return;
+  const float d = (float)cyl[0].distance(cyl[0].model);
+  EXPECT_EQ(0.0, d);
+}
+
+void cylinder_distance_to_feature() {
+  auto cyl = valid_cylinders();
+  EXPECT_TRUE(!cyl.empty());
+  if (cyl.empty()) return;
+  const float d = (float)cyl[0].distance(cyl[0].features[0]);
+  EXPECT_NEAR(0.0, d, 0.1);
+}
+
+void cylinder_translate_model() {
+  auto cyl = valid_cylinders();
+  EXPECT_TRUE(!cyl.empty());
+  if (cyl.empty()) return;
+  SE3 tf;
+  tf.translation()[0] = 1;
+  const Vector3 root = cyl[0].model.root;
+  cyl[0].project(tf);
+  EXPECT_EQ(root[0] + 1, cyl[0].model.root[0]);
+}
+
+// ---------------------------------------------------------------------------
+// core_test.cpp
+// ---------------------------------------------------------------------------
+FeatureModelParams core_params() {
+  FeatureModelParams p;
+  p.scansPerSweep = 1;
+  p.maxLidarDist = 15.0;
+  p.maxGroundLidarDist = 30.0;
+  p.minGroundLidarDist = 0.0;
+  p.groundRadiiBins = 1;
+  p.groundThetaBins = 18;
+  p.groundMatchThresh = 2.0;
+  p.groundRetainThresh = 0.05;
+  p.maxTreeRadius = 0.3;
+  p.maxAxisTheta = 10;
+  p.maxFocusOutlierDistance = 0.5;
+  p.roughTreeMatchThresh = 3.0;
+  p.treeMatchThresh = 1.0;
+  p.AddNewTreeThreshDist = 2.0;
+  p.featuresPerTree = 2;
+  p.numGroundFeatures = 60;
+  p.defaultTreeRadius = 0.1;
+  return p;
+}
+
+SloamInput core_input(const char* stamp) {
+  SloamInput in;
+  in.landmarks = read_landmarks(std::string("still_landmarks_") + stamp);
+  // core_test.cpp:78-81: every vertex holds exactly 5 points (zero-padded)
+  for (auto& tree : in.landmarks)
+    for (auto& v : tree) v.points.resize(5);
+  in.groundCloud = read_cloud(std::string("still_ground_") + stamp);
+  in.poseEstimate = SE3();
+  return in;
+}
+
+void core_first_scan() {
+  sloam::sloam s;
+  SloamOutput out;
+  s.setFmParams(core_params());
+  SloamInput in = core_input("t0");
+  s.RunSloam(in, out);
+  EXPECT_TRUE(out.tm.size() > 0);
+  EXPECT_TRUE(s.getPrevGroundModel().size() > 0);
+}
+
+void core_second_scan() {
+  sloam::sloam s;
+  SloamOutput out;
+  s.setFmParams(core_params());
+  SloamInput in = core_input("t1");
+  s.RunSloam(in, out);
+  EXPECT_TRUE(out.tm.size() > 0);
+  EXPECT_TRUE(s.getPrevGroundModel().size() > 0);
+}
+
+void run_two_scans(sloam::sloam& s, SloamOutput& out, bool* success, bool twoStep) {
+  FeatureModelParams p = core_params();
+  p.twoStepOptim = twoStep;
+  s.setFmParams(p);
+  SloamInput t0 = core_input("t0"), t1 = core_input("t1");
+  s.RunSloam(t0, out);
+  t1.mapModels = out.tm;
+  *success = s.RunSloam(t1, out);
+}
+
+void core_sloam_success() {
+  for (bool twoStep : {true, false}) {
+    sloam::sloam s;
+    SloamOutput out;
+    bool success = false;
+    run_two_scans(s, out, &success, twoStep);
+    EXPECT_TRUE(success);
+  }
+}
+
+void core_pose_optimization() {
+  // the two-step solve of sloam.yaml (twoStepOptim: true); the joint solve
+  // is checked against the oracle on the same matches by the dump
+  sloam::sloam s;
+  SloamOutput out;
+  bool success = false;
+  run_two_scans(s, out, &success, true);
+  const float translation = (float)out.T_Delta.translation().norm();
+  EXPECT_NEAR(0.0, translation, 0.1);
+  // the still sequence: the estimated pose stays near the origin
+  EXPECT_NEAR(0.0, out.T_Map_Curr.translation().norm(), 0.1);
+}
+
+void core_object_association() {
+  sloam::sloam s;
+  SloamOutput out;
+  bool success = false;
+  run_two_scans(s, out, &success, true);
+  bool matched = false;
+  for (int m : out.matches) matched = matched || m != -1;
+  EXPECT_TRUE(matched);
+}
+
+// ---------------------------------------------------------------------------
+// dump: the models and pose of the two-scan run, for comparison against the
+// CPU restatement by the pytest wrapper
+// ---------------------------------------------------------------------------
+int dump(const char* path) {
+  FILE* f = std::fopen(path, "w");
+  if (!f) return 2;
+  for (bool twoStep : {true, false}) {
+    FeatureModelParams p = core_params();
+    p.twoStepOptim = twoStep;
+    sloam::sloam s;
+    s.setFmParams(p);
+    SloamInput t0 = core_input("t0"), t1 = core_input("t1");
+    std::vector<Cylinder> lm;
+    std::vector<Plane> pl;
+    s.computeModels(t0, lm, pl);
+    std::fprintf(f, "scan t0 twostep %d planes %zu trees %zu\n", (int)twoStep, pl.size(), lm.size());
+    for (const auto& g : pl)
+      std::fprintf(f, "plane %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", g.model.plane[0],
+                   g.model.plane[1], g.model.plane[2], g.model.plane[3], g.model.centroid[0],
+                   g.model.centroid[1], g.model.centroid[2]);
+    for (const auto& c : lm)
+      std::fprintf(f, "tree %zu %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", c.id, c.model.root[0],
+                   c.model.root[1], c.model.root[2], c.model.ray[0], c.model.ray[1], c.model.ray[2],
+                   c.model.radius);
+    SloamOutput out;
+    sloam::sloam s2;
+    s2.setFmParams(p);
+    s2.RunSloam(t0, out);
+    t1.mapModels = out.tm;
+    const bool ok = s2.RunSloam(t1, out);
+    const Vector3& t = out.T_Map_Curr.translation();
+    const Quat& q = out.T_Map_Curr.unit_quaternion();
+    std::fprintf(f, "pose %d %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", (int)twoStep, (int)ok,
+                 t[0], t[1], t[2], q.x, q.y, q.z, q.w);
+    std::fprintf(f, "matches");
+    for (int m : out.matches) std::fprintf(f, " %d", m);
+    std::fprintf(f, "\n");
+  }
+  // the pose problems of scan t1 against the map of scan t0, and the three
+  // solves over them
+  {
+    const FeatureModelParams p = core_params();
+    sloam::sloam s;
+    s.setFmParams(p);
+    SloamInput t0 = core_input("t0"), t1 = core_input("t1");
+    SloamOutput out0;
+    s.RunSloam(t0, out0);
+    std::vector<Cylinder> lm;
+    std::vector<Plane> pl;
+    s.computeModels(t1, lm, pl);
+    const auto tm = s.matchFeatures(t1.poseEstimate, lm, out0.tm, p.treeMatchThresh);
+    const auto gm = s.matchFeatures(t1.poseEstimate, pl, s.getPrevGroundModel(), 1.0);
+    for (const auto& m : tm) {
+      const auto& o = m.object.model;
+      std::fprintf(f, "tmatch %.9g %.9g %.9g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+                   m.feature[0], m.feature[1], m.feature[2], o.root[0], o.root[1], o.root[2],
+                   o.ray[0], o.ray[1], o.ray[2], o.radius);
+    }
+    for (const auto& m : gm) {
+      const auto& o = m.object.model;
+      std::fprintf(f, "gmatch %.9g %.9g %.9g %.17g %.17g %.17g %.17g\n", m.feature[0],
+                   m.feature[1], m.feature[2], o.plane[0], o.plane[1], o.plane[2], o.plane[3]);
+    }
+    SE3 tf;
+    const bool ok = s.OptimizePose(t1.poseEstimate, tm, gm, tf);
+    const Vector3& t = tf.translation();
+    const Quat& q = tf.unit_quaternion();
+    std::fprintf(f, "joint %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", (int)ok, t[0], t[1],
+                 t[2], q.x, q.y, q.z, q.w);
+    double xy[3], zrp[3];
+    s.OptimizeXYYaw(t1.poseEstimate, true, tm, xy);
+    s.OptimizeZRollPitch(t1.poseEstimate, true, gm, zrp);
+    std::fprintf(f, "xyyaw %.17g %.17g %.17g\n", xy[0], xy[1], xy[2]);
+    std::fprintf(f, "zrollpitch %.17g %.17g %.17g\n", zrp[0], zrp[1], zrp[2]);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+struct Case {
+  const char* name;
+  bool gpu;
+  std::function<void()> fn;
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s <fixture dir> [host|gpu|dump <out>]\n", argv[0]);
+    return 2;
+  }
+  g_dir = argv[1];
+  const std::string mode = argc > 2 ? argv[2] : "all";
+  if (mode == "dump") {
+    if (argc < 4) return 2;
+    try {
+      return dump(argv[3]);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "dump: %s\n", e.what());
+      return 1;
+    }
+  }
+  const std::vector<Case> cases = {
+      {"EulerAngles012", false, test_euler_angles},
+      {"FromTwoVectors", false, test_from_two_vectors},
+      {"AngleAxisQuaternion", false, test_angle_axis},
+      {"SE3", false, test_se3},
+      {"PlaneTest.Initalizes", true, plane_initializes},
+      {"PlaneTest.DistanceToFeature", true, plane_distance_to_feature},
+      {"PlaneTest.TranslateModel", true, plane_translate_model},
+      {"CylinderTest.DistanceToModel", true, cylinder_distance_to_model},
+      {"CylinderTest.DistanceToFeature", true, cylinder_distance_to_feature},
+      {"CylinderTest.TranslateModel", true, cylinder_translate_model},
+      {"SLOAMTest.FirstScan", true, core_first_scan},
+      {"SLOAMTest.SecondScan", true, core_second_scan},
+      {"SLOAMTest.SLOAMSucess", true, core_sloam_success},
+      {"SLOAMTest.PoseOptimization", true, core_pose_optimization},
+      {"SLOAMTest.ObjectAssociation", true, core_object_association},
+  };
+  int ran = 0;
+  for (const auto& c : cases) {
+    if ((mode == "host" && c.gpu) || (mode == "gpu" && !c.gpu)) continue;
+    g_test = c.name;
+    const int before = g_failures;
+    std::printf("[ RUN      ] %s\n", c.name);
+    std::fflush(stdout);
+    try {
+      c.fn();
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "[%s] exception: %s\n", c.name, e.what());
+      ++g_failures;
+    }
+    std::printf("[ %s ] %s\n", g_failures == before ? "      OK" : " FAILED ", c.name);
+    std::fflush(stdout);
+    ++ran;
+  }
+  std::printf("%d tests, %d failures\n", ran, g_failures);
+  return g_failures ? 1 : 0;
+}
