@@ -289,6 +289,53 @@ int sloam_fit_lines(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets
                     const sloam_line_fit_cfg* cfg, float* d_coeff, int32_t* d_stats,
                     uint8_t* d_inliers);
 
+
+/* ---- tree instances: organized connected components + beam vertices -------
+ * Replaces: Instance::findClusters, findTrees, computeTreeVertex and
+ * computeVertexProperties (sloam/src/segmentation/trellis.cpp:15-132; the
+ * clustering is PCL's OrganizedConnectedComponentSegmentation with a
+ * EuclideanClusterComparator, threshold 1.0 m, not depth dependent).
+ * d_cloud: organized H x W cloud, 4 floats per pixel (x, y, z, intensity),
+ * row-major; a pixel takes part iff x is finite (the tree cloud of
+ * sloam_project_partition).  Components are numbered by their first pixel in
+ * row-major order.  Each component with more than min_label_points pixels
+ * is a candidate tree; its rows from the last to the first with more than
+ * min_row_points pixels give a vertex: medians of x, y, z (element
+ * floor(n/2) of each sorted axis), the points within max_dist_to_centroid
+ * of the median in (z, y, x, column) order, valid with at least 2 points,
+ * radius = radius_scale x the distance of the first and last kept points.
+ * Trees with more than min_vertices valid vertices are kept, cut to the
+ * first max_vertices (bottom up).
+ * Outputs (device; capacities in entries):
+ *   d_labels        (nullable) H*W int32 component id, -1 for empty pixels
+ *   d_tree_offsets  [cap_trees + 1] into vertices
+ *   d_vertex_int    [cap_vertices][4] treeId (component), row, points, valid
+ *   d_vertex_f64    [cap_vertices][4] radius, median x, y, z
+ *   d_point_offsets [cap_vertices + 1] into points
+ *   d_points        [cap_points][4] kept points (x, y, z, intensity)
+ *   d_counts        8 int32: trees, vertices, points, components, overflow
+ *                   (1 when a capacity was too small: outputs are then
+ *                   truncated), beams, reserved...
+ * Bounds: cap_trees <= H*W / (min_label_points + 1), cap_vertices <=
+ * cap_trees * max_vertices, cap_points <= H*W always suffice.  H, W <=
+ * 65535 and H*W < 2^20.                                                      */
+typedef struct sloam_trellis_cfg {
+  float cluster_dist;          /* 1.0 m (trellis.cpp:23)                      */
+  float max_dist_to_centroid;  /* 0.2 m (sloamNode.cpp:62)                     */
+  int32_t min_label_points;    /* 80  (trellis.cpp:109)                       */
+  int32_t min_row_points;      /* 3   (trellis.cpp:119)                       */
+  int32_t min_vertices;        /* 16  (trellis.cpp:124)                       */
+  int32_t max_vertices;        /* 56  (trellis.cpp:125)                       */
+  double radius_scale;         /* 1.0 (trellis.cpp:98); 0.5 reproduces the
+                                  reference's test fixtures                   */
+} sloam_trellis_cfg;
+
+int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t width,
+                  const sloam_trellis_cfg* cfg, int32_t* d_labels, int32_t cap_trees,
+                  int32_t cap_vertices, int32_t cap_points, int32_t* d_tree_offsets,
+                  int32_t* d_vertex_int, double* d_vertex_f64, int32_t* d_point_offsets,
+                  float* d_points, int32_t* d_counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,12 @@ def _declare(lib):
     lib.oracle_pose_lm_batch.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P, P, P, C.c_int]
     lib.oracle_left_jacobian.argtypes = [P, P]
     lib.oracle_aa_quat.argtypes = [P, P, P]
+    lib.oracle_trellis_clusters.argtypes = [P, C.c_int, C.c_int, C.c_float, P]
+    lib.oracle_trellis_clusters.restype = C.c_int
+    lib.oracle_trellis_trees.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_float, C.c_int, C.c_int, C.c_double, C.c_int,
+                                         C.c_int, C.c_int, P, P, P, P, P, P]
+    lib.oracle_trellis_trees.restype = C.c_int
 
 
 def _p(a):
@@ -274,3 +280,46 @@ def line_ransac_batch(xyz, offsets, **kw):
     return {"coeff": np.array([r["coeff"] for r in res]).reshape(-1, 6),
             "stats": np.array([r["stats"] for r in res]).reshape(-1, 4),
             "inliers": np.concatenate([r["inliers"] for r in res]) if res else np.zeros(0, np.uint8)}
+
+
+TRELLIS_DEFAULTS = dict(dist_thr=1.0, min_label_points=80, min_row_points=3, max_dist=0.2,
+                        min_vertices=16, max_vertices=56, radius_scale=1.0)
+
+
+def trellis_clusters(cloud, H, W, dist_thr=1.0):
+    x = np.ascontiguousarray(cloud, np.float32).reshape(H * W, 4)
+    labels = np.zeros(H * W, np.int32)
+    n = load().oracle_trellis_clusters(_p(x), H, W, dist_thr, _p(labels))
+    return labels, n
+
+
+def trellis(cloud, H, W, **kw):
+    """Components and trees of an organized H x W tree cloud (x NaN = empty)."""
+    c = dict(TRELLIS_DEFAULTS)
+    c.update(kw)
+    x = np.ascontiguousarray(cloud, np.float32).reshape(H * W, 4)
+    labels, n = trellis_clusters(x, H, W, c["dist_thr"])
+    cap_t = H * W // (c["min_label_points"] + 1) + 1
+    cap_v = cap_t * c["max_vertices"]
+    cap_p = H * W
+    out = {"labels": labels, "n_labels": n,
+           "tree_offsets": np.zeros(cap_t + 1, np.int32),
+           "vertex_int": np.zeros((cap_v, 4), np.int32),
+           "vertex_f64": np.zeros((cap_v, 4)),
+           "point_offsets": np.zeros(cap_v + 1, np.int32),
+           "points": np.zeros((cap_p, 4), np.float32)}
+    counts = np.zeros(3, np.int32)
+    rc = load().oracle_trellis_trees(_p(x), _p(labels), H, W, n, c["min_label_points"],
+                                     c["min_row_points"], c["max_dist"], c["min_vertices"],
+                                     c["max_vertices"], c["radius_scale"], cap_t, cap_v, cap_p,
+                                     _p(out["tree_offsets"]), _p(out["vertex_int"]),
+                                     _p(out["vertex_f64"]), _p(out["point_offsets"]),
+                                     _p(out["points"]), _p(counts))
+    assert rc == 0
+    nt, nv, npt = (int(v) for v in counts)
+    out["tree_offsets"] = out["tree_offsets"][:nt + 1]
+    out["vertex_int"] = out["vertex_int"][:nv]
+    out["vertex_f64"] = out["vertex_f64"][:nv]
+    out["point_offsets"] = out["point_offsets"][:nv + 1]
+    out["points"] = out["points"][:npt]
+    return out

@@ -51,6 +51,18 @@ class PlaneFitCfg(C.Structure):
     ]
 
 
+class TrellisCfg(C.Structure):
+    _fields_ = [
+        ("cluster_dist", C.c_float),
+        ("max_dist_to_centroid", C.c_float),
+        ("min_label_points", C.c_int32),
+        ("min_row_points", C.c_int32),
+        ("min_vertices", C.c_int32),
+        ("max_vertices", C.c_int32),
+        ("radius_scale", C.c_double),
+    ]
+
+
 class LineFitCfg(C.Structure):
     _fields_ = [
         ("threshold", C.c_double),
@@ -115,6 +127,8 @@ PROTOTYPES = {
     "sloam_plane_plan_create": (C.c_int, [_P, _P, _I32, C.POINTER(PlaneFitCfg), C.POINTER(_P)]),
     "sloam_plane_plan_destroy": (None, [_P]),
     "sloam_fit_lines": (C.c_int, [_P, _P, _P, _I32, C.POINTER(LineFitCfg), _P, _P, _P]),
+    "sloam_trellis": (C.c_int, [_P, _P, _I32, _I32, C.POINTER(TrellisCfg), _P, _I32, _I32, _I32,
+                                _P, _P, _P, _P, _P, _P]),
     "sloam_plane_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }
 

@@ -10,7 +10,7 @@ import torch
 
 from . import _lib
 from ._lib import (CYL_MODEL_STRIDE, CylFitCfg, GroundCfg, LineFitCfg, PlaneFitCfg, PoseCfg,
-                   ProjCfg, check)
+                   ProjCfg, TrellisCfg, check)
 
 
 def _ptr(t):
@@ -380,3 +380,54 @@ def fit_lines(ctx, xyz, offsets_host, threshold=0.25, probability=0.99, max_iter
     ctx.sync()
     return {"coeff": coeff.cpu().numpy()[:S], "stats": stats.cpu().numpy()[:S],
             "inliers": inl.cpu().numpy()[:N]}
+
+
+TRELLIS_DEFAULTS = dict(cluster_dist=1.0, max_dist_to_centroid=0.2, min_label_points=80,
+                        min_row_points=3, min_vertices=16, max_vertices=56, radius_scale=1.0)
+
+
+def trellis_dev(ctx, cloud, H, W, out=None, **kw):
+    """Tree instances of an organized H x W cloud already on the device
+    (float32 [H*W, 4]); device outputs, counts left on the device."""
+    c = dict(TRELLIS_DEFAULTS)
+    c.update(kw)
+    dev = torch.device("cuda", ctx.device)
+    N = H * W
+    cap_t = N // (c["min_label_points"] + 1) + 1
+    cap_v = cap_t * c["max_vertices"]
+    cap_p = max(N, 1)
+    if out is None:
+        out = {
+            "labels": torch.empty((max(N, 1),), dtype=torch.int32, device=dev),
+            "tree_offsets": torch.empty((cap_t + 1,), dtype=torch.int32, device=dev),
+            "vertex_int": torch.empty((cap_v, 4), dtype=torch.int32, device=dev),
+            "vertex_f64": torch.empty((cap_v, 4), dtype=torch.float64, device=dev),
+            "point_offsets": torch.empty((cap_v + 1,), dtype=torch.int32, device=dev),
+            "points": torch.empty((cap_p, 4), dtype=torch.float32, device=dev),
+            "counts": torch.empty((8,), dtype=torch.int32, device=dev),
+        }
+    cfg = TrellisCfg(**c)
+    check(ctx.lib.sloam_trellis(ctx.handle, _ptr(cloud), H, W, C.byref(cfg), _ptr(out["labels"]),
+                                cap_t, cap_v, cap_p, _ptr(out["tree_offsets"]),
+                                _ptr(out["vertex_int"]), _ptr(out["vertex_f64"]),
+                                _ptr(out["point_offsets"]), _ptr(out["points"]),
+                                _ptr(out["counts"])), ctx.handle)
+    return out
+
+
+def trellis(ctx, cloud, H, W, **kw):
+    """Host in / host out: numpy arrays trimmed to the counts."""
+    dev = torch.device("cuda", ctx.device)
+    x = torch.as_tensor(np.ascontiguousarray(cloud, np.float32).reshape(-1, 4)).to(dev)
+    torch.cuda.synchronize()
+    out = trellis_dev(ctx, x, H, W, **kw)
+    ctx.sync()
+    cnt = out["counts"].cpu().numpy()
+    nt, nv, npt = int(cnt[0]), int(cnt[1]), int(cnt[2])
+    return {"labels": out["labels"].cpu().numpy()[:H * W], "n_labels": int(cnt[3]),
+            "overflow": int(cnt[4]), "beams": int(cnt[5]),
+            "tree_offsets": out["tree_offsets"].cpu().numpy()[:nt + 1],
+            "vertex_int": out["vertex_int"].cpu().numpy()[:nv],
+            "vertex_f64": out["vertex_f64"].cpu().numpy()[:nv],
+            "point_offsets": out["point_offsets"].cpu().numpy()[:nv + 1],
+            "points": out["points"].cpu().numpy()[:npt]}

@@ -246,4 +246,55 @@ std::vector<PoseResult> solvePoses(const std::vector<PoseProblem>& problems,
   return out;
 }
 
+TrellisResult trellis(const VectorType& cloud, int height, int width,
+                      const sloam_trellis_cfg& cfg, bool want_labels) {
+  TrellisResult out;
+  sloam_ctx* ctx = context();
+  const int N = height * width;
+  const int cap_t = N / (cfg.min_label_points + 1) + 1;
+  const int cap_v = cap_t * cfg.max_vertices;
+  const int cap_p = N > 0 ? N : 1;
+  std::vector<float> pts(4 * (size_t)N);
+  for (int i = 0; i < N; ++i) {
+    pts[4 * (size_t)i] = cloud[i].x;
+    pts[4 * (size_t)i + 1] = cloud[i].y;
+    pts[4 * (size_t)i + 2] = cloud[i].z;
+    pts[4 * (size_t)i + 3] = cloud[i].intensity;
+  }
+  Dev d_cloud(sizeof(float) * (pts.empty() ? 4 : pts.size()));
+  Dev d_labels(sizeof(int32_t) * (want_labels && N > 0 ? N : 1));
+  Dev d_to(sizeof(int32_t) * (cap_t + 1));
+  Dev d_vi(sizeof(int32_t) * 4 * (size_t)cap_v);
+  Dev d_vf(sizeof(double) * 4 * (size_t)cap_v);
+  Dev d_po(sizeof(int32_t) * (cap_v + 1));
+  Dev d_pts(sizeof(float) * 4 * (size_t)cap_p);
+  Dev d_cnt(sizeof(int32_t) * 8);
+  up(d_cloud, pts);
+  check(sloam_trellis(ctx, d_cloud.as<float>(), height, width, &cfg,
+                      want_labels ? d_labels.as<int32_t>() : nullptr, cap_t, cap_v, cap_p,
+                      d_to.as<int32_t>(), d_vi.as<int32_t>(), d_vf.as<double>(),
+                      d_po.as<int32_t>(), d_pts.as<float>(), d_cnt.as<int32_t>()),
+        "sloam_trellis");
+  std::vector<int32_t> cnt(8);
+  down(cnt, d_cnt);
+  check(sloam_sync(ctx), "sloam_sync");
+  if (cnt[4]) throw std::runtime_error("sloam_trellis: output capacity exceeded");
+  const int nt = cnt[0], nv = cnt[1], np = cnt[2];
+  out.n_labels = cnt[3];
+  out.tree_offsets.resize(nt + 1);
+  out.vertex_int.resize(4 * (size_t)nv);
+  out.vertex_f64.resize(4 * (size_t)nv);
+  out.point_offsets.resize(nv + 1);
+  out.points.resize(4 * (size_t)np);
+  if (want_labels) out.labels.resize(N);
+  down(out.tree_offsets, d_to);
+  down(out.vertex_int, d_vi);
+  down(out.vertex_f64, d_vf);
+  down(out.point_offsets, d_po);
+  down(out.points, d_pts);
+  if (want_labels) down(out.labels, d_labels);
+  check(sloam_sync(ctx), "sloam_sync");
+  return out;
+}
+
 }  // namespace sloam_gpu

@@ -55,4 +55,16 @@ struct PoseResult {
 std::vector<PoseResult> solvePoses(const std::vector<PoseProblem>& problems,
                                    const sloam_pose_cfg& cfg);
 
+struct TrellisResult {
+  std::vector<int32_t> labels;  // per pixel (when asked for)
+  int n_labels = 0;
+  std::vector<int32_t> tree_offsets, vertex_int, point_offsets;
+  std::vector<double> vertex_f64;
+  std::vector<float> points;
+};
+
+// sloam_trellis over an organized cloud (height x width PointT, row-major)
+TrellisResult trellis(const VectorType& cloud, int height, int width,
+                      const sloam_trellis_cfg& cfg, bool want_labels);
+
 }  // namespace sloam_gpu

@@ -13,6 +13,7 @@
 //         beam, prevVertexSize, isValid, float64 radius, float32 coords[3],
 //         int32 n_points, float32 points[3 n_points]
 //   .xyz: int32 n; float32 xyz[3 n]
+//   .org: int32 height, width; float32 xyz[3 height width] (NaN = empty)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -27,6 +28,7 @@
 #include "definitions.h"
 #include "plane.h"
 #include "sloam.h"
+#include "trellis.h"
 
 namespace {
 
@@ -116,6 +118,24 @@ CloudT::Ptr read_cloud(const std::string& name) {
   std::fclose(f);
   CloudT::Ptr c(new CloudT);
   for (int i = 0; i < n; ++i) c->push_back(PointT{p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.f});
+  return c;
+}
+
+CloudT::Ptr read_organized(const std::string& name) {
+  const std::string path = g_dir + "/" + name + ".org";
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + path);
+  int32_t hw[2];
+  rd(f, hw, 2, path);
+  std::vector<float> p(3 * (size_t)hw[0] * hw[1]);
+  rd(f, p.data(), p.size(), path);
+  std::fclose(f);
+  CloudT::Ptr c(new CloudT);
+  for (size_t i = 0; i < p.size() / 3; ++i)
+    c->points.push_back(PointT{p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.f});
+  c->height = (uint32_t)hw[0];
+  c->width = (uint32_t)hw[1];
+  c->is_dense = false;
   return c;
 }
 
@@ -326,6 +346,53 @@ void cylinder_translate_model() {
   const Vector3 root = cyl[0].model.root;
   cyl[0].project(tf);
   EXPECT_EQ(root[0] + 1, cyl[0].model.root[0]);
+}
+
+// ---------------------------------------------------------------------------
+// trellis: the reference's tree clouds give its landmark archives
+// (sloam/src/tests/aux/<seq>_tree_<t>.pcd -> <seq>_landmarks_<t>); the
+// archives hold half the code's vertex radius
+// ---------------------------------------------------------------------------
+void instance_reference_clouds() {
+  for (const char* stamp : {"t0", "t1"}) {
+    Instance inst;
+    inst.set_params(Instance::Params());
+    inst.reset_tree_id();
+    std::vector<std::vector<TreeVertex>> lm;
+    CloudT::Ptr tree = read_organized(std::string("still_tree_") + stamp);
+    inst.computeGraph(tree, tree, lm);
+    const Landmarks ref = read_landmarks(std::string("still_landmarks_") + stamp);
+    EXPECT_EQ(lm.size(), ref.size());
+    if (lm.size() != ref.size()) continue;
+    for (size_t t = 0; t < lm.size(); ++t) {
+      EXPECT_EQ(lm[t].size(), ref[t].size());
+      if (lm[t].size() != ref[t].size()) continue;
+      for (size_t v = 0; v < lm[t].size(); ++v) {
+        const TreeVertex &a = lm[t][v], &b = ref[t][v];
+        EXPECT_EQ(a.treeId, b.treeId);
+        EXPECT_EQ(a.points.size(), b.points.size());
+        EXPECT_NEAR(a.coords.x, b.coords.x, 1e-6 * (1 + std::fabs(b.coords.x)));
+        EXPECT_NEAR(a.coords.y, b.coords.y, 1e-6 * (1 + std::fabs(b.coords.y)));
+        EXPECT_NEAR(a.coords.z, b.coords.z, 1e-6 * (1 + std::fabs(b.coords.z)));
+        EXPECT_NEAR(a.radius, 2.0 * b.radius, 2e-6);
+      }
+    }
+  }
+}
+
+void instance_find_clusters() {
+  Instance inst;
+  CloudT::Ptr tree = read_organized("still_tree_t0");
+  std::vector<int> labels;
+  std::vector<std::vector<int>> idx;
+  inst.findClusters(tree, labels, idx);
+  EXPECT_EQ(labels.size(), tree->points.size());
+  size_t n = 0;
+  for (const auto& l : idx) n += l.size();
+  size_t finite = 0;
+  for (const auto& p : tree->points) finite += std::isfinite(p.x) ? 1 : 0;
+  EXPECT_EQ(n, finite);
+  EXPECT_TRUE(!idx.empty());
 }
 
 // ---------------------------------------------------------------------------
@@ -542,6 +609,8 @@ int main(int argc, char** argv) {
       {"CylinderTest.DistanceToModel", true, cylinder_distance_to_model},
       {"CylinderTest.DistanceToFeature", true, cylinder_distance_to_feature},
       {"CylinderTest.TranslateModel", true, cylinder_translate_model},
+      {"InstanceTest.ReferenceClouds", true, instance_reference_clouds},
+      {"InstanceTest.FindClusters", true, instance_find_clusters},
       {"SLOAMTest.FirstScan", true, core_first_scan},
       {"SLOAMTest.SecondScan", true, core_second_scan},
       {"SLOAMTest.SLOAMSucess", true, core_sloam_success},

@@ -3,8 +3,9 @@ fixtures under tests/golden/.  Run in a container that has the reference
 mounted at /root/reference; the outputs are committed, the reference is not
 needed afterwards.
 
-  still_tree_t0.npz       organised 2048 x 64 tree cloud (ASCII PCD): the
-                          flat indices and xyz of its finite points
+  {still,moving}_tree_t{0,1}.npz
+                          organised 2048 x 64 tree clouds (ASCII PCD): the
+                          flat indices and xyz of their finite points
   {still,moving}_landmarks_t{0,1}.npz
                           boost text_iarchive (v17) of
                           std::vector<std::vector<TreeVertex>>: per vertex
@@ -119,11 +120,12 @@ def main():
     if not os.path.isdir(AUX):
         print("reference test data not found", file=sys.stderr)
         return 1
-    data, w, h = read_pcd(os.path.join(AUX, "still_tree_t0.pcd"))
-    fin = np.isfinite(data[:, :3]).all(1)
-    np.savez_compressed(os.path.join(OUT, "still_tree_t0.npz"), width=w, height=h,
-                        index=np.nonzero(fin)[0].astype(np.int32),
-                        xyz=data[fin, :3].astype(np.float32))
+    for name in ("still_tree_t0", "still_tree_t1", "moving_tree_t0", "moving_tree_t1"):
+        data, w, h = read_pcd(os.path.join(AUX, name + ".pcd"))
+        fin = np.isfinite(data[:, :3]).all(1)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), width=w, height=h,
+                            index=np.nonzero(fin)[0].astype(np.int32),
+                            xyz=data[fin, :3].astype(np.float32))
     for pre in ("still", "moving"):
         for t in ("t0", "t1"):
             trees = read_landmarks(os.path.join(AUX, f"{pre}_landmarks_{t}"))
