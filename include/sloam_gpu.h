@@ -118,6 +118,24 @@ void sloam_cyl_plan_destroy(sloam_cyl_plan* plan);
 int sloam_cyl_plan_exec(sloam_cyl_plan* plan, const float* d_xyz, const int32_t* d_offsets,
                         int32_t* d_counts, int32_t* d_samples, uint32_t* d_mask,
                         int32_t* d_best, double* d_model);
+/* The same with two optional inputs:
+ *   d_proposal [n_clusters * SLOAM_PN_PROPOSAL_STRIDE] floats (sloam_pn_infer):
+ *     for a cluster with a valid proposal the seed axis is the proposed axis
+ *     (no PCA) and hypothesis 0 is the proposed circle (its samples -1);
+ *   d_active [n_clusters] int32: clusters with 0 are left untouched (no
+ *     seed, scoring or refit; their model records keep their contents).
+ * d_mask must be NULL when d_active is given.                               */
+int sloam_cyl_plan_exec_ex(sloam_cyl_plan* plan, const float* d_xyz, const int32_t* d_offsets,
+                           const float* d_proposal, const int32_t* d_active, int32_t* d_counts,
+                           int32_t* d_samples, uint32_t* d_mask, int32_t* d_best,
+                           double* d_model);
+/* d_active[c] = 1 for clusters (>= 3 points) whose model is invalid or keeps
+ * fewer than min_inlier_ratio x n points as inliers (model[7]); d_n_active
+ * (nullable, device) gets their number.  With exec_ex this runs a fallback
+ * fit of exactly those clusters.                                            */
+int sloam_cyl_fallback(sloam_ctx* ctx, const int32_t* d_offsets, int32_t n_clusters,
+                       const double* d_model, float min_inlier_ratio, int32_t* d_active,
+                       int32_t* d_n_active);
 
 /* ---- batched pose estimation (Levenberg-Marquardt, Huber loss) ------------
  * Replaces: sloam::sloam::OptimizePose (sloam/src/core/sloam.cpp:174-255) and
@@ -335,6 +353,61 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
                   int32_t cap_vertices, int32_t cap_points, int32_t* d_tree_offsets,
                   int32_t* d_vertex_int, double* d_vertex_f64, int32_t* d_point_offsets,
                   float* d_points, int32_t* d_counts);
+
+
+/* ---- learned cylinder proposal (PointNet, BF16 MFMA) -----------------------
+ * No counterpart in the reference.  It proposes a cylinder per cluster, so that
+ * a fit needs a few verifying hypotheses instead of a full RANSAC (the
+ * predicted model's output space is CylinderParameters {root, ray, radius},
+ * sloam/include/objects/cylinder.h:14-22).
+ * Input per cluster: up to SLOAM_PN_POINTS points, taken at indices
+ * floor(j n / m), m = min(n, 512), shifted by the centre of their bounding box
+ * and scaled by 1 / (largest half extent).  Network: shared per-point MLP
+ * 3 -> 64 -> 128 -> 256 (ReLU), max-pool over the cluster's points, MLP
+ * 256 -> 128 (ReLU) -> 7 = (axis direction, axis point nearest the box
+ * centre, radius) in the normalised frame.  Weights are BF16 (rounded from
+ * the FP32 master copy), layers 2 and 3 run on v_mfma_f32_16x16x32_bf16 with
+ * FP32 accumulation, activations between them are BF16.
+ * Parameters: one FP32 array, PyTorch order (weight [out][in], then bias):
+ * W1 [64][3], b1, W2 [128][64], b2, W3 [256][128], b3, W4 [128][256], b4,
+ * W5 [7][128], b5: SLOAM_PN_PARAMS floats.                                   */
+#define SLOAM_PN_POINTS 512
+#define SLOAM_PN_PARAMS 75399
+#define SLOAM_PN_PROPOSAL_STRIDE 8
+
+/* Proposals, one record of 8 floats per cluster (world frame): axis[3]
+ * (unit), centre[3] (a point on the axis), radius, valid (1 / 0).          */
+int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                   const int32_t* d_offsets, int32_t n_clusters, float* d_proposal);
+
+/* Training loss against cylinder models (SLOAM_CYL_MODEL_STRIDE doubles per
+ * cluster, e.g. the plain fit's output; clusters whose radius is <= 0 carry
+ * no loss):  L_c = w_axis (1 - (a_hat . a)^2) + w_centre |c_hat - c*|^2 +
+ * w_radius (r_hat - r*)^2 in the normalised frame, L = mean over the batch.  */
+typedef struct sloam_pn_train_cfg {
+  float lr;          /* Adam step size                                        */
+  float beta1;       /* 0.9                                                   */
+  float beta2;       /* 0.999                                                 */
+  float eps;         /* 1e-8                                                  */
+  float w_axis;      /* loss weights                                          */
+  float w_centre;
+  float w_radius;
+  int32_t reserved;
+} sloam_pn_train_cfg;
+
+/* Loss per cluster (d_loss, n_clusters floats) and the batch-mean gradient
+ * (d_grad, SLOAM_PN_PARAMS floats); parameters unchanged.                   */
+int sloam_pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                       const int32_t* d_offsets, int32_t n_clusters, const double* d_targets,
+                       const sloam_pn_train_cfg* cfg, float* d_loss, float* d_grad);
+
+/* One Adam step (step = 1, 2, ... for the bias correction) on d_params with
+ * moments d_m, d_v (SLOAM_PN_PARAMS floats each); d_loss (nullable) gets the
+ * per-cluster losses before the step.                                       */
+int sloam_pn_train_step(sloam_ctx* ctx, float* d_params, float* d_m, float* d_v, int32_t step,
+                        const sloam_pn_train_cfg* cfg, const float* d_xyz,
+                        const int32_t* d_offsets, int32_t n_clusters, const double* d_targets,
+                        float* d_loss);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,9 @@ def _declare(lib):
     lib.oracle_pose_lm_batch.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P, P, P, C.c_int]
     lib.oracle_left_jacobian.argtypes = [P, P]
     lib.oracle_aa_quat.argtypes = [P, P, P]
+    lib.oracle_pn_forward.argtypes = [P, P, P, C.c_int, C.c_int, P, P, P, P]
+    lib.oracle_pn_loss_grad.argtypes = [P, P, P, C.c_int, P, P, C.c_int, P, P]
+    lib.oracle_pn_param_count.restype = C.c_int
     lib.oracle_trellis_clusters.argtypes = [P, C.c_int, C.c_int, C.c_float, P]
     lib.oracle_trellis_clusters.restype = C.c_int
     lib.oracle_trellis_trees.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -323,3 +326,34 @@ def trellis(cloud, H, W, **kw):
     out["point_offsets"] = out["point_offsets"][:nv + 1]
     out["points"] = out["points"][:npt]
     return out
+
+
+PN_LOSS_WEIGHTS = (1.0, 1.0, 1.0)
+
+
+def pn_forward(params, xyz, offsets, emulate=True):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(xyz, np.float32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    S = len(off) - 1
+    out = np.zeros((S, 7))
+    f = np.zeros((S, 256))
+    fidx = np.zeros((S, 256), np.int32)
+    prop = np.zeros((S, 8))
+    load().oracle_pn_forward(_p(p), _p(x), _p(off), S, int(emulate), _p(out), _p(f), _p(fidx),
+                             _p(prop))
+    return {"out": out, "f": f, "fidx": fidx, "proposal": prop}
+
+
+def pn_loss_grad(params, xyz, offsets, targets, weights=PN_LOSS_WEIGHTS, emulate=True):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(xyz, np.float32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    t = np.ascontiguousarray(targets, np.float64)
+    w = np.ascontiguousarray(weights, np.float32)
+    S = len(off) - 1
+    loss = np.zeros(S)
+    grad = np.zeros(load().oracle_pn_param_count())
+    load().oracle_pn_loss_grad(_p(p), _p(x), _p(off), S, _p(t), _p(w), int(emulate), _p(loss),
+                               _p(grad))
+    return loss, grad

@@ -1,0 +1,262 @@
+// TEST INFRASTRUCTURE ONLY -- FP64 CPU oracle of the learned cylinder proposal
+// network (sloam_amd/csrc/pointnet.hip; no counterpart in the reference).
+//
+// Same input preparation as the GPU (float): up to 512 points at indices
+// floor(j n / m), shifted by their bounding-box centre and scaled by
+// 1 / max(largest half extent, 1e-3).  Network in FP64: shared MLP
+// 3 -> 64 -> 128 -> 256 (ReLU), max-pool over the m points (first maximising
+// point per channel; channels whose outputs are all 0 pool to 0 and pass no
+// gradient), MLP 256 -> 128 (ReLU) -> 7.  With `emulate` set, weights and
+// biases are rounded to BF16 and the layer-1 / layer-2 activations to BF16
+// after the ReLU, as the GPU stores them; without it everything is FP64 (for
+// central-difference checks of the analytic gradient).  Loss per cluster
+// against a cylinder model (root, ray, radius; radius <= 0 -> no loss):
+//   w_axis (1 - (a_hat . a)^2) + w_centre |c_hat - c*|^2 + w_radius (r_hat - r*)^2
+// with a_hat = out[0:3] / |out[0:3]|, c* the axis point nearest the box
+// centre and r* the radius, both in the normalised frame; the gradient is
+// the batch mean.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int kP = 512, kC1 = 64, kC2 = 128, kC3 = 256, kC4 = 128, kOut = 7;
+constexpr int kW1 = 0, kB1 = kW1 + kC1 * 3;
+constexpr int kW2 = kB1 + kC1, kB2 = kW2 + kC2 * kC1;
+constexpr int kW3 = kB2 + kC2, kB3 = kW3 + kC3 * kC2;
+constexpr int kW4 = kB3 + kC3, kB4 = kW4 + kC4 * kC3;
+constexpr int kW5 = kB4 + kC4, kB5 = kW5 + kOut * kC4;
+constexpr int kNP = kB5 + kOut;
+
+float bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+struct Prep {
+  int m = 0, n = 0;
+  float cen[3] = {0, 0, 0};
+  float s = 1.f;
+  std::vector<float> x;  // m x 3 normalised
+};
+
+Prep prepare(const float* P, int n) {
+  Prep r;
+  r.n = n;
+  r.m = n < kP ? n : kP;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int j = 0; j < r.m; ++j) {
+    const int i = (int)(((long long)j * n) / r.m);
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::fmin(lo[k], P[3 * i + k]);
+      hi[k] = std::fmax(hi[k], P[3 * i + k]);
+    }
+  }
+  float ext = 1e-3f;
+  for (int k = 0; k < 3; ++k) {
+    r.cen[k] = r.m > 0 ? 0.5f * (lo[k] + hi[k]) : 0.0f;
+    if (r.m > 0) ext = std::fmax(ext, 0.5f * (hi[k] - lo[k]));
+  }
+  r.s = 1.0f / ext;
+  r.x.resize(3 * (size_t)r.m);
+  for (int j = 0; j < r.m; ++j) {
+    const int i = (int)(((long long)j * n) / r.m);
+    for (int k = 0; k < 3; ++k) r.x[3 * j + k] = (P[3 * i + k] - r.cen[k]) * r.s;
+  }
+  return r;
+}
+
+struct Fwd {
+  std::vector<double> h1, h2;  // m x 64, m x 128 (after ReLU, rounded if emulating)
+  double f[kC3];
+  int fidx[kC3];
+  double h4[kC4], out[kOut];
+};
+
+void forward(const std::vector<double>& W, const Prep& pr, bool emulate, Fwd& F) {
+  const int m = pr.m;
+  F.h1.assign((size_t)m * kC1, 0.0);
+  F.h2.assign((size_t)m * kC2, 0.0);
+  auto act = [&](double z) {
+    const double r = z > 0.0 ? z : 0.0;
+    return emulate ? (double)bf16((float)r) : r;
+  };
+  for (int p = 0; p < m; ++p) {
+    const double x = pr.x[3 * p], y = pr.x[3 * p + 1], z = pr.x[3 * p + 2];
+    for (int c = 0; c < kC1; ++c) {
+      const double s = ((W[kB1 + c] + W[kW1 + 3 * c] * x) + W[kW1 + 3 * c + 1] * y) +
+                       W[kW1 + 3 * c + 2] * z;
+      F.h1[(size_t)p * kC1 + c] = act(s);
+    }
+    for (int j = 0; j < kC2; ++j) {
+      double s = W[kB2 + j];
+      for (int k = 0; k < kC1; ++k) s += W[kW2 + j * kC1 + k] * F.h1[(size_t)p * kC1 + k];
+      F.h2[(size_t)p * kC2 + j] = act(s);
+    }
+  }
+  for (int c = 0; c < kC3; ++c) {
+    F.f[c] = 0.0;
+    F.fidx[c] = -1;
+  }
+  for (int p = 0; p < m; ++p)
+    for (int c = 0; c < kC3; ++c) {
+      double s = W[kB3 + c];
+      for (int k = 0; k < kC2; ++k) s += W[kW3 + c * kC2 + k] * F.h2[(size_t)p * kC2 + k];
+      if (s > F.f[c]) {
+        F.f[c] = s;
+        F.fidx[c] = p;
+      }
+    }
+  for (int t = 0; t < kC4; ++t) {
+    double s = W[kB4 + t];
+    for (int k = 0; k < kC3; ++k) s += W[kW4 + t * kC3 + k] * F.f[k];
+    F.h4[t] = s > 0.0 ? s : 0.0;
+  }
+  for (int o = 0; o < kOut; ++o) {
+    double s = W[kB5 + o];
+    for (int k = 0; k < kC4; ++k) s += W[kW5 + o * kC4 + k] * F.h4[k];
+    F.out[o] = s;
+  }
+}
+
+// loss and dL/dout
+double loss(const Prep& pr, const double* tg, const double* out, const float* wts, double* d) {
+  for (int k = 0; k < kOut; ++k) d[k] = 0.0;
+  if (!(tg[6] > 0.0) || pr.m < 3) return 0.0;
+  const double an = std::sqrt(tg[3] * tg[3] + tg[4] * tg[4] + tg[5] * tg[5]);
+  const double A[3] = {tg[3] / an, tg[4] / an, tg[5] / an};
+  double r[3];
+  for (int k = 0; k < 3; ++k) r[k] = (tg[k] - pr.cen[k]) * pr.s;
+  const double pj = r[0] * A[0] + r[1] * A[1] + r[2] * A[2];
+  double cs[3];
+  for (int k = 0; k < 3; ++k) cs[k] = r[k] - pj * A[k];
+  const double rs = tg[6] * pr.s;
+  const double nv = std::sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]) + 1e-12;
+  double ah[3];
+  for (int k = 0; k < 3; ++k) ah[k] = out[k] / nv;
+  const double dd = ah[0] * A[0] + ah[1] * A[1] + ah[2] * A[2];
+  double L = wts[0] * (1.0 - dd * dd);
+  for (int k = 0; k < 3; ++k) d[k] = wts[0] * (-2.0 * dd * (A[k] - dd * ah[k]) / nv);
+  for (int k = 0; k < 3; ++k) {
+    const double e = out[3 + k] - cs[k];
+    L += wts[1] * e * e;
+    d[3 + k] = wts[1] * 2.0 * e;
+  }
+  const double er = out[6] - rs;
+  L += wts[2] * er * er;
+  d[6] = wts[2] * 2.0 * er;
+  return L;
+}
+
+void backward(const std::vector<double>& W, const Prep& pr, const Fwd& F, const double* dout,
+              double scale, std::vector<double>& g) {
+  double dz4[kC4], df[kC3];
+  for (int o = 0; o < kOut; ++o) {
+    for (int k = 0; k < kC4; ++k) g[kW5 + o * kC4 + k] += scale * dout[o] * F.h4[k];
+    g[kB5 + o] += scale * dout[o];
+  }
+  for (int t = 0; t < kC4; ++t) {
+    double s = 0.0;
+    for (int o = 0; o < kOut; ++o) s += W[kW5 + o * kC4 + t] * dout[o];
+    dz4[t] = F.h4[t] > 0.0 ? s : 0.0;
+    for (int k = 0; k < kC3; ++k) g[kW4 + t * kC3 + k] += scale * dz4[t] * F.f[k];
+    g[kB4 + t] += scale * dz4[t];
+  }
+  for (int k = 0; k < kC3; ++k) {
+    double s = 0.0;
+    for (int t = 0; t < kC4; ++t) s += W[kW4 + t * kC3 + k] * dz4[t];
+    df[k] = F.fidx[k] >= 0 ? s : 0.0;
+  }
+  const int m = pr.m;
+  std::vector<double> dh2((size_t)m * kC2, 0.0), dh1((size_t)m * kC1, 0.0);
+  for (int c = 0; c < kC3; ++c) {
+    const int p = F.fidx[c];
+    if (p < 0) continue;
+    for (int k = 0; k < kC2; ++k) {
+      g[kW3 + c * kC2 + k] += scale * df[c] * F.h2[(size_t)p * kC2 + k];
+      dh2[(size_t)p * kC2 + k] += W[kW3 + c * kC2 + k] * df[c];
+    }
+    g[kB3 + c] += scale * df[c];
+  }
+  for (int p = 0; p < m; ++p)
+    for (int j = 0; j < kC2; ++j) {
+      const double dz = F.h2[(size_t)p * kC2 + j] > 0.0 ? dh2[(size_t)p * kC2 + j] : 0.0;
+      if (dz == 0.0) continue;
+      for (int k = 0; k < kC1; ++k) {
+        g[kW2 + j * kC1 + k] += scale * dz * F.h1[(size_t)p * kC1 + k];
+        dh1[(size_t)p * kC1 + k] += W[kW2 + j * kC1 + k] * dz;
+      }
+      g[kB2 + j] += scale * dz;
+    }
+  for (int p = 0; p < m; ++p)
+    for (int k = 0; k < kC1; ++k) {
+      const double dz = F.h1[(size_t)p * kC1 + k] > 0.0 ? dh1[(size_t)p * kC1 + k] : 0.0;
+      for (int d = 0; d < 3; ++d) g[kW1 + 3 * k + d] += scale * dz * pr.x[3 * p + d];
+      g[kB1 + k] += scale * dz;
+    }
+}
+
+std::vector<double> weights(const float* params, bool emulate) {
+  std::vector<double> W(kNP);
+  for (int i = 0; i < kNP; ++i) W[i] = emulate ? (double)bf16(params[i]) : (double)params[i];
+  return W;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_pn_param_count() { return kNP; }
+
+// Forward of each cluster: out [S][7], f [S][256], fidx [S][256] (nullable),
+// proposal [S][8] in the world frame (nullable).
+void oracle_pn_forward(const float* params, const float* xyz, const int32_t* offsets, int S,
+                       int emulate, double* out, double* f, int32_t* fidx, double* proposal) {
+  const std::vector<double> W = weights(params, emulate != 0);
+  for (int c = 0; c < S; ++c) {
+    const Prep pr = prepare(xyz + 3 * (size_t)offsets[c], offsets[c + 1] - offsets[c]);
+    Fwd F;
+    forward(W, pr, emulate != 0, F);
+    for (int k = 0; k < kOut; ++k) out[kOut * (size_t)c + k] = F.out[k];
+    if (f)
+      for (int k = 0; k < kC3; ++k) f[kC3 * (size_t)c + k] = F.f[k];
+    if (fidx)
+      for (int k = 0; k < kC3; ++k) fidx[kC3 * (size_t)c + k] = F.fidx[k];
+    if (proposal) {
+      double* q = proposal + 8 * (size_t)c;
+      const double nv = std::sqrt(F.out[0] * F.out[0] + F.out[1] * F.out[1] + F.out[2] * F.out[2]);
+      for (int k = 0; k < 3; ++k) {
+        q[k] = nv > 0 ? F.out[k] / nv : 0.0;
+        q[3 + k] = pr.cen[k] + F.out[3 + k] / pr.s;
+      }
+      q[6] = F.out[6] / pr.s;
+      q[7] = (pr.m >= 3 && nv > 0 && q[6] > 0) ? 1.0 : 0.0;
+    }
+  }
+}
+
+// Per-cluster losses and the batch-mean gradient (FP64).
+void oracle_pn_loss_grad(const float* params, const float* xyz, const int32_t* offsets, int S,
+                         const double* targets, const float* wts, int emulate, double* loss_out,
+                         double* grad) {
+  const std::vector<double> W = weights(params, emulate != 0);
+  std::vector<double> g(kNP, 0.0);
+  for (int c = 0; c < S; ++c) {
+    const Prep pr = prepare(xyz + 3 * (size_t)offsets[c], offsets[c + 1] - offsets[c]);
+    Fwd F;
+    forward(W, pr, emulate != 0, F);
+    double d[kOut];
+    loss_out[c] = loss(pr, targets + 8 * (size_t)c, F.out, wts, d);
+    backward(W, pr, F, d, 1.0 / S, g);
+  }
+  for (int i = 0; i < kNP; ++i) grad[i] = g[i];
+}
+
+}  // extern "C"

@@ -51,6 +51,24 @@ class PlaneFitCfg(C.Structure):
     ]
 
 
+class PnTrainCfg(C.Structure):
+    _fields_ = [
+        ("lr", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("w_axis", C.c_float),
+        ("w_centre", C.c_float),
+        ("w_radius", C.c_float),
+        ("reserved", C.c_int32),
+    ]
+
+
+PN_PARAMS = 75399
+PN_POINTS = 512
+PN_PROPOSAL_STRIDE = 8
+
+
 class TrellisCfg(C.Structure):
     _fields_ = [
         ("cluster_dist", C.c_float),
@@ -129,6 +147,12 @@ PROTOTYPES = {
     "sloam_fit_lines": (C.c_int, [_P, _P, _P, _I32, C.POINTER(LineFitCfg), _P, _P, _P]),
     "sloam_trellis": (C.c_int, [_P, _P, _I32, _I32, C.POINTER(TrellisCfg), _P, _I32, _I32, _I32,
                                 _P, _P, _P, _P, _P, _P]),
+    "sloam_cyl_plan_exec_ex": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sloam_cyl_fallback": (C.c_int, [_P, _P, _I32, _P, C.c_float, _P, _P]),
+    "sloam_pn_infer": (C.c_int, [_P, _P, _P, _P, _I32, _P]),
+    "sloam_pn_loss_grad": (C.c_int, [_P, _P, _P, _P, _I32, _P, C.POINTER(PnTrainCfg), _P, _P]),
+    "sloam_pn_train_step": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(PnTrainCfg), _P, _P, _I32,
+                                      _P, _P]),
     "sloam_plane_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }
 

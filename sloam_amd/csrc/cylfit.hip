@@ -65,22 +65,12 @@ __device__ inline void local_uvs(const CylFrame& f, float x, float y, float z, f
   *s = uu * uu + vv * vv;
 }
 
-// Circumcircle of three points of the local plane and the predicate parameters.
-__device__ inline HypOut make_hyp(float ua, float va, float ub, float vb, float uc, float vc,
-                                  float thr, float max_radius) {
+// Predicate parameters of the circle (cu, cv, r) of the local plane.
+__device__ inline HypOut hyp_from_circle(float cu, float cv, float r, float thr,
+                                         float max_radius) {
   HypOut h;
-  const float bx = ub - ua;
-  const float by = vb - va;
-  const float cx = uc - ua;
-  const float cy = vc - va;
-  const float d = 2.0f * (bx * cy - by * cx);
-  const float b2 = bx * bx + by * by;
-  const float c2 = cx * cx + cy * cy;
-  const float ox = (cy * b2 - by * c2) / d;
-  const float oy = (bx * c2 - cx * b2) / d;
-  const float r = sqrtf(ox * ox + oy * oy);
-  h.cu = ua + ox;
-  h.cv = va + oy;
+  h.cu = cu;
+  h.cv = cv;
   h.r = r;
   h.valid = (r <= max_radius) ? 1 : 0;  // false for NaN / inf (degenerate triangle)
   if (h.valid) {
@@ -100,9 +90,40 @@ __device__ inline HypOut make_hyp(float ua, float va, float ub, float vb, float 
   return h;
 }
 
+// Circumcircle of three points of the local plane and the predicate parameters.
+__device__ inline HypOut make_hyp(float ua, float va, float ub, float vb, float uc, float vc,
+                                  float thr, float max_radius) {
+  const float bx = ub - ua;
+  const float by = vb - va;
+  const float cx = uc - ua;
+  const float cy = vc - va;
+  const float d = 2.0f * (bx * cy - by * cx);
+  const float b2 = bx * bx + by * by;
+  const float c2 = cx * cx + cy * cy;
+  const float ox = (cy * b2 - by * c2) / d;
+  const float oy = (bx * c2 - cx * b2) / d;
+  const float r = sqrtf(ox * ox + oy * oy);
+  return hyp_from_circle(ua + ox, va + oy, r, thr, max_radius);
+}
+
+// A proposed circle (centre in the local frame, radius) as a hypothesis.
+__device__ inline HypOut proposal_hyp(const CylFrame& f, const float* prop, float thr,
+                                      float max_radius) {
+  float cu, cv, s;
+  local_uvs(f, prop[3], prop[4], prop[5], &cu, &cv, &s);
+  return hyp_from_circle(cu, cv, prop[6], thr, max_radius);
+}
+
+// Hypothesis h of cluster c: the proposal for h == 0 when the cluster has a
+// valid one (prop_c != nullptr), else the circumcircle of a Philox sample.
 __device__ inline HypOut hyp_for(const float* __restrict__ P, int n, int c, int h,
                                  const CylFrame& f, float thr, float max_radius, uint32_t k0,
-                                 uint32_t k1, uint32_t tag, int32_t idx[3]) {
+                                 uint32_t k1, uint32_t tag, int32_t idx[3],
+                                 const float* prop_c) {
+  if (prop_c && h == 0) {
+    idx[0] = idx[1] = idx[2] = -1;
+    return proposal_hyp(f, prop_c, thr, max_radius);
+  }
   SlU4 ctr = {(uint32_t)h, (uint32_t)c, tag, 0u};
   const SlU4 x = sl_philox4x32_10(ctr, k0, k1);
   sl_draw3(x.x, x.y, x.z, (uint32_t)n, &idx[0], &idx[1], &idx[2]);
@@ -123,14 +144,26 @@ __device__ inline bool is_inlier(float u, float v, float s, const HypOut& h) {
 // ---------------------------------------------------------------------------
 // seed: bounding-box centre, integer covariance, FP64 Jacobi, basis
 // ---------------------------------------------------------------------------
+// prop (nullable): proposals (SLOAM_PN_PROPOSAL_STRIDE floats per cluster);
+// a valid proposal's axis replaces the PCA axis.  active (nullable): clusters
+// with active[c] == 0 are skipped.
+__device__ inline const float* cluster_prop(const float* prop, int c) {
+  if (!prop) return nullptr;
+  const float* q = prop + (size_t)c * SLOAM_PN_PROPOSAL_STRIDE;
+  return q[7] > 0.0f ? q : nullptr;
+}
+
 __global__ __launch_bounds__(kBlock) void cyl_seed_kernel(const float* __restrict__ xyz,
                                                           const int32_t* __restrict__ offsets,
                                                           int n_clusters,
+                                                          const float* __restrict__ prop,
+                                                          const int32_t* __restrict__ active,
                                                           CylFrame* __restrict__ frames) {
   __shared__ float red_f[kWaves][6];
   __shared__ long long red_i[kWaves][9];
   const int c = blockIdx.x;
   if (c >= n_clusters) return;
+  if (active && !active[c]) return;
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
   const float* P = xyz + 3 * (size_t)off;
@@ -213,7 +246,14 @@ __global__ __launch_bounds__(kBlock) void cyl_seed_kernel(const float* __restric
   f.n = n;
   f.reserved = 0.0;
   double axis[3] = {0.0, 0.0, 1.0};
-  if (n >= 2) {
+  const float* pc = cluster_prop(prop, c);
+  if (pc) {
+    const double ax = pc[0], ay = pc[1], az = pc[2];
+    const double an = sqrt(ax * ax + ay * ay + az * az);
+    axis[0] = ax / an;
+    axis[1] = ay / an;
+    axis[2] = az / an;
+  } else if (n >= 2) {
     const double dn = (double)n;
     const double S0 = (double)t[0], S1 = (double)t[1], S2 = (double)t[2];
     double a6[6];
@@ -246,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void cyl_score_kernel(
     const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
     const CylFrame* __restrict__ frames, const int4* __restrict__ work, int n_hyp, int hcap,
     int tile, float thr, float max_radius, uint32_t k0, uint32_t k1, uint32_t tag,
+    const float* __restrict__ prop, const int32_t* __restrict__ active,
     int32_t* __restrict__ counts, int32_t* __restrict__ samples) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float4* hp = reinterpret_cast<float4*>(smem_raw);
@@ -256,6 +297,8 @@ __global__ __launch_bounds__(kBlock) void cyl_score_kernel(
 
   const int4 w = work[blockIdx.x];
   const int c = w.x, h0 = w.y, hc = w.z;
+  if (active && !active[c]) return;
+  const float* pc = cluster_prop(prop, c);
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
   const float* P = xyz + 3 * (size_t)off;
@@ -265,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void cyl_score_kernel(
     float4 prm = make_float4(0.f, 0.f, INFINITY, -INFINITY);
     if (t < hc && n >= 3) {
       int32_t idx[3];
-      const HypOut h = hyp_for(P, n, c, h0 + t, f, thr, max_radius, k0, k1, tag, idx);
+      const HypOut h = hyp_for(P, n, c, h0 + t, f, thr, max_radius, k0, k1, tag, idx, pc);
       prm = make_float4(h.A, h.B, h.L, h.U);
       if (samples) {
         int32_t* o = samples + 3 * ((size_t)c * n_hyp + h0 + t);
@@ -442,14 +485,16 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
     const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
     const CylFrame* __restrict__ frames, const int32_t* __restrict__ counts,
     const int32_t* __restrict__ list, int n_hyp, float thr, float max_radius, uint32_t k0,
-    uint32_t k1, uint32_t tag, int refit_rounds, int refit_iters, uint32_t* __restrict__ mask,
-    int32_t* __restrict__ best_out, double* __restrict__ model) {
+    uint32_t k1, uint32_t tag, int refit_rounds, int refit_iters,
+    const float* __restrict__ prop, const int32_t* __restrict__ active,
+    uint32_t* __restrict__ mask, int32_t* __restrict__ best_out, double* __restrict__ model) {
   constexpr int W = BLOCK / 64;
   __shared__ double red[W * 20];
   __shared__ long long redk[W];
   __shared__ HypOut best_h;
   extern __shared__ uint32_t bits[];  // membership of the current round
   const int c = list[blockIdx.x];
+  if (active && !active[c]) return;
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
   const float* P = xyz + 3 * (size_t)off;
@@ -481,7 +526,8 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
   if (threadIdx.x == 0) {
     if (best >= 0) {
       int32_t idx[3];
-      best_h = hyp_for(P, n, c, best, f, thr, max_radius, k0, k1, tag, idx);
+      best_h = hyp_for(P, n, c, best, f, thr, max_radius, k0, k1, tag, idx,
+                       cluster_prop(prop, c));
     } else {
       best_h.valid = 0;
       best_h.A = best_h.B = 0.f;
@@ -699,6 +745,26 @@ struct sloam_cyl_plan {
 
 static int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// active[c] = 1 when cluster c's model keeps fewer than min_ratio of its
+// points as inliers (or is invalid); *n_active counts them
+__global__ __launch_bounds__(256) void cyl_fallback_kernel(const int32_t* __restrict__ offsets,
+                                                           int n_clusters,
+                                                           const double* __restrict__ model,
+                                                           float min_ratio,
+                                                           int32_t* __restrict__ active,
+                                                           int32_t* __restrict__ n_active) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  bool a = false;
+  if (c < n_clusters) {
+    const int n = offsets[c + 1] - offsets[c];
+    const double* m = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
+    a = n >= 3 && !(m[6] > 0.0 && m[7] >= (double)min_ratio * (double)n);
+    active[c] = a ? 1 : 0;
+  }
+  const uint64_t b = __builtin_amdgcn_ballot_w64(a);
+  if ((threadIdx.x & 63) == 0 && b && n_active) atomicAdd(n_active, __builtin_popcountll(b));
+}
+
 extern "C" {
 
 int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
@@ -801,6 +867,27 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
   return SLOAM_OK;
 }
 
+int sloam_cyl_fallback(sloam_ctx* ctx, const int32_t* d_offsets, int32_t n_clusters,
+                       const double* d_model, float min_inlier_ratio, int32_t* d_active,
+                       int32_t* d_n_active) {
+  if (!ctx || n_clusters < 0 || (n_clusters > 0 && (!d_offsets || !d_model || !d_active)))
+    return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  if (d_n_active) {
+    int st = sloam_fill_u32(ctx, (uint32_t*)d_n_active, 1, 0u);
+    if (st) return st;
+  }
+  if (n_clusters == 0) return SLOAM_OK;
+  {
+    SloamKernelScope ks(ctx, "cyl_fallback");
+    hipLaunchKernelGGL(cyl_fallback_kernel, dim3((n_clusters + 255) / 256), dim3(256), 0,
+                       ctx->stream, d_offsets, n_clusters, d_model, min_inlier_ratio, d_active,
+                       d_n_active);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
 void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
   if (!p) return;
   sloam_dev_free(p->d_work);
@@ -813,7 +900,16 @@ void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
 int sloam_cyl_plan_exec(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_offsets,
                         int32_t* d_counts, int32_t* d_samples, uint32_t* d_mask, int32_t* d_best,
                         double* d_model) {
+  return sloam_cyl_plan_exec_ex(p, d_xyz, d_offsets, nullptr, nullptr, d_counts, d_samples, d_mask,
+                                d_best, d_model);
+}
+
+int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_offsets,
+                           const float* d_proposal, const int32_t* d_active, int32_t* d_counts,
+                           int32_t* d_samples, uint32_t* d_mask, int32_t* d_best,
+                           double* d_model) {
   if (!p || !d_offsets || !d_model || (p->n_points > 0 && !d_xyz)) return SLOAM_ERR_INVALID_ARG;
+  if (d_active && d_mask) return SLOAM_ERR_INVALID_ARG;  // the mask is cleared for all clusters
   sloam_ctx* ctx = p->ctx;
   if (p->n_clusters == 0) return SLOAM_OK;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
@@ -829,7 +925,7 @@ int sloam_cyl_plan_exec(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
   {
     SloamKernelScope ks(ctx, "cyl_seed");
     hipLaunchKernelGGL(cyl_seed_kernel, dim3(p->n_clusters), dim3(kBlock), 0, ctx->stream, d_xyz,
-                       d_offsets, p->n_clusters, p->d_frames);
+                       d_offsets, p->n_clusters, d_proposal, d_active, p->d_frames);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (p->n_work > 0) {
@@ -837,7 +933,7 @@ int sloam_cyl_plan_exec(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
     SloamKernelScope ks(ctx, "cyl_score");
     hipLaunchKernelGGL(cyl_score_kernel, dim3(p->n_work), dim3(kBlock), lds, ctx->stream, d_xyz,
                        d_offsets, p->d_frames, p->d_work, cfg.n_hyp, p->hcap, p->tile, cfg.thr,
-                       cfg.max_radius, k0, k1, tag, counts, d_samples);
+                       cfg.max_radius, k0, k1, tag, d_proposal, d_active, counts, d_samples);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {
@@ -846,12 +942,13 @@ int sloam_cyl_plan_exec(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
       hipLaunchKernelGGL(cyl_refit_kernel<64>, dim3(p->n_small), dim3(64), p->bits_small,
                          ctx->stream, d_xyz, d_offsets, p->d_frames, counts, p->d_lists,
                          cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, cfg.refit_rounds,
-                         cfg.refit_iters, d_mask, d_best, d_model);
+                         cfg.refit_iters, d_proposal, d_active, d_mask, d_best, d_model);
     if (p->n_large > 0)
       hipLaunchKernelGGL(cyl_refit_kernel<256>, dim3(p->n_large), dim3(256), p->bits_large,
                          ctx->stream, d_xyz, d_offsets, p->d_frames, counts,
                          p->d_lists + p->n_small, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1,
-                         tag, cfg.refit_rounds, cfg.refit_iters, d_mask, d_best, d_model);
+                         tag, cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, d_mask,
+                         d_best, d_model);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;

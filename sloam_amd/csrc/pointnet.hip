@@ -1,0 +1,696 @@
+// Learned cylinder proposal for gfx950: a PointNet over each cluster, BF16
+// weights on MFMA with FP32 accumulation.
+//
+//   pn_prep     one workgroup per cluster: up to 512 points at indices
+//               floor(j n / m), bounding-box centre and scale, normalised
+//               float4 points (w = 1 real, 0 padding)
+//   pn_fwd      one workgroup (4 waves) per cluster, 64 points at a time:
+//               layer 1 (3 -> 64) on the VALU, layers 2 (64 -> 128) and
+//               3 (128 -> 256) on v_mfma_f32_16x16x32_bf16 with the BF16
+//               activations in LDS and each wave's weight fragments in VGPRs
+//               (wave w owns output columns 32w.. of layer 2 and 64w.. of
+//               layer 3), masked max-pool with the first maximising point
+//               kept per channel, head 256 -> 128 -> 7 on the VALU
+//   pn_bwd      one workgroup per cluster: loss, head gradients, and the
+//               max-pool gradient routed to its argmax points only (at most
+//               256 of them), back through layers 3, 2, 1 from the
+//               activations the forward pass saved
+//   pn_reduce   batch mean of the per-cluster gradients (fixed order)
+//   pn_adam     Adam update of the FP32 master weights
+#include "common.hpp"
+
+#include <cmath>
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kP = SLOAM_PN_POINTS;  // points per cluster
+constexpr int kC1 = 64, kC2 = 128, kC3 = 256, kC4 = 128, kOut = 7;
+constexpr int kChunk = 64;           // points per pass of the forward kernel
+constexpr int kS1 = kC1 + 8;         // LDS row strides (bf16), 16-byte multiples
+constexpr int kS2 = kC2 + 8;
+constexpr int kFB = 256;             // forward / backward workgroup
+
+// parameter offsets (weight [out][in], then bias)
+constexpr int kW1 = 0, kB1 = kW1 + kC1 * 3;
+constexpr int kW2 = kB1 + kC1, kB2 = kW2 + kC2 * kC1;
+constexpr int kW3 = kB2 + kC2, kB3 = kW3 + kC3 * kC2;
+constexpr int kW4 = kB3 + kC3, kB4 = kW4 + kC4 * kC3;
+constexpr int kW5 = kB4 + kC4, kB5 = kW5 + kOut * kC4;
+constexpr int kNP = kB5 + kOut;
+static_assert(kNP == SLOAM_PN_PARAMS, "parameter layout");
+
+// saved activations per cluster (floats): f[256], argmax[256] (int bits),
+// h4[128], out[8]
+constexpr int kAF = 0, kAI = kAF + kC3, kAH = kAI + kC3, kAO = kAH + kC4, kAct = kAO + 8;
+
+struct PnMeta {
+  float cx, cy, cz, s;
+  int m, n, pad0, pad1;
+};
+
+struct LossCfg {
+  float wa, wc, wr, inv_batch;
+};
+
+// round to nearest even
+__device__ inline unsigned short bf_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf_round(float x) { return __uint_as_float((unsigned)bf_bits(x) << 16); }
+__device__ inline float bf_val(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// 8 consecutive FP32 weights -> BF16 fragment
+__device__ inline bf16x8 frag_from_f32(const float* p) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  const float4 b = *reinterpret_cast<const float4*>(p + 4);
+  u32x4 u;
+  u.x = (unsigned)bf_bits(a.x) | ((unsigned)bf_bits(a.y) << 16);
+  u.y = (unsigned)bf_bits(a.z) | ((unsigned)bf_bits(a.w) << 16);
+  u.z = (unsigned)bf_bits(b.x) | ((unsigned)bf_bits(b.y) << 16);
+  u.w = (unsigned)bf_bits(b.z) | ((unsigned)bf_bits(b.w) << 16);
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+// 8 consecutive BF16 activations from LDS (16-byte aligned)
+__device__ inline bf16x8 frag_from_lds(const unsigned short* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ xyz,
+                                                      const int32_t* __restrict__ offsets,
+                                                      int n_clusters, float4* __restrict__ X,
+                                                      PnMeta* __restrict__ meta) {
+  __shared__ float red[kFB / 64][6];
+  const int c = blockIdx.x;
+  if (c >= n_clusters) return;
+  const int off = offsets[c];
+  const int n = offsets[c + 1] - off;
+  const int m = n < kP ? n : kP;
+  const float* P = xyz + 3 * (size_t)off;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int j = threadIdx.x; j < m; j += kFB) {
+    const int i = (int)(((long long)j * n) / m);
+    for (int k = 0; k < 3; ++k) {
+      const float q = P[3 * (size_t)i + k];
+      lo[k] = fminf(lo[k], q);
+      hi[k] = fmaxf(hi[k], q);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(lo[k], __shfl_xor(lo[k], o, 64));
+      hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o, 64));
+    }
+  if (lane == 0)
+    for (int k = 0; k < 3; ++k) {
+      red[wave][k] = lo[k];
+      red[wave][3 + k] = hi[k];
+    }
+  __syncthreads();
+  float cen[3], ext = 1e-3f;
+  for (int k = 0; k < 3; ++k) {
+    float a = red[0][k], b = red[0][3 + k];
+    for (int w = 1; w < kFB / 64; ++w) {
+      a = fminf(a, red[w][k]);
+      b = fmaxf(b, red[w][3 + k]);
+    }
+    cen[k] = m > 0 ? 0.5f * (a + b) : 0.0f;
+    if (m > 0) ext = fmaxf(ext, 0.5f * (b - a));
+  }
+  const float s = 1.0f / ext;
+  float4* Xc = X + (size_t)c * kP;
+  for (int j = threadIdx.x; j < kP; j += kFB) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j < m) {
+      const int i = (int)(((long long)j * n) / m);
+      v = make_float4((P[3 * (size_t)i] - cen[0]) * s, (P[3 * (size_t)i + 1] - cen[1]) * s,
+                      (P[3 * (size_t)i + 2] - cen[2]) * s, 1.0f);
+    }
+    Xc[j] = v;
+  }
+  if (threadIdx.x == 0) {
+    PnMeta mt;
+    mt.cx = cen[0];
+    mt.cy = cen[1];
+    mt.cz = cen[2];
+    mt.s = s;
+    mt.m = m;
+    mt.n = n;
+    mt.pad0 = mt.pad1 = 0;
+    meta[c] = mt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// forward.  act (nullable): saved f / argmax / h4 / out; H1, H2 (nullable):
+// the BF16 activations of layers 1 and 2 of every point; prop (nullable):
+// proposal records in the world frame.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFB) void pn_fwd_kernel(
+    const float* __restrict__ prm, const float4* __restrict__ X, const PnMeta* __restrict__ meta,
+    int n_clusters, float* __restrict__ act, unsigned short* __restrict__ H1,
+    unsigned short* __restrict__ H2, float* __restrict__ prop) {
+  __shared__ __align__(16) unsigned short h1s[kChunk * kS1];
+  __shared__ __align__(16) unsigned short h2s[kChunk * kS2];
+  __shared__ float fsh[kC3];
+  __shared__ int fidx[kC3];
+  __shared__ float h4s[kC4];
+  __shared__ float outs[8];
+  __shared__ float4 w1s[kC1];
+  const int c = blockIdx.x;
+  if (c >= n_clusters) return;
+  const PnMeta mt = meta[c];
+  const int m = mt.m;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  // weight fragments of this wave (B operands: lane -> column lr, k = 8 lg + e)
+  bf16x8 w2f[2][2], w3f[4][4];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int j = 32 * wave + 16 * ct + lr;
+      w2f[ct][ks] = frag_from_f32(prm + kW2 + j * kC1 + 32 * ks + 8 * lg);
+    }
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int j = 64 * wave + 16 * ct + lr;
+      w3f[ct][ks] = frag_from_f32(prm + kW3 + j * kC2 + 32 * ks + 8 * lg);
+    }
+  float b2v[2], b3v[4];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) b2v[ct] = bf_round(prm[kB2 + 32 * wave + 16 * ct + lr]);
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) b3v[ct] = bf_round(prm[kB3 + 64 * wave + 16 * ct + lr]);
+  // layer-1 weights (w, bias) per channel
+  const int g1 = t & 3;
+  if (t < kC1)
+    w1s[t] = make_float4(bf_round(prm[kW1 + 3 * t]), bf_round(prm[kW1 + 3 * t + 1]),
+                         bf_round(prm[kW1 + 3 * t + 2]), bf_round(prm[kB1 + t]));
+  __syncthreads();
+
+  float best[4] = {0.f, 0.f, 0.f, 0.f};
+  int bidx[4] = {-1, -1, -1, -1};
+  const float4* Xc = X + (size_t)c * kP;
+  for (int base = 0; base < m; base += kChunk) {
+    // layer 1: thread -> point t / 4, channels 16 (t % 4) ..
+    {
+      const int i = t >> 2;
+      const float4 x = Xc[base + i];
+#pragma unroll 4
+      for (int q = 0; q < 16; ++q) {
+        const float4 w = w1s[16 * g1 + q];
+        const float z = ((w.w + w.x * x.x) + w.y * x.y) + w.z * x.z;
+        const unsigned short h = bf_bits(z > 0.f ? z : 0.f);
+        h1s[i * kS1 + 16 * g1 + q] = h;
+        if (H1) H1[((size_t)c * kP + base + i) * kC1 + 16 * g1 + q] = h;
+      }
+    }
+    __syncthreads();
+    // layer 2: 64 points x this wave's 32 channels
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      f32x4 acc[2];
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        const bf16x8 a = frag_from_lds(h1s + (16 * rt + lr) * kS1 + 32 * ks + 8 * lg);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w2f[ct][ks], acc[ct], 0, 0, 0);
+      }
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const int j = 32 * wave + 16 * ct + lr;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = 16 * rt + 4 * lg + e;
+          const float z = acc[ct][e] + b2v[ct];
+          const unsigned short h = bf_bits(z > 0.f ? z : 0.f);
+          h2s[i * kS2 + j] = h;
+          if (H2) H2[((size_t)c * kP + base + i) * kC2 + j] = h;
+        }
+      }
+    }
+    __syncthreads();
+    // layer 3 + max-pool: 64 points x this wave's 64 channels
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      f32x4 acc[4];
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 a = frag_from_lds(h2s + (16 * rt + lr) * kS2 + 32 * ks + 8 * lg);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w3f[ct][ks], acc[ct], 0, 0, 0);
+      }
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int p = base + 16 * rt + 4 * lg + e;
+          const float y = acc[ct][e] + b3v[ct];
+          if (p < m && y > best[ct]) {
+            best[ct] = y;
+            bidx[ct] = p;
+          }
+        }
+    }
+    __syncthreads();
+  }
+  // max over the lanes holding the same channel (groups lg), first point on ties
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) {
+    float v = best[ct];
+    int ix = bidx[ct];
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      const float v2 = __shfl_xor(v, o, 64);
+      const int i2 = __shfl_xor(ix, o, 64);
+      if (v2 > v || (v2 == v && i2 >= 0 && (ix < 0 || i2 < ix))) {
+        v = v2;
+        ix = i2;
+      }
+    }
+    if (lg == 0) {
+      const int ch = 64 * wave + 16 * ct + lr;
+      fsh[ch] = v;
+      fidx[ch] = ix;
+    }
+  }
+  __syncthreads();
+  // head: 256 -> 128 (ReLU) -> 7
+  if (t < kC4) {
+    float z = bf_round(prm[kB4 + t]);
+    const float* w = prm + kW4 + t * kC3;
+    for (int k = 0; k < kC3; ++k) z += bf_round(w[k]) * fsh[k];
+    h4s[t] = z > 0.f ? z : 0.f;
+  }
+  __syncthreads();
+  if (t < kOut) {
+    float z = bf_round(prm[kB5 + t]);
+    const float* w = prm + kW5 + t * kC4;
+    for (int k = 0; k < kC4; ++k) z += bf_round(w[k]) * h4s[k];
+    outs[t] = z;
+  }
+  __syncthreads();
+  if (act) {
+    float* a = act + (size_t)c * kAct;
+    for (int k = t; k < kC3; k += kFB) {
+      a[kAF + k] = fsh[k];
+      a[kAI + k] = __int_as_float(fidx[k]);
+    }
+    if (t < kC4) a[kAH + t] = h4s[t];
+    if (t < 8) a[kAO + t] = t < kOut ? outs[t] : 0.f;
+  }
+  if (prop && t == 0) {
+    float* pr = prop + (size_t)c * SLOAM_PN_PROPOSAL_STRIDE;
+    const float vx = outs[0], vy = outs[1], vz = outs[2];
+    const float nv = sqrtf(vx * vx + vy * vy + vz * vz);
+    const float inv = 1.0f / mt.s;
+    const float r = outs[6] * inv;
+    const bool ok = m >= 3 && nv > 0.f && r > 0.f && isfinite(r) && isfinite(nv);
+    pr[0] = ok ? vx / nv : 0.f;
+    pr[1] = ok ? vy / nv : 0.f;
+    pr[2] = ok ? vz / nv : 1.f;
+    pr[3] = mt.cx + outs[3] * inv;
+    pr[4] = mt.cy + outs[4] * inv;
+    pr[5] = mt.cz + outs[5] * inv;
+    pr[6] = ok ? r : 0.f;
+    pr[7] = ok ? 1.f : 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward.  G: per-cluster gradients [B][kNP]; dz2g / dz1g: per-cluster
+// scratch [B][256][128] / [B][256][64] for the active points' gradients.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFB) void pn_bwd_kernel(
+    const float* __restrict__ prm, const float4* __restrict__ X, const PnMeta* __restrict__ meta,
+    const float* __restrict__ act, const unsigned short* __restrict__ H1,
+    const unsigned short* __restrict__ H2, const double* __restrict__ targets, LossCfg lc,
+    int n_clusters, float* __restrict__ G, float* __restrict__ dz2g, float* __restrict__ dz1g,
+    float* __restrict__ loss_out) {
+  __shared__ float f[kC3], dz3[kC3], h4[kC4], dz4[kC4], dout[8];
+  __shared__ int fidx[kC3];
+  __shared__ int slot_of[kP];
+  __shared__ int plist[kC3];
+  __shared__ int ccount[kC3 + 1], cstart[kC3 + 1], corder[kC3];
+  __shared__ int nact;
+  const int c = blockIdx.x;
+  if (c >= n_clusters) return;
+  const int t = threadIdx.x;
+  const float* a = act + (size_t)c * kAct;
+  float* Gc = G + (size_t)c * kNP;
+  const unsigned short* H1c = H1 + (size_t)c * kP * kC1;
+  const unsigned short* H2c = H2 + (size_t)c * kP * kC2;
+  const float4* Xc = X + (size_t)c * kP;
+  float* dz2 = dz2g + (size_t)c * kC3 * kC2;
+  float* dz1 = dz1g + (size_t)c * kC3 * kC1;
+  for (int k = t; k < kC3; k += kFB) {
+    f[k] = a[kAF + k];
+    fidx[k] = __float_as_int(a[kAI + k]);
+  }
+  if (t < kC4) h4[t] = a[kAH + t];
+  for (int p = t; p < kP; p += kFB) slot_of[p] = -1;
+  if (t <= kC3) ccount[t] = 0;
+  __syncthreads();
+  // loss and its gradient w.r.t. the 7 outputs
+  if (t == 0) {
+    const PnMeta mt = meta[c];
+    const double* tg = targets + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
+    float o[7];
+    for (int k = 0; k < 7; ++k) o[k] = a[kAO + k];
+    float L = 0.f;
+    float d[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (tg[6] > 0.0 && mt.m >= 3) {
+      const double an = sqrt(tg[3] * tg[3] + tg[4] * tg[4] + tg[5] * tg[5]);
+      const double ax = tg[3] / an, ay = tg[4] / an, az = tg[5] / an;
+      const double rx = (tg[0] - mt.cx) * mt.s, ry = (tg[1] - mt.cy) * mt.s,
+                   rz = (tg[2] - mt.cz) * mt.s;
+      const double pr = rx * ax + ry * ay + rz * az;
+      const float cs[3] = {(float)(rx - pr * ax), (float)(ry - pr * ay), (float)(rz - pr * az)};
+      const float A[3] = {(float)ax, (float)ay, (float)az};
+      const float rs = (float)(tg[6] * mt.s);
+      const float nv = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
+      const float ah[3] = {o[0] / nv, o[1] / nv, o[2] / nv};
+      const float dd = ah[0] * A[0] + ah[1] * A[1] + ah[2] * A[2];
+      L = lc.wa * (1.f - dd * dd);
+      for (int k = 0; k < 3; ++k) d[k] = lc.wa * (-2.f * dd * (A[k] - dd * ah[k]) / nv);
+      for (int k = 0; k < 3; ++k) {
+        const float e = o[3 + k] - cs[k];
+        L += lc.wc * e * e;
+        d[3 + k] = lc.wc * 2.f * e;
+      }
+      const float er = o[6] - rs;
+      L += lc.wr * er * er;
+      d[6] = lc.wr * 2.f * er;
+    }
+    for (int k = 0; k < 7; ++k) dout[k] = d[k] * lc.inv_batch;
+    dout[7] = 0.f;
+    if (loss_out) loss_out[c] = L;
+  }
+  __syncthreads();
+  // head: W5, b5, then dz4 = relu'(h4) * W5^T dout
+  for (int i = t; i < kOut * kC4; i += kFB) Gc[kW5 + i] = dout[i / kC4] * h4[i % kC4];
+  if (t < kOut) Gc[kB5 + t] = dout[t];
+  if (t < kC4) {
+    float s = 0.f;
+    for (int o = 0; o < kOut; ++o) s += bf_round(prm[kW5 + o * kC4 + t]) * dout[o];
+    dz4[t] = h4[t] > 0.f ? s : 0.f;
+  }
+  __syncthreads();
+  for (int i = t; i < kC4 * kC3; i += kFB) Gc[kW4 + i] = dz4[i / kC3] * f[i % kC3];
+  if (t < kC4) Gc[kB4 + t] = dz4[t];
+  // df, dz3 (the max-pool passes the gradient to the argmax point)
+  for (int k = t; k < kC3; k += kFB) {
+    float s = 0.f;
+    for (int o = 0; o < kC4; ++o) s += bf_round(prm[kW4 + o * kC3 + k]) * dz4[o];
+    dz3[k] = fidx[k] >= 0 ? s : 0.f;
+  }
+  __syncthreads();
+  // active points, in point order, and each one's channels in channel order
+  for (int k = t; k < kC3; k += kFB)
+    if (fidx[k] >= 0) slot_of[fidx[k]] = 0;
+  __syncthreads();
+  if (t == 0) {
+    int s = 0;
+    for (int p = 0; p < kP; ++p)
+      if (slot_of[p] == 0) {
+        slot_of[p] = s;
+        plist[s] = p;
+        ++s;
+      } else {
+        slot_of[p] = -1;
+      }
+    nact = s;
+  }
+  __syncthreads();
+  const int P = nact;
+  for (int k = t; k < kC3; k += kFB)
+    if (fidx[k] >= 0) atomicAdd(&ccount[slot_of[fidx[k]]], 1);
+  __syncthreads();
+  if (t == 0) {
+    int s = 0;
+    for (int q = 0; q < P; ++q) {
+      cstart[q] = s;
+      s += ccount[q];
+    }
+    cstart[P] = s;
+  }
+  __syncthreads();
+  for (int k = t; k < kC3; k += kFB) {
+    if (fidx[k] < 0) continue;
+    int before = 0;
+    for (int k2 = 0; k2 < k; ++k2) before += (fidx[k2] == fidx[k]) ? 1 : 0;
+    corder[cstart[slot_of[fidx[k]]] + before] = k;
+  }
+  __syncthreads();
+  // layer 3 weights: dW3[ch][k] = dz3[ch] h2[argmax ch][k]
+  for (int i = t; i < kC3 * kC2; i += kFB) {
+    const int ch = i / kC2, k = i % kC2;
+    Gc[kW3 + i] = fidx[ch] >= 0 ? dz3[ch] * bf_val(H2c[(size_t)fidx[ch] * kC2 + k]) : 0.f;
+  }
+  if (t < kC3) Gc[kB3 + t] = dz3[t];
+  // dz2 of the active points
+  for (int i = t; i < P * kC2; i += kFB) {
+    const int q = i / kC2, j = i % kC2;
+    float s = 0.f;
+    for (int r = cstart[q]; r < cstart[q + 1]; ++r) {
+      const int ch = corder[r];
+      s += bf_round(prm[kW3 + ch * kC2 + j]) * dz3[ch];
+    }
+    const float h2 = bf_val(H2c[(size_t)plist[q] * kC2 + j]);
+    dz2[q * kC2 + j] = h2 > 0.f ? s : 0.f;
+  }
+  __syncthreads();
+  // layer 2 weights and dz1
+  for (int i = t; i < kC2 * kC1; i += kFB) {
+    const int j = i / kC1, k = i % kC1;
+    float s = 0.f;
+    for (int q = 0; q < P; ++q) s += dz2[q * kC2 + j] * bf_val(H1c[(size_t)plist[q] * kC1 + k]);
+    Gc[kW2 + i] = s;
+  }
+  if (t < kC2) {
+    float s = 0.f;
+    for (int q = 0; q < P; ++q) s += dz2[q * kC2 + t];
+    Gc[kB2 + t] = s;
+  }
+  for (int i = t; i < P * kC1; i += kFB) {
+    const int q = i / kC1, k = i % kC1;
+    float s = 0.f;
+    for (int j = 0; j < kC2; ++j) s += bf_round(prm[kW2 + j * kC1 + k]) * dz2[q * kC2 + j];
+    const float h1 = bf_val(H1c[(size_t)plist[q] * kC1 + k]);
+    dz1[q * kC1 + k] = h1 > 0.f ? s : 0.f;
+  }
+  __syncthreads();
+  // layer 1
+  if (t < kC1) {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, sb = 0.f;
+    for (int q = 0; q < P; ++q) {
+      const float d = dz1[q * kC1 + t];
+      const float4 x = Xc[plist[q]];
+      s0 += d * x.x;
+      s1 += d * x.y;
+      s2 += d * x.z;
+      sb += d;
+    }
+    Gc[kW1 + 3 * t] = s0;
+    Gc[kW1 + 3 * t + 1] = s1;
+    Gc[kW1 + 3 * t + 2] = s2;
+    Gc[kB1 + t] = sb;
+  }
+}
+
+// sum over the batch of the per-cluster gradients (already scaled by 1/B)
+__global__ __launch_bounds__(256) void pn_reduce_kernel(const float* __restrict__ G, int B,
+                                                        float* __restrict__ grad) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kNP) return;
+  float s = 0.f;
+  for (int c = 0; c < B; ++c) s += G[(size_t)c * kNP + i];
+  grad[i] = s;
+}
+
+__global__ __launch_bounds__(256) void pn_adam_kernel(float* __restrict__ p,
+                                                      float* __restrict__ m1,
+                                                      float* __restrict__ m2,
+                                                      const float* __restrict__ g, float lr,
+                                                      float b1, float b2, float eps, float c1,
+                                                      float c2) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kNP) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi;
+  const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+struct PnBuffers {
+  float4* X;
+  PnMeta* meta;
+  float* act;
+  unsigned short* H1;
+  unsigned short* H2;
+  float* G;
+  float* dz2;
+  float* dz1;
+  float* grad;
+};
+
+int pn_buffers(sloam_ctx* ctx, int B, bool train, PnBuffers* b) {
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t nB = (size_t)(B > 0 ? B : 1);
+  size_t need = al(sizeof(float4) * kP * nB) + al(sizeof(PnMeta) * nB);
+  if (train)
+    need += al(sizeof(float) * kAct * nB) + al(2 * (size_t)kP * kC1 * nB) +
+            al(2 * (size_t)kP * kC2 * nB) + al(sizeof(float) * kNP * nB) +
+            al(sizeof(float) * kC3 * kC2 * nB) + al(sizeof(float) * kC3 * kC1 * nB) +
+            al(sizeof(float) * kNP);
+  void* base = nullptr;
+  const int st = sloam_ctx_scratch(ctx, 3, need, &base);
+  if (st) return st;
+  char* q = (char*)base;
+  auto take = [&](size_t bytes) {
+    char* r = q;
+    q += al(bytes);
+    return (void*)r;
+  };
+  *b = PnBuffers{};
+  b->X = (float4*)take(sizeof(float4) * kP * nB);
+  b->meta = (PnMeta*)take(sizeof(PnMeta) * nB);
+  if (train) {
+    b->act = (float*)take(sizeof(float) * kAct * nB);
+    b->H1 = (unsigned short*)take(2 * (size_t)kP * kC1 * nB);
+    b->H2 = (unsigned short*)take(2 * (size_t)kP * kC2 * nB);
+    b->G = (float*)take(sizeof(float) * kNP * nB);
+    b->dz2 = (float*)take(sizeof(float) * kC3 * kC2 * nB);
+    b->dz1 = (float*)take(sizeof(float) * kC3 * kC1 * nB);
+    b->grad = (float*)take(sizeof(float) * kNP);
+  }
+  return SLOAM_OK;
+}
+
+int pn_check_cfg(const sloam_pn_train_cfg* cfg) {
+  if (!cfg) return SLOAM_ERR_INVALID_ARG;
+  if (!(cfg->w_axis >= 0.f) || !(cfg->w_centre >= 0.f) || !(cfg->w_radius >= 0.f))
+    return SLOAM_ERR_INVALID_ARG;
+  return SLOAM_OK;
+}
+
+// prep + forward (saving activations) + backward + reduce into b.grad / d_grad
+int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                 const int32_t* d_offsets, int B, const double* d_targets,
+                 const sloam_pn_train_cfg* cfg, float* d_loss, float* d_grad, PnBuffers* b) {
+  int st = pn_buffers(ctx, B, true, b);
+  if (st) return st;
+  LossCfg lc{cfg->w_axis, cfg->w_centre, cfg->w_radius, 1.0f / (float)B};
+  {
+    SloamKernelScope ks(ctx, "pn_prep");
+    hipLaunchKernelGGL(pn_prep_kernel, dim3(B), dim3(kFB), 0, ctx->stream, d_xyz, d_offsets, B,
+                       b->X, b->meta);
+  }
+  {
+    SloamKernelScope ks(ctx, "pn_fwd");
+    hipLaunchKernelGGL(pn_fwd_kernel, dim3(B), dim3(kFB), 0, ctx->stream, d_params, b->X,
+                       b->meta, B, b->act, b->H1, b->H2, (float*)nullptr);
+  }
+  {
+    SloamKernelScope ks(ctx, "pn_bwd");
+    hipLaunchKernelGGL(pn_bwd_kernel, dim3(B), dim3(kFB), 0, ctx->stream, d_params, b->X,
+                       b->meta, b->act, b->H1, b->H2, d_targets, lc, B, b->G, b->dz2, b->dz1,
+                       d_loss);
+  }
+  {
+    SloamKernelScope ks(ctx, "pn_reduce");
+    hipLaunchKernelGGL(pn_reduce_kernel, dim3((kNP + 255) / 256), dim3(256), 0, ctx->stream,
+                       b->G, B, d_grad ? d_grad : b->grad);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                   const int32_t* d_offsets, int32_t n_clusters, float* d_proposal) {
+  if (!ctx || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
+  if (n_clusters == 0) return SLOAM_OK;
+  if (!d_params || !d_xyz || !d_offsets || !d_proposal) return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  PnBuffers b;
+  int st = pn_buffers(ctx, n_clusters, false, &b);
+  if (st) return st;
+  {
+    SloamKernelScope ks(ctx, "pn_prep");
+    hipLaunchKernelGGL(pn_prep_kernel, dim3(n_clusters), dim3(kFB), 0, ctx->stream, d_xyz,
+                       d_offsets, n_clusters, b.X, b.meta);
+  }
+  {
+    SloamKernelScope ks(ctx, "pn_fwd");
+    hipLaunchKernelGGL(pn_fwd_kernel, dim3(n_clusters), dim3(kFB), 0, ctx->stream, d_params, b.X,
+                       b.meta, n_clusters, (float*)nullptr, (unsigned short*)nullptr,
+                       (unsigned short*)nullptr, d_proposal);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
+int sloam_pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                       const int32_t* d_offsets, int32_t n_clusters, const double* d_targets,
+                       const sloam_pn_train_cfg* cfg, float* d_loss, float* d_grad) {
+  if (!ctx || n_clusters <= 0 || !d_params || !d_xyz || !d_offsets || !d_targets || !d_grad)
+    return SLOAM_ERR_INVALID_ARG;
+  int st = pn_check_cfg(cfg);
+  if (st) return st;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  PnBuffers b;
+  return pn_loss_grad(ctx, d_params, d_xyz, d_offsets, n_clusters, d_targets, cfg, d_loss, d_grad,
+                      &b);
+}
+
+int sloam_pn_train_step(sloam_ctx* ctx, float* d_params, float* d_m, float* d_v, int32_t step,
+                        const sloam_pn_train_cfg* cfg, const float* d_xyz,
+                        const int32_t* d_offsets, int32_t n_clusters, const double* d_targets,
+                        float* d_loss) {
+  if (!ctx || n_clusters <= 0 || step < 1 || !d_params || !d_m || !d_v || !d_xyz || !d_offsets ||
+      !d_targets)
+    return SLOAM_ERR_INVALID_ARG;
+  int st = pn_check_cfg(cfg);
+  if (st) return st;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  PnBuffers b;
+  st = pn_loss_grad(ctx, d_params, d_xyz, d_offsets, n_clusters, d_targets, cfg, d_loss, nullptr,
+                    &b);
+  if (st) return st;
+  const float c1 = 1.0f - powf(cfg->beta1, (float)step);
+  const float c2 = 1.0f - powf(cfg->beta2, (float)step);
+  {
+    SloamKernelScope ks(ctx, "pn_adam");
+    hipLaunchKernelGGL(pn_adam_kernel, dim3((kNP + 255) / 256), dim3(256), 0, ctx->stream,
+                       d_params, d_m, d_v, b.grad, cfg->lr, cfg->beta1, cfg->beta2, cfg->eps, c1,
+                       c2);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
+}  // extern "C"

@@ -32,6 +32,7 @@ class Context:
         check(self.lib.sloam_ctx_create(int(device), C.byref(h)))
         self.handle = h
         self.device = device
+        self._keep = []
 
     def close(self):
         if self.handle:
@@ -44,8 +45,15 @@ class Context:
         except Exception:
             pass
 
+    def keep(self, *tensors):
+        """Holds tensors that enqueued work reads or writes until the next
+        sync(): the context's stream runs asynchronously, and a tensor freed
+        before then could be handed to another allocation by torch."""
+        self._keep.extend(t for t in tensors if t is not None)
+
     def sync(self):
         check(self.lib.sloam_sync(self.handle), self.handle)
+        self._keep.clear()
 
     @property
     def stream_ptr(self):
@@ -117,16 +125,25 @@ class CylinderFitPlan:
             out["best"] = torch.empty((self.n_clusters,), dtype=torch.int32, device=dev)
         return out
 
-    def exec(self, xyz, offsets, out):
-        """Enqueues one batch on the context stream (asynchronous)."""
+    def exec(self, xyz, offsets, out, proposal=None, active=None):
+        """Enqueues one batch on the context stream (asynchronous).  proposal:
+        float32 [n_clusters, 8] (sloam_pn_infer); active: int32 [n_clusters]
+        (clusters with 0 are skipped)."""
         if xyz.dtype != torch.float32 or xyz.shape != (self.n_points, 3):
             raise ValueError("xyz must be float32 [n_points, 3]")
         if offsets.dtype != torch.int32 or offsets.numel() != self.n_clusters + 1:
             raise ValueError("offsets must be int32 [n_clusters + 1]")
-        check(self.lib.sloam_cyl_plan_exec(
-            self.handle, _ptr(xyz), _ptr(offsets), _ptr(out.get("counts")),
-            _ptr(out.get("samples")), _ptr(out.get("mask")), _ptr(out.get("best")),
-            _ptr(out["model"])), self.ctx.handle)
+        if proposal is None and active is None:
+            check(self.lib.sloam_cyl_plan_exec(
+                self.handle, _ptr(xyz), _ptr(offsets), _ptr(out.get("counts")),
+                _ptr(out.get("samples")), _ptr(out.get("mask")), _ptr(out.get("best")),
+                _ptr(out["model"])), self.ctx.handle)
+        else:
+            check(self.lib.sloam_cyl_plan_exec_ex(
+                self.handle, _ptr(xyz), _ptr(offsets), _ptr(proposal), _ptr(active),
+                _ptr(out.get("counts")), _ptr(out.get("samples")), _ptr(out.get("mask")),
+                _ptr(out.get("best")), _ptr(out["model"])), self.ctx.handle)
+        self.ctx.keep(xyz, offsets, proposal, active, *out.values())
         return out
 
 

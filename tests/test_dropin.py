@@ -45,7 +45,6 @@ def _landmarks(name):
 
 
 def _write_fixtures(d):
-    d_ = d
     for stamp in ("t0", "t1"):
         trees = _landmarks(f"still_landmarks_{stamp}")
         with open(os.path.join(d, f"still_landmarks_{stamp}.lm"), "wb") as f:
@@ -58,11 +57,11 @@ def _write_fixtures(d):
                     f.write(np.asarray(v["coords"], "<f4").tobytes())
                     f.write(struct.pack("<i", len(v["points"])))
                     f.write(np.ascontiguousarray(v["points"], "<f4").tobytes())
-        d = np.load(os.path.join(GOLDEN, f"still_tree_{stamp}.npz"))
-        W, H = int(d["width"]), int(d["height"])
+        tc = np.load(os.path.join(GOLDEN, f"still_tree_{stamp}.npz"))
+        W, H = int(tc["width"]), int(tc["height"])
         org = np.full((H * W, 3), np.nan, np.float32)
-        org[d["index"]] = d["xyz"]
-        with open(os.path.join(d_, f"still_tree_{stamp}.org"), "wb") as f:
+        org[tc["index"]] = tc["xyz"]
+        with open(os.path.join(d, f"still_tree_{stamp}.org"), "wb") as f:
             f.write(struct.pack("<ii", H, W))
             f.write(org.astype("<f4").tobytes())
         xyz = np.load(os.path.join(GOLDEN, f"still_ground_{stamp}.npz"))["xyz"]

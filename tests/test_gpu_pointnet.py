@@ -1,0 +1,114 @@
+"""GPU learned proposal (sloam_pn_*) against the FP64 oracle with BF16
+emulation of weights and activations: proposals, losses and the batch
+gradient; Adam training reduces the loss; the learned fitting pipeline's
+fallback reproduces the plain fit exactly for the clusters it refits.
+
+Tolerances: the GPU sums layer 2/3 products on MFMA in FP32 and rounds the
+activations to BF16, the oracle sums in FP64 and rounds the same
+activations; a last-bit difference before a BF16 rounding moves that
+activation by one BF16 step (2^-8 relative), so results agree to ~1e-2
+relative, not bit-exactly."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from sloam_amd import proposal, synth
+from sloam_amd.gpu import CylinderFitPlan
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(S, seed, n_max=2000):
+    d = synth.make_clusters(S, seed=seed, n_min=60, n_max=n_max)
+    t = np.zeros((S, 8))
+    for c in range(S):
+        t[c, :3] = d["base"][c]
+        t[c, 3:6] = d["axis"][c] / np.linalg.norm(d["axis"][c])
+        t[c, 6] = d["radius"][c]
+    return d, t
+
+
+def _dev(d, t):
+    x = torch.as_tensor(d["xyz"]).cuda()
+    o = torch.as_tensor(d["offsets"].astype(np.int32)).cuda()
+    tt = torch.as_tensor(t).cuda()
+    return x, o, tt
+
+
+def test_forward_and_gradient_vs_oracle(ctx):
+    d, t = _batch(24, seed=11)
+    p = proposal.init_params(seed=4)
+    _, _, ow, ob = proposal.LAYERS[-1]
+    p[ow:ob] *= 8.0
+    p[ob + 6] = 2.0  # positive radii: valid proposals
+    print("[pn] forward/gradient 24 clusters", flush=True)
+    net = proposal.ProposalNet(ctx, p)
+    x, o, tt = _dev(d, t)
+    prop = net.infer(x, o)
+    cfg = proposal.train_cfg()
+    loss, grad = net.loss_grad(x, o, tt, cfg)
+    ctx.sync()
+    prop, loss, grad = prop.cpu().numpy(), loss.cpu().numpy(), grad.cpu().numpy()
+    ref = oracle.pn_forward(p, d["xyz"], d["offsets"], emulate=True)
+    rl, rg = oracle.pn_loss_grad(p, d["xyz"], d["offsets"], t,
+                                 weights=(cfg.w_axis, cfg.w_centre, cfg.w_radius), emulate=True)
+    # proposals: axis (unit), centre and radius relative to the cluster scale
+    ext = np.array([np.ptp(d["xyz"][d["offsets"][c]:d["offsets"][c + 1]], 0).max()
+                    for c in range(24)])
+    assert np.all(prop[:, 7] == ref["proposal"][:, 7])
+    ok = prop[:, 7] > 0
+    assert ok.sum() >= 20
+    dots = (prop[ok, :3] * ref["proposal"][ok, :3]).sum(1)
+    assert np.abs(np.abs(dots) - 1).max() < 1e-2
+    assert (np.abs(prop[:, 3:7] - ref["proposal"][:, 3:7]).max(1) / ext).max() < 1e-2
+    assert np.allclose(loss, rl, rtol=2e-2, atol=1e-4)
+    rel = np.linalg.norm(grad - rg) / np.linalg.norm(rg)
+    cos = grad @ rg / (np.linalg.norm(grad) * np.linalg.norm(rg))
+    print(f"[pn] gradient relative error {rel:.2e}, cosine {cos:.6f}", flush=True)
+    assert rel < 3e-2 and cos > 0.999
+
+
+def test_training_reduces_loss(ctx):
+    d, t = _batch(64, seed=12)
+    net = proposal.ProposalNet(ctx, proposal.init_params(seed=5))
+    x, o, tt = _dev(d, t)
+    cfg = proposal.train_cfg(lr=2e-3)
+    loss = torch.empty(64, dtype=torch.float32, device="cuda")
+    first = None
+    for _ in range(300):
+        net.train_step(x, o, tt, cfg, loss)
+        if first is None:
+            ctx.sync()
+            first = float(loss.mean())
+    ctx.sync()
+    last = float(loss.mean())
+    print(f"[pn] loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < 0.2 * first
+
+
+def test_learned_pipeline_fallback_matches_plain(ctx):
+    """With untrained weights most proposals fail verification; every cluster
+    sent to the fallback must end with exactly the plain fit's model."""
+    d = synth.make_clusters(200, seed=13)
+    x = torch.as_tensor(d["xyz"]).cuda()
+    o = torch.as_tensor(d["offsets"].astype(np.int32)).cuda()
+    net = proposal.ProposalNet(ctx, proposal.init_params(seed=6))
+    fit = proposal.LearnedCylinderFit(ctx, net, d["offsets"], n_hyp=16, plain_hyp=256)
+    out = fit.alloc()
+    fit.exec(x, o, out)
+    plain = CylinderFitPlan(ctx, d["offsets"], n_hyp=256, stream_tag=0)
+    pout = plain.alloc_outputs(mask=False, best=False)
+    plain.exec(x, o, pout)
+    ctx.sync()
+    act = out["active"].cpu().numpy().astype(bool)
+    m, pm = out["model"].cpu().numpy(), pout["model"].cpu().numpy()
+    nf = int(out["n_fallback"].cpu().numpy()[0])
+    print(f"[pn] fallback {nf} of 200", flush=True)
+    assert nf == act.sum() and nf > 0
+    assert np.array_equal(m[act], pm[act])
+    n = np.diff(d["offsets"])
+    kept = ~act & (n >= 3)
+    assert np.all(m[kept, 7] >= 0.5 * n[kept])
+    fit.close()
+    plain.close()
