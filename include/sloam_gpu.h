@@ -120,8 +120,9 @@ int sloam_cyl_plan_exec(sloam_cyl_plan* plan, const float* d_xyz, const int32_t*
                         int32_t* d_best, double* d_model);
 /* The same with two optional inputs:
  *   d_proposal [n_clusters * SLOAM_PN_PROPOSAL_STRIDE] floats (sloam_pn_infer):
- *     for a cluster with a valid proposal the seed axis is the proposed axis
- *     (no PCA) and hypothesis 0 is the proposed circle (its samples -1);
+ *     for a cluster with a valid proposal, hypothesis 0 is the proposed
+ *     circle (centre taken into the cluster's PCA seed frame, proposed
+ *     radius; its samples are -1);
  *   d_active [n_clusters] int32: clusters with 0 are left untouched (no
  *     seed, scoring or refit; their model records keep their contents).
  * d_mask must be NULL when d_active is given.                               */
@@ -361,8 +362,9 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
  * predicted model's output space is CylinderParameters {root, ray, radius},
  * sloam/include/objects/cylinder.h:14-22).
  * Input per cluster: up to SLOAM_PN_POINTS points, taken at indices
- * floor(j n / m), m = min(n, 512), shifted by the centre of their bounding box
- * and scaled by 1 / (largest half extent).  Network: shared per-point MLP
+ * floor(j n / m), m = min(n, 512), shifted by the centre of their bounding box,
+ * x and y scaled by 1 / (larger horizontal half extent), z by 1 / (vertical
+ * half extent).  Network: shared per-point MLP
  * 3 -> 64 -> 128 -> 256 (ReLU), max-pool over the cluster's points, MLP
  * 256 -> 128 (ReLU) -> 7 = (axis direction, axis point nearest the box
  * centre, radius) in the normalised frame.  Weights are BF16 (rounded from
@@ -383,7 +385,9 @@ int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
 /* Training loss against cylinder models (SLOAM_CYL_MODEL_STRIDE doubles per
  * cluster, e.g. the plain fit's output; clusters whose radius is <= 0 carry
  * no loss):  L_c = w_axis (1 - (a_hat . a)^2) + w_centre |c_hat - c*|^2 +
- * w_radius (r_hat - r*)^2 in the normalised frame, L = mean over the batch.  */
+ * w_radius (r_hat - r*)^2 in the normalised frame (a: the target axis mapped
+ * into it; c*: the point of that axis line nearest the box centre; r*: the
+ * radius at the horizontal scale), L = mean over the batch.                 */
 typedef struct sloam_pn_train_cfg {
   float lr;          /* Adam step size                                        */
   float beta1;       /* 0.9                                                   */

@@ -39,7 +39,7 @@ def _declare(lib):
     lib.oracle_cyl_hypothesis.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                           C.c_uint64, C.c_int, P, P, P]
     lib.oracle_cyl_fit_batch.argtypes = [P, P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
-                                         C.c_int, C.c_uint64, C.c_int, P, P, P, P, P, C.c_int]
+                                         C.c_int, C.c_uint64, C.c_int, P, P, P, P, P, P, C.c_int]
     lib.oracle_cyl_distances_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float,
                                               C.c_float, C.c_uint64, C.c_int, P]
     lib.oracle_pose_residuals.argtypes = [P, C.c_int, P, P, P, P, P]
@@ -117,9 +117,10 @@ def cyl_hypothesis(xyz, cluster, h, thr, max_radius, seed, tag=0):
 
 def cyl_fit_batch(xyz, offsets, n_hyp, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=2,
                   seed=0x5EED,
-                  tag=0, counts=True, samples=False, mask=True, threads=1):
+                  tag=0, counts=True, samples=False, mask=True, threads=1, proposal=None):
     x = np.ascontiguousarray(xyz, np.float32)
     off = np.ascontiguousarray(offsets, np.int32)
+    prop = None if proposal is None else np.ascontiguousarray(proposal, np.float32)
     C_ = len(off) - 1
     N = int(off[-1])
     out = {
@@ -134,7 +135,9 @@ def cyl_fit_batch(xyz, offsets, n_hyp, thr=0.05, max_radius=1.0, refit_rounds=3,
         out["mask"] = np.zeros((N + 31) // 32, np.uint32)
     st = load().oracle_cyl_fit_batch(_p(x), _p(off), C_, n_hyp, thr, max_radius, refit_rounds,
                                      refit_iters,
-                                     seed, tag, _p(out.get("counts")), _p(out.get("samples")),
+                                     seed, tag,
+                                     _p(prop),
+                                     _p(out.get("counts")), _p(out.get("samples")),
                                      _p(out.get("mask")), _p(out["best"]), _p(out["model"]),
                                      threads)
     if st:

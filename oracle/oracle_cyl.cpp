@@ -96,19 +96,11 @@ Frame seed_frame(const float* P, int n) {
   return f;
 }
 
-// Circumcircle of (a, b, c) in the local plane, FP32, and predicate bounds.
-Hyp circle_hyp(const float a[2], const float b[2], const float c[2], float thr, float max_radius) {
+// Predicate bounds of the circle (cu, cv, r) of the local plane, FP32.
+Hyp circle_params(float cu, float cv, float r, float thr, float max_radius) {
   Hyp h;
-  const float bx = b[0] - a[0], by = b[1] - a[1];
-  const float cx = c[0] - a[0], cy = c[1] - a[1];
-  const float den = 2.0f * (bx * cy - by * cx);
-  const float bb = bx * bx + by * by;
-  const float cc = cx * cx + cy * cy;
-  const float ox = (cy * bb - by * cc) / den;
-  const float oy = (bx * cc - cx * bb) / den;
-  const float r = std::sqrt(ox * ox + oy * oy);
-  h.cu = a[0] + ox;
-  h.cv = a[1] + oy;
+  h.cu = cu;
+  h.cv = cv;
   h.r = r;
   h.valid = r <= max_radius;
   if (h.valid) {
@@ -127,8 +119,31 @@ Hyp circle_hyp(const float a[2], const float b[2], const float c[2], float thr, 
   return h;
 }
 
+// Circumcircle of (a, b, c) in the local plane, FP32, and predicate bounds.
+Hyp circle_hyp(const float a[2], const float b[2], const float c[2], float thr, float max_radius) {
+  const float bx = b[0] - a[0], by = b[1] - a[1];
+  const float cx = c[0] - a[0], cy = c[1] - a[1];
+  const float den = 2.0f * (bx * cy - by * cx);
+  const float bb = bx * bx + by * by;
+  const float cc = cx * cx + cy * cy;
+  const float ox = (cy * bb - by * cc) / den;
+  const float oy = (bx * cc - cx * bb) / den;
+  const float r = std::sqrt(ox * ox + oy * oy);
+  return circle_params(a[0] + ox, a[1] + oy, r, thr, max_radius);
+}
+
+// Hypothesis h: the proposed circle for h == 0 when the cluster has a valid
+// proposal (prop_c: axis[3], centre[3], radius, valid; the centre is taken
+// into the seed frame), else the circumcircle of a Philox sample.
 Hyp hypothesis(const float* P, int n, int cluster, int h, const Frame& f, float thr,
-               float max_radius, uint32_t k0, uint32_t k1, uint32_t tag, int32_t idx[3]) {
+               float max_radius, uint32_t k0, uint32_t k1, uint32_t tag, int32_t idx[3],
+               const float* prop_c = nullptr) {
+  if (prop_c && h == 0) {
+    idx[0] = idx[1] = idx[2] = -1;
+    float cu, cv, s;
+    to_local(f, prop_c + 3, cu, cv, s);
+    return circle_params(cu, cv, prop_c[6], thr, max_radius);
+  }
   const U4 ctr = {{(uint32_t)h, (uint32_t)cluster, tag, 0u}};
   const U4 x = philox4x32_10(ctr, k0, k1);
   draw3(x.v[0], x.v[1], x.v[2], (uint32_t)n, idx);
@@ -349,10 +364,12 @@ int oracle_cyl_hypothesis(const float* xyz, int n, int cluster, int h, float thr
 
 // The whole batch, same outputs as sloam_cyl_plan_exec.  n_threads > 1 runs
 // clusters in parallel (OpenMP); results do not depend on it.
+// proposal (nullable): 8 floats per cluster (axis, centre, radius, valid);
+// a valid one is hypothesis 0.
 int oracle_cyl_fit_batch(const float* xyz, const int32_t* offsets, int n_clusters, int n_hyp,
                          float thr, float max_radius, int refit_rounds, int refit_iters, uint64_t seed,
-                         int tag, int32_t* counts, int32_t* samples, uint32_t* mask, int32_t* best_out,
-                         double* model, int n_threads) {
+                         int tag, const float* proposal, int32_t* counts, int32_t* samples,
+                         uint32_t* mask, int32_t* best_out, double* model, int n_threads) {
   if (n_clusters < 0 || n_hyp < 1) return -1;
   const int64_t n_points = offsets[n_clusters];
   if (mask) std::memset(mask, 0, sizeof(uint32_t) * (size_t)((n_points + 31) / 32));
@@ -363,6 +380,8 @@ int oracle_cyl_fit_batch(const float* xyz, const int32_t* offsets, int n_cluster
     const int n = offsets[c + 1] - off;
     const float* P = xyz + 3 * (size_t)off;
     const Frame f = seed_frame(P, n);
+    const float* pc = (proposal && proposal[8 * (size_t)c + 7] > 0.0f) ? proposal + 8 * (size_t)c
+                                                                         : nullptr;
     std::vector<float> U((size_t)n), V((size_t)n), S((size_t)n);
     for (int i = 0; i < n; ++i) to_local(f, P + 3 * i, U[i], V[i], S[i]);
     int best = -1, best_count = -1;
@@ -370,7 +389,8 @@ int oracle_cyl_fit_batch(const float* xyz, const int32_t* offsets, int n_cluster
       int count = 0;
       int32_t idx[3] = {-1, -1, -1};
       if (n >= 3) {
-        const Hyp hy = hypothesis(P, n, c, h, f, thr, max_radius, k0, k1, (uint32_t)tag, idx);
+        const Hyp hy =
+            hypothesis(P, n, c, h, f, thr, max_radius, k0, k1, (uint32_t)tag, idx, pc);
         for (int i = 0; i < n; ++i) count += inlier(U[i], V[i], S[i], hy) ? 1 : 0;
       }
       if (counts) counts[(size_t)c * n_hyp + h] = count;
@@ -392,7 +412,7 @@ int oracle_cyl_fit_batch(const float* xyz, const int32_t* offsets, int n_cluster
       continue;
     }
     int32_t idx[3];
-    const Hyp hb = hypothesis(P, n, c, best, f, thr, max_radius, k0, k1, (uint32_t)tag, idx);
+    const Hyp hb = hypothesis(P, n, c, best, f, thr, max_radius, k0, k1, (uint32_t)tag, idx, pc);
     if (mask) {
       for (int i = 0; i < n; ++i) {
         if (!inlier(U[i], V[i], S[i], hb)) continue;

@@ -2,8 +2,9 @@
 // network (sloam_amd/csrc/pointnet.hip; no counterpart in the reference).
 //
 // Same input preparation as the GPU (float): up to 512 points at indices
-// floor(j n / m), shifted by their bounding-box centre and scaled by
-// 1 / max(largest half extent, 1e-3).  Network in FP64: shared MLP
+// floor(j n / m), shifted by their bounding-box centre, x and y scaled by
+// sh = 1 / max(larger horizontal half extent, 1e-3), z by
+// sz = 1 / max(vertical half extent, 1e-3).  Network in FP64: shared MLP
 // 3 -> 64 -> 128 -> 256 (ReLU), max-pool over the m points (first maximising
 // point per channel; channels whose outputs are all 0 pool to 0 and pass no
 // gradient), MLP 256 -> 128 (ReLU) -> 7.  With `emulate` set, weights and
@@ -12,9 +13,9 @@
 // central-difference checks of the analytic gradient).  Loss per cluster
 // against a cylinder model (root, ray, radius; radius <= 0 -> no loss):
 //   w_axis (1 - (a_hat . a)^2) + w_centre |c_hat - c*|^2 + w_radius (r_hat - r*)^2
-// with a_hat = out[0:3] / |out[0:3]|, c* the axis point nearest the box
-// centre and r* the radius, both in the normalised frame; the gradient is
-// the batch mean.
+// with a_hat = out[0:3] / |out[0:3]|, a the target axis mapped to the
+// normalised frame, c* the point of the mapped axis line nearest the box
+// centre and r* = radius x sh; the gradient is the batch mean.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -43,8 +44,8 @@ float bf16(float x) {
 struct Prep {
   int m = 0, n = 0;
   float cen[3] = {0, 0, 0};
-  float s = 1.f;
-  std::vector<float> x;  // m x 3 normalised
+  float sc[3] = {1.f, 1.f, 1.f};  // per-axis scale: (sh, sh, sz)
+  std::vector<float> x;           // m x 3 normalised
 };
 
 Prep prepare(const float* P, int n) {
@@ -59,16 +60,19 @@ Prep prepare(const float* P, int n) {
       hi[k] = std::fmax(hi[k], P[3 * i + k]);
     }
   }
-  float ext = 1e-3f;
+  float half[3];
   for (int k = 0; k < 3; ++k) {
     r.cen[k] = r.m > 0 ? 0.5f * (lo[k] + hi[k]) : 0.0f;
-    if (r.m > 0) ext = std::fmax(ext, 0.5f * (hi[k] - lo[k]));
+    half[k] = r.m > 0 ? 0.5f * (hi[k] - lo[k]) : 0.0f;
   }
-  r.s = 1.0f / ext;
+  const float sh = 1.0f / std::fmax(std::fmax(half[0], half[1]), 1e-3f);
+  const float sz = 1.0f / std::fmax(half[2], 1e-3f);
+  r.sc[0] = r.sc[1] = sh;
+  r.sc[2] = sz;
   r.x.resize(3 * (size_t)r.m);
   for (int j = 0; j < r.m; ++j) {
     const int i = (int)(((long long)j * n) / r.m);
-    for (int k = 0; k < 3; ++k) r.x[3 * j + k] = (P[3 * i + k] - r.cen[k]) * r.s;
+    for (int k = 0; k < 3; ++k) r.x[3 * j + k] = (P[3 * i + k] - r.cen[k]) * r.sc[k];
   }
   return r;
 }
@@ -130,14 +134,15 @@ void forward(const std::vector<double>& W, const Prep& pr, bool emulate, Fwd& F)
 double loss(const Prep& pr, const double* tg, const double* out, const float* wts, double* d) {
   for (int k = 0; k < kOut; ++k) d[k] = 0.0;
   if (!(tg[6] > 0.0) || pr.m < 3) return 0.0;
-  const double an = std::sqrt(tg[3] * tg[3] + tg[4] * tg[4] + tg[5] * tg[5]);
-  const double A[3] = {tg[3] / an, tg[4] / an, tg[5] / an};
+  const double B[3] = {tg[3] * pr.sc[0], tg[4] * pr.sc[1], tg[5] * pr.sc[2]};
+  const double an = std::sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+  const double A[3] = {B[0] / an, B[1] / an, B[2] / an};
   double r[3];
-  for (int k = 0; k < 3; ++k) r[k] = (tg[k] - pr.cen[k]) * pr.s;
+  for (int k = 0; k < 3; ++k) r[k] = (tg[k] - pr.cen[k]) * pr.sc[k];
   const double pj = r[0] * A[0] + r[1] * A[1] + r[2] * A[2];
   double cs[3];
   for (int k = 0; k < 3; ++k) cs[k] = r[k] - pj * A[k];
-  const double rs = tg[6] * pr.s;
+  const double rs = tg[6] * pr.sc[0];
   const double nv = std::sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]) + 1e-12;
   double ah[3];
   for (int k = 0; k < 3; ++k) ah[k] = out[k] / nv;
@@ -231,12 +236,14 @@ void oracle_pn_forward(const float* params, const float* xyz, const int32_t* off
       for (int k = 0; k < kC3; ++k) fidx[kC3 * (size_t)c + k] = F.fidx[k];
     if (proposal) {
       double* q = proposal + 8 * (size_t)c;
-      const double nv = std::sqrt(F.out[0] * F.out[0] + F.out[1] * F.out[1] + F.out[2] * F.out[2]);
+      double v[3];
+      for (int k = 0; k < 3; ++k) v[k] = F.out[k] / pr.sc[k];
+      const double nv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
       for (int k = 0; k < 3; ++k) {
-        q[k] = nv > 0 ? F.out[k] / nv : 0.0;
-        q[3 + k] = pr.cen[k] + F.out[3 + k] / pr.s;
+        q[k] = nv > 0 ? v[k] / nv : 0.0;
+        q[3 + k] = pr.cen[k] + F.out[3 + k] / pr.sc[k];
       }
-      q[6] = F.out[6] / pr.s;
+      q[6] = F.out[6] / pr.sc[0];
       q[7] = (pr.m >= 3 && nv > 0 && q[6] > 0) ? 1.0 : 0.0;
     }
   }

@@ -145,8 +145,8 @@ __device__ inline bool is_inlier(float u, float v, float s, const HypOut& h) {
 // seed: bounding-box centre, integer covariance, FP64 Jacobi, basis
 // ---------------------------------------------------------------------------
 // prop (nullable): proposals (SLOAM_PN_PROPOSAL_STRIDE floats per cluster);
-// a valid proposal's axis replaces the PCA axis.  active (nullable): clusters
-// with active[c] == 0 are skipped.
+// a valid proposal's circle, taken into the cluster's seed frame, is
+// hypothesis 0.  active (nullable): clusters with active[c] == 0 are skipped.
 __device__ inline const float* cluster_prop(const float* prop, int c) {
   if (!prop) return nullptr;
   const float* q = prop + (size_t)c * SLOAM_PN_PROPOSAL_STRIDE;
@@ -156,7 +156,6 @@ __device__ inline const float* cluster_prop(const float* prop, int c) {
 __global__ __launch_bounds__(kBlock) void cyl_seed_kernel(const float* __restrict__ xyz,
                                                           const int32_t* __restrict__ offsets,
                                                           int n_clusters,
-                                                          const float* __restrict__ prop,
                                                           const int32_t* __restrict__ active,
                                                           CylFrame* __restrict__ frames) {
   __shared__ float red_f[kWaves][6];
@@ -246,14 +245,7 @@ __global__ __launch_bounds__(kBlock) void cyl_seed_kernel(const float* __restric
   f.n = n;
   f.reserved = 0.0;
   double axis[3] = {0.0, 0.0, 1.0};
-  const float* pc = cluster_prop(prop, c);
-  if (pc) {
-    const double ax = pc[0], ay = pc[1], az = pc[2];
-    const double an = sqrt(ax * ax + ay * ay + az * az);
-    axis[0] = ax / an;
-    axis[1] = ay / an;
-    axis[2] = az / an;
-  } else if (n >= 2) {
+  if (n >= 2) {
     const double dn = (double)n;
     const double S0 = (double)t[0], S1 = (double)t[1], S2 = (double)t[2];
     double a6[6];
@@ -925,7 +917,7 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
   {
     SloamKernelScope ks(ctx, "cyl_seed");
     hipLaunchKernelGGL(cyl_seed_kernel, dim3(p->n_clusters), dim3(kBlock), 0, ctx->stream, d_xyz,
-                       d_offsets, p->n_clusters, d_proposal, d_active, p->d_frames);
+                       d_offsets, p->n_clusters, d_active, p->d_frames);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (p->n_work > 0) {

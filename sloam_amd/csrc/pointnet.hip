@@ -2,8 +2,9 @@
 // weights on MFMA with FP32 accumulation.
 //
 //   pn_prep     one workgroup per cluster: up to 512 points at indices
-//               floor(j n / m), bounding-box centre and scale, normalised
-//               float4 points (w = 1 real, 0 padding)
+//               floor(j n / m), shifted by their bounding-box centre, x and y
+//               scaled by 1 / (larger horizontal half extent), z by
+//               1 / (vertical half extent); float4 (w = 1 real, 0 padding)
 //   pn_fwd      one workgroup (4 waves) per cluster, 64 points at a time:
 //               layer 1 (3 -> 64) on the VALU, layers 2 (64 -> 128) and
 //               3 (128 -> 256) on v_mfma_f32_16x16x32_bf16 with the BF16
@@ -47,9 +48,10 @@ static_assert(kNP == SLOAM_PN_PARAMS, "parameter layout");
 // h4[128], out[8]
 constexpr int kAF = 0, kAI = kAF + kC3, kAH = kAI + kC3, kAO = kAH + kC4, kAct = kAO + 8;
 
+// normalised p' = ((x - cx) sh, (y - cy) sh, (z - cz) sz)
 struct PnMeta {
-  float cx, cy, cz, s;
-  int m, n, pad0, pad1;
+  float cx, cy, cz, sh, sz;
+  int m, n, pad;
 };
 
 struct LossCfg {
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ 
       red[wave][3 + k] = hi[k];
     }
   __syncthreads();
-  float cen[3], ext = 1e-3f;
+  float cen[3], half[3];
   for (int k = 0; k < 3; ++k) {
     float a = red[0][k], b = red[0][3 + k];
     for (int w = 1; w < kFB / 64; ++w) {
@@ -123,16 +125,19 @@ __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ 
       b = fmaxf(b, red[w][3 + k]);
     }
     cen[k] = m > 0 ? 0.5f * (a + b) : 0.0f;
-    if (m > 0) ext = fmaxf(ext, 0.5f * (b - a));
+    half[k] = m > 0 ? 0.5f * (b - a) : 0.0f;
   }
-  const float s = 1.0f / ext;
+  // horizontal and vertical scales: the cross-section keeps its own
+  // resolution however tall the trunk is
+  const float sh = 1.0f / fmaxf(fmaxf(half[0], half[1]), 1e-3f);
+  const float sz = 1.0f / fmaxf(half[2], 1e-3f);
   float4* Xc = X + (size_t)c * kP;
   for (int j = threadIdx.x; j < kP; j += kFB) {
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (j < m) {
       const int i = (int)(((long long)j * n) / m);
-      v = make_float4((P[3 * (size_t)i] - cen[0]) * s, (P[3 * (size_t)i + 1] - cen[1]) * s,
-                      (P[3 * (size_t)i + 2] - cen[2]) * s, 1.0f);
+      v = make_float4((P[3 * (size_t)i] - cen[0]) * sh, (P[3 * (size_t)i + 1] - cen[1]) * sh,
+                      (P[3 * (size_t)i + 2] - cen[2]) * sz, 1.0f);
     }
     Xc[j] = v;
   }
@@ -141,10 +146,11 @@ __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ 
     mt.cx = cen[0];
     mt.cy = cen[1];
     mt.cz = cen[2];
-    mt.s = s;
+    mt.sh = sh;
+    mt.sz = sz;
     mt.m = m;
     mt.n = n;
-    mt.pad0 = mt.pad1 = 0;
+    mt.pad = 0;
     meta[c] = mt;
   }
 }
@@ -319,17 +325,17 @@ __global__ __launch_bounds__(kFB) void pn_fwd_kernel(
   }
   if (prop && t == 0) {
     float* pr = prop + (size_t)c * SLOAM_PN_PROPOSAL_STRIDE;
-    const float vx = outs[0], vy = outs[1], vz = outs[2];
+    // back to the world frame: directions and offsets divide by the scales
+    const float vx = outs[0] / mt.sh, vy = outs[1] / mt.sh, vz = outs[2] / mt.sz;
     const float nv = sqrtf(vx * vx + vy * vy + vz * vz);
-    const float inv = 1.0f / mt.s;
-    const float r = outs[6] * inv;
+    const float r = outs[6] / mt.sh;
     const bool ok = m >= 3 && nv > 0.f && r > 0.f && isfinite(r) && isfinite(nv);
     pr[0] = ok ? vx / nv : 0.f;
     pr[1] = ok ? vy / nv : 0.f;
     pr[2] = ok ? vz / nv : 1.f;
-    pr[3] = mt.cx + outs[3] * inv;
-    pr[4] = mt.cy + outs[4] * inv;
-    pr[5] = mt.cz + outs[5] * inv;
+    pr[3] = mt.cx + outs[3] / mt.sh;
+    pr[4] = mt.cy + outs[4] / mt.sh;
+    pr[5] = mt.cz + outs[5] / mt.sz;
     pr[6] = ok ? r : 0.f;
     pr[7] = ok ? 1.f : 0.f;
   }
@@ -378,14 +384,17 @@ __global__ __launch_bounds__(kFB) void pn_bwd_kernel(
     float L = 0.f;
     float d[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (tg[6] > 0.0 && mt.m >= 3) {
-      const double an = sqrt(tg[3] * tg[3] + tg[4] * tg[4] + tg[5] * tg[5]);
-      const double ax = tg[3] / an, ay = tg[4] / an, az = tg[5] / an;
-      const double rx = (tg[0] - mt.cx) * mt.s, ry = (tg[1] - mt.cy) * mt.s,
-                   rz = (tg[2] - mt.cz) * mt.s;
+      // the target axis line in the normalised frame, its point nearest the
+      // box centre, and the radius at the horizontal scale
+      const double bx = tg[3] * mt.sh, by = tg[4] * mt.sh, bz = tg[5] * mt.sz;
+      const double an = sqrt(bx * bx + by * by + bz * bz);
+      const double ax = bx / an, ay = by / an, az = bz / an;
+      const double rx = (tg[0] - mt.cx) * mt.sh, ry = (tg[1] - mt.cy) * mt.sh,
+                   rz = (tg[2] - mt.cz) * mt.sz;
       const double pr = rx * ax + ry * ay + rz * az;
       const float cs[3] = {(float)(rx - pr * ax), (float)(ry - pr * ay), (float)(rz - pr * az)};
       const float A[3] = {(float)ax, (float)ay, (float)az};
-      const float rs = (float)(tg[6] * mt.s);
+      const float rs = (float)(tg[6] * mt.sh);
       const float nv = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
       const float ah[3] = {o[0] / nv, o[1] / nv, o[2] / nv};
       const float dd = ah[0] * A[0] + ah[1] * A[1] + ah[2] * A[2];

@@ -1,8 +1,8 @@
 """Trains the learned cylinder proposal on the GPU.
 
 Training data: synthetic trunk clusters (sloam_amd.synth.make_clusters:
-log-uniform 50-5000 points, radius U(0.05, 0.6) m, 20 % outliers, 2 cm
-noise), labelled by the plain method (1024-hypothesis RANSAC + LO refit,
+log-uniform 50-5000 points, radius U(0.05, 0.6) m, 2 cm noise, an outlier
+share drawn per chunk from --outliers), labelled by the plain method (1024-hypothesis RANSAC + LO refit,
 sloam_cyl_plan) on the same device.  Optimiser: Adam on the FP32 master
 weights through sloam_pn_train_step, cosine-decayed step size.
 
@@ -23,13 +23,13 @@ from sloam_amd import proposal, synth
 from sloam_amd.gpu import Context, CylinderFitPlan
 
 
-def make_chunk(ctx, seed, n, batch):
+def make_chunk(ctx, seed, n, batch, outlier_frac=0.2):
     """One chunk of labelled clusters and its training batches.  Every device
     tensor a batch uses is created here and kept for the whole run: the
     library's stream reads them asynchronously, after the host has moved on,
     so none may be freed (and reused by torch's allocator) while steps are in
     flight."""
-    d = synth.make_clusters(n, seed=seed)
+    d = synth.make_clusters(n, seed=seed, outlier_frac=outlier_frac)
     x = torch.as_tensor(d["xyz"]).cuda()
     o = torch.as_tensor(d["offsets"].astype(np.int32)).cuda()
     plan = CylinderFitPlan(ctx, d["offsets"], n_hyp=1024, thr=0.05, max_radius=1.0)
@@ -57,14 +57,19 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-min", type=float, default=5e-5)
     ap.add_argument("--seed", type=int, default=13)
+    ap.add_argument("--outliers", default="0.2,0.6",
+                    help="outlier fraction range of the chunks (uniform)")
     ap.add_argument("--init", default=None, help="start from these weights (.npy)")
     ap.add_argument("--out", default=proposal.DEFAULT_WEIGHTS)
     args = ap.parse_args(argv)
     ctx = Context(0)
     t0 = time.time()
     chunks = []
+    lo, hi = (float(v) for v in args.outliers.split(","))
+    orng = np.random.default_rng(args.seed + 1)
     for k in range(args.chunks):
-        chunks.append(make_chunk(ctx, args.seed * 1000 + k, args.chunk_size, args.batch))
+        chunks.append(make_chunk(ctx, args.seed * 1000 + k, args.chunk_size, args.batch,
+                                 outlier_frac=float(orng.uniform(lo, hi))))
         if k % 5 == 0 or k == args.chunks - 1:
             print(f"[train] chunk {k + 1}/{args.chunks} ready ({time.time() - t0:.0f} s)",
                   flush=True)

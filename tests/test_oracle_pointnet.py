@@ -77,6 +77,11 @@ def test_proposal_frame():
         Q = P[(np.arange(m) * len(P)) // m]
         lo, hi = Q.min(0), Q.max(0)
         cen = 0.5 * (lo + hi)
-        s = np.float32(1) / max(np.float32(1e-3), (0.5 * (hi - lo)).max())
-        assert np.allclose(o["proposal"][c, 3:6], cen + o["out"][c, 3:6] / s, rtol=1e-6)
-        assert np.isclose(o["proposal"][c, 6], o["out"][c, 6] / s, rtol=1e-6)
+        half = 0.5 * (hi - lo)
+        sh = np.float32(1) / max(np.float32(1e-3), half[:2].max())
+        sz = np.float32(1) / max(np.float32(1e-3), half[2])
+        sc = np.array([sh, sh, sz], np.float64)
+        assert np.allclose(o["proposal"][c, 3:6], cen + o["out"][c, 3:6] / sc, rtol=1e-6)
+        assert np.isclose(o["proposal"][c, 6], o["out"][c, 6] / sh, rtol=1e-6)
+        v = o["out"][c, :3] / sc
+        assert np.allclose(o["proposal"][c, :3], v / np.linalg.norm(v), rtol=1e-9)
