@@ -35,6 +35,12 @@ def parse():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-pose", action="store_true", help="skip the batched pose leg")
     ap.add_argument("--pose-steps", type=int, default=10)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--learned-clusters", type=int, default=4096)
+    ap.add_argument("--no-keyframes", action="store_true", help="skip the keyframe-stream leg")
+    ap.add_argument("--keyframes", type=int, default=16)
+    ap.add_argument("--kf-passes", type=int, default=5)
+    ap.add_argument("--kf-height", type=int, default=128)
     return ap.parse_args()
 
 
@@ -98,6 +104,127 @@ def bench_pose(ctx, args):
         "eval_hbm_frac": eval_bytes / (eval_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "pose_err_vs_truth": {"max_m": float(dt.max()), "max_deg": float(np.degrees(dr.max()))},
     }
+
+
+PN_FLOP_PER_POINT = 2 * (3 * 64 + 64 * 128 + 128 * 256)  # shared per-point MLP
+PN_FLOP_HEAD = 2 * (256 * 128 + 128 * 7)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_learned(ctx, args):
+    """Learned proposal + 16 verifying hypotheses + LO refit, plain
+    1024-hypothesis fallback below 0.5 inlier ratio, on 4096 clusters of the
+    same generator (20 % outliers).  Accuracy against the plain fit of the same
+    clusters; network throughput from the pn_fwd kernel time."""
+    import numpy as np
+    import torch
+
+    from sloam_amd import proposal, synth
+    from sloam_amd.gpu import CylinderFitPlan
+
+    S = args.learned_clusters
+    d = synth.make_clusters(S, seed=3)
+    dev = torch.device("cuda", 0)
+    xyz = torch.as_tensor(d["xyz"]).to(dev)
+    off = torch.as_tensor(d["offsets"]).to(dev)
+    n = np.diff(d["offsets"])
+    m = np.minimum(n, 512)
+    net_flop = float(PN_FLOP_PER_POINT * m.sum() + PN_FLOP_HEAD * S)
+    plain = CylinderFitPlan(ctx, d["offsets"], n_hyp=args.hyp, stream_tag=0)
+    pout = plain.alloc_outputs(mask=False, best=False)
+    net = proposal.ProposalNet(ctx, proposal.load_weights())
+    fit = proposal.LearnedCylinderFit(ctx, net, d["offsets"], n_hyp=16, plain_hyp=args.hyp)
+    lout = fit.alloc()
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        plain.exec(xyz, off, pout)
+        fit.exec(xyz, off, lout)
+    ctx.sync()
+    stream = ctx.torch_stream()
+
+    def timed(fn):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        ctx.sync()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        ctx.sync()
+        return e0.elapsed_time(e1) / args.steps
+
+    ms_plain = timed(lambda: plain.exec(xyz, off, pout))
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    ms_learned = timed(lambda: fit.exec(xyz, off, lout))
+    ctx.enable_timing(False)
+    kern = {}
+    for name in ("pn_prep", "pn_fwd", "cyl_seed", "cyl_score", "cyl_refit", "cyl_fallback"):
+        ms, k = ctx.kernel_time(name)
+        kern[name] = ms / max(k, 1)
+    ref = pout["model"].cpu().numpy()
+    lm = lout["model"].cpu().numpy()
+    nf = int(lout["n_fallback"].cpu().numpy()[0])
+    ok = (ref[:, 6] > 0) & (lm[:, 6] > 0)
+    err = np.abs(lm[ok, 6] - ref[ok, 6]) * 100
+    fwd_s = kern["pn_fwd"] / 1e3
+    plain.close()
+    fit.close()
+    return {
+        "workload": f"{S} clusters (log-U[50,5000] points, 20 % outliers), proposal net on "
+                    f"<= 512 points each, 16 verifying hypotheses + LO refit, plain "
+                    f"{args.hyp}-hypothesis fallback below 0.5 inlier ratio",
+        "fits_per_s": S / (ms_learned / 1e3),
+        "ms_per_batch": ms_learned,
+        "plain_fits_per_s": S / (ms_plain / 1e3),
+        "plain_ms_per_batch": ms_plain,
+        "speedup_vs_plain": ms_plain / ms_learned,
+        "fallback_fraction": nf / S,
+        "radius_diff_vs_plain_cm": {"median": float(np.median(err)),
+                                    "p90": float(np.percentile(err, 90)),
+                                    "frac_over_1cm": float(np.mean(err > 1.0))},
+        "kernel_ms": kern,
+        "net_mfma": {"achieved": net_flop / fwd_s / 1e12, "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": net_flop / fwd_s / 1e12 / BF16_PEAK_TFLOPS,
+                     "flop_per_launch": net_flop},
+    }
+
+
+def bench_keyframes(args):
+    """Keyframe stream: scans of a 300-tree forest (128 x 2048, 40 % understory
+    clutter, partial trunk occlusion) every 0.5 m along a weaving trajectory,
+    noisy odometry between them; each keyframe runs the GPU front end (one
+    HIP graph: projection + partition, tree instances, ground cells) and the
+    estimator (models, association, two-step pose).  The C++ program
+    sloam_amd/keyframe_bench times each keyframe end to end (host buffers in,
+    pose out)."""
+    import subprocess
+    import tempfile
+
+    from sloam_amd import synth
+
+    exe = os.path.join(ROOT, "sloam_amd", "keyframe_bench")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is missing: build first (python -m sloam_amd.build)")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "sequence.bin")
+        t0 = time.perf_counter()
+        synth.write_keyframe_sequence(path, n_keyframes=args.keyframes, H=args.kf_height)
+        gen_s = time.perf_counter() - t0
+        res = {}
+        for graph in (1, 0):
+            r = subprocess.run([exe, path, str(args.kf_passes), str(graph)], capture_output=True,
+                               text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
+            res["graph" if graph else "eager"] = json.loads(r.stdout.strip().splitlines()[-1])
+    g = res["graph"]
+    g["eager_frontend_p50_ms"] = res["eager"]["stage_p50_ms"]["frontend"]
+    g["eager_latency_p50_ms"] = res["eager"]["latency_ms"]["p50"]
+    g["workload"] = (f"{args.keyframes} keyframes of {args.kf_height}x2048 scans, 0.5 m apart, "
+                     f"300-tree forest, 40 % clutter; {args.kf_passes} passes; sequence "
+                     f"generated on the host in {gen_s:.1f} s (not timed)")
+    return g
 
 
 def main():
@@ -176,6 +303,12 @@ def main():
     pose = None
     if not args.no_pose:
         pose = bench_pose(ctx, args)
+    learned = None
+    if not args.no_learned:
+        learned = bench_learned(ctx, args)
+    keyframes = None
+    if not args.no_keyframes:
+        keyframes = bench_keyframes(args)
 
     cpu = None
     if not args.no_cpu:
@@ -239,7 +372,11 @@ def main():
         "radius_err_vs_truth_cm": {"median": float(np.median(rerr) * 100),
                                    "p90": float(np.percentile(rerr, 90) * 100)},
         "cpu_baseline": cpu,
+        "keyframe_latency_ms": None if keyframes is None else
+        {"p50": keyframes["latency_ms"]["p50"], "p99": keyframes["latency_ms"]["p99"]},
         "pose": pose,
+        "learned": learned,
+        "keyframe": keyframes,
     }
     print(json.dumps(line), flush=True)
     plan.close()

@@ -56,6 +56,10 @@ void* sloam_ctx_stream(sloam_ctx* ctx);
  * every launch of a named kernel is bracketed by events; after sloam_sync()
  * sloam_ctx_kernel_time() returns the summed milliseconds and launch count. */
 int sloam_ctx_enable_timing(sloam_ctx* ctx, int enable);
+/* Copy of `bytes` bytes between host and device memory (either direction,
+ * unified addressing) on the context stream; host memory must stay valid
+ * until sloam_sync.                                                         */
+int sloam_memcpy_async(sloam_ctx* ctx, void* dst, const void* src, size_t bytes);
 int sloam_ctx_kernel_time(sloam_ctx* ctx, const char* kernel, double* ms_total, int64_t* launches);
 int sloam_ctx_reset_timing(sloam_ctx* ctx);
 
@@ -355,6 +359,51 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
                   int32_t* d_vertex_int, double* d_vertex_f64, int32_t* d_point_offsets,
                   float* d_points, int32_t* d_counts);
 
+
+/* ---- keyframe front end (one plan, optionally one HIP graph) --------------
+ * Projection + partition (sloam_project_partition), tree instances
+ * (sloam_trellis) and ground-plane cells (sloam_ground_planes) of one
+ * organized height x width scan, on buffers the plan owns: the per-scan part
+ * of Segmentation::run / maskCloud, Instance::computeGraph and
+ * sloam::computeModels' ground cells (sloam/src/segmentation/inference.cpp:
+ * 80-300, trellis.cpp:134-141, sloam.cpp:388-414).  With use_graph the first
+ * exec runs eagerly (sizing the scratch buffers) and captures the sequence
+ * into a HIP graph; every later exec is one graph launch.  The caller writes
+ * the scan into out.xyzi / out.label (device) before each exec.             */
+typedef struct sloam_frontend sloam_frontend;
+typedef struct sloam_frontend_cfg {
+  sloam_proj_cfg proj;
+  sloam_trellis_cfg trellis;
+  sloam_ground_cfg ground;
+  int32_t use_graph;
+  int32_t reserved;
+} sloam_frontend_cfg;
+typedef struct sloam_frontend_out { /* device pointers owned by the plan */
+  float* xyzi;               /* in:  H*W x 4                               */
+  uint8_t* label;            /* in:  H*W                                   */
+  uint8_t* mask;             /* H*W label mask                             */
+  float* ground;             /* H*W x 4 ground points, count in n_ground   */
+  int32_t* n_ground;
+  float* tree;               /* H*W x 4 dense tree cloud                   */
+  int32_t cap_trees, cap_vertices, cap_points;
+  int32_t* tree_offsets;     /* sloam_trellis outputs                      */
+  int32_t* vertex_int;
+  double* vertex_f64;
+  int32_t* point_offsets;
+  float* points;
+  int32_t* trellis_counts;   /* 8 int32                                    */
+  int32_t n_cells;
+  int32_t* cell_info;        /* sloam_ground_planes outputs                */
+  double* cell_model;
+  float* cell_features;
+  int32_t* kept_offsets;
+  int32_t* kept_index;
+} sloam_frontend_out;
+
+int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_frontend** out);
+void sloam_frontend_destroy(sloam_frontend* f);
+int sloam_frontend_buffers(const sloam_frontend* f, sloam_frontend_out* out);
+int sloam_frontend_exec(sloam_frontend* f);
 
 /* ---- learned cylinder proposal (PointNet, BF16 MFMA) -----------------------
  * No counterpart in the reference.  It proposes a cylinder per cluster, so that

@@ -116,6 +116,26 @@ class PoseCfg(C.Structure):
         ("reserved", C.c_int32),
     ]
 
+class FrontendCfg(C.Structure):
+    _fields_ = [
+        ("proj", ProjCfg),
+        ("trellis", TrellisCfg),
+        ("ground", GroundCfg),
+        ("use_graph", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class FrontendOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("xyzi", "label", "mask", "ground", "n_ground", "tree")]
+    _fields_ += [("cap_trees", C.c_int32), ("cap_vertices", C.c_int32), ("cap_points", C.c_int32)]
+    _fields_ += [(n, C.c_void_p) for n in ("tree_offsets", "vertex_int", "vertex_f64",
+                                           "point_offsets", "points", "trellis_counts")]
+    _fields_ += [("n_cells", C.c_int32)]
+    _fields_ += [(n, C.c_void_p) for n in ("cell_info", "cell_model", "cell_features",
+                                           "kept_offsets", "kept_index")]
+
+
 _P = C.c_void_p
 _I32 = C.c_int32
 _I64 = C.c_int64
@@ -147,6 +167,11 @@ PROTOTYPES = {
     "sloam_fit_lines": (C.c_int, [_P, _P, _P, _I32, C.POINTER(LineFitCfg), _P, _P, _P]),
     "sloam_trellis": (C.c_int, [_P, _P, _I32, _I32, C.POINTER(TrellisCfg), _P, _I32, _I32, _I32,
                                 _P, _P, _P, _P, _P, _P]),
+    "sloam_memcpy_async": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "sloam_frontend_create": (C.c_int, [_P, C.POINTER(FrontendCfg), C.POINTER(_P)]),
+    "sloam_frontend_destroy": (None, [_P]),
+    "sloam_frontend_buffers": (C.c_int, [_P, C.POINTER(FrontendOut)]),
+    "sloam_frontend_exec": (C.c_int, [_P]),
     "sloam_cyl_plan_exec_ex": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sloam_cyl_fallback": (C.c_int, [_P, _P, _I32, _P, C.c_float, _P, _P]),
     "sloam_pn_infer": (C.c_int, [_P, _P, _P, _P, _I32, _P]),

@@ -76,6 +76,21 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
+    # benchmark programs over the library (sloam_amd/bench/*.cpp -> sloam_amd/<name>)
+    bench_dir = os.path.join(HERE, "bench")
+    if os.path.isdir(bench_dir):
+        for f in sorted(os.listdir(bench_dir)):
+            if not f.endswith(".cpp"):
+                continue
+            exe = os.path.join(HERE, f[:-4])
+            cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", INCLUDE, "-I", HOST,
+                   os.path.join(bench_dir, f), "-o", exe, "-L", HERE, "-lsloam_gpu",
+                   "-Wl,-rpath,$ORIGIN"]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"bench build failed: {' '.join(cmd)}\n{r.stderr}")
+            if verbose:
+                print(f"built {exe}")
     return LIB
 
 

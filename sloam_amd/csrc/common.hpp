@@ -67,6 +67,15 @@ void sloam_dev_free(void* p);
 // kernel nodes only).
 int sloam_fill_u32(sloam_ctx* ctx, uint32_t* d, size_t n, uint32_t value);
 
+// Stable LSD radix sort (radix.hip) of n (key, value) pairs by key bits
+// [0, end_bit), on the context stream, without memsets (graph-capturable).
+// keys / vals (nullable) hold the input and receive the result; keys_tmp /
+// vals_tmp are same-sized scratch; hist holds sloam_radix_hist_ints(n) ints.
+int sloam_radix_sort(sloam_ctx* ctx, unsigned long long* keys, int32_t* vals,
+                     unsigned long long* keys_tmp, int32_t* vals_tmp, int n, int end_bit,
+                     int32_t* hist);
+size_t sloam_radix_hist_ints(int n);
+
 // ---------------------------------------------------------------------------
 // counter-based random numbers: Philox4x32-10
 // J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw, "Parallel random

@@ -151,6 +151,13 @@ const char* sloam_ctx_last_error(const sloam_ctx* ctx) {
 
 void* sloam_ctx_stream(sloam_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int sloam_memcpy_async(sloam_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return SLOAM_ERR_INVALID_ARG;
+  if (!bytes) return SLOAM_OK;
+  SLOAM_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
+  return SLOAM_OK;
+}
+
 int sloam_ctx_enable_timing(sloam_ctx* ctx, int enable) {
   if (!ctx) return SLOAM_ERR_INVALID_ARG;
   ctx->timing = enable != 0;

@@ -23,7 +23,6 @@
 //    refit over the inliers.  Predicate: Plane::distance(const PointT&)
 //    (sloam/src/objects/plane.cpp:136-151) for a unit normal.
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.hpp"
 
@@ -53,7 +52,7 @@ struct GroundCfgDev {
 __global__ __launch_bounds__(kGB) void ground_bin_kernel(
     const float4* __restrict__ pts, int cap, const int32_t* __restrict__ d_n, GroundCfgDev cfg,
     unsigned long long* __restrict__ key_idx, unsigned long long* __restrict__ key_z,
-    int32_t* __restrict__ vals, int32_t* __restrict__ counts) {
+    int32_t* __restrict__ vals_idx, int32_t* __restrict__ vals_z, int32_t* __restrict__ counts) {
   const int i = blockIdx.x * kGB + threadIdx.x;
   if (i >= cap) return;
   const int n = d_n ? *d_n : cap;
@@ -78,9 +77,11 @@ __global__ __launch_bounds__(kGB) void ground_bin_kernel(
       }
     }
   }
-  key_idx[i] = (cell << 32) | (unsigned long long)(uint32_t)i;
+  // (cell) and (cell, z): the stable sort keeps input order within equal keys
+  key_idx[i] = cell;
   key_z[i] = (cell << 32) | (unsigned long long)orderable(z);
-  vals[i] = i;
+  vals_idx[i] = i;
+  vals_z[i] = i;
 }
 
 __global__ __launch_bounds__(kGB) void ground_offsets_kernel(const int32_t* __restrict__ counts,
@@ -638,20 +639,14 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
   dc.ox = cfg->origin[0];
   dc.oy = cfg->origin[1];
   const int cap = std::max(capacity, 1);
-  // scratch: 2 key arrays x 2 (in/out), 2 value arrays x 2, counts, offsets,
-  // radix-sort temporary storage
-  size_t sort_tmp = 0;
-  {
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, sort_tmp, (unsigned long long*)nullptr,
-                                             (unsigned long long*)nullptr, (int32_t*)nullptr,
-                                             (int32_t*)nullptr, (size_t)cap, 0, 48, ctx->stream);
-    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "radix_sort_pairs(size)");
-  }
+  // scratch: 2 key arrays + their sort buffer, 2 value arrays + sort buffer,
+  // counts, offsets, sort histogram
   const size_t kb = sizeof(unsigned long long) * (size_t)cap;
   const size_t vb = sizeof(int32_t) * (size_t)cap;
   const size_t cb = sizeof(int32_t) * (size_t)(n_cells + 1);
+  const size_t hb = sizeof(int32_t) * sloam_radix_hist_ints(cap);
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
-  const size_t need = 4 * al(kb) + 4 * al(vb) + 2 * al(cb) + al(sort_tmp);
+  const size_t need = 3 * al(kb) + 3 * al(vb) + 2 * al(cb) + al(hb);
   void* base = nullptr;
   int st = sloam_ctx_scratch(ctx, 1, need, &base);
   if (st) return st;
@@ -662,17 +657,14 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
     return (void*)r;
   };
   unsigned long long* kA = (unsigned long long*)take(kb);
-  unsigned long long* kA2 = (unsigned long long*)take(kb);
   unsigned long long* kB = (unsigned long long*)take(kb);
-  unsigned long long* kB2 = (unsigned long long*)take(kb);
-  int32_t* v = (int32_t*)take(vb);
+  unsigned long long* kT = (unsigned long long*)take(kb);
   int32_t* vA = (int32_t*)take(vb);
   int32_t* vB = (int32_t*)take(vb);
-  int32_t* v2 = (int32_t*)take(vb);
+  int32_t* vT = (int32_t*)take(vb);
   int32_t* counts = (int32_t*)take(cb);
   int32_t* offs = (int32_t*)take(cb);
-  void* tmp = take(sort_tmp);
-  (void)v2;
+  int32_t* hist = (int32_t*)take(hb);
   st = sloam_fill_u32(ctx, (uint32_t*)counts, (size_t)n_cells, 0u);
   if (st) return st;
   {
@@ -680,18 +672,16 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
     if (capacity > 0)
       hipLaunchKernelGGL(ground_bin_kernel, dim3((capacity + kGB - 1) / kGB), dim3(kGB), 0,
                          ctx->stream, reinterpret_cast<const float4*>(d_points), capacity,
-                         d_count, dc, kA, kB, v, counts);
+                         d_count, dc, kA, kB, vA, vB, counts);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (capacity > 0) {
     SloamKernelScope ks(ctx, "ground_sort");
-    size_t t1 = sort_tmp;
-    hipError_t e = rocprim::radix_sort_pairs(tmp, t1, kA, kA2, v, vA, (size_t)capacity, 0, 48,
-                                             ctx->stream);
-    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "radix_sort_pairs(cell,index)");
-    t1 = sort_tmp;
-    e = rocprim::radix_sort_pairs(tmp, t1, kB, kB2, v, vB, (size_t)capacity, 0, 48, ctx->stream);
-    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "radix_sort_pairs(cell,z)");
+    // cell ids fit 16 bits (out-of-range points: 0xFFFF)
+    st = sloam_radix_sort(ctx, kA, vA, kT, vT, capacity, 16, hist);
+    if (st) return st;
+    st = sloam_radix_sort(ctx, kB, vB, kT, vT, capacity, 48, hist);
+    if (st) return st;
   }
   {
     SloamKernelScope ks(ctx, "ground_fit");

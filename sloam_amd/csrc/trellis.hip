@@ -12,7 +12,8 @@
 //                       roots ranked in pixel order (ordered block scan) give
 //                       the component numbers; per-component sizes
 //   tr_key + radix sort pixels of components with more than 80 points, keyed
-//                       (component, bottom-up row, column)
+//                       (component, bottom-up row, column); the sort is the
+//                       memset-free LSD sort of radix.hip
 //   tr_segflag .. tr_segscatter
 //                       one segment per (component, row) = one beam of a tree
 //   tr_vertex           one wave per segment: medians of x, y, z (rank
@@ -24,7 +25,6 @@
 #include "common.hpp"
 
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 namespace {
 
@@ -33,11 +33,14 @@ constexpr int kItems = 1024;    // pixels per block of the ordered scans
 constexpr int kAsm = 1024;      // assemble workgroup
 constexpr int kSegLds = 1024;   // beam points staged in LDS (larger beams read HBM)
 constexpr int kVtxBlocks = 2048;  // waves of the grid-stride vertex / copy kernels
-constexpr int kKeyBits = 52;    // component (20) | bottom-up row (16) | column (16)
-constexpr unsigned long long kPad = (1ull << kKeyBits) - 1;
+// sort key: component | bottom-up row | column, with rb / cb bits for the row
+// and column and the component above them; the padding key (all ones below
+// key_bits) sorts last and matches no real pixel
 
 struct TrCfgDev {
   int H, W, N;
+  int rb, cb, key_bits;
+  unsigned long long pad;
   float thr, max_dist;
   int min_label_points, min_row_points, min_vertices, max_vertices;
   double radius_scale;
@@ -232,28 +235,29 @@ __global__ __launch_bounds__(kTB) void tr_key_kernel(const int* __restrict__ lab
   const int p = blockIdx.x * kTB + threadIdx.x;
   if (p >= cfg.N) return;
   const int l = label[p];
-  unsigned long long k = kPad;
+  unsigned long long k = cfg.pad;
   if (l >= 0 && size[l] > cfg.min_label_points) {
     const int r = p / cfg.W, c = p % cfg.W;
-    k = ((unsigned long long)l << 32) | ((unsigned long long)(cfg.H - 1 - r) << 16) |
-        (unsigned long long)c;
+    k = ((unsigned long long)l << (cfg.rb + cfg.cb)) |
+        ((unsigned long long)(cfg.H - 1 - r) << cfg.cb) | (unsigned long long)c;
   }
   keys[p] = k;
 }
 
-__device__ inline bool seg_start_at(const unsigned long long* keys, int i) {
+__device__ inline bool seg_start_at(const unsigned long long* keys, int i, const TrCfgDev& cfg) {
   const unsigned long long k = keys[i];
-  return k != kPad && (i == 0 || (keys[i - 1] >> 16) != (k >> 16));
+  return k != cfg.pad && (i == 0 || (keys[i - 1] >> cfg.cb) != (k >> cfg.cb));
 }
 
 __global__ __launch_bounds__(kTB) void tr_segcount_kernel(const unsigned long long* __restrict__ keys,
-                                                          int N, int* __restrict__ block_counts) {
+                                                          int N, TrCfgDev cfg,
+                                                          int* __restrict__ block_counts) {
   __shared__ int wsum[kTB / 64];
   const int base = blockIdx.x * kItems;
   int c = 0;
   for (int j = threadIdx.x; j < kItems; j += kTB) {
     const int i = base + j;
-    if (i < N && seg_start_at(keys, i)) ++c;
+    if (i < N && seg_start_at(keys, i, cfg)) ++c;
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
@@ -266,8 +270,8 @@ __global__ __launch_bounds__(kTB) void tr_segcount_kernel(const unsigned long lo
 }
 
 __global__ __launch_bounds__(kTB) void tr_segscatter_kernel(
-    const unsigned long long* __restrict__ keys, int N, const int* __restrict__ block_offsets,
-    int* __restrict__ seg_start) {
+    const unsigned long long* __restrict__ keys, int N, TrCfgDev cfg,
+    const int* __restrict__ block_offsets, int* __restrict__ seg_start) {
   __shared__ int wcount[kTB / 64];
   __shared__ int run;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -276,7 +280,7 @@ __global__ __launch_bounds__(kTB) void tr_segscatter_kernel(
   __syncthreads();
   for (int j0 = 0; j0 < kItems; j0 += kTB) {
     const int i = base + j0 + (int)threadIdx.x;
-    const bool f = i < N && seg_start_at(keys, i);
+    const bool f = i < N && seg_start_at(keys, i, cfg);
     const uint64_t bits = __builtin_amdgcn_ballot_w64(f);
     const int before = __builtin_popcountll(bits & ((1ull << lane) - 1ull));
     if (lane == 0) wcount[wave] = __builtin_popcountll(bits);
@@ -324,13 +328,13 @@ __global__ __launch_bounds__(64) void tr_vertex_kernel(
   for (int s = blockIdx.x; s < S; s += gridDim.x) {
     const int start = seg_start[s];
     const unsigned long long k0 = keys[start];
-    const unsigned long long hi = k0 >> 16;
-    const int row = cfg.H - 1 - (int)((k0 >> 16) & 0xFFFFull);
+    const unsigned long long hi = k0 >> cfg.cb;
+    const int row = cfg.H - 1 - (int)((k0 >> cfg.cb) & ((1ull << cfg.rb) - 1ull));
     // beam length: the run of equal (component, row) keys
     int n = 0;
     for (;;) {
       const int i = start + n + lane;
-      const bool same = i < cfg.N && (keys[i] >> 16) == hi;
+      const bool same = i < cfg.N && (keys[i] >> cfg.cb) == hi;
       const uint64_t b = __builtin_amdgcn_ballot_w64(same);
       // keys are sorted: the run is a prefix of the lanes
       const int run = (b == ~0ull) ? 64 : __builtin_ctzll(~b);
@@ -344,7 +348,9 @@ __global__ __launch_bounds__(64) void tr_vertex_kernel(
     o.m = 0;
     if (n > cfg.min_row_points) {
       const bool in_lds = n <= kSegLds;
-      auto col_of = [&](int j) -> int { return (int)(keys[start + j] & 0xFFFFull); };
+      auto col_of = [&](int j) -> int {
+        return (int)(keys[start + j] & ((1ull << cfg.cb) - 1ull));
+      };
       if (in_lds) {
         for (int j = lane; j < n; j += 64) {
           const int c = col_of(j);
@@ -436,8 +442,9 @@ __global__ __launch_bounds__(64) void tr_vertex_kernel(
 
 // --- assembly ----------------------------------------------------------------
 
-__device__ inline int seg_comp(const unsigned long long* keys, const int* seg_start, int s) {
-  return (int)(keys[seg_start[s]] >> 32);
+__device__ inline int seg_comp(const unsigned long long* keys, const int* seg_start, int s,
+                               const TrCfgDev& cfg) {
+  return (int)(keys[seg_start[s]] >> (cfg.rb + cfg.cb));
 }
 
 __global__ __launch_bounds__(kAsm) void tr_assemble_kernel(
@@ -458,16 +465,16 @@ __global__ __launch_bounds__(kAsm) void tr_assemble_kernel(
     const int e = block_excl<kAsm>(v, wtot, &all);
     if (s < S) {
       segP[s] = carry + e;
-      const int c = seg_comp(keys, seg_start, s);
-      if (s == 0 || seg_comp(keys, seg_start, s - 1) != c) compbase[c] = carry + e;
+      const int c = seg_comp(keys, seg_start, s, cfg);
+      if (s == 0 || seg_comp(keys, seg_start, s - 1, cfg) != c) compbase[c] = carry + e;
     }
     carry += all;
   }
   __syncthreads();
   // B: valid vertices per component
   for (int s = threadIdx.x; s < S; s += kAsm) {
-    const int c = seg_comp(keys, seg_start, s);
-    if (s == S - 1 || seg_comp(keys, seg_start, s + 1) != c)
+    const int c = seg_comp(keys, seg_start, s, cfg);
+    if (s == S - 1 || seg_comp(keys, seg_start, s + 1, cfg) != c)
       comptotal[c] = segP[s] + seg_out[s].valid - compbase[c];
   }
   __syncthreads();
@@ -478,8 +485,8 @@ __global__ __launch_bounds__(kAsm) void tr_assemble_kernel(
     int tf = 0, vf = 0, pf = 0, c = -1, rank = 0;
     bool start = false;
     if (s < S) {
-      c = seg_comp(keys, seg_start, s);
-      start = (s == 0 || seg_comp(keys, seg_start, s - 1) != c);
+      c = seg_comp(keys, seg_start, s, cfg);
+      start = (s == 0 || seg_comp(keys, seg_start, s - 1, cfg) != c);
       const bool tree_kept = comptotal[c] > cfg.min_vertices;
       rank = segP[s] - compbase[c];
       const bool vk = seg_out[s].valid && tree_kept && rank < cfg.max_vertices;
@@ -496,7 +503,8 @@ __global__ __launch_bounds__(kAsm) void tr_assemble_kernel(
       if (tf && ti < cfg.cap_trees) tree_offsets[ti] = vi;
       if (vf && vi < cfg.cap_vertices) {
         const SegOut o = seg_out[s];
-        const int row = cfg.H - 1 - (int)((keys[seg_start[s]] >> 16) & 0xFFFFull);
+        const int row =
+            cfg.H - 1 - (int)((keys[seg_start[s]] >> cfg.cb) & ((1ull << cfg.rb) - 1ull));
         vertex_int[4 * (size_t)vi + 0] = c;
         vertex_int[4 * (size_t)vi + 1] = row;
         vertex_int[4 * (size_t)vi + 2] = o.m;
@@ -583,15 +591,18 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
   dc.cap_trees = cap_trees;
   dc.cap_vertices = cap_vertices;
   dc.cap_points = cap_points;
+  auto bits = [](long long v) {
+    int b = 0;
+    while ((1ll << b) < v) ++b;
+    return b;
+  };
+  dc.rb = bits(height);
+  dc.cb = bits(width);
+  dc.key_bits = dc.rb + dc.cb + bits(N64 + 1);
+  dc.pad = (1ull << dc.key_bits) - 1ull;
   const int n1 = std::max(N, 1);
   const int n_blocks = (n1 + kItems - 1) / kItems;
-  size_t sort_tmp = 0;
-  {
-    hipError_t e = rocprim::radix_sort_keys(nullptr, sort_tmp, (unsigned long long*)nullptr,
-                                            (unsigned long long*)nullptr, (size_t)n1, 0, kKeyBits,
-                                            ctx->stream);
-    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "radix_sort_keys(size)");
-  }
+  const size_t sort_tmp = sizeof(int32_t) * sloam_radix_hist_ints(n1);
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
   const size_t ib = sizeof(int) * (size_t)n1;
   const size_t kb = sizeof(unsigned long long) * (size_t)n1;
@@ -654,30 +665,29 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
     SLOAM_HIP(ctx, hipGetLastError());
     {
       SloamKernelScope ks(ctx, "trellis_sort");
-      size_t t1 = sort_tmp;
-      hipError_t e = rocprim::radix_sort_keys(sort_buf, t1, keys, keys2, (size_t)N, 0, kKeyBits,
-                                              ctx->stream);
-      if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "radix_sort_keys");
+      st = sloam_radix_sort(ctx, keys, nullptr, keys2, nullptr, N, dc.key_bits,
+                            (int32_t*)sort_buf);
+      if (st) return st;
     }
     {
       SloamKernelScope ks(ctx, "trellis_segments");
-      hipLaunchKernelGGL(tr_segcount_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, keys2, N,
+      hipLaunchKernelGGL(tr_segcount_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, keys, N, dc,
                          bc);
       hipLaunchKernelGGL(tr_scan_kernel, dim3(1), dim3(kTB), 0, ctx->stream, bc, n_blocks,
                          counters + kCntSegs);
-      hipLaunchKernelGGL(tr_segscatter_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, keys2,
-                         N, bc, seg_start);
+      hipLaunchKernelGGL(tr_segscatter_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, keys,
+                         N, dc, bc, seg_start);
     }
     {
       SloamKernelScope ks(ctx, "trellis_vertex");
       hipLaunchKernelGGL(tr_vertex_kernel, dim3(kVtxBlocks), dim3(64), 0, ctx->stream, cloud,
-                         keys2, seg_start, counters, dc, seg_out, tmp);
+                         keys, seg_start, counters, dc, seg_out, tmp);
     }
     SLOAM_HIP(ctx, hipGetLastError());
   }
   {
     SloamKernelScope ks(ctx, "trellis_assemble");
-    hipLaunchKernelGGL(tr_assemble_kernel, dim3(1), dim3(kAsm), 0, ctx->stream, keys2, seg_start,
+    hipLaunchKernelGGL(tr_assemble_kernel, dim3(1), dim3(kAsm), 0, ctx->stream, keys, seg_start,
                        seg_out, counters, dc, segP, compbase, comptotal, seg_vi, seg_po,
                        d_tree_offsets, d_vertex_int, d_vertex_f64, d_point_offsets, d_counts);
     if (N > 0)

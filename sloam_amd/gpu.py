@@ -9,8 +9,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (CYL_MODEL_STRIDE, CylFitCfg, GroundCfg, LineFitCfg, PlaneFitCfg, PoseCfg,
-                   ProjCfg, TrellisCfg, check)
+from ._lib import (CYL_MODEL_STRIDE, CylFitCfg, FrontendCfg, FrontendOut, GroundCfg, LineFitCfg,
+                   PlaneFitCfg, PoseCfg, ProjCfg, TrellisCfg, check)
 
 
 def _ptr(t):
@@ -448,3 +448,92 @@ def trellis(ctx, cloud, H, W, **kw):
             "vertex_f64": out["vertex_f64"].cpu().numpy()[:nv],
             "point_offsets": out["point_offsets"].cpu().numpy()[:nv + 1],
             "points": out["points"].cpu().numpy()[:npt]}
+
+
+# sloam.yaml ground settings (2 x 18 bins, 5-25 m, bottom 5 %, 5 features)
+GROUND_DEFAULTS = dict(radii_bins=2, theta_bins=18, min_dist=5.0, max_dist=25.0,
+                       retain_thresh=0.05, num_features=5)
+
+
+class Frontend:
+    """Keyframe front end (sloam_frontend_*): projection + partition, tree
+    instances and ground cells of one organised H x W scan on plan-owned
+    buffers, optionally replayed as one HIP graph."""
+
+    def __init__(self, ctx, H, W, use_graph=True, fov_up=22.5, fov_down=-22.5, destagger=False,
+                 trellis=None, ground=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.H, self.W, self.N = H, W, H * W
+        tc = dict(TRELLIS_DEFAULTS)
+        tc.update(trellis or {})
+        gc = dict(GROUND_DEFAULTS)
+        gc.update(ground or {})
+        self.nf = gc["num_features"]
+        self.cfg = FrontendCfg(proj=ProjCfg(height=H, width=W, fov_up_deg=fov_up,
+                                            fov_down_deg=fov_down, destagger=int(destagger)),
+                               trellis=TrellisCfg(**tc), ground=GroundCfg(**gc),
+                               use_graph=int(use_graph), reserved=0)
+        h = C.c_void_p()
+        check(self.lib.sloam_frontend_create(ctx.handle, C.byref(self.cfg), C.byref(h)),
+              ctx.handle)
+        self.handle = h
+        self.out = FrontendOut()
+        check(self.lib.sloam_frontend_buffers(h, C.byref(self.out)), ctx.handle)
+        self._host = []
+
+    def close(self):
+        if self.handle:
+            self.lib.sloam_frontend_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _copy(self, dst, src, nbytes):
+        check(self.lib.sloam_memcpy_async(self.ctx.handle, dst, src, nbytes), self.ctx.handle)
+
+    def upload(self, xyzi, label):
+        x = np.ascontiguousarray(xyzi, np.float32).reshape(self.N, 4)
+        lab = np.ascontiguousarray(label, np.uint8).reshape(self.N)
+        self._host = [x, lab]  # kept until the next sync
+        self._copy(self.out.xyzi, x.ctypes.data, x.nbytes)
+        self._copy(self.out.label, lab.ctypes.data, lab.nbytes)
+
+    def exec(self):
+        check(self.lib.sloam_frontend_exec(self.handle), self.ctx.handle)
+
+    def _get(self, ptr, shape, dtype):
+        a = np.empty(shape, dtype)
+        if a.nbytes:
+            self._copy(a.ctypes.data, ptr, a.nbytes)
+        return a
+
+    def download(self):
+        """Synchronises and returns the outputs as numpy arrays."""
+        o = self.out
+        cnt = self._get(o.trellis_counts, 8, np.int32)
+        ng = self._get(o.n_ground, 1, np.int32)
+        self.ctx.sync()
+        nt, nv, npt = int(cnt[0]), int(cnt[1]), int(cnt[2])
+        nc = o.n_cells
+        r = {
+            "mask": self._get(o.mask, self.N, np.uint8),
+            "ground": self._get(o.ground, (int(ng[0]), 4), np.float32),
+            "tree": self._get(o.tree, (self.N, 4), np.float32),
+            "trellis_counts": cnt,
+            "tree_offsets": self._get(o.tree_offsets, nt + 1, np.int32),
+            "vertex_int": self._get(o.vertex_int, (nv, 4), np.int32),
+            "vertex_f64": self._get(o.vertex_f64, (nv, 4), np.float64),
+            "point_offsets": self._get(o.point_offsets, nv + 1, np.int32),
+            "points": self._get(o.points, (npt, 4), np.float32),
+            "cell_info": self._get(o.cell_info, (nc, 4), np.int32),
+            "cell_model": self._get(o.cell_model, (nc, 8), np.float64),
+            "cell_features": self._get(o.cell_features, (nc, self.nf, 4), np.float32),
+        }
+        self.ctx.sync()
+        self._host = []
+        return r

@@ -303,15 +303,19 @@ void sloam::binGroundPoints(const SE3 pose, const VectorType& points, GroundGrid
 void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
                           std::vector<Plane>& planes) {
   // ground: all cells of the scan in one call (binned at the origin,
-  // sloam.cpp:390-393)
-  const auto cells = sloam_gpu::groundPlanes(in.groundCloud->points, groundCfg(SE3()));
+  // sloam.cpp:390-393), unless the caller brings them
+  std::vector<Plane> cells = in.groundCells;
+  if (cells.empty()) {
+    for (const auto& cell : sloam_gpu::groundPlanes(in.groundCloud->points, groundCfg(SE3()))) {
+      PlaneParameters fitted;
+      for (int i = 0; i < 4; ++i) fitted.plane(i) = cell.plane[i];
+      for (int i = 0; i < 3; ++i) fitted.centroid(i) = cell.centroid[i];
+      cells.emplace_back(fitted, cell.features, cell.valid);
+    }
+  }
   const Vector3 src_vec(0, 0, 1);
   const Matrix4 tfm = in.poseEstimate.inverse().matrix().transpose();
-  for (const auto& cell : cells) {
-    PlaneParameters fitted;
-    for (int i = 0; i < 4; ++i) fitted.plane(i) = cell.plane[i];
-    for (int i = 0; i < 3; ++i) fitted.centroid(i) = cell.centroid[i];
-    Plane ground(fitted, cell.features, cell.valid);
+  for (const Plane& ground : cells) {
     if (!ground.isValid) continue;
     // roughly horizontal in the map frame (sloam.cpp:401-406)
     const Vector4 normal = tfm * ground.model.plane;

@@ -26,6 +26,10 @@ struct SloamInput {
   CloudT::Ptr groundCloud;
   std::vector<Cylinder> mapModels;
   std::vector<std::vector<TreeVertex>> landmarks;
+  // Optional: the ground cells already fitted on the device (one Plane per
+  // radii x theta cell, r-major, isValid as fitted), e.g. by the keyframe
+  // front end; computeModels then skips binning groundCloud.
+  std::vector<Plane> groundCells;
 };
 
 struct SloamOutput {

@@ -312,6 +312,154 @@ def make_scan(seed=0, H=64, W=2048, fov_deg=22.5, n_trees=40, r_min=0.1, r_max=0
             "ground": np.concatenate([gn, [gd]]), "H": H, "W": W, "fov_deg": fov_deg}
 
 
+def make_forest(seed=4, n_trees=300, length=200.0, width=60.0, r_min=0.1, r_max=0.4,
+                h_min=2.0, h_max=8.0, tilt_max_deg=5.0, slope_max_deg=3.0):
+    """A static forest: n_trees tilted trunks uniformly in [0, length] x
+    [-width/2, width/2] (keeping 1.5 m off the path y = 0), on a plane
+    z = g0 x + g1 y (slope <= slope_max_deg)."""
+    rng = np.random.default_rng(seed)
+    slope = math.radians(rng.uniform(0, slope_max_deg))
+    gaz = rng.uniform(-math.pi, math.pi)
+    g = (math.tan(slope) * math.cos(gaz), math.tan(slope) * math.sin(gaz))
+    trees = []
+    while len(trees) < n_trees:
+        x = rng.uniform(0.0, length)
+        y = rng.uniform(-width / 2, width / 2)
+        if abs(y) < 1.5:
+            continue
+        tilt = math.radians(rng.uniform(0, tilt_max_deg))
+        ta = rng.uniform(-math.pi, math.pi)
+        ax = np.array([math.sin(tilt) * math.cos(ta), math.sin(tilt) * math.sin(ta),
+                       math.cos(tilt)])
+        trees.append({"base": np.array([x, y, g[0] * x + g[1] * y]), "axis": ax,
+                      "radius": rng.uniform(r_min, r_max), "height": rng.uniform(h_min, h_max)})
+    return {"trees": trees, "ground": g}
+
+
+def trajectory_pose(s, sensor_height=1.5):
+    """Pose (R, t) at arc length s of the path: along x with a gentle lateral
+    weave, heading along the path tangent, sensor 1.5 m above the origin's
+    ground."""
+    y = 1.0 * math.sin(s / 25.0)
+    dy = math.cos(s / 25.0) / 25.0
+    yaw = math.atan2(dy, 1.0)
+    c, sn = math.cos(yaw), math.sin(yaw)
+    R = np.array([[c, -sn, 0.0], [sn, c, 0.0], [0.0, 0.0, 1.0]])
+    return R, np.array([s, y, sensor_height])
+
+
+def scan_from(forest, R, t, seed=0, H=128, W=2048, fov_deg=22.5, clutter_frac=0.4,
+              tree_noise=0.02, ground_noise=0.03, range_min=1.0, range_max=40.0,
+              occlusion=0.1):
+    """One organised H x W scan of `forest` from the sensor pose (R, t)
+    (world = R sensor + t), points in the sensor frame.  Understory clutter
+    replaces clutter_frac of the returns; `occlusion` of each trunk's hits (a
+    random vertical band) are lost to foliage.  Same layout and outputs as
+    make_scan."""
+    rng = np.random.default_rng(seed)
+    fov = math.radians(fov_deg)
+    pitch = fov - (np.arange(H) + 0.5) / H * (2 * fov)
+    yaw = (np.arange(W) + 0.5) / W * 2 * math.pi - math.pi
+    P, A = np.meshgrid(pitch, -yaw, indexing="ij")
+    ds = np.stack([np.cos(P) * np.cos(A), np.cos(P) * np.sin(A), np.sin(P)], -1).reshape(-1, 3)
+    dw = ds @ R.T
+    n = H * W
+    best = np.full(n, np.inf)
+    label = np.zeros(n, np.uint8)
+    inst = np.full(n, -1, np.int32)
+    g0, g1 = forest["ground"]
+    # ground z = g0 x + g1 y  <=>  gn . p + gd = 0
+    gn = np.array([-g0, -g1, 1.0])
+    den = dw @ gn
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tg = -(gn @ t) / den
+    ok = (den < -1e-6) & (tg > 0) & (tg < range_max)
+    best[ok] = tg[ok]
+    label[ok] = 1
+    azw = np.arctan2(dw[:, 1], dw[:, 0])
+    for k, tr in enumerate(forest["trees"]):
+        D = np.hypot(*(tr["base"][:2] - t[:2]))
+        if D > range_max + 1.0:
+            continue
+        z0 = rng.uniform(0, tr["height"])  # occluded band (drawn for every tree in range)
+        ax, base, rad = tr["axis"], tr["base"] - t, tr["radius"]
+        # only rays within the trunk's azimuth window (radius + lean margin)
+        margin = rad + tr["height"] * math.sin(math.acos(min(1.0, ax[2]))) + 0.3
+        half = math.asin(min(1.0, margin / D)) if D > margin else math.pi
+        bearing = math.atan2(base[1], base[0])
+        sel = np.nonzero(np.abs((azw - bearing + math.pi) % (2 * math.pi) - math.pi) <= half)[0]
+        if sel.size == 0:
+            continue
+        dsel = dw[sel]
+        w = dsel - (dsel @ ax)[:, None] * ax[None, :]
+        c0 = -base - (-base @ ax) * ax
+        qa = (w * w).sum(1)
+        qb = 2 * (w @ c0)
+        qc = c0 @ c0 - rad * rad
+        disc = qb * qb - 4 * qa * qc
+        hit = disc > 0
+        tt = np.full(sel.size, np.inf)
+        tt[hit] = (-qb[hit] - np.sqrt(disc[hit])) / (2 * qa[hit])
+        with np.errstate(invalid="ignore"):
+            hz = (np.where(hit, tt, 0.0)[:, None] * dsel - base) @ ax
+        hit &= (tt > 0) & (hz >= 0) & (hz <= tr["height"])
+        if occlusion > 0:
+            hit &= ~((hz > z0) & (hz < z0 + occlusion * tr["height"]))
+        closer = hit & (tt < best[sel])
+        idx = sel[closer]
+        best[idx] = tt[closer]
+        label[idx] = 2
+        inst[idx] = k
+    noise = np.where(label == 2, tree_noise, ground_noise)
+    hitm = np.isfinite(best) & (best >= range_min)
+    rv = best + rng.normal(0, 1, n) * noise
+    cl = hitm & (rng.uniform(0, 1, n) < clutter_frac)
+    rv[cl] = rng.uniform(range_min, np.minimum(best[cl], range_max))
+    label[cl] = 0
+    inst[cl] = -1
+    xyz = rv[:, None] * ds
+    xyz[~hitm] = np.nan
+    label[~hitm] = 0
+    inst[~hitm] = -1
+    xyzi = np.empty((n, 4), np.float32)
+    xyzi[:, :3] = xyz
+    xyzi[:, 3] = rng.uniform(0, 255, n)
+    return {"xyzi": xyzi, "label": label, "instance": inst, "H": H, "W": W, "fov_deg": fov_deg}
+
+
+def write_keyframe_sequence(path, n_keyframes=16, H=128, W=2048, spacing=0.5, start=20.0,
+                            seed=4, n_trees=300, clutter_frac=0.4, odom_sigma_t=0.05,
+                            odom_sigma_yaw_deg=0.5):
+    """Writes the keyframe sequence file of sloam_amd/bench/keyframe_bench.cpp:
+    scans of one forest (make_forest) from poses `spacing` metres apart along
+    trajectory_pose, with noisy odometry increments between them."""
+    import struct
+    forest = make_forest(seed=seed, n_trees=n_trees)
+    rng = np.random.default_rng(seed + 1)
+    prev = None
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iii", n_keyframes, H, W))
+        for k in range(n_keyframes):
+            R, t = trajectory_pose(start + k * spacing)
+            truth = np.concatenate([t, R_to_quat(R)])
+            if prev is None:
+                odom = np.array([0, 0, 0, 0, 0, 0, 1.0])
+            else:
+                Rp, tp = prev
+                Rd = Rp.T @ R
+                td = Rp.T @ (t - tp) + rng.normal(0, odom_sigma_t, 3) * np.array([1, 1, 0.2])
+                Rd = _rotvec_to_R(np.array([0, 0, math.radians(
+                    rng.normal(0, odom_sigma_yaw_deg))])) @ Rd
+                odom = np.concatenate([td, R_to_quat(Rd)])
+            prev = (R, t)
+            sc = scan_from(forest, R, t, seed=seed * 1000 + k, H=H, W=W,
+                           clutter_frac=clutter_frac)
+            f.write(np.asarray(truth, "<f8").tobytes())
+            f.write(np.asarray(odom, "<f8").tobytes())
+            f.write(np.ascontiguousarray(sc["xyzi"], "<f4").tobytes())
+            f.write(np.ascontiguousarray(sc["label"], np.uint8).tobytes())
+
+
 XYYAW_MASK = 0b100011  # t_x, t_y, w_z free (sloam.cpp:75-77 fixes {2, 3, 4})
 ZROLLPITCH_MASK = 0b011100  # t_z, w_x, w_y free (sloam.cpp:136-138 fixes {0, 1, 5})
 
