@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--hyp-chunk", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default launch shape instead of sloam_cyl_autotune's")
     ap.add_argument("--no-pose", action="store_true", help="skip the batched pose leg")
     ap.add_argument("--pose-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
@@ -240,17 +242,24 @@ def main():
     import torch
 
     from sloam_amd import synth
-    from sloam_amd.gpu import Context, CylinderFitPlan
+    from sloam_amd.gpu import Context, CylinderFitPlan, autotune_cylinder_fit
 
     thr, max_radius, seed = 0.05, 1.0, 0x5EED
     d = synth.make_clusters(args.clusters, seed=1)
     n_points = int(d["offsets"][-1])
     ctx = Context(0)
-    plan = CylinderFitPlan(ctx, d["offsets"], n_hyp=args.hyp, thr=thr, max_radius=max_radius,
-                           seed=seed, hyp_chunk=args.hyp_chunk, tile_points=args.tile)
     dev = torch.device("cuda", 0)
     xyz = torch.as_tensor(d["xyz"]).to(dev)
     off = torch.as_tensor(d["offsets"]).to(dev)
+    chunk, tile = args.hyp_chunk, args.tile
+    tuned = False
+    if chunk == 0 and tile == 0 and not args.no_autotune:
+        # launch shape picked by sloam_cyl_autotune on this batch (untimed)
+        chunk, tile = autotune_cylinder_fit(ctx, xyz, off, d["offsets"], n_hyp=args.hyp, thr=thr,
+                                            max_radius=max_radius, seed=seed)
+        tuned = True
+    plan = CylinderFitPlan(ctx, d["offsets"], n_hyp=args.hyp, thr=thr, max_radius=max_radius,
+                           seed=seed, hyp_chunk=chunk, tile_points=tile)
     out = plan.alloc_outputs(counts=False, mask=True, best=True)
     torch.cuda.synchronize()
 
@@ -353,6 +362,7 @@ def main():
             "hypotheses": args.hyp,
             "points": n_points,
             "parallelism": "single GPU",
+            "launch_shape": {"hyp_chunk": chunk, "tile_points": tile, "autotuned": tuned},
         },
         "roofline": {
             "bound": "valu",

@@ -134,6 +134,14 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* plan, const float* d_xyz, const int32
                            const float* d_proposal, const int32_t* d_active, int32_t* d_counts,
                            int32_t* d_samples, uint32_t* d_mask, int32_t* d_best,
                            double* d_model);
+/* Launch-shape autotuner: times plans with hyp_chunk in {64, 128, 256, 512}
+ * and tile_points in {1024, 2048, 4096} on this batch (d_xyz / d_offsets on
+ * the device, h_offsets on the host) and returns cfg with the fastest pair in
+ * *out.  Every shape gives identical results; the choice is cached per
+ * (device, clusters, points, hypotheses).                                   */
+int sloam_cyl_autotune(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
+                       const sloam_cyl_fit_cfg* cfg, const float* d_xyz,
+                       const int32_t* d_offsets, sloam_cyl_fit_cfg* out);
 /* d_active[c] = 1 for clusters (>= 3 points) whose model is invalid or keeps
  * fewer than min_inlier_ratio x n points as inliers (model[7]); d_n_active
  * (nullable, device) gets their number.  With exec_ex this runs a fallback

@@ -174,6 +174,8 @@ PROTOTYPES = {
     "sloam_frontend_exec": (C.c_int, [_P]),
     "sloam_cyl_plan_exec_ex": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sloam_cyl_fallback": (C.c_int, [_P, _P, _I32, _P, C.c_float, _P, _P]),
+    "sloam_cyl_autotune": (C.c_int, [_P, _P, _I32, C.POINTER(CylFitCfg), _P, _P,
+                                     C.POINTER(CylFitCfg)]),
     "sloam_pn_infer": (C.c_int, [_P, _P, _P, _P, _I32, _P]),
     "sloam_pn_loss_grad": (C.c_int, [_P, _P, _P, _P, _I32, _P, C.POINTER(PnTrainCfg), _P, _P]),
     "sloam_pn_train_step": (C.c_int, [_P, _P, _P, _P, _I32, C.POINTER(PnTrainCfg), _P, _P, _I32,

@@ -34,7 +34,17 @@ struct sloam_ctx {
   std::vector<hipEvent_t> event_pool;
   // grow-on-demand scratch buffers, one per slot
   std::vector<std::pair<void*, size_t>> scratch;
+  // auxiliary streams (default priority) and their fork / join events, for
+  // independent launches that overlap; created on first use
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[2] = {nullptr, nullptr};
 };
+
+// Makes aux stream i wait for the work enqueued so far on ctx->stream.
+int sloam_ctx_fork(sloam_ctx* ctx, int i);
+// Makes ctx->stream wait for the work enqueued so far on aux stream i.
+int sloam_ctx_join(sloam_ctx* ctx, int i);
 
 // Scratch buffer `slot` of at least `bytes` bytes (reallocated when too small;
 // the previous contents are not kept).  Callers own a slot for the duration of

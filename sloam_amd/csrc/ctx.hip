@@ -90,6 +90,23 @@ int sloam_fill_u32(sloam_ctx* ctx, uint32_t* d, size_t n, uint32_t value) {
   return SLOAM_OK;
 }
 
+int sloam_ctx_fork(sloam_ctx* ctx, int i) {
+  if (!ctx->aux[i]) {
+    SLOAM_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));
+    SLOAM_HIP(ctx, hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
+  }
+  if (!ctx->fork_ev) SLOAM_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+  SLOAM_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+  SLOAM_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], ctx->fork_ev, 0));
+  return SLOAM_OK;
+}
+
+int sloam_ctx_join(sloam_ctx* ctx, int i) {
+  SLOAM_HIP(ctx, hipEventRecord(ctx->join_ev[i], ctx->aux[i]));
+  SLOAM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0));
+  return SLOAM_OK;
+}
+
 extern "C" {
 
 int sloam_ctx_create(int device, sloam_ctx** out) {
@@ -115,6 +132,14 @@ void sloam_ctx_destroy(sloam_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->aux[i]) {
+      (void)hipStreamSynchronize(ctx->aux[i]);
+      (void)hipStreamDestroy(ctx->aux[i]);
+    }
+    if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   for (auto& kv : ctx->timers)
     for (auto& pr : kv.second.pending) {
       (void)hipEventDestroy(pr.first);

@@ -27,8 +27,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <numeric>
+#include <tuple>
 
 namespace {
 
@@ -289,6 +292,7 @@ __global__ __launch_bounds__(kBlock) void cyl_score_kernel(
 
   const int4 w = work[blockIdx.x];
   const int c = w.x, h0 = w.y, hc = w.z;
+  if (c < 0) return;  // padding of the per-XCD work order
   if (active && !active[c]) return;
   const float* pc = cluster_prop(prop, c);
   const int off = offsets[c];
@@ -481,7 +485,7 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
     const float* __restrict__ prop, const int32_t* __restrict__ active,
     uint32_t* __restrict__ mask, int32_t* __restrict__ best_out, double* __restrict__ model) {
   constexpr int W = BLOCK / 64;
-  __shared__ double red[W * 20];
+  __shared__ double red[W * 21];  // bsum of up to 21 values per wave
   __shared__ long long redk[W];
   __shared__ HypOut best_h;
   extern __shared__ uint32_t bits[];  // membership of the current round
@@ -733,6 +737,7 @@ struct sloam_cyl_plan {
   int bits_small = 0, bits_large = 0;  // LDS bytes of the membership bits
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
+  int n_xcd = 1;
 };
 
 static int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -794,21 +799,43 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
   }
   p->tile = std::min(p->tile, ((kMaxLds - hyp_bytes) / 12) / 64 * 64);
 
-  // work list: clusters by decreasing size (ties: lower index), chunks in order
+  // work list: clusters by decreasing size (ties: lower index), each cluster
+  // split into hypothesis chunks.  Workgroups go to the XCDs round robin
+  // (workgroup b on XCD b mod n_xcd), each XCD with its own L2: clusters are
+  // dealt to per-XCD bins by least load (largest first) and the launch order
+  // interleaves the bins, so all chunks of a cluster run on one XCD and read
+  // its points through one L2.  Bins are padded with empty items (cluster -1).
   std::vector<int> order(n_clusters);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return (h_offsets[a + 1] - h_offsets[a]) > (h_offsets[b + 1] - h_offsets[b]);
   });
-  std::vector<int4> work;
-  work.reserve((size_t)n_clusters * ((cfg->n_hyp + p->hcap - 1) / p->hcap));
+  int n_xcd = 1;
+  if (hipDeviceGetAttribute(&n_xcd, hipDeviceAttributeNumberOfXccs, ctx->device) != hipSuccess ||
+      n_xcd < 1)
+    n_xcd = 1;
+  (void)hipGetLastError();
+  const int chunks = (cfg->n_hyp + p->hcap - 1) / p->hcap;
+  std::vector<std::vector<int4>> bins(n_xcd);
+  std::vector<long long> load(n_xcd, 0);
   for (int c : order) {
+    const int len = h_offsets[c + 1] - h_offsets[c];
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     for (int h0 = 0; h0 < cfg->n_hyp; h0 += p->hcap) {
       const int hc = std::min(p->hcap, cfg->n_hyp - h0);
-      work.push_back(make_int4(c, h0, hc, h_offsets[c + 1] - h_offsets[c]));
+      bins[x].push_back(make_int4(c, h0, hc, len));
     }
+    load[x] += (long long)std::max(len, 1) * chunks;
   }
+  size_t longest = 0;
+  for (const auto& b : bins) longest = std::max(longest, b.size());
+  std::vector<int4> work;
+  work.reserve(longest * n_xcd);
+  for (size_t k = 0; k < longest; ++k)
+    for (int x = 0; x < n_xcd; ++x)
+      work.push_back(k < bins[x].size() ? bins[x][k] : make_int4(-1, 0, 0, 0));
   p->n_work = (int)work.size();
+  p->n_xcd = n_xcd;
   // refit lists: clusters up to kSmallRefit points get one wave, larger ones
   // a 256-thread workgroup; both in decreasing size
   std::vector<int32_t> small, large;
@@ -880,6 +907,76 @@ int sloam_cyl_fallback(sloam_ctx* ctx, const int32_t* d_offsets, int32_t n_clust
   return SLOAM_OK;
 }
 
+int sloam_cyl_autotune(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
+                       const sloam_cyl_fit_cfg* cfg, const float* d_xyz, const int32_t* d_offsets,
+                       sloam_cyl_fit_cfg* out) {
+  if (!ctx || !h_offsets || !cfg || !out || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
+  *out = *cfg;
+  if (n_clusters == 0) return SLOAM_OK;
+  if (!d_xyz || !d_offsets) return SLOAM_ERR_INVALID_ARG;
+  // choices are cached per batch shape: every shape gives the same results
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, std::pair<int, int>> cache;
+  const auto key = std::make_tuple(ctx->device, n_clusters, (int)h_offsets[n_clusters], cfg->n_hyp);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      out->hyp_chunk = it->second.first;
+      out->tile_points = it->second.second;
+      return SLOAM_OK;
+    }
+  }
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  double* d_model = nullptr;
+  int st = sloam_dev_alloc(ctx, (void**)&d_model,
+                           sizeof(double) * SLOAM_CYL_MODEL_STRIDE * (size_t)n_clusters);
+  if (st) return st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float best_ms = 1e30f;
+  int best_chunk = 0, best_tile = 0;
+  const int chunks[] = {64, 128, 256, 512};
+  const int tiles[] = {1024, 2048, 4096};
+  for (int ch : chunks) {
+    if (ch > round_up(cfg->n_hyp, kWaves * kHB) && ch != 64) continue;
+    for (int tl : tiles) {
+      sloam_cyl_fit_cfg c = *cfg;
+      c.hyp_chunk = ch;
+      c.tile_points = tl;
+      sloam_cyl_plan* p = nullptr;
+      if (sloam_cyl_plan_create(ctx, h_offsets, n_clusters, &c, &p) != SLOAM_OK) continue;
+      float ms = 0.f;
+      bool ok = sloam_cyl_plan_exec(p, d_xyz, d_offsets, nullptr, nullptr, nullptr, nullptr,
+                                    d_model) == SLOAM_OK;
+      ok = ok && hipEventRecord(e0, ctx->stream) == hipSuccess;
+      for (int r = 0; r < 3 && ok; ++r)
+        ok = sloam_cyl_plan_exec(p, d_xyz, d_offsets, nullptr, nullptr, nullptr, nullptr,
+                                 d_model) == SLOAM_OK;
+      ok = ok && hipEventRecord(e1, ctx->stream) == hipSuccess &&
+           hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+      sloam_cyl_plan_destroy(p);
+      if (ok && ms < best_ms) {
+        best_ms = ms;
+        best_chunk = ch;
+        best_tile = tl;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  sloam_dev_free(d_model);
+  if (best_chunk == 0) return SLOAM_ERR_HIP;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = std::make_pair(best_chunk, best_tile);
+  }
+  out->hyp_chunk = best_chunk;
+  out->tile_points = best_tile;
+  return SLOAM_OK;
+}
+
 void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
   if (!p) return;
   sloam_dev_free(p->d_work);
@@ -929,18 +1026,26 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {
+    // the two size tiers run concurrently: the 4-wave tier on an auxiliary
+    // stream forked from and joined back into the context stream
     SloamKernelScope ks(ctx, "cyl_refit");
+    int st = SLOAM_OK;
+    if (p->n_large > 0) {
+      st = sloam_ctx_fork(ctx, 0);
+      if (st) return st;
+      hipLaunchKernelGGL(cyl_refit_kernel<256>, dim3(p->n_large), dim3(256), p->bits_large,
+                         ctx->aux[0], d_xyz, d_offsets, p->d_frames, counts,
+                         p->d_lists + p->n_small, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1,
+                         tag, cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, d_mask,
+                         d_best, d_model);
+    }
     if (p->n_small > 0)
       hipLaunchKernelGGL(cyl_refit_kernel<64>, dim3(p->n_small), dim3(64), p->bits_small,
                          ctx->stream, d_xyz, d_offsets, p->d_frames, counts, p->d_lists,
                          cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, cfg.refit_rounds,
                          cfg.refit_iters, d_proposal, d_active, d_mask, d_best, d_model);
-    if (p->n_large > 0)
-      hipLaunchKernelGGL(cyl_refit_kernel<256>, dim3(p->n_large), dim3(256), p->bits_large,
-                         ctx->stream, d_xyz, d_offsets, p->d_frames, counts,
-                         p->d_lists + p->n_small, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1,
-                         tag, cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, d_mask,
-                         d_best, d_model);
+    if (p->n_large > 0) st = sloam_ctx_join(ctx, 0);
+    if (st) return st;
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;

@@ -5,11 +5,11 @@
 //               floor(j n / m), shifted by their bounding-box centre, x and y
 //               scaled by 1 / (larger horizontal half extent), z by
 //               1 / (vertical half extent); float4 (w = 1 real, 0 padding)
-//   pn_fwd      one workgroup (4 waves) per cluster, 64 points at a time:
+//   pn_fwd      one workgroup (8 waves) per cluster, 64 points at a time:
 //               layer 1 (3 -> 64) on the VALU, layers 2 (64 -> 128) and
 //               3 (128 -> 256) on v_mfma_f32_16x16x32_bf16 with the BF16
 //               activations in LDS and each wave's weight fragments in VGPRs
-//               (wave w owns output columns 32w.. of layer 2 and 64w.. of
+//               (wave w owns output columns 16w.. of layer 2 and 32w.. of
 //               layer 3), masked max-pool with the first maximising point
 //               kept per channel, head 256 -> 128 -> 7 on the VALU
 //   pn_bwd      one workgroup per cluster: loss, head gradients, and the
@@ -33,7 +33,8 @@ constexpr int kC1 = 64, kC2 = 128, kC3 = 256, kC4 = 128, kOut = 7;
 constexpr int kChunk = 64;           // points per pass of the forward kernel
 constexpr int kS1 = kC1 + 8;         // LDS row strides (bf16), 16-byte multiples
 constexpr int kS2 = kC2 + 8;
-constexpr int kFB = 256;             // forward / backward workgroup
+constexpr int kFB = 256;             // backward / prep workgroup
+constexpr int kFW = 512;             // forward workgroup (8 waves)
 
 // parameter offsets (weight [out][in], then bias)
 constexpr int kW1 = 0, kB1 = kW1 + kC1 * 3;
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ 
 // the BF16 activations of layers 1 and 2 of every point; prop (nullable):
 // proposal records in the world frame.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kFB) void pn_fwd_kernel(
+__global__ __launch_bounds__(kFW) void pn_fwd_kernel(
     const float* __restrict__ prm, const float4* __restrict__ X, const PnMeta* __restrict__ meta,
     int n_clusters, float* __restrict__ act, unsigned short* __restrict__ H1,
     unsigned short* __restrict__ H2, float* __restrict__ prop) {
@@ -178,109 +179,111 @@ __global__ __launch_bounds__(kFB) void pn_fwd_kernel(
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lg = lane >> 4;
 
-  // weight fragments of this wave (B operands: lane -> column lr, k = 8 lg + e)
-  bf16x8 w2f[2][2], w3f[4][4];
+  // weight fragments of this wave (B operands: lane -> column lr, k = 8 lg + e):
+  // wave w owns layer-2 columns 16w.. and layer-3 columns 32w..
+  bf16x8 w2f[2], w3f[2][4];
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+    w2f[ks] = frag_from_f32(prm + kW2 + (16 * wave + lr) * kC1 + 32 * ks + 8 * lg);
 #pragma unroll
   for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const int j = 32 * wave + 16 * ct + lr;
-      w2f[ct][ks] = frag_from_f32(prm + kW2 + j * kC1 + 32 * ks + 8 * lg);
-    }
+    for (int ks = 0; ks < 4; ++ks)
+      w3f[ct][ks] = frag_from_f32(prm + kW3 + (32 * wave + 16 * ct + lr) * kC2 + 32 * ks + 8 * lg);
+  const float b2v = bf_round(prm[kB2 + 16 * wave + lr]);
+  float b3v[2];
 #pragma unroll
-  for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const int j = 64 * wave + 16 * ct + lr;
-      w3f[ct][ks] = frag_from_f32(prm + kW3 + j * kC2 + 32 * ks + 8 * lg);
-    }
-  float b2v[2], b3v[4];
-#pragma unroll
-  for (int ct = 0; ct < 2; ++ct) b2v[ct] = bf_round(prm[kB2 + 32 * wave + 16 * ct + lr]);
-#pragma unroll
-  for (int ct = 0; ct < 4; ++ct) b3v[ct] = bf_round(prm[kB3 + 64 * wave + 16 * ct + lr]);
+  for (int ct = 0; ct < 2; ++ct) b3v[ct] = bf_round(prm[kB3 + 32 * wave + 16 * ct + lr]);
   // layer-1 weights (w, bias) per channel
-  const int g1 = t & 3;
   if (t < kC1)
     w1s[t] = make_float4(bf_round(prm[kW1 + 3 * t]), bf_round(prm[kW1 + 3 * t + 1]),
                          bf_round(prm[kW1 + 3 * t + 2]), bf_round(prm[kB1 + t]));
   __syncthreads();
 
-  float best[4] = {0.f, 0.f, 0.f, 0.f};
-  int bidx[4] = {-1, -1, -1, -1};
+  float best[2] = {0.f, 0.f};
+  int bidx[2] = {-1, -1};
   const float4* Xc = X + (size_t)c * kP;
+  const int g1 = t & 7;  // layer 1: thread -> point t / 8, channels 8 (t % 8) ..
   for (int base = 0; base < m; base += kChunk) {
-    // layer 1: thread -> point t / 4, channels 16 (t % 4) ..
     {
-      const int i = t >> 2;
+      const int i = t >> 3;
       const float4 x = Xc[base + i];
-#pragma unroll 4
-      for (int q = 0; q < 16; ++q) {
-        const float4 w = w1s[16 * g1 + q];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float4 w = w1s[8 * g1 + q];
         const float z = ((w.w + w.x * x.x) + w.y * x.y) + w.z * x.z;
         const unsigned short h = bf_bits(z > 0.f ? z : 0.f);
-        h1s[i * kS1 + 16 * g1 + q] = h;
-        if (H1) H1[((size_t)c * kP + base + i) * kC1 + 16 * g1 + q] = h;
+        h1s[i * kS1 + 8 * g1 + q] = h;
+        if (H1) H1[((size_t)c * kP + base + i) * kC1 + 8 * g1 + q] = h;
       }
     }
     __syncthreads();
-    // layer 2: 64 points x this wave's 32 channels
+    // layer 2: 64 points x this wave's 16 channels, four row tiles in flight
+    {
+      f32x4 acc[4];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt) {
-      f32x4 acc[2];
+      for (int rt = 0; rt < 4; ++rt) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        const bf16x8 a = frag_from_lds(h1s + (16 * rt + lr) * kS1 + 32 * ks + 8 * lg);
+        for (int rt = 0; rt < 4; ++rt) {
+          const bf16x8 a = frag_from_lds(h1s + (16 * rt + lr) * kS1 + 32 * ks + 8 * lg);
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w2f[ks], acc[rt], 0, 0, 0);
+        }
+      const int j = 16 * wave + lr;
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w2f[ct][ks], acc[ct], 0, 0, 0);
-      }
-#pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        const int j = 32 * wave + 16 * ct + lr;
+      for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int i = 16 * rt + 4 * lg + e;
-          const float z = acc[ct][e] + b2v[ct];
+          const float z = acc[rt][e] + b2v;
           const unsigned short h = bf_bits(z > 0.f ? z : 0.f);
           h2s[i * kS2 + j] = h;
           if (H2) H2[((size_t)c * kP + base + i) * kC2 + j] = h;
         }
-      }
     }
     __syncthreads();
-    // layer 3 + max-pool: 64 points x this wave's 64 channels
+    // layer 3 + max-pool: 64 points x this wave's 32 channels, two row tiles
+    // x two column tiles in flight
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt) {
-      f32x4 acc[4];
+    for (int rp = 0; rp < 4; rp += 2) {
+      f32x4 acc[2][2];
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[r][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
-        const bf16x8 a = frag_from_lds(h2s + (16 * rt + lr) * kS2 + 32 * ks + 8 * lg);
+        bf16x8 a[2];
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
-          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w3f[ct][ks], acc[ct], 0, 0, 0);
+        for (int r = 0; r < 2; ++r)
+          a[r] = frag_from_lds(h2s + (16 * (rp + r) + lr) * kS2 + 32 * ks + 8 * lg);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct)
+            acc[r][ct] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], w3f[ct][ks], acc[r][ct], 0, 0, 0);
       }
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int p = base + 16 * rt + 4 * lg + e;
-          const float y = acc[ct][e] + b3v[ct];
-          if (p < m && y > best[ct]) {
-            best[ct] = y;
-            bidx[ct] = p;
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int p = base + 16 * (rp + r) + 4 * lg + e;
+            const float y = acc[r][ct][e] + b3v[ct];
+            if (p < m && y > best[ct]) {
+              best[ct] = y;
+              bidx[ct] = p;
+            }
           }
-        }
     }
     __syncthreads();
   }
   // max over the lanes holding the same channel (groups lg), first point on ties
 #pragma unroll
-  for (int ct = 0; ct < 4; ++ct) {
+  for (int ct = 0; ct < 2; ++ct) {
     float v = best[ct];
     int ix = bidx[ct];
 #pragma unroll
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(kFB) void pn_fwd_kernel(
       }
     }
     if (lg == 0) {
-      const int ch = 64 * wave + 16 * ct + lr;
+      const int ch = 32 * wave + 16 * ct + lr;
       fsh[ch] = v;
       fidx[ch] = ix;
     }
@@ -316,7 +319,7 @@ __global__ __launch_bounds__(kFB) void pn_fwd_kernel(
   __syncthreads();
   if (act) {
     float* a = act + (size_t)c * kAct;
-    for (int k = t; k < kC3; k += kFB) {
+    for (int k = t; k < kC3; k += kFW) {
       a[kAF + k] = fsh[k];
       a[kAI + k] = __int_as_float(fidx[k]);
     }
@@ -617,7 +620,7 @@ int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
-    hipLaunchKernelGGL(pn_fwd_kernel, dim3(B), dim3(kFB), 0, ctx->stream, d_params, b->X,
+    hipLaunchKernelGGL(pn_fwd_kernel, dim3(B), dim3(kFW), 0, ctx->stream, d_params, b->X,
                        b->meta, B, b->act, b->H1, b->H2, (float*)nullptr);
   }
   {
@@ -655,7 +658,7 @@ int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
-    hipLaunchKernelGGL(pn_fwd_kernel, dim3(n_clusters), dim3(kFB), 0, ctx->stream, d_params, b.X,
+    hipLaunchKernelGGL(pn_fwd_kernel, dim3(n_clusters), dim3(kFW), 0, ctx->stream, d_params, b.X,
                        b.meta, n_clusters, (float*)nullptr, (unsigned short*)nullptr,
                        (unsigned short*)nullptr, d_proposal);
   }

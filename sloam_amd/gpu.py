@@ -77,6 +77,23 @@ class Context:
         return ms.value, n.value
 
 
+def autotune_cylinder_fit(ctx, xyz, offsets_dev, offsets_host, n_hyp=1024, thr=0.05,
+                          max_radius=1.0, refit_rounds=3, refit_iters=2, seed=0x5EED,
+                          stream_tag=0):
+    """(hyp_chunk, tile_points) of the fastest launch shape for this batch
+    (sloam_cyl_autotune; cached per batch shape)."""
+    off = np.ascontiguousarray(offsets_host, dtype=np.int32)
+    cfg = CylFitCfg(n_hyp=n_hyp, thr=thr, max_radius=max_radius, refit_iters=refit_iters,
+                    seed=seed, stream_tag=stream_tag, hyp_chunk=0, tile_points=0,
+                    refit_rounds=refit_rounds)
+    out = CylFitCfg()
+    torch.cuda.synchronize()
+    check(ctx.lib.sloam_cyl_autotune(ctx.handle, off.ctypes.data_as(C.c_void_p), len(off) - 1,
+                                     C.byref(cfg), _ptr(xyz), _ptr(offsets_dev), C.byref(out)),
+          ctx.handle)
+    return out.hyp_chunk, out.tile_points
+
+
 class CylinderFitPlan:
     """Batched point-level cylinder RANSAC over clusters (sloam_cyl_plan)."""
 

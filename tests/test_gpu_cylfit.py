@@ -112,3 +112,27 @@ def test_deterministic_repeat(ctx):
     for k in outs[0]:
         assert np.array_equal(outs[0][k], outs[1][k]), k
     plan.close()
+
+
+def test_autotune_choice_is_valid_and_cached(ctx):
+    """sloam_cyl_autotune returns one of its candidate shapes, the same one
+    again from its cache, and that shape reproduces the default results."""
+    import torch
+    from sloam_amd.gpu import CylinderFitPlan, autotune_cylinder_fit
+    d = synth.make_clusters(300, seed=31)
+    x = torch.as_tensor(d["xyz"]).cuda()
+    o = torch.as_tensor(d["offsets"]).cuda()
+    a = autotune_cylinder_fit(ctx, x, o, d["offsets"], n_hyp=256)
+    b = autotune_cylinder_fit(ctx, x, o, d["offsets"], n_hyp=256)
+    print(f"[autotune] chose hyp_chunk {a[0]}, tile {a[1]}", flush=True)
+    assert a == b and a[0] in (64, 128, 256, 512) and a[1] in (1024, 2048, 4096)
+    outs = []
+    for chunk, tile in ((0, 0), a):
+        p = CylinderFitPlan(ctx, d["offsets"], n_hyp=256, hyp_chunk=chunk, tile_points=tile)
+        out = p.alloc_outputs(counts=True, mask=True, best=True)
+        p.exec(x, o, out)
+        ctx.sync()
+        outs.append({k: v.cpu().numpy() for k, v in out.items()})
+        p.close()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
