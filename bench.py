@@ -131,7 +131,7 @@ def bench_learned(ctx, args):
     off = torch.as_tensor(d["offsets"]).to(dev)
     n = np.diff(d["offsets"])
     m = np.minimum(n, 512)
-    net_flop = float(PN_FLOP_PER_POINT * m.sum() + PN_FLOP_HEAD * S)
+    net_flop = float(PN_FLOP_PER_POINT * m.sum())  # the per-point MLP of pn_fwd
     plain = CylinderFitPlan(ctx, d["offsets"], n_hyp=args.hyp, stream_tag=0)
     pout = plain.alloc_outputs(mask=False, best=False)
     net = proposal.ProposalNet(ctx, proposal.load_weights())
@@ -161,7 +161,8 @@ def bench_learned(ctx, args):
     ms_learned = timed(lambda: fit.exec(xyz, off, lout))
     ctx.enable_timing(False)
     kern = {}
-    for name in ("pn_prep", "pn_fwd", "cyl_seed", "cyl_score", "cyl_refit", "cyl_fallback"):
+    for name in ("pn_prep", "pn_fwd", "pn_head", "cyl_seed", "cyl_score", "cyl_refit",
+                 "cyl_fallback"):
         ms, k = ctx.kernel_time(name)
         kern[name] = ms / max(k, 1)
     ref = pout["model"].cpu().numpy()

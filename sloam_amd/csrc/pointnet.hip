@@ -11,7 +11,9 @@
 //               activations in LDS and each wave's weight fragments in VGPRs
 //               (wave w owns output columns 16w.. of layer 2 and 32w.. of
 //               layer 3), masked max-pool with the first maximising point
-//               kept per channel, head 256 -> 128 -> 7 on the VALU
+//               kept per channel; persistent: a fixed grid loops over the
+//               clusters with the weight fragments loaded once
+//   pn_head     256 -> 128 -> 7 on the VALU for 8 clusters per workgroup
 //   pn_bwd      one workgroup per cluster: loss, head gradients, and the
 //               max-pool gradient routed to its argmax points only (at most
 //               256 of them), back through layers 3, 2, 1 from the
@@ -20,6 +22,7 @@
 //   pn_adam     Adam update of the FP32 master weights
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -35,6 +38,7 @@ constexpr int kS1 = kC1 + 8;         // LDS row strides (bf16), 16-byte multiple
 constexpr int kS2 = kC2 + 8;
 constexpr int kFB = 256;             // backward / prep workgroup
 constexpr int kFW = 512;             // forward workgroup (8 waves)
+constexpr int kHC = 8;               // clusters per head workgroup
 
 // parameter offsets (weight [out][in], then bias)
 constexpr int kW1 = 0, kB1 = kW1 + kC1 * 3;
@@ -164,18 +168,10 @@ __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ 
 __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
     const float* __restrict__ prm, const float4* __restrict__ X, const PnMeta* __restrict__ meta,
     int n_clusters, float* __restrict__ act, unsigned short* __restrict__ H1,
-    unsigned short* __restrict__ H2, float* __restrict__ prop) {
+    unsigned short* __restrict__ H2) {
   __shared__ __align__(16) unsigned short h1s[kChunk * kS1];
   __shared__ __align__(16) unsigned short h2s[kChunk * kS2];
-  __shared__ float fsh[kC3];
-  __shared__ int fidx[kC3];
-  __shared__ float h4s[kC4];
-  __shared__ float outs[8];
   __shared__ float4 w1s[kC1];
-  const int c = blockIdx.x;
-  if (c >= n_clusters) return;
-  const PnMeta mt = meta[c];
-  const int m = mt.m;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lg = lane >> 4;
 
@@ -200,6 +196,9 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
                          bf_round(prm[kW1 + 3 * t + 2]), bf_round(prm[kB1 + t]));
   __syncthreads();
 
+  // persistent over clusters: the weight fragments stay in registers
+  for (int c = blockIdx.x; c < n_clusters; c += gridDim.x) {
+  const int m = meta[c].m;
   float best[2] = {0.f, 0.f};
   int bidx[2] = {-1, -1};
   const float4* Xc = X + (size_t)c * kP;
@@ -297,48 +296,77 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
     }
     if (lg == 0) {
       const int ch = 32 * wave + 16 * ct + lr;
-      fsh[ch] = v;
-      fidx[ch] = ix;
+      float* a = act + (size_t)c * kAct;
+      a[kAF + ch] = v;
+      a[kAI + ch] = __int_as_float(ix);
     }
   }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// head: 256 -> 128 (ReLU) -> 7 for kHC clusters per workgroup (each weight row
+// read once per workgroup), proposals in the world frame (prop nullable)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFB) void pn_head_kernel(const float* __restrict__ prm,
+                                                      const PnMeta* __restrict__ meta,
+                                                      int n_clusters, float* __restrict__ act,
+                                                      float* __restrict__ prop) {
+  __shared__ float fs[kHC][kC3];
+  __shared__ float h4s[kHC][kC4];
+  __shared__ float outs[kHC][8];
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.x * kHC;
+  const int nc = min(kHC, n_clusters - c0);
+  for (int i = t; i < nc * kC3; i += kFB)
+    fs[i / kC3][i % kC3] = act[(size_t)(c0 + i / kC3) * kAct + kAF + i % kC3];
   __syncthreads();
-  // head: 256 -> 128 (ReLU) -> 7
   if (t < kC4) {
-    float z = bf_round(prm[kB4 + t]);
+    const float b = bf_round(prm[kB4 + t]);
+    float z[kHC];
+#pragma unroll
+    for (int j = 0; j < kHC; ++j) z[j] = b;
     const float* w = prm + kW4 + t * kC3;
-    for (int k = 0; k < kC3; ++k) z += bf_round(w[k]) * fsh[k];
-    h4s[t] = z > 0.f ? z : 0.f;
-  }
-  __syncthreads();
-  if (t < kOut) {
-    float z = bf_round(prm[kB5 + t]);
-    const float* w = prm + kW5 + t * kC4;
-    for (int k = 0; k < kC4; ++k) z += bf_round(w[k]) * h4s[k];
-    outs[t] = z;
-  }
-  __syncthreads();
-  if (act) {
-    float* a = act + (size_t)c * kAct;
-    for (int k = t; k < kC3; k += kFW) {
-      a[kAF + k] = fsh[k];
-      a[kAI + k] = __int_as_float(fidx[k]);
+    for (int k = 0; k < kC3; ++k) {
+      const float wk = bf_round(w[k]);
+#pragma unroll
+      for (int j = 0; j < kHC; ++j) z[j] += wk * fs[j][k];
     }
-    if (t < kC4) a[kAH + t] = h4s[t];
-    if (t < 8) a[kAO + t] = t < kOut ? outs[t] : 0.f;
+#pragma unroll
+    for (int j = 0; j < kHC; ++j) h4s[j][t] = z[j] > 0.f ? z[j] : 0.f;
   }
-  if (prop && t == 0) {
+  __syncthreads();
+  if (t < kOut * kHC) {
+    const int j = t / kOut, o = t % kOut;
+    if (j < nc) {
+      float z = bf_round(prm[kB5 + o]);
+      const float* w = prm + kW5 + o * kC4;
+      for (int k = 0; k < kC4; ++k) z += bf_round(w[k]) * h4s[j][k];
+      outs[j][o] = z;
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < nc * kC4; i += kFB)
+    act[(size_t)(c0 + i / kC4) * kAct + kAH + i % kC4] = h4s[i / kC4][i % kC4];
+  for (int i = t; i < nc * 8; i += kFB)
+    act[(size_t)(c0 + i / 8) * kAct + kAO + i % 8] = (i % 8) < kOut ? outs[i / 8][i % 8] : 0.f;
+  if (prop && t < nc) {
+    const int c = c0 + t;
+    const PnMeta mt = meta[c];
+    const int m = mt.m;
+    const float* o = outs[t];
     float* pr = prop + (size_t)c * SLOAM_PN_PROPOSAL_STRIDE;
     // back to the world frame: directions and offsets divide by the scales
-    const float vx = outs[0] / mt.sh, vy = outs[1] / mt.sh, vz = outs[2] / mt.sz;
+    const float vx = o[0] / mt.sh, vy = o[1] / mt.sh, vz = o[2] / mt.sz;
     const float nv = sqrtf(vx * vx + vy * vy + vz * vz);
-    const float r = outs[6] / mt.sh;
+    const float r = o[6] / mt.sh;
     const bool ok = m >= 3 && nv > 0.f && r > 0.f && isfinite(r) && isfinite(nv);
     pr[0] = ok ? vx / nv : 0.f;
     pr[1] = ok ? vy / nv : 0.f;
     pr[2] = ok ? vz / nv : 1.f;
-    pr[3] = mt.cx + outs[3] / mt.sh;
-    pr[4] = mt.cy + outs[4] / mt.sh;
-    pr[5] = mt.cz + outs[5] / mt.sz;
+    pr[3] = mt.cx + o[3] / mt.sh;
+    pr[4] = mt.cy + o[4] / mt.sh;
+    pr[5] = mt.cz + o[5] / mt.sz;
     pr[6] = ok ? r : 0.f;
     pr[7] = ok ? 1.f : 0.f;
   }
@@ -554,6 +582,18 @@ __global__ __launch_bounds__(256) void pn_adam_kernel(float* __restrict__ p,
   p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+// persistent forward grid: two 8-wave workgroups per CU (the kernel's
+// occupancy), at most one per cluster
+int fwd_grid(sloam_ctx* ctx, int n_clusters) {
+  int cus = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+          hipSuccess ||
+      cus < 1)
+    cus = 256;
+  (void)hipGetLastError();
+  return std::max(1, std::min(n_clusters, 2 * cus));
+}
+
 struct PnBuffers {
   float4* X;
   PnMeta* meta;
@@ -569,9 +609,10 @@ struct PnBuffers {
 int pn_buffers(sloam_ctx* ctx, int B, bool train, PnBuffers* b) {
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
   const size_t nB = (size_t)(B > 0 ? B : 1);
-  size_t need = al(sizeof(float4) * kP * nB) + al(sizeof(PnMeta) * nB);
+  size_t need = al(sizeof(float4) * kP * nB) + al(sizeof(PnMeta) * nB) +
+                al(sizeof(float) * kAct * nB);
   if (train)
-    need += al(sizeof(float) * kAct * nB) + al(2 * (size_t)kP * kC1 * nB) +
+    need += al(2 * (size_t)kP * kC1 * nB) +
             al(2 * (size_t)kP * kC2 * nB) + al(sizeof(float) * kNP * nB) +
             al(sizeof(float) * kC3 * kC2 * nB) + al(sizeof(float) * kC3 * kC1 * nB) +
             al(sizeof(float) * kNP);
@@ -587,8 +628,8 @@ int pn_buffers(sloam_ctx* ctx, int B, bool train, PnBuffers* b) {
   *b = PnBuffers{};
   b->X = (float4*)take(sizeof(float4) * kP * nB);
   b->meta = (PnMeta*)take(sizeof(PnMeta) * nB);
+  b->act = (float*)take(sizeof(float) * kAct * nB);
   if (train) {
-    b->act = (float*)take(sizeof(float) * kAct * nB);
     b->H1 = (unsigned short*)take(2 * (size_t)kP * kC1 * nB);
     b->H2 = (unsigned short*)take(2 * (size_t)kP * kC2 * nB);
     b->G = (float*)take(sizeof(float) * kNP * nB);
@@ -620,8 +661,10 @@ int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
-    hipLaunchKernelGGL(pn_fwd_kernel, dim3(B), dim3(kFW), 0, ctx->stream, d_params, b->X,
-                       b->meta, B, b->act, b->H1, b->H2, (float*)nullptr);
+    hipLaunchKernelGGL(pn_fwd_kernel, dim3(fwd_grid(ctx, B)), dim3(kFW), 0, ctx->stream,
+                       d_params, b->X, b->meta, B, b->act, b->H1, b->H2);
+    hipLaunchKernelGGL(pn_head_kernel, dim3((B + kHC - 1) / kHC), dim3(kFB), 0, ctx->stream,
+                       d_params, b->meta, B, b->act, (float*)nullptr);
   }
   {
     SloamKernelScope ks(ctx, "pn_bwd");
@@ -658,9 +701,14 @@ int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
-    hipLaunchKernelGGL(pn_fwd_kernel, dim3(n_clusters), dim3(kFW), 0, ctx->stream, d_params, b.X,
-                       b.meta, n_clusters, (float*)nullptr, (unsigned short*)nullptr,
-                       (unsigned short*)nullptr, d_proposal);
+    hipLaunchKernelGGL(pn_fwd_kernel, dim3(fwd_grid(ctx, n_clusters)), dim3(kFW), 0,
+                       ctx->stream, d_params, b.X, b.meta, n_clusters, b.act,
+                       (unsigned short*)nullptr, (unsigned short*)nullptr);
+  }
+  {
+    SloamKernelScope ks(ctx, "pn_head");
+    hipLaunchKernelGGL(pn_head_kernel, dim3((n_clusters + kHC - 1) / kHC), dim3(kFB), 0,
+                       ctx->stream, d_params, b.meta, n_clusters, b.act, d_proposal);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
