@@ -11,6 +11,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -20,6 +21,23 @@ sys.path.insert(0, ROOT)
 METRIC = "cylinder fits/sec + keyframe latency p50/p99 (ms) at ≤1 cm radius err vs plain"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "cyl_score_plain_2000x1024_pmc.md")
+
+
+def pmc_traffic(clusters, hyp, points, chunk, tile):
+    """HBM bytes per cyl_score launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by a copy of known size), or None when that
+    summary was taken on another workload or launch shape."""
+    try:
+        text = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    fields = dict(re.findall(r"^traffic_(\w+): (.+)$", text, re.M))
+    if fields.get("workload") != f"clusters={clusters} hypotheses={hyp} points={points}":
+        return None
+    if fields.get("launch_shape") != f"hyp_chunk={chunk} tile={tile}":
+        return None
+    return int(fields["bytes_per_launch"])
 
 
 def parse():
@@ -304,6 +322,7 @@ def main():
     score_bytes = 12.0 * n_points + 4.0 * args.clusters * args.hyp
     achieved_tflops = ops / score_s / 1e12
     achieved_gbs = score_bytes / score_s / 1e9
+    traffic = pmc_traffic(args.clusters, args.hyp, n_points, chunk, tile)
 
     # accuracy against the generator's truth (informational)
     ctx.sync()
@@ -372,7 +391,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": None if traffic is None else os.path.relpath(PMC_SUMMARY, ROOT),
             "work_per_launch": f"{pairs} point-hypothesis pairs x 6 FP32 ops",
             "avg_launch_ms": kern["cyl_score"],
             "hbm_achieved_gbs": achieved_gbs,
