@@ -111,6 +111,42 @@ def bench_pose(ctx, args):
     eval_bytes = 16.0 * n_res + 32.0 * len(prob["cyl"]) + 16.0 * len(prob["planes"])
     res = {k: v.cpu().numpy() for k, v in out.items()}
     dt, dr = synth.pose_error(res["pose"], prob["pose_true"])
+    # the same solves in the fused mode (one workgroup per problem runs all
+    # iterations in one launch), for the comparison of the two modes
+    plan.set_mode(2)
+    plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+              t["pose_init"], out)
+    ctx.sync()
+    e0.record(stream)
+    for _ in range(args.pose_steps):
+        plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+                  t["pose_init"], out)
+    e1.record(stream)
+    ctx.sync()
+    fused_ms = e0.elapsed_time(e1) / args.pose_steps
+    fused_same = all(np.array_equal(res[k], v.cpu().numpy()) for k, v in out.items())
+    # one keyframe-sized problem (2 x 3250 residuals, 50 iterations, stop at
+    # convergence): fused against batched
+    small = synth.make_pose_problems(2, n_cyl=50, n_planes=5, seed=36)
+    ts = {k: torch.as_tensor(np.ascontiguousarray(small[k])).to(dev)
+          for k in ("points", "cyl", "cyl_offsets", "planes", "plane_offsets", "pose_init")}
+    small_ms = {}
+    for mode, name in ((1, "batched"), (2, "fused")):
+        sp = PosePlan(ctx, small["res_offsets"], len(small["cyl"]), len(small["planes"]),
+                      max_iters=50, stop_on_convergence=True, mode=mode)
+        so = sp.alloc_outputs()
+        torch.cuda.synchronize()
+        sp.exec(ts["points"], ts["cyl"], ts["cyl_offsets"], ts["planes"], ts["plane_offsets"],
+                ts["pose_init"], so)
+        ctx.sync()
+        e0.record(stream)
+        for _ in range(args.pose_steps):
+            sp.exec(ts["points"], ts["cyl"], ts["cyl_offsets"], ts["planes"],
+                    ts["plane_offsets"], ts["pose_init"], so)
+        e1.record(stream)
+        ctx.sync()
+        small_ms[name] = e0.elapsed_time(e1) / args.pose_steps
+        sp.close()
     plan.close()
     return {
         "workload": f"64 problems x (500 cylinders + 20 ground patches), {n_res} residuals, "
@@ -123,6 +159,9 @@ def bench_pose(ctx, args):
         "eval_hbm_gbs": eval_bytes / (eval_ms / 1e3) / 1e9,
         "eval_hbm_frac": eval_bytes / (eval_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "pose_err_vs_truth": {"max_m": float(dt.max()), "max_deg": float(np.degrees(dr.max()))},
+        "fused_ms_per_solve": fused_ms,
+        "fused_bitwise_equal": bool(fused_same),
+        "keyframe_size_solve_ms": small_ms,
     }
 
 

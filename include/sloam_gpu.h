@@ -200,6 +200,30 @@ int sloam_pose_plan_create(sloam_ctx* ctx, const int32_t* h_res_offsets, int32_t
                            int32_t n_cylinders, int32_t n_planes, const sloam_pose_cfg* cfg,
                            sloam_pose_plan** out);
 void sloam_pose_plan_destroy(sloam_pose_plan* plan);
+/* Re-targets an existing plan at another problem set (same argument meaning
+ * as sloam_pose_plan_create; cfg NULL keeps the plan's configuration).  Its
+ * device buffers only grow, so a caller solving one small problem set per
+ * keyframe allocates nothing after the first.  Synchronises the context's
+ * stream first (the previous exec may still read the plan). */
+int sloam_pose_plan_reset(sloam_pose_plan* plan, const int32_t* h_res_offsets,
+                          int32_t n_problems, int32_t n_cylinders, int32_t n_planes,
+                          const sloam_pose_cfg* cfg);
+/* Execution of sloam_pose_plan_exec:
+ *   SLOAM_POSE_MODE_BATCHED: per LM iteration one residual/Jacobian kernel
+ *     over 2048-residual chunks of every problem and one LM-step kernel
+ *     (2 * (max_iters + 1) launches; for many or large problems);
+ *   SLOAM_POSE_MODE_FUSED: one workgroup per problem runs every iteration in
+ *     one launch (for few small problems, e.g. one keyframe's solve);
+ *   SLOAM_POSE_MODE_AUTO (default): FUSED when no problem has more than
+ *     8 chunks (16384 residuals), else BATCHED.
+ * Both modes evaluate the same chunks in the same order and give
+ * bit-identical results. */
+#define SLOAM_POSE_MODE_AUTO 0
+#define SLOAM_POSE_MODE_BATCHED 1
+#define SLOAM_POSE_MODE_FUSED 2
+int sloam_pose_plan_set_mode(sloam_pose_plan* plan, int32_t mode);
+/* 1 if the next exec runs the fused kernel, 0 if the batched loop. */
+int sloam_pose_plan_uses_fused(const sloam_pose_plan* plan);
 int sloam_pose_plan_exec(sloam_pose_plan* plan, const float* d_points, const double* d_cyl,
                          const int32_t* d_cyl_offsets, const double* d_planes,
                          const int32_t* d_plane_offsets, const double* d_pose_in,

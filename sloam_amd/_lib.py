@@ -102,6 +102,9 @@ class ProjCfg(C.Structure):
     ]
 
 
+POSE_MODE_AUTO, POSE_MODE_BATCHED, POSE_MODE_FUSED = 0, 1, 2
+
+
 class PoseCfg(C.Structure):
     _fields_ = [
         ("max_iters", C.c_int32),
@@ -158,6 +161,9 @@ PROTOTYPES = {
                                          C.POINTER(_P)]),
     "sloam_pose_plan_destroy": (None, [_P]),
     "sloam_pose_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sloam_pose_plan_reset": (C.c_int, [_P, _P, _I32, _I32, _I32, C.POINTER(PoseCfg)]),
+    "sloam_pose_plan_set_mode": (C.c_int, [_P, _I32]),
+    "sloam_pose_plan_uses_fused": (C.c_int, [_P]),
     "sloam_project_partition": (C.c_int, [_P, _P, _P, _I64, C.POINTER(ProjCfg), _P, _P, _P, _P,
                                           _P, _P, _P]),
     "sloam_ground_planes": (C.c_int, [_P, _P, _I32, _P, C.POINTER(GroundCfg), _P, _P, _P, _P,

@@ -16,6 +16,12 @@
 //               order, then the LM trust-region logic and the FP64 6x6
 //               Cholesky solve of (J'WJ + D/radius) dx = -J'Wr; writes the next
 //               pose to evaluate.
+// pose_fused    one workgroup per problem runs every iteration in one launch:
+//               the same chunk evaluations and the same FP64 sum of chunk
+//               partials in chunk order, then the same LM step in thread 0,
+//               so its results are bit-identical to the two-kernel loop.  It
+//               serves small problems (a keyframe's solve), where the loop's
+//               2 * (max_iters + 1) launches cost more than the work.
 // Coordinates are evaluated relative to a per-problem anchor (the initial
 // translation) so that FP32 residuals keep sub-micrometre resolution.
 #include "common.hpp"
@@ -257,59 +263,60 @@ __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
 
 // --------------------------------------------------------------------------
 // eval: residuals, Jacobians, compensated partial sums
-// work item = {problem, first residual, end residual, unused}
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
-    const float4* __restrict__ pts, const int4* __restrict__ work, const LMState* __restrict__ st,
-    const float4* __restrict__ cylf, const float4* __restrict__ plf, float delta, int param,
-    double* __restrict__ partials) {
-  __shared__ float red_s[kPBlock / 64][kSums];
-  __shared__ float red_c[kPBlock / 64][kSums];
-  const int4 w = work[blockIdx.x];
-  const int k = w.x;
-  if (st[k].done) return;
-  // pose to evaluate, relative to the anchor, in FP32; for the angle-axis
-  // parametrization the rotation columns of the Jacobian are mapped by the
-  // left Jacobian Jl(w) and fixed parameters are masked
+// pose to evaluate, relative to the anchor, in FP32; for the angle-axis
+// parametrization the rotation columns of the Jacobian are mapped by the left
+// Jacobian Jl(w) and fixed parameters are masked
+struct EvalFrame {
   float R[3][3], t[3], Jl[3][3], msk[6];
-  {
-    double q[4] = {st[k].xe[3], st[k].xe[4], st[k].xe[5], st[k].xe[6]};
-    double Jd[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    if (param == 1) {
-      const double w[3] = {st[k].aae[3], st[k].aae[4], st[k].aae[5]};
-      aa_to_quat(w, q);
-      left_jacobian(w, Jd);
-    }
-    double Rd[3][3];
-    quat_to_R(q, Rd);
-    const double* tt = param == 1 ? st[k].aae : st[k].xe;
-    const int fm = st[k].free_mask;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        R[i][j] = (float)Rd[i][j];
-        Jl[i][j] = (float)Jd[i][j];
-      }
-      t[i] = (float)(tt[i] - st[k].anchor[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) msk[i] = ((fm >> i) & 1) ? 1.0f : 0.0f;
+};
+
+__device__ inline void eval_frame(const LMState& s, int param, EvalFrame& F) {
+  double q[4] = {s.xe[3], s.xe[4], s.xe[5], s.xe[6]};
+  double Jd[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  if (param == 1) {
+    const double w[3] = {s.aae[3], s.aae[4], s.aae[5]};
+    aa_to_quat(w, q);
+    left_jacobian(w, Jd);
   }
+  double Rd[3][3];
+  quat_to_R(q, Rd);
+  const double* tt = param == 1 ? s.aae : s.xe;
+  const int fm = s.free_mask;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      F.R[i][j] = (float)Rd[i][j];
+      F.Jl[i][j] = (float)Jd[i][j];
+    }
+    F.t[i] = (float)(tt[i] - s.anchor[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) F.msk[i] = ((fm >> i) & 1) ? 1.0f : 0.0f;
+}
+
+// One chunk [b, e) of at most kPBlock * kRpt residuals over a workgroup of
+// kPBlock threads: thread t takes residuals b + j * kPBlock + t.  Returns the
+// chunk's 28 sums as FP64 (sum + compensation) in threads 0..27.
+__device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e,
+                                    const EvalFrame& F, const float4* __restrict__ cylf,
+                                    const float4* __restrict__ plf, float delta,
+                                    float (*red_s)[kSums], float (*red_c)[kSums]) {
   const float d2 = delta * delta;
   float acc[kSums];
 #pragma unroll
   for (int i = 0; i < kSums; ++i) acc[i] = 0.f;
   for (int j = 0; j < kRpt; ++j) {
-    const int i = w.y + j * kPBlock + (int)threadIdx.x;
-    if (i >= w.z) break;
+    const int i = b + j * kPBlock + (int)threadIdx.x;
+    if (i >= e) break;
     const float4 p = pts[i];
     const int id = __float_as_int(p.w);
     float rp[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      rp[a] = __builtin_fmaf(R[a][0], p.x, __builtin_fmaf(R[a][1], p.y, R[a][2] * p.z));
-    const float lp[3] = {rp[0] + t[0], rp[1] + t[1], rp[2] + t[2]};
+      rp[a] = __builtin_fmaf(F.R[a][0], p.x, __builtin_fmaf(F.R[a][1], p.y, F.R[a][2] * p.z));
+    const float lp[3] = {rp[0] + F.t[0], rp[1] + F.t[1], rp[2] + F.t[2]};
     float dir[3], r;
     if (id >= 0) {
       const float4 L0 = cylf[2 * (size_t)id];
@@ -336,17 +343,18 @@ __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
       dir[2] = sg * L.z;
     }
     float J[6];
-    J[0] = dir[0] * msk[0];
-    J[1] = dir[1] * msk[1];
-    J[2] = dir[2] * msk[2];
+    J[0] = dir[0] * F.msk[0];
+    J[1] = dir[1] * F.msk[1];
+    J[2] = dir[2] * F.msk[2];
     {
       const float jt0 = __builtin_fmaf(rp[1], dir[2], -rp[2] * dir[1]);
       const float jt1 = __builtin_fmaf(rp[2], dir[0], -rp[0] * dir[2]);
       const float jt2 = __builtin_fmaf(rp[0], dir[1], -rp[1] * dir[0]);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        J[3 + c] = __builtin_fmaf(jt0, Jl[0][c], __builtin_fmaf(jt1, Jl[1][c], jt2 * Jl[2][c])) *
-                   msk[3 + c];
+        J[3 + c] =
+            __builtin_fmaf(jt0, F.Jl[0][c], __builtin_fmaf(jt1, F.Jl[1][c], jt2 * F.Jl[2][c])) *
+            F.msk[3 + c];
     }
     const float s2 = r * r;
     float wgt, rho;
@@ -365,8 +373,8 @@ __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
 #pragma unroll
     for (int a = 0; a < 6; ++a)
 #pragma unroll
-      for (int b = a; b < 6; ++b) {
-        acc[tt] = __builtin_fmaf(wJ[a], J[b], acc[tt]);
+      for (int bb = a; bb < 6; ++bb) {
+        acc[tt] = __builtin_fmaf(wJ[a], J[bb], acc[tt]);
         ++tt;
       }
 #pragma unroll
@@ -383,10 +391,10 @@ __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
     for (int i = 0; i < kSums; ++i) {
       const float os = __shfl_xor(acc[i], o, 64);
       const float oc = __shfl_xor(cmp[i], o, 64);
-      float s, e;
-      two_sum(acc[i], os, &s, &e);
+      float s, err;
+      two_sum(acc[i], os, &s, &err);
       acc[i] = s;
-      cmp[i] = cmp[i] + oc + e;
+      cmp[i] = cmp[i] + oc + err;
     }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -398,39 +406,43 @@ __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
     }
   }
   __syncthreads();
+  double out = 0.0;
   if (threadIdx.x < kSums) {
     const int i = threadIdx.x;
     float s = red_s[0][i], c = red_c[0][i];
     for (int v = 1; v < kPBlock / 64; ++v) {
-      float ns, e;
-      two_sum(s, red_s[v][i], &ns, &e);
+      float ns, err;
+      two_sum(s, red_s[v][i], &ns, &err);
       s = ns;
-      c = c + red_c[v][i] + e;
+      c = c + red_c[v][i] + err;
     }
-    partials[(size_t)blockIdx.x * kSums + i] = (double)s + (double)c;
+    out = (double)s + (double)c;
   }
+  __syncthreads();  // red_s / red_c are reused by the next chunk
+  return out;
+}
+
+// work item = {problem, first residual, end residual, unused}
+__global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
+    const float4* __restrict__ pts, const int4* __restrict__ work, const LMState* __restrict__ st,
+    const float4* __restrict__ cylf, const float4* __restrict__ plf, float delta, int param,
+    double* __restrict__ partials) {
+  __shared__ float red_s[kPBlock / 64][kSums];
+  __shared__ float red_c[kPBlock / 64][kSums];
+  const int4 w = work[blockIdx.x];
+  const int k = w.x;
+  if (st[k].done) return;
+  EvalFrame F;
+  eval_frame(st[k], param, F);
+  const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red_s, red_c);
+  if (threadIdx.x < kSums) partials[(size_t)blockIdx.x * kSums + threadIdx.x] = p;
 }
 
 // --------------------------------------------------------------------------
-// solve: FP64 chunk sums + LM trust-region logic, one wave per problem
+// solve: LM trust-region logic and the FP64 6x6 step on the summed evaluation
+// tot[28] of the pose S.xe (one thread)
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void pose_solve_kernel(
-    int n_problems, const int32_t* __restrict__ first_work, const double* __restrict__ partials,
-    PoseCfgDev cfg, LMState* __restrict__ st, double* __restrict__ pose_out,
-    int32_t* __restrict__ status, double* __restrict__ cost, double* __restrict__ params_out) {
-  __shared__ double tot[kSums];
-  const int k = blockIdx.x;
-  if (k >= n_problems) return;
-  if (st[k].done) return;
-  if (threadIdx.x < kSums) {
-    double s = 0.0;
-    for (int wk = first_work[k]; wk < first_work[k + 1]; ++wk)
-      s += partials[(size_t)wk * kSums + threadIdx.x];
-    tot[threadIdx.x] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  LMState S = st[k];
+__device__ inline void lm_step(LMState& S, const double* tot, const PoseCfgDev& cfg) {
   // evaluation result of S.xe
   double A[6][6], g[6];
   {
@@ -549,7 +561,11 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
   }
   S.phase = 1;
   if (!have_candidate) S.done = 1;
-  st[k] = S;
+}
+
+__device__ inline void write_outputs(const LMState& S, int k, const PoseCfgDev& cfg,
+                                     double* __restrict__ pose_out, int32_t* __restrict__ status,
+                                     double* __restrict__ cost, double* __restrict__ params_out) {
   for (int i = 0; i < 7; ++i) pose_out[7 * (size_t)k + i] = S.x[i];
   if (params_out) {
     if (cfg.param == 1) {
@@ -573,22 +589,170 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
   }
 }
 
+// FP64 chunk sums in chunk order, one wave per problem
+__global__ __launch_bounds__(64) void pose_solve_kernel(
+    int n_problems, const int32_t* __restrict__ first_work, const double* __restrict__ partials,
+    PoseCfgDev cfg, LMState* __restrict__ st, double* __restrict__ pose_out,
+    int32_t* __restrict__ status, double* __restrict__ cost, double* __restrict__ params_out) {
+  __shared__ double tot[kSums];
+  const int k = blockIdx.x;
+  if (k >= n_problems) return;
+  if (st[k].done) return;
+  if (threadIdx.x < kSums) {
+    double s = 0.0;
+    for (int wk = first_work[k]; wk < first_work[k + 1]; ++wk)
+      s += partials[(size_t)wk * kSums + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  LMState S = st[k];
+  lm_step(S, tot, cfg);
+  st[k] = S;
+  write_outputs(S, k, cfg, pose_out, status, cost, params_out);
+}
+
+// --------------------------------------------------------------------------
+// fused: every iteration of one problem in one workgroup
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kPBlock) void pose_fused_kernel(
+    int n_problems, const float4* __restrict__ pts, const int4* __restrict__ work,
+    const int32_t* __restrict__ first_work, const float4* __restrict__ cylf,
+    const float4* __restrict__ plf, PoseCfgDev cfg, float delta, LMState* __restrict__ st,
+    double* __restrict__ pose_out, int32_t* __restrict__ status, double* __restrict__ cost,
+    double* __restrict__ params_out) {
+  __shared__ float red_s[kPBlock / 64][kSums];
+  __shared__ float red_c[kPBlock / 64][kSums];
+  __shared__ double tot[kSums];
+  __shared__ LMState sS;
+  const int k = blockIdx.x;
+  if (k >= n_problems) return;
+  if (threadIdx.x == 0) sS = st[k];
+  __syncthreads();
+  const int w0 = first_work[k], w1 = first_work[k + 1];
+  bool ran = false;
+  for (int it = 0; it <= cfg.max_iters; ++it) {
+    if (sS.done) break;
+    EvalFrame F;
+    eval_frame(sS, cfg.param, F);
+    double acc = 0.0;
+    for (int wk = w0; wk < w1; ++wk) {
+      const int4 w = work[wk];
+      const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red_s, red_c);
+      if (threadIdx.x < kSums) acc += p;
+    }
+    if (threadIdx.x < kSums) tot[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      LMState S = sS;
+      lm_step(S, tot, cfg);
+      sS = S;
+    }
+    ran = true;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && ran) {
+    const LMState S = sS;
+    st[k] = S;
+    write_outputs(S, k, cfg, pose_out, status, cost, params_out);
+  }
+}
+
 }  // namespace
 
 struct sloam_pose_plan {
   sloam_ctx* ctx = nullptr;
   sloam_pose_cfg cfg{};
+  int32_t mode = SLOAM_POSE_MODE_AUTO;
   int32_t n_problems = 0;
   int64_t n_res = 0;
   int32_t n_cyl = 0, n_pl = 0;
   int n_work = 0;
+  int max_chunks = 0;  // most chunks of one problem
+  std::vector<int4> h_work;
+  std::vector<int32_t> h_first;
+  // device buffers, grow-only (capacities in elements)
   int4* d_work = nullptr;
   int32_t* d_first = nullptr;
   LMState* d_state = nullptr;
   double* d_partials = nullptr;
   float4* d_cylf = nullptr;
   float4* d_plf = nullptr;
+  size_t cap_work = 0, cap_first = 0, cap_state = 0, cap_partials = 0, cap_cyl = 0, cap_pl = 0;
 };
+
+namespace {
+
+// a problem set whose largest problem has at most this many chunks is solved
+// by pose_fused in AUTO mode
+constexpr int kFusedAutoChunks = 8;
+
+int grow(sloam_ctx* ctx, void** p, size_t* cap, size_t want, size_t elem) {
+  if (want <= *cap) return SLOAM_OK;
+  sloam_dev_free(*p);
+  *p = nullptr;
+  *cap = 0;
+  const int st = sloam_dev_alloc(ctx, p, want * elem);
+  if (st) return st;
+  *cap = want;
+  return SLOAM_OK;
+}
+
+// (Re)targets the plan at a problem set: work list, chunk table, buffers.
+int plan_build(sloam_pose_plan* p, const int32_t* h_res_offsets, int32_t n_problems,
+               int32_t n_cylinders, int32_t n_planes) {
+  if (!h_res_offsets || n_problems < 0 || n_cylinders < 0 || n_planes < 0)
+    return SLOAM_ERR_INVALID_ARG;
+  if (h_res_offsets[0] != 0) return SLOAM_ERR_INVALID_ARG;
+  for (int k = 0; k < n_problems; ++k)
+    if (h_res_offsets[k + 1] < h_res_offsets[k]) return SLOAM_ERR_INVALID_ARG;
+  sloam_ctx* ctx = p->ctx;
+  // the stream may still read the previous work list and buffers
+  SLOAM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  p->n_problems = n_problems;
+  p->n_res = h_res_offsets[n_problems];
+  p->n_cyl = n_cylinders;
+  p->n_pl = n_planes;
+  const int chunk = kPBlock * kRpt;
+  p->h_work.clear();
+  p->h_first.assign(n_problems + 1, 0);
+  p->max_chunks = 0;
+  for (int k = 0; k < n_problems; ++k) {
+    p->h_first[k] = (int32_t)p->h_work.size();
+    const int b = h_res_offsets[k], e = h_res_offsets[k + 1];
+    if (e == b) p->h_work.push_back(make_int4(k, b, e, 0));  // keeps one (empty) chunk
+    for (int s = b; s < e; s += chunk)
+      p->h_work.push_back(make_int4(k, s, std::min(e, s + chunk), 0));
+    p->max_chunks = std::max(p->max_chunks, (int)p->h_work.size() - p->h_first[k]);
+  }
+  p->h_first[n_problems] = (int32_t)p->h_work.size();
+  p->n_work = (int)p->h_work.size();
+  int st = grow(ctx, (void**)&p->d_work, &p->cap_work, p->h_work.size(), sizeof(int4));
+  if (!st) st = grow(ctx, (void**)&p->d_first, &p->cap_first, p->h_first.size(), sizeof(int32_t));
+  if (!st) st = grow(ctx, (void**)&p->d_state, &p->cap_state, (size_t)n_problems, sizeof(LMState));
+  if (!st)
+    st = grow(ctx, (void**)&p->d_partials, &p->cap_partials, (size_t)kSums * p->h_work.size(),
+              sizeof(double));
+  if (!st) st = grow(ctx, (void**)&p->d_cylf, &p->cap_cyl, 2 * (size_t)n_cylinders, sizeof(float4));
+  if (!st) st = grow(ctx, (void**)&p->d_plf, &p->cap_pl, (size_t)n_planes, sizeof(float4));
+  if (st) return st;
+  // the host vectors stay alive in the plan until the next build, which
+  // synchronises the stream first
+  if (!p->h_work.empty())
+    SLOAM_HIP(ctx, hipMemcpyAsync(p->d_work, p->h_work.data(), sizeof(int4) * p->h_work.size(),
+                                  hipMemcpyHostToDevice, ctx->stream));
+  SLOAM_HIP(ctx, hipMemcpyAsync(p->d_first, p->h_first.data(),
+                                sizeof(int32_t) * p->h_first.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+  return SLOAM_OK;
+}
+
+bool valid_cfg(const sloam_pose_cfg* cfg) {
+  return cfg->max_iters >= 0 && cfg->huber_delta > 0.0 && cfg->initial_radius > 0.0 &&
+         cfg->parametrization >= 0 && cfg->parametrization <= 1;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -599,49 +763,13 @@ int sloam_pose_plan_create(sloam_ctx* ctx, const int32_t* h_res_offsets, int32_t
       n_planes < 0)
     return SLOAM_ERR_INVALID_ARG;
   *out = nullptr;
-  if (cfg->max_iters < 0 || !(cfg->huber_delta > 0.0) || !(cfg->initial_radius > 0.0) ||
-      cfg->parametrization < 0 || cfg->parametrization > 1)
-    return SLOAM_ERR_INVALID_ARG;
-  if (h_res_offsets[0] != 0) return SLOAM_ERR_INVALID_ARG;
-  for (int k = 0; k < n_problems; ++k)
-    if (h_res_offsets[k + 1] < h_res_offsets[k]) return SLOAM_ERR_INVALID_ARG;
+  if (!valid_cfg(cfg)) return SLOAM_ERR_INVALID_ARG;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
   sloam_pose_plan* p = new (std::nothrow) sloam_pose_plan();
   if (!p) return SLOAM_ERR_ALLOC;
   p->ctx = ctx;
   p->cfg = *cfg;
-  p->n_problems = n_problems;
-  p->n_res = h_res_offsets[n_problems];
-  p->n_cyl = n_cylinders;
-  p->n_pl = n_planes;
-  const int chunk = kPBlock * kRpt;
-  std::vector<int4> work;
-  std::vector<int32_t> first(n_problems + 1, 0);
-  for (int k = 0; k < n_problems; ++k) {
-    first[k] = (int32_t)work.size();
-    const int b = h_res_offsets[k], e = h_res_offsets[k + 1];
-    if (e == b) work.push_back(make_int4(k, b, e, 0));  // keeps one (empty) chunk
-    for (int s = b; s < e; s += chunk) work.push_back(make_int4(k, s, std::min(e, s + chunk), 0));
-  }
-  first[n_problems] = (int32_t)work.size();
-  p->n_work = (int)work.size();
-  int st = SLOAM_OK;
-  auto upload = [&](void** dst, const void* src, size_t bytes) {
-    if (st || bytes == 0) return;
-    st = sloam_dev_alloc(ctx, dst, bytes);
-    if (st) return;
-    hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) st = sloam_record_hip_error(ctx, e, "hipMemcpy(pose plan)");
-  };
-  upload((void**)&p->d_work, work.data(), sizeof(int4) * work.size());
-  upload((void**)&p->d_first, first.data(), sizeof(int32_t) * first.size());
-  if (!st && n_problems > 0) st = sloam_dev_alloc(ctx, (void**)&p->d_state, sizeof(LMState) * n_problems);
-  if (!st && p->n_work > 0)
-    st = sloam_dev_alloc(ctx, (void**)&p->d_partials, sizeof(double) * kSums * p->n_work);
-  if (!st && n_cylinders > 0)
-    st = sloam_dev_alloc(ctx, (void**)&p->d_cylf, sizeof(float4) * 2 * (size_t)n_cylinders);
-  if (!st && n_planes > 0)
-    st = sloam_dev_alloc(ctx, (void**)&p->d_plf, sizeof(float4) * (size_t)n_planes);
+  const int st = plan_build(p, h_res_offsets, n_problems, n_cylinders, n_planes);
   if (st) {
     sloam_pose_plan_destroy(p);
     return st;
@@ -650,8 +778,33 @@ int sloam_pose_plan_create(sloam_ctx* ctx, const int32_t* h_res_offsets, int32_t
   return SLOAM_OK;
 }
 
+int sloam_pose_plan_reset(sloam_pose_plan* p, const int32_t* h_res_offsets, int32_t n_problems,
+                          int32_t n_cylinders, int32_t n_planes, const sloam_pose_cfg* cfg) {
+  if (!p) return SLOAM_ERR_INVALID_ARG;
+  if (cfg) {
+    if (!valid_cfg(cfg)) return SLOAM_ERR_INVALID_ARG;
+    p->cfg = *cfg;
+  }
+  SLOAM_HIP(p->ctx, hipSetDevice(p->ctx->device));
+  return plan_build(p, h_res_offsets, n_problems, n_cylinders, n_planes);
+}
+
+int sloam_pose_plan_set_mode(sloam_pose_plan* p, int32_t mode) {
+  if (!p || mode < SLOAM_POSE_MODE_AUTO || mode > SLOAM_POSE_MODE_FUSED)
+    return SLOAM_ERR_INVALID_ARG;
+  p->mode = mode;
+  return SLOAM_OK;
+}
+
+int sloam_pose_plan_uses_fused(const sloam_pose_plan* p) {
+  if (!p) return SLOAM_ERR_INVALID_ARG;
+  return p->mode == SLOAM_POSE_MODE_FUSED ||
+         (p->mode == SLOAM_POSE_MODE_AUTO && p->max_chunks <= kFusedAutoChunks);
+}
+
 void sloam_pose_plan_destroy(sloam_pose_plan* p) {
   if (!p) return;
+  if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
   sloam_dev_free(p->d_work);
   sloam_dev_free(p->d_first);
   sloam_dev_free(p->d_state);
@@ -688,10 +841,18 @@ int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double
     SloamKernelScope ks(ctx, "pose_prep");
     hipLaunchKernelGGL(pose_prep_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
                        p->n_problems, d_pose_in, d_free_mask, d_cyl, d_cyl_offsets, d_planes,
-                       d_plane_offsets,
-                       dc, p->d_state, p->d_cylf, p->d_plf);
+                       d_plane_offsets, dc, p->d_state, p->d_cylf, p->d_plf);
   }
   SLOAM_HIP(ctx, hipGetLastError());
+  if (sloam_pose_plan_uses_fused(p)) {
+    SloamKernelScope ks(ctx, "pose_fused");
+    hipLaunchKernelGGL(pose_fused_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
+                       p->n_problems, reinterpret_cast<const float4*>(d_points), p->d_work,
+                       p->d_first, p->d_cylf, p->d_plf, dc, (float)c.huber_delta, p->d_state,
+                       d_pose_out, d_status, d_cost, d_params_out);
+    SLOAM_HIP(ctx, hipGetLastError());
+    return SLOAM_OK;
+  }
   for (int it = 0; it <= c.max_iters; ++it) {
     {
       SloamKernelScope ks(ctx, "pose_eval");

@@ -188,7 +188,7 @@ class PosePlan:
     def __init__(self, ctx, res_offsets, n_cylinders, n_planes, max_iters=10,
                  stop_on_convergence=False, huber_delta=0.1, function_tolerance=1e-6,
                  gradient_tolerance=1e-10, parameter_tolerance=1e-8, initial_radius=1e4,
-                 min_relative_decrease=1e-3, parametrization=0):
+                 min_relative_decrease=1e-3, parametrization=0, mode=0):
         self.ctx = ctx
         self.lib = ctx.lib
         off = np.ascontiguousarray(res_offsets, dtype=np.int32)
@@ -206,6 +206,28 @@ class PosePlan:
                                               self.n_problems, int(n_cylinders), int(n_planes),
                                               C.byref(self.cfg), C.byref(h)), ctx.handle)
         self.handle = h
+        self.set_mode(mode)
+
+    def set_mode(self, mode):
+        """0 auto, 1 batched (two kernels per LM iteration), 2 fused (one
+        workgroup per problem runs every iteration in one launch)."""
+        check(self.lib.sloam_pose_plan_set_mode(self.handle, int(mode)), self.ctx.handle)
+
+    @property
+    def uses_fused(self):
+        r = self.lib.sloam_pose_plan_uses_fused(self.handle)
+        if r < 0:
+            check(r, self.ctx.handle)
+        return bool(r)
+
+    def reset(self, res_offsets, n_cylinders, n_planes):
+        """Re-targets the plan at another problem set (grow-only buffers)."""
+        off = np.ascontiguousarray(res_offsets, dtype=np.int32)
+        check(self.lib.sloam_pose_plan_reset(self.handle, off.ctypes.data_as(C.c_void_p),
+                                             len(off) - 1, int(n_cylinders), int(n_planes),
+                                             None), self.ctx.handle)
+        self.n_problems = len(off) - 1
+        self.n_res = int(off[-1])
 
     def close(self):
         if self.handle:

@@ -36,17 +36,48 @@ struct CtxHolder {
   }
 };
 
-// RAII device buffer
+// Per-thread grow-only device buffers.  All work of the bridge runs on the
+// context's one stream and every call ends with a stream synchronisation, so
+// a call reuses the buffers of the previous one: the k-th buffer a call asks
+// for is slot k.  A slot that must grow is freed after the stream drains.
+struct Arena {
+  std::vector<std::pair<void*, size_t>> slots;
+  size_t next = 0;
+  ~Arena() {
+    for (auto& s : slots)
+      if (s.first) (void)hipFree(s.first);
+  }
+  void* take(size_t bytes) {
+    if (next == slots.size()) slots.emplace_back(nullptr, 0);
+    auto& s = slots[next++];
+    if (s.second < bytes) {
+      hip_check(hipStreamSynchronize((hipStream_t)sloam_ctx_stream(context())),
+                "hipStreamSynchronize");
+      if (s.first) (void)hipFree(s.first);
+      s.first = nullptr;
+      s.second = 0;
+      const size_t sz = bytes + bytes / 2;
+      hip_check(hipMalloc(&s.first, sz), "hipMalloc");
+      s.second = sz;
+    }
+    return s.first;
+  }
+};
+
+Arena& arena() {
+  thread_local Arena a;
+  return a;
+}
+
+// opened at the start of every bridge call: slots are numbered from 0 again
+struct CallScope {
+  CallScope() { arena().next = 0; }
+};
+
+// a device buffer of at least `bytes` for the rest of the call
 struct Dev {
   void* p = nullptr;
-  explicit Dev(size_t bytes) {
-    if (bytes) hip_check(hipMalloc(&p, bytes), "hipMalloc");
-  }
-  ~Dev() {
-    if (p) (void)hipFree(p);
-  }
-  Dev(const Dev&) = delete;
-  Dev& operator=(const Dev&) = delete;
+  explicit Dev(size_t bytes) { p = arena().take(bytes ? bytes : 16); }
   template <typename T>
   T* as() const {
     return static_cast<T*>(p);
@@ -83,6 +114,7 @@ sloam_ctx* context() {
 }
 
 std::vector<CellFit> groundPlanes(const VectorType& points, const sloam_ground_cfg& cfg) {
+  CallScope scope;
   sloam_ctx* ctx = context();
   const int n_cells = cfg.radii_bins * cfg.theta_bins;
   const int nf = cfg.num_features;
@@ -134,6 +166,7 @@ std::vector<CellFit> groundPlanes(const VectorType& points, const sloam_ground_c
 
 std::vector<LineResult> fitLines(const std::vector<VectorType>& sets,
                                  const sloam_line_fit_cfg& cfg) {
+  CallScope scope;
   std::vector<LineResult> out(sets.size());
   if (sets.empty()) return out;
   sloam_ctx* ctx = context();
@@ -172,6 +205,7 @@ std::vector<LineResult> fitLines(const std::vector<VectorType>& sets,
 
 std::vector<PoseResult> solvePoses(const std::vector<PoseProblem>& problems,
                                    const sloam_pose_cfg& cfg) {
+  CallScope scope;
   std::vector<PoseResult> out(problems.size());
   if (problems.empty()) return out;
   sloam_ctx* ctx = context();
@@ -201,10 +235,20 @@ std::vector<PoseResult> solvePoses(const std::vector<PoseProblem>& problems,
     masks.push_back(pr.free_mask);
   }
   const int P = (int)problems.size();
-  sloam_pose_plan* plan = nullptr;
-  check(sloam_pose_plan_create(ctx, res_off.data(), P, cyl_off.back(), pl_off.back(), &cfg, &plan),
-        "sloam_pose_plan_create");
-  std::unique_ptr<sloam_pose_plan, void (*)(sloam_pose_plan*)> guard(plan, sloam_pose_plan_destroy);
+  // one plan per thread, re-targeted at each problem set (grow-only buffers)
+  thread_local std::unique_ptr<sloam_pose_plan, void (*)(sloam_pose_plan*)> cached(
+      nullptr, sloam_pose_plan_destroy);
+  if (!cached) {
+    sloam_pose_plan* p = nullptr;
+    check(sloam_pose_plan_create(ctx, res_off.data(), P, cyl_off.back(), pl_off.back(), &cfg, &p),
+          "sloam_pose_plan_create");
+    cached.reset(p);
+  } else {
+    check(sloam_pose_plan_reset(cached.get(), res_off.data(), P, cyl_off.back(), pl_off.back(),
+                                &cfg),
+          "sloam_pose_plan_reset");
+  }
+  sloam_pose_plan* plan = cached.get();
   Dev d_pts(sizeof(float) * (pts.empty() ? 4 : pts.size()));
   Dev d_cyl(sizeof(double) * (cyl.empty() ? 8 : cyl.size()));
   Dev d_pl(sizeof(double) * (pl.empty() ? 4 : pl.size()));
@@ -248,6 +292,7 @@ std::vector<PoseResult> solvePoses(const std::vector<PoseProblem>& problems,
 
 TrellisResult trellis(const VectorType& cloud, int height, int width,
                       const sloam_trellis_cfg& cfg, bool want_labels) {
+  CallScope scope;
   TrellisResult out;
   sloam_ctx* ctx = context();
   const int N = height * width;

@@ -98,3 +98,88 @@ def test_two_step_angle_axis_subsets(ctx):
         # conversion uses atan2, whose last bit may differ between libms)
         assert np.allclose(g["params"][k, fixed], o["params"][k, fixed], atol=1e-12, rtol=0)
     assert np.allclose(g["params"], o["params"], atol=1e-4)
+
+
+def _empty_middle(prob):
+    off = prob["res_offsets"]
+    keep = np.r_[0:off[1], off[2]:off[3]]
+    prob["points"] = prob["points"][keep]
+    n1 = off[2] - off[1]
+    prob["res_offsets"] = np.array([0, off[1], off[1], off[3] - n1], np.int32)
+    return prob
+
+
+FUSED_CASES = [
+    ("fixed-10", lambda: synth.make_pose_problems(8, n_cyl=100, n_planes=20, seed=31),
+     dict(max_iters=10)),
+    ("converge-50", lambda: synth.make_pose_problems(6, n_cyl=60, n_planes=15, seed=32),
+     dict(max_iters=50, stop_on_convergence=True)),
+    ("two-step", lambda: synth.split_two_step(
+        synth.make_pose_problems(4, n_cyl=100, n_planes=20, seed=33)),
+     dict(max_iters=50, stop_on_convergence=True, parametrization=1)),
+    # 300 x 60 + 20 x 50 = 19000 residuals: 10 chunks per problem
+    ("multi-chunk", lambda: synth.make_pose_problems(3, n_cyl=300, n_planes=20, seed=34),
+     dict(max_iters=12)),
+    ("empty-problem", lambda: _empty_middle(synth.make_pose_problems(
+        3, n_cyl=5, n_planes=2, pts_per_cyl=3, pts_per_plane=3, seed=35)),
+     dict(max_iters=10)),
+]
+
+
+@pytest.mark.parametrize("name,make,kw", FUSED_CASES, ids=[c[0] for c in FUSED_CASES])
+def test_fused_matches_batched_bitwise(ctx, name, make, kw):
+    """The one-launch LM kernel evaluates the same chunks in the same order and
+    runs the same step: every output equals the two-kernel loop's bit for bit."""
+    prob = make()
+    print(f"[pose] fused vs batched: {name}, {len(prob['res_offsets']) - 1} problems, "
+          f"{prob['res_offsets'][-1]} residuals", flush=True)
+    b = solve_poses(ctx, prob, mode=1, **kw)
+    f = solve_poses(ctx, prob, mode=2, **kw)
+    for k in ("pose", "params", "status", "cost"):
+        assert np.array_equal(b[k], f[k]), k
+
+
+def test_auto_mode_choice(ctx):
+    import torch
+    from sloam_amd.gpu import PosePlan
+    small = synth.make_pose_problems(2, n_cyl=50, n_planes=5, seed=36)     # 1-2 chunks
+    large = synth.make_pose_problems(2, n_cyl=300, n_planes=20, seed=37)   # 10 chunks
+    p = PosePlan(ctx, small["res_offsets"], len(small["cyl"]), len(small["planes"]))
+    assert p.uses_fused
+    p.reset(large["res_offsets"], len(large["cyl"]), len(large["planes"]))
+    assert not p.uses_fused
+    p.set_mode(2)
+    assert p.uses_fused
+    p.close()
+    torch.cuda.synchronize()
+
+
+def test_plan_reset_reuses_buffers(ctx):
+    """One plan re-targeted at three problem sets in a row (grow, then shrink)
+    gives the results of a fresh plan for each."""
+    import torch
+    from sloam_amd.gpu import PosePlan
+    dev = torch.device("cuda", 0)
+    sets = [synth.make_pose_problems(2, n_cyl=40, n_planes=5, seed=41),
+            synth.make_pose_problems(5, n_cyl=120, n_planes=20, seed=42),
+            synth.make_pose_problems(1, n_cyl=20, n_planes=3, seed=43)]
+    plan = None
+    for i, prob in enumerate(sets):
+        print(f"[pose] reset {i}: {prob['res_offsets'][-1]} residuals", flush=True)
+        if plan is None:
+            plan = PosePlan(ctx, prob["res_offsets"], len(prob["cyl"]), len(prob["planes"]),
+                            max_iters=20, stop_on_convergence=True)
+        else:
+            plan.reset(prob["res_offsets"], len(prob["cyl"]), len(prob["planes"]))
+        t = {k: torch.as_tensor(np.ascontiguousarray(prob[k])).to(dev)
+             for k in ("points", "cyl", "cyl_offsets", "planes", "plane_offsets", "pose_init")}
+        out = plan.alloc_outputs()
+        torch.cuda.synchronize()
+        plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+                  t["pose_init"], out)
+        ctx.sync()
+        got = {k: v.cpu().numpy() for k, v in out.items()}
+        ref = solve_poses(ctx, prob, max_iters=20, stop_on_convergence=True)
+        for k in ("pose", "params", "status", "cost"):
+            assert np.array_equal(got[k], ref[k]), (i, k)
+    plan.close()
