@@ -78,12 +78,14 @@ void sloam_dev_free(void* p);
 int sloam_fill_u32(sloam_ctx* ctx, uint32_t* d, size_t n, uint32_t value);
 
 // Stable LSD radix sort (radix.hip) of n (key, value) pairs by key bits
-// [0, end_bit), on the context stream, without memsets (graph-capturable).
+// [begin_bit, end_bit), on the context stream, without memsets (graph-capturable).
 // keys / vals (nullable) hold the input and receive the result; keys_tmp /
 // vals_tmp are same-sized scratch; hist holds sloam_radix_hist_ints(n) ints.
+// Sorts by the key bits [begin_bit, end_bit); d_n (nullable) is a device-side
+// count that bounds the sort below n (the capacity the launches are sized for).
 int sloam_radix_sort(sloam_ctx* ctx, unsigned long long* keys, int32_t* vals,
-                     unsigned long long* keys_tmp, int32_t* vals_tmp, int n, int end_bit,
-                     int32_t* hist);
+                     unsigned long long* keys_tmp, int32_t* vals_tmp, int n, int begin_bit,
+                     int end_bit, const int32_t* d_n, int32_t* hist);
 size_t sloam_radix_hist_ints(int n);
 
 // ---------------------------------------------------------------------------

@@ -3,9 +3,11 @@
 // (sloam/src/objects/cylinder.cpp:116-139, PCL SACSegmentation with
 // SACMODEL_LINE / SAC_RANSAC, threshold 0.25, optimized coefficients).
 //
-// One wavefront per point set (tree).  Lane 0 runs the sequential part of the
-// sample-consensus loop exactly as the library defines it -- MT19937 seeded
-// 12345 (state in LDS), rnd() = mt() >> 1, a persistent swap-shuffle of the
+// One wavefront per point set (tree), its points staged in LDS.  Lane 0 runs
+// the sequential part of the sample-consensus loop exactly as the library
+// defines it -- MT19937 seeded 12345 (state in LDS; the state after seeding
+// and the first twist is the same for every set, so the host computes it once
+// per seed and each workgroup copies it), rnd() = mt() >> 1, a persistent swap-shuffle of the
 // index array, up to 1000 draws until the sample is good (all three
 // coordinates differ), the adaptive iteration bound k = log(1 - p) /
 // log(1 - w^2) -- while the whole wave scores each line with one ballot per
@@ -16,10 +18,15 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
+
+// MT19937 state after seeding with g_mt_seed and one twist (per device)
+__device__ uint32_t g_mt_state[624];
 
 namespace {
 
 constexpr int kLineMax = 4096;  // points per set (LDS index array)
+constexpr int kStage = 1024;    // sets up to this size are read from LDS
 
 struct LineCfgDev {
   double sqr_thr;
@@ -95,12 +102,17 @@ __global__ __launch_bounds__(64) void line_ransac_kernel(const float* __restrict
   __shared__ int sel[3];
   __shared__ int in_list[kLineMax];
   __shared__ float model_s[6];
+  __shared__ float stage[3 * kStage];
   const int s = blockIdx.x;
   if (s >= n_sets) return;
   const int lane = threadIdx.x;
   const int off = offsets[s];
   const int n = offsets[s + 1] - off;
   const float* P = xyz + 3 * (size_t)off;
+  if (n > 0 && n <= kStage) {
+    for (int i = lane; i < 3 * n; i += 64) stage[i] = P[i];
+    P = stage;  // flat pointer into LDS
+  }
   float* co = coeff_out + 6 * (size_t)s;
   int32_t* so = stats_out + 4 * (size_t)s;
   if (inl_out)
@@ -111,11 +123,8 @@ __global__ __launch_bounds__(64) void line_ransac_kernel(const float* __restrict
     return;
   }
   for (int i = lane; i < n; i += 64) shuf[i] = i;
-  if (lane == 0) {
-    mt[0] = cfg.seed;
-    for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
-    mt_idx = 624;
-  }
+  for (int i = lane; i < 624; i += 64) mt[i] = g_mt_state[i];
+  if (lane == 0) mt_idx = 0;
   __syncthreads();
   const double one_over = 1.0 / (double)n;
   double k = DBL_MAX;
@@ -310,6 +319,34 @@ int sloam_fit_lines(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets
   dc.max_iter = cfg->max_iterations;
   dc.optimize = cfg->optimize;
   dc.seed = cfg->seed;
+  {
+    // the seeded and twisted generator state, uploaded when the seed changes
+    static std::mutex mu;
+    static uint32_t host_state[64][624];
+    static int64_t host_seed[64];
+    static bool init = false;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!init) {
+      for (int d = 0; d < 64; ++d) host_seed[d] = -1;
+      init = true;
+    }
+    if (ctx->device < 0 || ctx->device >= 64) return SLOAM_ERR_INVALID_ARG;
+    if (host_seed[ctx->device] != (int64_t)cfg->seed) {
+      // an earlier upload from this buffer may still be in flight
+      SLOAM_HIP(ctx, hipDeviceSynchronize());
+      uint32_t* mt = host_state[ctx->device];
+      mt[0] = cfg->seed;
+      for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+      for (int i = 0; i < 624; ++i) {
+        const uint32_t y = (mt[i] & 0x80000000u) | (mt[i + 1 < 624 ? i + 1 : 0] & 0x7fffffffu);
+        const int j = i + 397 < 624 ? i + 397 : i + 397 - 624;
+        mt[i] = mt[j] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+      }
+      SLOAM_HIP(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mt_state), mt, sizeof(uint32_t) * 624, 0,
+                                            hipMemcpyHostToDevice, ctx->stream));
+      host_seed[ctx->device] = (int64_t)cfg->seed;
+    }
+  }
   {
     SloamKernelScope ks(ctx, "line_ransac");
     hipLaunchKernelGGL(line_ransac_kernel, dim3(n_sets), dim3(64), 0, ctx->stream, d_xyz,

@@ -49,18 +49,31 @@ struct GroundCfgDev {
   float ox, oy;
 };
 
+// points per workgroup of ground_bin; cells counted in LDS up to kBinLds cells
+constexpr int kBinPts = 2048;
+constexpr int kBinLds = 4096;
+
 __global__ __launch_bounds__(kGB) void ground_bin_kernel(
     const float4* __restrict__ pts, int cap, const int32_t* __restrict__ d_n, GroundCfgDev cfg,
     unsigned long long* __restrict__ key_idx, unsigned long long* __restrict__ key_z,
     int32_t* __restrict__ vals_idx, int32_t* __restrict__ vals_z, int32_t* __restrict__ counts) {
-  const int i = blockIdx.x * kGB + threadIdx.x;
-  if (i >= cap) return;
-  const int n = d_n ? *d_n : cap;
-  unsigned long long cell = 0xFFFFull;
-  float z = 0.f;
-  if (i < n) {
+  // per-workgroup cell counts in LDS, one global atomic per non-empty cell
+  // (all points of a scan fall in few cells: per-point global atomics on the
+  // same counters serialise)
+  __shared__ int hcount[kBinLds];
+  const bool lds = cfg.n_cells <= kBinLds;
+  if (lds)
+    for (int c = threadIdx.x; c < cfg.n_cells; c += kGB) hcount[c] = 0;
+  __syncthreads();
+  const int n = min(d_n ? *d_n : cap, cap);
+  const int base = blockIdx.x * kBinPts;
+  for (int j = threadIdx.x; j < kBinPts; j += kGB) {
+    const int i = base + j;
+    if (i >= n) break;  // the sorts stop at n
+    // points outside the grid get cell n_cells: after every cell in both sorts
+    unsigned long long cell = (unsigned long long)cfg.n_cells;
     const float4 p = pts[i];
-    z = p.z;
+    const float z = p.z;
     if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
       const float dx = cfg.ox - p.x, dy = cfg.oy - p.y;
       const float a = (float)((double)dx * (double)dx);
@@ -73,15 +86,24 @@ __global__ __launch_bounds__(kGB) void ground_bin_kernel(
         rb = max(0, min(rb, cfg.R - 1));
         tb = max(0, min(tb, cfg.T - 1));
         cell = (unsigned long long)(rb * cfg.T + tb);
-        atomicAdd(counts + cell, 1);
+        if (lds)
+          atomicAdd(&hcount[(int)cell], 1);
+        else
+          atomicAdd(counts + cell, 1);
       }
     }
+    // (cell) and (cell, z): the stable sort keeps input order within equal keys
+    key_idx[i] = cell;
+    key_z[i] = (cell << 32) | (unsigned long long)orderable(z);
+    vals_idx[i] = i;
+    vals_z[i] = i;
   }
-  // (cell) and (cell, z): the stable sort keeps input order within equal keys
-  key_idx[i] = cell;
-  key_z[i] = (cell << 32) | (unsigned long long)orderable(z);
-  vals_idx[i] = i;
-  vals_z[i] = i;
+  __syncthreads();
+  if (lds)
+    for (int c = threadIdx.x; c < cfg.n_cells; c += kGB) {
+      const int v = hcount[c];
+      if (v) atomicAdd(counts + c, v);
+    }
 }
 
 __global__ __launch_bounds__(kGB) void ground_offsets_kernel(const int32_t* __restrict__ counts,
@@ -670,17 +692,20 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
   {
     SloamKernelScope ks(ctx, "ground_bin");
     if (capacity > 0)
-      hipLaunchKernelGGL(ground_bin_kernel, dim3((capacity + kGB - 1) / kGB), dim3(kGB), 0,
+      hipLaunchKernelGGL(ground_bin_kernel, dim3((capacity + kBinPts - 1) / kBinPts), dim3(kGB), 0,
                          ctx->stream, reinterpret_cast<const float4*>(d_points), capacity,
                          d_count, dc, kA, kB, vA, vB, counts);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (capacity > 0) {
     SloamKernelScope ks(ctx, "ground_sort");
-    // cell ids fit 16 bits (out-of-range points: 0xFFFF)
-    st = sloam_radix_sort(ctx, kA, vA, kT, vT, capacity, 16, hist);
+    // cell ids 0..n_cells (n_cells: outside the grid) in cell_bits bits; the
+    // (cell, z) keys hold the orderable z in the low 32 bits
+    int cell_bits = 1;
+    while ((1 << cell_bits) <= n_cells) ++cell_bits;
+    st = sloam_radix_sort(ctx, kA, vA, kT, vT, capacity, 0, cell_bits, d_count, hist);
     if (st) return st;
-    st = sloam_radix_sort(ctx, kB, vB, kT, vT, capacity, 48, hist);
+    st = sloam_radix_sort(ctx, kB, vB, kT, vT, capacity, 0, 32 + cell_bits, d_count, hist);
     if (st) return st;
   }
   {

@@ -235,13 +235,15 @@ __global__ __launch_bounds__(kTB) void tr_key_kernel(const int* __restrict__ lab
   const int p = blockIdx.x * kTB + threadIdx.x;
   if (p >= cfg.N) return;
   const int l = label[p];
+  const int r = p / cfg.W, c = p % cfg.W;
   unsigned long long k = cfg.pad;
   if (l >= 0 && size[l] > cfg.min_label_points) {
-    const int r = p / cfg.W, c = p % cfg.W;
     k = ((unsigned long long)l << (cfg.rb + cfg.cb)) |
         ((unsigned long long)(cfg.H - 1 - r) << cfg.cb) | (unsigned long long)c;
   }
-  keys[p] = k;
+  // stored in (H-1-r, c) order: a stable sort by the label bits alone then
+  // yields the order of the whole key (label, H-1-r, c)
+  keys[(cfg.H - 1 - r) * cfg.W + c] = k;
 }
 
 __device__ inline bool seg_start_at(const unsigned long long* keys, int i, const TrCfgDev& cfg) {
@@ -665,8 +667,8 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
     SLOAM_HIP(ctx, hipGetLastError());
     {
       SloamKernelScope ks(ctx, "trellis_sort");
-      st = sloam_radix_sort(ctx, keys, nullptr, keys2, nullptr, N, dc.key_bits,
-                            (int32_t*)sort_buf);
+      st = sloam_radix_sort(ctx, keys, nullptr, keys2, nullptr, N, dc.rb + dc.cb, dc.key_bits,
+                            nullptr, (int32_t*)sort_buf);
       if (st) return st;
     }
     {
