@@ -71,7 +71,7 @@ int main(int argc, char** argv) {
   std::fclose(f);
 
   FeatureModelParams fm;  // sloam.yaml
-  std::vector<double> total, fe, up, down, est;
+  std::vector<double> total, fe, up, down, est, build, ph_models, ph_match, ph_pose, ph_finish;
   double trees = 0, planes = 0;
   int samples = 0, ok_count = 0;
   double final_drift = 0, max_drift = 0, max_rot = 0;
@@ -94,6 +94,11 @@ int main(int argc, char** argv) {
         up.push_back(t.upload_ms);
         down.push_back(t.download_ms);
         est.push_back(t.estimator_ms);
+        build.push_back(t.build_ms);
+        ph_models.push_back(t.phases.models);
+        ph_match.push_back(t.phases.match);
+        ph_pose.push_back(t.phases.pose);
+        ph_finish.push_back(t.phases.finish);
         trees += t.trees;
         planes += t.planes;
         ++samples;
@@ -117,13 +122,16 @@ int main(int argc, char** argv) {
       "{\"keyframes\": %d, \"passes\": %d, \"height\": %d, \"width\": %d, \"graph\": %s, "
       "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"mean\": %.4f, \"max\": %.4f}, "
       "\"stage_p50_ms\": {\"upload\": %.4f, \"frontend\": %.4f, \"download\": %.4f, "
-      "\"estimator\": %.4f}, \"frontend_p99_ms\": %.4f, \"trees_mean\": %.2f, "
+      "\"estimator\": %.4f}, \"estimator_p50_ms\": {\"build\": %.4f, \"models\": %.4f, "
+      "\"match\": %.4f, \"pose\": %.4f, \"finish\": %.4f}, "
+      "\"frontend_p99_ms\": %.4f, \"trees_mean\": %.2f, "
       "\"planes_mean\": %.2f, \"success_rate\": %.4f, \"drift\": {\"final_m\": %.4f, "
       "\"max_m\": %.4f, \"max_deg\": %.4f}}\n",
       K, passes, H, W, use_graph ? "true" : "false", pct(total, 0.5), pct(total, 0.99),
       samples ? [&] { double s = 0; for (double x : total) s += x; return s / samples; }() : 0.0,
       total.empty() ? 0.0 : *std::max_element(total.begin(), total.end()), pct(up, 0.5),
-      pct(fe, 0.5), pct(down, 0.5), pct(est, 0.5), pct(fe, 0.99),
+      pct(fe, 0.5), pct(down, 0.5), pct(est, 0.5), pct(build, 0.5), pct(ph_models, 0.5),
+      pct(ph_match, 0.5), pct(ph_pose, 0.5), pct(ph_finish, 0.5), pct(fe, 0.99),
       samples ? trees / samples : 0.0, samples ? planes / samples : 0.0,
       samples ? (double)ok_count / samples : 0.0, final_drift, max_drift, max_rot);
   return 0;

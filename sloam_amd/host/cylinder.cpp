@@ -168,9 +168,19 @@ Scalar Cylinder::distance(const PointT& point) const {
   return (p - proj).norm() - model.radius;
 }
 
-void Cylinder::project(const SE3& tf) {
+CylinderParameters Cylinder::projectModel(const SE3& tf) const {
+  CylinderParameters m;
+  m.radius = model.radius;
+  m.lambda = model.lambda;
   Vector3 other = model.root + model.ray;
-  model.root = tf * model.root;
+  m.root = tf * model.root;
   other = tf * other;
-  model.ray = other - model.root;
+  m.ray = other - m.root;
+  return m;
+}
+
+void Cylinder::project(const SE3& tf) {
+  const CylinderParameters m = projectModel(tf);
+  model.root = m.root;
+  model.ray = m.ray;
 }

@@ -38,6 +38,9 @@ class Cylinder : public SemanticObject<CylinderParameters> {
   Scalar distance(const CylinderParameters& model) const override;
   Scalar distance(const PointT& point) const override;
   void project(const SE3& tf) override;
+  // the model's root and ray as project() leaves them, without touching this
+  // object (radius and lambda copied; radii and vertices left empty)
+  CylinderParameters projectModel(const SE3& tf) const;
   PointT asPoint(const Scalar height);
 
   // cylinder.cpp:3-13: vertex [1] within maxLidarDist (2-D) of the plane centroid

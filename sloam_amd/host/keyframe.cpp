@@ -131,6 +131,7 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
       }
     in.groundCells.emplace_back(pp, f, valid);
   }
+  const auto t3b = Clock::now();
   const bool ok = sloam_.RunSloam(in, out);
   const auto t4 = Clock::now();
   if (timing) {
@@ -138,6 +139,8 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
     timing->frontend_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     timing->download_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
     timing->estimator_ms = std::chrono::duration<double, std::milli>(t4 - t3).count();
+    timing->build_ms = std::chrono::duration<double, std::milli>(t3b - t3).count();
+    timing->phases = sloam_.lastPhaseTimes();
     timing->total_ms = ms_since(t0);
     timing->trees = (int)out.tm.size();
     timing->planes = (int)sloam_.getPrevGroundModel().size();

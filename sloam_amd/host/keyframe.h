@@ -15,6 +15,8 @@
 
 struct KeyframeTiming {
   double upload_ms = 0, frontend_ms = 0, download_ms = 0, estimator_ms = 0, total_ms = 0;
+  double build_ms = 0;      // estimator input assembled from the front end's arrays
+  SloamPhaseTimes phases;   // RunSloam's phases
   int trees = 0, planes = 0;
 };
 

@@ -56,6 +56,12 @@ void Plane::project(const SE3& tf) {
     p.y = (float)(tfm(1, 0) * x + tfm(1, 1) * y + tfm(1, 2) * z + tfm(1, 3));
     p.z = (float)(tfm(2, 0) * x + tfm(2, 1) * y + tfm(2, 2) * z + tfm(2, 3));
   }
-  model.plane = tfm.inverse().transpose() * model.plane;
-  model.centroid = tf * model.centroid;
+  model = projectModel(tf);
+}
+
+PlaneParameters Plane::projectModel(const SE3& tf) const {
+  PlaneParameters m = model;
+  m.plane = tf.matrix().inverse().transpose() * model.plane;
+  m.centroid = tf * model.centroid;
+  return m;
 }

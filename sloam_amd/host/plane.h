@@ -23,4 +23,6 @@ class Plane : public SemanticObject<PlaneParameters> {
   Scalar distance(const PlaneParameters& tgt) const override;
   Scalar distance(const PointT& point) const override;
   void project(const SE3& tf) override;
+  // the model as project() leaves it, without touching this object
+  PlaneParameters projectModel(const SE3& tf) const;
 };

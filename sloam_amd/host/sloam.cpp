@@ -4,8 +4,10 @@
 // (joint, or the two steps together) in one sloam_pose_plan batch.
 #include "sloam.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <utility>
 
 #include "gpu_bridge.h"
 
@@ -50,13 +52,56 @@ bool same_plane(const double* a, const PlaneParameters& m) {
   return a[0] == m.plane[0] && a[1] == m.plane[1] && a[2] == m.plane[2] && a[3] == m.plane[3];
 }
 
-// Each ObjectMatch carries its own copy of the landmark; consecutive matches
-// against the same landmark share one table entry (the residuals are the
-// same as with one entry per match).
-void add_tree_matches(const std::vector<ObjectMatch<Cylinder>>& matches,
-                      sloam_gpu::PoseProblem& pr) {
+// A feature matched to a map model, pointing at the map object instead of
+// copying it (ObjectMatch<T> holds a full copy of T, vertices and points
+// included): the form RunSloam uses between association and the pose solve.
+template <typename M>
+struct ModelMatch {
+  Vector3 feature;
+  const M* model;
+  double dist;
+};
+
+const CylinderParameters& model_of(const ObjectMatch<Cylinder>& m) { return m.object.model; }
+const PlaneParameters& model_of(const ObjectMatch<Plane>& m) { return m.object.model; }
+template <typename M>
+const M& model_of(const ModelMatch<M>& m) {
+  return *m.model;
+}
+
+// matchFeatures (sloam.cpp:443-471) without the copies: the same nearest map
+// model (strict <, first wins) for each current object's projected model, and
+// one match per feature of the current object.
+template <typename T, typename M>
+std::vector<ModelMatch<M>> match_features(const SE3& tf, const std::vector<T>& currObjects,
+                                          const std::vector<T>& mapObjects,
+                                          const Scalar distThresh) {
+  std::vector<ModelMatch<M>> matches;
+  if (mapObjects.empty()) return matches;
+  for (const auto& co : currObjects) {
+    const M pm = co.projectModel(tf);
+    Scalar bestDist = distThresh + 100;
+    size_t best = 0;
+    for (size_t i = 0; i < mapObjects.size(); ++i) {
+      const Scalar d = mapObjects[i].distance(pm);
+      if (d < bestDist) {
+        bestDist = d;
+        best = i;
+      }
+    }
+    if (bestDist < distThresh)
+      for (const auto& f : co.features)
+        matches.push_back(ModelMatch<M>{Vector3(f.x, f.y, f.z), &mapObjects[best].model, bestDist});
+  }
+  return matches;
+}
+
+// Consecutive matches against the same landmark share one table entry (the
+// residuals are the same as with one entry per match).
+template <typename V>
+void add_tree_matches(const V& matches, sloam_gpu::PoseProblem& pr) {
   for (const auto& m : matches) {
-    const CylinderParameters& o = m.object.model;
+    const CylinderParameters& o = model_of(m);
     const size_t n = pr.cyl.size() / 8;
     if (n == 0 || !same_cylinder(&pr.cyl[8 * (n - 1)], o)) {
       const double row[8] = {o.root[0], o.root[1], o.root[2], o.ray[0],
@@ -67,10 +112,10 @@ void add_tree_matches(const std::vector<ObjectMatch<Cylinder>>& matches,
   }
 }
 
-void add_ground_matches(const std::vector<ObjectMatch<Plane>>& matches,
-                        sloam_gpu::PoseProblem& pr) {
+template <typename V>
+void add_ground_matches(const V& matches, sloam_gpu::PoseProblem& pr) {
   for (const auto& m : matches) {
-    const PlaneParameters& o = m.object.model;
+    const PlaneParameters& o = model_of(m);
     const size_t n = pr.planes.size() / 4;
     if (n == 0 || !same_plane(&pr.planes[4 * (n - 1)], o))
       for (int i = 0; i < 4; ++i) pr.planes.push_back(o.plane[i]);
@@ -100,10 +145,9 @@ void pose_params(const SE3& pose, double params[6]) {
 // Runs the XY-yaw (trees) and Z-roll-pitch (ground) solves of a pose as one
 // batch; a step that is not run or does not converge keeps the initial
 // values (sloam.cpp:99-110, 151-161).
-void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround,
-              const std::vector<ObjectMatch<Cylinder>>& allTMatch,
-              const std::vector<ObjectMatch<Plane>>& allGMatch, double treeOut[3],
-              double groundOut[3]) {
+template <typename TV, typename GV>
+void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround, const TV& allTMatch,
+              const GV& allGMatch, double treeOut[3], double groundOut[3]) {
   double p0[6];
   pose_params(poseEstimate, p0);
   treeOut[0] = p0[0];
@@ -145,29 +189,10 @@ void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround,
   }
 }
 
-}  // namespace
-
-sloam::sloam() {
-  T_Map_Anchor_ = SE3();
-  firstScan_ = true;
-  minPlanes_ = (fmParams_.groundRadiiBins * fmParams_.groundThetaBins) * 0.1;
-}
-
-std::vector<Plane> sloam::getPrevGroundModel() { return prevGPlanes_; }
-
-CloudT sloam::getPrevGroundFeatures() {
-  CloudT pc;
-  for (const auto& ground : prevGPlanes_)
-    for (const auto& pt : ground.features) pc.points.push_back(pt);
-  pc.height = 1;
-  pc.width = (uint32_t)pc.points.size();
-  return pc;
-}
-
-bool sloam::TwoStepOptimizePose(const SE3& poseEstimate, const bool optimTrees,
-                                const bool optimGround,
-                                const std::vector<ObjectMatch<Cylinder>>& allTMatch,
-                                const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
+// TwoStepOptimizePose (sloam.cpp:33-53) over either match form
+template <typename TV, typename GV>
+bool two_step_pose(const SE3& poseEstimate, bool optimTrees, bool optimGround, const TV& allTMatch,
+                   const GV& allGMatch, SE3& tf) {
   double treeOut[3], groundOut[3];
   two_step(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, treeOut, groundOut);
   // roll, pitch from the ground step, yaw from the tree step (sloam.cpp:41-51)
@@ -181,21 +206,9 @@ bool sloam::TwoStepOptimizePose(const SE3& poseEstimate, const bool optimTrees,
   return true;
 }
 
-void sloam::OptimizeXYYaw(const SE3& poseEstimate, const bool optimize,
-                          const std::vector<ObjectMatch<Cylinder>>& allTMatch, double* out) {
-  double ground[3];
-  two_step(poseEstimate, optimize, false, allTMatch, {}, out, ground);
-}
-
-void sloam::OptimizeZRollPitch(const SE3& poseEstimate, const bool optimize,
-                               const std::vector<ObjectMatch<Plane>>& allGMatch, double* out) {
-  double tree[3];
-  two_step(poseEstimate, false, optimize, {}, allGMatch, tree, out);
-}
-
-bool sloam::OptimizePose(const SE3& poseEstimate,
-                         const std::vector<ObjectMatch<Cylinder>>& allTMatch,
-                         const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
+// OptimizePose (sloam.cpp:174-255) over either match form
+template <typename TV, typename GV>
+bool joint_pose(const SE3& poseEstimate, const TV& allTMatch, const GV& allGMatch, SE3& tf) {
   std::vector<sloam_gpu::PoseProblem> probs(1);
   set_initial_pose(poseEstimate, probs[0]);
   add_tree_matches(allTMatch, probs[0]);
@@ -219,6 +232,52 @@ bool sloam::OptimizePose(const SE3& poseEstimate,
   return success;
 }
 
+}  // namespace
+
+sloam::sloam() {
+  T_Map_Anchor_ = SE3();
+  firstScan_ = true;
+  minPlanes_ = (fmParams_.groundRadiiBins * fmParams_.groundThetaBins) * 0.1;
+}
+
+std::vector<Plane> sloam::getPrevGroundModel() { return prevGPlanes_; }
+
+CloudT sloam::getPrevGroundFeatures() {
+  CloudT pc;
+  for (const auto& ground : prevGPlanes_)
+    for (const auto& pt : ground.features) pc.points.push_back(pt);
+  pc.height = 1;
+  pc.width = (uint32_t)pc.points.size();
+  return pc;
+}
+
+bool sloam::TwoStepOptimizePose(const SE3& poseEstimate, const bool optimTrees,
+                                const bool optimGround,
+                                const std::vector<ObjectMatch<Cylinder>>& allTMatch,
+                                const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
+  return two_step_pose(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, tf);
+}
+
+void sloam::OptimizeXYYaw(const SE3& poseEstimate, const bool optimize,
+                          const std::vector<ObjectMatch<Cylinder>>& allTMatch, double* out) {
+  double ground[3];
+  two_step(poseEstimate, optimize, false, allTMatch, std::vector<ObjectMatch<Plane>>(), out,
+           ground);
+}
+
+void sloam::OptimizeZRollPitch(const SE3& poseEstimate, const bool optimize,
+                               const std::vector<ObjectMatch<Plane>>& allGMatch, double* out) {
+  double tree[3];
+  two_step(poseEstimate, false, optimize, std::vector<ObjectMatch<Cylinder>>(), allGMatch, tree,
+           out);
+}
+
+bool sloam::OptimizePose(const SE3& poseEstimate,
+                         const std::vector<ObjectMatch<Cylinder>>& allTMatch,
+                         const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
+  return joint_pose(poseEstimate, allTMatch, allGMatch, tf);
+}
+
 template <typename T>
 std::vector<ObjectMatch<T>> sloam::matchFeatures(const SE3 tf, const std::vector<T>& currObjects,
                                                  const std::vector<T>& mapObjects,
@@ -226,13 +285,13 @@ std::vector<ObjectMatch<T>> sloam::matchFeatures(const SE3 tf, const std::vector
   std::vector<ObjectMatch<T>> matches;
   if (mapObjects.empty()) return matches;  // the reference reads mapObjects[0]
   for (const auto& co : currObjects) {
-    T proj_obj = co;
-    proj_obj.project(tf);
+    // the model of the object projected by tf (project() without the copy)
+    const auto pm = co.projectModel(tf);
     // closest map model; ties keep the first (strict <)
     Scalar bestDist = distThresh + 100;
     size_t best = 0;
     for (size_t i = 0; i < mapObjects.size(); ++i) {
-      const Scalar d = mapObjects[i].distance(proj_obj.getModel());
+      const Scalar d = mapObjects[i].distance(pm);
       if (d < bestDist) {
         bestDist = d;
         best = i;
@@ -258,7 +317,7 @@ void sloam::matchModels(const std::vector<T>& currObjects, const std::vector<T>&
     Scalar bestDist = fmParams_.treeMatchThresh + 100;
     size_t bestKey = 0;
     for (size_t key = 0; key < mapObjects.size(); ++key) {
-      const Scalar d = mapObjects[key].distance(co.getModel());
+      const Scalar d = mapObjects[key].distance(co.model);
       if (d < bestDist) {
         bestDist = d;
         bestKey = key;
@@ -378,10 +437,21 @@ void sloam::projectModels(const SE3& tf, std::vector<Cylinder>& landmarks,
   for (auto& l : landmarks) l.project(tf);
 }
 
+namespace {
+using PhaseClock = std::chrono::steady_clock;
+double ms_between(PhaseClock::time_point a, PhaseClock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+}  // namespace
+
 bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
+  phase_ = SloamPhaseTimes();
+  const auto p0 = PhaseClock::now();
   std::vector<Plane> planes;
   std::vector<Cylinder> landmarks;
   computeModels(in, landmarks, planes);
+  const auto p1 = PhaseClock::now();
+  phase_.models = ms_between(p0, p1);
 
   std::vector<int> matchIndices(landmarks.size(), -1);
   bool success = true;
@@ -398,10 +468,15 @@ bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
   if (in.mapModels.empty()) return false;                // empty map
   if (planes.empty() || landmarks.empty()) return false;  // no models in this scan
 
-  const auto treeMatches =
-      matchFeatures(in.poseEstimate, landmarks, in.mapModels, fmParams_.treeMatchThresh);
+  // matchFeatures' associations, pointing at the map objects instead of
+  // copying them (they stay alive until the pose solve is done)
+  const auto treeMatches = match_features<Cylinder, CylinderParameters>(
+      in.poseEstimate, landmarks, in.mapModels, fmParams_.treeMatchThresh);
   // the ground association threshold is fixed at 1.0 (sloam.cpp:482)
-  const auto planeMatches = matchFeatures(in.poseEstimate, planes, prevGPlanes_, 1.0);
+  const auto planeMatches =
+      match_features<Plane, PlaneParameters>(in.poseEstimate, planes, prevGPlanes_, 1.0);
+  const auto p2 = PhaseClock::now();
+  phase_.match = ms_between(p1, p2);
 
   SE3 T_Delta = SE3();
   SE3 currPose = in.poseEstimate;
@@ -410,20 +485,23 @@ bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
   const bool groundCheck =
       planeMatches.size() > fmParams_.minGroundModels && planes.size() > minPlanes_;
   if (fmParams_.twoStepOptim) {
-    success = TwoStepOptimizePose(in.poseEstimate, treeCheck, groundCheck, treeMatches,
-                                  planeMatches, currPose);
+    success = two_step_pose(in.poseEstimate, treeCheck, groundCheck, treeMatches, planeMatches,
+                            currPose);
   } else if (treeCheck && groundCheck) {
-    success = OptimizePose(in.poseEstimate, treeMatches, planeMatches, T_Delta);
+    success = joint_pose(in.poseEstimate, treeMatches, planeMatches, T_Delta);
     if (success) currPose = T_Delta;
   }
+  const auto p3 = PhaseClock::now();
+  phase_.pose = ms_between(p2, p3);
 
   projectModels(currPose, landmarks, planes);
   matchModels(landmarks, in.mapModels, matchIndices);
-  prevGPlanes_ = planes;
+  prevGPlanes_ = std::move(planes);
   out.matches = matchIndices;
   out.T_Map_Curr = currPose;
   out.T_Delta = T_Delta;
-  out.tm = landmarks;
+  out.tm = std::move(landmarks);
+  phase_.finish = ms_between(p3, PhaseClock::now());
   return success;
 }
 

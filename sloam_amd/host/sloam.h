@@ -43,6 +43,12 @@ struct SloamOutput {
 // ground cells [radii bin][theta bin] (the reference's boost::multi_array)
 using GroundGrid = std::vector<std::vector<VectorType>>;
 
+// wall time (ms) of the phases of the last RunSloam call: model fits,
+// feature association, pose solve, model projection + map matching
+struct SloamPhaseTimes {
+  double models = 0, match = 0, pose = 0, finish = 0;
+};
+
 namespace sloam {
 
 class sloam {
@@ -84,6 +90,7 @@ class sloam {
 
   std::vector<Plane> getPrevGroundModel();
   CloudT getPrevGroundFeatures();
+  const SloamPhaseTimes& lastPhaseTimes() const { return phase_; }
 
  private:
   // ground binning / fit settings of fmParams_, binned around `pose`
@@ -96,6 +103,7 @@ class sloam {
   FeatureModelParams fmParams_;
   bool firstScan_ = true;
   double minPlanes_ = 0.0;
+  SloamPhaseTimes phase_;
 };
 
 }  // namespace sloam
