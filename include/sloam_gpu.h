@@ -401,14 +401,18 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
  * 80-300, trellis.cpp:134-141, sloam.cpp:388-414).  With use_graph the first
  * exec runs eagerly (sizing the scratch buffers) and captures the sequence
  * into a HIP graph; every later exec is one graph launch.  The caller writes
- * the scan into out.xyzi / out.label (device) before each exec.             */
+ * the scan into out.xyzi / out.label (device) before each exec.  With
+ * host_results the sequence ends with a kernel that writes the used part of
+ * the tree and ground-cell outputs into a pinned host buffer
+ * (sloam_frontend_host_view), so they are in host memory as soon as the exec's
+ * work has completed, without further copies.                               */
 typedef struct sloam_frontend sloam_frontend;
 typedef struct sloam_frontend_cfg {
   sloam_proj_cfg proj;
   sloam_trellis_cfg trellis;
   sloam_ground_cfg ground;
   int32_t use_graph;
-  int32_t reserved;
+  int32_t host_results;
 } sloam_frontend_cfg;
 typedef struct sloam_frontend_out { /* device pointers owned by the plan */
   float* xyzi;               /* in:  H*W x 4                               */
@@ -432,10 +436,27 @@ typedef struct sloam_frontend_out { /* device pointers owned by the plan */
   int32_t* kept_index;
 } sloam_frontend_out;
 
+typedef struct sloam_frontend_host { /* host pointers into the plan's pinned buffer */
+  const int32_t* trellis_counts;  /* 8 int32, as sloam_trellis' d_counts       */
+  const int32_t* tree_offsets;    /* trellis_counts[0] + 1                       */
+  const int32_t* vertex_int;      /* 4 x trellis_counts[1]                       */
+  const double* vertex_f64;       /* 4 x trellis_counts[1]                       */
+  const int32_t* point_offsets;   /* trellis_counts[1] + 1                       */
+  const float* points;            /* 4 x trellis_counts[2]                       */
+  const int32_t* cell_info;       /* 4 x n_cells                                 */
+  const double* cell_model;       /* 8 x n_cells                                 */
+  const float* cell_features;     /* 4 x n_cells x num_features                  */
+} sloam_frontend_host;
+
 int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_frontend** out);
 void sloam_frontend_destroy(sloam_frontend* f);
 int sloam_frontend_buffers(const sloam_frontend* f, sloam_frontend_out* out);
 int sloam_frontend_exec(sloam_frontend* f);
+/* Host view of the results of the last exec (valid once its work has
+ * completed, e.g. after sloam_sync); SLOAM_ERR_INVALID_ARG unless the plan was
+ * created with host_results.  Counts past the capacities (overflow flag
+ * trellis_counts[4]) are clamped to the capacities in the copied arrays. */
+int sloam_frontend_host_view(const sloam_frontend* f, sloam_frontend_host* out);
 
 /* ---- learned cylinder proposal (PointNet, BF16 MFMA) -----------------------
  * No counterpart in the reference.  It proposes a cylinder per cluster, so that

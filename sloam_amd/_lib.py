@@ -125,8 +125,14 @@ class FrontendCfg(C.Structure):
         ("trellis", TrellisCfg),
         ("ground", GroundCfg),
         ("use_graph", C.c_int32),
-        ("reserved", C.c_int32),
+        ("host_results", C.c_int32),
     ]
+
+
+class FrontendHost(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("trellis_counts", "tree_offsets", "vertex_int",
+                                          "vertex_f64", "point_offsets", "points", "cell_info",
+                                          "cell_model", "cell_features")]
 
 
 class FrontendOut(C.Structure):
@@ -178,6 +184,7 @@ PROTOTYPES = {
     "sloam_frontend_destroy": (None, [_P]),
     "sloam_frontend_buffers": (C.c_int, [_P, C.POINTER(FrontendOut)]),
     "sloam_frontend_exec": (C.c_int, [_P]),
+    "sloam_frontend_host_view": (C.c_int, [_P, C.POINTER(FrontendHost)]),
     "sloam_cyl_plan_exec_ex": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sloam_cyl_fallback": (C.c_int, [_P, _P, _I32, _P, C.c_float, _P, _P]),
     "sloam_cyl_autotune": (C.c_int, [_P, _P, _I32, C.POINTER(CylFitCfg), _P, _P,

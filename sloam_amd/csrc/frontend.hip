@@ -16,9 +16,50 @@ struct sloam_frontend {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int runs = 0;
+  // host_results: pinned host buffer (host and device address) and the word
+  // offsets of its regions
+  void* host = nullptr;
+  void* host_dev = nullptr;
+  size_t region[9] = {};
 };
 
 namespace {
+
+// Regions of the pinned result buffer, in 32-bit words: counts, tree offsets,
+// vertex ints, vertex doubles, point offsets, points, cell info, cell models,
+// cell features; each starts on a 64-word boundary.
+enum { kRCounts, kRTreeOff, kRVtxInt, kRVtxF64, kRPtOff, kRPoints, kRCellInfo, kRCellModel,
+       kRCellFeat, kRegions };
+
+struct PackArgs {
+  const int32_t* counts;
+  const uint32_t* src[kRegions];
+  uint32_t* dst[kRegions];
+  int cap_trees, cap_vertices, cap_points, n_cells, nf;
+};
+
+// Copies the used part of every output into the pinned host buffer (one
+// grid-stride loop per region; lengths from the device-side counts, clamped
+// to the capacities).
+__global__ __launch_bounds__(256) void fe_pack_kernel(PackArgs a) {
+  const int nt = min(max(a.counts[0], 0), a.cap_trees);
+  const int nv = min(max(a.counts[1], 0), a.cap_vertices);
+  const int np = min(max(a.counts[2], 0), a.cap_points);
+  const size_t len[kRegions] = {8,
+                                (size_t)nt + 1,
+                                4 * (size_t)nv,
+                                8 * (size_t)nv,
+                                (size_t)nv + 1,
+                                4 * (size_t)np,
+                                4 * (size_t)a.n_cells,
+                                16 * (size_t)a.n_cells,
+                                4 * (size_t)a.n_cells * (size_t)a.nf};
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kRegions; ++r)
+    for (size_t i = t0; i < len[r]; i += stride) a.dst[r][i] = a.src[r][i];
+}
 
 int enqueue(sloam_frontend* f) {
   sloam_ctx* ctx = f->ctx;
@@ -30,8 +71,27 @@ int enqueue(sloam_frontend* f) {
                      nullptr, o.cap_trees, o.cap_vertices, o.cap_points, o.tree_offsets,
                      o.vertex_int, o.vertex_f64, o.point_offsets, o.points, o.trellis_counts);
   if (st) return st;
-  return sloam_ground_planes(ctx, o.ground, f->N, o.n_ground, &f->cfg.ground, o.cell_info,
-                             o.cell_model, o.cell_features, o.kept_offsets, o.kept_index);
+  st = sloam_ground_planes(ctx, o.ground, f->N, o.n_ground, &f->cfg.ground, o.cell_info,
+                           o.cell_model, o.cell_features, o.kept_offsets, o.kept_index);
+  if (st || !f->host) return st;
+  PackArgs a;
+  a.counts = o.trellis_counts;
+  const void* src[kRegions] = {o.trellis_counts, o.tree_offsets, o.vertex_int, o.vertex_f64,
+                               o.point_offsets,  o.points,       o.cell_info,  o.cell_model,
+                               o.cell_features};
+  for (int r = 0; r < kRegions; ++r) {
+    a.src[r] = static_cast<const uint32_t*>(src[r]);
+    a.dst[r] = static_cast<uint32_t*>(f->host_dev) + f->region[r];
+  }
+  a.cap_trees = o.cap_trees;
+  a.cap_vertices = o.cap_vertices;
+  a.cap_points = o.cap_points;
+  a.n_cells = o.n_cells;
+  a.nf = f->cfg.ground.num_features;
+  SloamKernelScope ks(ctx, "frontend_pack");
+  hipLaunchKernelGGL(fe_pack_kernel, dim3(64), dim3(256), 0, ctx->stream, a);
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
 }
 
 }  // namespace
@@ -91,6 +151,29 @@ int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_f
       return st;
     }
   }
+  if (cfg->host_results) {
+    const size_t words[kRegions] = {8,
+                                    (size_t)o.cap_trees + 1,
+                                    4 * (size_t)o.cap_vertices,
+                                    8 * (size_t)o.cap_vertices,
+                                    (size_t)o.cap_vertices + 1,
+                                    4 * (size_t)o.cap_points,
+                                    4 * (size_t)n_cells,
+                                    16 * (size_t)n_cells,
+                                    4 * (size_t)n_cells * nf};
+    size_t total = 0;
+    for (int r = 0; r < kRegions; ++r) {
+      f->region[r] = total;
+      total += (words[r] + 63) / 64 * 64;
+    }
+    hipError_t e = hipHostMalloc(&f->host, 4 * total, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&f->host_dev, f->host, 0);
+    if (e != hipSuccess) {
+      sloam_record_hip_error(ctx, e, "hipHostMalloc(frontend results)");
+      sloam_frontend_destroy(f);
+      return SLOAM_ERR_ALLOC;
+    }
+  }
   *out = f;
   return SLOAM_OK;
 }
@@ -106,12 +189,28 @@ void sloam_frontend_destroy(sloam_frontend* f) {
                   o.points,        o.trellis_counts, o.cell_info, o.cell_model,  o.cell_features,
                   o.kept_offsets,  o.kept_index};
   for (void* p : ptrs) sloam_dev_free(p);
+  if (f->host) (void)hipHostFree(f->host);
   delete f;
 }
 
 int sloam_frontend_buffers(const sloam_frontend* f, sloam_frontend_out* out) {
   if (!f || !out) return SLOAM_ERR_INVALID_ARG;
   *out = f->out;
+  return SLOAM_OK;
+}
+
+int sloam_frontend_host_view(const sloam_frontend* f, sloam_frontend_host* out) {
+  if (!f || !out || !f->host) return SLOAM_ERR_INVALID_ARG;
+  const uint32_t* h = static_cast<const uint32_t*>(f->host);
+  out->trellis_counts = reinterpret_cast<const int32_t*>(h + f->region[kRCounts]);
+  out->tree_offsets = reinterpret_cast<const int32_t*>(h + f->region[kRTreeOff]);
+  out->vertex_int = reinterpret_cast<const int32_t*>(h + f->region[kRVtxInt]);
+  out->vertex_f64 = reinterpret_cast<const double*>(h + f->region[kRVtxF64]);
+  out->point_offsets = reinterpret_cast<const int32_t*>(h + f->region[kRPtOff]);
+  out->points = reinterpret_cast<const float*>(h + f->region[kRPoints]);
+  out->cell_info = reinterpret_cast<const int32_t*>(h + f->region[kRCellInfo]);
+  out->cell_model = reinterpret_cast<const double*>(h + f->region[kRCellModel]);
+  out->cell_features = reinterpret_cast<const float*>(h + f->region[kRCellFeat]);
   return SLOAM_OK;
 }
 

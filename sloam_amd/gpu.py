@@ -500,7 +500,7 @@ class Frontend:
     buffers, optionally replayed as one HIP graph."""
 
     def __init__(self, ctx, H, W, use_graph=True, fov_up=22.5, fov_down=-22.5, destagger=False,
-                 trellis=None, ground=None):
+                 trellis=None, ground=None, host_results=False):
         self.ctx = ctx
         self.lib = ctx.lib
         self.H, self.W, self.N = H, W, H * W
@@ -512,7 +512,7 @@ class Frontend:
         self.cfg = FrontendCfg(proj=ProjCfg(height=H, width=W, fov_up_deg=fov_up,
                                             fov_down_deg=fov_down, destagger=int(destagger)),
                                trellis=TrellisCfg(**tc), ground=GroundCfg(**gc),
-                               use_graph=int(use_graph), reserved=0)
+                               use_graph=int(use_graph), host_results=int(host_results))
         h = C.c_void_p()
         check(self.lib.sloam_frontend_create(ctx.handle, C.byref(self.cfg), C.byref(h)),
               ctx.handle)
@@ -576,3 +576,32 @@ class Frontend:
         self.ctx.sync()
         self._host = []
         return r
+
+    def host_view(self):
+        """Synchronises and returns copies of the results the front end wrote
+        into its pinned host buffer (plans created with host_results)."""
+        self.ctx.sync()
+        hv = _lib.FrontendHost()
+        check(self.lib.sloam_frontend_host_view(self.handle, C.byref(hv)), self.ctx.handle)
+
+        def arr(ptr, shape, ctype, dtype):
+            n = int(np.prod(shape))
+            if n == 0:
+                return np.empty(shape, dtype)
+            p = C.cast(ptr, C.POINTER(ctype))
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True).reshape(shape)
+
+        cnt = arr(hv.trellis_counts, 8, C.c_int32, np.int32)
+        nt, nv, npt = int(cnt[0]), int(cnt[1]), int(cnt[2])
+        nc = self.out.n_cells
+        return {
+            "trellis_counts": cnt,
+            "tree_offsets": arr(hv.tree_offsets, nt + 1, C.c_int32, np.int32),
+            "vertex_int": arr(hv.vertex_int, (nv, 4), C.c_int32, np.int32),
+            "vertex_f64": arr(hv.vertex_f64, (nv, 4), C.c_double, np.float64),
+            "point_offsets": arr(hv.point_offsets, nv + 1, C.c_int32, np.int32),
+            "points": arr(hv.points, (npt, 4), C.c_float, np.float32),
+            "cell_info": arr(hv.cell_info, (nc, 4), C.c_int32, np.int32),
+            "cell_model": arr(hv.cell_model, (nc, 8), C.c_double, np.float64),
+            "cell_features": arr(hv.cell_features, (nc, self.nf, 4), C.c_float, np.float32),
+        }

@@ -24,12 +24,6 @@ void check(int st, const char* what) {
   }
 }
 
-template <typename T>
-void get(std::vector<T>& v, const void* src) {
-  check(sloam_memcpy_async(sloam_gpu::context(), v.data(), src, sizeof(T) * v.size()),
-        "sloam_memcpy_async");
-}
-
 }  // namespace
 
 KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& fmParams,
@@ -57,8 +51,11 @@ KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& 
   cfg.ground.retain_thresh = fmParams.groundRetainThresh;
   cfg.ground.num_features = fmParams.numGroundFeatures;
   cfg.use_graph = use_graph ? 1 : 0;
+  // results land in pinned host memory at the end of every front-end run
+  cfg.host_results = 1;
   check(sloam_frontend_create(sloam_gpu::context(), &cfg, &fe_), "sloam_frontend_create");
   check(sloam_frontend_buffers(fe_, &buf_), "sloam_frontend_buffers");
+  check(sloam_frontend_host_view(fe_, &host_), "sloam_frontend_host_view");
   sloam_.setFmParams(fmParams);
 }
 
@@ -77,32 +74,26 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
   check(sloam_frontend_exec(fe_), "sloam_frontend_exec");
   check(sloam_sync(ctx), "sloam_sync");
   const auto t2 = Clock::now();
-  // results: sizes first, then the arrays
-  std::vector<int32_t> cnt(8);
-  get(cnt, buf_.trellis_counts);
-  check(sloam_sync(ctx), "sloam_sync");
+  // results: already in pinned host memory (written by the front end's last
+  // kernel)
+  const sloam_frontend_host& hv = host_;
+  const int32_t* cnt = hv.trellis_counts;
   if (cnt[4]) throw std::runtime_error("keyframe: trellis output capacity exceeded");
-  const int nt = cnt[0], nv = cnt[1], np = cnt[2], nc = buf_.n_cells;
+  const int nt = cnt[0], nc = buf_.n_cells;
   const int nf = fm_.numGroundFeatures;
-  std::vector<int32_t> to(nt + 1), vi(4 * (size_t)nv), po(nv + 1), info(4 * (size_t)nc);
-  std::vector<double> vf(4 * (size_t)nv), model(8 * (size_t)nc);
-  std::vector<float> pts(4 * (size_t)np), feat(4 * (size_t)nc * nf);
-  get(to, buf_.tree_offsets);
-  get(vi, buf_.vertex_int);
-  get(vf, buf_.vertex_f64);
-  get(po, buf_.point_offsets);
-  get(pts, buf_.points);
-  get(info, buf_.cell_info);
-  get(model, buf_.cell_model);
-  get(feat, buf_.cell_features);
-  check(sloam_sync(ctx), "sloam_sync");
+  const int32_t *to = hv.tree_offsets, *vi = hv.vertex_int, *po = hv.point_offsets,
+                *info = hv.cell_info;
+  const double *vf = hv.vertex_f64, *model = hv.cell_model;
+  const float *pts = hv.points, *feat = hv.cell_features;
   const auto t3 = Clock::now();
 
   SloamInput in;
   in.poseEstimate = poseEstimate;
   in.mapModels = mapModels;
+  in.landmarks.reserve(nt);
   for (int t = 0; t < nt; ++t) {
     std::vector<TreeVertex> tree;
+    tree.reserve(to[t + 1] - to[t]);
     for (int v = to[t]; v < to[t + 1]; ++v) {
       TreeVertex x;
       x.treeId = vi[4 * (size_t)v];
@@ -111,6 +102,7 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
       x.coords.x = (float)vf[4 * (size_t)v + 1];
       x.coords.y = (float)vf[4 * (size_t)v + 2];
       x.coords.z = (float)vf[4 * (size_t)v + 3];
+      x.points.reserve(po[v + 1] - po[v]);
       for (int p = po[v]; p < po[v + 1]; ++p)
         x.points.push_back(PointT{pts[4 * (size_t)p], pts[4 * (size_t)p + 1],
                                   pts[4 * (size_t)p + 2], pts[4 * (size_t)p + 3]});

@@ -43,6 +43,7 @@ class KeyframeRunner {
   int H_, W_;
   sloam_frontend* fe_ = nullptr;
   sloam_frontend_out buf_{};
+  sloam_frontend_host host_{};
   FeatureModelParams fm_;
   sloam::sloam sloam_;
 };

@@ -70,3 +70,22 @@ def test_eager_matches_oracle(ctx):
     assert np.array_equal(r["cell_model"][valid, 4:7], gp["model"][valid, 4:7])
     assert np.allclose(r["cell_model"][valid, :4], gp["model"][valid, :4], atol=1e-9, rtol=0)
     f.close()
+
+
+def test_host_results_match_device_buffers(ctx):
+    """With host_results the front end's last kernel writes the used part of
+    the tree and ground-cell outputs into pinned host memory: after every
+    graph replay the host view equals the device buffers."""
+    scans = _scans(3)
+    g = Frontend(ctx, H, W, use_graph=True, host_results=True)
+    for rep in range(2):
+        for i, sc in enumerate(scans):
+            print(f"[frontend] host results pass {rep} scan {i}", flush=True)
+            g.upload(sc["xyzi"], sc["label"])
+            g.exec()
+            hv = g.host_view()
+            rd = g.download()
+            for k, v in hv.items():
+                assert v.shape == rd[k].shape, k
+                assert np.array_equal(v, rd[k], equal_nan=True), k
+    g.close()
