@@ -413,6 +413,13 @@ typedef struct sloam_frontend_cfg {
   sloam_ground_cfg ground;
   int32_t use_graph;
   int32_t host_results;
+  /* fit_lines: after the tree instances, every tree's axis line is fitted to
+   * its vertex medians (as float) with `line`, as sloam_fit_lines would fit
+   * that tree alone (Cylinder::computeModel's sample consensus,
+   * sloam/src/objects/cylinder.cpp:116-139)                                  */
+  sloam_line_fit_cfg line;
+  int32_t fit_lines;
+  int32_t reserved;
 } sloam_frontend_cfg;
 typedef struct sloam_frontend_out { /* device pointers owned by the plan */
   float* xyzi;               /* in:  H*W x 4                               */
@@ -434,6 +441,8 @@ typedef struct sloam_frontend_out { /* device pointers owned by the plan */
   float* cell_features;
   int32_t* kept_offsets;
   int32_t* kept_index;
+  float* line_coeff;         /* fit_lines: cap_trees x 6 (sloam_fit_lines)  */
+  int32_t* line_stats;       /* fit_lines: cap_trees x 4                    */
 } sloam_frontend_out;
 
 typedef struct sloam_frontend_host { /* host pointers into the plan's pinned buffer */
@@ -446,6 +455,8 @@ typedef struct sloam_frontend_host { /* host pointers into the plan's pinned buf
   const int32_t* cell_info;       /* 4 x n_cells                                 */
   const double* cell_model;       /* 8 x n_cells                                 */
   const float* cell_features;     /* 4 x n_cells x num_features                  */
+  const float* line_coeff;        /* fit_lines: 6 x trellis_counts[0]            */
+  const int32_t* line_stats;      /* fit_lines: 4 x trellis_counts[0]            */
 } sloam_frontend_host;
 
 int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_frontend** out);

@@ -126,13 +126,17 @@ class FrontendCfg(C.Structure):
         ("ground", GroundCfg),
         ("use_graph", C.c_int32),
         ("host_results", C.c_int32),
+        ("line", LineFitCfg),
+        ("fit_lines", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
 class FrontendHost(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("trellis_counts", "tree_offsets", "vertex_int",
                                           "vertex_f64", "point_offsets", "points", "cell_info",
-                                          "cell_model", "cell_features")]
+                                          "cell_model", "cell_features", "line_coeff",
+                                          "line_stats")]
 
 
 class FrontendOut(C.Structure):
@@ -142,7 +146,8 @@ class FrontendOut(C.Structure):
                                            "point_offsets", "points", "trellis_counts")]
     _fields_ += [("n_cells", C.c_int32)]
     _fields_ += [(n, C.c_void_p) for n in ("cell_info", "cell_model", "cell_features",
-                                           "kept_offsets", "kept_index")]
+                                           "kept_offsets", "kept_index", "line_coeff",
+                                           "line_stats")]
 
 
 _P = C.c_void_p

@@ -88,6 +88,12 @@ int sloam_radix_sort(sloam_ctx* ctx, unsigned long long* keys, int32_t* vals,
                      int end_bit, const int32_t* d_n, int32_t* hist);
 size_t sloam_radix_hist_ints(int n);
 
+// sloam_fit_lines over n_sets sets of which the first *d_n_sets (device,
+// nullable: all) are fitted; the others are left untouched.
+int sloam_fit_lines_dev(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets,
+                        int32_t n_sets, const int32_t* d_n_sets, const sloam_line_fit_cfg* cfg,
+                        float* d_coeff, int32_t* d_stats, uint8_t* d_inliers);
+
 // ---------------------------------------------------------------------------
 // counter-based random numbers: Philox4x32-10
 // J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw, "Parallel random

@@ -20,22 +20,24 @@ struct sloam_frontend {
   // offsets of its regions
   void* host = nullptr;
   void* host_dev = nullptr;
-  size_t region[9] = {};
+  size_t region[11] = {};
+  float* medians = nullptr;  // fit_lines: vertex medians as float, 3 per vertex
 };
 
 namespace {
 
 // Regions of the pinned result buffer, in 32-bit words: counts, tree offsets,
 // vertex ints, vertex doubles, point offsets, points, cell info, cell models,
-// cell features; each starts on a 64-word boundary.
+// cell features, line coefficients, line stats; each starts on a 64-word
+// boundary.
 enum { kRCounts, kRTreeOff, kRVtxInt, kRVtxF64, kRPtOff, kRPoints, kRCellInfo, kRCellModel,
-       kRCellFeat, kRegions };
+       kRCellFeat, kRLineCoeff, kRLineStats, kRegions };
 
 struct PackArgs {
   const int32_t* counts;
   const uint32_t* src[kRegions];
   uint32_t* dst[kRegions];
-  int cap_trees, cap_vertices, cap_points, n_cells, nf;
+  int cap_trees, cap_vertices, cap_points, n_cells, nf, lines;
 };
 
 // Copies the used part of every output into the pinned host buffer (one
@@ -53,12 +55,28 @@ __global__ __launch_bounds__(256) void fe_pack_kernel(PackArgs a) {
                                 4 * (size_t)np,
                                 4 * (size_t)a.n_cells,
                                 16 * (size_t)a.n_cells,
-                                4 * (size_t)a.n_cells * (size_t)a.nf};
+                                4 * (size_t)a.n_cells * (size_t)a.nf,
+                                a.lines ? 6 * (size_t)nt : 0,
+                                a.lines ? 4 * (size_t)nt : 0};
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 #pragma unroll
   for (int r = 0; r < kRegions; ++r)
     for (size_t i = t0; i < len[r]; i += stride) a.dst[r][i] = a.src[r][i];
+}
+
+// Vertex medians (vertex_f64 x, y, z) as float, the points the line fit of a
+// tree takes.
+__global__ __launch_bounds__(256) void fe_medians_kernel(const double* __restrict__ vf,
+                                                         const int32_t* __restrict__ counts,
+                                                         int cap_vertices,
+                                                         float* __restrict__ med) {
+  const int nv = min(max(counts[1], 0), cap_vertices);
+  for (int v = blockIdx.x * 256 + threadIdx.x; v < nv; v += gridDim.x * 256) {
+    med[3 * (size_t)v] = (float)vf[4 * (size_t)v + 1];
+    med[3 * (size_t)v + 1] = (float)vf[4 * (size_t)v + 2];
+    med[3 * (size_t)v + 2] = (float)vf[4 * (size_t)v + 3];
+  }
 }
 
 int enqueue(sloam_frontend* f) {
@@ -71,14 +89,23 @@ int enqueue(sloam_frontend* f) {
                      nullptr, o.cap_trees, o.cap_vertices, o.cap_points, o.tree_offsets,
                      o.vertex_int, o.vertex_f64, o.point_offsets, o.points, o.trellis_counts);
   if (st) return st;
+  if (f->cfg.fit_lines) {
+    hipLaunchKernelGGL(fe_medians_kernel, dim3(64), dim3(256), 0, ctx->stream, o.vertex_f64,
+                       o.trellis_counts, o.cap_vertices, f->medians);
+    SLOAM_HIP(ctx, hipGetLastError());
+    // one set per tree (up to the device-side tree count)
+    st = sloam_fit_lines_dev(ctx, f->medians, o.tree_offsets, o.cap_trees, o.trellis_counts,
+                             &f->cfg.line, o.line_coeff, o.line_stats, nullptr);
+    if (st) return st;
+  }
   st = sloam_ground_planes(ctx, o.ground, f->N, o.n_ground, &f->cfg.ground, o.cell_info,
                            o.cell_model, o.cell_features, o.kept_offsets, o.kept_index);
   if (st || !f->host) return st;
   PackArgs a;
   a.counts = o.trellis_counts;
-  const void* src[kRegions] = {o.trellis_counts, o.tree_offsets, o.vertex_int, o.vertex_f64,
-                               o.point_offsets,  o.points,       o.cell_info,  o.cell_model,
-                               o.cell_features};
+  const void* src[kRegions] = {o.trellis_counts, o.tree_offsets,  o.vertex_int, o.vertex_f64,
+                               o.point_offsets,  o.points,        o.cell_info,  o.cell_model,
+                               o.cell_features,  o.line_coeff,    o.line_stats};
   for (int r = 0; r < kRegions; ++r) {
     a.src[r] = static_cast<const uint32_t*>(src[r]);
     a.dst[r] = static_cast<uint32_t*>(f->host_dev) + f->region[r];
@@ -88,6 +115,7 @@ int enqueue(sloam_frontend* f) {
   a.cap_points = o.cap_points;
   a.n_cells = o.n_cells;
   a.nf = f->cfg.ground.num_features;
+  a.lines = f->cfg.fit_lines ? 1 : 0;
   SloamKernelScope ks(ctx, "frontend_pack");
   hipLaunchKernelGGL(fe_pack_kernel, dim3(64), dim3(256), 0, ctx->stream, a);
   SLOAM_HIP(ctx, hipGetLastError());
@@ -108,6 +136,10 @@ int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_f
   const int n_cells = cfg->ground.radii_bins * cfg->ground.theta_bins;
   if (n_cells < 1 || n_cells > 65535 || cfg->ground.num_features < 1 ||
       cfg->trellis.max_vertices < 1)
+    return SLOAM_ERR_INVALID_ARG;
+  if (cfg->fit_lines &&
+      (!(cfg->line.threshold > 0.0) || cfg->line.max_iterations < 0 ||
+       !(cfg->line.probability > 0.0) || !(cfg->line.probability < 1.0)))
     return SLOAM_ERR_INVALID_ARG;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
   sloam_frontend* f = new (std::nothrow) sloam_frontend();
@@ -143,6 +175,9 @@ int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_f
       {(void**)&o.cell_features, 16 * (size_t)n_cells * nf},
       {(void**)&o.kept_offsets, 4 * (size_t)(n_cells + 1)},
       {(void**)&o.kept_index, 4 * n},
+      {(void**)&o.line_coeff, cfg->fit_lines ? 24 * (size_t)o.cap_trees : 0},
+      {(void**)&o.line_stats, cfg->fit_lines ? 16 * (size_t)o.cap_trees : 0},
+      {(void**)&f->medians, cfg->fit_lines ? 12 * (size_t)o.cap_vertices : 0},
   };
   for (auto& b : bufs) {
     const int st = sloam_dev_alloc(ctx, b.p, b.bytes);
@@ -160,7 +195,9 @@ int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_f
                                     4 * (size_t)o.cap_points,
                                     4 * (size_t)n_cells,
                                     16 * (size_t)n_cells,
-                                    4 * (size_t)n_cells * nf};
+                                    4 * (size_t)n_cells * nf,
+                                    cfg->fit_lines ? 6 * (size_t)o.cap_trees : 0,
+                                    cfg->fit_lines ? 4 * (size_t)o.cap_trees : 0};
     size_t total = 0;
     for (int r = 0; r < kRegions; ++r) {
       f->region[r] = total;
@@ -187,7 +224,7 @@ void sloam_frontend_destroy(sloam_frontend* f) {
   void* ptrs[] = {o.xyzi,          o.label,      o.mask,          o.ground,      o.n_ground,
                   o.tree,          o.tree_offsets, o.vertex_int,  o.vertex_f64,  o.point_offsets,
                   o.points,        o.trellis_counts, o.cell_info, o.cell_model,  o.cell_features,
-                  o.kept_offsets,  o.kept_index};
+                  o.kept_offsets,  o.kept_index, o.line_coeff,  o.line_stats,  f->medians};
   for (void* p : ptrs) sloam_dev_free(p);
   if (f->host) (void)hipHostFree(f->host);
   delete f;
@@ -211,6 +248,10 @@ int sloam_frontend_host_view(const sloam_frontend* f, sloam_frontend_host* out) 
   out->cell_info = reinterpret_cast<const int32_t*>(h + f->region[kRCellInfo]);
   out->cell_model = reinterpret_cast<const double*>(h + f->region[kRCellModel]);
   out->cell_features = reinterpret_cast<const float*>(h + f->region[kRCellFeat]);
+  out->line_coeff =
+      f->cfg.fit_lines ? reinterpret_cast<const float*>(h + f->region[kRLineCoeff]) : nullptr;
+  out->line_stats =
+      f->cfg.fit_lines ? reinterpret_cast<const int32_t*>(h + f->region[kRLineStats]) : nullptr;
   return SLOAM_OK;
 }
 

@@ -92,7 +92,9 @@ __device__ inline int wave_count(const float* __restrict__ P, int n, const LineE
 // stats: {ok, inliers, iterations, skipped}
 __global__ __launch_bounds__(64) void line_ransac_kernel(const float* __restrict__ xyz,
                                                          const int32_t* __restrict__ offsets,
-                                                         int n_sets, LineCfgDev cfg,
+                                                         int n_sets,
+                                                         const int32_t* __restrict__ d_n_sets,
+                                                         LineCfgDev cfg,
                                                          float* __restrict__ coeff_out,
                                                          int32_t* __restrict__ stats_out,
                                                          uint8_t* __restrict__ inl_out) {
@@ -105,6 +107,7 @@ __global__ __launch_bounds__(64) void line_ransac_kernel(const float* __restrict
   __shared__ float stage[3 * kStage];
   const int s = blockIdx.x;
   if (s >= n_sets) return;
+  if (d_n_sets && s >= *d_n_sets) return;
   const int lane = threadIdx.x;
   const int off = offsets[s];
   const int n = offsets[s + 1] - off;
@@ -301,11 +304,9 @@ __global__ __launch_bounds__(64) void line_ransac_kernel(const float* __restrict
 
 }  // namespace
 
-extern "C" {
-
-int sloam_fit_lines(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets, int32_t n_sets,
-                    const sloam_line_fit_cfg* cfg, float* d_coeff, int32_t* d_stats,
-                    uint8_t* d_inliers) {
+int sloam_fit_lines_dev(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets,
+                        int32_t n_sets, const int32_t* d_n_sets, const sloam_line_fit_cfg* cfg,
+                        float* d_coeff, int32_t* d_stats, uint8_t* d_inliers) {
   if (!ctx || !cfg || n_sets < 0) return SLOAM_ERR_INVALID_ARG;
   if (n_sets == 0) return SLOAM_OK;
   if (!d_xyz || !d_offsets || !d_coeff || !d_stats) return SLOAM_ERR_INVALID_ARG;
@@ -350,10 +351,19 @@ int sloam_fit_lines(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets
   {
     SloamKernelScope ks(ctx, "line_ransac");
     hipLaunchKernelGGL(line_ransac_kernel, dim3(n_sets), dim3(64), 0, ctx->stream, d_xyz,
-                       d_offsets, n_sets, dc, d_coeff, d_stats, d_inliers);
+                       d_offsets, n_sets, d_n_sets, dc, d_coeff, d_stats, d_inliers);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
+}
+
+extern "C" {
+
+int sloam_fit_lines(sloam_ctx* ctx, const float* d_xyz, const int32_t* d_offsets, int32_t n_sets,
+                    const sloam_line_fit_cfg* cfg, float* d_coeff, int32_t* d_stats,
+                    uint8_t* d_inliers) {
+  return sloam_fit_lines_dev(ctx, d_xyz, d_offsets, n_sets, nullptr, cfg, d_coeff, d_stats,
+                             d_inliers);
 }
 
 }  // extern "C"

@@ -500,7 +500,7 @@ class Frontend:
     buffers, optionally replayed as one HIP graph."""
 
     def __init__(self, ctx, H, W, use_graph=True, fov_up=22.5, fov_down=-22.5, destagger=False,
-                 trellis=None, ground=None, host_results=False):
+                 trellis=None, ground=None, host_results=False, fit_lines=False):
         self.ctx = ctx
         self.lib = ctx.lib
         self.H, self.W, self.N = H, W, H * W
@@ -512,7 +512,12 @@ class Frontend:
         self.cfg = FrontendCfg(proj=ProjCfg(height=H, width=W, fov_up_deg=fov_up,
                                             fov_down_deg=fov_down, destagger=int(destagger)),
                                trellis=TrellisCfg(**tc), ground=GroundCfg(**gc),
-                               use_graph=int(use_graph), host_results=int(host_results))
+                               use_graph=int(use_graph), host_results=int(host_results),
+                               line=LineFitCfg(threshold=0.25, probability=0.99,
+                                               max_iterations=50, optimize=1, seed=12345,
+                                               reserved=0),
+                               fit_lines=int(fit_lines), reserved=0)
+        self.fit_lines = bool(fit_lines)
         h = C.c_void_p()
         check(self.lib.sloam_frontend_create(ctx.handle, C.byref(self.cfg), C.byref(h)),
               ctx.handle)
@@ -573,6 +578,9 @@ class Frontend:
             "cell_model": self._get(o.cell_model, (nc, 8), np.float64),
             "cell_features": self._get(o.cell_features, (nc, self.nf, 4), np.float32),
         }
+        if self.fit_lines:
+            r["line_coeff"] = self._get(o.line_coeff, (nt, 6), np.float32)
+            r["line_stats"] = self._get(o.line_stats, (nt, 4), np.int32)
         self.ctx.sync()
         self._host = []
         return r
@@ -604,4 +612,7 @@ class Frontend:
             "cell_info": arr(hv.cell_info, (nc, 4), C.c_int32, np.int32),
             "cell_model": arr(hv.cell_model, (nc, 8), C.c_double, np.float64),
             "cell_features": arr(hv.cell_features, (nc, self.nf, 4), C.c_float, np.float32),
+            **({"line_coeff": arr(hv.line_coeff, (nt, 6), C.c_float, np.float32),
+                "line_stats": arr(hv.line_stats, (nt, 4), C.c_int32, np.int32)}
+               if self.fit_lines else {}),
         }

@@ -53,6 +53,9 @@ KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& 
   cfg.use_graph = use_graph ? 1 : 0;
   // results land in pinned host memory at the end of every front-end run
   cfg.host_results = 1;
+  // every tree's axis line, fitted inside the front end
+  cfg.line = Cylinder::lineFitCfg();
+  cfg.fit_lines = 1;
   check(sloam_frontend_create(sloam_gpu::context(), &cfg, &fe_), "sloam_frontend_create");
   check(sloam_frontend_buffers(fe_, &buf_), "sloam_frontend_buffers");
   check(sloam_frontend_host_view(fe_, &host_), "sloam_frontend_host_view");
@@ -109,6 +112,11 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
       tree.push_back(std::move(x));
     }
     in.landmarks.push_back(std::move(tree));
+    LineFit fit;
+    fit.ok = hv.line_stats[4 * (size_t)t] != 0;
+    fit.inliers = hv.line_stats[4 * (size_t)t + 1];
+    for (int i = 0; i < 6; ++i) fit.coeff[i] = hv.line_coeff[6 * (size_t)t + i];
+    in.treeLineFits.push_back(fit);
   }
   for (int c = 0; c < nc; ++c) {
     PlaneParameters pp;

@@ -395,6 +395,9 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
 
   // trees: nearest plane to vertex [1] of each, then one batched line fit
   // for the trees in range and tall enough (sloam.cpp:416-435)
+  // every set is fitted on its own, so a fit made earlier on the device for
+  // the same landmark is the same fit
+  const bool have_fits = in.treeLineFits.size() == in.landmarks.size();
   std::vector<size_t> best(in.landmarks.size(), 0);
   std::vector<VectorType> sets;
   std::vector<int> set_of(in.landmarks.size(), -1);
@@ -411,20 +414,26 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
     }
     if (Cylinder::withinRange(t, planes[best[k]], fmParams_) && Cylinder::needsLineFit(t)) {
       VectorType medians;
-      for (const auto& v : t) medians.push_back(v.coords);
+      if (!have_fits)
+        for (const auto& v : t) medians.push_back(v.coords);
       set_of[k] = (int)sets.size();
       sets.push_back(std::move(medians));
     }
   }
-  const auto fits = sloam_gpu::fitLines(sets, Cylinder::lineFitCfg());
+  std::vector<sloam_gpu::LineResult> fits;
+  if (!have_fits) fits = sloam_gpu::fitLines(sets, Cylinder::lineFitCfg());
   for (size_t k = 0; k < in.landmarks.size(); ++k) {
     if (in.landmarks[k].size() < 3) continue;
     LineFit fit;
     if (set_of[k] >= 0) {
-      const auto& r = fits[set_of[k]];
-      fit.ok = r.ok;
-      fit.inliers = r.inliers;
-      std::memcpy(fit.coeff, r.coeff, sizeof(fit.coeff));
+      if (have_fits) {
+        fit = in.treeLineFits[k];
+      } else {
+        const auto& r = fits[set_of[k]];
+        fit.ok = r.ok;
+        fit.inliers = r.inliers;
+        std::memcpy(fit.coeff, r.coeff, sizeof(fit.coeff));
+      }
     }
     Cylinder c(in.landmarks[k], planes[best[k]], fmParams_, fit);
     if (c.isValid) landmarks.push_back(c);

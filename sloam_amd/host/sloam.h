@@ -30,6 +30,11 @@ struct SloamInput {
   // radii x theta cell, r-major, isValid as fitted), e.g. by the keyframe
   // front end; computeModels then skips binning groundCloud.
   std::vector<Plane> groundCells;
+  // Optional: one axis line fit per landmark, fitted on the device to the
+  // landmark's vertex medians with Cylinder::lineFitCfg() (e.g. by the
+  // keyframe front end); computeModels then takes the fit of every tree that
+  // needs one from here instead of fitting the lines itself.
+  std::vector<LineFit> treeLineFits;
 };
 
 struct SloamOutput {

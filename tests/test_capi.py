@@ -44,3 +44,49 @@ def test_invalid_args_rejected_without_device():
     assert lib.sloam_ctx_create(0, None) == -1
     assert lib.sloam_sync(None) == -1
     assert lib.sloam_cyl_plan_exec(None, None, None, None, None, None, None, None) == -1
+
+
+# C struct name -> ctypes mirror in sloam_amd/_lib.py
+STRUCTS = {
+    "sloam_cyl_fit_cfg": "CylFitCfg",
+    "sloam_pose_cfg": "PoseCfg",
+    "sloam_proj_cfg": "ProjCfg",
+    "sloam_ground_cfg": "GroundCfg",
+    "sloam_plane_fit_cfg": "PlaneFitCfg",
+    "sloam_line_fit_cfg": "LineFitCfg",
+    "sloam_trellis_cfg": "TrellisCfg",
+    "sloam_frontend_cfg": "FrontendCfg",
+    "sloam_frontend_out": "FrontendOut",
+    "sloam_frontend_host": "FrontendHost",
+    "sloam_pn_train_cfg": "PnTrainCfg",
+}
+
+
+def test_ctypes_struct_layouts_match_c(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives
+    the header's struct (compiled with gcc, no device code)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "sloam_gpu.h"', "int main(void) {"]
+    for c_name, py_name in STRUCTS.items():
+        st = getattr(_lib, py_name)
+        lines.append(f'  printf("{py_name} size %zu\\n", sizeof({c_name}));')
+        for field, _ in st._fields_:
+            lines.append(f'  printf("{py_name} {field} %zu\\n", offsetof({c_name}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        py_name, field, value = line.split()
+        st = getattr(_lib, py_name)
+        if field == "size":
+            assert ctypes.sizeof(st) == int(value), py_name
+        else:
+            assert getattr(st, field).offset == int(value), (py_name, field)

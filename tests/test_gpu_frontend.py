@@ -89,3 +89,28 @@ def test_host_results_match_device_buffers(ctx):
                 assert v.shape == rd[k].shape, k
                 assert np.array_equal(v, rd[k], equal_nan=True), k
     g.close()
+
+
+def test_device_line_fits_match_fit_lines(ctx):
+    """With fit_lines every tree's axis line is fitted inside the front end:
+    per tree it equals sloam_fit_lines on that tree's vertex medians (as float),
+    bit for bit, on every graph replay, and the host view carries the same."""
+    from sloam_amd.gpu import fit_lines
+    scans = _scans(3)
+    g = Frontend(ctx, H, W, use_graph=True, host_results=True, fit_lines=True)
+    for rep in range(2):
+        for i, sc in enumerate(scans):
+            print(f"[frontend] line fits pass {rep} scan {i}", flush=True)
+            g.upload(sc["xyzi"], sc["label"])
+            g.exec()
+            hv = g.host_view()
+            r = g.download()
+            nt = int(r["trellis_counts"][0])
+            assert nt > 0
+            med = r["vertex_f64"][:, 1:4].astype(np.float32)
+            ref = fit_lines(ctx, med, r["tree_offsets"])
+            assert np.array_equal(r["line_coeff"], ref["coeff"])
+            assert np.array_equal(r["line_stats"], ref["stats"])
+            assert np.array_equal(hv["line_coeff"], r["line_coeff"])
+            assert np.array_equal(hv["line_stats"], r["line_stats"])
+    g.close()
