@@ -27,17 +27,16 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "cyl_score_plain_2000x1024_pmc.md")
 def pmc_traffic(clusters, hyp, points, chunk, tile):
     """HBM bytes per cyl_score launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (scaled by a copy of known size), or None when that
-    summary was taken on another workload or launch shape."""
+    summary holds no line for this workload and launch shape."""
     try:
         text = open(PMC_SUMMARY).read()
     except OSError:
         return None
-    fields = dict(re.findall(r"^traffic_(\w+): (.+)$", text, re.M))
-    if fields.get("workload") != f"clusters={clusters} hypotheses={hyp} points={points}":
+    m = re.search(r"^traffic_workload: (.+)$", text, re.M)
+    if not m or m.group(1).strip() != f"clusters={clusters} hypotheses={hyp} points={points}":
         return None
-    if fields.get("launch_shape") != f"hyp_chunk={chunk} tile={tile}":
-        return None
-    return int(fields["bytes_per_launch"])
+    m = re.search(rf"^traffic_shape: hyp_chunk={chunk} tile={tile} bytes=(\d+)$", text, re.M)
+    return int(m.group(1)) if m else None
 
 
 def parse():
@@ -49,6 +48,8 @@ def parse():
     ap.add_argument("--hyp", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--cpu-pool", type=int, default=16,
+                    help="threads of the second, multi-threaded CPU measurement (0: skip)")
     ap.add_argument("--hyp-chunk", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
@@ -381,26 +382,40 @@ def main():
         keyframes = bench_keyframes(args)
 
     cpu = None
+    cpu_threaded = None
     if not args.no_cpu:
         import oracle
-        # bounded sample: the first 600 clusters (~30 % of the points), one thread
-        ns = min(600, args.clusters)
-        sub_off = d["offsets"][:ns + 1]
-        sub_xyz = d["xyz"][:sub_off[-1]]
+        # bounded sample: the whole batch, fitted three times (median), one
+        # thread (the reference is single-threaded); then the same with
+        # --cpu-pool threads over clusters
         oracle.load()
-        c0 = time.perf_counter()
-        oracle.cyl_fit_batch(sub_xyz, sub_off, args.hyp, thr=thr, max_radius=max_radius,
-                             seed=seed, counts=False, mask=True, threads=args.cpu_threads)
-        c1 = time.perf_counter()
+
+        def cpu_fit(threads, reps):
+            times = []
+            for _ in range(reps):
+                c0 = time.perf_counter()
+                oracle.cyl_fit_batch(d["xyz"], d["offsets"], args.hyp, thr=thr,
+                                     max_radius=max_radius, seed=seed, counts=False, mask=True,
+                                     threads=threads)
+                times.append(time.perf_counter() - c0)
+            return float(np.median(times)), float(np.sum(times))
+
+        med, tot = cpu_fit(args.cpu_threads, 3)
         cpu = {
-            "value": ns / (c1 - c0),
+            "value": args.clusters / med,
             "unit": "cylinder fits/s",
             "cores": args.cpu_threads,
             "kind": "port",
-            "sample": f"first {ns} of the {args.clusters} clusters ({int(sub_off[-1])} points), "
-                      f"{args.hyp} hypotheses, same seed, oracle C++ -O3, "
-                      f"{args.cpu_threads} thread(s); {c1 - c0:.2f} s",
+            "sample": f"all {args.clusters} clusters ({n_points} points), {args.hyp} "
+                      f"hypotheses, same seed, oracle C++ -O3, {args.cpu_threads} thread(s); "
+                      f"median of 3 runs, {tot:.1f} s in total",
         }
+        if args.cpu_pool > 1:
+            med_t, tot_t = cpu_fit(args.cpu_pool, 3)
+            cpu_threaded = {"value": args.clusters / med_t, "unit": "cylinder fits/s",
+                            "cores": args.cpu_pool,
+                            "sample": f"the same, {args.cpu_pool} threads over clusters; "
+                                      f"median of 3 runs, {tot_t:.1f} s in total"}
 
     line = {
         "metric": METRIC,
@@ -444,6 +459,7 @@ def main():
         "radius_err_vs_truth_cm": {"median": float(np.median(rerr) * 100),
                                    "p90": float(np.percentile(rerr, 90) * 100)},
         "cpu_baseline": cpu,
+        "cpu_multithreaded": cpu_threaded,
         "keyframe_latency_ms": None if keyframes is None else
         {"p50": keyframes["latency_ms"]["p50"], "p99": keyframes["latency_ms"]["p99"]},
         "pose": pose,
