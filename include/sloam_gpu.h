@@ -134,6 +134,23 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* plan, const float* d_xyz, const int32
                            const float* d_proposal, const int32_t* d_active, int32_t* d_counts,
                            int32_t* d_samples, uint32_t* d_mask, int32_t* d_best,
                            double* d_model);
+/* Learned fit in one pass over the batch (no counterpart in the reference):
+ * one axis seed per cluster; the `verify` plan's hypotheses (hypothesis 0
+ * the proposal of d_proposal, as sloam_cyl_plan_exec_ex) scored for every
+ * cluster; a cluster whose best verifying hypothesis holds fewer than
+ * min_inlier_ratio * n inliers is flagged in d_fallback (count in
+ * d_n_fallback, nullable) and scored with the `plain` plan's hypotheses
+ * (no proposal); one LO refit of every cluster from its best hypothesis of
+ * the set it was scored with.  A flagged cluster's model equals the plain
+ * plan's own fit of it, an unflagged one the verify plan's.  Both plans must
+ * be built on the same offsets with the same thr, max_radius, seed and refit
+ * settings.  d_counts_verify / d_counts_plain (nullable: plan scratch) hold
+ * n_clusters x n_hyp counts of their plan.                                   */
+int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const float* d_xyz,
+                           const int32_t* d_offsets, const float* d_proposal,
+                           float min_inlier_ratio, int32_t* d_counts_verify,
+                           int32_t* d_counts_plain, int32_t* d_fallback, int32_t* d_n_fallback,
+                           int32_t* d_best, double* d_model);
 /* Launch-shape autotuner: times plans with hyp_chunk in {64, 128, 256, 512}
  * and tile_points in {1024, 2048, 4096} on this batch (d_xyz / d_offsets on
  * the device, h_offsets on the host) and returns cfg with the fastest pair in

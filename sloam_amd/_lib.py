@@ -191,6 +191,7 @@ PROTOTYPES = {
     "sloam_frontend_exec": (C.c_int, [_P]),
     "sloam_frontend_host_view": (C.c_int, [_P, C.POINTER(FrontendHost)]),
     "sloam_cyl_plan_exec_ex": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sloam_cyl_learned_exec": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, _P, _P, _P, _P, _P, _P]),
     "sloam_cyl_fallback": (C.c_int, [_P, _P, _I32, _P, C.c_float, _P, _P]),
     "sloam_cyl_autotune": (C.c_int, [_P, _P, _I32, C.POINTER(CylFitCfg), _P, _P,
                                      C.POINTER(CylFitCfg)]),

@@ -39,7 +39,17 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kHB = 8;  // hypotheses held in registers per pass over a tile
 constexpr int kMaxLds = 65536;  // LDS bytes per workgroup this kernel asks for
-constexpr int kSmallRefit = 768;  // clusters up to this size are refitted by one wave
+// Refit tiers: a cluster of n points is refitted by one workgroup of
+// kTierBlock[t] threads, t the first tier with n <= kTierMax[t].  The tiers run
+// concurrently (the context stream and its auxiliary streams).  A workgroup
+// stays on one CU, so a cluster's time is set by its points per SIMD whatever
+// the workgroup size; measured on the 2000-cluster batch, 64 / 256 threads
+// split at 768 points beat three tiers with 512-thread workgroups and beat
+// staging the points in LDS (which lowers occupancy).
+constexpr int kTiers = 2;
+constexpr int kTierMax[kTiers] = {768, 1 << 30};
+constexpr int kTierBlock[kTiers] = {64, 256};
+static_assert(kTiers >= 1 && kTiers <= 3, "one stream per tier: the context stream and two auxiliary ones");
 
 struct CylFrame {
   float m[3];
@@ -443,6 +453,8 @@ __device__ inline bool cholesky_solve(const double (&A)[N][N], const double (&b)
 }
 
 // Sums K doubles over a block of BLOCK threads; every thread gets the totals.
+// lds holds (BLOCK / 64 + 1) * K doubles: the wave sums, then the totals,
+// added in wave order by K threads.
 template <int BLOCK, int K>
 __device__ inline void bsum(double (&v)[K], double* lds) {
 #pragma unroll
@@ -457,18 +469,21 @@ __device__ inline void bsum(double (&v)[K], double* lds) {
     for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
   }
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
     double t = lds[k];
 #pragma unroll
     for (int w = 1; w < W; ++w) t += lds[w * K + k];
-    v[k] = t;
+    lds[W * K + k] = t;
   }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = lds[W * K + k];
 }
 
 // ---------------------------------------------------------------------------
 // refit: best hypothesis, mask, FP64 LO-RANSAC cylinder refit.
-// One workgroup per cluster (BLOCK = 64 for small clusters, 256 for large).
+// One workgroup per cluster, its size by the cluster's size (refit tiers).
 // Rounds of Gauss-Newton steps on the 5-parameter geometric cylinder residual
 //   d_i = ||(p_i - c) - ((p_i - c).a) a|| - r
 // over the round's inliers: the FP32 inliers of the best hypothesis in round
@@ -483,9 +498,11 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
     const int32_t* __restrict__ list, int n_hyp, float thr, float max_radius, uint32_t k0,
     uint32_t k1, uint32_t tag, int refit_rounds, int refit_iters,
     const float* __restrict__ prop, const int32_t* __restrict__ active,
-    uint32_t* __restrict__ mask, int32_t* __restrict__ best_out, double* __restrict__ model) {
+    const int32_t* __restrict__ alt_flags, const int32_t* __restrict__ alt_counts, int alt_n_hyp,
+    uint32_t alt_tag, uint32_t* __restrict__ mask, int32_t* __restrict__ best_out,
+    double* __restrict__ model) {
   constexpr int W = BLOCK / 64;
-  __shared__ double red[W * 21];  // bsum of up to 21 values per wave
+  __shared__ double red[(W + 1) * 21];  // bsum of up to 21 values
   __shared__ long long redk[W];
   __shared__ HypOut best_h;
   extern __shared__ uint32_t bits[];  // membership of the current round
@@ -498,11 +515,18 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   double* out = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
+  // a flagged cluster takes its hypotheses from the alternate set (its own
+  // counts, hypothesis count and sample stream, no proposal)
+  const bool alt = alt_flags && alt_flags[c];
+  const int32_t* cnts = alt ? alt_counts : counts;
+  const int nh = alt ? alt_n_hyp : n_hyp;
+  const uint32_t tg = alt ? alt_tag : tag;
+  const float* pc = alt ? nullptr : cluster_prop(prop, c);
 
   // 1. best hypothesis: key = count * 2^32 + (2^32 - 1 - index), max wins
   long long key = -1;
-  for (int h = threadIdx.x; h < n_hyp; h += BLOCK) {
-    const long long kk = ((long long)counts[(size_t)c * n_hyp + h] << 32) |
+  for (int h = threadIdx.x; h < nh; h += BLOCK) {
+    const long long kk = ((long long)cnts[(size_t)c * nh + h] << 32) |
                          (long long)(0xFFFFFFFFu - (uint32_t)h);
     key = kk > key ? kk : key;
   }
@@ -518,12 +542,11 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
 #pragma unroll
     for (int w = 1; w < W; ++w) key = redk[w] > key ? redk[w] : key;
   }
-  const int best = (n >= 3 && n_hyp > 0) ? (int)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFll)) : -1;
+  const int best = (n >= 3 && nh > 0) ? (int)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFll)) : -1;
   if (threadIdx.x == 0) {
     if (best >= 0) {
       int32_t idx[3];
-      best_h = hyp_for(P, n, c, best, f, thr, max_radius, k0, k1, tag, idx,
-                       cluster_prop(prop, c));
+      best_h = hyp_for(P, n, c, best, f, thr, max_radius, k0, k1, tg, idx, pc);
     } else {
       best_h.valid = 0;
       best_h.A = best_h.B = 0.f;
@@ -733,8 +756,9 @@ struct sloam_cyl_plan {
   int n_work = 0;
   int4* d_work = nullptr;
   int32_t* d_lists = nullptr;   // small-cluster list then large-cluster list
-  int n_small = 0, n_large = 0;
-  int bits_small = 0, bits_large = 0;  // LDS bytes of the membership bits
+  int tier_n[kTiers] = {};      // clusters per refit tier
+  int tier_off[kTiers] = {};    // first list entry of each tier
+  int tier_bits[kTiers] = {};   // LDS bytes of the membership bits
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
   int n_xcd = 1;
@@ -744,6 +768,83 @@ static int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // active[c] = 1 when cluster c's model keeps fewer than min_ratio of its
 // points as inliers (or is invalid); *n_active counts them
+// Launches the refit tiers of plan p: concurrently, the larger tiers on
+// auxiliary streams forked from and joined back into the context stream.
+// alt_flags (nullable) marks the clusters that take their best hypothesis
+// from alt_counts (alt_n_hyp hypotheses of sample stream alt_tag, no
+// proposal) instead of counts.
+static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_offsets,
+                 const int32_t* counts, const float* d_proposal, const int32_t* d_active,
+                 const int32_t* alt_flags, const int32_t* alt_counts, int alt_n_hyp,
+                 uint32_t alt_tag, uint32_t* d_mask, int32_t* d_best, double* d_model) {
+  sloam_ctx* ctx = p->ctx;
+  const sloam_cyl_fit_cfg& cfg = p->cfg;
+  const uint32_t k0 = (uint32_t)cfg.seed;
+  const uint32_t k1 = (uint32_t)(cfg.seed >> 32);
+  const uint32_t tag = (uint32_t)cfg.stream_tag;
+  int st = SLOAM_OK;
+  for (int t = kTiers - 1; t >= 0; --t) {
+    if (p->tier_n[t] == 0) continue;
+    hipStream_t s = ctx->stream;
+    if (t > 0) {
+      st = sloam_ctx_fork(ctx, t - 1);
+      if (st) return st;
+      s = ctx->aux[t - 1];
+    }
+    const int32_t* list = p->d_lists + p->tier_off[t];
+    const dim3 g(p->tier_n[t]);
+    const size_t lds = p->tier_bits[t];
+#define SLOAM_REFIT_ARGS                                                                      \
+  d_xyz, d_offsets, p->d_frames, counts, list, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, \
+      cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, alt_flags, alt_counts, alt_n_hyp, \
+      alt_tag, d_mask, d_best, d_model
+    switch (kTierBlock[t]) {
+      case 64:
+        hipLaunchKernelGGL(cyl_refit_kernel<64>, g, dim3(64), lds, s, SLOAM_REFIT_ARGS);
+        break;
+      case 256:
+        hipLaunchKernelGGL(cyl_refit_kernel<256>, g, dim3(256), lds, s, SLOAM_REFIT_ARGS);
+        break;
+      case 512:
+        hipLaunchKernelGGL(cyl_refit_kernel<512>, g, dim3(512), lds, s, SLOAM_REFIT_ARGS);
+        break;
+      default:
+        hipLaunchKernelGGL(cyl_refit_kernel<1024>, g, dim3(1024), lds, s, SLOAM_REFIT_ARGS);
+        break;
+    }
+#undef SLOAM_REFIT_ARGS
+    SLOAM_HIP(ctx, hipGetLastError());
+  }
+  for (int t = 1; t < kTiers; ++t)
+    if (p->tier_n[t] > 0) {
+      st = sloam_ctx_join(ctx, t - 1);
+      if (st) return st;
+    }
+  return SLOAM_OK;
+}
+
+// Fallback flags of the learned fit: a cluster (n >= 3) is refitted by the
+// plain method when its best verifying hypothesis holds fewer than
+// min_ratio * n inliers.
+__global__ __launch_bounds__(256) void cyl_decide_kernel(const int32_t* __restrict__ offsets,
+                                                         int n_clusters,
+                                                         const int32_t* __restrict__ counts,
+                                                         int n_hyp, float min_ratio,
+                                                         int32_t* __restrict__ flags,
+                                                         int32_t* __restrict__ n_flagged) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  bool a = false;
+  if (c < n_clusters) {
+    const int n = offsets[c + 1] - offsets[c];
+    int best = 0;
+    for (int h = 0; h < n_hyp; ++h) best = max(best, counts[(size_t)c * n_hyp + h]);
+    a = n >= 3 && (double)best < (double)min_ratio * (double)n;
+    flags[c] = a ? 1 : 0;
+  }
+  const uint64_t b = __builtin_amdgcn_ballot_w64(a);
+  if ((threadIdx.x & 63) == 0 && b && n_flagged) atomicAdd(n_flagged, __builtin_popcountll(b));
+}
+
 __global__ __launch_bounds__(256) void cyl_fallback_kernel(const int32_t* __restrict__ offsets,
                                                            int n_clusters,
                                                            const double* __restrict__ model,
@@ -836,26 +937,23 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
       work.push_back(k < bins[x].size() ? bins[x][k] : make_int4(-1, 0, 0, 0));
   p->n_work = (int)work.size();
   p->n_xcd = n_xcd;
-  // refit lists: clusters up to kSmallRefit points get one wave, larger ones
-  // a 256-thread workgroup; both in decreasing size
-  std::vector<int32_t> small, large;
-  int max_small = 0, max_large = 0;
+  // refit lists, one per tier, each in decreasing size
+  std::vector<int32_t> tier_list[kTiers];
+  int tier_maxlen[kTiers] = {};
   for (int c : order) {
     const int len = h_offsets[c + 1] - h_offsets[c];
-    if (len <= kSmallRefit) {
-      small.push_back(c);
-      max_small = std::max(max_small, len);
-    } else {
-      large.push_back(c);
-      max_large = std::max(max_large, len);
-    }
+    int t = 0;
+    while (t + 1 < kTiers && len > kTierMax[t]) ++t;
+    tier_list[t].push_back(c);
+    tier_maxlen[t] = std::max(tier_maxlen[t], len);
   }
-  p->n_small = (int)small.size();
-  p->n_large = (int)large.size();
-  p->bits_small = round_up(std::max(max_small, 1), 64) / 8;
-  p->bits_large = round_up(std::max(max_large, 1), 256) / 8;
-  std::vector<int32_t> lists(small);
-  lists.insert(lists.end(), large.begin(), large.end());
+  std::vector<int32_t> lists;
+  for (int t = 0; t < kTiers; ++t) {
+    p->tier_n[t] = (int)tier_list[t].size();
+    p->tier_off[t] = (int)lists.size();
+    p->tier_bits[t] = round_up(std::max(tier_maxlen[t], 1), kTierBlock[t]) / 8;
+    lists.insert(lists.end(), tier_list[t].begin(), tier_list[t].end());
+  }
   int st = SLOAM_OK;
   if (!lists.empty()) {
     st = sloam_dev_alloc(ctx, (void**)&p->d_lists, sizeof(int32_t) * lists.size());
@@ -904,6 +1002,76 @@ int sloam_cyl_fallback(sloam_ctx* ctx, const int32_t* d_offsets, int32_t n_clust
                        d_n_active);
   }
   SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
+int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const float* d_xyz,
+                           const int32_t* d_offsets, const float* d_proposal,
+                           float min_inlier_ratio, int32_t* d_counts_verify,
+                           int32_t* d_counts_plain, int32_t* d_fallback, int32_t* d_n_fallback,
+                           int32_t* d_best, double* d_model) {
+  if (!verify || !plain || !d_offsets || !d_model || !d_fallback) return SLOAM_ERR_INVALID_ARG;
+  if (verify->ctx != plain->ctx || verify->n_clusters != plain->n_clusters ||
+      verify->n_points != plain->n_points)
+    return SLOAM_ERR_INVALID_ARG;
+  const sloam_cyl_fit_cfg& vc = verify->cfg;
+  const sloam_cyl_fit_cfg& pc = plain->cfg;
+  if (vc.thr != pc.thr || vc.max_radius != pc.max_radius || vc.seed != pc.seed ||
+      vc.refit_rounds != pc.refit_rounds || vc.refit_iters != pc.refit_iters)
+    return SLOAM_ERR_INVALID_ARG;
+  if (verify->n_points > 0 && !d_xyz) return SLOAM_ERR_INVALID_ARG;
+  sloam_ctx* ctx = verify->ctx;
+  if (verify->n_clusters == 0) return SLOAM_OK;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  const uint32_t k0 = (uint32_t)vc.seed;
+  const uint32_t k1 = (uint32_t)(vc.seed >> 32);
+  int32_t* cv = d_counts_verify ? d_counts_verify : verify->d_counts;
+  int32_t* cp = d_counts_plain ? d_counts_plain : plain->d_counts;
+  const int S = verify->n_clusters;
+  {
+    // one axis seed per cluster serves both hypothesis sets
+    SloamKernelScope ks(ctx, "cyl_seed");
+    hipLaunchKernelGGL(cyl_seed_kernel, dim3(S), dim3(kBlock), 0, ctx->stream, d_xyz, d_offsets,
+                       S, nullptr, verify->d_frames);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  if (verify->n_work > 0) {
+    const size_t lds =
+        (size_t)verify->hcap * (sizeof(float4) + sizeof(int)) + 3 * sizeof(float) * verify->tile;
+    SloamKernelScope ks(ctx, "cyl_score");
+    hipLaunchKernelGGL(cyl_score_kernel, dim3(verify->n_work), dim3(kBlock), lds, ctx->stream,
+                       d_xyz, d_offsets, verify->d_frames, verify->d_work, vc.n_hyp, verify->hcap,
+                       verify->tile, vc.thr, vc.max_radius, k0, k1, (uint32_t)vc.stream_tag,
+                       d_proposal, nullptr, cv, nullptr);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  {
+    SloamKernelScope ks(ctx, "cyl_decide");
+    if (d_n_fallback) {
+      const int st = sloam_fill_u32(ctx, (uint32_t*)d_n_fallback, 1, 0u);
+      if (st) return st;
+    }
+    hipLaunchKernelGGL(cyl_decide_kernel, dim3((S + 255) / 256), dim3(256), 0, ctx->stream,
+                       d_offsets, S, cv, vc.n_hyp, min_inlier_ratio, d_fallback, d_n_fallback);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  if (plain->n_work > 0) {
+    // the plain hypotheses of the flagged clusters, in the same seed frames
+    const size_t lds =
+        (size_t)plain->hcap * (sizeof(float4) + sizeof(int)) + 3 * sizeof(float) * plain->tile;
+    SloamKernelScope ks(ctx, "cyl_score");
+    hipLaunchKernelGGL(cyl_score_kernel, dim3(plain->n_work), dim3(kBlock), lds, ctx->stream,
+                       d_xyz, d_offsets, verify->d_frames, plain->d_work, pc.n_hyp, plain->hcap,
+                       plain->tile, pc.thr, pc.max_radius, k0, k1, (uint32_t)pc.stream_tag,
+                       nullptr, d_fallback, cp, nullptr);
+  }
+  SLOAM_HIP(ctx, hipGetLastError());
+  {
+    SloamKernelScope ks(ctx, "cyl_refit");
+    const int st = launch_refit(verify, d_xyz, d_offsets, cv, d_proposal, nullptr, d_fallback, cp,
+                                pc.n_hyp, (uint32_t)pc.stream_tag, nullptr, d_best, d_model);
+    if (st) return st;
+  }
   return SLOAM_OK;
 }
 
@@ -1026,25 +1194,9 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {
-    // the two size tiers run concurrently: the 4-wave tier on an auxiliary
-    // stream forked from and joined back into the context stream
     SloamKernelScope ks(ctx, "cyl_refit");
-    int st = SLOAM_OK;
-    if (p->n_large > 0) {
-      st = sloam_ctx_fork(ctx, 0);
-      if (st) return st;
-      hipLaunchKernelGGL(cyl_refit_kernel<256>, dim3(p->n_large), dim3(256), p->bits_large,
-                         ctx->aux[0], d_xyz, d_offsets, p->d_frames, counts,
-                         p->d_lists + p->n_small, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1,
-                         tag, cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, d_mask,
-                         d_best, d_model);
-    }
-    if (p->n_small > 0)
-      hipLaunchKernelGGL(cyl_refit_kernel<64>, dim3(p->n_small), dim3(64), p->bits_small,
-                         ctx->stream, d_xyz, d_offsets, p->d_frames, counts, p->d_lists,
-                         cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, cfg.refit_rounds,
-                         cfg.refit_iters, d_proposal, d_active, d_mask, d_best, d_model);
-    if (p->n_large > 0) st = sloam_ctx_join(ctx, 0);
+    const int st = launch_refit(p, d_xyz, d_offsets, counts, d_proposal, d_active, nullptr,
+                                nullptr, 0, 0u, d_mask, d_best, d_model);
     if (st) return st;
   }
   SLOAM_HIP(ctx, hipGetLastError());

@@ -127,8 +127,23 @@ class LearnedCylinderFit:
                 "n_fallback": torch.empty((1,), dtype=torch.int32, device=dev)}
 
     def exec(self, xyz, offsets, out):
-        """Enqueues the learned fit (asynchronous); out["n_fallback"] holds the
-        number of clusters refitted by the plain method."""
+        """Enqueues the learned fit (asynchronous, sloam_cyl_learned_exec: one
+        seed, the verifying hypotheses for every cluster, the plain ones for
+        the clusters whose best verifying hypothesis holds fewer than
+        min_inlier_ratio x n inliers, one refit); out["active"] flags those
+        clusters and out["n_fallback"] counts them."""
+        self.net.infer(xyz, offsets, out["proposal"])
+        check(self.ctx.lib.sloam_cyl_learned_exec(
+            self.verify.handle, self.plain.handle, _ptr(xyz), _ptr(offsets), _ptr(out["proposal"]),
+            C.c_float(self.min_inlier_ratio), None, None, _ptr(out["active"]),
+            _ptr(out["n_fallback"]), None, _ptr(out["model"])), self.ctx.handle)
+        self.ctx.keep(xyz, offsets, *out.values())
+        return out
+
+    def exec_two_pass(self, xyz, offsets, out):
+        """The learned fit as two full passes: verify (seed, score, refit) for
+        every cluster, then the plain method again for the clusters whose
+        refitted model keeps fewer than min_inlier_ratio x n inliers."""
         self.net.infer(xyz, offsets, out["proposal"])
         self.verify.exec(xyz, offsets, {"model": out["model"]}, proposal=out["proposal"])
         check(self.ctx.lib.sloam_cyl_fallback(self.ctx.handle, _ptr(offsets), self.n_clusters,

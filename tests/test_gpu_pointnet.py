@@ -89,8 +89,11 @@ def test_training_reduces_loss(ctx):
 
 
 def test_learned_pipeline_fallback_matches_plain(ctx):
-    """With untrained weights most proposals fail verification; every cluster
-    sent to the fallback must end with exactly the plain fit's model."""
+    """sloam_cyl_learned_exec with untrained weights (most proposals fail
+    verification): a cluster is flagged exactly when its best verifying
+    hypothesis holds fewer than half its points; every flagged cluster ends
+    with exactly the plain fit's model, every other one with exactly the
+    verify-only fit's model."""
     d = synth.make_clusters(200, seed=13)
     x = torch.as_tensor(d["xyz"]).cuda()
     o = torch.as_tensor(d["offsets"].astype(np.int32)).cuda()
@@ -101,11 +104,43 @@ def test_learned_pipeline_fallback_matches_plain(ctx):
     plain = CylinderFitPlan(ctx, d["offsets"], n_hyp=256, stream_tag=0)
     pout = plain.alloc_outputs(mask=False, best=False)
     plain.exec(x, o, pout)
+    verify = CylinderFitPlan(ctx, d["offsets"], n_hyp=16, stream_tag=1)
+    vout = verify.alloc_outputs(counts=True, mask=False, best=False)
+    verify.exec(x, o, vout, proposal=out["proposal"])
+    ctx.sync()
+    act = out["active"].cpu().numpy().astype(bool)
+    m, pm, vm = out["model"].cpu().numpy(), pout["model"].cpu().numpy(), vout["model"].cpu().numpy()
+    nf = int(out["n_fallback"].cpu().numpy()[0])
+    n = np.diff(d["offsets"])
+    best = vout["counts"].cpu().numpy().max(axis=1)
+    print(f"[pn] fallback {nf} of 200", flush=True)
+    assert nf == act.sum() and nf > 0
+    assert np.array_equal(act, (n >= 3) & (best < 0.5 * n))
+    assert np.array_equal(m[act], pm[act])
+    assert np.array_equal(m[~act], vm[~act])
+    fit.close()
+    plain.close()
+    verify.close()
+
+
+def test_two_pass_learned_pipeline_fallback_matches_plain(ctx):
+    """The two-pass form (verify + refit for all, then the plain method for
+    the clusters whose refitted model keeps fewer than half the points):
+    every cluster sent to the fallback ends with exactly the plain fit."""
+    d = synth.make_clusters(200, seed=13)
+    x = torch.as_tensor(d["xyz"]).cuda()
+    o = torch.as_tensor(d["offsets"].astype(np.int32)).cuda()
+    net = proposal.ProposalNet(ctx, proposal.init_params(seed=6))
+    fit = proposal.LearnedCylinderFit(ctx, net, d["offsets"], n_hyp=16, plain_hyp=256)
+    out = fit.alloc()
+    fit.exec_two_pass(x, o, out)
+    plain = CylinderFitPlan(ctx, d["offsets"], n_hyp=256, stream_tag=0)
+    pout = plain.alloc_outputs(mask=False, best=False)
+    plain.exec(x, o, pout)
     ctx.sync()
     act = out["active"].cpu().numpy().astype(bool)
     m, pm = out["model"].cpu().numpy(), pout["model"].cpu().numpy()
     nf = int(out["n_fallback"].cpu().numpy()[0])
-    print(f"[pn] fallback {nf} of 200", flush=True)
     assert nf == act.sum() and nf > 0
     assert np.array_equal(m[act], pm[act])
     n = np.diff(d["offsets"])
