@@ -5,14 +5,16 @@
 //               floor(j n / m), shifted by their bounding-box centre, x and y
 //               scaled by 1 / (larger horizontal half extent), z by
 //               1 / (vertical half extent); float4 (w = 1 real, 0 padding)
-//   pn_fwd      one workgroup (8 waves) per cluster, 64 points at a time:
+//   pn_fwd      one workgroup (8 waves) per cluster, 128 points at a time:
 //               layer 1 (3 -> 64) on the VALU, layers 2 (64 -> 128) and
 //               3 (128 -> 256) on v_mfma_f32_16x16x32_bf16 with the BF16
 //               activations in LDS and each wave's weight fragments in VGPRs
-//               (wave w owns output columns 16w.. of layer 2 and 32w.. of
-//               layer 3), masked max-pool with the first maximising point
-//               kept per channel; persistent: a fixed grid loops over the
-//               clusters with the weight fragments loaded once
+//               (wave w owns output channels 16w.. of layer 2, computed
+//               transposed so that a lane's four results are consecutive
+//               channels of one point and leave in one 8-byte store, and
+//               columns 32w.. of layer 3), masked max-pool with the first
+//               maximising point kept per channel; persistent: a fixed grid
+//               loops over the clusters with the weight fragments loaded once
 //   pn_head     256 -> 128 -> 7 on the VALU for 8 clusters per workgroup
 //   pn_bwd      one workgroup per cluster: loss, head gradients, and the
 //               max-pool gradient routed to its argmax points only (at most
@@ -33,7 +35,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kP = SLOAM_PN_POINTS;  // points per cluster
 constexpr int kC1 = 64, kC2 = 128, kC3 = 256, kC4 = 128, kOut = 7;
-constexpr int kChunk = 64;           // points per pass of the forward kernel
+constexpr int kChunk = 128;          // points per pass of the forward kernel
 constexpr int kS1 = kC1 + 8;         // LDS row strides (bf16), 16-byte multiples
 constexpr int kS2 = kC2 + 8;
 constexpr int kFB = 256;             // backward / prep workgroup
@@ -70,6 +72,11 @@ __device__ inline unsigned short bf_bits(float x) {
   return (unsigned short)(u >> 16);
 }
 __device__ inline float bf_round(float x) { return __uint_as_float((unsigned)bf_bits(x) << 16); }
+// the same rounding by the gfx950 conversion instruction (v_cvt_pk_bf16_f32,
+// round to nearest even) for the forward pass's activations
+__device__ inline unsigned short bf_bits_hw(float x) {
+  return __builtin_bit_cast(unsigned short, (__bf16)x);
+}
 __device__ inline float bf_val(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
 // 8 consecutive FP32 weights -> BF16 fragment
@@ -165,6 +172,10 @@ __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ 
 // the BF16 activations of layers 1 and 2 of every point; prop (nullable):
 // proposal records in the world frame.
 // ---------------------------------------------------------------------------
+// kTrain: also the first maximising point of every channel (for the
+// backward pass) and the BF16 activations H1 / H2; inference keeps the
+// maxima only.
+template <bool kTrain>
 __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
     const float* __restrict__ prm, const float4* __restrict__ X, const PnMeta* __restrict__ meta,
     int n_clusters, float* __restrict__ act, unsigned short* __restrict__ H1,
@@ -175,8 +186,9 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lg = lane >> 4;
 
-  // weight fragments of this wave (B operands: lane -> column lr, k = 8 lg + e):
-  // wave w owns layer-2 columns 16w.. and layer-3 columns 32w..
+  // weight fragments of this wave (lane -> output channel lr of its tile,
+  // k = 8 lg + e): layer 2 (A operand) channels 16w.., layer 3 (B operand)
+  // columns 32w..
   bf16x8 w2f[2], w3f[2][4];
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks)
@@ -186,7 +198,10 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
       w3f[ct][ks] = frag_from_f32(prm + kW3 + (32 * wave + 16 * ct + lr) * kC2 + 32 * ks + 8 * lg);
-  const float b2v = bf_round(prm[kB2 + 16 * wave + lr]);
+  // layer-2 biases of the channels 16w + 4lg + e this lane's results hold
+  float b2v[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) b2v[e] = bf_round(prm[kB2 + 16 * wave + 4 * lg + e]);
   float b3v[2];
 #pragma unroll
   for (int ct = 0; ct < 2; ++ct) b3v[ct] = bf_round(prm[kB3 + 32 * wave + 16 * ct + lr]);
@@ -202,50 +217,68 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
   float best[2] = {0.f, 0.f};
   int bidx[2] = {-1, -1};
   const float4* Xc = X + (size_t)c * kP;
-  const int g1 = t & 7;  // layer 1: thread -> point t / 8, channels 8 (t % 8) ..
   for (int base = 0; base < m; base += kChunk) {
+    // layer 1: thread -> point t / 4, channels 16 (t % 4) .. +15, two
+    // 16-byte stores
     {
-      const int i = t >> 3;
+      const int i = t >> 2, g = t & 3;
       const float4 x = Xc[base + i];
+      unsigned hv[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const float4 w = w1s[8 * g1 + q];
-        const float z = ((w.w + w.x * x.x) + w.y * x.y) + w.z * x.z;
-        const unsigned short h = bf_bits(z > 0.f ? z : 0.f);
-        h1s[i * kS1 + 8 * g1 + q] = h;
-        if (H1) H1[((size_t)c * kP + base + i) * kC1 + 8 * g1 + q] = h;
+        unsigned short h2[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const float4 w = w1s[16 * g + 2 * q + r];
+          const float z = ((w.w + w.x * x.x) + w.y * x.y) + w.z * x.z;
+          h2[r] = bf_bits_hw(fmaxf(z, 0.f));
+        }
+        hv[q] = (unsigned)h2[0] | ((unsigned)h2[1] << 16);
+      }
+      const u32x4 lo = {hv[0], hv[1], hv[2], hv[3]}, hi = {hv[4], hv[5], hv[6], hv[7]};
+      u32x4* d = reinterpret_cast<u32x4*>(h1s + i * kS1 + 16 * g);
+      d[0] = lo;
+      d[1] = hi;
+      if (kTrain) {
+        u32x4* o = reinterpret_cast<u32x4*>(H1 + ((size_t)c * kP + base + i) * kC1 + 16 * g);
+        o[0] = lo;
+        o[1] = hi;
       }
     }
     __syncthreads();
-    // layer 2: 64 points x this wave's 16 channels, four row tiles in flight
+    // layer 2, transposed: D[channel 16w + 4lg + e][point 16pt + lr] for the
+    // chunk's eight point tiles
     {
-      f32x4 acc[4];
+      f32x4 acc[kChunk / 16];
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int pt = 0; pt < kChunk / 16; ++pt) acc[pt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-          const bf16x8 a = frag_from_lds(h1s + (16 * rt + lr) * kS1 + 32 * ks + 8 * lg);
-          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w2f[ks], acc[rt], 0, 0, 0);
+        for (int pt = 0; pt < kChunk / 16; ++pt) {
+          const bf16x8 b = frag_from_lds(h1s + (16 * pt + lr) * kS1 + 32 * ks + 8 * lg);
+          acc[pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2f[ks], b, acc[pt], 0, 0, 0);
         }
-      const int j = 16 * wave + lr;
+      const int j = 16 * wave + 4 * lg;
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt)
+      for (int pt = 0; pt < kChunk / 16; ++pt) {
+        const int i = 16 * pt + lr;
+        unsigned short h[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const int i = 16 * rt + 4 * lg + e;
-          const float z = acc[rt][e] + b2v;
-          const unsigned short h = bf_bits(z > 0.f ? z : 0.f);
-          h2s[i * kS2 + j] = h;
-          if (H2) H2[((size_t)c * kP + base + i) * kC2 + j] = h;
+          h[e] = bf_bits_hw(fmaxf(acc[pt][e] + b2v[e], 0.f));
         }
+        const uint2 v = make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16),
+                                   (unsigned)h[2] | ((unsigned)h[3] << 16));
+        *reinterpret_cast<uint2*>(h2s + i * kS2 + j) = v;
+        if (kTrain) *reinterpret_cast<uint2*>(H2 + ((size_t)c * kP + base + i) * kC2 + j) = v;
+      }
     }
     __syncthreads();
-    // layer 3 + max-pool: 64 points x this wave's 32 channels, two row tiles
-    // x two column tiles in flight
+    // layer 3 + max-pool: the chunk's points x this wave's 32 channels, two
+    // row tiles x two column tiles in flight
 #pragma unroll
-    for (int rp = 0; rp < 4; rp += 2) {
+    for (int rp = 0; rp < kChunk / 16; rp += 2) {
       f32x4 acc[2][2];
 #pragma unroll
       for (int r = 0; r < 2; ++r)
@@ -264,19 +297,39 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
             acc[r][ct] =
                 __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], w3f[ct][ks], acc[r][ct], 0, 0, 0);
       }
+      if (kTrain) {
 #pragma unroll
-      for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
+          for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int p = base + 16 * (rp + r) + 4 * lg + e;
-            const float y = acc[r][ct][e] + b3v[ct];
-            if (p < m && y > best[ct]) {
-              best[ct] = y;
-              bidx[ct] = p;
+            for (int e = 0; e < 4; ++e) {
+              const int p = base + 16 * (rp + r) + 4 * lg + e;
+              const float y = acc[r][ct][e] + b3v[ct];
+              if (p < m && y > best[ct]) {
+                best[ct] = y;
+                bidx[ct] = p;
+              }
             }
-          }
+      } else if (base + kChunk <= m) {
+        // maxima of the biased outputs (ReLU: best starts at 0)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) best[ct] = fmaxf(best[ct], acc[r][ct][e] + b3v[ct]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int p = base + 16 * (rp + r) + 4 * lg + e;
+              if (p < m) best[ct] = fmaxf(best[ct], acc[r][ct][e] + b3v[ct]);
+            }
+      }
     }
     __syncthreads();
   }
@@ -288,17 +341,21 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
 #pragma unroll
     for (int o = 16; o < 64; o <<= 1) {
       const float v2 = __shfl_xor(v, o, 64);
-      const int i2 = __shfl_xor(ix, o, 64);
-      if (v2 > v || (v2 == v && i2 >= 0 && (ix < 0 || i2 < ix))) {
-        v = v2;
-        ix = i2;
+      if (kTrain) {
+        const int i2 = __shfl_xor(ix, o, 64);
+        if (v2 > v || (v2 == v && i2 >= 0 && (ix < 0 || i2 < ix))) {
+          v = v2;
+          ix = i2;
+        }
+      } else {
+        v = fmaxf(v, v2);
       }
     }
     if (lg == 0) {
       const int ch = 32 * wave + 16 * ct + lr;
       float* a = act + (size_t)c * kAct;
       a[kAF + ch] = v;
-      a[kAI + ch] = __int_as_float(ix);
+      if (kTrain) a[kAI + ch] = __int_as_float(ix);
     }
   }
   }
@@ -661,7 +718,7 @@ int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
-    hipLaunchKernelGGL(pn_fwd_kernel, dim3(fwd_grid(ctx, B)), dim3(kFW), 0, ctx->stream,
+    hipLaunchKernelGGL(pn_fwd_kernel<true>, dim3(fwd_grid(ctx, B)), dim3(kFW), 0, ctx->stream,
                        d_params, b->X, b->meta, B, b->act, b->H1, b->H2);
     hipLaunchKernelGGL(pn_head_kernel, dim3((B + kHC - 1) / kHC), dim3(kFB), 0, ctx->stream,
                        d_params, b->meta, B, b->act, (float*)nullptr);
@@ -701,7 +758,7 @@ int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
-    hipLaunchKernelGGL(pn_fwd_kernel, dim3(fwd_grid(ctx, n_clusters)), dim3(kFW), 0,
+    hipLaunchKernelGGL(pn_fwd_kernel<false>, dim3(fwd_grid(ctx, n_clusters)), dim3(kFW), 0,
                        ctx->stream, d_params, b.X, b.meta, n_clusters, b.act,
                        (unsigned short*)nullptr, (unsigned short*)nullptr);
   }
