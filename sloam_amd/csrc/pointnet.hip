@@ -182,7 +182,6 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
     unsigned short* __restrict__ H2) {
   __shared__ __align__(16) unsigned short h1s[kChunk * kS1];
   __shared__ __align__(16) unsigned short h2s[kChunk * kS2];
-  __shared__ float4 w1s[kC1];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lg = lane >> 4;
 
@@ -205,44 +204,42 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
   float b3v[2];
 #pragma unroll
   for (int ct = 0; ct < 2; ++ct) b3v[ct] = bf_round(prm[kB3 + 32 * wave + 16 * ct + lr]);
-  // layer-1 weights (w, bias) per channel
-  if (t < kC1)
-    w1s[t] = make_float4(bf_round(prm[kW1 + 3 * t]), bf_round(prm[kW1 + 3 * t + 1]),
-                         bf_round(prm[kW1 + 3 * t + 2]), bf_round(prm[kB1 + t]));
-  __syncthreads();
+  // layer 1 on v_mfma_f32_16x16x4f32, transposed: A[channel i][k] with
+  // k = (x, y, z, 1) -> (w0, w1, w2, bias); lane -> i = lr, k = lg; one
+  // fragment per 16-channel tile
+  float w1f[kC1 / 16];
+#pragma unroll
+  for (int ct = 0; ct < kC1 / 16; ++ct) {
+    const int ch = 16 * ct + lr;
+    w1f[ct] = bf_round(lg < 3 ? prm[kW1 + 3 * ch + lg] : prm[kB1 + ch]);
+  }
 
   // persistent over clusters: the weight fragments stay in registers
   for (int c = blockIdx.x; c < n_clusters; c += gridDim.x) {
   const int m = meta[c].m;
-  float best[2] = {0.f, 0.f};
+  // training: biased outputs, ReLU floor 0; inference: raw products
+  float best[2] = {kTrain ? 0.f : -INFINITY, kTrain ? 0.f : -INFINITY};
   int bidx[2] = {-1, -1};
   const float4* Xc = X + (size_t)c * kP;
   for (int base = 0; base < m; base += kChunk) {
-    // layer 1: thread -> point t / 4, channels 16 (t % 4) .. +15, two
-    // 16-byte stores
+    // layer 1, transposed: wave w takes points 16w.. of the chunk;
+    // D[channel 16ct + 4lg + e][point 16w + lr], B[k][point] = (x, y, z, 1)
     {
-      const int i = t >> 2, g = t & 3;
-      const float4 x = Xc[base + i];
-      unsigned hv[8];
+      const float4 xp = Xc[base + 16 * wave + lr];
+      const float xb = lg == 0 ? xp.x : (lg == 1 ? xp.y : (lg == 2 ? xp.z : 1.0f));
+      const int i = 16 * wave + lr;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        unsigned short h2[2];
+      for (int ct = 0; ct < kC1 / 16; ++ct) {
+        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(w1f[ct], xb,
+                                                             f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        unsigned short h[4];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          const float4 w = w1s[16 * g + 2 * q + r];
-          const float z = ((w.w + w.x * x.x) + w.y * x.y) + w.z * x.z;
-          h2[r] = bf_bits_hw(fmaxf(z, 0.f));
-        }
-        hv[q] = (unsigned)h2[0] | ((unsigned)h2[1] << 16);
-      }
-      const u32x4 lo = {hv[0], hv[1], hv[2], hv[3]}, hi = {hv[4], hv[5], hv[6], hv[7]};
-      u32x4* d = reinterpret_cast<u32x4*>(h1s + i * kS1 + 16 * g);
-      d[0] = lo;
-      d[1] = hi;
-      if (kTrain) {
-        u32x4* o = reinterpret_cast<u32x4*>(H1 + ((size_t)c * kP + base + i) * kC1 + 16 * g);
-        o[0] = lo;
-        o[1] = hi;
+        for (int e = 0; e < 4; ++e) h[e] = bf_bits_hw(fmaxf(d[e], 0.f));
+        const uint2 v = make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16),
+                                   (unsigned)h[2] | ((unsigned)h[3] << 16));
+        const int j = 16 * ct + 4 * lg;
+        *reinterpret_cast<uint2*>(h1s + i * kS1 + j) = v;
+        if (kTrain) *reinterpret_cast<uint2*>(H1 + ((size_t)c * kP + base + i) * kC1 + j) = v;
       }
     }
     __syncthreads();
@@ -312,13 +309,14 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
               }
             }
       } else if (base + kChunk <= m) {
-        // maxima of the biased outputs (ReLU: best starts at 0)
+        // maxima of the products; the bias is added after the maximum (the
+        // rounded sum is monotone in the product)
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
           for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) best[ct] = fmaxf(best[ct], acc[r][ct][e] + b3v[ct]);
+            for (int e = 0; e < 4; ++e) best[ct] = fmaxf(best[ct], acc[r][ct][e]);
       } else {
 #pragma unroll
         for (int r = 0; r < 2; ++r)
@@ -327,7 +325,7 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
               const int p = base + 16 * (rp + r) + 4 * lg + e;
-              if (p < m) best[ct] = fmaxf(best[ct], acc[r][ct][e] + b3v[ct]);
+              if (p < m) best[ct] = fmaxf(best[ct], acc[r][ct][e]);
             }
       }
     }
@@ -354,8 +352,12 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
     if (lg == 0) {
       const int ch = 32 * wave + 16 * ct + lr;
       float* a = act + (size_t)c * kAct;
-      a[kAF + ch] = v;
-      if (kTrain) a[kAI + ch] = __int_as_float(ix);
+      if (kTrain) {
+        a[kAF + ch] = v;
+        a[kAI + ch] = __int_as_float(ix);
+      } else {
+        a[kAF + ch] = fmaxf(v + b3v[ct], 0.f);  // ReLU of the max of the biased outputs
+      }
     }
   }
   }
