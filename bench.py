@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="default launch shape instead of sloam_cyl_autotune's")
     ap.add_argument("--no-pose", action="store_true", help="skip the batched pose leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the eager launches only (no HIP-graph replay)")
     ap.add_argument("--pose-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--learned-clusters", type=int, default=4096)
@@ -359,6 +361,32 @@ def main():
         ms, n = ctx.kernel_time(name)
         kern[name] = ms / max(n, 1)
 
+    # the same batch recorded once into a HIP graph and replayed, one launch
+    # per step; the replays' models must equal the eager run's bit for bit
+    graph_ms = None
+    if not args.no_graph:
+        ref_model = out["model"].clone()
+        ctx.graph_begin()
+        plan.exec(xyz, off, out)
+        graph = ctx.graph_end()
+        graph.launch()
+        ctx.sync()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            graph.launch()
+        ev1.record(stream)
+        ctx.sync()
+        graph_ms = ev0.elapsed_time(ev1) / args.steps
+        if not torch.equal(out["model"], ref_model):
+            raise RuntimeError("graph replay differs from the eager fit")
+        graph.close()
+    eager_fits_per_s = fits_per_s
+    launch = "eager"
+    if graph_ms is not None and graph_ms < ms_per_step:
+        launch = "graph"
+        ms_per_step = graph_ms
+        fits_per_s = args.clusters / (graph_ms / 1e3)
+
     # dominant kernel: cyl_score.  Algorithmic work per (point, hypothesis)
     # pair: 2 FMA + 2 compares = 6 FP32 ops; bytes: points once (12 B) +
     # counts (4 B per hypothesis).
@@ -443,7 +471,10 @@ def main():
             "points": n_points,
             "parallelism": "single GPU",
             "launch_shape": {"hyp_chunk": chunk, "tile_points": tile, "autotuned": tuned},
+            "launch": launch,
         },
+        "eager_fits_per_s": eager_fits_per_s,
+        "graph_ms_per_step": graph_ms,
         "roofline": {
             "bound": "valu",
             "kernel": "cyl_score",

@@ -63,6 +63,23 @@ int sloam_memcpy_async(sloam_ctx* ctx, void* dst, const void* src, size_t bytes)
 int sloam_ctx_kernel_time(sloam_ctx* ctx, const char* kernel, double* ms_total, int64_t* launches);
 int sloam_ctx_reset_timing(sloam_ctx* ctx);
 
+/* ---- HIP graphs of library calls ---------------------------------------
+ * No counterpart in the reference.  Between sloam_graph_begin and
+ * sloam_graph_end, the entry points called on ctx are recorded into a HIP
+ * graph instead of run (with their fork / join over the context's auxiliary
+ * streams); sloam_graph_launch replays the whole sequence as one launch on
+ * the context stream, over the same buffers and arguments.  Run the sequence
+ * once before capturing it (it sizes the context's scratch buffers; a
+ * buffer that would have to grow during capture makes the call fail); entry
+ * points that synchronise, copy from the host or capture graphs themselves
+ * (sloam_frontend_exec, plan creation) do not belong in a capture.  Timing
+ * is off during capture; a replay is timed as "graph".                     */
+typedef struct sloam_graph sloam_graph;
+int sloam_graph_begin(sloam_ctx* ctx);
+int sloam_graph_end(sloam_ctx* ctx, sloam_graph** out);
+int sloam_graph_launch(sloam_graph* graph);
+void sloam_graph_destroy(sloam_graph* graph);
+
 /* ---- batched cylinder RANSAC fitting (point-level, fixed hypotheses) ------
  * For each cluster c (points d_xyz[offsets[c] .. offsets[c+1])):
  *   1. axis seed: largest-eigenvector of the cluster's point covariance

@@ -165,6 +165,10 @@ PROTOTYPES = {
     "sloam_ctx_enable_timing": (C.c_int, [_P, C.c_int]),
     "sloam_ctx_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_I64)]),
     "sloam_ctx_reset_timing": (C.c_int, [_P]),
+    "sloam_graph_begin": (C.c_int, [_P]),
+    "sloam_graph_end": (C.c_int, [_P, C.POINTER(_P)]),
+    "sloam_graph_launch": (C.c_int, [_P]),
+    "sloam_graph_destroy": (None, [_P]),
     "sloam_cyl_plan_create": (C.c_int, [_P, _P, _I32, C.POINTER(CylFitCfg), C.POINTER(_P)]),
     "sloam_cyl_plan_destroy": (None, [_P]),
     "sloam_cyl_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),

@@ -39,6 +39,10 @@ struct sloam_ctx {
   hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t fork_ev = nullptr;
   hipEvent_t join_ev[2] = {nullptr, nullptr};
+  // sloam_graph_begin .. sloam_graph_end: the stream is being captured (no
+  // synchronisation or allocation may happen); timing is off meanwhile
+  bool capturing = false;
+  bool timing_before_capture = false;
 };
 
 // Makes aux stream i wait for the work enqueued so far on ctx->stream.

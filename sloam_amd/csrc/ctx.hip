@@ -59,6 +59,11 @@ int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out) {
   if (slot < 0) return SLOAM_ERR_INVALID_ARG;
   if ((int)ctx->scratch.size() <= slot) ctx->scratch.resize(slot + 1, {nullptr, 0});
   auto& s = ctx->scratch[slot];
+  if (s.second < bytes && ctx->capturing) {
+    ctx->last_error = "scratch buffer too small during graph capture: run the sequence once "
+                      "before capturing it";
+    return SLOAM_ERR_INVALID_ARG;
+  }
   if (s.second < bytes) {
     // the stream may still use the old buffer
     SLOAM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -175,6 +180,73 @@ const char* sloam_ctx_last_error(const sloam_ctx* ctx) {
 }
 
 void* sloam_ctx_stream(sloam_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+struct sloam_graph {
+  sloam_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int sloam_graph_begin(sloam_ctx* ctx) {
+  if (!ctx || ctx->capturing) return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  // the stream's earlier work is not part of the graph
+  SLOAM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->timing_before_capture = ctx->timing;
+  ctx->timing = false;
+  const hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    ctx->timing = ctx->timing_before_capture;
+    return sloam_record_hip_error(ctx, e, "hipStreamBeginCapture");
+  }
+  ctx->capturing = true;
+  return SLOAM_OK;
+}
+
+int sloam_graph_end(sloam_ctx* ctx, sloam_graph** out) {
+  if (!ctx || !out || !ctx->capturing) return SLOAM_ERR_INVALID_ARG;
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->capturing = false;
+  ctx->timing = ctx->timing_before_capture;
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return sloam_record_hip_error(ctx, e, "hipStreamEndCapture");
+  }
+  sloam_graph* r = new (std::nothrow) sloam_graph();
+  if (!r) {
+    (void)hipGraphDestroy(g);
+    return SLOAM_ERR_ALLOC;
+  }
+  r->ctx = ctx;
+  r->graph = g;
+  const hipError_t ei = hipGraphInstantiate(&r->exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    delete r;
+    return sloam_record_hip_error(ctx, ei, "hipGraphInstantiate");
+  }
+  *out = r;
+  return SLOAM_OK;
+}
+
+int sloam_graph_launch(sloam_graph* g) {
+  if (!g) return SLOAM_ERR_INVALID_ARG;
+  sloam_ctx* ctx = g->ctx;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  SloamKernelScope ks(ctx, "graph");
+  SLOAM_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+  return SLOAM_OK;
+}
+
+void sloam_graph_destroy(sloam_graph* g) {
+  if (!g) return;
+  if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
 
 int sloam_memcpy_async(sloam_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx || (bytes && (!dst || !src))) return SLOAM_ERR_INVALID_ARG;

@@ -76,6 +76,44 @@ class Context:
               self.handle)
         return ms.value, n.value
 
+    def graph_begin(self):
+        """Starts recording the library calls made on this context into a HIP
+        graph (sloam_graph_begin); run the sequence once before capturing it."""
+        check(self.lib.sloam_graph_begin(self.handle), self.handle)
+        self._capture_from = len(self._keep)
+
+    def graph_end(self):
+        """Ends the recording; returns the Graph, which holds the tensors the
+        recorded calls use."""
+        h = C.c_void_p()
+        check(self.lib.sloam_graph_end(self.handle, C.byref(h)), self.handle)
+        return Graph(self, h, self._keep[self._capture_from:])
+
+
+class Graph:
+    """A recorded sequence of library calls, replayed as one launch on the
+    context's stream over the same buffers (sloam_graph_*)."""
+
+    def __init__(self, ctx, handle, refs):
+        self.ctx = ctx
+        self.handle = handle
+        self.refs = list(refs)
+
+    def launch(self):
+        check(self.ctx.lib.sloam_graph_launch(self.handle), self.ctx.handle)
+        self.ctx.keep(*self.refs)
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.sloam_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def autotune_cylinder_fit(ctx, xyz, offsets_dev, offsets_host, n_hyp=1024, thr=0.05,
                           max_radius=1.0, refit_rounds=3, refit_iters=2, seed=0x5EED,
