@@ -394,6 +394,30 @@ struct CylModel {
   double a[3], c[3], r;
 };
 
+// As cyl_dist, and 1 / ||w|| from one reciprocal square root refined by two
+// Newton steps (no FP64 divide in the Gauss-Newton loop). A point on the axis
+// (||w|| = 0) keeps cyl_dist's answer: distance -r and a zero reciprocal.
+__device__ inline double cyl_dist_inv(const CylModel& M, const double p[3], double w[3],
+                                      double* h, double* inv) {
+  const double q[3] = {p[0] - M.c[0], p[1] - M.c[1], p[2] - M.c[2]};
+  const double hh = q[0] * M.a[0] + q[1] * M.a[1] + q[2] * M.a[2];
+  w[0] = q[0] - hh * M.a[0];
+  w[1] = q[1] - hh * M.a[1];
+  w[2] = q[2] - hh * M.a[2];
+  *h = hh;
+  const double s2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  if (!(s2 > 0.0)) {
+    *inv = 0.0;
+    return -M.r;
+  }
+  double y = __builtin_amdgcn_rsq(s2);
+  const double hs = 0.5 * s2;
+  y = y * __builtin_fma(-hs * y, y, 1.5);
+  y = y * __builtin_fma(-hs * y, y, 1.5);
+  *inv = y;
+  return s2 * y - M.r;
+}
+
 __device__ inline double cyl_dist(const CylModel& M, const double p[3], double w[3], double* h) {
   const double q[3] = {p[0] - M.c[0], p[1] - M.c[1], p[2] - M.c[2]};
   const double hh = q[0] * M.a[0] + q[1] * M.a[1] + q[2] * M.a[2];
@@ -628,17 +652,17 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
       for (int base = 0; base < n; base += BLOCK) {
         const int i = base + (int)threadIdx.x;
         bool in = false;
-        double dd = 0.0, hh = 0.0, w[3] = {0.0, 0.0, 0.0};
+        double dd = 0.0, hh = 0.0, w[3] = {0.0, 0.0, 0.0}, iw = 0.0;
         if (i < n) {
           const float* q = P + 3 * (size_t)i;
           const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
                                (double)q[2] - (double)f.m[2]};
           if (reselect) {
-            dd = cyl_dist(M, x, w, &hh);
+            dd = cyl_dist_inv(M, x, w, &hh, &iw);
             in = dd < dthr && dd > -dthr;
           } else {
             in = (bits[i >> 5] >> (i & 31)) & 1u;
-            if (in) dd = cyl_dist(M, x, w, &hh);
+            if (in) dd = cyl_dist_inv(M, x, w, &hh, &iw);
           }
         }
         if (reselect) {
@@ -654,7 +678,7 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
           g[20] += 1.0;
           const double rho = dd + M.r;
           if (rho > 0.0) {
-            const double inv = 1.0 / rho;
+            const double inv = iw;
             const double n1 = (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]) * inv;
             const double n2 = (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]) * inv;
             const double J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0};
