@@ -60,6 +60,12 @@ int sloam_ctx_enable_timing(sloam_ctx* ctx, int enable);
  * unified addressing) on the context stream; host memory must stay valid
  * until sloam_sync.                                                         */
 int sloam_memcpy_async(sloam_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* d_xyz[3 i + k] = d_xyzi[4 i + k], k < 3, for n points (device buffers,
+ * d_xyzi 16-byte aligned): the x y z intensity points of the front end
+ * (sloam_frontend_out.points, the reference's PointT) as the packed xyz the
+ * cylinder fit reads.  No counterpart in the reference (its fits read PCL
+ * clouds in place).                                                        */
+int sloam_xyz_from_xyzi(sloam_ctx* ctx, const float* d_xyzi, int64_t n, float* d_xyz);
 int sloam_ctx_kernel_time(sloam_ctx* ctx, const char* kernel, double* ms_total, int64_t* launches);
 int sloam_ctx_reset_timing(sloam_ctx* ctx);
 
@@ -128,6 +134,12 @@ typedef struct sloam_cyl_plan sloam_cyl_plan;
 int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
                           const sloam_cyl_fit_cfg* cfg, sloam_cyl_plan** out);
 void sloam_cyl_plan_destroy(sloam_cyl_plan* plan);
+/* Re-plan for another batch with the same settings (no counterpart in the
+ * reference, whose fits allocate per call): waits for the context stream,
+ * then lays out h_offsets (n_clusters + 1 entries) as sloam_cyl_plan_create
+ * does, reusing the plan's device buffers (they only grow).  Not allowed
+ * during graph capture.  On error the plan holds an empty batch.          */
+int sloam_cyl_plan_reset(sloam_cyl_plan* plan, const int32_t* h_offsets, int32_t n_clusters);
 /* Run a planned batch.  d_offsets must hold the same offsets as h_offsets.
  * Optional outputs may be NULL:
  *   d_counts  [n_clusters * n_hyp] int32 inlier count per hypothesis

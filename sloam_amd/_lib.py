@@ -170,6 +170,7 @@ PROTOTYPES = {
     "sloam_graph_launch": (C.c_int, [_P]),
     "sloam_graph_destroy": (None, [_P]),
     "sloam_cyl_plan_create": (C.c_int, [_P, _P, _I32, C.POINTER(CylFitCfg), C.POINTER(_P)]),
+    "sloam_cyl_plan_reset": (C.c_int, [_P, _P, _I32]),
     "sloam_cyl_plan_destroy": (None, [_P]),
     "sloam_cyl_plan_exec": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "sloam_pose_plan_create": (C.c_int, [_P, _P, _I32, _I32, _I32, C.POINTER(PoseCfg),
@@ -189,6 +190,7 @@ PROTOTYPES = {
     "sloam_trellis": (C.c_int, [_P, _P, _I32, _I32, C.POINTER(TrellisCfg), _P, _I32, _I32, _I32,
                                 _P, _P, _P, _P, _P, _P]),
     "sloam_memcpy_async": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "sloam_xyz_from_xyzi": (C.c_int, [_P, _P, C.c_int64, _P]),
     "sloam_frontend_create": (C.c_int, [_P, C.POINTER(FrontendCfg), C.POINTER(_P)]),
     "sloam_frontend_destroy": (None, [_P]),
     "sloam_frontend_buffers": (C.c_int, [_P, C.POINTER(FrontendOut)]),

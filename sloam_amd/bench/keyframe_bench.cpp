@@ -4,6 +4,15 @@
 // medians and the pose drift against the sequence's true poses.
 //
 //   keyframe_bench <sequence file> [passes] [use_graph 0/1]
+//                  [--tree-model lines|plain|learned] [--weights <npy>]
+//                  [--traj-out <file>] [--log]
+//
+// --tree-model picks how each tree's model is made (KeyframeRunner's
+// TreeModel; default lines, the reference's); --traj-out writes the first
+// pass's pose estimates, K x 7 float64 (t, q x y z w), to compare the
+// trajectories of two tree models; --log prints the first pass's
+// keyframes (trees, matches, pose error before and after the solve) to
+// stderr.
 //
 // Sequence file (little endian): int32 K, H, W; then per keyframe:
 // float64 true pose [7] (t, q x y z w; map <- sensor), float64 odometry
@@ -44,15 +53,44 @@ double pct(std::vector<double> v, double q) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2) {
-    std::fprintf(stderr, "usage: %s <sequence> [passes] [use_graph]\n", argv[0]);
+  std::vector<std::string> pos;
+  std::string model_name = "lines", weights, traj_out;
+  bool log = false;
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    if (a == "--log") {
+      log = true;
+    } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out") && i + 1 < argc) {
+      (a == "--tree-model" ? model_name : a == "--weights" ? weights : traj_out) = argv[++i];
+    } else if (a.rfind("--", 0) == 0) {
+      std::fprintf(stderr, "unknown option %s\n", a.c_str());
+      return 2;
+    } else {
+      pos.push_back(a);
+    }
+  }
+  if (pos.empty()) {
+    std::fprintf(stderr,
+                 "usage: %s <sequence> [passes] [use_graph] [--tree-model lines|plain|learned] "
+                 "[--weights npy] [--traj-out file]\n",
+                 argv[0]);
     return 2;
   }
-  const int passes = argc > 2 ? std::atoi(argv[2]) : 3;
-  const bool use_graph = argc > 3 ? std::atoi(argv[3]) != 0 : true;
-  FILE* f = std::fopen(argv[1], "rb");
+  TreeModel tree_model = TreeModel::MedianLines;
+  if (model_name == "plain") {
+    tree_model = TreeModel::Plain;
+  } else if (model_name == "learned") {
+    tree_model = TreeModel::Learned;
+  } else if (model_name != "lines") {
+    std::fprintf(stderr, "unknown tree model %s\n", model_name.c_str());
+    return 2;
+  }
+  const int passes = pos.size() > 1 ? std::atoi(pos[1].c_str()) : 3;
+  const bool use_graph = pos.size() > 2 ? std::atoi(pos[2].c_str()) != 0 : true;
+  const char* seq_path = pos[0].c_str();
+  FILE* f = std::fopen(seq_path, "rb");
   if (!f) {
-    std::fprintf(stderr, "cannot open %s\n", argv[1]);
+    std::fprintf(stderr, "cannot open %s\n", seq_path);
     return 2;
   }
   int32_t hdr[3];
@@ -79,19 +117,48 @@ int main(int argc, char** argv) {
   double trees = 0, planes = 0;
   int samples = 0, ok_count = 0;
   double final_drift = 0, max_drift = 0, max_rot = 0;
+  std::vector<double> fit, traj;
+  long long fitted = 0, fallback = 0;
   try {
     for (int pass = 0; pass < passes; ++pass) {
       KeyframeRunner kf(H, W, fm, use_graph);
+      kf.setTreeModel(tree_model, weights);
       SE3 estimate = to_se3(seq[0].truth);
       std::vector<Cylinder> map;
       for (int k = 0; k < K; ++k) {
         if (k > 0) estimate = estimate * to_se3(seq[k].odom);
         SloamOutput out;
         KeyframeTiming t;
+        const SE3 before = estimate;
         const bool ok = kf.run(seq[k].xyzi.data(), seq[k].label.data(), estimate, map, out, &t);
+        if (log && pass == 0) {
+          const SE3 truth = to_se3(seq[k].truth);
+          auto err = [&](const SE3& e, double* deg) {
+            const Quat dq = e.unit_quaternion().conjugate() * truth.unit_quaternion();
+            *deg = 2.0 * std::acos(std::min(1.0, std::fabs(dq.w))) * 180.0 / M_PI;
+            return (e.translation() - truth.translation()).norm();
+          };
+          double d0, d1;
+          const double e0 = err(before, &d0);
+          const double e1 = err(ok ? out.T_Map_Curr : before, &d1);
+          int matched = 0;
+          for (int m : out.matches) matched += m >= 0 ? 1 : 0;
+          std::fprintf(stderr,
+                       "keyframe %d: ok %d, trees %d (fitted %d, fallback %d), map %zu, "
+                       "matched %d, planes %d, error before %.4f m %.3f deg, after %.4f m "
+                       "%.3f deg\n",
+                       k, ok ? 1 : 0, t.trees, t.fitted, t.fallback, map.size(), matched,
+                       t.planes, e0, d0, e1, d1);
+        }
         if (ok) estimate = out.T_Map_Curr;
         // the map for the next keyframe: this keyframe's trees (world frame)
         if (!out.tm.empty()) map = out.tm;
+        if (pass == 0) {
+          const Vector3 tr = estimate.translation();
+          const Quat q = estimate.unit_quaternion();
+          const double row[7] = {tr[0], tr[1], tr[2], q.x, q.y, q.z, q.w};
+          traj.insert(traj.end(), row, row + 7);
+        }
         if (pass == 0 && k > 0) {
           const SE3 truth = to_se3(seq[k].truth);
           const double d = (estimate.translation() - truth.translation()).norm();
@@ -116,6 +183,9 @@ int main(int argc, char** argv) {
         down.push_back(t.download_ms);
         est.push_back(t.estimator_ms);
         build.push_back(t.build_ms);
+        fit.push_back(t.fit_ms);
+        fitted += t.fitted;
+        fallback += t.fallback;
         ph_models.push_back(t.phases.models);
         ph_match.push_back(t.phases.match);
         ph_pose.push_back(t.phases.pose);
@@ -130,8 +200,21 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "keyframe_bench: %s\n", e.what());
     return 1;
   }
+  if (!traj_out.empty()) {
+    FILE* o = std::fopen(traj_out.c_str(), "wb");
+    if (!o || std::fwrite(traj.data(), 8, traj.size(), o) != traj.size()) {
+      std::fprintf(stderr, "cannot write %s\n", traj_out.c_str());
+      if (o) std::fclose(o);
+      return 1;
+    }
+    std::fclose(o);
+  }
   std::printf(
-      "{\"keyframes\": %d, \"passes\": %d, \"height\": %d, \"width\": %d, \"graph\": %s, "
+      "{\"tree_model\": \"%s\", \"tree_fit_p50_ms\": %.4f, \"trees_fitted\": %lld, "
+      "\"trees_fallback\": %lld, ",
+      model_name.c_str(), pct(fit, 0.5), fitted, fallback);
+  std::printf(
+      "\"keyframes\": %d, \"passes\": %d, \"height\": %d, \"width\": %d, \"graph\": %s, "
       "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"mean\": %.4f, \"max\": %.4f}, "
       "\"stage_p50_ms\": {\"upload\": %.4f, \"frontend\": %.4f, \"download\": %.4f, "
       "\"estimator\": %.4f}, \"estimator_p50_ms\": {\"build\": %.4f, \"models\": %.4f, "

@@ -95,6 +95,29 @@ int sloam_fill_u32(sloam_ctx* ctx, uint32_t* d, size_t n, uint32_t value) {
   return SLOAM_OK;
 }
 
+// xyz[i] = the first three floats of xyzi[i] (one point per thread)
+__global__ __launch_bounds__(256) void sl_xyz_kernel(const float4* __restrict__ xyzi, int64_t n,
+                                                      float* __restrict__ xyz) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = xyzi[i];
+  xyz[3 * i] = v.x;
+  xyz[3 * i + 1] = v.y;
+  xyz[3 * i + 2] = v.z;
+}
+
+extern "C" int sloam_xyz_from_xyzi(sloam_ctx* ctx, const float* d_xyzi, int64_t n, float* d_xyz) {
+  if (!ctx || n < 0 || (n > 0 && (!d_xyzi || !d_xyz))) return SLOAM_ERR_INVALID_ARG;
+  if (((uintptr_t)d_xyzi & 15) != 0) return SLOAM_ERR_INVALID_ARG;  // read as float4
+  if (n == 0) return SLOAM_OK;
+  if ((n + 255) / 256 > 0x7fffffff) return SLOAM_ERR_TOO_LARGE;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(sl_xyz_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const float4*)d_xyzi, n, d_xyz);
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
 int sloam_ctx_fork(sloam_ctx* ctx, int i) {
   if (!ctx->aux[i]) {
     SLOAM_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));

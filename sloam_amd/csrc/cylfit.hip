@@ -786,6 +786,8 @@ struct sloam_cyl_plan {
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
   int n_xcd = 1;
+  // allocated sizes (elements): buffers only grow across sloam_cyl_plan_reset
+  size_t cap_work = 0, cap_lists = 0, cap_frames = 0, cap_counts = 0;
 };
 
 static int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -887,42 +889,33 @@ __global__ __launch_bounds__(256) void cyl_fallback_kernel(const int32_t* __rest
   if ((threadIdx.x & 63) == 0 && b && n_active) atomicAdd(n_active, __builtin_popcountll(b));
 }
 
-extern "C" {
+// Grows the device buffer *ptr (element size elem) to hold n elements;
+// contents are not kept.
+static int grow(sloam_ctx* ctx, void** ptr, size_t* cap, size_t n, size_t elem) {
+  if (n <= *cap) return SLOAM_OK;
+  sloam_dev_free(*ptr);
+  *ptr = nullptr;
+  *cap = 0;
+  const int st = sloam_dev_alloc(ctx, ptr, n * elem);
+  if (st) return st;
+  *cap = n;
+  return SLOAM_OK;
+}
 
-int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
-                          const sloam_cyl_fit_cfg* cfg, sloam_cyl_plan** out) {
-  if (!ctx || !h_offsets || !cfg || !out || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
-  *out = nullptr;
-  if (cfg->n_hyp < 1 || cfg->n_hyp > (1 << 20) || !(cfg->thr > 0.0f) ||
-      !(cfg->max_radius > 0.0f) || cfg->refit_iters < 0 || cfg->refit_rounds < 0 || cfg->hyp_chunk < 0 ||
-      cfg->tile_points < 0)
-    return SLOAM_ERR_INVALID_ARG;
+// Lays the batch h_offsets (n_clusters + 1 entries) out on plan p, whose cfg,
+// hcap and tile are set: the scoring work list and the refit tier lists,
+// uploaded into buffers that only grow.
+static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_clusters) {
+  sloam_ctx* ctx = p->ctx;
+  const sloam_cyl_fit_cfg& cfg = p->cfg;
   if (h_offsets[0] != 0) return SLOAM_ERR_INVALID_ARG;
   for (int i = 0; i < n_clusters; ++i) {
     const int64_t len = (int64_t)h_offsets[i + 1] - h_offsets[i];
     if (len < 0) return SLOAM_ERR_INVALID_ARG;
     if (len > (1 << 17)) return SLOAM_ERR_TOO_LARGE;  // integer-exact covariance bound
   }
-  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
-  sloam_cyl_plan* p = new (std::nothrow) sloam_cyl_plan();
-  if (!p) return SLOAM_ERR_ALLOC;
-  p->ctx = ctx;
-  p->cfg = *cfg;
   p->n_clusters = n_clusters;
   p->n_points = h_offsets[n_clusters];
-
-  // hypotheses per workgroup: a multiple of kWaves * kHB
-  int chunk = cfg->hyp_chunk > 0 ? cfg->hyp_chunk : 256;
-  chunk = std::min(chunk, round_up(cfg->n_hyp, kWaves * kHB));
-  p->hcap = round_up(chunk, kWaves * kHB);
-  p->tile = cfg->tile_points > 0 ? round_up(cfg->tile_points, 64) : 2048;
-  // keep the workgroup's LDS within 64 KiB: shrink the point tile if needed
-  const int hyp_bytes = p->hcap * (int)(sizeof(float4) + sizeof(int));
-  if (hyp_bytes > kMaxLds - 64 * 12) {
-    delete p;
-    return SLOAM_ERR_TOO_LARGE;
-  }
-  p->tile = std::min(p->tile, ((kMaxLds - hyp_bytes) / 12) / 64 * 64);
 
   // work list: clusters by decreasing size (ties: lower index), each cluster
   // split into hypothesis chunks.  Workgroups go to the XCDs round robin
@@ -935,19 +928,15 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return (h_offsets[a + 1] - h_offsets[a]) > (h_offsets[b + 1] - h_offsets[b]);
   });
-  int n_xcd = 1;
-  if (hipDeviceGetAttribute(&n_xcd, hipDeviceAttributeNumberOfXccs, ctx->device) != hipSuccess ||
-      n_xcd < 1)
-    n_xcd = 1;
-  (void)hipGetLastError();
-  const int chunks = (cfg->n_hyp + p->hcap - 1) / p->hcap;
+  const int n_xcd = p->n_xcd;
+  const int chunks = (cfg.n_hyp + p->hcap - 1) / p->hcap;
   std::vector<std::vector<int4>> bins(n_xcd);
   std::vector<long long> load(n_xcd, 0);
   for (int c : order) {
     const int len = h_offsets[c + 1] - h_offsets[c];
     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    for (int h0 = 0; h0 < cfg->n_hyp; h0 += p->hcap) {
-      const int hc = std::min(p->hcap, cfg->n_hyp - h0);
+    for (int h0 = 0; h0 < cfg.n_hyp; h0 += p->hcap) {
+      const int hc = std::min(p->hcap, cfg.n_hyp - h0);
       bins[x].push_back(make_int4(c, h0, hc, len));
     }
     load[x] += (long long)std::max(len, 1) * chunks;
@@ -960,7 +949,6 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
     for (int x = 0; x < n_xcd; ++x)
       work.push_back(k < bins[x].size() ? bins[x][k] : make_int4(-1, 0, 0, 0));
   p->n_work = (int)work.size();
-  p->n_xcd = n_xcd;
   // refit lists, one per tier, each in decreasing size
   std::vector<int32_t> tier_list[kTiers];
   int tier_maxlen[kTiers] = {};
@@ -978,34 +966,85 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
     p->tier_bits[t] = round_up(std::max(tier_maxlen[t], 1), kTierBlock[t]) / 8;
     lists.insert(lists.end(), tier_list[t].begin(), tier_list[t].end());
   }
-  int st = SLOAM_OK;
+  int st = grow(ctx, (void**)&p->d_lists, &p->cap_lists, lists.size(), sizeof(int32_t));
+  if (!st) st = grow(ctx, (void**)&p->d_work, &p->cap_work, work.size(), sizeof(int4));
+  if (!st) st = grow(ctx, (void**)&p->d_frames, &p->cap_frames, n_clusters, sizeof(CylFrame));
+  if (!st)
+    st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)n_clusters * cfg.n_hyp,
+              sizeof(int32_t));
+  if (st) return st;
   if (!lists.empty()) {
-    st = sloam_dev_alloc(ctx, (void**)&p->d_lists, sizeof(int32_t) * lists.size());
-    if (!st) {
-      hipError_t e = hipMemcpy(p->d_lists, lists.data(), sizeof(int32_t) * lists.size(),
-                               hipMemcpyHostToDevice);
-      if (e != hipSuccess) st = sloam_record_hip_error(ctx, e, "hipMemcpy(lists)");
-    }
+    hipError_t e = hipMemcpy(p->d_lists, lists.data(), sizeof(int32_t) * lists.size(),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipMemcpy(lists)");
   }
-  if (!st && p->n_work > 0) {
-    st = sloam_dev_alloc(ctx, (void**)&p->d_work, sizeof(int4) * work.size());
-    if (!st) {
-      hipError_t e = hipMemcpy(p->d_work, work.data(), sizeof(int4) * work.size(),
-                               hipMemcpyHostToDevice);
-      if (e != hipSuccess) st = sloam_record_hip_error(ctx, e, "hipMemcpy(work)");
-    }
+  if (!work.empty()) {
+    hipError_t e = hipMemcpy(p->d_work, work.data(), sizeof(int4) * work.size(),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipMemcpy(work)");
   }
-  if (!st && n_clusters > 0)
-    st = sloam_dev_alloc(ctx, (void**)&p->d_frames, sizeof(CylFrame) * n_clusters);
-  if (!st && n_clusters > 0)
-    st = sloam_dev_alloc(ctx, (void**)&p->d_counts,
-                         sizeof(int32_t) * (size_t)n_clusters * cfg->n_hyp);
+  return SLOAM_OK;
+}
+
+extern "C" {
+
+int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
+                          const sloam_cyl_fit_cfg* cfg, sloam_cyl_plan** out) {
+  if (!ctx || !h_offsets || !cfg || !out || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (cfg->n_hyp < 1 || cfg->n_hyp > (1 << 20) || !(cfg->thr > 0.0f) ||
+      !(cfg->max_radius > 0.0f) || cfg->refit_iters < 0 || cfg->refit_rounds < 0 || cfg->hyp_chunk < 0 ||
+      cfg->tile_points < 0)
+    return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  sloam_cyl_plan* p = new (std::nothrow) sloam_cyl_plan();
+  if (!p) return SLOAM_ERR_ALLOC;
+  p->ctx = ctx;
+  p->cfg = *cfg;
+
+  // hypotheses per workgroup: a multiple of kWaves * kHB
+  int chunk = cfg->hyp_chunk > 0 ? cfg->hyp_chunk : 256;
+  chunk = std::min(chunk, round_up(cfg->n_hyp, kWaves * kHB));
+  p->hcap = round_up(chunk, kWaves * kHB);
+  p->tile = cfg->tile_points > 0 ? round_up(cfg->tile_points, 64) : 2048;
+  // keep the workgroup's LDS within 64 KiB: shrink the point tile if needed
+  const int hyp_bytes = p->hcap * (int)(sizeof(float4) + sizeof(int));
+  if (hyp_bytes > kMaxLds - 64 * 12) {
+    delete p;
+    return SLOAM_ERR_TOO_LARGE;
+  }
+  p->tile = std::min(p->tile, ((kMaxLds - hyp_bytes) / 12) / 64 * 64);
+  int n_xcd = 1;
+  if (hipDeviceGetAttribute(&n_xcd, hipDeviceAttributeNumberOfXccs, ctx->device) != hipSuccess ||
+      n_xcd < 1)
+    n_xcd = 1;
+  (void)hipGetLastError();
+  p->n_xcd = n_xcd;
+  const int st = plan_layout(p, h_offsets, n_clusters);
   if (st) {
     sloam_cyl_plan_destroy(p);
     return st;
   }
   *out = p;
   return SLOAM_OK;
+}
+
+int sloam_cyl_plan_reset(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_clusters) {
+  if (!p || !h_offsets || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
+  sloam_ctx* ctx = p->ctx;
+  if (ctx->capturing) return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  // earlier runs of the plan read the buffers about to be rewritten
+  SLOAM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int st = plan_layout(p, h_offsets, n_clusters);
+  if (st) {
+    // no half-made layout: the plan holds an empty batch
+    p->n_clusters = 0;
+    p->n_points = 0;
+    p->n_work = 0;
+    for (int t = 0; t < kTiers; ++t) p->tier_n[t] = 0;
+  }
+  return st;
 }
 
 int sloam_cyl_fallback(sloam_ctx* ctx, const int32_t* d_offsets, int32_t n_clusters,

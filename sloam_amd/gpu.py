@@ -164,6 +164,21 @@ class CylinderFitPlan:
         except Exception:
             pass
 
+    def reset(self, offsets):
+        """Lays out another batch on the same plan (sloam_cyl_plan_reset:
+        waits for the context stream; device buffers only grow)."""
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        st = self.lib.sloam_cyl_plan_reset(self.handle, off.ctypes.data_as(C.c_void_p),
+                                           len(off) - 1)
+        if st != 0:
+            self.offsets_host = np.zeros(1, np.int32)
+            self.n_clusters = 0
+            self.n_points = 0
+            check(st, self.ctx.handle)
+        self.offsets_host = off
+        self.n_clusters = len(off) - 1
+        self.n_points = int(off[-1])
+
     def alloc_outputs(self, counts=False, samples=False, mask=True, best=True):
         dev = torch.device("cuda", self.ctx.device)
         out = {"model": torch.empty((self.n_clusters, CYL_MODEL_STRIDE), dtype=torch.float64,

@@ -52,20 +52,26 @@ Cylinder::Cylinder(const std::vector<TreeVertex> vertices, const Plane& gplane,
     fit.inliers = r[0].inliers;
     std::copy(r[0].coeff, r[0].coeff + 6, fit.coeff);
   }
-  build(vertices, gplane, fmParams, &fit);
+  build(vertices, gplane, fmParams, &fit, nullptr);
 }
 
 Cylinder::Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
                    const FeatureModelParams& fmParams, const LineFit& fit) {
-  build(vertices, gplane, fmParams, &fit);
+  build(vertices, gplane, fmParams, &fit, nullptr);
+}
+
+Cylinder::Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
+                   const FeatureModelParams& fmParams, const CylinderFit& fit) {
+  build(vertices, gplane, fmParams, nullptr, &fit);
 }
 
 void Cylinder::build(const std::vector<TreeVertex>& vertices, const Plane& gplane,
-                     const FeatureModelParams& fmParams, const LineFit* fit) {
+                     const FeatureModelParams& fmParams, const LineFit* fit,
+                     const CylinderFit* cfit) {
   isValid = false;
   if (vertices.size() < 3) return;  // the reference indexes vertices [1] and [2]
   if (withinRange(vertices, gplane, fmParams)) {
-    computeModel(vertices, fmParams.defaultTreeRadius, fmParams.featuresPerTree, *fit);
+    computeModel(vertices, fmParams.defaultTreeRadius, fmParams.featuresPerTree, fit, cfit);
     const bool validZ = groundBasedRoot(gplane);
     const bool validNorm = model.root.norm() > 0.01;
     const bool validRadius = model.radius > 0.0;
@@ -105,7 +111,7 @@ bool Cylinder::filter(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta, c
 
 void Cylinder::computeModel(const std::vector<TreeVertex>& landmarkVtxs,
                             const Scalar defaultTreeRadius, const Scalar featuresPerTree,
-                            const LineFit& fit) {
+                            const LineFit* fit, const CylinderFit* cfit) {
   const TreeVertex& firstVtx = landmarkVtxs[2];
   std::vector<float> validRadii;
   for (const auto& vtx : landmarkVtxs) {
@@ -124,11 +130,31 @@ void Cylinder::computeModel(const std::vector<TreeVertex>& landmarkVtxs,
     model.radius = -1;
     return;
   }
-  if (!fit.ok || fit.inliers == 0) {
+  if (cfit) {
+    // axis, root and radius of the cylinder fit; the radius floor of the
+    // medians' model
+    if (!cfit->ok || !(cfit->radius > 0.0)) {
+      model.radius = -1;
+      return;
+    }
+    const double sgn = cfit->ray[2] < 0.0 ? -1.0 : 1.0;  // axis pointing up
+    model.ray = Vector3(sgn * cfit->ray[0], sgn * cfit->ray[1], sgn * cfit->ray[2]);
+    // root: the foot on the fitted axis of the bottom vertex, which the
+    // medians' model takes as its root (groundBasedRoot then keeps it: the
+    // axis points up, away from the ground)
+    const Vector3 c(cfit->root[0], cfit->root[1], cfit->root[2]);
+    model.root = c + (model.root - c).dot(model.ray) * model.ray;
+    model.radius = cfit->radius < defaultTreeRadius ? defaultTreeRadius : cfit->radius;
+    model.vertices = landmarkVtxs;
+    model.lambda = 1.0;
+    features.resize((size_t)featuresPerTree);
+    return;
+  }
+  if (!fit->ok || fit->inliers == 0) {
     model.radius = -1;
     return;
   }
-  model.ray = Vector3(fit.coeff[3], fit.coeff[4], fit.coeff[5]);
+  model.ray = Vector3(fit->coeff[3], fit->coeff[4], fit->coeff[5]);
   if (!validRadii.empty()) {
     std::sort(validRadii.begin(), validRadii.end());
     const int d = (int)(std::find_if(validRadii.begin(), validRadii.end(),

@@ -27,6 +27,17 @@ struct LineFit {
   int inliers = 0;
 };
 
+// Result of the batched GPU cylinder fit of one tree's points
+// (sloam_cyl_plan_exec / sloam_cyl_learned_exec model record): a point on the
+// axis, the unit axis, the radius and the inlier count.
+struct CylinderFit {
+  bool ok = false;
+  double root[3] = {0, 0, 0};
+  double ray[3] = {0, 0, 0};
+  double radius = -1.0;
+  int inliers = 0;
+};
+
 class Cylinder : public SemanticObject<CylinderParameters> {
  public:
   explicit Cylinder(const std::vector<TreeVertex> vertices, const Plane& gplane,
@@ -35,6 +46,11 @@ class Cylinder : public SemanticObject<CylinderParameters> {
   // `fit` is ignored when the tree is out of range or too short.
   Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
            const FeatureModelParams& fmParams, const LineFit& fit);
+  // Construction from a cylinder fit of the tree's points (`fit`): axis,
+  // root and radius from the fit instead of the medians' line, the radius
+  // and validity rules of the line-fit model otherwise.
+  Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
+           const FeatureModelParams& fmParams, const CylinderFit& fit);
   Scalar distance(const CylinderParameters& model) const override;
   Scalar distance(const PointT& point) const override;
   void project(const SE3& tf) override;
@@ -52,10 +68,11 @@ class Cylinder : public SemanticObject<CylinderParameters> {
   static sloam_line_fit_cfg lineFitCfg();
 
  private:
+  // exactly one of fit / cfit is given
   void build(const std::vector<TreeVertex>& vertices, const Plane& gplane,
-             const FeatureModelParams& fmParams, const LineFit* fit);
+             const FeatureModelParams& fmParams, const LineFit* fit, const CylinderFit* cfit);
   bool filter(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta, const Plane& gplane);
   void computeModel(const std::vector<TreeVertex>& landmarkVtxs, const Scalar defaultTreeRadius,
-                    const Scalar featuresPerTree, const LineFit& fit);
+                    const Scalar featuresPerTree, const LineFit* fit, const CylinderFit* cfit);
   bool groundBasedRoot(const Plane& gplane);
 };

@@ -1,7 +1,11 @@
 // One SLOAM keyframe on the GPU (front-end plan + estimator).
 #include "keyframe.h"
 
+#include <hip/hip_runtime.h>
+
 #include <chrono>
+#include <cstdio>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -24,7 +28,172 @@ void check(int st, const char* what) {
   }
 }
 
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// n little-endian float32 values of a version 1 / 2 .npy file holding
+// exactly that 1-D array
+std::vector<float> load_npy_f32(const std::string& path, size_t n) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + path);
+  std::vector<float> out;
+  std::string err;
+  unsigned char pre[10];
+  if (std::fread(pre, 1, 8, f) != 8 || std::memcmp(pre, "\x93NUMPY", 6) != 0) {
+    err = "not an .npy file";
+  } else {
+    const int major = pre[6];
+    uint32_t hlen = 0;
+    if (major == 1) {
+      unsigned char b[2];
+      if (std::fread(b, 1, 2, f) == 2) hlen = b[0] | (b[1] << 8);
+    } else {
+      unsigned char b[4];
+      if (std::fread(b, 1, 4, f) == 4) hlen = b[0] | (b[1] << 8) | (b[2] << 16) | ((uint32_t)b[3] << 24);
+    }
+    std::string hdr(hlen, '\0');
+    if (hlen == 0 || std::fread(&hdr[0], 1, hlen, f) != hlen) {
+      err = "short header";
+    } else if (hdr.find("'<f4'") == std::string::npos ||
+               hdr.find("'fortran_order': False") == std::string::npos ||
+               hdr.find("(" + std::to_string(n) + ",)") == std::string::npos) {
+      err = "expected float32 [" + std::to_string(n) + "]";
+    } else {
+      out.resize(n);
+      if (std::fread(out.data(), 4, n, f) != n) err = "short data";
+    }
+  }
+  std::fclose(f);
+  if (!err.empty()) throw std::runtime_error(path + ": " + err);
+  return out;
+}
+
 }  // namespace
+
+void KeyframeRunner::DevBuf::ensure(size_t n) {
+  if (n <= bytes) return;
+  if (p) hip_check(hipFree(p), "hipFree");
+  p = nullptr;
+  bytes = 0;
+  const size_t sz = n + n / 2;
+  hip_check(hipMalloc(&p, sz), "hipMalloc");
+  bytes = sz;
+}
+
+KeyframeRunner::DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+void KeyframeRunner::setTreeModel(TreeModel mode, const std::string& weights_npy) {
+  if (mode == TreeModel::Learned) {
+    if (weights_npy.empty()) throw std::runtime_error("keyframe: the learned model needs weights");
+    const std::vector<float> w = load_npy_f32(weights_npy, SLOAM_PN_PARAMS);
+    d_params_.ensure(sizeof(float) * w.size());
+    hip_check(hipMemcpy(d_params_.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice),
+              "hipMemcpy(weights)");
+  }
+  mode_ = mode;
+}
+
+void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTiming* timing) {
+  sloam_ctx* ctx = sloam_gpu::context();
+  const auto t0 = Clock::now();
+  const int32_t *to = host_.tree_offsets, *po = host_.point_offsets;
+  // tree t's points are those of its vertices, contiguous in the front end's
+  // point list
+  const int32_t base = po[to[0]];
+  h_off_.resize((size_t)nt + 1);
+  for (int t = 0; t <= nt; ++t) h_off_[t] = po[to[t]] - base;
+  const int64_t np = h_off_[nt];
+  // plans: built on the first keyframe, laid out again on every other
+  sloam_cyl_fit_cfg cfg{};
+  cfg.n_hyp = 1024;
+  cfg.thr = 0.05f;
+  cfg.max_radius = 1.0f;
+  cfg.refit_iters = 2;
+  cfg.refit_rounds = 3;
+  cfg.seed = 0x5EED;
+  cfg.stream_tag = 0;
+  cfg.hyp_chunk = 128;
+  cfg.tile_points = 1024;
+  if (!plain_) {
+    check(sloam_cyl_plan_create(ctx, h_off_.data(), nt, &cfg, &plain_), "sloam_cyl_plan_create");
+  } else {
+    check(sloam_cyl_plan_reset(plain_, h_off_.data(), nt), "sloam_cyl_plan_reset");
+  }
+  const bool learned = mode_ == TreeModel::Learned;
+  if (learned) {
+    sloam_cyl_fit_cfg vc = cfg;
+    vc.n_hyp = 16;
+    vc.stream_tag = 1;
+    vc.hyp_chunk = 16;
+    if (!verify_) {
+      check(sloam_cyl_plan_create(ctx, h_off_.data(), nt, &vc, &verify_), "sloam_cyl_plan_create");
+    } else {
+      check(sloam_cyl_plan_reset(verify_, h_off_.data(), nt), "sloam_cyl_plan_reset");
+    }
+  }
+  d_xyz_.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(np, 1));
+  d_off_.ensure(sizeof(int32_t) * ((size_t)nt + 1));
+  d_model_.ensure(sizeof(double) * SLOAM_CYL_MODEL_STRIDE * (size_t)nt);
+  d_prop_.ensure(sizeof(float) * SLOAM_PN_PROPOSAL_STRIDE * (size_t)nt);
+  d_flags_.ensure(sizeof(int32_t) * ((size_t)nt + 1));
+  if (h_model_cap_ < (size_t)nt) {
+    if (h_model_) hip_check(hipHostFree(h_model_), "hipHostFree");
+    h_model_ = nullptr;
+    h_model_cap_ = 0;
+    const size_t cap = (size_t)nt + nt / 2;
+    hip_check(hipHostMalloc((void**)&h_model_, sizeof(double) * (SLOAM_CYL_MODEL_STRIDE * cap + 1)),
+              "hipHostMalloc");
+    h_model_cap_ = cap;
+  }
+  float* xyz = static_cast<float*>(d_xyz_.p);
+  int32_t* off = static_cast<int32_t*>(d_off_.p);
+  double* model = static_cast<double*>(d_model_.p);
+  int32_t* flags = static_cast<int32_t*>(d_flags_.p);
+  int32_t* n_fallback = flags + nt;
+  check(sloam_memcpy_async(ctx, off, h_off_.data(), sizeof(int32_t) * ((size_t)nt + 1)),
+        "upload tree offsets");
+  check(sloam_xyz_from_xyzi(ctx, buf_.points + 4 * (size_t)base, np, xyz), "sloam_xyz_from_xyzi");
+  if (learned) {
+    float* prop = static_cast<float*>(d_prop_.p);
+    check(sloam_pn_infer(ctx, static_cast<const float*>(d_params_.p), xyz, off, nt, prop),
+          "sloam_pn_infer");
+    check(sloam_cyl_learned_exec(verify_, plain_, xyz, off, prop, 0.5f, nullptr, nullptr, flags,
+                                 n_fallback, nullptr, model),
+          "sloam_cyl_learned_exec");
+  } else {
+    check(sloam_cyl_plan_exec(plain_, xyz, off, nullptr, nullptr, nullptr, nullptr, model),
+          "sloam_cyl_plan_exec");
+  }
+  check(sloam_memcpy_async(ctx, h_model_, model, sizeof(double) * SLOAM_CYL_MODEL_STRIDE * nt),
+        "download models");
+  int32_t fb = 0;
+  if (learned)
+    check(sloam_memcpy_async(ctx, h_model_ + SLOAM_CYL_MODEL_STRIDE * h_model_cap_, n_fallback,
+                             sizeof(int32_t)),
+          "download fallback count");
+  check(sloam_sync(ctx), "sloam_sync");
+  if (learned) std::memcpy(&fb, h_model_ + SLOAM_CYL_MODEL_STRIDE * h_model_cap_, sizeof(int32_t));
+  fits.assign(nt, CylinderFit());
+  for (int t = 0; t < nt; ++t) {
+    const double* m = h_model_ + SLOAM_CYL_MODEL_STRIDE * (size_t)t;
+    CylinderFit& c = fits[t];
+    for (int i = 0; i < 3; ++i) {
+      c.root[i] = m[i];
+      c.ray[i] = m[3 + i];
+    }
+    c.radius = m[6];
+    c.inliers = (int)m[7];
+    c.ok = m[6] > 0.0 && m[7] >= 3.0;
+  }
+  if (timing) {
+    timing->fit_ms = ms_since(t0);
+    timing->fitted = nt;
+    timing->fallback = fb;
+  }
+}
 
 KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& fmParams,
                                bool use_graph, float fov_up_deg, float fov_down_deg)
@@ -62,13 +231,23 @@ KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& 
   sloam_.setFmParams(fmParams);
 }
 
-KeyframeRunner::~KeyframeRunner() { sloam_frontend_destroy(fe_); }
+KeyframeRunner::~KeyframeRunner() {
+  sloam_cyl_plan_destroy(plain_);
+  sloam_cyl_plan_destroy(verify_);
+  if (h_model_) (void)hipHostFree(h_model_);
+  sloam_frontend_destroy(fe_);
+}
 
 bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
                          const std::vector<Cylinder>& mapModels, SloamOutput& out,
                          KeyframeTiming* timing) {
   sloam_ctx* ctx = sloam_gpu::context();
   const size_t N = (size_t)H_ * W_;
+  if (timing) {
+    timing->fit_ms = 0;
+    timing->fitted = 0;
+    timing->fallback = 0;
+  }
   const auto t0 = Clock::now();
   check(sloam_memcpy_async(ctx, buf_.xyzi, xyzi, 16 * N), "upload xyzi");
   check(sloam_memcpy_async(ctx, buf_.label, labels, N), "upload labels");
@@ -131,6 +310,11 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
       }
     in.groundCells.emplace_back(pp, f, valid);
   }
+  if (mode_ != TreeModel::MedianLines && nt > 0) {
+    // the tree models from cylinder fits of the trees' points
+    fitTrees(nt, in.treeCylinderFits, timing);
+    in.treeLineFits.clear();
+  }
   const auto t3b = Clock::now();
   const bool ok = sloam_.RunSloam(in, out);
   const auto t4 = Clock::now();
@@ -139,7 +323,8 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
     timing->frontend_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     timing->download_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
     timing->estimator_ms = std::chrono::duration<double, std::milli>(t4 - t3).count();
-    timing->build_ms = std::chrono::duration<double, std::milli>(t3b - t3).count();
+    timing->build_ms =
+        std::chrono::duration<double, std::milli>(t3b - t3).count() - timing->fit_ms;
     timing->phases = sloam_.lastPhaseTimes();
     timing->total_ms = ms_since(t0);
     timing->trees = (int)out.tm.size();

@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "definitions.h"
@@ -16,8 +17,25 @@
 struct KeyframeTiming {
   double upload_ms = 0, frontend_ms = 0, download_ms = 0, estimator_ms = 0, total_ms = 0;
   double build_ms = 0;      // estimator input assembled from the front end's arrays
+  double fit_ms = 0;        // tree cylinder fits (TreeModel::Plain / Learned)
   SloamPhaseTimes phases;   // RunSloam's phases
   int trees = 0, planes = 0;
+  int fitted = 0;           // trees given a cylinder fit
+  int fallback = 0;         // of those, refitted by the plain method (Learned)
+};
+
+// How each tree's model is made.
+enum class TreeModel {
+  // the reference's: a sample-consensus line through the vertex medians
+  // (fitted in the front end) and the median vertex radius
+  MedianLines = 0,
+  // batched point-level cylinder RANSAC over each tree's points (1024
+  // hypotheses, 5 cm threshold, LO refit): axis, root and radius
+  Plain = 1,
+  // the learned proposal (PointNet) verified by 16 hypotheses, with the plain
+  // fit as the fallback for trees whose best verifying hypothesis keeps
+  // fewer than half their points
+  Learned = 2,
 };
 
 class KeyframeRunner {
@@ -39,11 +57,33 @@ class KeyframeRunner {
 
   sloam::sloam& estimator() { return sloam_; }
 
+  // Tree model of the following keyframes; Learned needs the proposal
+  // network's parameters (float32 .npy of SLOAM_PN_PARAMS values).
+  void setTreeModel(TreeModel mode, const std::string& weights_npy = std::string());
+  TreeModel treeModel() const { return mode_; }
+
  private:
+  // grow-only device buffer
+  struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n);
+    ~DevBuf();
+  };
+  // cylinder fits of the nt trees of the last front-end run into fits
+  void fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTiming* timing);
+
   int H_, W_;
   sloam_frontend* fe_ = nullptr;
   sloam_frontend_out buf_{};
   sloam_frontend_host host_{};
   FeatureModelParams fm_;
   sloam::sloam sloam_;
+  TreeModel mode_ = TreeModel::MedianLines;
+  sloam_cyl_plan* plain_ = nullptr;
+  sloam_cyl_plan* verify_ = nullptr;
+  std::vector<int32_t> h_off_;
+  DevBuf d_params_, d_xyz_, d_off_, d_model_, d_prop_, d_flags_;
+  double* h_model_ = nullptr;   // pinned: models of the last fit + fallback count
+  size_t h_model_cap_ = 0;      // trees
 };

@@ -397,7 +397,8 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
   // for the trees in range and tall enough (sloam.cpp:416-435)
   // every set is fitted on its own, so a fit made earlier on the device for
   // the same landmark is the same fit
-  const bool have_fits = in.treeLineFits.size() == in.landmarks.size();
+  const bool have_cyl = !in.landmarks.empty() && in.treeCylinderFits.size() == in.landmarks.size();
+  const bool have_fits = have_cyl || in.treeLineFits.size() == in.landmarks.size();
   std::vector<size_t> best(in.landmarks.size(), 0);
   std::vector<VectorType> sets;
   std::vector<int> set_of(in.landmarks.size(), -1);
@@ -424,6 +425,14 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
   if (!have_fits) fits = sloam_gpu::fitLines(sets, Cylinder::lineFitCfg());
   for (size_t k = 0; k < in.landmarks.size(); ++k) {
     if (in.landmarks[k].size() < 3) continue;
+    if (have_cyl) {
+      // a tree outside the gates gets no fit (as it gets no line fit)
+      const CylinderFit none;
+      Cylinder c(in.landmarks[k], planes[best[k]], fmParams_,
+                 set_of[k] >= 0 ? in.treeCylinderFits[k] : none);
+      if (c.isValid) landmarks.push_back(c);
+      continue;
+    }
     LineFit fit;
     if (set_of[k] >= 0) {
       if (have_fits) {

@@ -35,6 +35,11 @@ struct SloamInput {
   // keyframe front end); computeModels then takes the fit of every tree that
   // needs one from here instead of fitting the lines itself.
   std::vector<LineFit> treeLineFits;
+  // Optional: one cylinder fit per landmark of the landmark's points (e.g. by
+  // the keyframe runner's batched plain or learned fit); when given, every
+  // tree takes its axis, root and radius from here (Cylinder's CylinderFit
+  // constructor) and treeLineFits is not used.
+  std::vector<CylinderFit> treeCylinderFits;
 };
 
 struct SloamOutput {

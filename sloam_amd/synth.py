@@ -350,12 +350,15 @@ def trajectory_pose(s, sensor_height=1.5):
 
 def scan_from(forest, R, t, seed=0, H=128, W=2048, fov_deg=22.5, clutter_frac=0.4,
               tree_noise=0.02, ground_noise=0.03, range_min=1.0, range_max=40.0,
-              occlusion=0.1):
+              occlusion=0.1, clutter="understory", understory_height=1.5):
     """One organised H x W scan of `forest` from the sensor pose (R, t)
-    (world = R sensor + t), points in the sensor frame.  Understory clutter
-    replaces clutter_frac of the returns; `occlusion` of each trunk's hits (a
-    random vertical band) are lost to foliage.  Same layout and outputs as
-    make_scan."""
+    (world = R sensor + t), points in the sensor frame.  Clutter makes up
+    clutter_frac of the returns: with clutter="understory" (low vegetation)
+    they replace returns whose surface point lies less than understory_height
+    above the ground (ground and trunk bases), at a range in the last half of
+    the ray; with clutter="uniform" they replace returns drawn at random, at a
+    uniform range.  `occlusion` of each trunk's hits (a random vertical band)
+    are lost to foliage.  Same layout and outputs as make_scan."""
     rng = np.random.default_rng(seed)
     fov = math.radians(fov_deg)
     pitch = fov - (np.arange(H) + 0.5) / H * (2 * fov)
@@ -413,8 +416,21 @@ def scan_from(forest, R, t, seed=0, H=128, W=2048, fov_deg=22.5, clutter_frac=0.
     noise = np.where(label == 2, tree_noise, ground_noise)
     hitm = np.isfinite(best) & (best >= range_min)
     rv = best + rng.normal(0, 1, n) * noise
-    cl = hitm & (rng.uniform(0, 1, n) < clutter_frac)
-    rv[cl] = rng.uniform(range_min, np.minimum(best[cl], range_max))
+    u = rng.uniform(0, 1, n)
+    if clutter == "understory":
+        with np.errstate(invalid="ignore"):
+            pw = t[None, :] + np.where(hitm, best, 0.0)[:, None] * dw
+            above = pw[:, 2] - (g0 * pw[:, 0] + g1 * pw[:, 1])
+        low = hitm & (above < understory_height)
+        frac = min(1.0, clutter_frac * hitm.sum() / max(low.sum(), 1))
+        cl = low & (u < frac)
+        lo = np.maximum(range_min, 0.5 * (best[cl] + range_min))
+        rv[cl] = rng.uniform(lo, np.minimum(best[cl], range_max))
+    elif clutter == "uniform":
+        cl = hitm & (u < clutter_frac)
+        rv[cl] = rng.uniform(range_min, np.minimum(best[cl], range_max))
+    else:
+        raise ValueError(f"unknown clutter model {clutter!r}")
     label[cl] = 0
     inst[cl] = -1
     xyz = rv[:, None] * ds

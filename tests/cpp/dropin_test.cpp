@@ -348,6 +348,93 @@ void cylinder_translate_model() {
   EXPECT_EQ(root[0] + 1, cyl[0].model.root[0]);
 }
 
+// Cylinder from a cylinder fit (no device calls): axis pointing up, root the
+// bottom vertex's foot on the fitted axis (within 2 m of the ground, so
+// valid; the reference's ray-plane intersection only moves a root forward
+// along the ray, so an upward axis keeps it), the radius floor and the
+// validity rules of the line-fit model.
+void cylinder_from_fit() {
+  FeatureModelParams fm;  // sloam.yaml defaults
+  PlaneParameters pp;
+  pp.plane = Vector4(0, 0, 1, 0);  // z = 0
+  pp.centroid = Vector3(0, 0, 0);
+  const Plane ground(pp, VectorType(), true);
+  auto tree = [](double x, double y, double z0, double z1) {
+    std::vector<TreeVertex> v;
+    for (int k = 0; k < 10; ++k) {
+      TreeVertex t;
+      t.treeId = 7;
+      t.radius = 0.25;
+      t.isValid = true;
+      t.coords.x = (float)x;
+      t.coords.y = (float)y;
+      t.coords.z = (float)(z0 + (z1 - z0) * k / 9.0);
+      for (int i = 0; i < 5; ++i) t.points.push_back(t.coords);
+      v.push_back(t);
+    }
+    return v;
+  };
+  auto fit = [](double cx, double cy, double cz, double ax, double ay, double az, double r) {
+    CylinderFit f;
+    f.ok = true;
+    f.root[0] = cx;
+    f.root[1] = cy;
+    f.root[2] = cz;
+    const double n = std::sqrt(ax * ax + ay * ay + az * az);
+    f.ray[0] = ax / n;
+    f.ray[1] = ay / n;
+    f.ray[2] = az / n;
+    f.radius = r;
+    f.inliers = 50;
+    return f;
+  };
+  const auto t5 = tree(5.0, 0.0, 0.3, 3.0);  // vertex [1] at z = 0.6
+  {
+    // vertical axis given pointing down
+    Cylinder c(t5, ground, fm, fit(5.1, 0.05, 1.7, 0, 0, -1, 0.25));
+    EXPECT_TRUE(c.isValid);
+    EXPECT_NEAR(1.0, c.model.ray[2], 1e-12);
+    EXPECT_NEAR(5.1, c.model.root[0], 1e-12);
+    EXPECT_NEAR(0.05, c.model.root[1], 1e-12);
+    EXPECT_NEAR((double)t5[1].coords.z, c.model.root[2], 1e-12);
+    EXPECT_EQ(0.25, c.model.radius);
+    EXPECT_EQ((size_t)7, c.id);
+    EXPECT_EQ((size_t)fm.featuresPerTree, c.features.size());
+  }
+  {
+    // tilted axis through (5, 0, 1): the foot of vertex [1] = (5, 0, z1) is
+    // (5, 0, 1) + (z1 - 1) cos(a) (sin a, 0, cos a)
+    const double a = 4.0 * PIDEF / 180.0;
+    Cylinder c(t5, ground, fm, fit(5.0, 0.0, 1.0, std::sin(a), 0, std::cos(a), 0.22));
+    EXPECT_TRUE(c.isValid);
+    const double s = ((double)t5[1].coords.z - 1.0) * std::cos(a);
+    EXPECT_NEAR(5.0 + s * std::sin(a), c.model.root[0], 1e-12);
+    EXPECT_NEAR(1.0 + s * std::cos(a), c.model.root[2], 1e-12);
+  }
+  {
+    // radius below the default: raised to it
+    Cylinder c(t5, ground, fm, fit(5.0, 0.0, 1.0, 0, 0, 1, 0.08));
+    EXPECT_TRUE(c.isValid);
+    EXPECT_EQ(fm.defaultTreeRadius, c.model.radius);
+  }
+  // rejected: too wide, axis too tilted, failed fit, tree too short, out of range
+  EXPECT_TRUE(!Cylinder(t5, ground, fm, fit(5.0, 0.0, 1.0, 0, 0, 1, 0.35)).isValid);
+  {
+    const double a = 15.0 * PIDEF / 180.0;
+    EXPECT_TRUE(!Cylinder(t5, ground, fm, fit(5.0, 0.0, 1.0, std::sin(a), 0, std::cos(a), 0.25))
+                     .isValid);
+  }
+  {
+    CylinderFit bad = fit(5.0, 0.0, 1.0, 0, 0, 1, 0.25);
+    bad.ok = false;
+    EXPECT_TRUE(!Cylinder(t5, ground, fm, bad).isValid);
+  }
+  EXPECT_TRUE(!Cylinder(tree(5.0, 0.0, 0.3, 1.0), ground, fm, fit(5.0, 0.0, 1.0, 0, 0, 1, 0.25))
+                   .isValid);
+  EXPECT_TRUE(!Cylinder(tree(25.0, 0.0, 0.3, 3.0), ground, fm, fit(25.0, 0.0, 1.0, 0, 0, 1, 0.25))
+                   .isValid);
+}
+
 // ---------------------------------------------------------------------------
 // trellis: the reference's tree clouds give its landmark archives
 // (sloam/src/tests/aux/<seq>_tree_<t>.pcd -> <seq>_landmarks_<t>); the
@@ -603,6 +690,7 @@ int main(int argc, char** argv) {
       {"FromTwoVectors", false, test_from_two_vectors},
       {"AngleAxisQuaternion", false, test_angle_axis},
       {"SE3", false, test_se3},
+      {"CylinderTest.FromCylinderFit", false, cylinder_from_fit},
       {"PlaneTest.Initalizes", true, plane_initializes},
       {"PlaneTest.DistanceToFeature", true, plane_distance_to_feature},
       {"PlaneTest.TranslateModel", true, plane_translate_model},

@@ -114,6 +114,77 @@ def test_deterministic_repeat(ctx):
     plan.close()
 
 
+def _exec_plan(ctx, plan, d):
+    dev = torch.device("cuda", 0)
+    xyz = torch.as_tensor(np.ascontiguousarray(d["xyz"], np.float32)).to(dev)
+    off = torch.as_tensor(np.ascontiguousarray(d["offsets"], np.int32)).to(dev)
+    o = plan.alloc_outputs(counts=True, samples=True, mask=True, best=True)
+    torch.cuda.synchronize()
+    plan.exec(xyz, off, o)
+    ctx.sync()
+    res = {k: v.cpu().numpy() for k, v in o.items()}
+    res["mask"] = res["mask"].view(np.uint32)
+    return res
+
+
+def test_plan_reset_matches_new_plans(ctx):
+    """sloam_cyl_plan_reset: a plan laid out again for a larger, then a
+    smaller batch gives bit-for-bit the outputs of a plan created for that
+    batch (and the oracle's); an invalid batch leaves an empty plan that runs
+    as a no-op and can be reset again."""
+    batches = [synth.make_clusters(40, seed=2), synth.make_clusters(220, seed=3),
+               synth.make_clusters(9, seed=4)]
+    n_hyp = 256
+    plan = CylinderFitPlan(ctx, batches[0]["offsets"], n_hyp=n_hyp)
+    for i, d in enumerate(batches):
+        print(f"[cylfit] reset batch {i}: clusters={len(d['offsets']) - 1} "
+              f"points={d['offsets'][-1]}", flush=True)
+        if i > 0:
+            plan.reset(d["offsets"])
+        got = _exec_plan(ctx, plan, d)
+        fresh = CylinderFitPlan(ctx, d["offsets"], n_hyp=n_hyp)
+        want = _exec_plan(ctx, fresh, d)
+        fresh.close()
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (i, k)
+    o = oracle.cyl_fit_batch(batches[2]["xyz"], batches[2]["offsets"], n_hyp, samples=True)
+    _compare(got, o, batches[2]["offsets"][-1])
+    from sloam_amd._lib import SloamError
+    with pytest.raises(SloamError):
+        plan.reset(np.array([3, 10, 20], np.int32))  # offsets must start at 0
+    assert plan.n_clusters == 0
+    plan.reset(batches[0]["offsets"])
+    got = _exec_plan(ctx, plan, batches[0])
+    fresh = CylinderFitPlan(ctx, batches[0]["offsets"], n_hyp=n_hyp)
+    want = _exec_plan(ctx, fresh, batches[0])
+    fresh.close()
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    plan.close()
+
+
+def test_xyz_from_xyzi(ctx):
+    """sloam_xyz_from_xyzi copies the first three floats of every point
+    (bit-exact, NaN payloads included) and nothing past n."""
+    from sloam_amd._lib import check
+    from sloam_amd.gpu import _ptr
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(5)
+    for n in (0, 1, 255, 256, 257, 70001):
+        print(f"[xyz] n={n}", flush=True)
+        a = rng.normal(size=(n, 4)).astype(np.float32)
+        if n > 3:
+            a[3, 1] = np.nan
+        src = torch.as_tensor(a).to(dev)
+        dst = torch.full((n + 8, 3), 7.0, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        check(ctx.lib.sloam_xyz_from_xyzi(ctx.handle, _ptr(src), n, _ptr(dst)), ctx.handle)
+        ctx.sync()
+        got = dst.cpu().numpy()
+        assert np.array_equal(got[:n].view(np.uint32), a[:, :3].view(np.uint32))
+        assert np.all(got[n:] == 7.0)
+
+
 def test_autotune_choice_is_valid_and_cached(ctx):
     """sloam_cyl_autotune returns one of its candidate shapes, the same one
     again from its cache, and that shape reproduces the default results."""

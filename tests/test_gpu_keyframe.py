@@ -1,0 +1,70 @@
+"""Keyframe stream (KeyframeRunner through sloam_amd/keyframe_bench) with the
+three tree models: the reference's median lines, the batched plain cylinder
+fit and the learned-proposal fit with its plain fallback.
+
+Bar: the learned trajectory within 1 cm / 0.1 deg of the plain one at every
+keyframe; the front end's HIP-graph replay gives the same trajectory as its
+eager launches, bit for bit."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from sloam_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "sloam_amd", "keyframe_bench")
+WEIGHTS = os.path.join(ROOT, "sloam_amd", "weights", "cylinder_proposal.npy")
+KEYFRAMES = 6
+
+
+@pytest.fixture(scope="module")
+def sequence(tmp_path_factory):
+    path = str(tmp_path_factory.mktemp("kf") / "sequence.bin")
+    synth.write_keyframe_sequence(path, n_keyframes=KEYFRAMES, H=128)
+    return path
+
+
+def _run(sequence, tmp, model, graph=1):
+    traj = os.path.join(str(tmp), f"traj_{model}_{graph}.bin")
+    cmd = [EXE, sequence, "1", str(graph), "--tree-model", model, "--weights", WEIGHTS,
+           "--traj-out", traj]
+    print("[keyframe]", " ".join(cmd[3:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return res, np.fromfile(traj, dtype=np.float64).reshape(-1, 7)
+
+
+def _diff(a, b):
+    dt = np.linalg.norm(a[:, :3] - b[:, :3], axis=1)
+    dot = np.clip(np.abs(np.sum(a[:, 3:] * b[:, 3:], axis=1)), 0, 1)
+    return dt.max(), np.degrees(2 * np.arccos(dot)).max()
+
+
+def test_tree_models_and_learned_vs_plain(sequence, tmp_path):
+    assert os.path.exists(EXE), "build first"
+    lines, t_lines = _run(sequence, tmp_path, "lines")
+    plain, t_plain = _run(sequence, tmp_path, "plain")
+    learned, t_learned = _run(sequence, tmp_path, "learned")
+    for r, t in ((lines, t_lines), (plain, t_plain), (learned, t_learned)):
+        assert t.shape == (KEYFRAMES, 7)
+        assert np.all(np.isfinite(t))
+        assert r["success_rate"] > 0.5
+    assert lines["trees_fitted"] == 0
+    # every front-end tree of keyframes 1.. is fitted once per keyframe
+    assert plain["trees_fitted"] > 0 and plain["trees_fitted"] == learned["trees_fitted"]
+    assert 0 <= learned["trees_fallback"] <= learned["trees_fitted"]
+    dm, dd = _diff(t_learned, t_plain)
+    print(f"[keyframe] learned vs plain: {dm:.2e} m, {dd:.2e} deg", flush=True)
+    assert dm <= 0.01 and dd <= 0.1
+
+
+def test_graph_replay_matches_eager(sequence, tmp_path):
+    _, t_graph = _run(sequence, tmp_path, "plain", graph=1)
+    _, t_eager = _run(sequence, tmp_path, "plain", graph=0)
+    assert np.array_equal(t_graph, t_eager)
