@@ -262,37 +262,66 @@ def bench_keyframes(args):
     """Keyframe stream: scans of a 300-tree forest (128 x 2048, 40 % understory
     clutter, partial trunk occlusion) every 0.5 m along a weaving trajectory,
     noisy odometry between them; each keyframe runs the GPU front end (one
-    HIP graph: projection + partition, tree instances, ground cells) and the
-    estimator (models, association, two-step pose).  The C++ program
-    sloam_amd/keyframe_bench times each keyframe end to end (host buffers in,
-    pose out)."""
+    HIP graph: projection + partition, tree instances, ground cells), the
+    tree models and the estimator (ground and tree models, association,
+    two-step pose).  The C++ program sloam_amd/keyframe_bench times each
+    keyframe end to end (host buffers in, pose out).  The headline runs the
+    learned tree model (proposal network + 16 verifying hypotheses + plain
+    fallback over each tree's points); the plain cylinder fit and the
+    reference's median-line model run beside it, and the learned trajectory
+    is compared with the plain one."""
     import subprocess
     import tempfile
+
+    import numpy as np
 
     from sloam_amd import synth
 
     exe = os.path.join(ROOT, "sloam_amd", "keyframe_bench")
+    weights = os.path.join(ROOT, "sloam_amd", "weights", "cylinder_proposal.npy")
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} is missing: build first (python -m sloam_amd.build)")
+    runs = (("learned", "learned", 1), ("plain", "plain", 1), ("lines", "lines", 1),
+            ("lines_eager", "lines", 0))
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "sequence.bin")
         t0 = time.perf_counter()
         synth.write_keyframe_sequence(path, n_keyframes=args.keyframes, H=args.kf_height)
         gen_s = time.perf_counter() - t0
-        res = {}
-        for graph in (1, 0):
-            r = subprocess.run([exe, path, str(args.kf_passes), str(graph)], capture_output=True,
-                               text=True, timeout=600)
+        res, traj = {}, {}
+        for name, model, graph in runs:
+            tp = os.path.join(d, f"traj_{name}.bin")
+            print(f"[bench] keyframe stream: {name}", file=sys.stderr, flush=True)
+            r = subprocess.run([exe, path, str(args.kf_passes), str(graph), "--tree-model", model,
+                                "--weights", weights, "--traj-out", tp],
+                               capture_output=True, text=True, timeout=600)
             if r.returncode != 0:
                 raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
-            res["graph" if graph else "eager"] = json.loads(r.stdout.strip().splitlines()[-1])
-    g = res["graph"]
-    g["eager_frontend_p50_ms"] = res["eager"]["stage_p50_ms"]["frontend"]
-    g["eager_latency_p50_ms"] = res["eager"]["latency_ms"]["p50"]
+            res[name] = json.loads(r.stdout.strip().splitlines()[-1])
+            traj[name] = np.fromfile(tp, dtype=np.float64).reshape(-1, 7)
+
+    def traj_diff(a, b):
+        dt = np.linalg.norm(a[:, :3] - b[:, :3], axis=1)
+        dot = np.clip(np.abs(np.sum(a[:, 3:] * b[:, 3:], axis=1)), 0, 1)
+        ang = np.degrees(2 * np.arccos(dot))
+        return {"max_m": float(dt.max()), "final_m": float(dt[-1]), "max_deg": float(ang.max())}
+
+    g = res["learned"]
+    g["drift_vs_plain"] = traj_diff(traj["learned"], traj["plain"])
+    g["tree_models"] = {
+        name: {"latency_ms": res[name]["latency_ms"],
+               "tree_fit_p50_ms": res[name]["tree_fit_p50_ms"],
+               "trees_mean": res[name]["trees_mean"],
+               "drift_vs_truth": res[name]["drift"]}
+        for name in ("learned", "plain", "lines")}
+    g["tree_models"]["lines"]["drift_vs_plain"] = traj_diff(traj["lines"], traj["plain"])
+    g["lines_eager_frontend_p50_ms"] = res["lines_eager"]["stage_p50_ms"]["frontend"]
+    g["lines_eager_latency_p50_ms"] = res["lines_eager"]["latency_ms"]["p50"]
     g["workload"] = (f"{args.keyframes} keyframes of {args.kf_height}x2048 scans, 0.5 m apart, "
-                     f"300-tree forest, 40 % clutter; {args.kf_passes} passes, the first a "
-                     f"warm-up reported apart; sequence generated on the host in {gen_s:.1f} s "
-                     f"(not timed)")
+                     f"300-tree forest, 40 % understory clutter; {args.kf_passes} passes, the "
+                     f"first a warm-up reported apart; learned tree models (plain and the "
+                     f"reference's median lines beside them); sequence generated on the host "
+                     f"in {gen_s:.1f} s (not timed)")
     return g
 
 

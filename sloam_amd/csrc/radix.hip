@@ -31,6 +31,10 @@ constexpr int kFMaxTiles = 512;
 constexpr int kRounds = 16;
 constexpr int kTile = kRB * kRounds;  // 4096 keys per tile
 
+// row stride (tiles, a multiple of 4) of the digit-major count table of the
+// two-launch path: 16-byte aligned rows
+__host__ __device__ inline int count_stride(int n_tiles) { return (n_tiles + 3) & ~3; }
+
 __device__ inline int eff_n(const int32_t* d_n, int cap) {
   if (!d_n) return cap;
   const int m = *d_n;
@@ -72,7 +76,8 @@ __global__ __launch_bounds__(kRB) void rs_hist_rows_kernel(const u64* __restrict
     if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 0xFFull)], 1);
   }
   __syncthreads();
-  cnt[blockIdx.x * kDigits + t] = h[t];
+  // digit-major: row t holds digit t's count of every tile
+  cnt[(size_t)t * count_stride(gridDim.x) + blockIdx.x] = h[t];
 }
 
 __global__ __launch_bounds__(kRB) void rs_pass_kernel(
@@ -86,15 +91,37 @@ __global__ __launch_bounds__(kRB) void rs_pass_kernel(
   const int n = eff_n(d_n, cap);
   const int base = blockIdx.x * kFTile;
   if (base >= n) return;  // uniform over the block
+  // the tile's keys, loaded before the offsets are summed so that both
+  // latencies overlap
+  u64 kr[kFRounds];
+  int32_t vr[kFRounds];
+#pragma unroll
+  for (int r = 0; r < kFRounds; ++r) {
+    const int i = base + r * kRB + t;
+    kr[r] = 0;
+    vr[r] = 0;
+    if (i < n) {
+      kr[r] = kin[i];
+      if (vin) vr[r] = vin[i];
+    }
+  }
   // offsets of this tile: digit totals (exclusive scan over digits) plus the
-  // counts of the tiles before it
+  // counts of the tiles before it, from digit t's row (16-byte loads; the
+  // row's padding past n_tiles is never written and is masked)
   int tot = 0, before = 0;
   const int me = blockIdx.x;
-#pragma unroll 16
-  for (int b = 0; b < n_tiles; ++b) {
-    const int v = cnt[b * kDigits + t];
-    tot += v;
-    before += b < me ? v : 0;
+  const int4* row = reinterpret_cast<const int4*>(cnt + (size_t)t * count_stride(n_tiles));
+#pragma unroll 8
+  for (int q = 0; q < count_stride(n_tiles) / 4; ++q) {
+    const int4 v = row[q];
+    const int b = 4 * q;
+    const int c0 = b < n_tiles ? v.x : 0;
+    const int c1 = b + 1 < n_tiles ? v.y : 0;
+    const int c2 = b + 2 < n_tiles ? v.z : 0;
+    const int c3 = b + 3 < n_tiles ? v.w : 0;
+    tot += c0 + c1 + c2 + c3;
+    before += (b < me ? c0 : 0) + (b + 1 < me ? c1 : 0) + (b + 2 < me ? c2 : 0) +
+              (b + 3 < me ? c3 : 0);
   }
   int s = tot;
 #pragma unroll
@@ -109,20 +136,16 @@ __global__ __launch_bounds__(kRB) void rs_pass_kernel(
   for (int w = 0; w < kRB / 64; ++w) wo += w < wave ? wtot[w] : 0;
   off[t] = wo + s - tot + before;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
   for (int r = 0; r < kFRounds; ++r) {
 #pragma unroll
     for (int w = 0; w < kRB / 64; ++w) wc[w][t] = 0;
     __syncthreads();
     const int i = base + r * kRB + t;
     const bool act = i < n;
-    u64 k = 0;
-    int32_t v = 0;
-    int d = 0;
-    if (act) {
-      k = kin[i];
-      if (vin) v = vin[i];
-      d = (int)((k >> shift) & 0xFFull);
-    }
+    const u64 k = kr[r];
+    const int32_t v = vr[r];
+    const int d = act ? (int)((k >> shift) & 0xFFull) : 0;
     const int rank = wave_rank(act, d, lane, lt, wc[wave]);
     __syncthreads();
     int dest = 0;
@@ -251,7 +274,7 @@ int n_tiles_fused(int n) { return (n + kFTile - 1) / kFTile; }
 size_t sloam_radix_hist_ints(int n) {
   const int tf = n_tiles_fused(n > 0 ? n : 1);
   const int t3 = (n + kTile - 1) / kTile;
-  const size_t a = (size_t)kDigits * (size_t)tf;
+  const size_t a = (size_t)kDigits * (size_t)count_stride(tf);
   const size_t b = (size_t)kDigits * (size_t)(t3 > 0 ? t3 : 1);
   return a > b ? a : b;
 }

@@ -217,16 +217,31 @@ __global__ __launch_bounds__(kTB) void tr_rootid_kernel(const int* __restrict__ 
   }
 }
 
+// label[p] = component id of pixel p (-1: no component); size[l] counts the
+// pixels of component l.  Threads walk the image column by column (a wave
+// holds 64 rows of one column: mostly one trunk) and each wave adds one
+// atomic per distinct label it holds, so a large component does not
+// serialise on its counter.
 __global__ __launch_bounds__(kTB) void tr_label_kernel(const int* __restrict__ root,
-                                                       const int* __restrict__ cid, int N,
+                                                       const int* __restrict__ cid, TrCfgDev cfg,
                                                        int* __restrict__ label,
                                                        int* __restrict__ size) {
-  const int p = blockIdx.x * kTB + threadIdx.x;
-  if (p >= N) return;
-  const int r = root[p];
-  const int l = r < 0 ? -1 : cid[r];
-  label[p] = l;
-  if (l >= 0) atomicAdd(size + l, 1);
+  const int idx = blockIdx.x * kTB + threadIdx.x;
+  int l = -1;
+  if (idx < cfg.N) {
+    const int p = (idx % cfg.H) * cfg.W + idx / cfg.H;
+    const int r = root[p];
+    l = r < 0 ? -1 : cid[r];
+    label[p] = l;
+  }
+  bool pending = l >= 0;
+  for (uint64_t m = __builtin_amdgcn_ballot_w64(pending); m != 0;
+       m = __builtin_amdgcn_ballot_w64(pending)) {
+    const int lv = __shfl(l, (int)__builtin_ctzll(m), 64);
+    const uint64_t same = __builtin_amdgcn_ballot_w64(pending && l == lv);
+    if ((threadIdx.x & 63) == (int)__builtin_ctzll(m)) atomicAdd(size + lv, __builtin_popcountll(same));
+    if (l == lv) pending = false;
+  }
 }
 
 __global__ __launch_bounds__(kTB) void tr_key_kernel(const int* __restrict__ label,
@@ -660,7 +675,7 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
                          counters + kCntComps);
       hipLaunchKernelGGL(tr_rootid_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, root, N,
                          bc, cid);
-      hipLaunchKernelGGL(tr_label_kernel, g, dim3(kTB), 0, ctx->stream, root, cid, N, label,
+      hipLaunchKernelGGL(tr_label_kernel, g, dim3(kTB), 0, ctx->stream, root, cid, dc, label,
                          size);
       hipLaunchKernelGGL(tr_key_kernel, g, dim3(kTB), 0, ctx->stream, label, size, dc, keys);
     }
