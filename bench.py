@@ -281,19 +281,21 @@ def bench_keyframes(args):
     weights = os.path.join(ROOT, "sloam_amd", "weights", "cylinder_proposal.npy")
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} is missing: build first (python -m sloam_amd.build)")
-    runs = (("learned", "learned", 1), ("plain", "plain", 1), ("lines", "lines", 1),
-            ("lines_eager", "lines", 0))
+    # name, tree model, graph, pinned input
+    runs = (("learned", "learned", 1, 1), ("learned_pageable", "learned", 1, 0),
+            ("plain", "plain", 1, 1), ("lines", "lines", 1, 1), ("lines_eager", "lines", 0, 1))
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "sequence.bin")
         t0 = time.perf_counter()
         synth.write_keyframe_sequence(path, n_keyframes=args.keyframes, H=args.kf_height)
         gen_s = time.perf_counter() - t0
         res, traj = {}, {}
-        for name, model, graph in runs:
+        for name, model, graph, pinned in runs:
             tp = os.path.join(d, f"traj_{name}.bin")
             print(f"[bench] keyframe stream: {name}", file=sys.stderr, flush=True)
             r = subprocess.run([exe, path, str(args.kf_passes), str(graph), "--tree-model", model,
-                                "--weights", weights, "--traj-out", tp],
+                                "--weights", weights, "--traj-out", tp] +
+                               (["--pinned"] if pinned else []),
                                capture_output=True, text=True, timeout=600)
             if r.returncode != 0:
                 raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
@@ -315,13 +317,16 @@ def bench_keyframes(args):
                "drift_vs_truth": res[name]["drift"]}
         for name in ("learned", "plain", "lines")}
     g["tree_models"]["lines"]["drift_vs_plain"] = traj_diff(traj["lines"], traj["plain"])
+    g["pageable_input"] = {"latency_ms": res["learned_pageable"]["latency_ms"],
+                           "upload_p50_ms": res["learned_pageable"]["stage_p50_ms"]["upload"]}
     g["lines_eager_frontend_p50_ms"] = res["lines_eager"]["stage_p50_ms"]["frontend"]
     g["lines_eager_latency_p50_ms"] = res["lines_eager"]["latency_ms"]["p50"]
     g["workload"] = (f"{args.keyframes} keyframes of {args.kf_height}x2048 scans, 0.5 m apart, "
                      f"300-tree forest, 40 % understory clutter; {args.kf_passes} passes, the "
                      f"first a warm-up reported apart; learned tree models (plain and the "
-                     f"reference's median lines beside them); sequence generated on the host "
-                     f"in {gen_s:.1f} s (not timed)")
+                     f"reference's median lines beside them); each scan handed over in the "
+                     f"runner's pinned input buffers (pageable arrays: pageable_input); "
+                     f"sequence generated on the host in {gen_s:.1f} s (not timed)")
     return g
 
 

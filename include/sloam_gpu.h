@@ -531,7 +531,8 @@ int sloam_frontend_host_view(const sloam_frontend* f, sloam_frontend_host* out);
  * FP32 accumulation, activations between them are BF16.
  * Parameters: one FP32 array, PyTorch order (weight [out][in], then bias):
  * W1 [64][3], b1, W2 [128][64], b2, W3 [256][128], b3, W4 [128][256], b4,
- * W5 [7][128], b5: SLOAM_PN_PARAMS floats.                                   */
+ * W5 [7][128], b5: SLOAM_PN_PARAMS floats, 16-byte aligned (the entry points
+ * taking d_params return SLOAM_ERR_INVALID_ARG otherwise).                  */
 #define SLOAM_PN_POINTS 512
 #define SLOAM_PN_PARAMS 75399
 #define SLOAM_PN_PROPOSAL_STRIDE 8

@@ -5,14 +5,16 @@
 //
 //   keyframe_bench <sequence file> [passes] [use_graph 0/1]
 //                  [--tree-model lines|plain|learned] [--weights <npy>]
-//                  [--traj-out <file>] [--log]
+//                  [--traj-out <file>] [--log] [--pinned]
 //
 // --tree-model picks how each tree's model is made (KeyframeRunner's
 // TreeModel; default lines, the reference's); --traj-out writes the first
 // pass's pose estimates, K x 7 float64 (t, q x y z w), to compare the
 // trajectories of two tree models; --log prints the first pass's
 // keyframes (trees, matches, pose error before and after the solve) to
-// stderr.
+// stderr; --pinned writes each scan into the runner's pinned input buffers
+// (as a sensor driver would) before its timed call, instead of passing the
+// sequence's pageable arrays.
 //
 // Sequence file (little endian): int32 K, H, W; then per keyframe:
 // float64 true pose [7] (t, q x y z w; map <- sensor), float64 odometry
@@ -25,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -55,11 +58,13 @@ double pct(std::vector<double> v, double q) {
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
   std::string model_name = "lines", weights, traj_out;
-  bool log = false;
+  bool log = false, pinned = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if (a == "--log") {
       log = true;
+    } else if (a == "--pinned") {
+      pinned = true;
     } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out") && i + 1 < argc) {
       (a == "--tree-model" ? model_name : a == "--weights" ? weights : traj_out) = argv[++i];
     } else if (a.rfind("--", 0) == 0) {
@@ -130,7 +135,16 @@ int main(int argc, char** argv) {
         SloamOutput out;
         KeyframeTiming t;
         const SE3 before = estimate;
-        const bool ok = kf.run(seq[k].xyzi.data(), seq[k].label.data(), estimate, map, out, &t);
+        const float* xyzi = seq[k].xyzi.data();
+        const uint8_t* label = seq[k].label.data();
+        if (pinned) {
+          // the scan arrives in the pinned buffers (not timed)
+          std::memcpy(kf.pinnedXyzi(), xyzi, sizeof(float) * seq[k].xyzi.size());
+          std::memcpy(kf.pinnedLabels(), label, seq[k].label.size());
+          xyzi = kf.pinnedXyzi();
+          label = kf.pinnedLabels();
+        }
+        const bool ok = kf.run(xyzi, label, estimate, map, out, &t);
         if (log && pass == 0) {
           const SE3 truth = to_se3(seq[k].truth);
           auto err = [&](const SE3& e, double* deg) {
@@ -210,9 +224,9 @@ int main(int argc, char** argv) {
     std::fclose(o);
   }
   std::printf(
-      "{\"tree_model\": \"%s\", \"tree_fit_p50_ms\": %.4f, \"trees_fitted\": %lld, "
-      "\"trees_fallback\": %lld, ",
-      model_name.c_str(), pct(fit, 0.5), fitted, fallback);
+      "{\"tree_model\": \"%s\", \"pinned_input\": %s, \"tree_fit_p50_ms\": %.4f, "
+      "\"trees_fitted\": %lld, \"trees_fallback\": %lld, ",
+      model_name.c_str(), pinned ? "true" : "false", pct(fit, 0.5), fitted, fallback);
   std::printf(
       "\"keyframes\": %d, \"passes\": %d, \"height\": %d, \"width\": %d, \"graph\": %s, "
       "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"mean\": %.4f, \"max\": %.4f}, "

@@ -40,14 +40,19 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kHB = 8;  // hypotheses held in registers per pass over a tile
 constexpr int kMaxLds = 65536;  // LDS bytes per workgroup this kernel asks for
 // Refit tiers: a cluster of n points is refitted by one workgroup of
-// kTierBlock[t] threads, t the first tier with n <= kTierMax[t].  The tiers run
-// concurrently (the context stream and its auxiliary streams).  A workgroup
-// stays on one CU, so a cluster's time is set by its points per SIMD whatever
-// the workgroup size; measured on the 2000-cluster batch, 64 / 256 threads
-// split at 768 points beat three tiers with 512-thread workgroups and beat
-// staging the points in LDS (which lowers occupancy).
+// kTierBlock[t] threads, t the first tier with n <= the tier's bound.  The
+// tiers run concurrently (the context stream and its auxiliary streams).
+// When the batch fills every SIMD, a cluster's time is set by its points per
+// SIMD whatever the workgroup size: measured on the 2000-cluster batch, 64 /
+// 256 threads split at 768 points beat three tiers with 512-thread
+// workgroups and beat staging the points in LDS (which lowers occupancy).  A
+// batch with fewer clusters than the device has SIMDs leaves SIMDs idle; a
+// 256-thread workgroup then spreads its cluster over the four SIMDs of its CU
+// (a quarter of the serial FP64 work per SIMD), so only clusters of up to
+// kTierMaxSmall points keep one wave.
 constexpr int kTiers = 2;
 constexpr int kTierMax[kTiers] = {768, 1 << 30};
+constexpr int kTierMaxSmall[kTiers] = {256, 1 << 30};
 constexpr int kTierBlock[kTiers] = {64, 256};
 static_assert(kTiers >= 1 && kTiers <= 3, "one stream per tier: the context stream and two auxiliary ones");
 
@@ -786,8 +791,13 @@ struct sloam_cyl_plan {
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
   int n_xcd = 1;
+  int n_simd = 1024;  // SIMDs of the device (4 per CU)
   // allocated sizes (elements): buffers only grow across sloam_cyl_plan_reset
   size_t cap_work = 0, cap_lists = 0, cap_frames = 0, cap_counts = 0;
+  // pinned staging of the work and refit lists (bytes): their uploads are
+  // asynchronous on the context stream, ahead of the runs that read them
+  void* h_stage = nullptr;
+  size_t cap_stage = 0;
 };
 
 static int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -950,12 +960,13 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
       work.push_back(k < bins[x].size() ? bins[x][k] : make_int4(-1, 0, 0, 0));
   p->n_work = (int)work.size();
   // refit lists, one per tier, each in decreasing size
+  const int* tier_max = n_clusters < p->n_simd ? kTierMaxSmall : kTierMax;
   std::vector<int32_t> tier_list[kTiers];
   int tier_maxlen[kTiers] = {};
   for (int c : order) {
     const int len = h_offsets[c + 1] - h_offsets[c];
     int t = 0;
-    while (t + 1 < kTiers && len > kTierMax[t]) ++t;
+    while (t + 1 < kTiers && len > tier_max[t]) ++t;
     tier_list[t].push_back(c);
     tier_maxlen[t] = std::max(tier_maxlen[t], len);
   }
@@ -973,15 +984,27 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
     st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)n_clusters * cfg.n_hyp,
               sizeof(int32_t));
   if (st) return st;
-  if (!lists.empty()) {
-    hipError_t e = hipMemcpy(p->d_lists, lists.data(), sizeof(int32_t) * lists.size(),
-                             hipMemcpyHostToDevice);
-    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipMemcpy(lists)");
+  // both lists through the pinned staging buffer (the caller has made sure
+  // that no earlier upload from it is pending)
+  const size_t wb = sizeof(int4) * work.size(), lb = sizeof(int32_t) * lists.size();
+  if (wb + lb > p->cap_stage) {
+    if (p->h_stage) (void)hipHostFree(p->h_stage);
+    p->h_stage = nullptr;
+    p->cap_stage = 0;
+    const size_t sz = (wb + lb) + (wb + lb) / 2;
+    hipError_t e = hipHostMalloc(&p->h_stage, sz);
+    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipHostMalloc(plan staging)");
+    p->cap_stage = sz;
   }
-  if (!work.empty()) {
-    hipError_t e = hipMemcpy(p->d_work, work.data(), sizeof(int4) * work.size(),
-                             hipMemcpyHostToDevice);
-    if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipMemcpy(work)");
+  char* stage = static_cast<char*>(p->h_stage);
+  if (wb) {
+    std::memcpy(stage, work.data(), wb);
+    SLOAM_HIP(ctx, hipMemcpyAsync(p->d_work, stage, wb, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (lb) {
+    std::memcpy(stage + wb, lists.data(), lb);
+    SLOAM_HIP(ctx,
+              hipMemcpyAsync(p->d_lists, stage + wb, lb, hipMemcpyHostToDevice, ctx->stream));
   }
   return SLOAM_OK;
 }
@@ -1020,6 +1043,13 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
     n_xcd = 1;
   (void)hipGetLastError();
   p->n_xcd = n_xcd;
+  int n_cu = 256;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+          hipSuccess ||
+      n_cu < 1)
+    n_cu = 256;
+  (void)hipGetLastError();
+  p->n_simd = 4 * n_cu;
   const int st = plan_layout(p, h_offsets, n_clusters);
   if (st) {
     sloam_cyl_plan_destroy(p);
@@ -1210,6 +1240,11 @@ int sloam_cyl_autotune(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clust
 
 void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
   if (!p) return;
+  // pending uploads read the staging buffer
+  if (p->h_stage) {
+    (void)hipStreamSynchronize(p->ctx->stream);
+    (void)hipHostFree(p->h_stage);
+  }
   sloam_dev_free(p->d_work);
   sloam_dev_free(p->d_lists);
   sloam_dev_free(p->d_frames);

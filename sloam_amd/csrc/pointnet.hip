@@ -49,6 +49,8 @@ constexpr int kW3 = kB2 + kC2, kB3 = kW3 + kC3 * kC2;
 constexpr int kW4 = kB3 + kC3, kB4 = kW4 + kC4 * kC3;
 constexpr int kW5 = kB4 + kC4, kB5 = kW5 + kOut * kC4;
 constexpr int kNP = kB5 + kOut;
+static_assert(kW4 % 4 == 0 && kW5 % 4 == 0 && kC3 % 4 == 0 && kC4 % 4 == 0,
+              "head weight rows are read as float4");
 static_assert(kNP == SLOAM_PN_PARAMS, "parameter layout");
 
 // saved activations per cluster (floats): f[256], argmax[256] (int bits),
@@ -385,11 +387,18 @@ __global__ __launch_bounds__(kFB) void pn_head_kernel(const float* __restrict__ 
     float z[kHC];
 #pragma unroll
     for (int j = 0; j < kHC; ++j) z[j] = b;
-    const float* w = prm + kW4 + t * kC3;
-    for (int k = 0; k < kC3; ++k) {
-      const float wk = bf_round(w[k]);
+    // row t of W4 as 16-byte loads, several in flight (the row is 16-byte
+    // aligned: kW4 and kC3 are multiples of 4 and the parameters 16-byte
+    // aligned); the sum runs over k in order
+    const float4* w = reinterpret_cast<const float4*>(prm + kW4 + t * kC3);
+#pragma unroll 8
+    for (int q = 0; q < kC3 / 4; ++q) {
+      const float4 w4 = w[q];
+      const float wk[4] = {bf_round(w4.x), bf_round(w4.y), bf_round(w4.z), bf_round(w4.w)};
 #pragma unroll
-      for (int j = 0; j < kHC; ++j) z[j] += wk * fs[j][k];
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int j = 0; j < kHC; ++j) z[j] += wk[e] * fs[j][4 * q + e];
     }
 #pragma unroll
     for (int j = 0; j < kHC; ++j) h4s[j][t] = z[j] > 0.f ? z[j] : 0.f;
@@ -399,8 +408,15 @@ __global__ __launch_bounds__(kFB) void pn_head_kernel(const float* __restrict__ 
     const int j = t / kOut, o = t % kOut;
     if (j < nc) {
       float z = bf_round(prm[kB5 + o]);
-      const float* w = prm + kW5 + o * kC4;
-      for (int k = 0; k < kC4; ++k) z += bf_round(w[k]) * h4s[j][k];
+      const float4* w = reinterpret_cast<const float4*>(prm + kW5 + o * kC4);
+#pragma unroll 8
+      for (int q = 0; q < kC4 / 4; ++q) {
+        const float4 w4 = w[q];
+        z += bf_round(w4.x) * h4s[j][4 * q];
+        z += bf_round(w4.y) * h4s[j][4 * q + 1];
+        z += bf_round(w4.z) * h4s[j][4 * q + 2];
+        z += bf_round(w4.w) * h4s[j][4 * q + 3];
+      }
       outs[j][o] = z;
     }
   }
@@ -749,6 +765,7 @@ int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   if (!ctx || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
   if (n_clusters == 0) return SLOAM_OK;
   if (!d_params || !d_xyz || !d_offsets || !d_proposal) return SLOAM_ERR_INVALID_ARG;
+  if (((uintptr_t)d_params & 15) != 0) return SLOAM_ERR_INVALID_ARG;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
   PnBuffers b;
   int st = pn_buffers(ctx, n_clusters, false, &b);
@@ -778,6 +795,7 @@ int sloam_pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz
                        const sloam_pn_train_cfg* cfg, float* d_loss, float* d_grad) {
   if (!ctx || n_clusters <= 0 || !d_params || !d_xyz || !d_offsets || !d_targets || !d_grad)
     return SLOAM_ERR_INVALID_ARG;
+  if (((uintptr_t)d_params & 15) != 0) return SLOAM_ERR_INVALID_ARG;
   int st = pn_check_cfg(cfg);
   if (st) return st;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
@@ -793,6 +811,7 @@ int sloam_pn_train_step(sloam_ctx* ctx, float* d_params, float* d_m, float* d_v,
   if (!ctx || n_clusters <= 0 || step < 1 || !d_params || !d_m || !d_v || !d_xyz || !d_offsets ||
       !d_targets)
     return SLOAM_ERR_INVALID_ARG;
+  if (((uintptr_t)d_params & 15) != 0) return SLOAM_ERR_INVALID_ARG;
   int st = pn_check_cfg(cfg);
   if (st) return st;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));

@@ -85,6 +85,18 @@ KeyframeRunner::DevBuf::~DevBuf() {
   if (p) (void)hipFree(p);
 }
 
+float* KeyframeRunner::pinnedXyzi() {
+  if (!h_in_) {
+    const size_t N = (size_t)H_ * W_;
+    hip_check(hipHostMalloc(&h_in_, 17 * N), "hipHostMalloc");
+  }
+  return static_cast<float*>(h_in_);
+}
+
+uint8_t* KeyframeRunner::pinnedLabels() {
+  return reinterpret_cast<uint8_t*>(pinnedXyzi() + 4 * (size_t)H_ * W_);
+}
+
 void KeyframeRunner::setTreeModel(TreeModel mode, const std::string& weights_npy) {
   if (mode == TreeModel::Learned) {
     if (weights_npy.empty()) throw std::runtime_error("keyframe: the learned model needs weights");
@@ -103,7 +115,14 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
   // tree t's points are those of its vertices, contiguous in the front end's
   // point list
   const int32_t base = po[to[0]];
-  h_off_.resize((size_t)nt + 1);
+  if (h_off_cap_ < (size_t)nt + 1) {
+    if (h_off_) hip_check(hipHostFree(h_off_), "hipHostFree");
+    h_off_ = nullptr;
+    h_off_cap_ = 0;
+    const size_t cap = (size_t)nt + 1 + nt / 2;
+    hip_check(hipHostMalloc((void**)&h_off_, sizeof(int32_t) * cap), "hipHostMalloc");
+    h_off_cap_ = cap;
+  }
   for (int t = 0; t <= nt; ++t) h_off_[t] = po[to[t]] - base;
   const int64_t np = h_off_[nt];
   // plans: built on the first keyframe, laid out again on every other
@@ -118,9 +137,9 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
   cfg.hyp_chunk = 128;
   cfg.tile_points = 1024;
   if (!plain_) {
-    check(sloam_cyl_plan_create(ctx, h_off_.data(), nt, &cfg, &plain_), "sloam_cyl_plan_create");
+    check(sloam_cyl_plan_create(ctx, h_off_, nt, &cfg, &plain_), "sloam_cyl_plan_create");
   } else {
-    check(sloam_cyl_plan_reset(plain_, h_off_.data(), nt), "sloam_cyl_plan_reset");
+    check(sloam_cyl_plan_reset(plain_, h_off_, nt), "sloam_cyl_plan_reset");
   }
   const bool learned = mode_ == TreeModel::Learned;
   if (learned) {
@@ -129,9 +148,9 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
     vc.stream_tag = 1;
     vc.hyp_chunk = 16;
     if (!verify_) {
-      check(sloam_cyl_plan_create(ctx, h_off_.data(), nt, &vc, &verify_), "sloam_cyl_plan_create");
+      check(sloam_cyl_plan_create(ctx, h_off_, nt, &vc, &verify_), "sloam_cyl_plan_create");
     } else {
-      check(sloam_cyl_plan_reset(verify_, h_off_.data(), nt), "sloam_cyl_plan_reset");
+      check(sloam_cyl_plan_reset(verify_, h_off_, nt), "sloam_cyl_plan_reset");
     }
   }
   d_xyz_.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(np, 1));
@@ -153,7 +172,7 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
   double* model = static_cast<double*>(d_model_.p);
   int32_t* flags = static_cast<int32_t*>(d_flags_.p);
   int32_t* n_fallback = flags + nt;
-  check(sloam_memcpy_async(ctx, off, h_off_.data(), sizeof(int32_t) * ((size_t)nt + 1)),
+  check(sloam_memcpy_async(ctx, off, h_off_, sizeof(int32_t) * ((size_t)nt + 1)),
         "upload tree offsets");
   check(sloam_xyz_from_xyzi(ctx, buf_.points + 4 * (size_t)base, np, xyz), "sloam_xyz_from_xyzi");
   if (learned) {
@@ -235,6 +254,8 @@ KeyframeRunner::~KeyframeRunner() {
   sloam_cyl_plan_destroy(plain_);
   sloam_cyl_plan_destroy(verify_);
   if (h_model_) (void)hipHostFree(h_model_);
+  if (h_in_) (void)hipHostFree(h_in_);
+  if (h_off_) (void)hipHostFree(h_off_);
   sloam_frontend_destroy(fe_);
 }
 

@@ -57,6 +57,13 @@ class KeyframeRunner {
 
   sloam::sloam& estimator() { return sloam_; }
 
+  // Pinned host buffers of one scan (height * width x 4 floats, height *
+  // width labels), owned by the runner: a scan written here (e.g. by the
+  // sensor driver) and passed to run() is uploaded by DMA straight from them,
+  // at about twice the rate of pageable memory.
+  float* pinnedXyzi();
+  uint8_t* pinnedLabels();
+
   // Tree model of the following keyframes; Learned needs the proposal
   // network's parameters (float32 .npy of SLOAM_PN_PARAMS values).
   void setTreeModel(TreeModel mode, const std::string& weights_npy = std::string());
@@ -82,8 +89,10 @@ class KeyframeRunner {
   TreeModel mode_ = TreeModel::MedianLines;
   sloam_cyl_plan* plain_ = nullptr;
   sloam_cyl_plan* verify_ = nullptr;
-  std::vector<int32_t> h_off_;
+  int32_t* h_off_ = nullptr;    // pinned: tree point offsets of the last fit
+  size_t h_off_cap_ = 0;
   DevBuf d_params_, d_xyz_, d_off_, d_model_, d_prop_, d_flags_;
   double* h_model_ = nullptr;   // pinned: models of the last fit + fallback count
+  void* h_in_ = nullptr;        // pinned scan: xyzi then labels
   size_t h_model_cap_ = 0;      // trees
 };
