@@ -792,8 +792,11 @@ struct sloam_cyl_plan {
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
   int n_xcd = 1;
   int n_simd = 1024;  // SIMDs of the device (4 per CU)
-  // allocated sizes (elements): buffers only grow across sloam_cyl_plan_reset
-  size_t cap_work = 0, cap_lists = 0, cap_frames = 0, cap_counts = 0;
+  // d_work and d_lists share one allocation (the work list, then the refit
+  // lists), uploaded in one copy; allocated sizes (bytes for the tables,
+  // elements otherwise) only grow across sloam_cyl_plan_reset
+  void* d_tables = nullptr;
+  size_t cap_tables = 0, cap_frames = 0, cap_counts = 0;
   // pinned staging of the work and refit lists (bytes): their uploads are
   // asynchronous on the context stream, ahead of the runs that read them
   void* h_stage = nullptr;
@@ -977,16 +980,17 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
     p->tier_bits[t] = round_up(std::max(tier_maxlen[t], 1), kTierBlock[t]) / 8;
     lists.insert(lists.end(), tier_list[t].begin(), tier_list[t].end());
   }
-  int st = grow(ctx, (void**)&p->d_lists, &p->cap_lists, lists.size(), sizeof(int32_t));
-  if (!st) st = grow(ctx, (void**)&p->d_work, &p->cap_work, work.size(), sizeof(int4));
+  const size_t wb = sizeof(int4) * work.size(), lb = sizeof(int32_t) * lists.size();
+  int st = grow(ctx, &p->d_tables, &p->cap_tables, wb + lb, 1);
   if (!st) st = grow(ctx, (void**)&p->d_frames, &p->cap_frames, n_clusters, sizeof(CylFrame));
   if (!st)
     st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)n_clusters * cfg.n_hyp,
               sizeof(int32_t));
   if (st) return st;
-  // both lists through the pinned staging buffer (the caller has made sure
-  // that no earlier upload from it is pending)
-  const size_t wb = sizeof(int4) * work.size(), lb = sizeof(int32_t) * lists.size();
+  p->d_work = static_cast<int4*>(p->d_tables);
+  p->d_lists = reinterpret_cast<int32_t*>(static_cast<char*>(p->d_tables) + wb);
+  // both lists through the pinned staging buffer, in one copy (the caller has
+  // made sure that no earlier upload from it is pending)
   if (wb + lb > p->cap_stage) {
     if (p->h_stage) (void)hipHostFree(p->h_stage);
     p->h_stage = nullptr;
@@ -997,15 +1001,10 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
     p->cap_stage = sz;
   }
   char* stage = static_cast<char*>(p->h_stage);
-  if (wb) {
-    std::memcpy(stage, work.data(), wb);
-    SLOAM_HIP(ctx, hipMemcpyAsync(p->d_work, stage, wb, hipMemcpyHostToDevice, ctx->stream));
-  }
-  if (lb) {
-    std::memcpy(stage + wb, lists.data(), lb);
-    SLOAM_HIP(ctx,
-              hipMemcpyAsync(p->d_lists, stage + wb, lb, hipMemcpyHostToDevice, ctx->stream));
-  }
+  if (wb) std::memcpy(stage, work.data(), wb);
+  if (lb) std::memcpy(stage + wb, lists.data(), lb);
+  if (wb + lb)
+    SLOAM_HIP(ctx, hipMemcpyAsync(p->d_tables, stage, wb + lb, hipMemcpyHostToDevice, ctx->stream));
   return SLOAM_OK;
 }
 
@@ -1245,8 +1244,7 @@ void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
     (void)hipStreamSynchronize(p->ctx->stream);
     (void)hipHostFree(p->h_stage);
   }
-  sloam_dev_free(p->d_work);
-  sloam_dev_free(p->d_lists);
+  sloam_dev_free(p->d_tables);
   sloam_dev_free(p->d_frames);
   sloam_dev_free(p->d_counts);
   delete p;

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 #include "gpu_bridge.h"
 
@@ -113,9 +114,17 @@ void Cylinder::computeModel(const std::vector<TreeVertex>& landmarkVtxs,
                             const Scalar defaultTreeRadius, const Scalar featuresPerTree,
                             const LineFit* fit, const CylinderFit* cfit) {
   const TreeVertex& firstVtx = landmarkVtxs[2];
+  // a model that is computed to the end keeps only its first featuresPerTree
+  // features (the resize below), so only those are copied; the early returns
+  // keep every point, as the reference's do
+  const bool toEnd = needsLineFit(landmarkVtxs) &&
+                     (cfit ? (cfit->ok && cfit->radius > 0.0) : (fit->ok && fit->inliers != 0));
+  const size_t keep =
+      toEnd ? (size_t)featuresPerTree : std::numeric_limits<size_t>::max();
   std::vector<float> validRadii;
   for (const auto& vtx : landmarkVtxs) {
     for (auto p : vtx.points) {
+      if (features.size() >= keep) break;
       p.intensity = (float)firstVtx.treeId;
       features.push_back(p);
     }

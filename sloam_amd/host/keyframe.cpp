@@ -155,9 +155,10 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
   }
   d_xyz_.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(np, 1));
   d_off_.ensure(sizeof(int32_t) * ((size_t)nt + 1));
-  d_model_.ensure(sizeof(double) * SLOAM_CYL_MODEL_STRIDE * (size_t)nt);
+  // the models, then the fallback count (one download for both)
+  d_model_.ensure(sizeof(double) * (SLOAM_CYL_MODEL_STRIDE * (size_t)nt + 1));
   d_prop_.ensure(sizeof(float) * SLOAM_PN_PROPOSAL_STRIDE * (size_t)nt);
-  d_flags_.ensure(sizeof(int32_t) * ((size_t)nt + 1));
+  d_flags_.ensure(sizeof(int32_t) * (size_t)nt);
   if (h_model_cap_ < (size_t)nt) {
     if (h_model_) hip_check(hipHostFree(h_model_), "hipHostFree");
     h_model_ = nullptr;
@@ -171,7 +172,7 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
   int32_t* off = static_cast<int32_t*>(d_off_.p);
   double* model = static_cast<double*>(d_model_.p);
   int32_t* flags = static_cast<int32_t*>(d_flags_.p);
-  int32_t* n_fallback = flags + nt;
+  int32_t* n_fallback = reinterpret_cast<int32_t*>(model + SLOAM_CYL_MODEL_STRIDE * (size_t)nt);
   check(sloam_memcpy_async(ctx, off, h_off_, sizeof(int32_t) * ((size_t)nt + 1)),
         "upload tree offsets");
   check(sloam_xyz_from_xyzi(ctx, buf_.points + 4 * (size_t)base, np, xyz), "sloam_xyz_from_xyzi");
@@ -186,15 +187,12 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
     check(sloam_cyl_plan_exec(plain_, xyz, off, nullptr, nullptr, nullptr, nullptr, model),
           "sloam_cyl_plan_exec");
   }
-  check(sloam_memcpy_async(ctx, h_model_, model, sizeof(double) * SLOAM_CYL_MODEL_STRIDE * nt),
+  check(sloam_memcpy_async(ctx, h_model_, model,
+                           sizeof(double) * (SLOAM_CYL_MODEL_STRIDE * (size_t)nt + (learned ? 1 : 0))),
         "download models");
   int32_t fb = 0;
-  if (learned)
-    check(sloam_memcpy_async(ctx, h_model_ + SLOAM_CYL_MODEL_STRIDE * h_model_cap_, n_fallback,
-                             sizeof(int32_t)),
-          "download fallback count");
   check(sloam_sync(ctx), "sloam_sync");
-  if (learned) std::memcpy(&fb, h_model_ + SLOAM_CYL_MODEL_STRIDE * h_model_cap_, sizeof(int32_t));
+  if (learned) std::memcpy(&fb, h_model_ + SLOAM_CYL_MODEL_STRIDE * (size_t)nt, sizeof(int32_t));
   fits.assign(nt, CylinderFit());
   for (int t = 0; t < nt; ++t) {
     const double* m = h_model_ + SLOAM_CYL_MODEL_STRIDE * (size_t)t;

@@ -363,15 +363,16 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
                           std::vector<Plane>& planes) {
   // ground: all cells of the scan in one call (binned at the origin,
   // sloam.cpp:390-393), unless the caller brings them
-  std::vector<Plane> cells = in.groundCells;
-  if (cells.empty()) {
+  std::vector<Plane> fittedCells;
+  if (in.groundCells.empty()) {
     for (const auto& cell : sloam_gpu::groundPlanes(in.groundCloud->points, groundCfg(SE3()))) {
       PlaneParameters fitted;
       for (int i = 0; i < 4; ++i) fitted.plane(i) = cell.plane[i];
       for (int i = 0; i < 3; ++i) fitted.centroid(i) = cell.centroid[i];
-      cells.emplace_back(fitted, cell.features, cell.valid);
+      fittedCells.emplace_back(fitted, cell.features, cell.valid);
     }
   }
+  const std::vector<Plane>& cells = in.groundCells.empty() ? fittedCells : in.groundCells;
   const Vector3 src_vec(0, 0, 1);
   const Matrix4 tfm = in.poseEstimate.inverse().matrix().transpose();
   for (const Plane& ground : cells) {
@@ -430,7 +431,7 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
       const CylinderFit none;
       Cylinder c(in.landmarks[k], planes[best[k]], fmParams_,
                  set_of[k] >= 0 ? in.treeCylinderFits[k] : none);
-      if (c.isValid) landmarks.push_back(c);
+      if (c.isValid) landmarks.push_back(std::move(c));
       continue;
     }
     LineFit fit;
@@ -445,7 +446,7 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
       }
     }
     Cylinder c(in.landmarks[k], planes[best[k]], fmParams_, fit);
-    if (c.isValid) landmarks.push_back(c);
+    if (c.isValid) landmarks.push_back(std::move(c));
   }
 }
 
