@@ -69,6 +69,48 @@ Arena& arena() {
   return a;
 }
 
+// Per-thread grow-only pinned host buffer that stages a call's uploads and
+// downloads (one copy each way); every call synchronises before it returns,
+// so the next call may rewrite it.
+struct Pinned {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  char* take(size_t n) {
+    if (n > bytes) {
+      if (p) hip_check(hipHostFree(p), "hipHostFree");
+      p = nullptr;
+      bytes = 0;
+      const size_t sz = n + n / 2;
+      hip_check(hipHostMalloc(&p, sz), "hipHostMalloc");
+      bytes = sz;
+    }
+    return static_cast<char*>(p);
+  }
+};
+
+Pinned& pinned() {
+  thread_local Pinned s;
+  return s;
+}
+
+// byte offsets of 256-byte aligned sections of one staging block
+struct Sections {
+  size_t size = 0;
+  size_t add(size_t bytes) {
+    const size_t o = size;
+    size += (bytes + 255) / 256 * 256;
+    return o;
+  }
+};
+
+template <typename T>
+void put(char* base, size_t off, const std::vector<T>& v) {
+  if (!v.empty()) std::memcpy(base + off, v.data(), sizeof(T) * v.size());
+}
+
 // opened at the start of every bridge call: slots are numbered from 0 again
 struct CallScope {
   CallScope() { arena().next = 0; }
@@ -249,36 +291,49 @@ std::vector<PoseResult> solvePoses(const std::vector<PoseProblem>& problems,
           "sloam_pose_plan_reset");
   }
   sloam_pose_plan* plan = cached.get();
-  Dev d_pts(sizeof(float) * (pts.empty() ? 4 : pts.size()));
-  Dev d_cyl(sizeof(double) * (cyl.empty() ? 8 : cyl.size()));
-  Dev d_pl(sizeof(double) * (pl.empty() ? 4 : pl.size()));
-  Dev d_co(sizeof(int32_t) * cyl_off.size());
-  Dev d_po(sizeof(int32_t) * pl_off.size());
-  Dev d_pose(sizeof(double) * pose.size());
-  Dev d_mask(sizeof(int32_t) * masks.size());
-  Dev d_out(sizeof(double) * 7 * P);
-  Dev d_par(sizeof(double) * 6 * P);
-  Dev d_st(sizeof(int32_t) * 2 * P);
-  Dev d_cost(sizeof(double) * 2 * P);
-  up(d_pts, pts);
-  up(d_cyl, cyl);
-  up(d_pl, pl);
-  up(d_co, cyl_off);
-  up(d_po, pl_off);
-  up(d_pose, pose);
-  up(d_mask, masks);
-  check(sloam_pose_plan_exec(plan, d_pts.as<float>(), d_cyl.as<double>(), d_co.as<int32_t>(),
-                             d_pl.as<double>(), d_po.as<int32_t>(), d_pose.as<double>(),
-                             d_mask.as<int32_t>(), d_out.as<double>(), d_par.as<double>(),
-                             d_st.as<int32_t>(), d_cost.as<double>()),
+  // inputs in one upload, outputs in one download, through pinned staging
+  Sections in, res;
+  const size_t o_pts = in.add(sizeof(float) * (pts.empty() ? 4 : pts.size()));
+  const size_t o_cyl = in.add(sizeof(double) * (cyl.empty() ? 8 : cyl.size()));
+  const size_t o_pl = in.add(sizeof(double) * (pl.empty() ? 4 : pl.size()));
+  const size_t o_co = in.add(sizeof(int32_t) * cyl_off.size());
+  const size_t o_po = in.add(sizeof(int32_t) * pl_off.size());
+  const size_t o_pose = in.add(sizeof(double) * pose.size());
+  const size_t o_mask = in.add(sizeof(int32_t) * masks.size());
+  const size_t o_out = res.add(sizeof(double) * 7 * P);
+  const size_t o_par = res.add(sizeof(double) * 6 * P);
+  const size_t o_st = res.add(sizeof(int32_t) * 2 * P);
+  const size_t o_cost = res.add(sizeof(double) * 2 * P);
+  Dev d_in(in.size);
+  Dev d_res(res.size);
+  char* h = pinned().take(in.size + res.size);
+  put(h, o_pts, pts);
+  put(h, o_cyl, cyl);
+  put(h, o_pl, pl);
+  put(h, o_co, cyl_off);
+  put(h, o_po, pl_off);
+  put(h, o_pose, pose);
+  put(h, o_mask, masks);
+  const hipStream_t stream = (hipStream_t)sloam_ctx_stream(ctx);
+  hip_check(hipMemcpyAsync(d_in.p, h, in.size, hipMemcpyHostToDevice, stream),
+            "hipMemcpyAsync(H2D)");
+  char* di = d_in.as<char>();
+  char* dr = d_res.as<char>();
+  check(sloam_pose_plan_exec(plan, (const float*)(di + o_pts), (const double*)(di + o_cyl),
+                             (const int32_t*)(di + o_co), (const double*)(di + o_pl),
+                             (const int32_t*)(di + o_po), (const double*)(di + o_pose),
+                             (const int32_t*)(di + o_mask), (double*)(dr + o_out),
+                             (double*)(dr + o_par), (int32_t*)(dr + o_st),
+                             (double*)(dr + o_cost)),
         "sloam_pose_plan_exec");
-  std::vector<double> o(7 * P), par(6 * P), cost(2 * P);
-  std::vector<int32_t> st(2 * P);
-  down(o, d_out);
-  down(par, d_par);
-  down(st, d_st);
-  down(cost, d_cost);
+  char* hr = h + in.size;
+  hip_check(hipMemcpyAsync(hr, d_res.p, res.size, hipMemcpyDeviceToHost, stream),
+            "hipMemcpyAsync(D2H)");
   check(sloam_sync(ctx), "sloam_sync");
+  const double* o = (const double*)(hr + o_out);
+  const double* par = (const double*)(hr + o_par);
+  const int32_t* st = (const int32_t*)(hr + o_st);
+  const double* cost = (const double*)(hr + o_cost);
   for (int k = 0; k < P; ++k) {
     std::memcpy(out[k].pose, &o[7 * k], 7 * sizeof(double));
     std::memcpy(out[k].params, &par[6 * k], 6 * sizeof(double));
