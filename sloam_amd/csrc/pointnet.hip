@@ -49,8 +49,8 @@ constexpr int kW3 = kB2 + kC2, kB3 = kW3 + kC3 * kC2;
 constexpr int kW4 = kB3 + kC3, kB4 = kW4 + kC4 * kC3;
 constexpr int kW5 = kB4 + kC4, kB5 = kW5 + kOut * kC4;
 constexpr int kNP = kB5 + kOut;
-static_assert(kW4 % 4 == 0 && kW5 % 4 == 0 && kC3 % 4 == 0 && kC4 % 4 == 0,
-              "head weight rows are read as float4");
+static_assert(kW4 % 4 == 0 && kW5 % 4 == 0 && kC3 % 128 == 0 && kC4 % 64 == 0 && kFB == 2 * kC4,
+              "head: weight half-rows read as batches of 16 float4 by 2 x kC4 threads");
 static_assert(kNP == SLOAM_PN_PARAMS, "parameter layout");
 
 // saved activations per cluster (floats): f[256], argmax[256] (int bits),
@@ -382,26 +382,42 @@ __global__ __launch_bounds__(kFB) void pn_head_kernel(const float* __restrict__ 
   for (int i = t; i < nc * kC3; i += kFB)
     fs[i / kC3][i % kC3] = act[(size_t)(c0 + i / kC3) * kAct + kAF + i % kC3];
   __syncthreads();
-  if (t < kC4) {
-    const float b = bf_round(prm[kB4 + t]);
+  // layer 4 on all 256 threads: output row o, half of k per thread (fused
+  // multiply-adds), the two halves added in LDS
+  {
+    const int o = t & (kC4 - 1);
+    const int half = t / kC4;
     float z[kHC];
 #pragma unroll
-    for (int j = 0; j < kHC; ++j) z[j] = b;
-    // row t of W4 as 16-byte loads, several in flight (the row is 16-byte
-    // aligned: kW4 and kC3 are multiples of 4 and the parameters 16-byte
-    // aligned); the sum runs over k in order
-    const float4* w = reinterpret_cast<const float4*>(prm + kW4 + t * kC3);
-#pragma unroll 8
-    for (int q = 0; q < kC3 / 4; ++q) {
-      const float4 w4 = w[q];
-      const float wk[4] = {bf_round(w4.x), bf_round(w4.y), bf_round(w4.z), bf_round(w4.w)};
+    for (int j = 0; j < kHC; ++j) z[j] = 0.f;
+    const float4* w = reinterpret_cast<const float4*>(prm + kW4 + o * kC3 + half * (kC3 / 2));
+    for (int q0 = 0; q0 < kC3 / 8; q0 += 16) {
+      float4 wv[16];
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
+      for (int r = 0; r < 16; ++r) wv[r] = w[q0 + r];
 #pragma unroll
-        for (int j = 0; j < kHC; ++j) z[j] += wk[e] * fs[j][4 * q + e];
+      for (int r = 0; r < 16; ++r) {
+        const int k = half * (kC3 / 2) + 4 * (q0 + r);
+        const float wk[4] = {bf_round(wv[r].x), bf_round(wv[r].y), bf_round(wv[r].z),
+                             bf_round(wv[r].w)};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int j = 0; j < kHC; ++j) z[j] = __builtin_fmaf(wk[e], fs[j][k + e], z[j]);
+      }
     }
+    if (half == 1)
 #pragma unroll
-    for (int j = 0; j < kHC; ++j) h4s[j][t] = z[j] > 0.f ? z[j] : 0.f;
+      for (int j = 0; j < kHC; ++j) h4s[j][o] = z[j];
+    __syncthreads();
+    if (half == 0) {
+      const float bias = bf_round(prm[kB4 + o]);
+#pragma unroll
+      for (int j = 0; j < kHC; ++j) {
+        const float v = bias + (z[j] + h4s[j][o]);
+        h4s[j][o] = v > 0.f ? v : 0.f;
+      }
+    }
   }
   __syncthreads();
   if (t < kOut * kHC) {
@@ -409,13 +425,18 @@ __global__ __launch_bounds__(kFB) void pn_head_kernel(const float* __restrict__ 
     if (j < nc) {
       float z = bf_round(prm[kB5 + o]);
       const float4* w = reinterpret_cast<const float4*>(prm + kW5 + o * kC4);
-#pragma unroll 8
-      for (int q = 0; q < kC4 / 4; ++q) {
-        const float4 w4 = w[q];
-        z += bf_round(w4.x) * h4s[j][4 * q];
-        z += bf_round(w4.y) * h4s[j][4 * q + 1];
-        z += bf_round(w4.z) * h4s[j][4 * q + 2];
-        z += bf_round(w4.w) * h4s[j][4 * q + 3];
+      for (int q0 = 0; q0 < kC4 / 4; q0 += 16) {
+        float4 wv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) wv[r] = w[q0 + r];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int q = q0 + r;
+          z = __builtin_fmaf(bf_round(wv[r].x), h4s[j][4 * q], z);
+          z = __builtin_fmaf(bf_round(wv[r].y), h4s[j][4 * q + 1], z);
+          z = __builtin_fmaf(bf_round(wv[r].z), h4s[j][4 * q + 2], z);
+          z = __builtin_fmaf(bf_round(wv[r].w), h4s[j][4 * q + 3], z);
+        }
       }
       outs[j][o] = z;
     }
