@@ -20,8 +20,11 @@
 // float64 true pose [7] (t, q x y z w; map <- sensor), float64 odometry
 // increment [7] (noisy relative motion from the previous keyframe), float32
 // xyzi [H * W * 4], uint8 labels [H * W].  The pose estimate of keyframe k is
-// the previous estimate composed with the increment; the map is the previous
-// keyframe's tree models.
+// the previous estimate composed with the increment.  The map is kept as
+// SLOAMNode::run keeps it (sloam/src/core/sloamNode.cpp:186-245): a
+// MapManager submap around the estimate goes in, the keyframe's trees and
+// their matches update the map; a keyframe after the first with an empty
+// submap is discarded (its estimate stays the odometry's).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -33,6 +36,7 @@
 #include <vector>
 
 #include "keyframe.h"
+#include "mapManager.h"
 
 namespace {
 
@@ -124,17 +128,22 @@ int main(int argc, char** argv) {
   double final_drift = 0, max_drift = 0, max_rot = 0;
   std::vector<double> fit, traj;
   long long fitted = 0, fallback = 0;
+  int discarded = 0;
   try {
     for (int pass = 0; pass < passes; ++pass) {
       KeyframeRunner kf(H, W, fm, use_graph);
       kf.setTreeModel(tree_model, weights);
       SE3 estimate = to_se3(seq[0].truth);
-      std::vector<Cylinder> map;
+      MapManager semanticMap;
+      bool nodeFirst = true;  // SLOAMNode's firstScan_
       for (int k = 0; k < K; ++k) {
         if (k > 0) estimate = estimate * to_se3(seq[k].odom);
         SloamOutput out;
         KeyframeTiming t;
         const SE3 before = estimate;
+        std::vector<Cylinder> map;
+        semanticMap.getSubmap(estimate, map);
+        const bool discard = !nodeFirst && map.empty();
         const float* xyzi = seq[k].xyzi.data();
         const uint8_t* label = seq[k].label.data();
         if (pinned) {
@@ -144,7 +153,14 @@ int main(int argc, char** argv) {
           xyzi = kf.pinnedXyzi();
           label = kf.pinnedLabels();
         }
-        const bool ok = kf.run(xyzi, label, estimate, map, out, &t);
+        bool ok = false;
+        if (discard) {
+          if (pass == 0) ++discarded;
+        } else {
+          ok = kf.run(xyzi, label, estimate, map, out, &t);
+          semanticMap.updateMap(out.tm, out.matches);
+          if (ok && nodeFirst && !out.tm.empty()) nodeFirst = false;
+        }
         if (log && pass == 0) {
           const SE3 truth = to_se3(seq[k].truth);
           auto err = [&](const SE3& e, double* deg) {
@@ -165,8 +181,6 @@ int main(int argc, char** argv) {
                        t.planes, e0, d0, e1, d1);
         }
         if (ok) estimate = out.T_Map_Curr;
-        // the map for the next keyframe: this keyframe's trees (world frame)
-        if (!out.tm.empty()) map = out.tm;
         if (pass == 0) {
           const Vector3 tr = estimate.translation();
           const Quat q = estimate.unit_quaternion();
@@ -182,7 +196,9 @@ int main(int argc, char** argv) {
           max_drift = std::max(max_drift, d);
           max_rot = std::max(max_rot, ang);
         }
-        if (k == 0) continue;  // first scan: map initialisation (and graph capture)
+        // first scan: map initialisation (and graph capture); discarded
+        // keyframes ran nothing
+        if (k == 0 || discard) continue;
         if (pass == 0 && passes > 1) {
           // warm-up pass (first use of every buffer, plan and kernel of the
           // process): reported apart
@@ -225,8 +241,8 @@ int main(int argc, char** argv) {
   }
   std::printf(
       "{\"tree_model\": \"%s\", \"pinned_input\": %s, \"tree_fit_p50_ms\": %.4f, "
-      "\"trees_fitted\": %lld, \"trees_fallback\": %lld, ",
-      model_name.c_str(), pinned ? "true" : "false", pct(fit, 0.5), fitted, fallback);
+      "\"trees_fitted\": %lld, \"trees_fallback\": %lld, \"discarded\": %d, ",
+      model_name.c_str(), pinned ? "true" : "false", pct(fit, 0.5), fitted, fallback, discarded);
   std::printf(
       "\"keyframes\": %d, \"passes\": %d, \"height\": %d, \"width\": %d, \"graph\": %s, "
       "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"mean\": %.4f, \"max\": %.4f}, "

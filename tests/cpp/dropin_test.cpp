@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "cylinder.h"
+#include "mapManager.h"
 #include "definitions.h"
 #include "plane.h"
 #include "sloam.h"
@@ -435,6 +436,95 @@ void cylinder_from_fit() {
                    .isValid);
 }
 
+// MapManager (mapManager.cpp): new landmarks, hits and replacement through
+// the last submap's indices, the hit threshold of getMap, and the submap's
+// nearest-first order over the 200 newest landmarks (at most 100).
+void map_manager() {
+  FeatureModelParams fm;
+  PlaneParameters pp;
+  pp.plane = Vector4(0, 0, 1, 0);
+  pp.centroid = Vector3(0, 0, 0);
+  const Plane ground(pp, VectorType(), true);
+  // a valid tree model rooted at (x, y, 0)
+  auto tree = [&](double x, double y) {
+    std::vector<TreeVertex> v;
+    for (int k = 0; k < 10; ++k) {
+      TreeVertex t;
+      t.treeId = 3;
+      t.radius = 0.25;
+      t.coords.x = (float)x;
+      t.coords.y = (float)y;
+      t.coords.z = (float)(0.3 * k);
+      for (int i = 0; i < 5; ++i) t.points.push_back(t.coords);
+      v.push_back(t);
+    }
+    CylinderFit f;
+    f.ok = true;
+    f.root[0] = x;
+    f.root[1] = y;
+    f.root[2] = 1.0;
+    f.ray[2] = 1.0;
+    f.radius = 0.25;
+    f.inliers = 40;
+    // the vertices lie within 20 m of the plane centroid only near the origin
+    PlaneParameters q = pp;
+    q.centroid = Vector3(x, y, 0);
+    return Cylinder(v, Plane(q, VectorType(), true), fm, f);
+  };
+  MapManager m;
+  std::vector<Cylinder> sub;
+  m.getSubmap(SE3(), sub);
+  EXPECT_TRUE(sub.empty());
+  // first scan: three new landmarks at x = 1, 5, 9
+  std::vector<Cylinder> obs = {tree(1, 0), tree(5, 0), tree(9, 0)};
+  for (const auto& c : obs) EXPECT_TRUE(c.isValid);
+  m.updateMap(obs, {-1, -1, -1});
+  EXPECT_EQ((size_t)3, m.size());
+  EXPECT_TRUE(m.getMap().empty());  // one hit each
+  // the submap from x = 6: nearest first (5, 9, 1)
+  SE3 at6;
+  at6.translation()[0] = 6;
+  m.getSubmap(at6, sub);
+  EXPECT_EQ((size_t)3, sub.size());
+  EXPECT_NEAR(5.0, sub[0].model.root[0], 1e-12);
+  EXPECT_NEAR(9.0, sub[1].model.root[0], 1e-12);
+  EXPECT_NEAR(1.0, sub[2].model.root[0], 1e-12);
+  // observation 0 matches submap entry 0 (map entry 1) and moves it; the
+  // other is new
+  std::vector<Cylinder> obs2 = {tree(5.2, 0), tree(20, 0)};
+  m.updateMap(obs2, {0, -1});
+  EXPECT_EQ((size_t)4, m.size());
+  // two more hits on the same landmark: it enters the map
+  for (int r = 0; r < 2; ++r) {
+    std::vector<Cylinder> s2;
+    m.getSubmap(at6, s2);
+    EXPECT_NEAR(5.2, s2[0].model.root[0], 1e-6);
+    std::vector<Cylinder> o = {tree(5.2, 0)};
+    m.updateMap(o, {0});
+  }
+  const auto full = m.getMap();
+  EXPECT_EQ((size_t)1, full.size());
+  if (!full.empty()) EXPECT_NEAR(5.2, full[0].model.root[0], 1e-6);
+  // only the 200 newest landmarks and at most 100 of them take part
+  MapManager big;
+  std::vector<Cylinder> many;
+  std::vector<int> none;
+  for (int i = 0; i < 250; ++i) {
+    many.push_back(tree(0.01 * i, 0));
+    none.push_back(-1);
+  }
+  big.updateMap(many, none);
+  // the nearest 100 of the 250 (from (0, 0, 1)) are landmarks 0..99; of
+  // those, only the newest 200 of the map (250 - index < 200: 51..99) count
+  std::vector<Cylinder> sb;
+  big.getSubmap(SE3(), sb);
+  EXPECT_EQ((size_t)49, sb.size());
+  bool ordered = true;
+  for (size_t i = 0; i < sb.size(); ++i)
+    ordered = ordered && std::fabs(sb[i].model.root[0] - 0.01 * (51 + (int)i)) < 1e-6;
+  EXPECT_TRUE(ordered);
+}
+
 // ---------------------------------------------------------------------------
 // trellis: the reference's tree clouds give its landmark archives
 // (sloam/src/tests/aux/<seq>_tree_<t>.pcd -> <seq>_landmarks_<t>); the
@@ -691,6 +781,7 @@ int main(int argc, char** argv) {
       {"AngleAxisQuaternion", false, test_angle_axis},
       {"SE3", false, test_se3},
       {"CylinderTest.FromCylinderFit", false, cylinder_from_fit},
+      {"MapManager", false, map_manager},
       {"PlaneTest.Initalizes", true, plane_initializes},
       {"PlaneTest.DistanceToFeature", true, plane_distance_to_feature},
       {"PlaneTest.TranslateModel", true, plane_translate_model},
