@@ -232,6 +232,7 @@ def bench_learned(ctx, args):
     ref = pout["model"].cpu().numpy()
     lm = lout["model"].cpu().numpy()
     nf = int(lout["n_fallback"].cpu().numpy()[0])
+    bins = bench_learned_size_bins(ctx, args, net, timed)
     ok = (ref[:, 6] > 0) & (lm[:, 6] > 0)
     err = np.abs(lm[ok, 6] - ref[ok, 6]) * 100
     fwd_s = kern["pn_fwd"] / 1e3
@@ -252,10 +253,55 @@ def bench_learned(ctx, args):
                                     "p90": float(np.percentile(err, 90)),
                                     "frac_over_1cm": float(np.mean(err > 1.0))},
         "kernel_ms_per_batch": kern,
+        "size_bins": bins,
         "net_mfma": {"achieved": net_flop / fwd_s / 1e12, "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": net_flop / fwd_s / 1e12 / BF16_PEAK_TFLOPS,
                      "flop_per_launch": net_flop},
     }
+
+
+def bench_learned_size_bins(ctx, args, net, timed, sizes=(256, 1024, 4096), clusters=1024):
+    """The learned fit against the plain one per cluster size: `clusters`
+    clusters of about n points for each n (same generator, 20 % outliers;
+    keyed by the size drawn).
+    The network sees at most 512 points of a cluster, so its share of the
+    work shrinks as n grows while the plain method's grows with n."""
+    import numpy as np
+    import torch
+
+    from sloam_amd import proposal, synth
+    from sloam_amd.gpu import CylinderFitPlan
+
+    dev = torch.device("cuda", 0)
+    out = {}
+    for n in sizes:
+        d = synth.make_clusters(clusters, seed=30 + n, n_min=n, n_max=n)
+        xyz = torch.as_tensor(d["xyz"]).to(dev)
+        off = torch.as_tensor(d["offsets"]).to(dev)
+        plain = CylinderFitPlan(ctx, d["offsets"], n_hyp=args.hyp, stream_tag=0)
+        pout = plain.alloc_outputs(mask=False, best=False)
+        fit = proposal.LearnedCylinderFit(ctx, net, d["offsets"], n_hyp=16, plain_hyp=args.hyp)
+        lout = fit.alloc()
+        torch.cuda.synchronize()
+        plain.exec(xyz, off, pout)
+        fit.exec(xyz, off, lout)
+        ms_plain = timed(lambda: plain.exec(xyz, off, pout))
+        ms_learned = timed(lambda: fit.exec(xyz, off, lout))
+        ref = pout["model"].cpu().numpy()
+        lm = lout["model"].cpu().numpy()
+        ok = (ref[:, 6] > 0) & (lm[:, 6] > 0)
+        err = np.abs(lm[ok, 6] - ref[ok, 6]) * 100
+        # the generator's size draw floors exp(log n): n or n - 1 points
+        out[str(int(np.median(np.diff(d["offsets"]))))] = {
+            "plain_fits_per_s": clusters / (ms_plain / 1e3),
+            "learned_fits_per_s": clusters / (ms_learned / 1e3),
+            "speedup_vs_plain": ms_plain / ms_learned,
+            "fallback_fraction": int(lout["n_fallback"].cpu().numpy()[0]) / clusters,
+            "radius_frac_over_1cm": float(np.mean(err > 1.0)) if err.size else None,
+        }
+        plain.close()
+        fit.close()
+    return out
 
 
 def bench_keyframes(args):

@@ -170,11 +170,12 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* plan, const float* d_xyz, const int32
  * min_inlier_ratio * n inliers is flagged in d_fallback (count in
  * d_n_fallback, nullable) and scored with the `plain` plan's hypotheses
  * (no proposal); one LO refit of every cluster from its best hypothesis of
- * the set it was scored with.  A flagged cluster's model equals the plain
- * plan's own fit of it, an unflagged one the verify plan's.  Both plans must
- * be built on the same offsets with the same thr, max_radius, seed and refit
- * settings.  d_counts_verify / d_counts_plain (nullable: plan scratch) hold
- * n_clusters x n_hyp counts of their plan.                                   */
+ * the set it was scored with, with that plan's refit_rounds / refit_iters.
+ * A flagged cluster's model equals the plain plan's own fit of it, an
+ * unflagged one the verify plan's.  Both plans must be built on the same
+ * offsets with the same thr, max_radius and seed.  d_counts_verify /
+ * d_counts_plain (nullable: plan scratch) hold n_clusters x n_hyp counts of
+ * their plan.                                                               */
 int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const float* d_xyz,
                            const int32_t* d_offsets, const float* d_proposal,
                            float min_inlier_ratio, int32_t* d_counts_verify,

@@ -528,8 +528,8 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
     uint32_t k1, uint32_t tag, int refit_rounds, int refit_iters,
     const float* __restrict__ prop, const int32_t* __restrict__ active,
     const int32_t* __restrict__ alt_flags, const int32_t* __restrict__ alt_counts, int alt_n_hyp,
-    uint32_t alt_tag, uint32_t* __restrict__ mask, int32_t* __restrict__ best_out,
-    double* __restrict__ model) {
+    uint32_t alt_tag, int alt_rounds, int alt_iters, uint32_t* __restrict__ mask,
+    int32_t* __restrict__ best_out, double* __restrict__ model) {
   constexpr int W = BLOCK / 64;
   __shared__ double red[(W + 1) * 21];  // bsum of up to 21 values
   __shared__ long long redk[W];
@@ -545,11 +545,14 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
   const int wave = threadIdx.x >> 6;
   double* out = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
   // a flagged cluster takes its hypotheses from the alternate set (its own
-  // counts, hypothesis count and sample stream, no proposal)
+  // counts, hypothesis count and sample stream, no proposal) and refits with
+  // the alternate round and step counts
   const bool alt = alt_flags && alt_flags[c];
   const int32_t* cnts = alt ? alt_counts : counts;
   const int nh = alt ? alt_n_hyp : n_hyp;
   const uint32_t tg = alt ? alt_tag : tag;
+  const int rounds = alt ? alt_rounds : refit_rounds;
+  const int iters = alt ? alt_iters : refit_iters;
   const float* pc = alt ? nullptr : cluster_prop(prop, c);
 
   // 1. best hypothesis: key = count * 2^32 + (2^32 - 1 - index), max wins
@@ -646,8 +649,8 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
   M.r = (double)h.r;
   const double dthr = (double)thr;
   bool stop = false;
-  for (int round = 0; round < refit_rounds && !stop; ++round) {
-    for (int it = 0; it < refit_iters; ++it) {
+  for (int round = 0; round < rounds && !stop; ++round) {
+    for (int it = 0; it < iters; ++it) {
       double e1[3], e2[3];
       sl_basis(M.a, e1, e2);
       double g[21];
@@ -815,7 +818,8 @@ static int round_up(int x, int m) { return (x + m - 1) / m * m; }
 static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_offsets,
                  const int32_t* counts, const float* d_proposal, const int32_t* d_active,
                  const int32_t* alt_flags, const int32_t* alt_counts, int alt_n_hyp,
-                 uint32_t alt_tag, uint32_t* d_mask, int32_t* d_best, double* d_model) {
+                 uint32_t alt_tag, int alt_rounds, int alt_iters, uint32_t* d_mask,
+                 int32_t* d_best, double* d_model) {
   sloam_ctx* ctx = p->ctx;
   const sloam_cyl_fit_cfg& cfg = p->cfg;
   const uint32_t k0 = (uint32_t)cfg.seed;
@@ -836,7 +840,7 @@ static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
 #define SLOAM_REFIT_ARGS                                                                      \
   d_xyz, d_offsets, p->d_frames, counts, list, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, \
       cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, alt_flags, alt_counts, alt_n_hyp, \
-      alt_tag, d_mask, d_best, d_model
+      alt_tag, alt_rounds, alt_iters, d_mask, d_best, d_model
     switch (kTierBlock[t]) {
       case 64:
         hipLaunchKernelGGL(cyl_refit_kernel<64>, g, dim3(64), lds, s, SLOAM_REFIT_ARGS);
@@ -1108,8 +1112,7 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
     return SLOAM_ERR_INVALID_ARG;
   const sloam_cyl_fit_cfg& vc = verify->cfg;
   const sloam_cyl_fit_cfg& pc = plain->cfg;
-  if (vc.thr != pc.thr || vc.max_radius != pc.max_radius || vc.seed != pc.seed ||
-      vc.refit_rounds != pc.refit_rounds || vc.refit_iters != pc.refit_iters)
+  if (vc.thr != pc.thr || vc.max_radius != pc.max_radius || vc.seed != pc.seed)
     return SLOAM_ERR_INVALID_ARG;
   if (verify->n_points > 0 && !d_xyz) return SLOAM_ERR_INVALID_ARG;
   sloam_ctx* ctx = verify->ctx;
@@ -1161,7 +1164,8 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
   {
     SloamKernelScope ks(ctx, "cyl_refit");
     const int st = launch_refit(verify, d_xyz, d_offsets, cv, d_proposal, nullptr, d_fallback, cp,
-                                pc.n_hyp, (uint32_t)pc.stream_tag, nullptr, d_best, d_model);
+                                pc.n_hyp, (uint32_t)pc.stream_tag, pc.refit_rounds,
+                                pc.refit_iters, nullptr, d_best, d_model);
     if (st) return st;
   }
   return SLOAM_OK;
@@ -1292,7 +1296,8 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
   {
     SloamKernelScope ks(ctx, "cyl_refit");
     const int st = launch_refit(p, d_xyz, d_offsets, counts, d_proposal, d_active, nullptr,
-                                nullptr, 0, 0u, d_mask, d_best, d_model);
+                                nullptr, 0, 0u, cfg.refit_rounds, cfg.refit_iters, d_mask, d_best,
+                                d_model);
     if (st) return st;
   }
   SLOAM_HIP(ctx, hipGetLastError());

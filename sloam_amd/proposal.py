@@ -108,12 +108,18 @@ class LearnedCylinderFit:
     cluster offsets (plans are built once)."""
 
     def __init__(self, ctx, net, offsets, n_hyp=16, plain_hyp=1024, thr=0.05, max_radius=1.0,
-                 min_inlier_ratio=0.5, seed=0x5EED, **kw):
+                 min_inlier_ratio=0.5, seed=0x5EED, verify_refit_rounds=None, **kw):
+        """verify_refit_rounds: LO refit rounds of the clusters that pass
+        verification (default: the plain fit's); the fallback clusters always
+        refit as the plain fit does."""
         self.ctx = ctx
         self.net = net
         self.min_inlier_ratio = float(min_inlier_ratio)
+        vkw = dict(kw)
+        if verify_refit_rounds is not None:
+            vkw["refit_rounds"] = int(verify_refit_rounds)
         self.verify = CylinderFitPlan(ctx, offsets, n_hyp=n_hyp, thr=thr, max_radius=max_radius,
-                                      seed=seed, stream_tag=1, **kw)
+                                      seed=seed, stream_tag=1, **vkw)
         self.plain = CylinderFitPlan(ctx, offsets, n_hyp=plain_hyp, thr=thr,
                                      max_radius=max_radius, seed=seed, stream_tag=0, **kw)
         self.n_clusters = self.verify.n_clusters

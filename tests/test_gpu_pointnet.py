@@ -88,23 +88,26 @@ def test_training_reduces_loss(ctx):
     assert last < 0.2 * first
 
 
-def test_learned_pipeline_fallback_matches_plain(ctx):
+@pytest.mark.parametrize("verify_rounds", [3, 1])
+def test_learned_pipeline_fallback_matches_plain(ctx, verify_rounds):
     """sloam_cyl_learned_exec with untrained weights (most proposals fail
     verification): a cluster is flagged exactly when its best verifying
     hypothesis holds fewer than half its points; every flagged cluster ends
-    with exactly the plain fit's model, every other one with exactly the
-    verify-only fit's model."""
+    with exactly the plain fit's model (its 3 refit rounds), every other one
+    with exactly the verify-only fit's model (verify_rounds refit rounds)."""
     d = synth.make_clusters(200, seed=13)
     x = torch.as_tensor(d["xyz"]).cuda()
     o = torch.as_tensor(d["offsets"].astype(np.int32)).cuda()
     net = proposal.ProposalNet(ctx, proposal.init_params(seed=6))
-    fit = proposal.LearnedCylinderFit(ctx, net, d["offsets"], n_hyp=16, plain_hyp=256)
+    fit = proposal.LearnedCylinderFit(ctx, net, d["offsets"], n_hyp=16, plain_hyp=256,
+                                      verify_refit_rounds=verify_rounds)
     out = fit.alloc()
     fit.exec(x, o, out)
     plain = CylinderFitPlan(ctx, d["offsets"], n_hyp=256, stream_tag=0)
     pout = plain.alloc_outputs(mask=False, best=False)
     plain.exec(x, o, pout)
-    verify = CylinderFitPlan(ctx, d["offsets"], n_hyp=16, stream_tag=1)
+    verify = CylinderFitPlan(ctx, d["offsets"], n_hyp=16, stream_tag=1,
+                             refit_rounds=verify_rounds)
     vout = verify.alloc_outputs(counts=True, mask=False, best=False)
     verify.exec(x, o, vout, proposal=out["proposal"])
     ctx.sync()
