@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -26,6 +27,7 @@
 
 #include "cylinder.h"
 #include "mapManager.h"
+#include "pcdIO.h"
 #include "definitions.h"
 #include "plane.h"
 #include "sloam.h"
@@ -525,6 +527,108 @@ void map_manager() {
   EXPECT_TRUE(ordered);
 }
 
+// PCD files (pcdIO.h): organised ascii and binary round trips with NaN
+// points, fields found by name in another order with extra fields of other
+// types and counts, and the failures PCL reports as -1.
+void pcd_io() {
+  const std::string dir = g_dir + "/";
+  CloudT c;
+  c.width = 4;
+  c.height = 3;
+  c.points.resize(12);
+  for (int i = 0; i < 12; ++i) {
+    c.points[i].x = 0.1f * i - 0.3f;
+    c.points[i].y = 1e-7f * i;
+    c.points[i].z = -12345.678f + i;
+    c.points[i].intensity = (float)(i % 5);
+  }
+  c.points[5].x = c.points[5].y = c.points[5].z = std::numeric_limits<float>::quiet_NaN();
+  c.is_dense = false;
+  auto same = [](const CloudT& a, const CloudT& b) {
+    if (a.width != b.width || a.height != b.height || a.points.size() != b.points.size() ||
+        a.is_dense != b.is_dense)
+      return false;
+    for (size_t i = 0; i < a.points.size(); ++i) {
+      const float va[4] = {a.points[i].x, a.points[i].y, a.points[i].z, a.points[i].intensity};
+      const float vb[4] = {b.points[i].x, b.points[i].y, b.points[i].z, b.points[i].intensity};
+      for (int k = 0; k < 4; ++k)
+        if (!(va[k] == vb[k] || (std::isnan(va[k]) && std::isnan(vb[k])))) return false;
+    }
+    return true;
+  };
+  CloudT r;
+  EXPECT_EQ(0, sloam_io::savePCDFileASCII(dir + "rt_ascii.pcd", c));
+  EXPECT_EQ(0, sloam_io::loadPCDFile(dir + "rt_ascii.pcd", r));
+  EXPECT_TRUE(same(c, r));
+  EXPECT_EQ(0, sloam_io::savePCDFileBinary(dir + "rt_bin.pcd", c));
+  EXPECT_EQ(0, sloam_io::loadPCDFile(dir + "rt_bin.pcd", r));
+  EXPECT_TRUE(same(c, r));
+  // fields in another order, an extra uint16 field and a two-element field
+  {
+    FILE* f = std::fopen((dir + "mixed.pcd").c_str(), "wb");
+    std::fprintf(f,
+                 "VERSION 0.7\nFIELDS intensity ring x y z t\nSIZE 4 2 4 8 4 4\n"
+                 "TYPE F U F F F F\nCOUNT 1 1 1 1 1 2\nWIDTH 2\nHEIGHT 1\nPOINTS 2\n"
+                 "DATA binary\n");
+    for (int i = 0; i < 2; ++i) {
+      const float in = 7.0f + i, x = 1.5f * i, z = -2.0f, t[2] = {9.f, 9.f};
+      const uint16_t ring = (uint16_t)(3 + i);
+      const double y = 0.25 + i;
+      std::fwrite(&in, 4, 1, f);
+      std::fwrite(&ring, 2, 1, f);
+      std::fwrite(&x, 4, 1, f);
+      std::fwrite(&y, 8, 1, f);
+      std::fwrite(&z, 4, 1, f);
+      std::fwrite(t, 4, 2, f);
+    }
+    std::fclose(f);
+    CloudT m;
+    EXPECT_EQ(0, sloam_io::loadPCDFile(dir + "mixed.pcd", m));
+    EXPECT_EQ((size_t)2, m.points.size());
+    if (m.points.size() == 2) {
+      EXPECT_EQ(1.5f, m.points[1].x);
+      EXPECT_EQ(1.25f, m.points[1].y);
+      EXPECT_EQ(-2.0f, m.points[1].z);
+      EXPECT_EQ(8.0f, m.points[1].intensity);
+      EXPECT_TRUE(m.is_dense);
+    }
+  }
+  // the reference's ascii header style with nan tokens and no intensity
+  {
+    FILE* f = std::fopen((dir + "ref.pcd").c_str(), "wb");
+    std::fprintf(f,
+                 "# .PCD v0.7 - Point Cloud Data file format\nVERSION 0.7\nFIELDS x y z\n"
+                 "SIZE 4 4 4\nTYPE F F F\nCOUNT 1 1 1\nWIDTH 3\nHEIGHT 1\n"
+                 "VIEWPOINT 0 0 0 1 0 0 0\nPOINTS 3\nDATA ascii\n1 2 3\nnan nan nan\n"
+                 "-2.8259385 2.4685013 1.4757745\n");
+    std::fclose(f);
+    CloudT a;
+    EXPECT_EQ(0, sloam_io::loadPCDFile(dir + "ref.pcd", a));
+    EXPECT_EQ((size_t)3, a.points.size());
+    if (a.points.size() == 3) {
+      EXPECT_TRUE(std::isnan(a.points[1].x));
+      EXPECT_EQ(-2.8259385f, a.points[2].x);
+      EXPECT_EQ(0.0f, a.points[2].intensity);
+      EXPECT_TRUE(!a.is_dense);
+    }
+  }
+  // failures: no file, compressed data, POINTS != WIDTH * HEIGHT
+  CloudT e;
+  EXPECT_EQ(-1, sloam_io::loadPCDFile(dir + "missing.pcd", e));
+  {
+    FILE* f = std::fopen((dir + "bad.pcd").c_str(), "wb");
+    std::fprintf(f, "FIELDS x y z\nSIZE 4 4 4\nTYPE F F F\nWIDTH 1\nPOINTS 1\n"
+                    "DATA binary_compressed\n");
+    std::fclose(f);
+    EXPECT_EQ(-1, sloam_io::loadPCDFile(dir + "bad.pcd", e));
+    f = std::fopen((dir + "bad2.pcd").c_str(), "wb");
+    std::fprintf(f, "FIELDS x y z\nSIZE 4 4 4\nTYPE F F F\nWIDTH 2\nHEIGHT 2\nPOINTS 3\n"
+                    "DATA ascii\n1 2 3\n");
+    std::fclose(f);
+    EXPECT_EQ(-1, sloam_io::loadPCDFile(dir + "bad2.pcd", e));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // trellis: the reference's tree clouds give its landmark archives
 // (sloam/src/tests/aux/<seq>_tree_<t>.pcd -> <seq>_landmarks_<t>); the
@@ -782,6 +886,7 @@ int main(int argc, char** argv) {
       {"SE3", false, test_se3},
       {"CylinderTest.FromCylinderFit", false, cylinder_from_fit},
       {"MapManager", false, map_manager},
+      {"PCD.ReadWrite", false, pcd_io},
       {"PlaneTest.Initalizes", true, plane_initializes},
       {"PlaneTest.DistanceToFeature", true, plane_distance_to_feature},
       {"PlaneTest.TranslateModel", true, plane_translate_model},
