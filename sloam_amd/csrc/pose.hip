@@ -36,6 +36,17 @@ namespace {
 constexpr int kPBlock = 256;
 constexpr int kRpt = 8;  // residuals per thread per workgroup
 constexpr int kSums = 28;
+// the chunk reduction: kParts threads per sum add kPBlock / kParts lanes'
+// values each, then one thread per sum adds the kParts partials
+constexpr int kParts = 8;
+static_assert(kSums * kParts <= kPBlock && kPBlock % kParts == 0, "chunk reduction layout");
+
+// LDS of the chunk reduction: every thread's sums, then the partials
+struct ChunkRed {
+  float x[kSums][kPBlock + 1];
+  float ps[kSums][kParts];
+  float pc[kSums][kParts];
+};
 
 struct LMState {
   double x[7];   // accepted pose: t, q (x, y, z, w)
@@ -301,8 +312,7 @@ __device__ inline void eval_frame(const LMState& s, int param, EvalFrame& F) {
 // chunk's 28 sums as FP64 (sum + compensation) in threads 0..27.
 __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e,
                                     const EvalFrame& F, const float4* __restrict__ cylf,
-                                    const float4* __restrict__ plf, float delta,
-                                    float (*red_s)[kSums], float (*red_c)[kSums]) {
+                                    const float4* __restrict__ plf, float delta, ChunkRed& red) {
   const float d2 = delta * delta;
   float acc[kSums];
 #pragma unroll
@@ -381,44 +391,39 @@ __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e
     for (int a = 0; a < 6; ++a) acc[21 + a] = __builtin_fmaf(wJ[a], r, acc[21 + a]);
     acc[27] = __builtin_fmaf(0.5f, rho, acc[27]);
   }
-  // error-free tree reduction: (sum, compensation) pairs
-  float cmp[kSums];
+  // error-free reduction in a fixed order: every thread's sums to LDS; each of
+  // kParts threads per sum runs a TwoSum chain over kPBlock / kParts lanes;
+  // one thread per sum chains the kParts (sum, compensation) pairs
 #pragma unroll
-  for (int i = 0; i < kSums; ++i) cmp[i] = 0.f;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-    for (int i = 0; i < kSums; ++i) {
-      const float os = __shfl_xor(acc[i], o, 64);
-      const float oc = __shfl_xor(cmp[i], o, 64);
-      float s, err;
-      two_sum(acc[i], os, &s, &err);
-      acc[i] = s;
-      cmp[i] = cmp[i] + oc + err;
+  for (int i = 0; i < kSums; ++i) red.x[i][threadIdx.x] = acc[i];
+  __syncthreads();
+  if (threadIdx.x < kSums * kParts) {
+    const int i = threadIdx.x % kSums, part = threadIdx.x / kSums;
+    const float* row = &red.x[i][part * (kPBlock / kParts)];
+    float sm = row[0], c = 0.f;
+    for (int j = 1; j < kPBlock / kParts; ++j) {
+      float ns, err;
+      two_sum(sm, row[j], &ns, &err);
+      sm = ns;
+      c += err;
     }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < kSums; ++i) {
-      red_s[wave][i] = acc[i];
-      red_c[wave][i] = cmp[i];
-    }
+    red.ps[i][part] = sm;
+    red.pc[i][part] = c;
   }
   __syncthreads();
   double out = 0.0;
   if (threadIdx.x < kSums) {
     const int i = threadIdx.x;
-    float s = red_s[0][i], c = red_c[0][i];
-    for (int v = 1; v < kPBlock / 64; ++v) {
+    float sm = red.ps[i][0], c = red.pc[i][0];
+    for (int v = 1; v < kParts; ++v) {
       float ns, err;
-      two_sum(s, red_s[v][i], &ns, &err);
-      s = ns;
-      c = c + red_c[v][i] + err;
+      two_sum(sm, red.ps[i][v], &ns, &err);
+      sm = ns;
+      c = c + red.pc[i][v] + err;
     }
-    out = (double)s + (double)c;
+    out = (double)sm + (double)c;
   }
-  __syncthreads();  // red_s / red_c are reused by the next chunk
+  __syncthreads();  // red is reused by the next chunk
   return out;
 }
 
@@ -427,14 +432,13 @@ __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
     const float4* __restrict__ pts, const int4* __restrict__ work, const LMState* __restrict__ st,
     const float4* __restrict__ cylf, const float4* __restrict__ plf, float delta, int param,
     double* __restrict__ partials) {
-  __shared__ float red_s[kPBlock / 64][kSums];
-  __shared__ float red_c[kPBlock / 64][kSums];
+  __shared__ ChunkRed red;
   const int4 w = work[blockIdx.x];
   const int k = w.x;
   if (st[k].done) return;
   EvalFrame F;
   eval_frame(st[k], param, F);
-  const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red_s, red_c);
+  const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red);
   if (threadIdx.x < kSums) partials[(size_t)blockIdx.x * kSums + threadIdx.x] = p;
 }
 
@@ -621,8 +625,7 @@ __global__ __launch_bounds__(kPBlock) void pose_fused_kernel(
     const float4* __restrict__ plf, PoseCfgDev cfg, float delta, LMState* __restrict__ st,
     double* __restrict__ pose_out, int32_t* __restrict__ status, double* __restrict__ cost,
     double* __restrict__ params_out) {
-  __shared__ float red_s[kPBlock / 64][kSums];
-  __shared__ float red_c[kPBlock / 64][kSums];
+  __shared__ ChunkRed red;
   __shared__ double tot[kSums];
   __shared__ LMState sS;
   const int k = blockIdx.x;
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(kPBlock) void pose_fused_kernel(
     double acc = 0.0;
     for (int wk = w0; wk < w1; ++wk) {
       const int4 w = work[wk];
-      const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red_s, red_c);
+      const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red);
       if (threadIdx.x < kSums) acc += p;
     }
     if (threadIdx.x < kSums) tot[threadIdx.x] = acc;
