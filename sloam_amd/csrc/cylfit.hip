@@ -54,6 +54,8 @@ constexpr int kTiers = 2;
 constexpr int kTierMax[kTiers] = {768, 1 << 30};
 constexpr int kTierMaxSmall[kTiers] = {256, 1 << 30};
 constexpr int kTierBlock[kTiers] = {64, 256};
+static_assert(kTierBlock[0] == 64 && kTierBlock[kTiers - 1] == 256,
+              "launch_refit instantiates the refit kernel for 64 and 256 threads");
 static_assert(kTiers >= 1 && kTiers <= 3, "one stream per tier: the context stream and two auxiliary ones");
 
 struct CylFrame {
@@ -510,6 +512,49 @@ __device__ inline void bsum(double (&v)[K], double* lds) {
   for (int k = 0; k < K; ++k) v[k] = lds[W * K + k];
 }
 
+// Sum over the workgroup of K doubles per thread, every thread getting the
+// totals, through LDS: each thread's values are stored, P threads per value
+// add BLOCK / P threads' values each in a fixed order, and K threads add the
+// P partials (the wave shuffles of bsum move each double through two LDS
+// permutes per step, six steps per value).
+template <int BLOCK, int K>
+struct BsumLds {
+  static constexpr int P = (BLOCK / K) < 16 ? (BLOCK / K) : 16;  // threads per value
+  static constexpr int L = (BLOCK + P - 1) / P;                   // values per thread
+  double x[K][BLOCK + 1];
+  double part[K][P];
+  double tot[K];
+};
+
+template <int BLOCK, int K>
+__device__ inline void bsum_lds(double (&v)[K], BsumLds<BLOCK, K>& r) {
+  constexpr int P = BsumLds<BLOCK, K>::P;
+  constexpr int L = BsumLds<BLOCK, K>::L;
+  static_assert(P >= 1 && K * P <= BLOCK, "bsum_lds layout");
+  const int t = threadIdx.x;
+  __syncthreads();  // r may still be read from the previous call
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.x[k][t] = v[k];
+  __syncthreads();
+  if (t < K * P) {
+    const int k = t % K, p = t / K;
+    const int b = p * L, e = min(BLOCK, b + L);
+    double s = 0.0;
+    for (int j = b; j < e; ++j) s += r.x[k][j];
+    r.part[k][p] = s;
+  }
+  __syncthreads();
+  if (t < K) {
+    double s = r.part[t][0];
+#pragma unroll
+    for (int p = 1; p < P; ++p) s += r.part[t][p];
+    r.tot[t] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = r.tot[k];
+}
+
 // ---------------------------------------------------------------------------
 // refit: best hypothesis, mask, FP64 LO-RANSAC cylinder refit.
 // One workgroup per cluster, its size by the cluster's size (refit tiers).
@@ -532,6 +577,7 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
     int32_t* __restrict__ best_out, double* __restrict__ model) {
   constexpr int W = BLOCK / 64;
   __shared__ double red[(W + 1) * 21];  // bsum of up to 21 values
+  __shared__ BsumLds<BLOCK, 21> red21;   // the Gauss-Newton sums
   __shared__ long long redk[W];
   __shared__ HypOut best_h;
   extern __shared__ uint32_t bits[];  // membership of the current round
@@ -700,7 +746,7 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
           }
         }
       }
-      bsum<BLOCK, 21>(g, red);
+      bsum_lds<BLOCK, 21>(g, red21);
       if (g[20] < 5.0) {
         stop = true;
         break;
@@ -841,20 +887,10 @@ static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
   d_xyz, d_offsets, p->d_frames, counts, list, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, \
       cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, alt_flags, alt_counts, alt_n_hyp, \
       alt_tag, alt_rounds, alt_iters, d_mask, d_best, d_model
-    switch (kTierBlock[t]) {
-      case 64:
-        hipLaunchKernelGGL(cyl_refit_kernel<64>, g, dim3(64), lds, s, SLOAM_REFIT_ARGS);
-        break;
-      case 256:
-        hipLaunchKernelGGL(cyl_refit_kernel<256>, g, dim3(256), lds, s, SLOAM_REFIT_ARGS);
-        break;
-      case 512:
-        hipLaunchKernelGGL(cyl_refit_kernel<512>, g, dim3(512), lds, s, SLOAM_REFIT_ARGS);
-        break;
-      default:
-        hipLaunchKernelGGL(cyl_refit_kernel<1024>, g, dim3(1024), lds, s, SLOAM_REFIT_ARGS);
-        break;
-    }
+    if (kTierBlock[t] == 64)
+      hipLaunchKernelGGL(cyl_refit_kernel<64>, g, dim3(64), lds, s, SLOAM_REFIT_ARGS);
+    else
+      hipLaunchKernelGGL(cyl_refit_kernel<256>, g, dim3(256), lds, s, SLOAM_REFIT_ARGS);
 #undef SLOAM_REFIT_ARGS
     SLOAM_HIP(ctx, hipGetLastError());
   }
