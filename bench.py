@@ -380,9 +380,8 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(the batch would shard by clusters with no exchange; see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU only (a batch would shard by clusters, "
+              "size-balanced, with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

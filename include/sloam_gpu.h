@@ -6,7 +6,7 @@
  * boundary is the C++ class API (sloam::sloam::RunSloam, Cylinder, Plane,
  * seg::Segmentation).  The C++ drop-in classes in sloam_amd/host/ keep those
  * names and call the entry points below; any other host language binds them
- * directly (ctypes / cgo / JNI stubs are shown in INTEGRATION.md).
+ * directly (plain C types and pointers: ctypes, cgo, JNI, N-API).
  *
  * Conventions
  *   - every function returns int status: 0 = OK, < 0 = error (sloam_strerror);

@@ -20,7 +20,7 @@
 //   * optimisation: float centroid and scatter matrix of the inliers, the
 //     eigenvector of the largest eigenvalue by the cross product of two rows of
 //     (C - lambda I) (the largest of the three products), re-selected inliers.
-// This build takes lambda from an FP64 Jacobi solve instead of PCL's
+// Here lambda comes from an FP64 Jacobi solve instead of PCL's
 // closed-form float eigen33 roots.
 #include <cfloat>
 #include <cmath>
