@@ -105,6 +105,13 @@ void KeyframeRunner::setTreeModel(TreeModel mode, const std::string& weights_npy
     hip_check(hipMemcpy(d_params_.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice),
               "hipMemcpy(weights)");
   }
+  // the front end fits the trees' median lines only for the model that
+  // uses them
+  const int want_lines = mode == TreeModel::MedianLines ? 1 : 0;
+  if (want_lines != fe_cfg_.fit_lines) {
+    fe_cfg_.fit_lines = want_lines;
+    makeFrontend();
+  }
   mode_ = mode;
 }
 
@@ -239,13 +246,25 @@ KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& 
   cfg.use_graph = use_graph ? 1 : 0;
   // results land in pinned host memory at the end of every front-end run
   cfg.host_results = 1;
-  // every tree's axis line, fitted inside the front end
+  // every tree's axis line, fitted inside the front end (the median-line
+  // tree model; the cylinder-fit models rebuild the plan without them)
   cfg.line = Cylinder::lineFitCfg();
   cfg.fit_lines = 1;
-  check(sloam_frontend_create(sloam_gpu::context(), &cfg, &fe_), "sloam_frontend_create");
+  fe_cfg_ = cfg;
+  makeFrontend();
+  sloam_.setFmParams(fmParams);
+}
+
+void KeyframeRunner::makeFrontend() {
+  sloam_ctx* ctx = sloam_gpu::context();
+  if (fe_) {
+    check(sloam_sync(ctx), "sloam_sync");
+    sloam_frontend_destroy(fe_);
+    fe_ = nullptr;
+  }
+  check(sloam_frontend_create(ctx, &fe_cfg_, &fe_), "sloam_frontend_create");
   check(sloam_frontend_buffers(fe_, &buf_), "sloam_frontend_buffers");
   check(sloam_frontend_host_view(fe_, &host_), "sloam_frontend_host_view");
-  sloam_.setFmParams(fmParams);
 }
 
 KeyframeRunner::~KeyframeRunner() {
@@ -310,11 +329,13 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
       tree.push_back(std::move(x));
     }
     in.landmarks.push_back(std::move(tree));
-    LineFit fit;
-    fit.ok = hv.line_stats[4 * (size_t)t] != 0;
-    fit.inliers = hv.line_stats[4 * (size_t)t + 1];
-    for (int i = 0; i < 6; ++i) fit.coeff[i] = hv.line_coeff[6 * (size_t)t + i];
-    in.treeLineFits.push_back(fit);
+    if (fe_cfg_.fit_lines) {
+      LineFit fit;
+      fit.ok = hv.line_stats[4 * (size_t)t] != 0;
+      fit.inliers = hv.line_stats[4 * (size_t)t + 1];
+      for (int i = 0; i < 6; ++i) fit.coeff[i] = hv.line_coeff[6 * (size_t)t + i];
+      in.treeLineFits.push_back(fit);
+    }
   }
   for (int c = 0; c < nc; ++c) {
     PlaneParameters pp;

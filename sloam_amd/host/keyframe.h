@@ -79,9 +79,12 @@ class KeyframeRunner {
   };
   // cylinder fits of the nt trees of the last front-end run into fits
   void fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTiming* timing);
+  // (re)creates the front-end plan from fe_cfg_
+  void makeFrontend();
 
   int H_, W_;
   sloam_frontend* fe_ = nullptr;
+  sloam_frontend_cfg fe_cfg_{};
   sloam_frontend_out buf_{};
   sloam_frontend_host host_{};
   FeatureModelParams fm_;
