@@ -5,7 +5,7 @@
 //
 //   keyframe_bench <sequence file> [passes] [use_graph 0/1]
 //                  [--tree-model lines|plain|learned] [--weights <npy>]
-//                  [--traj-out <file>] [--log] [--pinned]
+//                  [--traj-out <file>] [--log] [--pinned] [--pipelined]
 //
 // --tree-model picks how each tree's model is made (KeyframeRunner's
 // TreeModel; default lines, the reference's); --traj-out writes the first
@@ -14,7 +14,11 @@
 // keyframes (trees, matches, pose error before and after the solve) to
 // stderr; --pinned writes each scan into the runner's pinned input buffers
 // (as a sensor driver would) before its timed call, instead of passing the
-// sequence's pageable arrays.
+// sequence's pageable arrays; --pipelined submits each scan's front end
+// before the previous keyframe's estimator runs (KeyframeRunner::submit /
+// finish on two front-end slots).  Throughput is keyframes per second of
+// each timed pass (the copies into the pinned buffers, which a sensor driver
+// would make, are not counted).
 //
 // Sequence file (little endian): int32 K, H, W; then per keyframe:
 // float64 true pose [7] (t, q x y z w; map <- sensor), float64 odometry
@@ -62,13 +66,15 @@ double pct(std::vector<double> v, double q) {
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
   std::string model_name = "lines", weights, traj_out;
-  bool log = false, pinned = false;
+  bool log = false, pinned = false, pipelined = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if (a == "--log") {
       log = true;
     } else if (a == "--pinned") {
       pinned = true;
+    } else if (a == "--pipelined") {
+      pipelined = true;
     } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out") && i + 1 < argc) {
       (a == "--tree-model" ? model_name : a == "--weights" ? weights : traj_out) = argv[++i];
     } else if (a.rfind("--", 0) == 0) {
@@ -129,10 +135,34 @@ int main(int argc, char** argv) {
   std::vector<double> fit, traj;
   long long fitted = 0, fallback = 0;
   int discarded = 0;
+  std::vector<double> pass_rate;  // keyframes per second of each timed pass
   try {
     for (int pass = 0; pass < passes; ++pass) {
-      KeyframeRunner kf(H, W, fm, use_graph);
+      KeyframeRunner kf(H, W, fm, use_graph, 22.5f, -22.5f, pipelined);
       kf.setTreeModel(tree_model, weights);
+      double staging_ms = 0.0;  // copies into the pinned buffers (not counted)
+      // hands scan j to the runner: into its pinned buffers (--pinned), then
+      // submitted (--pipelined) or returned for run()
+      auto stage = [&](int j, const float** xp, const uint8_t** lp) {
+        *xp = seq[j].xyzi.data();
+        *lp = seq[j].label.data();
+        if (pinned) {
+          const auto c0 = std::chrono::steady_clock::now();
+          std::memcpy(kf.pinnedXyzi(), *xp, sizeof(float) * seq[j].xyzi.size());
+          std::memcpy(kf.pinnedLabels(), *lp, seq[j].label.size());
+          *xp = kf.pinnedXyzi();
+          *lp = kf.pinnedLabels();
+          staging_ms += std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - c0)
+                            .count();
+        }
+        if (pipelined) kf.submit(*xp, *lp);
+      };
+      const float* xyzi = nullptr;
+      const uint8_t* label = nullptr;
+      // every keyframe of the pass, the first (map start) included
+      const auto tpass = std::chrono::steady_clock::now();
+      if (pipelined) stage(0, &xyzi, &label);
       SE3 estimate = to_se3(seq[0].truth);
       MapManager semanticMap;
       bool nodeFirst = true;  // SLOAMNode's firstScan_
@@ -144,20 +174,19 @@ int main(int argc, char** argv) {
         std::vector<Cylinder> map;
         semanticMap.getSubmap(estimate, map);
         const bool discard = !nodeFirst && map.empty();
-        const float* xyzi = seq[k].xyzi.data();
-        const uint8_t* label = seq[k].label.data();
-        if (pinned) {
-          // the scan arrives in the pinned buffers (not timed)
-          std::memcpy(kf.pinnedXyzi(), xyzi, sizeof(float) * seq[k].xyzi.size());
-          std::memcpy(kf.pinnedLabels(), label, seq[k].label.size());
-          xyzi = kf.pinnedXyzi();
-          label = kf.pinnedLabels();
+        if (pipelined) {
+          // the next scan's front end runs while this keyframe's estimator does
+          if (k + 1 < K) stage(k + 1, &xyzi, &label);
+        } else {
+          stage(k, &xyzi, &label);
         }
         bool ok = false;
         if (discard) {
           if (pass == 0) ++discarded;
+          if (pipelined) kf.skip();
         } else {
-          ok = kf.run(xyzi, label, estimate, map, out, &t);
+          ok = pipelined ? kf.finish(estimate, map, out, &t)
+                         : kf.run(xyzi, label, estimate, map, out, &t);
           semanticMap.updateMap(out.tm, out.matches);
           if (ok && nodeFirst && !out.tm.empty()) nodeFirst = false;
         }
@@ -225,6 +254,13 @@ int main(int argc, char** argv) {
         ++samples;
         ok_count += ok ? 1 : 0;
       }
+      if (pass > 0 || passes == 1) {
+        const double ms = std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - tpass)
+                              .count() -
+                          staging_ms;
+        if (ms > 0) pass_rate.push_back(K / (ms / 1e3));
+      }
     }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "keyframe_bench: %s\n", e.what());
@@ -241,8 +277,10 @@ int main(int argc, char** argv) {
   }
   std::printf(
       "{\"tree_model\": \"%s\", \"pinned_input\": %s, \"tree_fit_p50_ms\": %.4f, "
-      "\"trees_fitted\": %lld, \"trees_fallback\": %lld, \"discarded\": %d, ",
-      model_name.c_str(), pinned ? "true" : "false", pct(fit, 0.5), fitted, fallback, discarded);
+      "\"trees_fitted\": %lld, \"trees_fallback\": %lld, \"discarded\": %d, "
+      "\"pipelined\": %s, \"throughput_kf_per_s\": %.1f, ",
+      model_name.c_str(), pinned ? "true" : "false", pct(fit, 0.5), fitted, fallback, discarded,
+      pipelined ? "true" : "false", pct(pass_rate, 0.5));
   std::printf(
       "\"keyframes\": %d, \"passes\": %d, \"height\": %d, \"width\": %d, \"graph\": %s, "
       "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"mean\": %.4f, \"max\": %.4f}, "

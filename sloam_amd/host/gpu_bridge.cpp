@@ -144,11 +144,16 @@ void down(std::vector<T>& v, const Dev& d) {
 
 }  // namespace
 
+int deviceIndex() {
+  int dev = 0;
+  if (const char* e = std::getenv("SLOAM_DEVICE")) dev = std::atoi(e);
+  return dev;
+}
+
 sloam_ctx* context() {
   thread_local CtxHolder holder;
   if (!holder.ctx) {
-    int dev = 0;
-    if (const char* e = std::getenv("SLOAM_DEVICE")) dev = std::atoi(e);
+    const int dev = deviceIndex();
     const int st = sloam_ctx_create(dev, &holder.ctx);
     if (st != SLOAM_OK) fail("sloam_ctx_create", st, nullptr);
   }

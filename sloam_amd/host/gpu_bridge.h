@@ -13,6 +13,9 @@
 namespace sloam_gpu {
 
 sloam_ctx* context();
+// the device the contexts of this library's host code use (0, or
+// $SLOAM_DEVICE)
+int deviceIndex();
 
 struct CellFit {
   bool valid = false;

@@ -86,11 +86,12 @@ KeyframeRunner::DevBuf::~DevBuf() {
 }
 
 float* KeyframeRunner::pinnedXyzi() {
-  if (!h_in_) {
+  Slot& sl = slots_[next_];
+  if (!sl.h_in) {
     const size_t N = (size_t)H_ * W_;
-    hip_check(hipHostMalloc(&h_in_, 17 * N), "hipHostMalloc");
+    hip_check(hipHostMalloc(&sl.h_in, 17 * N), "hipHostMalloc");
   }
-  return static_cast<float*>(h_in_);
+  return static_cast<float*>(sl.h_in);
 }
 
 uint8_t* KeyframeRunner::pinnedLabels() {
@@ -115,10 +116,11 @@ void KeyframeRunner::setTreeModel(TreeModel mode, const std::string& weights_npy
   mode_ = mode;
 }
 
-void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTiming* timing) {
+void KeyframeRunner::fitTrees(const Slot& sl, int nt, std::vector<CylinderFit>& fits,
+                              KeyframeTiming* timing) {
   sloam_ctx* ctx = sloam_gpu::context();
   const auto t0 = Clock::now();
-  const int32_t *to = host_.tree_offsets, *po = host_.point_offsets;
+  const int32_t *to = sl.host.tree_offsets, *po = sl.host.point_offsets;
   // tree t's points are those of its vertices, contiguous in the front end's
   // point list
   const int32_t base = po[to[0]];
@@ -182,7 +184,8 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
   int32_t* n_fallback = reinterpret_cast<int32_t*>(model + SLOAM_CYL_MODEL_STRIDE * (size_t)nt);
   check(sloam_memcpy_async(ctx, off, h_off_, sizeof(int32_t) * ((size_t)nt + 1)),
         "upload tree offsets");
-  check(sloam_xyz_from_xyzi(ctx, buf_.points + 4 * (size_t)base, np, xyz), "sloam_xyz_from_xyzi");
+  check(sloam_xyz_from_xyzi(ctx, sl.buf.points + 4 * (size_t)base, np, xyz),
+        "sloam_xyz_from_xyzi");
   if (learned) {
     float* prop = static_cast<float*>(d_prop_.p);
     check(sloam_pn_infer(ctx, static_cast<const float*>(d_params_.p), xyz, off, nt, prop),
@@ -220,7 +223,8 @@ void KeyframeRunner::fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTi
 }
 
 KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& fmParams,
-                               bool use_graph, float fov_up_deg, float fov_down_deg)
+                               bool use_graph, float fov_up_deg, float fov_down_deg,
+                               bool pipelined)
     : H_(height), W_(width), fm_(fmParams) {
   sloam_frontend_cfg cfg{};
   cfg.proj.height = height;
@@ -251,55 +255,118 @@ KeyframeRunner::KeyframeRunner(int height, int width, const FeatureModelParams& 
   cfg.line = Cylinder::lineFitCfg();
   cfg.fit_lines = 1;
   fe_cfg_ = cfg;
+  // one slot on the estimator's context, or two with streams of their own
+  slots_.resize(pipelined ? 2 : 1);
+  for (Slot& sl : slots_) {
+    if (pipelined) {
+      check(sloam_ctx_create(sloam_gpu::deviceIndex(), &sl.ctx), "sloam_ctx_create");
+      sl.own_ctx = true;
+    } else {
+      sl.ctx = sloam_gpu::context();
+    }
+    hipEvent_t ev = nullptr;
+    hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+    sl.done = ev;
+  }
   makeFrontend();
   sloam_.setFmParams(fmParams);
 }
 
 void KeyframeRunner::makeFrontend() {
-  sloam_ctx* ctx = sloam_gpu::context();
-  if (fe_) {
-    check(sloam_sync(ctx), "sloam_sync");
-    sloam_frontend_destroy(fe_);
-    fe_ = nullptr;
+  if (pending_) throw std::runtime_error("keyframe: front-end slots are busy");
+  for (Slot& sl : slots_) {
+    if (sl.fe) {
+      check(sloam_sync(sl.ctx), "sloam_sync");
+      sloam_frontend_destroy(sl.fe);
+      sl.fe = nullptr;
+    }
+    check(sloam_frontend_create(sl.ctx, &fe_cfg_, &sl.fe), "sloam_frontend_create");
+    check(sloam_frontend_buffers(sl.fe, &sl.buf), "sloam_frontend_buffers");
+    check(sloam_frontend_host_view(sl.fe, &sl.host), "sloam_frontend_host_view");
   }
-  check(sloam_frontend_create(ctx, &fe_cfg_, &fe_), "sloam_frontend_create");
-  check(sloam_frontend_buffers(fe_, &buf_), "sloam_frontend_buffers");
-  check(sloam_frontend_host_view(fe_, &host_), "sloam_frontend_host_view");
 }
 
 KeyframeRunner::~KeyframeRunner() {
   sloam_cyl_plan_destroy(plain_);
   sloam_cyl_plan_destroy(verify_);
   if (h_model_) (void)hipHostFree(h_model_);
-  if (h_in_) (void)hipHostFree(h_in_);
+
   if (h_off_) (void)hipHostFree(h_off_);
-  sloam_frontend_destroy(fe_);
+  for (Slot& sl : slots_) {
+    if (sl.ctx) (void)sloam_sync(sl.ctx);
+    sloam_frontend_destroy(sl.fe);
+    if (sl.done) (void)hipEventDestroy((hipEvent_t)sl.done);
+    if (sl.h_in) (void)hipHostFree(sl.h_in);
+    if (sl.own_ctx) sloam_ctx_destroy(sl.ctx);
+  }
 }
 
 bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
                          const std::vector<Cylinder>& mapModels, SloamOutput& out,
                          KeyframeTiming* timing) {
-  sloam_ctx* ctx = sloam_gpu::context();
+  if (pending_) throw std::runtime_error("keyframe: run() with submitted scans pending");
+  Slot& sl = slots_[next_];
+  next_ = (next_ + 1) % (int)slots_.size();
   const size_t N = (size_t)H_ * W_;
+  const auto t0 = Clock::now();
+  check(sloam_memcpy_async(sl.ctx, sl.buf.xyzi, xyzi, 16 * N), "upload xyzi");
+  check(sloam_memcpy_async(sl.ctx, sl.buf.label, labels, N), "upload labels");
+  check(sloam_sync(sl.ctx), "sloam_sync");
+  const auto t1 = Clock::now();
+  check(sloam_frontend_exec(sl.fe), "sloam_frontend_exec");
+  check(sloam_sync(sl.ctx), "sloam_sync");
+  return estimate(sl, poseEstimate, mapModels, out, timing, t0, t1, Clock::now());
+}
+
+void KeyframeRunner::submit(const float* xyzi, const uint8_t* labels) {
+  if (pending_ == (int)slots_.size())
+    throw std::runtime_error("keyframe: every front-end slot is busy (finish() first)");
+  Slot& sl = slots_[next_];
+  next_ = (next_ + 1) % (int)slots_.size();
+  const size_t N = (size_t)H_ * W_;
+  sl.t_submit = Clock::now();
+  check(sloam_memcpy_async(sl.ctx, sl.buf.xyzi, xyzi, 16 * N), "upload xyzi");
+  check(sloam_memcpy_async(sl.ctx, sl.buf.label, labels, N), "upload labels");
+  check(sloam_frontend_exec(sl.fe), "sloam_frontend_exec");
+  hip_check(hipEventRecord((hipEvent_t)sl.done, (hipStream_t)sloam_ctx_stream(sl.ctx)),
+            "hipEventRecord");
+  ++pending_;
+}
+
+bool KeyframeRunner::finish(const SE3& poseEstimate, const std::vector<Cylinder>& mapModels,
+                            SloamOutput& out, KeyframeTiming* timing) {
+  if (!pending_) throw std::runtime_error("keyframe: finish() with no submitted scan");
+  const int n = (int)slots_.size();
+  Slot& sl = slots_[(next_ - pending_ + n) % n];
+  hip_check(hipEventSynchronize((hipEvent_t)sl.done), "hipEventSynchronize");
+  --pending_;
+  return estimate(sl, poseEstimate, mapModels, out, timing, sl.t_submit, sl.t_submit,
+                  Clock::now());
+}
+
+void KeyframeRunner::skip() {
+  if (!pending_) throw std::runtime_error("keyframe: skip() with no submitted scan");
+  const int n = (int)slots_.size();
+  Slot& sl = slots_[(next_ - pending_ + n) % n];
+  hip_check(hipEventSynchronize((hipEvent_t)sl.done), "hipEventSynchronize");
+  --pending_;
+}
+
+bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
+                              const std::vector<Cylinder>& mapModels, SloamOutput& out,
+                              KeyframeTiming* timing, Clock::time_point t0,
+                              Clock::time_point t1, Clock::time_point t2) {
   if (timing) {
     timing->fit_ms = 0;
     timing->fitted = 0;
     timing->fallback = 0;
   }
-  const auto t0 = Clock::now();
-  check(sloam_memcpy_async(ctx, buf_.xyzi, xyzi, 16 * N), "upload xyzi");
-  check(sloam_memcpy_async(ctx, buf_.label, labels, N), "upload labels");
-  check(sloam_sync(ctx), "sloam_sync");
-  const auto t1 = Clock::now();
-  check(sloam_frontend_exec(fe_), "sloam_frontend_exec");
-  check(sloam_sync(ctx), "sloam_sync");
-  const auto t2 = Clock::now();
   // results: already in pinned host memory (written by the front end's last
   // kernel)
-  const sloam_frontend_host& hv = host_;
+  const sloam_frontend_host& hv = sl.host;
   const int32_t* cnt = hv.trellis_counts;
   if (cnt[4]) throw std::runtime_error("keyframe: trellis output capacity exceeded");
-  const int nt = cnt[0], nc = buf_.n_cells;
+  const int nt = cnt[0], nc = sl.buf.n_cells;
   const int nf = fm_.numGroundFeatures;
   const int32_t *to = hv.tree_offsets, *vi = hv.vertex_int, *po = hv.point_offsets,
                 *info = hv.cell_info;
@@ -352,7 +419,7 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
   }
   if (mode_ != TreeModel::MedianLines && nt > 0) {
     // the tree models from cylinder fits of the trees' points
-    fitTrees(nt, in.treeCylinderFits, timing);
+    fitTrees(sl, nt, in.treeCylinderFits, timing);
     in.treeLineFits.clear();
   }
   const auto t3b = Clock::now();

@@ -6,6 +6,7 @@
 // segmentation network.
 #pragma once
 
+#include <chrono>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -41,9 +42,11 @@ enum class TreeModel {
 class KeyframeRunner {
  public:
   // height x width organised scans; fmParams for the estimator; use_graph
-  // replays the front end as a HIP graph
+  // replays the front end as a HIP graph; pipelined gives the front end two
+  // slots on streams of their own, so that submit() can run a scan's front
+  // end while finish() works on the previous one
   KeyframeRunner(int height, int width, const FeatureModelParams& fmParams, bool use_graph = true,
-                 float fov_up_deg = 22.5f, float fov_down_deg = -22.5f);
+                 float fov_up_deg = 22.5f, float fov_down_deg = -22.5f, bool pipelined = false);
   ~KeyframeRunner();
   KeyframeRunner(const KeyframeRunner&) = delete;
   KeyframeRunner& operator=(const KeyframeRunner&) = delete;
@@ -55,12 +58,30 @@ class KeyframeRunner {
            const std::vector<Cylinder>& mapModels, SloamOutput& out,
            KeyframeTiming* timing = nullptr);
 
+  // The same step in two halves, for a stream of scans: submit() enqueues a
+  // scan's upload and front end and returns; finish() waits for the oldest
+  // submitted scan's front end and runs the estimator on it (the front end
+  // does not depend on the pose, the estimator does).  At most as many scans
+  // as front-end slots (1, or 2 when pipelined) may be submitted and not
+  // finished; run() needs none pending.  A finish()'s timing counts from its
+  // scan's submit(): upload_ms is 0 and frontend_ms is the wait for the
+  // front end.
+  void submit(const float* xyzi, const uint8_t* labels);
+  bool finish(const SE3& poseEstimate, const std::vector<Cylinder>& mapModels, SloamOutput& out,
+              KeyframeTiming* timing = nullptr);
+  // releases the oldest submitted scan without running the estimator (a
+  // keyframe the caller discards)
+  void skip();
+  int pending() const { return pending_; }
+  int slots() const { return (int)slots_.size(); }
+
   sloam::sloam& estimator() { return sloam_; }
 
   // Pinned host buffers of one scan (height * width x 4 floats, height *
-  // width labels), owned by the runner: a scan written here (e.g. by the
-  // sensor driver) and passed to run() is uploaded by DMA straight from them,
-  // at about twice the rate of pageable memory.
+  // width labels) of the front-end slot the next run() / submit() uses, owned
+  // by the runner: a scan written here (e.g. by the sensor driver) and passed
+  // on is uploaded by DMA straight from them, at about twice the rate of
+  // pageable memory.
   float* pinnedXyzi();
   uint8_t* pinnedLabels();
 
@@ -77,16 +98,33 @@ class KeyframeRunner {
     void ensure(size_t n);
     ~DevBuf();
   };
-  // cylinder fits of the nt trees of the last front-end run into fits
-  void fitTrees(int nt, std::vector<CylinderFit>& fits, KeyframeTiming* timing);
-  // (re)creates the front-end plan from fe_cfg_
+  // one front-end plan with its buffers, pinned scan input and stream
+  struct Slot {
+    sloam_ctx* ctx = nullptr;  // the bridge's context, or one of its own
+    bool own_ctx = false;
+    sloam_frontend* fe = nullptr;
+    sloam_frontend_out buf{};
+    sloam_frontend_host host{};
+    void* h_in = nullptr;   // pinned scan: xyzi then labels
+    void* done = nullptr;   // hipEvent_t recorded after the front end
+    std::chrono::steady_clock::time_point t_submit;
+  };
+  // cylinder fits of the nt trees of slot s's last front-end run into fits
+  void fitTrees(const Slot& s, int nt, std::vector<CylinderFit>& fits, KeyframeTiming* timing);
+  // (re)creates every slot's front-end plan from fe_cfg_
   void makeFrontend();
+  // the estimator on slot s's front-end results; t0..t2: submit, upload done,
+  // front end done
+  bool estimate(const Slot& s, const SE3& poseEstimate, const std::vector<Cylinder>& mapModels,
+                SloamOutput& out, KeyframeTiming* timing, std::chrono::steady_clock::time_point t0,
+                std::chrono::steady_clock::time_point t1,
+                std::chrono::steady_clock::time_point t2);
 
   int H_, W_;
-  sloam_frontend* fe_ = nullptr;
   sloam_frontend_cfg fe_cfg_{};
-  sloam_frontend_out buf_{};
-  sloam_frontend_host host_{};
+  std::vector<Slot> slots_;
+  int next_ = 0;     // slot of the next run() / submit()
+  int pending_ = 0;  // submitted and not finished
   FeatureModelParams fm_;
   sloam::sloam sloam_;
   TreeModel mode_ = TreeModel::MedianLines;
@@ -96,6 +134,5 @@ class KeyframeRunner {
   size_t h_off_cap_ = 0;
   DevBuf d_params_, d_xyz_, d_off_, d_model_, d_prop_, d_flags_;
   double* h_model_ = nullptr;   // pinned: models of the last fit + fallback count
-  void* h_in_ = nullptr;        // pinned scan: xyzi then labels
   size_t h_model_cap_ = 0;      // trees
 };

@@ -29,10 +29,10 @@ def sequence(tmp_path_factory):
     return path
 
 
-def _run(sequence, tmp, model, graph=1):
-    traj = os.path.join(str(tmp), f"traj_{model}_{graph}.bin")
+def _run(sequence, tmp, model, graph=1, extra=()):
+    traj = os.path.join(str(tmp), f"traj_{model}_{graph}_{'_'.join(extra)}.bin")
     cmd = [EXE, sequence, "1", str(graph), "--tree-model", model, "--weights", WEIGHTS,
-           "--traj-out", traj]
+           "--traj-out", traj, *extra]
     print("[keyframe]", " ".join(cmd[3:]), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -68,3 +68,16 @@ def test_graph_replay_matches_eager(sequence, tmp_path):
     _, t_graph = _run(sequence, tmp_path, "plain", graph=1)
     _, t_eager = _run(sequence, tmp_path, "plain", graph=0)
     assert np.array_equal(t_graph, t_eager)
+
+
+@pytest.mark.parametrize("model", ["plain", "learned"])
+def test_pipelined_matches_sequential(sequence, tmp_path, model):
+    """submit()/finish() on two front-end slots (each scan's front end running
+    while the previous keyframe's estimator does) gives the sequential
+    trajectory bit for bit, from pageable and from pinned input."""
+    r_seq, t_seq = _run(sequence, tmp_path, model)
+    r_pip, t_pip = _run(sequence, tmp_path, model, extra=("--pipelined",))
+    r_pin, t_pin = _run(sequence, tmp_path, model, extra=("--pipelined", "--pinned"))
+    assert r_pip["pipelined"] and r_pip["throughput_kf_per_s"] > 0
+    assert np.array_equal(t_seq, t_pip)
+    assert np.array_equal(t_seq, t_pin)
