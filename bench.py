@@ -327,21 +327,23 @@ def bench_keyframes(args):
     weights = os.path.join(ROOT, "sloam_amd", "weights", "cylinder_proposal.npy")
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} is missing: build first (python -m sloam_amd.build)")
-    # name, tree model, graph, pinned input
-    runs = (("learned", "learned", 1, 1), ("learned_pageable", "learned", 1, 0),
-            ("plain", "plain", 1, 1), ("lines", "lines", 1, 1), ("lines_eager", "lines", 0, 1))
+    # name, tree model, graph, pinned input, pipelined
+    runs = (("learned", "learned", 1, 1, 0), ("learned_pipelined", "learned", 1, 1, 1),
+            ("learned_pageable", "learned", 1, 0, 0), ("plain", "plain", 1, 1, 0),
+            ("lines", "lines", 1, 1, 0), ("lines_eager", "lines", 0, 1, 0))
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "sequence.bin")
         t0 = time.perf_counter()
         synth.write_keyframe_sequence(path, n_keyframes=args.keyframes, H=args.kf_height)
         gen_s = time.perf_counter() - t0
         res, traj = {}, {}
-        for name, model, graph, pinned in runs:
+        for name, model, graph, pinned, pipelined in runs:
             tp = os.path.join(d, f"traj_{name}.bin")
             print(f"[bench] keyframe stream: {name}", file=sys.stderr, flush=True)
             r = subprocess.run([exe, path, str(args.kf_passes), str(graph), "--tree-model", model,
                                 "--weights", weights, "--traj-out", tp] +
-                               (["--pinned"] if pinned else []),
+                               (["--pinned"] if pinned else []) +
+                               (["--pipelined"] if pipelined else []),
                                capture_output=True, text=True, timeout=600)
             if r.returncode != 0:
                 raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
@@ -356,9 +358,15 @@ def bench_keyframes(args):
 
     g = res["learned"]
     g["drift_vs_plain"] = traj_diff(traj["learned"], traj["plain"])
+    pip = res["learned_pipelined"]
+    g["pipelined"] = {"throughput_kf_per_s": pip["throughput_kf_per_s"],
+                      "latency_ms": pip["latency_ms"],
+                      "trajectory_equal": bool(np.array_equal(traj["learned_pipelined"],
+                                                              traj["learned"]))}
     g["tree_models"] = {
         name: {"latency_ms": res[name]["latency_ms"],
                "tree_fit_p50_ms": res[name]["tree_fit_p50_ms"],
+               "throughput_kf_per_s": res[name]["throughput_kf_per_s"],
                "trees_mean": res[name]["trees_mean"],
                "drift_vs_truth": res[name]["drift"]}
         for name in ("learned", "plain", "lines")}
@@ -372,6 +380,8 @@ def bench_keyframes(args):
                      f"first a warm-up reported apart; learned tree models (plain and the "
                      f"reference's median lines beside them); each scan handed over in the "
                      f"runner's pinned input buffers (pageable arrays: pageable_input); "
+                     f"throughput of the sequential runner and of the pipelined one (a "
+                     f"scan's front end overlapping the previous keyframe's estimator); "
                      f"sequence generated on the host in {gen_s:.1f} s (not timed)")
     return g
 
@@ -576,6 +586,9 @@ def main():
         "cpu_multithreaded": cpu_threaded,
         "keyframe_latency_ms": None if keyframes is None else
         {"p50": keyframes["latency_ms"]["p50"], "p99": keyframes["latency_ms"]["p99"]},
+        "keyframe_throughput_kf_per_s": None if keyframes is None else
+        {"sequential": keyframes["throughput_kf_per_s"],
+         "pipelined": keyframes["pipelined"]["throughput_kf_per_s"]},
         "pose": pose,
         "learned": learned,
         "keyframe": keyframes,
