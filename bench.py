@@ -65,6 +65,12 @@ def parse():
     ap.add_argument("--kf-passes", type=int, default=9,
                     help="passes over the keyframe sequence; the first is a warm-up")
     ap.add_argument("--kf-height", type=int, default=128)
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the long keyframe stream (--stream-keyframes scans, two passes)")
+    ap.add_argument("--stream-keyframes", type=int, default=1000)
+    ap.add_argument("--stream-spacing", type=float, default=0.2,
+                    help="metres between scans of the long stream (2 m/s at 10 Hz)")
+    ap.add_argument("--stream-forest-length", type=float, default=260.0)
     return ap.parse_args()
 
 
@@ -304,6 +310,40 @@ def bench_learned_size_bins(ctx, args, net, timed, sizes=(256, 1024, 4096), clus
     return out
 
 
+def traj_diff(a, b):
+    """Per-keyframe difference of two trajectories ([K, 7]: t, quaternion xyzw)."""
+    import numpy as np
+    dt = np.linalg.norm(a[:, :3] - b[:, :3], axis=1)
+    dot = np.clip(np.abs(np.sum(a[:, 3:] * b[:, 3:], axis=1)), 0, 1)
+    ang = np.degrees(2 * np.arccos(dot))
+    return {"max_m": float(dt.max()), "mean_m": float(dt.mean()), "final_m": float(dt[-1]),
+            "max_deg": float(ang.max()), "mean_deg": float(ang.mean())}
+
+
+def run_keyframe_bench(path, passes, model, pinned, pipelined, traj_path, graph=1, shadow=None):
+    """One run of the C++ program sloam_amd/keyframe_bench; returns its JSON
+    line and the trajectory it wrote.  shadow: a second tree model run beside
+    `model` on every keyframe of the first pass, from the same estimate and
+    submap (the one-keyframe pose difference of the two)."""
+    import subprocess
+
+    import numpy as np
+    exe = os.path.join(ROOT, "sloam_amd", "keyframe_bench")
+    weights = os.path.join(ROOT, "sloam_amd", "weights", "cylinder_proposal.npy")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is missing: build first (python -m sloam_amd.build)")
+    r = subprocess.run([exe, path, str(passes), str(graph), "--tree-model", model,
+                        "--weights", weights, "--traj-out", traj_path] +
+                       (["--pinned"] if pinned else []) +
+                       (["--pipelined"] if pipelined else []) +
+                       (["--shadow", shadow] if shadow else []),
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
+    return (json.loads(r.stdout.strip().splitlines()[-1]),
+            np.fromfile(traj_path, dtype=np.float64).reshape(-1, 7))
+
+
 def bench_keyframes(args):
     """Keyframe stream: scans of a 300-tree forest (128 x 2048, 40 % understory
     clutter, partial trunk occlusion) every 0.5 m along a weaving trajectory,
@@ -316,17 +356,12 @@ def bench_keyframes(args):
     fallback over each tree's points); the plain cylinder fit and the
     reference's median-line model run beside it, and the learned trajectory
     is compared with the plain one."""
-    import subprocess
     import tempfile
 
     import numpy as np
 
     from sloam_amd import synth
 
-    exe = os.path.join(ROOT, "sloam_amd", "keyframe_bench")
-    weights = os.path.join(ROOT, "sloam_amd", "weights", "cylinder_proposal.npy")
-    if not os.path.exists(exe):
-        raise RuntimeError(f"{exe} is missing: build first (python -m sloam_amd.build)")
     # name, tree model, graph, pinned input, pipelined
     runs = (("learned", "learned", 1, 1, 0), ("learned_pipelined", "learned", 1, 1, 1),
             ("learned_pageable", "learned", 1, 0, 0), ("plain", "plain", 1, 1, 0),
@@ -338,26 +373,15 @@ def bench_keyframes(args):
         gen_s = time.perf_counter() - t0
         res, traj = {}, {}
         for name, model, graph, pinned, pipelined in runs:
-            tp = os.path.join(d, f"traj_{name}.bin")
             print(f"[bench] keyframe stream: {name}", file=sys.stderr, flush=True)
-            r = subprocess.run([exe, path, str(args.kf_passes), str(graph), "--tree-model", model,
-                                "--weights", weights, "--traj-out", tp] +
-                               (["--pinned"] if pinned else []) +
-                               (["--pipelined"] if pipelined else []),
-                               capture_output=True, text=True, timeout=600)
-            if r.returncode != 0:
-                raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
-            res[name] = json.loads(r.stdout.strip().splitlines()[-1])
-            traj[name] = np.fromfile(tp, dtype=np.float64).reshape(-1, 7)
-
-    def traj_diff(a, b):
-        dt = np.linalg.norm(a[:, :3] - b[:, :3], axis=1)
-        dot = np.clip(np.abs(np.sum(a[:, 3:] * b[:, 3:], axis=1)), 0, 1)
-        ang = np.degrees(2 * np.arccos(dot))
-        return {"max_m": float(dt.max()), "final_m": float(dt[-1]), "max_deg": float(ang.max())}
+            res[name], traj[name] = run_keyframe_bench(
+                path, args.kf_passes, model, pinned, pipelined,
+                os.path.join(d, f"traj_{name}.bin"), graph=graph,
+                shadow="plain" if name == "learned" else None)
 
     g = res["learned"]
     g["drift_vs_plain"] = traj_diff(traj["learned"], traj["plain"])
+    g["step_vs_plain"] = g.pop("shadow")
     pip = res["learned_pipelined"]
     g["pipelined"] = {"throughput_kf_per_s": pip["throughput_kf_per_s"],
                       "latency_ms": pip["latency_ms"],
@@ -382,6 +406,84 @@ def bench_keyframes(args):
                      f"runner's pinned input buffers (pageable arrays: pageable_input); "
                      f"throughput of the sequential runner and of the pipelined one (a "
                      f"scan's front end overlapping the previous keyframe's estimator); "
+                     f"sequence generated on the host in {gen_s:.1f} s (not timed)")
+    return g
+
+
+def bench_keyframe_stream(args):
+    """The long keyframe stream: --stream-keyframes scans (128 x 2048) of a
+    300-tree forest along the weaving trajectory, one every --stream-spacing
+    metres (2 m/s at 10 Hz), 40 % understory clutter and partial trunk
+    occlusion; odometry noise per step scaled by sqrt(spacing / 0.5 m), so
+    the odometry's random walk has the same variance per metre as the short
+    sequence's.  Every scan is a keyframe.  Two passes per run: the first
+    warms every buffer and plan, the second is timed.  The learned tree model
+    runs sequentially and pipelined, the plain cylinder fit beside it; the
+    learned trajectory is compared with the plain one keyframe by keyframe."""
+    import math
+    import shutil
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    n = args.stream_keyframes
+    need = 12 + n * (112 + args.kf_height * 2048 * 17)
+    with tempfile.TemporaryDirectory() as d:
+        free = shutil.disk_usage(d).free
+        if free < need + (1 << 30):
+            msg = (f"the {need / 2**30:.1f} GiB sequence file needs more than the "
+                   f"{free / 2**30:.1f} GiB free under {d}")
+            print(f"[bench] long keyframe stream skipped: {msg}", file=sys.stderr, flush=True)
+            return {"skipped": msg}
+        path = os.path.join(d, "stream.bin")
+        scale = math.sqrt(args.stream_spacing / 0.5)
+        sigma_t, sigma_yaw = 0.05 * scale, 0.5 * scale
+        workers = max(1, min(16, os.cpu_count() or 1))
+        print(f"[bench] long keyframe stream: writing {n} scans ({workers} processes)",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        # a process of its own: its pool forks, and this process holds the GPU
+        subprocess.run([sys.executable, "-m", "sloam_amd.synth", path, str(n),
+                        str(args.stream_spacing), str(args.stream_forest_length),
+                        str(sigma_t), str(sigma_yaw), str(workers), str(args.kf_height)],
+                       check=True, timeout=1200, cwd=ROOT)
+        gen_s = time.perf_counter() - t0
+        res, traj = {}, {}
+        for name, model, pipelined in (("learned", "learned", 0),
+                                       ("learned_pipelined", "learned", 1),
+                                       ("plain", "plain", 0)):
+            print(f"[bench] long keyframe stream: {name}", file=sys.stderr, flush=True)
+            res[name], traj[name] = run_keyframe_bench(
+                path, 2, model, 1, pipelined, os.path.join(d, f"traj_{name}.bin"),
+                shadow="plain" if name == "learned" else None)
+
+    def summary(r):
+        return {k: r[k] for k in ("latency_ms", "throughput_kf_per_s", "stage_p50_ms",
+                                  "estimator_p50_ms", "tree_fit_p50_ms", "trees_mean",
+                                  "planes_mean", "success_rate", "discarded", "trees_fitted",
+                                  "trees_fallback", "drift")}
+
+    g = summary(res["learned"])
+    g["keyframes"] = n
+    g["drift_vs_plain"] = traj_diff(traj["learned"], traj["plain"])
+    g["step_vs_plain"] = res["learned"]["shadow"]
+    g["pipelined"] = {"throughput_kf_per_s": res["learned_pipelined"]["throughput_kf_per_s"],
+                      "latency_ms": res["learned_pipelined"]["latency_ms"],
+                      "trajectory_equal": bool(np.array_equal(traj["learned_pipelined"],
+                                                              traj["learned"]))}
+    g["plain"] = summary(res["plain"])
+    g["workload"] = (f"{n} keyframes of {args.kf_height}x2048 scans, {args.stream_spacing} m "
+                     f"apart ({n * args.stream_spacing:.0f} m), 300 trees over "
+                     f"{args.stream_forest_length:.0f} x 60 m, 40 % understory clutter, "
+                     f"odometry noise {sigma_t:.4f} m / {sigma_yaw:.3f} deg per step; "
+                     f"two passes, the second timed; learned tree model (sequential and "
+                     f"pipelined runner), plain cylinder fit beside it; pinned input "
+                     f"buffers; drift is the final/max keyframe position error against "
+                     f"the true poses, drift_vs_plain against the plain trajectory "
+                     f"(two maps built apart), step_vs_plain the per-keyframe pose "
+                     f"difference of a plain runner fed the learned run's estimate and "
+                     f"submap (first pass); "
                      f"sequence generated on the host in {gen_s:.1f} s (not timed)")
     return g
 
@@ -499,8 +601,11 @@ def main():
     if not args.no_learned:
         learned = bench_learned(ctx, args)
     keyframes = None
+    stream = None
     if not args.no_keyframes:
         keyframes = bench_keyframes(args)
+        if not args.no_stream and args.stream_keyframes > 0:
+            stream = bench_keyframe_stream(args)
 
     cpu = None
     cpu_threaded = None
@@ -538,6 +643,12 @@ def main():
                             "sample": f"the same, {args.cpu_pool} threads over clusters; "
                                       f"median of 3 runs, {tot_t:.1f} s in total"}
 
+    # keyframe headline: the long stream when it ran, else the short sequence
+    head, head_name = None, None
+    if stream is not None and "skipped" not in stream:
+        head, head_name = stream, "keyframe_stream"
+    elif keyframes is not None:
+        head, head_name = keyframes, "keyframe"
     line = {
         "metric": METRIC,
         "value": fits_per_s,
@@ -584,14 +695,18 @@ def main():
                                    "p90": float(np.percentile(rerr, 90) * 100)},
         "cpu_baseline": cpu,
         "cpu_multithreaded": cpu_threaded,
-        "keyframe_latency_ms": None if keyframes is None else
-        {"p50": keyframes["latency_ms"]["p50"], "p99": keyframes["latency_ms"]["p99"]},
-        "keyframe_throughput_kf_per_s": None if keyframes is None else
-        {"sequential": keyframes["throughput_kf_per_s"],
-         "pipelined": keyframes["pipelined"]["throughput_kf_per_s"]},
+        "keyframe_latency_ms": None if head is None else
+        {"p50": head["latency_ms"]["p50"], "p99": head["latency_ms"]["p99"]},
+        "keyframe_throughput_kf_per_s": None if head is None else
+        {"sequential": head["throughput_kf_per_s"],
+         "pipelined": head["pipelined"]["throughput_kf_per_s"]},
+        "keyframe_drift_vs_plain": None if head is None else head["drift_vs_plain"],
+        "keyframe_step_vs_plain": None if head is None else head["step_vs_plain"],
+        "keyframe_headline": head_name,
         "pose": pose,
         "learned": learned,
         "keyframe": keyframes,
+        "keyframe_stream": stream,
     }
     print(json.dumps(line), flush=True)
     plan.close()

@@ -6,6 +6,7 @@
 //   keyframe_bench <sequence file> [passes] [use_graph 0/1]
 //                  [--tree-model lines|plain|learned] [--weights <npy>]
 //                  [--traj-out <file>] [--log] [--pinned] [--pipelined]
+//                  [--shadow lines|plain|learned]
 //
 // --tree-model picks how each tree's model is made (KeyframeRunner's
 // TreeModel; default lines, the reference's); --traj-out writes the first
@@ -18,7 +19,12 @@
 // before the previous keyframe's estimator runs (KeyframeRunner::submit /
 // finish on two front-end slots).  Throughput is keyframes per second of
 // each timed pass (the copies into the pinned buffers, which a sensor driver
-// would make, are not counted).
+// would make, are not counted).  --shadow runs, in the first pass only, a
+// second runner with the named tree model on every keyframe, from the same
+// pose estimate and the same submap as the first, and reports how far its
+// pose lands from the first runner's (the one-keyframe difference of the two
+// tree models, without the compounding of two maps built apart); it is not
+// allowed with --pipelined.
 //
 // Sequence file (little endian): int32 K, H, W; then per keyframe:
 // float64 true pose [7] (t, q x y z w; map <- sensor), float64 odometry
@@ -35,6 +41,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -52,6 +59,26 @@ struct Frame {
 
 SE3 to_se3(const double* p) { return SE3(Quat(p[6], p[3], p[4], p[5]), Vector3(p[0], p[1], p[2])); }
 
+bool parse_tree_model(const std::string& name, TreeModel* m) {
+  if (name == "lines") {
+    *m = TreeModel::MedianLines;
+  } else if (name == "plain") {
+    *m = TreeModel::Plain;
+  } else if (name == "learned") {
+    *m = TreeModel::Learned;
+  } else {
+    return false;
+  }
+  return true;
+}
+
+// translation (m) and rotation (deg) between two poses
+void pose_diff(const SE3& a, const SE3& b, double* m, double* deg) {
+  const Quat dq = a.unit_quaternion().conjugate() * b.unit_quaternion();
+  *deg = 2.0 * std::acos(std::min(1.0, std::fabs(dq.w))) * 180.0 / M_PI;
+  *m = (a.translation() - b.translation()).norm();
+}
+
 double pct(std::vector<double> v, double q) {
   if (v.empty()) return 0.0;
   std::sort(v.begin(), v.end());
@@ -65,7 +92,7 @@ double pct(std::vector<double> v, double q) {
 
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
-  std::string model_name = "lines", weights, traj_out;
+  std::string model_name = "lines", weights, traj_out, shadow_name;
   bool log = false, pinned = false, pipelined = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -75,8 +102,13 @@ int main(int argc, char** argv) {
       pinned = true;
     } else if (a == "--pipelined") {
       pipelined = true;
-    } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out") && i + 1 < argc) {
-      (a == "--tree-model" ? model_name : a == "--weights" ? weights : traj_out) = argv[++i];
+    } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out" ||
+                a == "--shadow") &&
+               i + 1 < argc) {
+      (a == "--tree-model" ? model_name
+       : a == "--weights"  ? weights
+       : a == "--shadow"   ? shadow_name
+                           : traj_out) = argv[++i];
     } else if (a.rfind("--", 0) == 0) {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
@@ -91,13 +123,18 @@ int main(int argc, char** argv) {
                  argv[0]);
     return 2;
   }
-  TreeModel tree_model = TreeModel::MedianLines;
-  if (model_name == "plain") {
-    tree_model = TreeModel::Plain;
-  } else if (model_name == "learned") {
-    tree_model = TreeModel::Learned;
-  } else if (model_name != "lines") {
+  TreeModel tree_model = TreeModel::MedianLines, shadow_model = TreeModel::MedianLines;
+  if (!parse_tree_model(model_name, &tree_model)) {
     std::fprintf(stderr, "unknown tree model %s\n", model_name.c_str());
+    return 2;
+  }
+  const bool use_shadow = !shadow_name.empty();
+  if (use_shadow && !parse_tree_model(shadow_name, &shadow_model)) {
+    std::fprintf(stderr, "unknown shadow tree model %s\n", shadow_name.c_str());
+    return 2;
+  }
+  if (use_shadow && pipelined) {
+    std::fprintf(stderr, "--shadow runs keyframes one at a time: not with --pipelined\n");
     return 2;
   }
   const int passes = pos.size() > 1 ? std::atoi(pos[1].c_str()) : 3;
@@ -136,7 +173,15 @@ int main(int argc, char** argv) {
   long long fitted = 0, fallback = 0;
   int discarded = 0;
   std::vector<double> pass_rate;  // keyframes per second of each timed pass
+  // --shadow: per keyframe, the second runner's pose against the first's
+  std::vector<double> sh_m, sh_deg;
+  int sh_ok_differs = 0;
   try {
+    std::unique_ptr<KeyframeRunner> shadow;
+    if (use_shadow) {
+      shadow.reset(new KeyframeRunner(H, W, fm, use_graph, 22.5f, -22.5f, false));
+      shadow->setTreeModel(shadow_model, weights);
+    }
     for (int pass = 0; pass < passes; ++pass) {
       KeyframeRunner kf(H, W, fm, use_graph, 22.5f, -22.5f, pipelined);
       kf.setTreeModel(tree_model, weights);
@@ -187,6 +232,19 @@ int main(int argc, char** argv) {
         } else {
           ok = pipelined ? kf.finish(estimate, map, out, &t)
                          : kf.run(xyzi, label, estimate, map, out, &t);
+          if (shadow && pass == 0) {
+            SloamOutput so;
+            const bool sok =
+                shadow->run(seq[k].xyzi.data(), seq[k].label.data(), before, map, so, nullptr);
+            if (k > 0 && ok && sok) {
+              double m, deg;
+              pose_diff(out.T_Map_Curr, so.T_Map_Curr, &m, &deg);
+              sh_m.push_back(m);
+              sh_deg.push_back(deg);
+            } else if (k > 0 && ok != sok) {
+              ++sh_ok_differs;
+            }
+          }
           semanticMap.updateMap(out.tm, out.matches);
           if (ok && nodeFirst && !out.tm.empty()) nodeFirst = false;
         }
@@ -290,7 +348,7 @@ int main(int argc, char** argv) {
       "\"frontend_p99_ms\": %.4f, \"trees_mean\": %.2f, "
       "\"planes_mean\": %.2f, \"success_rate\": %.4f, \"drift\": {\"final_m\": %.4f, "
       "\"max_m\": %.4f, \"max_deg\": %.4f}, \"warmup_pass_latency_ms\": {\"p50\": %.4f, "
-      "\"max\": %.4f}}\n",
+      "\"max\": %.4f}, ",
       K, passes, H, W, use_graph ? "true" : "false", pct(total, 0.5), pct(total, 0.99),
       samples ? [&] { double s = 0; for (double x : total) s += x; return s / samples; }() : 0.0,
       total.empty() ? 0.0 : *std::max_element(total.begin(), total.end()), pct(up, 0.5),
@@ -299,6 +357,20 @@ int main(int argc, char** argv) {
       samples ? trees / samples : 0.0, samples ? planes / samples : 0.0,
       samples ? (double)ok_count / samples : 0.0, final_drift, max_drift, max_rot,
       pct(first, 0.5), first.empty() ? 0.0 : *std::max_element(first.begin(), first.end()));
+  if (use_shadow) {
+    int within = 0;
+    for (size_t i = 0; i < sh_m.size(); ++i) within += sh_m[i] <= 0.01 && sh_deg[i] <= 0.1;
+    std::printf(
+        "\"shadow\": {\"tree_model\": \"%s\", \"compared\": %zu, \"success_differs\": %d, "
+        "\"m\": {\"p50\": %.6f, \"p99\": %.6f, \"max\": %.6f}, \"deg\": {\"p50\": %.6f, "
+        "\"p99\": %.6f, \"max\": %.6f}, \"within_1cm_0p1deg\": %.4f}}\n",
+        shadow_name.c_str(), sh_m.size(), sh_ok_differs, pct(sh_m, 0.5), pct(sh_m, 0.99),
+        sh_m.empty() ? 0.0 : *std::max_element(sh_m.begin(), sh_m.end()), pct(sh_deg, 0.5),
+        pct(sh_deg, 0.99), sh_deg.empty() ? 0.0 : *std::max_element(sh_deg.begin(), sh_deg.end()),
+        sh_m.empty() ? 0.0 : (double)within / sh_m.size());
+  } else {
+    std::printf("\"shadow\": null}\n");
+  }
   // the five slowest keyframes, on stderr
   std::sort(slow.begin(), slow.end(), [](const auto& a, const auto& b) { return a[0] > b[0]; });
   for (size_t i = 0; i < slow.size() && i < 5; ++i)

@@ -336,16 +336,18 @@ def make_forest(seed=4, n_trees=300, length=200.0, width=60.0, r_min=0.1, r_max=
     return {"trees": trees, "ground": g}
 
 
-def trajectory_pose(s, sensor_height=1.5):
+def trajectory_pose(s, sensor_height=1.5, ground=None):
     """Pose (R, t) at arc length s of the path: along x with a gentle lateral
-    weave, heading along the path tangent, sensor 1.5 m above the origin's
-    ground."""
+    weave, heading along the path tangent, the sensor level (yaw only) and
+    sensor_height above the ground plane z = g0 x + g1 y of `ground` (a
+    forest's "ground"), or above z = 0 without one."""
     y = 1.0 * math.sin(s / 25.0)
     dy = math.cos(s / 25.0) / 25.0
     yaw = math.atan2(dy, 1.0)
     c, sn = math.cos(yaw), math.sin(yaw)
     R = np.array([[c, -sn, 0.0], [sn, c, 0.0], [0.0, 0.0, 1.0]])
-    return R, np.array([s, y, sensor_height])
+    z0 = 0.0 if ground is None else ground[0] * s + ground[1] * y
+    return R, np.array([s, y, z0 + sensor_height])
 
 
 def scan_from(forest, R, t, seed=0, H=128, W=2048, fov_deg=22.5, clutter_frac=0.4,
@@ -443,37 +445,69 @@ def scan_from(forest, R, t, seed=0, H=128, W=2048, fov_deg=22.5, clutter_frac=0.
     return {"xyzi": xyzi, "label": label, "instance": inst, "H": H, "W": W, "fov_deg": fov_deg}
 
 
+def _sequence_scan(job):
+    forest, s, seed, H, W, clutter_frac = job
+    R, t = trajectory_pose(s, ground=forest["ground"])
+    sc = scan_from(forest, R, t, seed=seed, H=H, W=W, clutter_frac=clutter_frac)
+    return (np.ascontiguousarray(sc["xyzi"], "<f4").tobytes(),
+            np.ascontiguousarray(sc["label"], np.uint8).tobytes())
+
+
 def write_keyframe_sequence(path, n_keyframes=16, H=128, W=2048, spacing=0.5, start=20.0,
                             seed=4, n_trees=300, clutter_frac=0.4, odom_sigma_t=0.05,
-                            odom_sigma_yaw_deg=0.5):
+                            odom_sigma_yaw_deg=0.5, forest_length=200.0, workers=1,
+                            log_every=0):
     """Writes the keyframe sequence file of sloam_amd/bench/keyframe_bench.cpp:
-    scans of one forest (make_forest) from poses `spacing` metres apart along
-    trajectory_pose, with noisy odometry increments between them."""
+    scans of one forest (make_forest, `forest_length` metres long) from poses
+    `spacing` metres apart along trajectory_pose (1.5 m above the forest's
+    ground), with noisy odometry
+    increments between them.  With workers > 1 the scans are cast by a pool
+    of that many processes; the file is the same byte for byte.  With
+    log_every > 0 a line goes to stderr after every log_every scans."""
     import struct
-    forest = make_forest(seed=seed, n_trees=n_trees)
+    forest = make_forest(seed=seed, n_trees=n_trees, length=forest_length)
     rng = np.random.default_rng(seed + 1)
     prev = None
-    with open(path, "wb") as f:
-        f.write(struct.pack("<iii", n_keyframes, H, W))
-        for k in range(n_keyframes):
-            R, t = trajectory_pose(start + k * spacing)
-            truth = np.concatenate([t, R_to_quat(R)])
-            if prev is None:
-                odom = np.array([0, 0, 0, 0, 0, 0, 1.0])
-            else:
-                Rp, tp = prev
-                Rd = Rp.T @ R
-                td = Rp.T @ (t - tp) + rng.normal(0, odom_sigma_t, 3) * np.array([1, 1, 0.2])
-                Rd = _rotvec_to_R(np.array([0, 0, math.radians(
-                    rng.normal(0, odom_sigma_yaw_deg))])) @ Rd
-                odom = np.concatenate([td, R_to_quat(Rd)])
-            prev = (R, t)
-            sc = scan_from(forest, R, t, seed=seed * 1000 + k, H=H, W=W,
-                           clutter_frac=clutter_frac)
-            f.write(np.asarray(truth, "<f8").tobytes())
-            f.write(np.asarray(odom, "<f8").tobytes())
-            f.write(np.ascontiguousarray(sc["xyzi"], "<f4").tobytes())
-            f.write(np.ascontiguousarray(sc["label"], np.uint8).tobytes())
+    header = []
+    for k in range(n_keyframes):
+        R, t = trajectory_pose(start + k * spacing, ground=forest["ground"])
+        truth = np.concatenate([t, R_to_quat(R)])
+        if prev is None:
+            odom = np.array([0, 0, 0, 0, 0, 0, 1.0])
+        else:
+            Rp, tp = prev
+            Rd = Rp.T @ R
+            td = Rp.T @ (t - tp) + rng.normal(0, odom_sigma_t, 3) * np.array([1, 1, 0.2])
+            Rd = _rotvec_to_R(np.array([0, 0, math.radians(
+                rng.normal(0, odom_sigma_yaw_deg))])) @ Rd
+            odom = np.concatenate([td, R_to_quat(Rd)])
+        prev = (R, t)
+        header.append((truth, odom))
+    jobs = [(forest, start + k * spacing, seed * 1000 + k, H, W, clutter_frac)
+            for k in range(n_keyframes)]
+    pool = None
+    if workers > 1 and n_keyframes > 1:
+        import multiprocessing
+        pool = multiprocessing.get_context("fork").Pool(min(workers, n_keyframes))
+        scans = pool.imap(_sequence_scan, jobs, chunksize=4)
+    else:
+        scans = map(_sequence_scan, jobs)
+    try:
+        with open(path, "wb") as f:
+            f.write(struct.pack("<iii", n_keyframes, H, W))
+            for k, ((truth, odom), (xyzi, label)) in enumerate(zip(header, scans)):
+                f.write(np.asarray(truth, "<f8").tobytes())
+                f.write(np.asarray(odom, "<f8").tobytes())
+                f.write(xyzi)
+                f.write(label)
+                if log_every > 0 and (k + 1) % log_every == 0:
+                    import sys
+                    print(f"[synth] {k + 1}/{n_keyframes} scans written", file=sys.stderr,
+                          flush=True)
+    finally:
+        if pool is not None:
+            pool.close()
+            pool.join()
 
 
 XYYAW_MASK = 0b100011  # t_x, t_y, w_z free (sloam.cpp:75-77 fixes {2, 3, 4})
@@ -515,3 +549,21 @@ def split_two_step(prob):
     out["cyl_offsets"] = np.array(cyl_off, np.int32)
     out["plane_offsets"] = np.array(pl_off, np.int32)
     return out
+
+
+if __name__ == "__main__":
+    # python -m sloam_amd.synth PATH N_KEYFRAMES [spacing forest_length odom_sigma_t
+    #     odom_sigma_yaw_deg workers H]: writes a keyframe sequence file from a
+    # process of its own (a pool may fork here; nothing here touches the GPU)
+    import sys
+    a = sys.argv[1:]
+    if len(a) < 2:
+        sys.exit("usage: python -m sloam_amd.synth PATH N_KEYFRAMES [spacing forest_length "
+                 "odom_sigma_t odom_sigma_yaw_deg workers H]")
+    kw = {}
+    for i, (name, cast) in enumerate((("spacing", float), ("forest_length", float),
+                                      ("odom_sigma_t", float), ("odom_sigma_yaw_deg", float),
+                                      ("workers", int), ("H", int))):
+        if len(a) > 2 + i:
+            kw[name] = cast(a[2 + i])
+    write_keyframe_sequence(a[0], n_keyframes=int(a[1]), log_every=100, **kw)
