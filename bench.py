@@ -340,6 +340,9 @@ def run_keyframe_bench(path, passes, model, pinned, pipelined, traj_path, graph=
                        capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
+    for line in r.stderr.splitlines():
+        if line.startswith("shadow keyframe"):
+            print(f"[bench] {line}", file=sys.stderr, flush=True)
     return (json.loads(r.stdout.strip().splitlines()[-1]),
             np.fromfile(traj_path, dtype=np.float64).reshape(-1, 7))
 

@@ -21,10 +21,10 @@
 // each timed pass (the copies into the pinned buffers, which a sensor driver
 // would make, are not counted).  --shadow runs, in the first pass only, a
 // second runner with the named tree model on every keyframe, from the same
-// pose estimate and the same submap as the first, and reports how far its
-// pose lands from the first runner's (the one-keyframe difference of the two
-// tree models, without the compounding of two maps built apart); it is not
-// allowed with --pipelined.
+// pose estimate, submap and previous ground models as the first, and
+// reports how far its pose lands from the first runner's (the one-keyframe
+// difference of the two tree models, without the compounding of two maps
+// built apart); it is not allowed with --pipelined.
 //
 // Sequence file (little endian): int32 K, H, W; then per keyframe:
 // float64 true pose [7] (t, q x y z w; map <- sensor), float64 odometry
@@ -119,7 +119,8 @@ int main(int argc, char** argv) {
   if (pos.empty()) {
     std::fprintf(stderr,
                  "usage: %s <sequence> [passes] [use_graph] [--tree-model lines|plain|learned] "
-                 "[--weights npy] [--traj-out file]\n",
+                 "[--weights npy] [--traj-out file] [--log] [--pinned] [--pipelined] "
+                 "[--shadow lines|plain|learned]\n",
                  argv[0]);
     return 2;
   }
@@ -176,6 +177,10 @@ int main(int argc, char** argv) {
   // --shadow: per keyframe, the second runner's pose against the first's
   std::vector<double> sh_m, sh_deg;
   int sh_ok_differs = 0;
+  // per compared keyframe: m, deg, k, then each runner's trees, map
+  // matches, tree step run, tree feature matches; largest radius difference
+  // of the landmarks both kept in the same order
+  std::vector<std::array<double, 12>> sh_rows;
   try {
     std::unique_ptr<KeyframeRunner> shadow;
     if (use_shadow) {
@@ -230,9 +235,14 @@ int main(int argc, char** argv) {
           if (pass == 0) ++discarded;
           if (pipelined) kf.skip();
         } else {
+          // the shadow starts from the first runner's state: its estimate,
+          // submap and previous ground models
+          std::vector<Plane> prevPlanes;
+          if (shadow && pass == 0) prevPlanes = kf.estimator().getPrevGroundModel();
           ok = pipelined ? kf.finish(estimate, map, out, &t)
                          : kf.run(xyzi, label, estimate, map, out, &t);
           if (shadow && pass == 0) {
+            shadow->estimator().setPrevGroundModel(std::move(prevPlanes));
             SloamOutput so;
             const bool sok =
                 shadow->run(seq[k].xyzi.data(), seq[k].label.data(), before, map, so, nullptr);
@@ -241,6 +251,24 @@ int main(int argc, char** argv) {
               pose_diff(out.T_Map_Curr, so.T_Map_Curr, &m, &deg);
               sh_m.push_back(m);
               sh_deg.push_back(deg);
+              auto matched = [](const SloamOutput& o) {
+                int c = 0;
+                for (int x : o.matches) c += x >= 0 ? 1 : 0;
+                return (double)c;
+              };
+              const SloamStepInfo a = kf.estimator().lastStep();
+              const SloamStepInfo b = shadow->estimator().lastStep();
+              double dr = -1.0;
+              if (out.tm.size() == so.tm.size()) {
+                dr = 0.0;
+                for (size_t i = 0; i < out.tm.size(); ++i)
+                  dr = std::max(dr, std::fabs(out.tm[i].getModel().radius -
+                                              so.tm[i].getModel().radius));
+              }
+              sh_rows.push_back({m, deg, (double)k, (double)out.tm.size(), matched(out),
+                                 a.treeCheck ? 1.0 : 0.0, (double)a.treeMatches,
+                                 (double)so.tm.size(), matched(so), b.treeCheck ? 1.0 : 0.0,
+                                 (double)b.treeMatches, dr});
             } else if (k > 0 && ok != sok) {
               ++sh_ok_differs;
             }
@@ -358,18 +386,47 @@ int main(int argc, char** argv) {
       samples ? (double)ok_count / samples : 0.0, final_drift, max_drift, max_rot,
       pct(first, 0.5), first.empty() ? 0.0 : *std::max_element(first.begin(), first.end()));
   if (use_shadow) {
-    int within = 0;
-    for (size_t i = 0; i < sh_m.size(); ++i) within += sh_m[i] <= 0.01 && sh_deg[i] <= 0.1;
-    std::printf(
-        "\"shadow\": {\"tree_model\": \"%s\", \"compared\": %zu, \"success_differs\": %d, "
-        "\"m\": {\"p50\": %.6f, \"p99\": %.6f, \"max\": %.6f}, \"deg\": {\"p50\": %.6f, "
-        "\"p99\": %.6f, \"max\": %.6f}, \"within_1cm_0p1deg\": %.4f}}\n",
-        shadow_name.c_str(), sh_m.size(), sh_ok_differs, pct(sh_m, 0.5), pct(sh_m, 0.99),
-        sh_m.empty() ? 0.0 : *std::max_element(sh_m.begin(), sh_m.end()), pct(sh_deg, 0.5),
-        pct(sh_deg, 0.99), sh_deg.empty() ? 0.0 : *std::max_element(sh_deg.begin(), sh_deg.end()),
-        sh_m.empty() ? 0.0 : (double)within / sh_m.size());
+    // all compared keyframes, then those where both runners kept the same
+    // number of trees and took the same XY-yaw decision
+    std::vector<double> same_m, same_deg;
+    for (const auto& r : sh_rows)
+      if (r[3] == r[7] && r[5] == r[9]) {
+        same_m.push_back(r[0]);
+        same_deg.push_back(r[1]);
+      }
+    auto stats = [&](const std::vector<double>& m, const std::vector<double>& deg) {
+      int within = 0;
+      for (size_t i = 0; i < m.size(); ++i) within += m[i] <= 0.01 && deg[i] <= 0.1;
+      std::printf(
+          "\"compared\": %zu, \"m\": {\"p50\": %.6f, \"p99\": %.6f, \"max\": %.6f}, "
+          "\"deg\": {\"p50\": %.6f, \"p99\": %.6f, \"max\": %.6f}, "
+          "\"within_1cm_0p1deg\": %.4f",
+          m.size(), pct(m, 0.5), pct(m, 0.99), m.empty() ? 0.0 : *std::max_element(m.begin(), m.end()),
+          pct(deg, 0.5), pct(deg, 0.99),
+          deg.empty() ? 0.0 : *std::max_element(deg.begin(), deg.end()),
+          m.empty() ? 0.0 : (double)within / m.size());
+    };
+    std::printf("\"shadow\": {\"tree_model\": \"%s\", \"success_differs\": %d, ",
+                shadow_name.c_str(), sh_ok_differs);
+    stats(sh_m, sh_deg);
+    std::printf(", \"same_decisions\": {");
+    stats(same_m, same_deg);
+    std::printf("}}}\n");
   } else {
     std::printf("\"shadow\": null}\n");
+  }
+  // --shadow: the five keyframes whose poses differ most, on stderr
+  std::sort(sh_rows.begin(), sh_rows.end(),
+            [](const auto& a, const auto& b) { return a[0] > b[0]; });
+  for (size_t i = 0; i < sh_rows.size() && i < 5; ++i) {
+    const auto& r = sh_rows[i];
+    std::fprintf(stderr,
+                 "shadow keyframe %d: %.4f m %.3f deg; %s trees %d matched %d tree step %d "
+                 "(%d feature matches); %s trees %d matched %d tree step %d (%d); largest "
+                 "radius difference %.4f m\n",
+                 (int)r[2], r[0], r[1], model_name.c_str(), (int)r[3], (int)r[4], (int)r[5],
+                 (int)r[6], shadow_name.c_str(), (int)r[7], (int)r[8], (int)r[9], (int)r[10],
+                 r[11]);
   }
   // the five slowest keyframes, on stderr
   std::sort(slow.begin(), slow.end(), [](const auto& a, const auto& b) { return a[0] > b[0]; });

@@ -465,6 +465,7 @@ double ms_between(PhaseClock::time_point a, PhaseClock::time_point b) {
 
 bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
   phase_ = SloamPhaseTimes();
+  step_ = SloamStepInfo();
   const auto p0 = PhaseClock::now();
   std::vector<Plane> planes;
   std::vector<Cylinder> landmarks;
@@ -503,6 +504,7 @@ bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
                          treeMatches.size() > 5 * fmParams_.featuresPerTree;
   const bool groundCheck =
       planeMatches.size() > fmParams_.minGroundModels && planes.size() > minPlanes_;
+  step_ = {treeCheck, groundCheck, (int)treeMatches.size(), (int)planeMatches.size()};
   if (fmParams_.twoStepOptim) {
     success = two_step_pose(in.poseEstimate, treeCheck, groundCheck, treeMatches, planeMatches,
                             currPose);

@@ -3,6 +3,7 @@
 // calls; the pose solves run on the GPU LM kernels.
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "cylinder.h"
@@ -59,6 +60,14 @@ struct SloamPhaseTimes {
   double models = 0, match = 0, pose = 0, finish = 0;
 };
 
+// what the last RunSloam call decided: whether its XY-yaw (tree) and
+// Z-roll-pitch (ground) steps had enough matched features to run, and how
+// many tree and ground feature matches it had (zero before association)
+struct SloamStepInfo {
+  bool treeCheck = false, groundCheck = false;
+  int treeMatches = 0, planeMatches = 0;
+};
+
 namespace sloam {
 
 class sloam {
@@ -99,8 +108,14 @@ class sloam {
                          std::vector<ObjectMatch<T>>& matches);
 
   std::vector<Plane> getPrevGroundModel();
+  // Replaces the previous scan's ground models, which the next RunSloam
+  // matches the scan's planes against (the reference has the getter only):
+  // a second estimator then starts a keyframe from another's state, so that
+  // two tree models can be compared one keyframe at a time.
+  void setPrevGroundModel(std::vector<Plane> planes) { prevGPlanes_ = std::move(planes); }
   CloudT getPrevGroundFeatures();
   const SloamPhaseTimes& lastPhaseTimes() const { return phase_; }
+  const SloamStepInfo& lastStep() const { return step_; }
 
  private:
   // ground binning / fit settings of fmParams_, binned around `pose`
@@ -114,6 +129,7 @@ class sloam {
   bool firstScan_ = true;
   double minPlanes_ = 0.0;
   SloamPhaseTimes phase_;
+  SloamStepInfo step_;
 };
 
 }  // namespace sloam
