@@ -8,6 +8,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "gpu_bridge.h"
 
@@ -374,9 +375,8 @@ bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
   const float *pts = hv.points, *feat = hv.cell_features;
   const auto t3 = Clock::now();
 
-  SloamInput in;
+  SloamInput in;  // the map trees go to RunSloam apart, not copied into it
   in.poseEstimate = poseEstimate;
-  in.mapModels = mapModels;
   in.landmarks.reserve(nt);
   for (int t = 0; t < nt; ++t) {
     std::vector<TreeVertex> tree;
@@ -415,7 +415,7 @@ bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
         const float* q = &feat[4 * ((size_t)c * nf + j)];
         f.push_back(PointT{q[0], q[1], q[2], q[3]});
       }
-    in.groundCells.emplace_back(pp, f, valid);
+    in.groundCells.emplace_back(pp, std::move(f), valid);
   }
   if (mode_ != TreeModel::MedianLines && nt > 0) {
     // the tree models from cylinder fits of the trees' points
@@ -423,7 +423,7 @@ bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
     in.treeLineFits.clear();
   }
   const auto t3b = Clock::now();
-  const bool ok = sloam_.RunSloam(in, out);
+  const bool ok = sloam_.RunSloam(in, mapModels, out);
   const auto t4 = Clock::now();
   if (timing) {
     timing->upload_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

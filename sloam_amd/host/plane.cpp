@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <limits>
+#include <utility>
 
 #include "gpu_bridge.h"
 
@@ -30,9 +31,9 @@ Plane::Plane(const VectorType& points, const FeatureModelParams& fmParams) {
   features = f.features;  // the first numGroundFeatures points (plane.cpp:14)
 }
 
-Plane::Plane(const PlaneParameters& fitted, const VectorType& feats, bool valid) {
+Plane::Plane(const PlaneParameters& fitted, VectorType feats, bool valid) {
   model = fitted;
-  features = feats;
+  features = std::move(feats);
   isValid = valid;
 }
 

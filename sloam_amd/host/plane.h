@@ -19,7 +19,7 @@ class Plane : public SemanticObject<PlaneParameters> {
   // than numGroundFeatures points; features keep the first numGroundFeatures.
   explicit Plane(const VectorType& points, const FeatureModelParams& fmParams);
   // A plane already fitted on the GPU (batched path of computeModels).
-  Plane(const PlaneParameters& fitted, const VectorType& features, bool valid);
+  Plane(const PlaneParameters& fitted, VectorType features, bool valid);
   Scalar distance(const PlaneParameters& tgt) const override;
   Scalar distance(const PointT& point) const override;
   void project(const SE3& tf) override;

@@ -463,7 +463,9 @@ double ms_between(PhaseClock::time_point a, PhaseClock::time_point b) {
 }
 }  // namespace
 
-bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
+bool sloam::RunSloam(SloamInput& in, SloamOutput& out) { return RunSloam(in, in.mapModels, out); }
+
+bool sloam::RunSloam(SloamInput& in, const std::vector<Cylinder>& mapModels, SloamOutput& out) {
   phase_ = SloamPhaseTimes();
   step_ = SloamStepInfo();
   const auto p0 = PhaseClock::now();
@@ -485,13 +487,13 @@ bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
     firstScan_ = false;
     return success;
   }
-  if (in.mapModels.empty()) return false;                // empty map
+  if (mapModels.empty()) return false;                   // empty map
   if (planes.empty() || landmarks.empty()) return false;  // no models in this scan
 
   // matchFeatures' associations, pointing at the map objects instead of
   // copying them (they stay alive until the pose solve is done)
   const auto treeMatches = match_features<Cylinder, CylinderParameters>(
-      in.poseEstimate, landmarks, in.mapModels, fmParams_.treeMatchThresh);
+      in.poseEstimate, landmarks, mapModels, fmParams_.treeMatchThresh);
   // the ground association threshold is fixed at 1.0 (sloam.cpp:482)
   const auto planeMatches =
       match_features<Plane, PlaneParameters>(in.poseEstimate, planes, prevGPlanes_, 1.0);
@@ -516,7 +518,7 @@ bool sloam::RunSloam(SloamInput& in, SloamOutput& out) {
   phase_.pose = ms_between(p2, p3);
 
   projectModels(currPose, landmarks, planes);
-  matchModels(landmarks, in.mapModels, matchIndices);
+  matchModels(landmarks, mapModels, matchIndices);
   prevGPlanes_ = std::move(planes);
   out.matches = matchIndices;
   out.T_Map_Curr = currPose;

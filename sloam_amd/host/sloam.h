@@ -80,6 +80,10 @@ class sloam {
     minPlanes_ = (fmParams_.groundRadiiBins * fmParams_.groundThetaBins) * 0.1;
   }
   bool RunSloam(SloamInput& in, SloamOutput& out);
+  // The same step with the map trees passed apart from `in` (in.mapModels
+  // is not read): a caller that keeps its submap need not copy it, with
+  // every tree's vertices and their points, into the input each keyframe.
+  bool RunSloam(SloamInput& in, const std::vector<Cylinder>& mapModels, SloamOutput& out);
 
   // Pose optimization
   void OptimizeXYYaw(const SE3& poseEstimate, const bool optimize,
