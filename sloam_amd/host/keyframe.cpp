@@ -117,10 +117,8 @@ void KeyframeRunner::setTreeModel(TreeModel mode, const std::string& weights_npy
   mode_ = mode;
 }
 
-void KeyframeRunner::fitTrees(const Slot& sl, int nt, std::vector<CylinderFit>& fits,
-                              KeyframeTiming* timing) {
+void KeyframeRunner::fitTreesLaunch(const Slot& sl, int nt) {
   sloam_ctx* ctx = sloam_gpu::context();
-  const auto t0 = Clock::now();
   const int32_t *to = sl.host.tree_offsets, *po = sl.host.point_offsets;
   // tree t's points are those of its vertices, contiguous in the front end's
   // point list
@@ -201,8 +199,14 @@ void KeyframeRunner::fitTrees(const Slot& sl, int nt, std::vector<CylinderFit>& 
   check(sloam_memcpy_async(ctx, h_model_, model,
                            sizeof(double) * (SLOAM_CYL_MODEL_STRIDE * (size_t)nt + (learned ? 1 : 0))),
         "download models");
+}
+
+void KeyframeRunner::fitTreesCollect(int nt, std::vector<CylinderFit>& fits, double launch_ms,
+                                     KeyframeTiming* timing) {
+  const auto t0 = Clock::now();
+  const bool learned = mode_ == TreeModel::Learned;
   int32_t fb = 0;
-  check(sloam_sync(ctx), "sloam_sync");
+  check(sloam_sync(sloam_gpu::context()), "sloam_sync");
   if (learned) std::memcpy(&fb, h_model_ + SLOAM_CYL_MODEL_STRIDE * (size_t)nt, sizeof(int32_t));
   fits.assign(nt, CylinderFit());
   for (int t = 0; t < nt; ++t) {
@@ -217,7 +221,7 @@ void KeyframeRunner::fitTrees(const Slot& sl, int nt, std::vector<CylinderFit>& 
     c.ok = m[6] > 0.0 && m[7] >= 3.0;
   }
   if (timing) {
-    timing->fit_ms = ms_since(t0);
+    timing->fit_ms = launch_ms + ms_since(t0);
     timing->fitted = nt;
     timing->fallback = fb;
   }
@@ -374,6 +378,13 @@ bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
   const double *vf = hv.vertex_f64, *model = hv.cell_model;
   const float *pts = hv.points, *feat = hv.cell_features;
   const auto t3 = Clock::now();
+  // the tree fits run on the GPU while the host assembles the input below
+  const bool fit_trees = mode_ != TreeModel::MedianLines && nt > 0;
+  double fit_launch_ms = 0.0;
+  if (fit_trees) {
+    fitTreesLaunch(sl, nt);
+    fit_launch_ms = ms_since(t3);
+  }
 
   SloamInput in;  // the map trees go to RunSloam apart, not copied into it
   in.poseEstimate = poseEstimate;
@@ -417,9 +428,9 @@ bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
       }
     in.groundCells.emplace_back(pp, std::move(f), valid);
   }
-  if (mode_ != TreeModel::MedianLines && nt > 0) {
+  if (fit_trees) {
     // the tree models from cylinder fits of the trees' points
-    fitTrees(sl, nt, in.treeCylinderFits, timing);
+    fitTreesCollect(nt, in.treeCylinderFits, fit_launch_ms, timing);
     in.treeLineFits.clear();
   }
   const auto t3b = Clock::now();

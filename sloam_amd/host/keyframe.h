@@ -109,8 +109,13 @@ class KeyframeRunner {
     void* done = nullptr;   // hipEvent_t recorded after the front end
     std::chrono::steady_clock::time_point t_submit;
   };
-  // cylinder fits of the nt trees of slot s's last front-end run into fits
-  void fitTrees(const Slot& s, int nt, std::vector<CylinderFit>& fits, KeyframeTiming* timing);
+  // cylinder fits of the nt trees of slot s's last front-end run: launches
+  // them and the download of their models and returns (the host assembles
+  // the estimator's input meanwhile); fitTreesCollect waits for them and
+  // unpacks the models into fits.  launch_ms: host time of the launch.
+  void fitTreesLaunch(const Slot& s, int nt);
+  void fitTreesCollect(int nt, std::vector<CylinderFit>& fits, double launch_ms,
+                       KeyframeTiming* timing);
   // (re)creates every slot's front-end plan from fe_cfg_
   void makeFrontend();
   // the estimator on slot s's front-end results; t0..t2: submit, upload done,
