@@ -24,7 +24,9 @@
 // pose estimate, submap and previous ground models as the first, and
 // reports how far its pose lands from the first runner's (the one-keyframe
 // difference of the two tree models, without the compounding of two maps
-// built apart); it is not allowed with --pipelined.
+// built apart); it is not allowed with --pipelined.  The per-keyframe
+// latencies never include the shadow's work, but with a single pass that
+// pass's throughput does: give two passes or more to time with --shadow.
 //
 // Sequence file (little endian): int32 K, H, W; then per keyframe:
 // float64 true pose [7] (t, q x y z w; map <- sensor), float64 odometry
