@@ -183,6 +183,8 @@ int main(int argc, char** argv) {
   // matches, tree step run, tree feature matches; largest radius difference
   // of the landmarks both kept in the same order
   std::vector<std::array<double, 12>> sh_rows;
+  // radius differences of the trees both runners kept (paired by tree id)
+  std::vector<double> sh_dr;
   try {
     std::unique_ptr<KeyframeRunner> shadow;
     if (use_shadow) {
@@ -260,6 +262,12 @@ int main(int argc, char** argv) {
               };
               const SloamStepInfo a = kf.estimator().lastStep();
               const SloamStepInfo b = shadow->estimator().lastStep();
+              for (const Cylinder& a_c : out.tm)
+                for (const Cylinder& b_c : so.tm)
+                  if (a_c.id == b_c.id) {
+                    sh_dr.push_back(std::fabs(a_c.getModel().radius - b_c.getModel().radius));
+                    break;
+                  }
               double dr = -1.0;
               if (out.tm.size() == so.tm.size()) {
                 dr = 0.0;
@@ -413,7 +421,14 @@ int main(int argc, char** argv) {
     stats(sh_m, sh_deg);
     std::printf(", \"same_decisions\": {");
     stats(same_m, same_deg);
-    std::printf("}}}\n");
+    int over = 0;
+    for (double x : sh_dr) over += x > 0.01 ? 1 : 0;
+    std::printf(
+        "}, \"tree_radius_diff_m\": {\"trees\": %zu, \"p50\": %.6f, \"p99\": %.6f, "
+        "\"max\": %.6f, \"frac_over_1cm\": %.4f}}}\n",
+        sh_dr.size(), pct(sh_dr, 0.5), pct(sh_dr, 0.99),
+        sh_dr.empty() ? 0.0 : *std::max_element(sh_dr.begin(), sh_dr.end()),
+        sh_dr.empty() ? 0.0 : (double)over / sh_dr.size());
   } else {
     std::printf("\"shadow\": null}\n");
   }

@@ -67,12 +67,15 @@ def test_tree_models_and_learned_vs_plain(sequence, tmp_path):
 def test_learned_vs_plain_one_keyframe(sequence, tmp_path):
     """A plain runner fed the learned run's estimate, submap and previous
     ground models at every keyframe (keyframe_bench --shadow plain) lands
-    within 1 cm / 0.1 deg of the learned pose at every keyframe."""
+    within 1 cm / 0.1 deg of the learned pose at every keyframe, and every
+    tree both keep has its radius within 1 cm of the learned one."""
     res, _ = _run(sequence, tmp_path, "learned", extra=("--shadow", "plain"))
     sh = res["shadow"]
     print(f"[keyframe] one-keyframe learned vs plain: {sh}", flush=True)
     assert sh["compared"] >= KEYFRAMES - 2 and sh["success_differs"] == 0
     assert sh["m"]["max"] <= 0.01 and sh["deg"]["max"] <= 0.1
+    rd = sh["tree_radius_diff_m"]
+    assert rd["trees"] > 0 and rd["max"] <= 0.01
 
 
 def test_graph_replay_matches_eager(sequence, tmp_path):
