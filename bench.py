@@ -466,7 +466,7 @@ def bench_keyframe_stream(args):
         return {k: r[k] for k in ("latency_ms", "throughput_kf_per_s", "stage_p50_ms",
                                   "estimator_p50_ms", "tree_fit_p50_ms", "trees_mean",
                                   "planes_mean", "success_rate", "discarded", "trees_fitted",
-                                  "trees_fallback", "drift")}
+                                  "trees_fallback", "drift", "map_p50_ms")}
 
     g = summary(res["learned"])
     g["keyframes"] = n

@@ -39,6 +39,7 @@
 // submap is discarded (its estimate stays the odometry's).
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -173,6 +174,10 @@ int main(int argc, char** argv) {
   int samples = 0, ok_count = 0;
   double final_drift = 0, max_drift = 0, max_rot = 0;
   std::vector<double> fit, traj;
+  // per keyframe of the timed passes: the map's submap query and update
+  // (MapManager::getSubmap + updateMap, as SLOAMNode::run makes them around
+  // each keyframe; counted in the throughput, not in the latency)
+  std::vector<double> map_ms;
   long long fitted = 0, fallback = 0;
   int discarded = 0;
   std::vector<double> pass_rate;  // keyframes per second of each timed pass
@@ -226,7 +231,11 @@ int main(int argc, char** argv) {
         KeyframeTiming t;
         const SE3 before = estimate;
         std::vector<Cylinder> map;
+        const auto m0 = std::chrono::steady_clock::now();
         semanticMap.getSubmap(estimate, map);
+        double kf_map_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0)
+                .count();
         const bool discard = !nodeFirst && map.empty();
         if (pipelined) {
           // the next scan's front end runs while this keyframe's estimator does
@@ -283,7 +292,11 @@ int main(int argc, char** argv) {
               ++sh_ok_differs;
             }
           }
+          const auto m1 = std::chrono::steady_clock::now();
           semanticMap.updateMap(out.tm, out.matches);
+          kf_map_ms += std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - m1)
+                           .count();
           if (ok && nodeFirst && !out.tm.empty()) nodeFirst = false;
         }
         if (log && pass == 0) {
@@ -331,6 +344,7 @@ int main(int argc, char** argv) {
           continue;
         }
         total.push_back(t.total_ms);
+        map_ms.push_back(kf_map_ms);
         slow.push_back({t.total_ms, (double)k, t.upload_ms, t.frontend_ms, t.estimator_ms,
                         t.phases.models, t.phases.pose});
         fe.push_back(t.frontend_ms);
@@ -383,7 +397,7 @@ int main(int argc, char** argv) {
       "\"stage_p50_ms\": {\"upload\": %.4f, \"frontend\": %.4f, \"download\": %.4f, "
       "\"estimator\": %.4f}, \"estimator_p50_ms\": {\"build\": %.4f, \"models\": %.4f, "
       "\"match\": %.4f, \"pose\": %.4f, \"finish\": %.4f}, "
-      "\"frontend_p99_ms\": %.4f, \"trees_mean\": %.2f, "
+      "\"frontend_p99_ms\": %.4f, \"map_p50_ms\": %.4f, \"trees_mean\": %.2f, "
       "\"planes_mean\": %.2f, \"success_rate\": %.4f, \"drift\": {\"final_m\": %.4f, "
       "\"max_m\": %.4f, \"max_deg\": %.4f}, \"warmup_pass_latency_ms\": {\"p50\": %.4f, "
       "\"max\": %.4f}, ",
@@ -391,7 +405,7 @@ int main(int argc, char** argv) {
       samples ? [&] { double s = 0; for (double x : total) s += x; return s / samples; }() : 0.0,
       total.empty() ? 0.0 : *std::max_element(total.begin(), total.end()), pct(up, 0.5),
       pct(fe, 0.5), pct(down, 0.5), pct(est, 0.5), pct(build, 0.5), pct(ph_models, 0.5),
-      pct(ph_match, 0.5), pct(ph_pose, 0.5), pct(ph_finish, 0.5), pct(fe, 0.99),
+      pct(ph_match, 0.5), pct(ph_pose, 0.5), pct(ph_finish, 0.5), pct(fe, 0.99), pct(map_ms, 0.5),
       samples ? trees / samples : 0.0, samples ? planes / samples : 0.0,
       samples ? (double)ok_count / samples : 0.0, final_drift, max_drift, max_rot,
       pct(first, 0.5), first.empty() ? 0.0 : *std::max_element(first.begin(), first.end()));
