@@ -72,7 +72,26 @@ def parse():
     ap.add_argument("--stream-spacing", type=float, default=0.2,
                     help="metres between scans of the long stream (2 m/s at 10 Hz)")
     ap.add_argument("--stream-forest-length", type=float, default=260.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device to device three times and exit: the run "
+                         "whose rocprofv3 FETCH_SIZE / WRITE_SIZE readings scale the byte "
+                         "counters of the library's kernels")
     return ap.parse_args()
+
+
+def copy_probe():
+    """A device copy of known size (256 MiB read and 256 MiB written, three
+    times): a counter's reading for it divided by 256 MiB is the factor that
+    turns the same counter's reading for a kernel into bytes moved."""
+    import torch
+    n = 64 * 1024 * 1024  # float32 elements: 256 MiB
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 3 * n * 4}), flush=True)
 
 
 def bench_pose(ctx, args):
@@ -499,6 +518,9 @@ def main():
         print("bench.py measures one GPU only (a batch would shard by clusters, "
               "size-balanced, with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import numpy as np
     import torch

@@ -79,7 +79,13 @@ int sloam_ctx_reset_timing(sloam_ctx* ctx);
  * buffer that would have to grow during capture makes the call fail); entry
  * points that synchronise, copy from the host or capture graphs themselves
  * (sloam_frontend_exec, plan creation) do not belong in a capture.  Timing
- * is off during capture; a replay is timed as "graph".                     */
+ * is off during capture; a replay is timed as "graph".
+ * A captured graph (this one, or a front end's) holds the context scratch
+ * buffers its kernels use until it is destroyed: while it lives, a call on
+ * the same context that would need a larger scratch buffer than the one the
+ * graph holds fails with SLOAM_ERR_INVALID_ARG (sloam_ctx_last_error names
+ * the buffer) instead of moving memory the graph still writes.  Run the
+ * largest inputs before capturing, or give them a context of their own.    */
 typedef struct sloam_graph sloam_graph;
 int sloam_graph_begin(sloam_ctx* ctx);
 int sloam_graph_end(sloam_ctx* ctx, sloam_graph** out);

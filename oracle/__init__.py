@@ -360,3 +360,117 @@ def pn_loss_grad(params, xyz, offsets, targets, weights=PN_LOSS_WEIGHTS, emulate
     load().oracle_pn_loss_grad(_p(p), _p(x), _p(off), S, _p(t), _p(w), int(emulate), _p(loss),
                                _p(grad))
     return loss, grad
+
+
+# ---- per-keyframe estimator (oracle_estimator.cpp) -------------------------
+
+def _declare_estimator(lib):
+    P = C.c_void_p
+    lib.oracle_sloam_create.argtypes = [P]
+    lib.oracle_sloam_create.restype = P
+    lib.oracle_sloam_destroy.argtypes = [P]
+    lib.oracle_sloam_set_state.argtypes = [P, C.c_int, C.c_int, P]
+    lib.oracle_sloam_run.argtypes = [P, P, C.c_int, P, P, C.c_int, P, C.c_int, P, P, P, P, P,
+                                     C.c_int, P, P, P, P, C.c_int, P, P, P, P, P, P, P, P, P]
+    lib.oracle_from_two_vectors_euler.argtypes = [P, P, P, P]
+
+
+# FeatureModelParams defaults (sloam/params/sloam.yaml, and the node's
+# hard-coded maxTreeRadius 0.3 / maxAxisTheta 10)
+SLOAM_PARAMS = dict(maxLidarDist=20.0, maxTreeRadius=0.3, maxAxisTheta=10.0,
+                    defaultTreeRadius=0.2, featuresPerTree=20, numGroundFeatures=5,
+                    groundRadiiBins=2, groundThetaBins=18, treeMatchThresh=0.5,
+                    AddNewTreeThreshDist=1.5, minTreeModels=5, minGroundModels=36,
+                    twoStepOptim=1)
+_PARAM_ORDER = ("maxLidarDist", "maxTreeRadius", "maxAxisTheta", "defaultTreeRadius",
+                "featuresPerTree", "numGroundFeatures", "groundRadiiBins", "groundThetaBins",
+                "treeMatchThresh", "AddNewTreeThreshDist", "minTreeModels", "minGroundModels",
+                "twoStepOptim")
+
+
+def from_two_vectors_euler(a, b):
+    """Rotation matrix of the smallest rotation taking a onto b and its
+    Euler angles (0, 1, 2), as the plane acceptance reads them."""
+    lib = load()
+    _declare_estimator(lib)
+    R = np.zeros(9)
+    e = np.zeros(3)
+    lib.oracle_from_two_vectors_euler(_p(np.ascontiguousarray(a, np.float64)),
+                                      _p(np.ascontiguousarray(b, np.float64)), _p(R), _p(e))
+    return R.reshape(3, 3), e
+
+
+class SloamEstimator:
+    """RunSloam restated on the CPU (oracle_estimator.cpp), with its state
+    (first scan, previous ground planes)."""
+
+    def __init__(self, **params):
+        lib = load()
+        _declare_estimator(lib)
+        p = dict(SLOAM_PARAMS)
+        p.update(params)
+        arr = np.array([float(p[k]) for k in _PARAM_ORDER], np.float64)
+        self.params = p
+        self.h = lib.oracle_sloam_create(_p(arr))
+        self.lib = lib
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_sloam_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, first, prev_planes):
+        """prev_planes: [n, 7] plane[4], centroid[3]."""
+        pp = np.zeros((len(prev_planes), 8))
+        if len(prev_planes):
+            pp[:, :7] = prev_planes
+        self.lib.oracle_sloam_set_state(self.h, int(first), len(pp), _p(pp))
+
+    def run(self, rec, tree_mode=None):
+        """One RunSloam call on a keyframe record (the dict of
+        tests' keyframe dump reader: pose, cells, trees, line/cylinder fits,
+        map).  tree_mode: 0 line fits computed here, 1 the record's line
+        fits, 2 the record's cylinder fits (default: cylinder fits when the
+        record has them, else its line fits, else computed)."""
+        nt = len(rec["tree_off"]) - 1
+        if tree_mode is None:
+            tree_mode = 2 if len(rec["cyl_ok"]) == nt and nt > 0 else \
+                (1 if len(rec["line_int"]) == nt and nt > 0 else 0)
+        nc = len(rec["cell_model"])
+        nf = rec["nf"]
+        cm = np.zeros((nc, 8))
+        if nc:
+            cm[:, :7] = rec["cell_model"]
+        i32 = lambda a: np.ascontiguousarray(a, np.int32)  # noqa: E731
+        f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)  # noqa: E731
+        info = np.zeros(10, np.int32)
+        pose = np.zeros(7)
+        delta = np.zeros(7)
+        tm_id = np.zeros(max(nt, 1), np.int32)
+        tm = np.zeros((max(nt, 1), 7))
+        matches = np.zeros(max(nt, 1), np.int32)
+        valid = np.zeros(max(nt, 1), np.int32)
+        planes = np.zeros((max(nc, 1), 8))
+        nm = len(rec["map"])
+        args = [self.h, _p(f64(rec["pose"])), nc, _p(cm), _p(i32(rec["cell_valid"])), nf,
+                _p(f32(rec["cell_feat"])), nt, _p(i32(rec["tree_off"])),
+                _p(i32(rec["vtx_int"])), _p(f64(rec["vtx_f64"])), _p(i32(rec["vtx_pt_off"])),
+                _p(f32(rec["pts"])), tree_mode,
+                _p(i32(rec["line_int"])) if tree_mode == 1 else None,
+                _p(f32(rec["line_coeff"])) if tree_mode == 1 else None,
+                _p(i32(rec["cyl_ok"])) if tree_mode == 2 else None,
+                _p(f64(rec["cyl_fit"])) if tree_mode == 2 else None,
+                nm, _p(f64(rec["map"]) if nm else np.zeros(7)), _p(info), _p(pose), _p(delta),
+                _p(tm_id), _p(tm), _p(matches), _p(valid), _p(planes)]
+        self.lib.oracle_sloam_run(*args)
+        k, n_pl = int(info[6]), int(info[7])
+        return {"ok": bool(info[0]), "early": bool(info[1]), "tree_step": bool(info[2]),
+                "ground_step": bool(info[3]), "tree_matches": int(info[4]),
+                "plane_matches": int(info[5]), "first": bool(info[8]),
+                "converged": bool(info[9]), "pose": pose, "delta": delta,
+                "tm_id": tm_id[:k].copy(), "tm": tm[:k].copy(), "matches": matches[:k].copy(),
+                "tree_valid": valid[:nt].copy(), "planes": planes[:n_pl, :7].copy()}

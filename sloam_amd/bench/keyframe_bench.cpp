@@ -24,7 +24,12 @@
 // pose estimate, submap and previous ground models as the first, and
 // reports how far its pose lands from the first runner's (the one-keyframe
 // difference of the two tree models, without the compounding of two maps
-// built apart); it is not allowed with --pipelined.  The per-keyframe
+// built apart); it is not allowed with --pipelined.  --dump <file> writes,
+// for every keyframe of the first pass, the estimator's input (as the
+// runner hands it to RunSloam), its state before the call and its output
+// (format: write_dump_* below), so that another implementation of RunSloam
+// can be run on the same inputs.  --max-tree-radius and --tree-match-thresh
+// replace those two estimator parameters (sloam.yaml: 0.3 and 0.5).  The per-keyframe
 // latencies never include the shadow's work, but with a single pass that
 // pass's throughput does: give two passes or more to time with --shadow.
 //
@@ -91,11 +96,117 @@ double pct(std::vector<double> v, double q) {
   return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
 }
 
+// --dump records (little endian), one per RunSloam call of the first pass:
+//   int32 keyframe, int32 first call (1 / 0)
+//   float64 pose estimate [7] (t, q x y z w)
+//   int32 cells, int32 features per cell; per cell: float64 plane[4],
+//     centroid[3], int32 valid, float32 features [nf][4] (zeros if invalid)
+//   int32 trees; per tree: int32 vertices; per vertex: int32 tree id,
+//     int32 valid, float64 radius, float64 x, y, z (the float coordinates),
+//     int32 points, float32 points [n][4]
+//   int32 line fits; per fit: int32 ok, int32 inliers, float32 coeff[6]
+//   int32 cylinder fits; per fit: int32 ok, float64 root[3], ray[3], radius
+//   int32 map trees; per tree: float64 root[3], ray[3], radius
+//   int32 previous planes; per plane: float64 plane[4], centroid[3]
+// then the output of the call:
+//   int32 returned, int32 tree step, int32 ground step, int32 tree feature
+//   matches, int32 plane feature matches; float64 T_Map_Curr[7],
+//   T_Delta[7]; int32 trees kept; per tree: int32 id, float64 root[3],
+//   ray[3], radius, int32 match; int32 planes after; per plane: float64
+//   plane[4], centroid[3]
+struct DumpWriter {
+  FILE* f = nullptr;
+  template <typename T>
+  void put(const T& v) {
+    std::fwrite(&v, sizeof(T), 1, f);
+  }
+  void pose(const SE3& p) {
+    const Vector3 t = p.translation();
+    const Quat q = p.unit_quaternion();
+    for (double v : {t[0], t[1], t[2], q.x, q.y, q.z, q.w}) put(v);
+  }
+  void plane_model(const PlaneParameters& m) {
+    for (int i = 0; i < 4; ++i) put((double)m.plane(i));
+    for (int i = 0; i < 3; ++i) put((double)m.centroid(i));
+  }
+  void cyl_model(const CylinderParameters& m) {
+    for (int i = 0; i < 3; ++i) put((double)m.root[i]);
+    for (int i = 0; i < 3; ++i) put((double)m.ray[i]);
+    put((double)m.radius);
+  }
+  void input(int k, bool first, const SloamInput& in, const std::vector<Cylinder>& map,
+             const std::vector<Plane>& prev, int nf) {
+    put((int32_t)k);
+    put((int32_t)(first ? 1 : 0));
+    pose(in.poseEstimate);
+    put((int32_t)in.groundCells.size());
+    put((int32_t)nf);
+    for (const Plane& c : in.groundCells) {
+      plane_model(c.model);
+      put((int32_t)(c.isValid ? 1 : 0));
+      for (int j = 0; j < nf; ++j) {
+        PointT p{0, 0, 0, 0};
+        if (c.isValid && j < (int)c.features.size()) p = c.features[j];
+        for (float v : {p.x, p.y, p.z, p.intensity}) put(v);
+      }
+    }
+    put((int32_t)in.landmarks.size());
+    for (const auto& tree : in.landmarks) {
+      put((int32_t)tree.size());
+      for (const TreeVertex& v : tree) {
+        put((int32_t)v.treeId);
+        put((int32_t)(v.isValid ? 1 : 0));
+        put((double)v.radius);
+        for (float c : {v.coords.x, v.coords.y, v.coords.z}) put((double)c);
+        put((int32_t)v.points.size());
+        for (const PointT& p : v.points)
+          for (float c : {p.x, p.y, p.z, p.intensity}) put(c);
+      }
+    }
+    put((int32_t)in.treeLineFits.size());
+    for (const LineFit& l : in.treeLineFits) {
+      put((int32_t)(l.ok ? 1 : 0));
+      put((int32_t)l.inliers);
+      for (int i = 0; i < 6; ++i) put(l.coeff[i]);
+    }
+    put((int32_t)in.treeCylinderFits.size());
+    for (const CylinderFit& c : in.treeCylinderFits) {
+      put((int32_t)(c.ok ? 1 : 0));
+      for (int i = 0; i < 3; ++i) put(c.root[i]);
+      for (int i = 0; i < 3; ++i) put(c.ray[i]);
+      put(c.radius);
+    }
+    put((int32_t)map.size());
+    for (const Cylinder& c : map) cyl_model(c.model);
+    put((int32_t)prev.size());
+    for (const Plane& p : prev) plane_model(p.model);
+  }
+  void output(bool ok, const SloamStepInfo& st, const SloamOutput& out,
+              const std::vector<Plane>& after) {
+    put((int32_t)(ok ? 1 : 0));
+    put((int32_t)(st.treeCheck ? 1 : 0));
+    put((int32_t)(st.groundCheck ? 1 : 0));
+    put((int32_t)st.treeMatches);
+    put((int32_t)st.planeMatches);
+    pose(out.T_Map_Curr);
+    pose(out.T_Delta);
+    put((int32_t)out.tm.size());
+    for (size_t i = 0; i < out.tm.size(); ++i) {
+      put((int32_t)out.tm[i].id);
+      cyl_model(out.tm[i].model);
+      put((int32_t)(i < out.matches.size() ? out.matches[i] : -1));
+    }
+    put((int32_t)after.size());
+    for (const Plane& p : after) plane_model(p.model);
+  }
+};
+
 }  // namespace
 
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
-  std::string model_name = "lines", weights, traj_out, shadow_name;
+  std::string model_name = "lines", weights, traj_out, shadow_name, dump_path;
+  double max_tree_radius = -1.0, tree_match_thresh = -1.0;
   bool log = false, pinned = false, pipelined = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -105,6 +216,10 @@ int main(int argc, char** argv) {
       pinned = true;
     } else if (a == "--pipelined") {
       pipelined = true;
+    } else if ((a == "--max-tree-radius" || a == "--tree-match-thresh") && i + 1 < argc) {
+      (a == "--max-tree-radius" ? max_tree_radius : tree_match_thresh) = std::atof(argv[++i]);
+    } else if (a == "--dump" && i + 1 < argc) {
+      dump_path = argv[++i];
     } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out" ||
                 a == "--shadow") &&
                i + 1 < argc) {
@@ -123,7 +238,8 @@ int main(int argc, char** argv) {
     std::fprintf(stderr,
                  "usage: %s <sequence> [passes] [use_graph] [--tree-model lines|plain|learned] "
                  "[--weights npy] [--traj-out file] [--log] [--pinned] [--pipelined] "
-                 "[--shadow lines|plain|learned]\n",
+                 "[--shadow lines|plain|learned] [--dump file] [--max-tree-radius m] "
+                 "[--tree-match-thresh m]\n",
                  argv[0]);
     return 2;
   }
@@ -166,6 +282,16 @@ int main(int argc, char** argv) {
   std::fclose(f);
 
   FeatureModelParams fm;  // sloam.yaml
+  if (max_tree_radius > 0) fm.maxTreeRadius = max_tree_radius;
+  if (tree_match_thresh > 0) fm.treeMatchThresh = tree_match_thresh;
+  DumpWriter dump;
+  if (!dump_path.empty()) {
+    dump.f = std::fopen(dump_path.c_str(), "wb");
+    if (!dump.f) {
+      std::fprintf(stderr, "cannot open %s\n", dump_path.c_str());
+      return 2;
+    }
+  }
   std::vector<double> total, fe, up, down, est, build, ph_models, ph_match, ph_pose, ph_finish;
   // per sample: total, keyframe, upload, front end, estimator, models, pose
   std::vector<std::array<double, 7>> slow;
@@ -190,6 +316,23 @@ int main(int argc, char** argv) {
   std::vector<std::array<double, 12>> sh_rows;
   // radius differences of the trees both runners kept (paired by tree id)
   std::vector<double> sh_dr;
+  // first pass: RunSloam calls after the first, and how many of them ran the
+  // XY-yaw (tree) and Z-roll-pitch (ground) steps
+  int step_calls = 0, tree_steps = 0, ground_steps = 0;
+  // odometry alone (the increments composed from the first true pose), against
+  // the true poses: final and largest translation error, largest rotation
+  double odom_final = 0, odom_max = 0, odom_max_deg = 0;
+  {
+    SE3 o = to_se3(seq[0].truth);
+    for (int k = 1; k < K; ++k) {
+      o = o * to_se3(seq[k].odom);
+      double m, deg;
+      pose_diff(o, to_se3(seq[k].truth), &m, &deg);
+      odom_final = m;
+      odom_max = std::max(odom_max, m);
+      odom_max_deg = std::max(odom_max_deg, deg);
+    }
+  }
   try {
     std::unique_ptr<KeyframeRunner> shadow;
     if (use_shadow) {
@@ -199,6 +342,13 @@ int main(int argc, char** argv) {
     for (int pass = 0; pass < passes; ++pass) {
       KeyframeRunner kf(H, W, fm, use_graph, 22.5f, -22.5f, pipelined);
       kf.setTreeModel(tree_model, weights);
+      int dump_k = 0, dump_calls = 0;
+      if (dump.f && pass == 0)
+        kf.setInputRecorder([&](const SloamInput& in, const std::vector<Cylinder>& m) {
+          dump.input(dump_k, dump_calls == 0, in, m, kf.estimator().getPrevGroundModel(),
+                     fm.numGroundFeatures);
+          ++dump_calls;
+        });
       double staging_ms = 0.0;  // copies into the pinned buffers (not counted)
       // hands scan j to the runner: into its pinned buffers (--pinned), then
       // submitted (--pipelined) or returned for run()
@@ -252,8 +402,17 @@ int main(int argc, char** argv) {
           // submap and previous ground models
           std::vector<Plane> prevPlanes;
           if (shadow && pass == 0) prevPlanes = kf.estimator().getPrevGroundModel();
+          dump_k = k;
           ok = pipelined ? kf.finish(estimate, map, out, &t)
                          : kf.run(xyzi, label, estimate, map, out, &t);
+          if (dump.f && pass == 0)
+            dump.output(ok, kf.estimator().lastStep(), out, kf.estimator().getPrevGroundModel());
+          if (pass == 0 && !nodeFirst) {
+            const SloamStepInfo st = kf.estimator().lastStep();
+            ++step_calls;
+            tree_steps += st.treeCheck ? 1 : 0;
+            ground_steps += st.groundCheck ? 1 : 0;
+          }
           if (shadow && pass == 0) {
             shadow->estimator().setPrevGroundModel(std::move(prevPlanes));
             SloamOutput so;
@@ -376,6 +535,10 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "keyframe_bench: %s\n", e.what());
     return 1;
   }
+  if (dump.f && std::fclose(dump.f) != 0) {
+    std::fprintf(stderr, "cannot write %s\n", dump_path.c_str());
+    return 1;
+  }
   if (!traj_out.empty()) {
     FILE* o = std::fopen(traj_out.c_str(), "wb");
     if (!o || std::fwrite(traj.data(), 8, traj.size(), o) != traj.size()) {
@@ -391,6 +554,15 @@ int main(int argc, char** argv) {
       "\"pipelined\": %s, \"throughput_kf_per_s\": %.1f, ",
       model_name.c_str(), pinned ? "true" : "false", pct(fit, 0.5), fitted, fallback, discarded,
       pipelined ? "true" : "false", pct(pass_rate, 0.5));
+  std::printf(
+      "\"params\": {\"max_tree_radius\": %.4f, \"tree_match_thresh\": %.4f, "
+      "\"max_axis_theta\": %.2f, \"two_step\": %s}, \"steps\": {\"calls\": %d, "
+      "\"tree_step_frac\": %.4f, \"ground_step_frac\": %.4f}, \"odometry_drift\": "
+      "{\"final_m\": %.4f, \"max_m\": %.4f, \"max_deg\": %.4f}, ",
+      (double)fm.maxTreeRadius, (double)fm.treeMatchThresh, (double)fm.maxAxisTheta,
+      fm.twoStepOptim ? "true" : "false", step_calls,
+      step_calls ? (double)tree_steps / step_calls : 0.0,
+      step_calls ? (double)ground_steps / step_calls : 0.0, odom_final, odom_max, odom_max_deg);
   std::printf(
       "\"keyframes\": %d, \"passes\": %d, \"height\": %d, \"width\": %d, \"graph\": %s, "
       "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"mean\": %.4f, \"max\": %.4f}, "

@@ -34,6 +34,11 @@ struct sloam_ctx {
   std::vector<hipEvent_t> event_pool;
   // grow-on-demand scratch buffers, one per slot
   std::vector<std::pair<void*, size_t>> scratch;
+  // per slot: number of live captured graphs (sloam_graph, front-end graphs)
+  // whose kernels hold the slot's address; a held slot never moves
+  std::vector<int> scratch_pins;
+  // slots requested since the current capture began (bit per slot)
+  uint32_t capture_slots = 0;
   // auxiliary streams (default priority) and their fork / join events, for
   // independent launches that overlap; created on first use
   hipStream_t aux[2] = {nullptr, nullptr};
@@ -50,10 +55,15 @@ int sloam_ctx_fork(sloam_ctx* ctx, int i);
 // Makes ctx->stream wait for the work enqueued so far on aux stream i.
 int sloam_ctx_join(sloam_ctx* ctx, int i);
 
-// Scratch buffer `slot` of at least `bytes` bytes (reallocated when too small;
-// the previous contents are not kept).  Callers own a slot for the duration of
-// one entry point.
+// Scratch buffer `slot` (0..31) of at least `bytes` bytes (reallocated when
+// too small; the previous contents are not kept).  Callers own a slot for the
+// duration of one entry point.  A slot used inside a capture is held by the
+// captured graph until the graph is destroyed; a request that would move a
+// held slot fails with SLOAM_ERR_INVALID_ARG instead of freeing memory the
+// graph still writes.
 int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out);
+// Adds `delta` (+1 / -1) to the hold count of every slot in the bit set.
+void sloam_ctx_pin_slots(sloam_ctx* ctx, uint32_t slots, int delta);
 
 // Records a HIP failure in the context and returns SLOAM_ERR_HIP.
 int sloam_record_hip_error(sloam_ctx* ctx, hipError_t e, const char* what);

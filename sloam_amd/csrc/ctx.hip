@@ -56,12 +56,22 @@ void sloam_dev_free(void* p) {
 
 int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out) {
   *out = nullptr;
-  if (slot < 0) return SLOAM_ERR_INVALID_ARG;
-  if ((int)ctx->scratch.size() <= slot) ctx->scratch.resize(slot + 1, {nullptr, 0});
+  if (slot < 0 || slot > 31) return SLOAM_ERR_INVALID_ARG;
+  if ((int)ctx->scratch.size() <= slot) {
+    ctx->scratch.resize(slot + 1, {nullptr, 0});
+    ctx->scratch_pins.resize(slot + 1, 0);
+  }
   auto& s = ctx->scratch[slot];
+  if (ctx->capturing) ctx->capture_slots |= 1u << slot;
   if (s.second < bytes && ctx->capturing) {
     ctx->last_error = "scratch buffer too small during graph capture: run the sequence once "
                       "before capturing it";
+    return SLOAM_ERR_INVALID_ARG;
+  }
+  if (s.second < bytes && ctx->scratch_pins[slot] > 0) {
+    ctx->last_error = "scratch slot " + std::to_string(slot) +
+                      " is held by a captured graph and is too small for this call: destroy "
+                      "the graph, or use a context of its own for the larger inputs";
     return SLOAM_ERR_INVALID_ARG;
   }
   if (s.second < bytes) {
@@ -77,6 +87,15 @@ int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out) {
   }
   *out = s.first;
   return SLOAM_OK;
+}
+
+void sloam_ctx_pin_slots(sloam_ctx* ctx, uint32_t slots, int delta) {
+  if (!ctx) return;
+  for (int i = 0; i < 32; ++i)
+    if ((slots >> i) & 1u) {
+      if ((int)ctx->scratch_pins.size() <= i) ctx->scratch_pins.resize(i + 1, 0);
+      ctx->scratch_pins[i] += delta;
+    }
 }
 
 __global__ void sl_fill_u32_kernel(uint32_t* __restrict__ d, size_t n, uint32_t value) {
@@ -208,6 +227,7 @@ struct sloam_graph {
   sloam_ctx* ctx = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint32_t slots = 0;  // scratch slots the graph's kernels use (held until destroyed)
 };
 
 int sloam_graph_begin(sloam_ctx* ctx) {
@@ -223,6 +243,7 @@ int sloam_graph_begin(sloam_ctx* ctx) {
     return sloam_record_hip_error(ctx, e, "hipStreamBeginCapture");
   }
   ctx->capturing = true;
+  ctx->capture_slots = 0;
   return SLOAM_OK;
 }
 
@@ -250,6 +271,8 @@ int sloam_graph_end(sloam_ctx* ctx, sloam_graph** out) {
     delete r;
     return sloam_record_hip_error(ctx, ei, "hipGraphInstantiate");
   }
+  r->slots = ctx->capture_slots;
+  sloam_ctx_pin_slots(ctx, r->slots, +1);
   *out = r;
   return SLOAM_OK;
 }
@@ -268,6 +291,7 @@ void sloam_graph_destroy(sloam_graph* g) {
   if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  sloam_ctx_pin_slots(g->ctx, g->slots, -1);
   delete g;
 }
 

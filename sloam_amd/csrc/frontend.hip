@@ -15,6 +15,7 @@ struct sloam_frontend {
   sloam_frontend_out out{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint32_t slots = 0;  // context scratch slots held by the graph
   int runs = 0;
   // host_results: pinned host buffer (host and device address) and the word
   // offsets of its regions
@@ -220,6 +221,7 @@ void sloam_frontend_destroy(sloam_frontend* f) {
   if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
   if (f->exec) (void)hipGraphExecDestroy(f->exec);
   if (f->graph) (void)hipGraphDestroy(f->graph);
+  sloam_ctx_pin_slots(f->ctx, f->slots, -1);
   sloam_frontend_out& o = f->out;
   void* ptrs[] = {o.xyzi,          o.label,      o.mask,          o.ground,      o.n_ground,
                   o.tree,          o.tree_offsets, o.vertex_int,  o.vertex_f64,  o.point_offsets,
@@ -271,12 +273,16 @@ int sloam_frontend_exec(sloam_frontend* f) {
   if (f->cfg.use_graph) {
     // the eager run has sized every scratch buffer: capture the same sequence
     SLOAM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->capturing) return SLOAM_ERR_INVALID_ARG;
     const bool timing = ctx->timing;
     ctx->timing = false;
     SLOAM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    ctx->capture_slots = 0;
     st = enqueue(f);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->capturing = false;
     ctx->timing = timing;
     if (st) {
       if (g) (void)hipGraphDestroy(g);
@@ -285,6 +291,8 @@ int sloam_frontend_exec(sloam_frontend* f) {
     if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipStreamEndCapture");
     f->graph = g;
     SLOAM_HIP(ctx, hipGraphInstantiate(&f->exec, g, nullptr, nullptr, 0));
+    f->slots = ctx->capture_slots;
+    sloam_ctx_pin_slots(ctx, f->slots, +1);
   }
   return SLOAM_OK;
 }

@@ -6,7 +6,8 @@
 //               scaled by 1 / (larger horizontal half extent), z by
 //               1 / (vertical half extent); float4 (w = 1 real, 0 padding)
 //   pn_fwd      one workgroup (8 waves) per cluster, 128 points at a time:
-//               layer 1 (3 -> 64) on the VALU, layers 2 (64 -> 128) and
+//               layer 1 (3 -> 64, bias as a fourth input of 1) on
+//               v_mfma_f32_16x16x4_f32, layers 2 (64 -> 128) and
 //               3 (128 -> 256) on v_mfma_f32_16x16x32_bf16 with the BF16
 //               activations in LDS and each wave's weight fragments in VGPRs
 //               (wave w owns output channels 16w.. of layer 2, computed

@@ -355,7 +355,14 @@ TrellisResult trellis(const VectorType& cloud, int height, int width,
   CallScope scope;
   TrellisResult out;
   sloam_ctx* ctx = context();
+  if (height < 1 || width < 1 || (long long)height * width > 0x7fffffffll)
+    throw std::runtime_error("trellis: bad organisation " + std::to_string(height) + " x " +
+                             std::to_string(width));
   const int N = height * width;
+  if (cloud.size() != (size_t)N)
+    throw std::runtime_error("trellis: cloud holds " + std::to_string(cloud.size()) +
+                             " points, its organisation " + std::to_string(height) + " x " +
+                             std::to_string(width) + " needs " + std::to_string(N));
   const int cap_t = N / (cfg.min_label_points + 1) + 1;
   const int cap_v = cap_t * cfg.max_vertices;
   const int cap_p = N > 0 ? N : 1;
@@ -400,6 +407,48 @@ TrellisResult trellis(const VectorType& cloud, int height, int width,
   if (want_labels) down(out.labels, d_labels);
   check(sloam_sync(ctx), "sloam_sync");
   return out;
+}
+
+ProjectionResult projectPartition(const std::vector<float>& xyzi,
+                                  const std::vector<uint8_t>& labels, const sloam_proj_cfg& cfg,
+                                  bool want_tree) {
+  CallScope scope;
+  sloam_ctx* ctx = context();
+  const size_t n = xyzi.size() / 4;
+  if (labels.size() != n)
+    throw std::runtime_error("projectPartition: " + std::to_string(labels.size()) +
+                             " labels for " + std::to_string(n) + " points");
+  const size_t pix = (size_t)cfg.height * (size_t)cfg.width;
+  Dev d_xyzi(sizeof(float) * 4 * (n ? n : 1));
+  Dev d_label(n ? n : 1);
+  Dev d_proj(sizeof(int32_t) * 2 * (n ? n : 1));
+  Dev d_mask(pix ? pix : 1);
+  Dev d_ground(sizeof(float) * 4 * (n ? n : 1));
+  Dev d_ng(sizeof(int32_t));
+  Dev d_tree(want_tree ? sizeof(float) * 4 * (n ? n : 1) : 16);
+  up(d_xyzi, xyzi);
+  up(d_label, labels);
+  check(sloam_project_partition(ctx, d_xyzi.as<float>(), d_label.as<uint8_t>(), (int64_t)n, &cfg,
+                                d_proj.as<int32_t>(), nullptr, nullptr, d_mask.as<uint8_t>(),
+                                d_ground.as<float>(), d_ng.as<int32_t>(),
+                                want_tree ? d_tree.as<float>() : nullptr),
+        "sloam_project_partition");
+  ProjectionResult r;
+  r.proj.resize(2 * n);
+  r.mask.resize(pix);
+  std::vector<int32_t> ng(1);
+  down(r.proj, d_proj);
+  down(r.mask, d_mask);
+  down(ng, d_ng);
+  if (want_tree) {
+    r.tree.resize(4 * n);
+    down(r.tree, d_tree);
+  }
+  check(sloam_sync(ctx), "sloam_sync");
+  r.ground.resize(4 * (size_t)ng[0]);
+  down(r.ground, d_ground);
+  check(sloam_sync(ctx), "sloam_sync");
+  return r;
 }
 
 }  // namespace sloam_gpu

@@ -70,4 +70,17 @@ struct TrellisResult {
 TrellisResult trellis(const VectorType& cloud, int height, int width,
                       const sloam_trellis_cfg& cfg, bool want_labels);
 
+struct ProjectionResult {
+  std::vector<int32_t> proj;   // per point: column, row (-1, -1 if non-finite)
+  std::vector<uint8_t> mask;   // H*W: 255 tree, 1 ground, 0 other / invalid
+  std::vector<float> ground;   // ground points (x, y, z, intensity), input order
+  std::vector<float> tree;     // dense tree cloud, NaN where not tree (n x 4)
+};
+
+// sloam_project_partition of n = xyzi.size() / 4 points with per-point labels
+// (0 other, 1 ground, 2 tree); `tree` is filled when want_tree
+ProjectionResult projectPartition(const std::vector<float>& xyzi,
+                                  const std::vector<uint8_t>& labels, const sloam_proj_cfg& cfg,
+                                  bool want_tree);
+
 }  // namespace sloam_gpu

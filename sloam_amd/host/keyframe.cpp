@@ -434,6 +434,7 @@ bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
     in.treeLineFits.clear();
   }
   const auto t3b = Clock::now();
+  if (recorder_) recorder_(in, mapModels);
   const bool ok = sloam_.RunSloam(in, mapModels, out);
   const auto t4 = Clock::now();
   if (timing) {

@@ -8,6 +8,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -77,6 +78,14 @@ class KeyframeRunner {
 
   sloam::sloam& estimator() { return sloam_; }
 
+  // Called with each keyframe's estimator input right before RunSloam (its
+  // landmarks, ground cells and tree fits as the front end made them, and
+  // the map trees): for recording keyframes, e.g. to replay them elsewhere.
+  // The input is valid only during the call.
+  using InputRecorder =
+      std::function<void(const SloamInput& in, const std::vector<Cylinder>& mapModels)>;
+  void setInputRecorder(InputRecorder r) { recorder_ = std::move(r); }
+
   // Pinned host buffers of one scan (height * width x 4 floats, height *
   // width labels) of the front-end slot the next run() / submit() uses, owned
   // by the runner: a scan written here (e.g. by the sensor driver) and passed
@@ -133,6 +142,7 @@ class KeyframeRunner {
   FeatureModelParams fm_;
   sloam::sloam sloam_;
   TreeModel mode_ = TreeModel::MedianLines;
+  InputRecorder recorder_;
   sloam_cyl_plan* plain_ = nullptr;
   sloam_cyl_plan* verify_ = nullptr;
   int32_t* h_off_ = nullptr;    // pinned: tree point offsets of the last fit

@@ -18,6 +18,10 @@ int fail(const std::string& path, const std::string& why) {
   return -1;
 }
 
+// largest cloud accepted (points, width, height) and values per field
+constexpr long long kMaxPoints = 0x7fffffffll;
+constexpr int kMaxFieldCount = 1 << 16;
+
 struct Field {
   std::string name;
   int size = 4;
@@ -63,6 +67,7 @@ int loadPCDFile(const std::string& path, CloudT& cloud) {
   std::ifstream in(path, std::ios::binary);
   if (!in) return fail(path, "cannot open");
   std::vector<Field> fields;
+  constexpr size_t kMaxFields = 1024;
   std::vector<int> sizes, counts;
   std::vector<char> types;
   long long width = -1, height = 1, points = -1;
@@ -76,6 +81,7 @@ int loadPCDFile(const std::string& path, CloudT& cloud) {
     std::string tok;
     if (key == "FIELDS" || key == "COLUMNS") {
       while (ss >> tok) {
+        if (fields.size() >= kMaxFields) return fail(path, "too many FIELDS");
         Field f;
         f.name = tok;
         fields.push_back(f);
@@ -97,6 +103,8 @@ int loadPCDFile(const std::string& path, CloudT& cloud) {
     }
   }
   if (data.empty()) return fail(path, "no DATA line");
+  if (data != "ascii" && data != "binary")
+    return fail(path, "DATA " + data + " is not supported (ascii or binary)");
   if (fields.empty()) return fail(path, "no FIELDS");
   if (sizes.size() != fields.size() || types.size() != fields.size())
     return fail(path, "SIZE / TYPE do not match FIELDS");
@@ -112,15 +120,31 @@ int loadPCDFile(const std::string& path, CloudT& cloud) {
     const bool ok_type = (f.type == 'F' && (f.size == 4 || f.size == 8)) ||
                          ((f.type == 'I' || f.type == 'U') &&
                           (f.size == 1 || f.size == 2 || f.size == 4 || f.size == 8));
-    if (!ok_type || f.count < 1) return fail(path, "unsupported field " + f.name);
+    if (!ok_type || f.count < 1 || f.count > kMaxFieldCount)
+      return fail(path, "unsupported field " + f.name);
     f.offset = rec;
     f.token = token;
     rec += (size_t)f.size * f.count;
     token += f.count;
   }
-  if (width < 0 || height < 1) return fail(path, "bad WIDTH / HEIGHT");
+  // bounds before any product or allocation: a header may claim any size
+  if (width < 0 || height < 1 || width > kMaxPoints || height > kMaxPoints)
+    return fail(path, "bad WIDTH / HEIGHT");
+  if (width * height > kMaxPoints) return fail(path, "more than 2^31 - 1 points");
   if (points < 0) points = width * height;
   if (points != width * height) return fail(path, "POINTS != WIDTH * HEIGHT");
+  // the data must fit in what is left of the file (binary: rec bytes per
+  // point; ascii: at least one byte per value), checked before allocating
+  const std::streampos here = in.tellg();
+  in.seekg(0, std::ios::end);
+  const std::streampos end = in.tellg();
+  in.seekg(here);
+  if (here < 0 || end < here) return fail(path, "cannot size the data section");
+  const unsigned long long left = (unsigned long long)(end - here);
+  const unsigned long long per_point = data == "binary" ? (unsigned long long)rec
+                                                        : (unsigned long long)token;
+  if ((unsigned long long)points > 0 && per_point > left / (unsigned long long)points)
+    return fail(path, "data section shorter than POINTS records");
   const Field* fx = nullptr;
   const Field* fy = nullptr;
   const Field* fz = nullptr;

@@ -30,6 +30,7 @@
 #include "pcdIO.h"
 #include "definitions.h"
 #include "plane.h"
+#include "segmentation.h"
 #include "sloam.h"
 #include "trellis.h"
 
@@ -627,6 +628,32 @@ void pcd_io() {
     std::fclose(f);
     EXPECT_EQ(-1, sloam_io::loadPCDFile(dir + "bad2.pcd", e));
   }
+  // hostile headers fail with -1 before any allocation of their claimed size:
+  // a width * height past 2^31 - 1, a width that overflows the product, a
+  // binary section shorter than POINTS records, an absurd COUNT
+  {
+    const char* headers[] = {
+        "FIELDS x y z\nSIZE 4 4 4\nTYPE F F F\nWIDTH 2000000000\nHEIGHT 2000000000\n"
+        "DATA binary\n",
+        "FIELDS x y z\nSIZE 4 4 4\nTYPE F F F\nWIDTH 9223372036854775807\nHEIGHT 2\n"
+        "DATA ascii\n1 2 3\n",
+        "FIELDS x y z\nSIZE 4 4 4\nTYPE F F F\nWIDTH 100000000\nHEIGHT 1\nPOINTS 100000000\n"
+        "DATA binary\n0123456789ab",
+        "FIELDS x y z\nSIZE 4 4 4\nTYPE F F F\nWIDTH 1000000\nHEIGHT 1\nDATA ascii\n1 2 3\n",
+        "FIELDS x y z\nSIZE 4 4 4\nTYPE F F F\nCOUNT 1 1 2000000000\nWIDTH 1\nHEIGHT 1\n"
+        "DATA binary\n0123456789ab",
+    };
+    int k = 0;
+    for (const char* h : headers) {
+      const std::string p = dir + "hostile" + std::to_string(k++) + ".pcd";
+      FILE* f = std::fopen(p.c_str(), "wb");
+      std::fputs(h, f);
+      std::fclose(f);
+      CloudT c;
+      EXPECT_EQ(-1, sloam_io::loadPCDFile(p, c));
+      EXPECT_EQ((size_t)0, c.points.size());
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -863,6 +890,64 @@ struct Case {
 
 }  // namespace
 
+// seg::Segmentation on a labelled scan: <in> holds int32 H, W, n, destagger,
+// float32 xyzi [n][4], uint8 labels [n]; <out> receives the mask (H*W
+// bytes), then for each of ground (val 1), the dense tree cloud (val 255,
+// dense), the non-ground non-tree points (val 0) and the two first again
+// from a mask edited in a pixel of value 0 (so selected from the kept
+// projection, not taken from the GPU partition): int32 count, float32
+// points [count][4]
+int segmentation(const char* in_path, const char* out_path) {
+  FILE* f = std::fopen(in_path, "rb");
+  if (!f) return 2;
+  int32_t hdr[4];
+  rd(f, hdr, 4, in_path);
+  const int H = hdr[0], W = hdr[1], n = hdr[2];
+  std::vector<float> xyzi(4 * (size_t)n);
+  std::vector<uint8_t> labels((size_t)n);
+  rd(f, xyzi.data(), xyzi.size(), in_path);
+  rd(f, labels.data(), labels.size(), in_path);
+  std::fclose(f);
+  auto cloud = std::make_shared<CloudT>();
+  cloud->points.resize((size_t)n);
+  for (int i = 0; i < n; ++i)
+    cloud->points[i] = PointT{xyzi[4 * (size_t)i], xyzi[4 * (size_t)i + 1],
+                              xyzi[4 * (size_t)i + 2], xyzi[4 * (size_t)i + 3]};
+  cloud->width = (uint32_t)W;
+  cloud->height = (uint32_t)H;
+  // as SLOAMNode makes it (sloamNode.cpp:81): fov, -fov, width, height, depth 1
+  seg::Segmentation segm("unused.onnx", 22.5f, -22.5f, W, H, 1, hdr[3] != 0);
+  segm.setPointLabels(labels);
+  seg::MaskImage mask;
+  segm.run(cloud, mask);
+  CloudT::Ptr ground, tree, other, ground2, tree2;
+  segm.maskCloud(cloud, mask, ground, 1);
+  segm.maskCloud(cloud, mask, tree, 255, true);
+  segm.maskCloud(cloud, mask, other, 0);
+  seg::MaskImage edited = mask;
+  for (auto& b : edited.bytes)
+    if (b == 0) {
+      b = 7;
+      break;
+    }
+  segm.maskCloud(cloud, edited, ground2, 1);
+  segm.maskCloud(cloud, edited, tree2, 255, true);
+  EXPECT_EQ(tree->width, (double)W);
+  EXPECT_EQ(tree->height, (double)H);
+  EXPECT_TRUE(tree->is_dense && !ground->is_dense);
+  EXPECT_EQ(ground->width, (double)ground->points.size());
+  FILE* o = std::fopen(out_path, "wb");
+  if (!o) return 2;
+  std::fwrite(mask.data(), 1, mask.bytes.size(), o);
+  for (const CloudT::Ptr& c : {ground, tree, other, ground2, tree2}) {
+    const int32_t cnt = (int32_t)c->points.size();
+    std::fwrite(&cnt, 4, 1, o);
+    std::fwrite(c->points.data(), sizeof(PointT), c->points.size(), o);
+  }
+  std::fclose(o);
+  return g_failures ? 1 : 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s <fixture dir> [host|gpu|dump <out>]\n", argv[0]);
@@ -870,6 +955,15 @@ int main(int argc, char** argv) {
   }
   g_dir = argv[1];
   const std::string mode = argc > 2 ? argv[2] : "all";
+  if (mode == "seg") {
+    if (argc < 4) return 2;
+    try {
+      return segmentation((g_dir + "/scan.bin").c_str(), argv[3]);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "seg: %s\n", e.what());
+      return 1;
+    }
+  }
   if (mode == "dump") {
     if (argc < 4) return 2;
     try {

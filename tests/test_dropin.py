@@ -217,3 +217,126 @@ def test_pose_solves_against_oracle(tmp_path):
     zrp = o["params"][1, [2, 3, 4]] if conv[1] else np.zeros(3)
     assert np.abs(solve["xyyaw"] - xy).max() < 1e-4
     assert np.abs(solve["zrollpitch"] - zrp).max() < 1e-4
+
+
+def _core_record(stamp):
+    """SLOAMTest's input of scan `stamp` (core_test.cpp:78-117) as a record of
+    the oracle's RunSloam: the ground cells fitted by the oracle, the
+    fixture's trees with 5 points per vertex (zero-padded, as core_test
+    pads them)."""
+    xyz = np.load(os.path.join(GOLDEN, f"still_ground_{stamp}.npz"))["xyz"]
+    pts = np.zeros((len(xyz), 4), np.float32)
+    pts[:, :3] = xyz
+    g = oracle.ground_planes(pts, **CORE)
+    trees = _landmarks(f"still_landmarks_{stamp}")
+    tree_off, vtx_int, vtx_f64, vtx_pt_off, vp = [0], [], [], [0], []
+    for verts in trees:
+        for v in verts:
+            vtx_int.append((int(v["int"][0]), int(v["int"][3])))
+            vtx_f64.append((v["radius"], *v["coords"].astype(np.float64)))
+            p = np.zeros((5, 4), np.float32)
+            k = min(5, len(v["points"]))
+            p[:k, :3] = v["points"][:k]
+            vp.append(p)
+            vtx_pt_off.append(vtx_pt_off[-1] + 5)
+        tree_off.append(tree_off[-1] + len(verts))
+    return {"pose": np.array([0, 0, 0, 0, 0, 0, 1.0]), "nf": CORE["num_features"],
+            "cell_model": g["model"][:, :7], "cell_valid": g["info"][:, 0].copy(),
+            "cell_feat": g["features"], "tree_off": np.array(tree_off, np.int32),
+            "vtx_int": np.array(vtx_int, np.int32), "vtx_f64": np.array(vtx_f64),
+            "vtx_pt_off": np.array(vtx_pt_off, np.int32), "pts": np.concatenate(vp),
+            "line_int": np.zeros((0, 2), np.int32), "line_coeff": np.zeros((0, 6), np.float32),
+            "cyl_ok": np.zeros(0, np.int32), "cyl_fit": np.zeros((0, 7)),
+            "map": np.zeros((0, 7))}
+
+
+# core_params() of tests/cpp/dropin_test.cpp (core_test.cpp:93-117), in the
+# oracle estimator's names
+CORE_EST = dict(maxLidarDist=15.0, maxTreeRadius=0.3, maxAxisTheta=10.0, defaultTreeRadius=0.1,
+                featuresPerTree=2, numGroundFeatures=60, groundRadiiBins=1, groundThetaBins=18,
+                treeMatchThresh=1.0, AddNewTreeThreshDist=2.0)
+
+
+@pytest.mark.gpu
+def test_run_sloam_against_oracle_estimator(tmp_path):
+    """SLOAMTest's two scans through the drop-in (computeModels of t0, then
+    RunSloam t0 -> t1) against the CPU restatement of RunSloam on the same
+    data, in both pose modes: the same accepted planes (within 1e-9), the same
+    trees (ids; axis and radius within 1e-9), the same map matches and the
+    pose within 1e-4."""
+    _write_fixtures(str(tmp_path))
+    out = os.path.join(str(tmp_path), "dump.txt")
+    r = _run([_binary(), str(tmp_path), "dump", out])
+    assert r.returncode == 0, r.stderr
+    runs, _ = _parse_dump(out)
+    t0, t1 = _core_record("t0"), _core_record("t1")
+    for run in runs:
+        est = oracle.SloamEstimator(twoStepOptim=run["twostep"], **CORE_EST)
+        o0 = est.run(t0, tree_mode=0)
+        assert o0["first"] and o0["ok"]
+        gp = np.array(run["planes"])
+        assert gp.shape == o0["planes"].shape
+        np.testing.assert_allclose(o0["planes"], gp, rtol=0, atol=1e-9)
+        assert [t[0] for t in run["trees"]] == list(o0["tm_id"])
+        np.testing.assert_allclose(o0["tm"], np.array([t[1] for t in run["trees"]]), rtol=0,
+                                   atol=1e-9)
+        t1m = dict(t1)
+        t1m["map"] = o0["tm"]
+        o1 = est.run(t1m, tree_mode=0)
+        print(f"[dropin] two-step {run['twostep']}: oracle pose {o1['pose']}, "
+              f"drop-in {run['pose']}", flush=True)
+        assert int(o1["ok"]) == run["ok"]
+        assert list(o1["matches"]) == run["matches"]
+        g = np.array(run["pose"])
+        q = o1["pose"][3:]
+        sign = 1.0 if np.dot(q, g[3:]) >= 0 else -1.0
+        assert np.abs(g[:3] - o1["pose"][:3]).max() < 1e-4
+        assert np.abs(sign * g[3:] - q).max() < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("destagger", [False, True])
+def test_segmentation_dropin_against_oracle(tmp_path, destagger):
+    """seg::Segmentation (run with given labels, then maskCloud as SLOAMNode
+    calls it: ground with value 1, the dense tree cloud with 255) on a 64 x
+    2048 scan against the oracle's projection and partition, bit for bit
+    (the scan's beams point at pixel centres, far from the 1e-3 px band in
+    which the device's atan2f / asinf may pick the neighbouring pixel); a
+    third selection (value 0) against the
+    oracle's projection and mask; and the same two clouds again from a mask
+    edited in an unused value, which maskCloud selects from its kept
+    projection instead of taking the GPU partition."""
+    from sloam_amd import synth
+    s = synth.make_scan(seed=9, H=64, W=2048)
+    xyzi = np.ascontiguousarray(s["xyzi"], np.float32)
+    lab = np.ascontiguousarray(s["label"], np.uint8)
+    n = len(xyzi)
+    with open(os.path.join(str(tmp_path), "scan.bin"), "wb") as f:
+        f.write(struct.pack("<iiii", 64, 2048, n, int(destagger)))
+        f.write(xyzi.tobytes())
+        f.write(lab.tobytes())
+    out = os.path.join(str(tmp_path), "seg.bin")
+    r = _run([_binary(), str(tmp_path), "seg", out])
+    assert r.returncode == 0, r.stderr
+    buf = open(out, "rb").read()
+    mask = np.frombuffer(buf, np.uint8, 64 * 2048).reshape(64, 2048)
+    off = 64 * 2048
+    clouds = []
+    for _ in range(5):
+        (cnt,) = struct.unpack_from("<i", buf, off)
+        off += 4
+        clouds.append(np.frombuffer(buf, np.float32, 4 * cnt, off).reshape(cnt, 4))
+        off += 16 * cnt
+    ground, tree, other, ground2, tree2 = clouds
+    o = oracle.project_partition(xyzi, lab, H=64, W=2048, destagger=destagger)
+    np.testing.assert_array_equal(mask, o["mask"].reshape(64, 2048))
+    np.testing.assert_array_equal(ground, o["ground"])
+    np.testing.assert_array_equal(tree, o["tree"].reshape(-1, 4))
+    # value 0: points whose pixel is other / invalid, input order; points
+    # without a pixel (non-finite) count as 0
+    px = o["proj"]
+    m = np.where(px[:, 0] >= 0, o["mask"].reshape(-1)[np.maximum(px[:, 1], 0) * 2048 +
+                                                       np.maximum(px[:, 0], 0)], 0)
+    np.testing.assert_array_equal(other, xyzi[m == 0])
+    np.testing.assert_array_equal(ground2, ground)
+    np.testing.assert_array_equal(tree2, tree)

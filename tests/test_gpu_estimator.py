@@ -1,0 +1,124 @@
+"""The drop-in estimator (sloam::sloam::RunSloam in sloam_amd/host/, fed by
+the GPU front end, tree fits and pose kernels through KeyframeRunner) against
+the CPU restatement of RunSloam (oracle/oracle_estimator.cpp), keyframe by
+keyframe on the same inputs: keyframe_bench --dump records every call's
+input (ground cells, trees with their vertices and points, tree fits, the
+submap) and the estimator's state and output; the oracle is run on each
+record from the same state.
+
+Bar per keyframe: the same return value, the same XY-yaw / Z-roll-pitch step
+decisions and feature-match counts (integer, exact); the same kept trees
+(ids, in order) and the same map matches; tree models and kept ground planes,
+taken back to the sensor frame, within 1e-6 m; the pose within 1e-4 (metres,
+quaternion components) of the FP64 oracle, whose LM is the restated Ceres
+loop.  With the reference's median-line tree model the oracle also fits the
+axis lines itself (PCL's sample consensus restated), checking the front
+end's in-graph line fits."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import kf_dump
+import oracle
+from sloam_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "sloam_amd", "keyframe_bench")
+WEIGHTS = os.path.join(ROOT, "sloam_amd", "weights", "cylinder_proposal.npy")
+
+
+@pytest.fixture(scope="module")
+def sequences(tmp_path_factory):
+    d = tmp_path_factory.mktemp("est")
+    short = str(d / "short.bin")
+    synth.write_keyframe_sequence(short, n_keyframes=8, H=128)
+    # the long stream's generator (0.2 m spacing, 260 m forest), 24 scans
+    stream = str(d / "stream.bin")
+    synth.write_keyframe_sequence(stream, n_keyframes=24, H=128, spacing=0.2,
+                                  forest_length=260.0, odom_sigma_t=0.032,
+                                  odom_sigma_yaw_deg=0.32, workers=8)
+    return {"short": short, "stream": stream}
+
+
+def _dump(seq, tmp, model, extra=()):
+    path = os.path.join(str(tmp), f"dump_{model}_{'_'.join(extra)}.bin")
+    cmd = [EXE, seq, "1", "1", "--tree-model", model, "--weights", WEIGHTS, "--dump", path,
+           *extra]
+    print("[estimator]", " ".join(cmd[3:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return kf_dump.read(path)
+
+
+def _compare(recs, params, tree_mode=None):
+    est = oracle.SloamEstimator(**params)
+    worst = 0.0
+    steps = 0
+    for i, rec in enumerate(recs):
+        est.set_state(rec["first"], rec["prev_planes"])
+        o = est.run(rec, tree_mode=tree_mode)
+        g = rec["out"]
+        k = rec["keyframe"]
+        assert o["ok"] == g["ok"], k
+        assert o["tree_step"] == g["tree_step"] and o["ground_step"] == g["ground_step"], k
+        assert o["tree_matches"] == g["tree_matches"], k
+        assert o["plane_matches"] == g["plane_matches"], k
+        steps += int(g["tree_step"])
+        if o["early"]:
+            # no state change and no output
+            assert len(g["tm"]) == 0, k
+            continue
+        np.testing.assert_array_equal(o["tm_id"], g["tm_id"], err_msg=f"keyframe {k}")
+        np.testing.assert_array_equal(o["matches"], g["matches"], err_msg=f"keyframe {k}")
+        dp = np.abs(o["pose"][:3] - g["pose"][:3]).max()
+        qo, qg = o["pose"][3:], g["pose"][3:]
+        dq = min(np.abs(qo - qg).max(), np.abs(qo + qg).max())
+        worst = max(worst, dp, dq)
+        assert dp <= 1e-4 and dq <= 1e-4, (k, dp, dq)
+        # models in the sensor frame (each taken back by its own pose)
+        to = kf_dump.unproject_trees(o["tm"], o["pose"])
+        tg = kf_dump.unproject_trees(g["tm"], g["pose"])
+        np.testing.assert_allclose(to, tg, rtol=0, atol=1e-6, err_msg=f"keyframe {k}")
+        po = kf_dump.unproject_planes(o["planes"], o["pose"])
+        pg = kf_dump.unproject_planes(g["planes"], g["pose"])
+        assert po.shape == pg.shape, k
+        np.testing.assert_allclose(po, pg, rtol=0, atol=1e-6, err_msg=f"keyframe {k}")
+        # the state the drop-in carries into the next keyframe is its output
+        if i + 1 < len(recs):
+            nxt = recs[i + 1]["prev_planes"]
+            np.testing.assert_array_equal(nxt, g["planes"])
+    return worst, steps
+
+
+@pytest.mark.parametrize("model", ["lines", "plain", "learned"])
+def test_estimator_matches_oracle_short(sequences, tmp_path, model):
+    recs = _dump(sequences["short"], tmp_path, model)
+    assert len(recs) >= 6
+    worst, steps = _compare(recs, {})
+    print(f"[estimator] {model}: {len(recs)} keyframes, tree steps {steps}, "
+          f"largest pose difference {worst:.2e}", flush=True)
+
+
+def test_estimator_oracle_fits_lines_itself(sequences, tmp_path):
+    """Median-line model: the oracle fits every tree's axis line to the
+    vertex medians itself instead of taking the front end's fits."""
+    recs = _dump(sequences["short"], tmp_path, "lines")
+    worst, _ = _compare(recs, {}, tree_mode=0)
+    print(f"[estimator] lines, oracle line fits: largest pose difference {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("model", ["plain", "learned"])
+def test_estimator_matches_oracle_stream(sequences, tmp_path, model):
+    """The long stream's scans with the stream's parameters (radius cap
+    0.45 m, tree match threshold 1.0 m): the XY-yaw step runs on most
+    keyframes, and every decision and pose agrees with the oracle."""
+    recs = _dump(sequences["stream"], tmp_path, model,
+                 extra=("--max-tree-radius", "0.45", "--tree-match-thresh", "1.0"))
+    worst, steps = _compare(recs, {"maxTreeRadius": 0.45, "treeMatchThresh": 1.0})
+    print(f"[estimator] stream {model}: {len(recs)} keyframes, tree steps {steps}, "
+          f"largest pose difference {worst:.2e}", flush=True)
+    assert steps >= len(recs) // 2

@@ -74,3 +74,45 @@ def test_learned_fit_graph_replays_match_eager(ctx):
             assert np.array_equal(v.cpu().numpy(), ref[k]), (rep, k)
     g.close()
     fit.close()
+
+
+def test_graph_holds_its_scratch():
+    """A captured graph holds the context scratch its kernels use: a later
+    call on the same context that needs a larger buffer is refused (status
+    -1, no memory moved under the graph) until the graph is destroyed; the
+    graph's replays stay correct meanwhile."""
+    from sloam_amd._lib import SloamError
+    from sloam_amd.gpu import Context, project_partition_dev
+
+    c = Context(0)
+    try:
+        dev = torch.device("cuda", 0)
+        small = synth.make_scan(seed=21, H=16, W=512)
+        big = synth.make_scan(seed=22, H=64, W=2048)
+        xs = torch.as_tensor(small["xyzi"]).to(dev)
+        ls = torch.as_tensor(small["label"]).to(dev)
+        xb = torch.as_tensor(big["xyzi"]).to(dev)
+        lb = torch.as_tensor(big["label"]).to(dev)
+        torch.cuda.synchronize()
+        out = project_partition_dev(c, xs, ls, H=16, W=512)
+        c.sync()
+        ref = {k: v.cpu().numpy() for k, v in out.items()}
+        c.graph_begin()
+        project_partition_dev(c, xs, ls, H=16, W=512, out=out)
+        g = c.graph_end()
+        print("[graph] larger call while the graph lives", flush=True)
+        with pytest.raises(SloamError, match="status -1"):
+            project_partition_dev(c, xb, lb, H=64, W=2048)
+        for v in out.values():
+            v.fill_(-3)
+        torch.cuda.synchronize()
+        g.launch()
+        c.sync()
+        for k, v in out.items():
+            assert np.array_equal(v.cpu().numpy(), ref[k], equal_nan=True), k
+        g.close()
+        big_out = project_partition_dev(c, xb, lb, H=64, W=2048)  # the slot may move now
+        c.sync()
+        assert int(big_out["n_ground"].cpu()[0]) > 0
+    finally:
+        c.close()
