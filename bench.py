@@ -72,6 +72,13 @@ def parse():
     ap.add_argument("--stream-spacing", type=float, default=0.2,
                     help="metres between scans of the long stream (2 m/s at 10 Hz)")
     ap.add_argument("--stream-forest-length", type=float, default=260.0)
+    ap.add_argument("--learned-cpu-clusters", type=int, default=512,
+                    help="clusters of the learned leg's CPU baseline sample")
+    ap.add_argument("--kf-cpu-frames", type=int, default=12,
+                    help="keyframes the CPU baseline of the keyframe sequence times")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the small-scan keyframe leg (64x2048, 40 trunks)")
+    ap.add_argument("--small-steps", type=int, default=50)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device to device three times and exit: the run "
                          "whose rocprofv3 FETCH_SIZE / WRITE_SIZE readings scale the byte "
@@ -178,6 +185,18 @@ def bench_pose(ctx, args):
         small_ms[name] = e0.elapsed_time(e1) / args.pose_steps
         sp.close()
     plan.close()
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        oracle.load()
+        med, tot = faithful(lambda: cpu_median(
+            lambda: oracle.pose_lm_batch(prob, threads=1, max_iters=iters)))
+        cpu = {"value": 64 * iters / med, "unit": "LM iterations/s", "cores": 1, "kind": "port",
+               "ms_per_solve": med * 1e3,
+               "sample": f"the same 64 problems x {iters} iterations through the oracle's FP64 "
+                         f"LM built with the reference's flags (-O3 -DNDEBUG), 1 thread, "
+                         f"median of 5 after 1 warm-up, {tot:.1f} s in total",
+               "gpu_speedup": med * 1e3 / ms}
     return {
         "workload": f"64 problems x (500 cylinders + 20 ground patches), {n_res} residuals, "
                     f"{iters} LM iterations",
@@ -192,6 +211,113 @@ def bench_pose(ctx, args):
         "fused_ms_per_solve": fused_ms,
         "fused_bitwise_equal": bool(fused_same),
         "keyframe_size_solve_ms": small_ms,
+        "cpu_baseline": cpu,
+    }
+
+
+def host_cpu():
+    """Model name (lscpu) and the number of cores this process may run on."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return {"model": model, "cores_available": n}
+
+
+def faithful(fn):
+    """Runs fn with the oracle built with the reference's flags selected."""
+    import oracle
+    oracle.select("faithful")
+    try:
+        return fn()
+    finally:
+        oracle.select("tuned")
+
+
+def cpu_median(fn, reps=5):
+    """Median wall time (s) of reps runs of fn after one warm-up run, and
+    the total time spent."""
+    t_all = time.perf_counter()
+    fn()
+    times = []
+    for _ in range(reps):
+        c0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - c0)
+    times.sort()
+    return times[len(times) // 2], time.perf_counter() - t_all
+
+
+def bench_small_keyframe(ctx, args):
+    """The small-scan plain keyframe: one 64 x 2048 scan of 40 trunks,
+    projection + partition, tree instances, per-tree cylinder RANSAC (200
+    hypotheses, 5 cm), ground RANSAC (200 hypotheses) and the LM pose against
+    the 40-trunk map from a 0.3 m / 3 deg perturbation.  Latency of the whole
+    keyframe from device-resident inputs (it synchronises three times: for
+    the tree and ground counts, for the fits the association reads, and for
+    the pose); the same steps on the CPU through the oracle."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from sloam_amd import synth
+    from sloam_amd.plain_keyframe import PlainKeyframe
+
+    s, submap, plane, est = synth.make_small_keyframe(0)
+    dev = torch.device("cuda", 0)
+    x = torch.as_tensor(s["xyzi"]).to(dev)
+    lab = torch.as_tensor(s["label"]).to(dev)
+    torch.cuda.synchronize()
+    kf = PlainKeyframe(ctx)
+    for _ in range(3):
+        r = kf.run(x, lab, est, submap, plane)
+    lat = []
+    for _ in range(args.small_steps):
+        t0 = time.perf_counter()
+        r = kf.run(x, lab, est, submap, plane)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    kf.close()
+    lat = np.array(lat)
+    et, er = synth.pose_error(r["pose"], np.array([0, 0, 0, 0, 0, 0, 1.0]))
+    cpu = None
+    if not args.no_cpu:
+        oracle.load()
+        holder = {}
+
+        def run_cpu():
+            holder["o"] = oracle.plain_keyframe(s["xyzi"], s["label"], est, submap, plane,
+                                                threads=1)
+
+        med, tot = faithful(lambda: cpu_median(run_cpu))
+        o = holder["o"]
+        dp = float(np.abs(o["pose"][:3] - r["pose"][:3]).max())
+        cpu = {"ms": med * 1e3, "cores": 1, "kind": "port",
+               "sample": f"the same keyframe through the oracle built with the reference's "
+                         f"flags (-O3 -DNDEBUG), 1 thread (projection, trellis, fits, "
+                         f"association in numpy, LM), median of 5 after 1 warm-up, "
+                         f"{tot:.1f} s in total",
+               "gpu_speedup": med * 1e3 / float(np.median(lat)),
+               "pose_diff_vs_gpu_m": dp}
+    return {
+        "workload": "one 64x2048 scan, 40 trunks (r U(0.1,0.4) m, tilt <= 5 deg, sigma 2 cm), "
+                    "ground slope <= 5 deg sigma 3 cm, 30 % clutter; per-tree cylinder RANSAC "
+                    "200 hyps at 5 cm, ground RANSAC 200 hyps, LM pose vs the 40-trunk map "
+                    "from 0.3 m / 3 deg",
+        "latency_ms": {"p50": float(np.median(lat)), "p99": float(np.percentile(lat, 99)),
+                       "min": float(lat.min())},
+        "trees": int(r["n_trees"]), "matched": int(np.sum(r["matches"] >= 0)),
+        "ground_points": int(r["n_ground"]),
+        "pose_err_vs_truth": {"m": float(et), "deg": float(np.degrees(er))},
+        "cpu_baseline": cpu,
     }
 
 
@@ -264,6 +390,22 @@ def bench_learned(ctx, args):
     fwd_s = kern["pn_fwd"] / 1e3
     plain.close()
     fit.close()
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        oracle.load()
+        k = min(args.learned_cpu_clusters, S)
+        sub_off = d["offsets"][:k + 1]
+        sub_xyz = d["xyz"][:int(sub_off[-1])]
+        params = proposal.load_weights()
+        med, tot = faithful(lambda: cpu_median(
+            lambda: oracle.learned_fit(params, sub_xyz, sub_off, plain_hyp=args.hyp)))
+        cpu = {"value": k / med, "unit": "cylinder fits/s", "cores": 1, "kind": "port",
+               "sample": f"the first {k} of the {S} clusters: the network in FP64, 16 "
+                         f"verifying hypotheses, plain {args.hyp}-hypothesis fallback, through "
+                         f"the oracle built with the reference's flags (-O3 -DNDEBUG), 1 "
+                         f"thread, median of 5 after 1 warm-up, {tot:.1f} s in total",
+               "gpu_speedup": (S / (ms_learned / 1e3)) / (k / med)}
     return {
         "workload": f"{S} clusters (log-U[50,5000] points, 20 % outliers), proposal net on "
                     f"<= 512 points each, 16 verifying hypotheses, plain {args.hyp}-hypothesis "
@@ -280,6 +422,7 @@ def bench_learned(ctx, args):
                                     "frac_over_1cm": float(np.mean(err > 1.0))},
         "kernel_ms_per_batch": kern,
         "size_bins": bins,
+        "cpu_baseline": cpu,
         "net_mfma": {"achieved": net_flop / fwd_s / 1e12, "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": net_flop / fwd_s / 1e12 / BF16_PEAK_TFLOPS,
                      "flop_per_launch": net_flop},
@@ -340,11 +483,20 @@ def traj_diff(a, b):
             "max_deg": float(ang.max()), "mean_deg": float(ang.mean())}
 
 
-def run_keyframe_bench(path, passes, model, pinned, pipelined, traj_path, graph=1, shadow=None):
+# estimator parameters of the keyframe streams: the forest's trunks reach
+# 0.4 m radius (the reference's hard-coded 0.3 m cap would drop them) and a
+# 3 deg heading error moves a trunk 0.3 m at 6 m height (the yaml's 0.5 m
+# tree match threshold is marginal)
+KF_PARAMS = ("--max-tree-radius", "0.45", "--tree-match-thresh", "1.0")
+
+
+def run_keyframe_bench(path, passes, model, pinned, pipelined, traj_path, graph=1, shadow=None,
+                       extra=KF_PARAMS):
     """One run of the C++ program sloam_amd/keyframe_bench; returns its JSON
     line and the trajectory it wrote.  shadow: a second tree model run beside
     `model` on every keyframe of the first pass, from the same estimate and
-    submap (the one-keyframe pose difference of the two)."""
+    submap (the one-keyframe pose difference of the two); extra: further
+    options (estimator parameters, --dump)."""
     import subprocess
 
     import numpy as np
@@ -356,7 +508,7 @@ def run_keyframe_bench(path, passes, model, pinned, pipelined, traj_path, graph=
                         "--weights", weights, "--traj-out", traj_path] +
                        (["--pinned"] if pinned else []) +
                        (["--pipelined"] if pipelined else []) +
-                       (["--shadow", shadow] if shadow else []),
+                       (["--shadow", shadow] if shadow else []) + list(extra),
                        capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         raise RuntimeError(f"keyframe_bench failed: {r.stderr[-2000:]}")
@@ -365,6 +517,57 @@ def run_keyframe_bench(path, passes, model, pinned, pipelined, traj_path, graph=
             print(f"[bench] {line}", file=sys.stderr, flush=True)
     return (json.loads(r.stdout.strip().splitlines()[-1]),
             np.fromfile(traj_path, dtype=np.float64).reshape(-1, 7))
+
+
+def keyframe_cpu_baseline(args, seq_path, tmp, gpu_plain):
+    """The keyframe's work on the CPU: for keyframes 1..args.kf_cpu_frames of
+    the sequence, the oracle's projection and partition, tree instances,
+    ground cells, plain cylinder fits (1024 hypotheses) of every tree and
+    RunSloam, from the estimate, submap and previous ground planes the GPU
+    runner (plain tree model) had at that keyframe (keyframe_bench --dump).
+    Faithful build (the reference's -O3 -DNDEBUG), one thread; each keyframe
+    timed once after one warm-up keyframe.  Also reports how far the CPU's
+    pose lands from the GPU runner's."""
+    import numpy as np
+
+    import oracle
+    from sloam_amd import synth
+    from sloam_amd import keyframe_dump as kf_dump
+
+    dump = os.path.join(tmp, "plain.dump")
+    run_keyframe_bench(seq_path, 1, "plain", 1, 0, os.path.join(tmp, "traj_dump.bin"),
+                       extra=KF_PARAMS + ("--dump", dump))
+    recs = [r for r in kf_dump.read(dump) if not r["first"]][:args.kf_cpu_frames + 1]
+    frames, (_, H, W) = synth.read_keyframe_sequence(seq_path, {r["keyframe"] for r in recs})
+    oracle.select("faithful")
+    try:
+        est = oracle.SloamEstimator(maxTreeRadius=0.45, treeMatchThresh=1.0)
+        times, dpose, agree = [], [], 0
+        for i, r in enumerate(recs):
+            _, _, xyzi, lab = frames[r["keyframe"]]
+            est.set_state(False, r["prev_planes"])
+            t0 = time.perf_counter()
+            o = oracle.keyframe_step(est, xyzi, lab, r["pose"], r["map"], H=H, W=W)
+            dt = time.perf_counter() - t0
+            if i == 0:
+                continue  # warm-up
+            times.append(dt)
+            if o["ok"] and r["out"]["ok"]:
+                dpose.append(float(np.abs(o["pose"][:3] - r["out"]["pose"][:3]).max()))
+            agree += int(o["ok"] == r["out"]["ok"] and
+                         o["tree_step"] == r["out"]["tree_step"])
+    finally:
+        oracle.select("tuned")
+    med = float(np.median(times))
+    return {"ms_per_keyframe": med * 1e3, "cores": 1, "kind": "port",
+            "sample": f"keyframes {recs[1]['keyframe']}-{recs[-1]['keyframe']} of the sequence "
+                      f"({len(times)} timed after one warm-up keyframe), oracle built with the "
+                      f"reference's flags (-O3 -DNDEBUG), 1 thread: front end, ground cells, "
+                      f"plain cylinder fits (1024 hypotheses), RunSloam",
+            "gpu_plain_latency_p50_ms": gpu_plain["latency_ms"]["p50"],
+            "gpu_speedup_vs_plain_p50": med * 1e3 / gpu_plain["latency_ms"]["p50"],
+            "same_decisions": f"{agree} of {len(times)}",
+            "pose_diff_vs_gpu_max_m": max(dpose) if dpose else None}
 
 
 def bench_keyframes(args):
@@ -401,6 +604,7 @@ def bench_keyframes(args):
                 path, args.kf_passes, model, pinned, pipelined,
                 os.path.join(d, f"traj_{name}.bin"), graph=graph,
                 shadow="plain" if name == "learned" else None)
+        cpu = None if args.no_cpu else keyframe_cpu_baseline(args, path, d, res["plain"])
 
     g = res["learned"]
     g["drift_vs_plain"] = traj_diff(traj["learned"], traj["plain"])
@@ -422,6 +626,7 @@ def bench_keyframes(args):
                            "upload_p50_ms": res["learned_pageable"]["stage_p50_ms"]["upload"]}
     g["lines_eager_frontend_p50_ms"] = res["lines_eager"]["stage_p50_ms"]["frontend"]
     g["lines_eager_latency_p50_ms"] = res["lines_eager"]["latency_ms"]["p50"]
+    g["cpu_baseline"] = cpu
     g["workload"] = (f"{args.keyframes} keyframes of {args.kf_height}x2048 scans, 0.5 m apart, "
                      f"300-tree forest, 40 % understory clutter; {args.kf_passes} passes, the "
                      f"first a warm-up reported apart; learned tree models (plain and the "
@@ -620,6 +825,9 @@ def main():
     model = out["model"].cpu().numpy()
     rerr = np.abs(model[:, 6] - d["radius"])
 
+    small = None
+    if not args.no_small:
+        small = bench_small_keyframe(ctx, args)
     pose = None
     if not args.no_pose:
         pose = bench_pose(ctx, args)
@@ -644,37 +852,36 @@ def main():
     cpu_threaded = None
     if not args.no_cpu:
         import oracle
-        # bounded sample: the whole batch, fitted three times (median), one
-        # thread (the reference is single-threaded); then the same with
-        # --cpu-pool threads over clusters
+        # bounded sample: the whole batch, fitted six times (one warm-up, the
+        # median of five), one thread (the reference is single-threaded); then
+        # the same with --cpu-pool threads over clusters
         oracle.load()
 
-        def cpu_fit(threads, reps):
-            times = []
-            for _ in range(reps):
-                c0 = time.perf_counter()
-                oracle.cyl_fit_batch(d["xyz"], d["offsets"], args.hyp, thr=thr,
-                                     max_radius=max_radius, seed=seed, counts=False, mask=True,
-                                     threads=threads)
-                times.append(time.perf_counter() - c0)
-            return float(np.median(times)), float(np.sum(times))
+        def cpu_fit(threads):
+            return cpu_median(lambda: oracle.cyl_fit_batch(
+                d["xyz"], d["offsets"], args.hyp, thr=thr, max_radius=max_radius, seed=seed,
+                counts=False, mask=True, threads=threads))
 
-        med, tot = cpu_fit(args.cpu_threads, 3)
+        med, tot = faithful(lambda: cpu_fit(1))
+        host = host_cpu()
         cpu = {
             "value": args.clusters / med,
             "unit": "cylinder fits/s",
-            "cores": args.cpu_threads,
+            "cores": 1,
             "kind": "port",
+            "host_cpu": host,
             "sample": f"all {args.clusters} clusters ({n_points} points), {args.hyp} "
-                      f"hypotheses, same seed, oracle C++ -O3, {args.cpu_threads} thread(s); "
-                      f"median of 3 runs, {tot:.1f} s in total",
+                      f"hypotheses, same seed, oracle built with the reference's flags (-O3 "
+                      f"-DNDEBUG), 1 thread (the reference is single-threaded); median of 5 "
+                      f"runs after 1 warm-up, {tot:.1f} s in total",
         }
         if args.cpu_pool > 1:
-            med_t, tot_t = cpu_fit(args.cpu_pool, 3)
+            med_t, tot_t = cpu_fit(args.cpu_pool)
             cpu_threaded = {"value": args.clusters / med_t, "unit": "cylinder fits/s",
                             "cores": args.cpu_pool,
-                            "sample": f"the same, {args.cpu_pool} threads over clusters; "
-                                      f"median of 3 runs, {tot_t:.1f} s in total"}
+                            "sample": f"the same, oracle built -O3 -mfma -fopenmp, "
+                                      f"{args.cpu_pool} threads over clusters; median of 5 "
+                                      f"runs after 1 warm-up, {tot_t:.1f} s in total"}
 
     # keyframe headline: the long stream when it ran, else the short sequence
     head, head_name = None, None
@@ -736,6 +943,7 @@ def main():
         "keyframe_drift_vs_plain": None if head is None else head["drift_vs_plain"],
         "keyframe_step_vs_plain": None if head is None else head["step_vs_plain"],
         "keyframe_headline": head_name,
+        "small_keyframe": small,
         "pose": pose,
         "learned": learned,
         "keyframe": keyframes,

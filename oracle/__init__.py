@@ -12,21 +12,40 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# the reference's Release flags, one thread (oracle/Makefile)
+FAITHFUL_PATH = os.path.join(HERE, "liboracle_faithful.so")
 
 _lib = None
+_libs = {}
+_variant = "tuned"
 
 
 def build():
     subprocess.run(["make", "-C", HERE, "-j8", "-s"], check=True)
 
 
+def select(variant):
+    """Which build the following calls use: "tuned" (liboracle.so, the
+    checker) or "faithful" (liboracle_faithful.so, the reference's flags)."""
+    global _variant, _lib
+    if variant not in ("tuned", "faithful"):
+        raise ValueError(variant)
+    _variant = variant
+    _lib = None
+
+
 def load():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = LIB_PATH if _variant == "tuned" else FAITHFUL_PATH
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB_PATH)
-        _declare(_lib)
+        if path not in _libs:
+            lib = C.CDLL(path)
+            _declare(lib)
+            _declare_estimator(lib)
+            _libs[path] = lib
+        _lib = _libs[path]
     return _lib
 
 
@@ -392,7 +411,6 @@ def from_two_vectors_euler(a, b):
     """Rotation matrix of the smallest rotation taking a onto b and its
     Euler angles (0, 1, 2), as the plane acceptance reads them."""
     lib = load()
-    _declare_estimator(lib)
     R = np.zeros(9)
     e = np.zeros(3)
     lib.oracle_from_two_vectors_euler(_p(np.ascontiguousarray(a, np.float64)),
@@ -406,7 +424,6 @@ class SloamEstimator:
 
     def __init__(self, **params):
         lib = load()
-        _declare_estimator(lib)
         p = dict(SLOAM_PARAMS)
         p.update(params)
         arr = np.array([float(p[k]) for k in _PARAM_ORDER], np.float64)
@@ -474,3 +491,151 @@ class SloamEstimator:
                 "converged": bool(info[9]), "pose": pose, "delta": delta,
                 "tm_id": tm_id[:k].copy(), "tm": tm[:k].copy(), "matches": matches[:k].copy(),
                 "tree_valid": valid[:nt].copy(), "planes": planes[:n_pl, :7].copy()}
+
+
+# ---- small-scan plain keyframe (the GPU's sloam_amd/plain_keyframe.py) ------
+
+def _axis_gap(root_a, ray_a, root_b, ray_b):
+    """Cylinder::distance(model) (sloam/src/objects/cylinder.cpp:175-194):
+    mean over z = 0, 3, 6 of |a(z) - b(z)|, a(z) = root + ((z - root_z) /
+    ray_z) ray."""
+    tot = 0.0
+    for z in (0.0, 3.0, 6.0):
+        a = root_a + ((z - root_a[2]) / ray_a[2]) * ray_a
+        b = root_b + ((z - root_b[2]) / ray_b[2]) * ray_b
+        tot += float(np.sqrt(np.sum((a - b) ** 2)))
+    return tot / 3.0
+
+
+def plain_keyframe(xyzi, label, estimate, submap, map_plane, H=64, W=2048, n_hyp=200, thr=0.05,
+                   max_radius=1.0, plane_hyp=200, max_tree_radius=0.45, tree_match=1.0,
+                   features_per_tree=20, ground_features=256, seed=0x5EED, threads=1):
+    """The small-scan plain-method keyframe on the CPU: projection and
+    partition, tree instances, per-tree cylinder RANSAC, ground RANSAC over
+    the ground list, association (each valid tree with radius <=
+    max_tree_radius to the map tree of smallest axis gap, strict <, if below
+    tree_match; its first features_per_tree points; every k-th ground inlier
+    up to ground_features) and the FP64 LM pose solve (Huber 0.1, 50
+    iterations, stop at convergence)."""
+    pp = project_partition(xyzi, label, H=H, W=W)
+    tr = trellis(pp["tree"], H, W)
+    nt = len(tr["tree_offsets"]) - 1
+    offsets = tr["point_offsets"][tr["tree_offsets"]].astype(np.int32) if nt else \
+        np.zeros(1, np.int32)
+    txyz = np.ascontiguousarray(tr["points"][:, :3])
+    gxyz = np.ascontiguousarray(pp["ground"][:, :3])
+    cyl = cyl_fit_batch(txyz, offsets, n_hyp, thr=thr, max_radius=max_radius, seed=seed,
+                        samples=True, threads=threads) if nt else None
+    g_off = np.array([0, len(gxyz)], np.int32)
+    pl = plane_fit_batch(gxyz, g_off, plane_hyp, thr=thr, seed=seed, tag=1, samples=True,
+                         threads=threads)
+    est = np.asarray(estimate, np.float64)
+    x, y, z, w = est[3:]
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    submap = np.asarray(submap, np.float64)
+    pts, ids = [], []
+    matches = np.full(nt, -1, np.int32)
+    for k in range(nt):
+        m = cyl["model"][k]
+        if not (0.0 < m[6] <= max_tree_radius) or len(submap) == 0:
+            continue
+        root = R @ m[:3] + est[:3]
+        ray = R @ (m[:3] + m[3:6]) + est[:3] - root
+        best, bj = tree_match + 100.0, -1
+        for j in range(len(submap)):
+            d = _axis_gap(submap[j, :3], submap[j, 3:6], root, ray)
+            if d < best:
+                best, bj = d, j
+        if not best < tree_match:
+            continue
+        matches[k] = bj
+        p = txyz[offsets[k]:offsets[k + 1]][:features_per_tree]
+        pts.append(p)
+        ids.append(np.full(len(p), bj, np.int32))
+    if pl["model"][0, 7] > 0 and len(gxyz):
+        inl = [i for i in range(len(gxyz)) if (pl["mask"][i >> 5] >> (i & 31)) & 1]
+        if inl:
+            step = max(1, -(-len(inl) // ground_features))
+            g = gxyz[inl[::step]]
+            pts.append(g)
+            ids.append(np.full(len(g), -1, np.int32))
+    P = np.concatenate(pts) if pts else np.zeros((0, 3), np.float32)
+    I = np.concatenate(ids) if ids else np.zeros(0, np.int32)
+    rec = np.empty((len(P), 4), np.float32)
+    rec[:, :3] = P
+    rec[:, 3] = I.view(np.float32)
+    table = np.zeros((max(len(submap), 1), 8))
+    table[:len(submap), :7] = submap
+    prob = {"points": rec, "res_offsets": np.array([0, len(rec)], np.int32), "cyl": table,
+            "planes": np.asarray(map_plane, np.float64).reshape(1, 4), "pose_init": est[None]}
+    sol = pose_lm_batch(prob, threads=1, max_iters=50, stop_on_convergence=True)
+    return {"pose": sol["pose"][0], "status": sol["status"][0], "matches": matches,
+            "proj": pp, "trellis": tr, "tree_offsets": offsets, "tree_xyz": txyz,
+            "ground_xyz": gxyz, "cyl": cyl, "plane": pl, "problem": prob}
+
+
+def learned_fit(params, xyz, offsets, n_hyp=16, plain_hyp=1024, thr=0.05, max_radius=1.0,
+                min_inlier_ratio=0.5, seed=0x5EED, threads=1):
+    """The learned fit on the CPU (the CPU baseline of the GPU's
+    sloam_cyl_learned_exec): proposals of the network in FP64, n_hyp
+    verifying hypotheses with the proposal as hypothesis 0 and LO refit;
+    the clusters whose best verifying hypothesis holds fewer than
+    min_inlier_ratio x n points are fitted again by the plain n_hyp-hypothesis
+    method."""
+    off = np.ascontiguousarray(offsets, np.int32)
+    prop = pn_forward(params, xyz, off, emulate=False)["proposal"].astype(np.float32)
+    v = cyl_fit_batch(xyz, off, n_hyp, thr=thr, max_radius=max_radius, seed=seed, tag=1,
+                      counts=True, mask=False, threads=threads, proposal=prop)
+    n = np.diff(off)
+    S = len(n)
+    best_count = v["counts"][np.arange(S), v["best"]]
+    fb = np.nonzero(best_count < min_inlier_ratio * n)[0]
+    model = v["model"].copy()
+    if len(fb):
+        pts = np.concatenate([xyz[off[c]:off[c + 1]] for c in fb])
+        sub = np.concatenate([[0], np.cumsum(n[fb])]).astype(np.int32)
+        p = cyl_fit_batch(pts, sub, plain_hyp, thr=thr, max_radius=max_radius, seed=seed, tag=0,
+                          counts=False, mask=False, threads=threads)
+        model[fb] = p["model"]
+    return {"model": model, "n_fallback": len(fb)}
+
+
+def keyframe_step(est, xyzi, label, pose, submap, H=128, W=2048, n_hyp=1024, thr=0.05,
+                  max_radius=1.0, seed=0x5EED, ground=None, threads=1):
+    """One keyframe of the keyframe stream on the CPU (the CPU baseline of the
+    GPU's KeyframeRunner with the plain tree model): projection and
+    partition, tree instances, ground cells (sloam.yaml: 2 x 18 bins of
+    5-25 m, bottom 5 %, 5 features), the plain cylinder fit of every tree's
+    points, then RunSloam (est: a SloamEstimator holding the state) against
+    the submap [K, 7]."""
+    g = dict(radii_bins=2, theta_bins=18, min_dist=5.0, max_dist=25.0, retain_thresh=0.05,
+             num_features=5)
+    if ground:
+        g.update(ground)
+    pp = project_partition(xyzi, label, H=H, W=W)
+    tr = trellis(pp["tree"], H, W)
+    cells = ground_planes(pp["ground"], **g)
+    nt = len(tr["tree_offsets"]) - 1
+    vo = tr["point_offsets"]
+    offsets = vo[tr["tree_offsets"]].astype(np.int32) if nt else np.zeros(1, np.int32)
+    if nt:
+        fit = cyl_fit_batch(np.ascontiguousarray(tr["points"][:, :3]), offsets, n_hyp, thr=thr,
+                            max_radius=max_radius, seed=seed, counts=False, mask=False,
+                            threads=threads)
+        model = fit["model"]
+    else:
+        model = np.zeros((0, 8))
+    rec = {"pose": np.asarray(pose, np.float64), "nf": g["num_features"],
+           "cell_model": cells["model"][:, :7], "cell_valid": cells["info"][:, 0].copy(),
+           "cell_feat": cells["features"], "tree_off": tr["tree_offsets"],
+           "vtx_int": np.ascontiguousarray(tr["vertex_int"][:, [0, 3]]),
+           "vtx_f64": tr["vertex_f64"].astype(np.float64), "vtx_pt_off": vo,
+           "pts": tr["points"], "line_int": np.zeros((0, 2), np.int32),
+           "line_coeff": np.zeros((0, 6), np.float32),
+           "cyl_ok": ((model[:, 6] > 0) & (model[:, 7] >= 3)).astype(np.int32) if nt else
+           np.zeros(0, np.int32),
+           "cyl_fit": np.ascontiguousarray(model[:, :7]),
+           "map": np.asarray(submap, np.float64).reshape(-1, 7)}
+    return est.run(rec, tree_mode=2 if nt else 0)

@@ -134,7 +134,7 @@ struct DumpWriter {
     for (int i = 0; i < 3; ++i) put((double)m.ray[i]);
     put((double)m.radius);
   }
-  void input(int k, bool first, const SloamInput& in, const std::vector<Cylinder>& map,
+  void input(int k, bool first, const SloamInput& in, const MapView& map,
              const std::vector<Plane>& prev, int nf) {
     put((int32_t)k);
     put((int32_t)(first ? 1 : 0));
@@ -177,7 +177,7 @@ struct DumpWriter {
       put(c.radius);
     }
     put((int32_t)map.size());
-    for (const Cylinder& c : map) cyl_model(c.model);
+    for (const Cylinder* c : map) cyl_model(c->model);
     put((int32_t)prev.size());
     for (const Plane& p : prev) plane_model(p.model);
   }
@@ -344,7 +344,7 @@ int main(int argc, char** argv) {
       kf.setTreeModel(tree_model, weights);
       int dump_k = 0, dump_calls = 0;
       if (dump.f && pass == 0)
-        kf.setInputRecorder([&](const SloamInput& in, const std::vector<Cylinder>& m) {
+        kf.setInputRecorder([&](const SloamInput& in, const MapView& m) {
           dump.input(dump_k, dump_calls == 0, in, m, kf.estimator().getPrevGroundModel(),
                      fm.numGroundFeatures);
           ++dump_calls;
@@ -380,7 +380,7 @@ int main(int argc, char** argv) {
         SloamOutput out;
         KeyframeTiming t;
         const SE3 before = estimate;
-        std::vector<Cylinder> map;
+        MapView map;  // the submap by reference (valid until updateMap)
         const auto m0 = std::chrono::steady_clock::now();
         semanticMap.getSubmap(estimate, map);
         double kf_map_ms =

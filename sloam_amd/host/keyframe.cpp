@@ -306,9 +306,28 @@ KeyframeRunner::~KeyframeRunner() {
   }
 }
 
+namespace {
+MapView view_of(const std::vector<Cylinder>& trees) {
+  MapView v;
+  v.reserve(trees.size());
+  for (const Cylinder& c : trees) v.push_back(&c);
+  return v;
+}
+}  // namespace
+
 bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
                          const std::vector<Cylinder>& mapModels, SloamOutput& out,
                          KeyframeTiming* timing) {
+  return run(xyzi, labels, poseEstimate, view_of(mapModels), out, timing);
+}
+
+bool KeyframeRunner::finish(const SE3& poseEstimate, const std::vector<Cylinder>& mapModels,
+                            SloamOutput& out, KeyframeTiming* timing) {
+  return finish(poseEstimate, view_of(mapModels), out, timing);
+}
+
+bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
+                         const MapView& mapModels, SloamOutput& out, KeyframeTiming* timing) {
   if (pending_) throw std::runtime_error("keyframe: run() with submitted scans pending");
   Slot& sl = slots_[next_];
   next_ = (next_ + 1) % (int)slots_.size();
@@ -338,8 +357,8 @@ void KeyframeRunner::submit(const float* xyzi, const uint8_t* labels) {
   ++pending_;
 }
 
-bool KeyframeRunner::finish(const SE3& poseEstimate, const std::vector<Cylinder>& mapModels,
-                            SloamOutput& out, KeyframeTiming* timing) {
+bool KeyframeRunner::finish(const SE3& poseEstimate, const MapView& mapModels, SloamOutput& out,
+                            KeyframeTiming* timing) {
   if (!pending_) throw std::runtime_error("keyframe: finish() with no submitted scan");
   const int n = (int)slots_.size();
   Slot& sl = slots_[(next_ - pending_ + n) % n];
@@ -358,7 +377,7 @@ void KeyframeRunner::skip() {
 }
 
 bool KeyframeRunner::estimate(const Slot& sl, const SE3& poseEstimate,
-                              const std::vector<Cylinder>& mapModels, SloamOutput& out,
+                              const MapView& mapModels, SloamOutput& out,
                               KeyframeTiming* timing, Clock::time_point t0,
                               Clock::time_point t1, Clock::time_point t2) {
   if (timing) {

@@ -58,6 +58,9 @@ class KeyframeRunner {
   bool run(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
            const std::vector<Cylinder>& mapModels, SloamOutput& out,
            KeyframeTiming* timing = nullptr);
+  // the same with the map trees by reference (MapManager::getSubmap's view)
+  bool run(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
+           const MapView& mapModels, SloamOutput& out, KeyframeTiming* timing = nullptr);
 
   // The same step in two halves, for a stream of scans: submit() enqueues a
   // scan's upload and front end and returns; finish() waits for the oldest
@@ -69,6 +72,8 @@ class KeyframeRunner {
   // front end.
   void submit(const float* xyzi, const uint8_t* labels);
   bool finish(const SE3& poseEstimate, const std::vector<Cylinder>& mapModels, SloamOutput& out,
+              KeyframeTiming* timing = nullptr);
+  bool finish(const SE3& poseEstimate, const MapView& mapModels, SloamOutput& out,
               KeyframeTiming* timing = nullptr);
   // releases the oldest submitted scan without running the estimator (a
   // keyframe the caller discards)
@@ -83,7 +88,7 @@ class KeyframeRunner {
   // the map trees): for recording keyframes, e.g. to replay them elsewhere.
   // The input is valid only during the call.
   using InputRecorder =
-      std::function<void(const SloamInput& in, const std::vector<Cylinder>& mapModels)>;
+      std::function<void(const SloamInput& in, const MapView& mapModels)>;
   void setInputRecorder(InputRecorder r) { recorder_ = std::move(r); }
 
   // Pinned host buffers of one scan (height * width x 4 floats, height *
@@ -129,7 +134,7 @@ class KeyframeRunner {
   void makeFrontend();
   // the estimator on slot s's front-end results; t0..t2: submit, upload done,
   // front end done
-  bool estimate(const Slot& s, const SE3& poseEstimate, const std::vector<Cylinder>& mapModels,
+  bool estimate(const Slot& s, const SE3& poseEstimate, const MapView& mapModels,
                 SloamOutput& out, KeyframeTiming* timing, std::chrono::steady_clock::time_point t0,
                 std::chrono::steady_clock::time_point t1,
                 std::chrono::steady_clock::time_point t2);

@@ -72,10 +72,19 @@ const M& model_of(const ModelMatch<M>& m) {
 // matchFeatures (sloam.cpp:443-471) without the copies: the same nearest map
 // model (strict <, first wins) for each current object's projected model, and
 // one match per feature of the current object.
-template <typename T, typename M>
+// the map objects as a list of references: a vector of objects or a view
+template <typename T>
+const T& map_at(const std::vector<T>& v, size_t i) {
+  return v[i];
+}
+template <typename T>
+const T& map_at(const std::vector<const T*>& v, size_t i) {
+  return *v[i];
+}
+
+template <typename T, typename M, typename MapList>
 std::vector<ModelMatch<M>> match_features(const SE3& tf, const std::vector<T>& currObjects,
-                                          const std::vector<T>& mapObjects,
-                                          const Scalar distThresh) {
+                                          const MapList& mapObjects, const Scalar distThresh) {
   std::vector<ModelMatch<M>> matches;
   if (mapObjects.empty()) return matches;
   for (const auto& co : currObjects) {
@@ -83,7 +92,7 @@ std::vector<ModelMatch<M>> match_features(const SE3& tf, const std::vector<T>& c
     Scalar bestDist = distThresh + 100;
     size_t best = 0;
     for (size_t i = 0; i < mapObjects.size(); ++i) {
-      const Scalar d = mapObjects[i].distance(pm);
+      const Scalar d = map_at(mapObjects, i).distance(pm);
       if (d < bestDist) {
         bestDist = d;
         best = i;
@@ -91,9 +100,31 @@ std::vector<ModelMatch<M>> match_features(const SE3& tf, const std::vector<T>& c
     }
     if (bestDist < distThresh)
       for (const auto& f : co.features)
-        matches.push_back(ModelMatch<M>{Vector3(f.x, f.y, f.z), &mapObjects[best].model, bestDist});
+        matches.push_back(
+            ModelMatch<M>{Vector3(f.x, f.y, f.z), &map_at(mapObjects, best).model, bestDist});
   }
   return matches;
+}
+
+// matchModels (sloam.cpp:299-328) over either map list form
+template <typename T, typename MapList>
+void match_models(const std::vector<T>& currObjects, const MapList& mapObjects,
+                  const FeatureModelParams& fm, std::vector<int>& matchIndices) {
+  if (mapObjects.empty()) return;
+  size_t counter = 0;
+  for (const auto& co : currObjects) {
+    Scalar bestDist = fm.treeMatchThresh + 100;
+    size_t bestKey = 0;
+    for (size_t key = 0; key < mapObjects.size(); ++key) {
+      const Scalar d = map_at(mapObjects, key).distance(co.model);
+      if (d < bestDist) {
+        bestDist = d;
+        bestKey = key;
+      }
+    }
+    if (bestDist < fm.AddNewTreeThreshDist) matchIndices[counter] = (int)bestKey;
+    counter++;
+  }
 }
 
 // Consecutive matches against the same landmark share one table entry (the
@@ -311,21 +342,7 @@ void sloam::addFeatureMatches(const VectorType& features, const T& object, const
 template <typename T>
 void sloam::matchModels(const std::vector<T>& currObjects, const std::vector<T>& mapObjects,
                         std::vector<int>& matchIndices) {
-  if (mapObjects.empty()) return;
-  size_t counter = 0;
-  for (const auto& co : currObjects) {
-    Scalar bestDist = fmParams_.treeMatchThresh + 100;
-    size_t bestKey = 0;
-    for (size_t key = 0; key < mapObjects.size(); ++key) {
-      const Scalar d = mapObjects[key].distance(co.model);
-      if (d < bestDist) {
-        bestDist = d;
-        bestKey = key;
-      }
-    }
-    if (bestDist < fmParams_.AddNewTreeThreshDist) matchIndices[counter] = (int)bestKey;
-    counter++;
-  }
+  match_models(currObjects, mapObjects, fmParams_, matchIndices);
 }
 
 template std::vector<ObjectMatch<Cylinder>> sloam::matchFeatures<Cylinder>(
@@ -466,6 +483,13 @@ double ms_between(PhaseClock::time_point a, PhaseClock::time_point b) {
 bool sloam::RunSloam(SloamInput& in, SloamOutput& out) { return RunSloam(in, in.mapModels, out); }
 
 bool sloam::RunSloam(SloamInput& in, const std::vector<Cylinder>& mapModels, SloamOutput& out) {
+  MapView view;
+  view.reserve(mapModels.size());
+  for (const Cylinder& c : mapModels) view.push_back(&c);
+  return RunSloam(in, view, out);
+}
+
+bool sloam::RunSloam(SloamInput& in, const MapView& mapModels, SloamOutput& out) {
   phase_ = SloamPhaseTimes();
   step_ = SloamStepInfo();
   const auto p0 = PhaseClock::now();
@@ -518,7 +542,7 @@ bool sloam::RunSloam(SloamInput& in, const std::vector<Cylinder>& mapModels, Slo
   phase_.pose = ms_between(p2, p3);
 
   projectModels(currPose, landmarks, planes);
-  matchModels(landmarks, mapModels, matchIndices);
+  match_models(landmarks, mapModels, fmParams_, matchIndices);
   prevGPlanes_ = std::move(planes);
   out.matches = matchIndices;
   out.T_Map_Curr = currPose;

@@ -51,6 +51,10 @@ struct SloamOutput {
   SE3 T_Delta;
 };
 
+// map trees handed to RunSloam by reference (e.g. MapManager::getSubmap's
+// view): pointers valid for the duration of the call
+using MapView = std::vector<const Cylinder*>;
+
 // ground cells [radii bin][theta bin] (the reference's boost::multi_array)
 using GroundGrid = std::vector<std::vector<VectorType>>;
 
@@ -84,6 +88,7 @@ class sloam {
   // is not read): a caller that keeps its submap need not copy it, with
   // every tree's vertices and their points, into the input each keyframe.
   bool RunSloam(SloamInput& in, const std::vector<Cylinder>& mapModels, SloamOutput& out);
+  bool RunSloam(SloamInput& in, const MapView& mapModels, SloamOutput& out);
 
   // Pose optimization
   void OptimizeXYYaw(const SE3& poseEstimate, const bool optimize,

@@ -1,4 +1,4 @@
-"""Reader of keyframe_bench --dump files: per RunSloam call of the first pass,
+"""Reader of sloam_amd/keyframe_bench --dump files: per RunSloam call of the first pass,
 the estimator's input as the keyframe runner hands it over, its state before
 the call and its output (record layout: sloam_amd/bench/keyframe_bench.cpp,
 DumpWriter)."""

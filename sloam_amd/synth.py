@@ -551,6 +551,45 @@ def split_two_step(prob):
     return out
 
 
+def make_small_keyframe(seed=0, t_err=0.3, r_err_deg=3.0):
+    """The small-scan keyframe workload: make_scan(seed, 64 x 2048, 40 trunks),
+    its trunks as the map (root, axis, radius; the world frame is the sensor
+    frame of the true pose, the identity), the ground plane (n, d) and an
+    estimate t_err metres and r_err_deg degrees off the true pose, in
+    directions drawn from seed 100 + seed.  Returns (scan, submap [40, 7],
+    map plane [4], estimate (t, q x y z w))."""
+    s = make_scan(seed=seed, H=64, W=2048)
+    submap = np.array([[*t["base"], *t["axis"], t["radius"]] for t in s["trees"]])
+    rng = np.random.default_rng(100 + seed)
+    dt = rng.normal(size=3)
+    dt *= t_err / np.linalg.norm(dt)
+    ax = rng.normal(size=3)
+    ax *= math.radians(r_err_deg) / np.linalg.norm(ax)
+    ang = np.linalg.norm(ax)
+    q = np.concatenate([ax / ang * math.sin(ang / 2), [math.cos(ang / 2)]])
+    return s, submap, s["ground"], np.concatenate([dt, q])
+
+
+def read_keyframe_sequence(path, frames=None):
+    """Reads a keyframe sequence file (write_keyframe_sequence): the frames
+    whose index is in `frames` (all when None), as a dict index -> (true pose
+    [7], odometry increment [7], xyzi [H*W, 4] float32, labels [H*W] uint8),
+    plus the (K, H, W) header."""
+    with open(path, "rb") as f:
+        K, H, W = np.frombuffer(f.read(12), "<i4")
+        rec = 112 + int(H) * int(W) * 17
+        out = {}
+        for k in range(int(K)):
+            if frames is not None and k not in frames:
+                f.seek(rec, 1)
+                continue
+            hdr = np.frombuffer(f.read(112), "<f8")
+            xyzi = np.frombuffer(f.read(int(H) * int(W) * 16), "<f4").reshape(-1, 4)
+            lab = np.frombuffer(f.read(int(H) * int(W)), np.uint8)
+            out[k] = (hdr[:7].copy(), hdr[7:].copy(), xyzi.copy(), lab.copy())
+    return out, (int(K), int(H), int(W))
+
+
 if __name__ == "__main__":
     # python -m sloam_amd.synth PATH N_KEYFRAMES [spacing forest_length odom_sigma_t
     #     odom_sigma_yaw_deg workers H]: writes a keyframe sequence file from a

@@ -526,6 +526,23 @@ void map_manager() {
   for (size_t i = 0; i < sb.size(); ++i)
     ordered = ordered && std::fabs(sb[i].model.root[0] - 0.01 * (51 + (int)i)) < 1e-6;
   EXPECT_TRUE(ordered);
+  // the submap by reference: the same trees in the same order, no copies;
+  // an update through it moves the referenced landmark
+  MapView view;
+  big.getSubmap(SE3(), view);
+  EXPECT_EQ(sb.size(), view.size());
+  bool same = view.size() == sb.size();
+  for (size_t i = 0; same && i < view.size(); ++i)
+    same = view[i]->model.root[0] == sb[i].model.root[0];
+  EXPECT_TRUE(same);
+  std::vector<Cylinder> moved = {tree(0.6, 0.3)};
+  big.updateMap(moved, {0});
+  EXPECT_EQ((size_t)250, big.size());
+  MapView view2;
+  big.getSubmap(SE3(), view2);
+  bool found = false;
+  for (const Cylinder* c : view2) found = found || std::fabs(c->model.root[1] - 0.3) < 1e-9;
+  EXPECT_TRUE(found);
 }
 
 // PCD files (pcdIO.h): organised ascii and binary round trips with NaN

@@ -20,8 +20,8 @@ import subprocess
 import numpy as np
 import pytest
 
-import kf_dump
 import oracle
+from sloam_amd import keyframe_dump as kf_dump
 from sloam_amd import synth
 
 pytestmark = pytest.mark.gpu
