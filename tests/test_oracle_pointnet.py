@@ -85,3 +85,54 @@ def test_proposal_frame():
         assert np.isclose(o["proposal"][c, 6], o["out"][c, 6] / sh, rtol=1e-6)
         v = o["out"][c, :3] / sc
         assert np.allclose(o["proposal"][c, :3], v / np.linalg.norm(v), rtol=1e-9)
+
+
+def _torch_forward(params, xyz, offsets):
+    """The network written again in torch, FP64, with the BF16 roundings done
+    by torch's own float -> bfloat16 conversion: weights and biases, and the
+    layer-1 and layer-2 activations after the ReLU; masked max-pool (channels
+    whose outputs are all <= 0 pool to 0)."""
+    import torch
+
+    def bf(t):
+        return t.to(torch.bfloat16).to(torch.float64)
+
+    W = []
+    p = torch.as_tensor(np.asarray(params, np.float32))
+    for fi, fo, ow, ob in proposal.LAYERS:
+        W.append((bf(p[ow:ow + fi * fo]).reshape(fo, fi), bf(p[ob:ob + fo])))
+    outs = []
+    for c in range(len(offsets) - 1):
+        P = xyz[offsets[c]:offsets[c + 1]].astype(np.float32)
+        n = len(P)
+        m = min(n, 512)
+        Q = P[(np.arange(m, dtype=np.int64) * n) // m]
+        lo, hi = Q.min(0), Q.max(0)
+        cen = np.float32(0.5) * (lo + hi)
+        half = np.float32(0.5) * (hi - lo)
+        sh = np.float32(1) / max(np.float32(1e-3), max(half[0], half[1]))
+        sz = np.float32(1) / max(np.float32(1e-3), half[2])
+        X = ((Q - cen) * np.array([sh, sh, sz], np.float32)).astype(np.float32)
+        x = torch.as_tensor(X.astype(np.float64))
+        h1 = bf(torch.relu(x @ W[0][0].T + W[0][1]))
+        h2 = bf(torch.relu(h1 @ W[1][0].T + W[1][1]))
+        s3 = h2 @ W[2][0].T + W[2][1]
+        f = torch.clamp(s3.max(0).values, min=0.0)
+        h4 = torch.relu(W[3][0] @ f + W[3][1])
+        outs.append((W[4][0] @ h4 + W[4][1]).numpy())
+    return np.array(outs)
+
+
+def test_forward_matches_independent_torch_bf16():
+    """oracle_pn_forward (emulated BF16) against the torch restatement of the
+    five layers, the max-pool and the BF16 roundings: the outputs agree to
+    FP64 summation-order noise, on trained and random weights."""
+    d = synth.make_clusters(12, seed=21, n_min=40, n_max=2000)
+    for params in (proposal.load_weights(), proposal.init_params(seed=5)):
+        ref = _torch_forward(params, d["xyz"], d["offsets"])
+        o = oracle.pn_forward(params, d["xyz"], d["offsets"], emulate=True)["out"]
+        assert np.allclose(o, ref, rtol=1e-9, atol=1e-12), np.abs(o - ref).max()
+    # the rounding matters: without it the outputs move by far more
+    un = oracle.pn_forward(proposal.load_weights(), d["xyz"], d["offsets"], emulate=False)["out"]
+    ref = _torch_forward(proposal.load_weights(), d["xyz"], d["offsets"])
+    assert np.abs(un - ref).max() > 1e-4
