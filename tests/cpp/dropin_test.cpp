@@ -30,6 +30,7 @@
 #include "pcdIO.h"
 #include "definitions.h"
 #include "plane.h"
+#include "rosmsg.h"
 #include "segmentation.h"
 #include "sloam.h"
 #include "trellis.h"
@@ -548,6 +549,188 @@ void map_manager() {
 // PCD files (pcdIO.h): organised ascii and binary round trips with NaN
 // points, fields found by name in another order with extra fields of other
 // types and counts, and the failures PCL reports as -1.
+// ROS 1 wire format: byte layouts worked out by hand from the
+// serialisation rules (little-endian scalars, uint32 counts before strings and
+// T[], fixed arrays inline), round trips of every message type, the
+// PointXYZI layout of pcl::toROSMsg, fields by name at any offset and type,
+// and malformed input refused.
+void ros_messages() {
+  using namespace sloam_wire;
+  {
+    Header h;
+    h.seq = 7;
+    h.stamp = {1, 2};
+    h.frame_id = "map";
+    std::vector<uint8_t> b;
+    serialize(h, b);
+    const uint8_t want[] = {7, 0, 0, 0, 1, 0, 0, 0, 2, 0, 0, 0, 3, 0, 0, 0, 'm', 'a', 'p'};
+    EXPECT_EQ(b.size(), sizeof(want));
+    EXPECT_TRUE(b.size() == sizeof(want) && std::memcmp(b.data(), want, sizeof(want)) == 0);
+    EXPECT_EQ(serializedLength(h), 19);
+    Header back;
+    EXPECT_TRUE(deserialize(b.data(), b.size(), back));
+    EXPECT_TRUE(back.seq == 7 && back.stamp.nsec == 2 && back.frame_id == "map");
+    EXPECT_TRUE(!deserialize(b.data(), b.size() - 1, back));  // truncated string
+    b.push_back(0);
+    EXPECT_TRUE(!deserialize(b.data(), b.size(), back));  // trailing byte
+  }
+  {
+    ROSCylinder c;
+    c.root[0] = 1.5f;
+    c.ray[2] = 1.0f;
+    c.radii = {0.25};
+    c.radius = 0.5f;
+    c.id = -3;
+    std::vector<uint8_t> b;
+    serialize(c, b);
+    // 3 + 3 floats, count 1, one double, one float, int64
+    EXPECT_EQ(b.size(), 12 + 12 + 4 + 8 + 4 + 8);
+    float root0;
+    std::memcpy(&root0, b.data(), 4);
+    EXPECT_EQ(root0, 1.5);
+    uint32_t cnt;
+    std::memcpy(&cnt, b.data() + 24, 4);
+    EXPECT_EQ(cnt, 1);
+    int64_t id;
+    std::memcpy(&id, b.data() + 40, 8);
+    EXPECT_EQ((double)id, -3);
+    // a count larger than the bytes left is refused before any allocation
+    std::vector<uint8_t> bad = b;
+    const uint32_t huge = 0x7fffffffu;
+    std::memcpy(bad.data() + 24, &huge, 4);
+    ROSCylinder back;
+    EXPECT_TRUE(!deserialize(bad.data(), bad.size(), back));
+  }
+  // pcl::toROSMsg layout of PointXYZI and the way back
+  CloudT cloud;
+  cloud.width = 3;
+  cloud.height = 2;
+  for (int i = 0; i < 6; ++i) {
+    PointT p;
+    p.x = 0.5f * i;
+    p.y = -1.0f * i;
+    p.z = 2.0f + i;
+    p.intensity = 10.0f * i;
+    cloud.points.push_back(p);
+  }
+  cloud.points[4].x = std::numeric_limits<float>::quiet_NaN();
+  cloud.is_dense = false;
+  PointCloud2 msg;
+  toROSMsg(cloud, msg);
+  EXPECT_TRUE(msg.height == 2 && msg.width == 3 && msg.point_step == 32 && msg.row_step == 96);
+  EXPECT_EQ(msg.fields.size(), 4);
+  EXPECT_TRUE(msg.fields[3].name == "intensity" && msg.fields[3].offset == 16 &&
+              msg.fields[3].datatype == FLOAT32 && msg.fields[3].count == 1);
+  EXPECT_EQ(msg.data.size(), 6 * 32);
+  float v;
+  std::memcpy(&v, msg.data.data() + 5 * 32 + 16, 4);
+  EXPECT_EQ(v, 50.0);
+  std::vector<uint8_t> b;
+  serialize(msg, b);
+  // header (seq, stamp, empty frame id: 16), height, width, fields (count,
+  // then per field 4 + name + offset 4 + datatype 1 + count 4), bigendian,
+  // steps, data (count + 192), is_dense
+  const size_t fields = 4 + (4 + 1 + 9) * 3 + (4 + 9 + 9);
+  EXPECT_EQ(b.size(), 16 + 8 + fields + 1 + 8 + 4 + 192 + 1);
+  PointCloud2 msg2;
+  EXPECT_TRUE(deserialize(b.data(), b.size(), msg2));
+  CloudT back;
+  EXPECT_TRUE(fromROSMsg(msg2, back));
+  EXPECT_TRUE(back.width == 3 && back.height == 2 && !back.is_dense);
+  bool same = back.points.size() == 6;
+  for (size_t i = 0; same && i < 6; ++i) {
+    const PointT &a = cloud.points[i], &c = back.points[i];
+    same = std::memcmp(&a, &c, sizeof(PointT)) == 0;
+  }
+  EXPECT_TRUE(same);
+  // another producer's layout: intensity as uint16 first, then x y z as
+  // float64, a padded row; fields found by name
+  PointCloud2 other;
+  other.height = 2;
+  other.width = 1;
+  other.fields = {PointField{"intensity", 0, UINT16, 1}, PointField{"x", 8, FLOAT64, 1},
+                  PointField{"y", 16, FLOAT64, 1}, PointField{"z", 24, FLOAT64, 1}};
+  other.point_step = 32;
+  other.row_step = 40;
+  other.data.assign(80, 0);
+  for (int r = 0; r < 2; ++r) {
+    const uint16_t inten = (uint16_t)(100 + r);
+    std::memcpy(other.data.data() + 40 * r, &inten, 2);
+    const double xyz[3] = {1.25 + r, -2.5, 3.0};
+    std::memcpy(other.data.data() + 40 * r + 8, xyz, 24);
+  }
+  other.is_dense = true;
+  CloudT oc;
+  EXPECT_TRUE(fromROSMsg(other, oc));
+  EXPECT_TRUE(oc.points.size() == 2 && oc.points[1].x == 2.25f && oc.points[1].y == -2.5f &&
+              oc.points[1].z == 3.0f && oc.points[1].intensity == 101.0f && oc.is_dense);
+  // malformed: short data, field past point_step, big-endian
+  PointCloud2 bad = other;
+  bad.data.resize(60);
+  EXPECT_TRUE(!fromROSMsg(bad, oc));
+  bad = other;
+  bad.fields[1].offset = 28;
+  EXPECT_TRUE(!fromROSMsg(bad, oc));
+  bad = other;
+  bad.is_bigendian = true;
+  EXPECT_TRUE(!fromROSMsg(bad, oc));
+  // an observation as sloamNode would publish it, through bytes and back
+  ROSObservation obs;
+  obs.header.frame_id = "map";
+  obs.header.stamp = {1700000000u, 5u};
+  SE3 pose(Quat(0.9, 0.1, -0.2, 0.3), Vector3(1.0, 2.0, 3.0));
+  obs.pose.pose = toPose(pose);
+  obs.initialGuess.pose = toPose(SE3());
+  obs.pc = msg;
+  ROSCylinder cyl;
+  cyl.radii = {0.1, 0.2};
+  cyl.id = 9;
+  obs.treeModels = {cyl, cyl};
+  obs.ground.coefs[2] = 1.0f;
+  obs.ground.features = msg;
+  obs.matches = {3, -1, 4};
+  obs.success = true;
+  b.clear();
+  serialize(obs, b);
+  ROSObservation ob;
+  EXPECT_TRUE(deserialize(b.data(), b.size(), ob));
+  std::vector<uint8_t> b2;
+  serialize(ob, b2);
+  EXPECT_TRUE(b == b2);
+  EXPECT_TRUE(ob.matches.size() == 3 && ob.matches[2] == 4 && ob.success &&
+              ob.treeModels.size() == 2 && ob.treeModels[1].radii[1] == 0.2);
+  const SE3 pb = fromPose(ob.pose.pose);
+  EXPECT_NEAR(pb.unit_quaternion().x, pose.unit_quaternion().x, 1e-15);
+  EXPECT_NEAR(pb.translation()[2], 3.0, 0);
+  for (size_t cut = 0; cut < b.size(); cut += 97)
+    EXPECT_TRUE(!deserialize(b.data(), cut, ob));  // every prefix is refused
+  // the remaining message types round-trip through their bytes
+  ROSSubMap sm;
+  sm.treeModels = {cyl};
+  sm.t_map_anchor.covariance[35] = 0.5;
+  ROSSweep sw;
+  ROSScan sc;
+  sc.treeModels = {cyl};
+  sw.scans = {sc, sc};
+  LoopClosure lc;
+  lc.source_idx = 4;
+  ObservationPair op;
+  op.source = obs;
+  op.target_idx = 2;
+  auto round = [](const auto& m) {
+    std::vector<uint8_t> x, y;
+    serialize(m, x);
+    typename std::decay<decltype(m)>::type back;
+    if (!deserialize(x.data(), x.size(), back)) return false;
+    serialize(back, y);
+    return x == y && serializedLength(m) == x.size();
+  };
+  EXPECT_TRUE(round(sm));
+  EXPECT_TRUE(round(sw));
+  EXPECT_TRUE(round(lc));
+  EXPECT_TRUE(round(op));
+}
+
 void pcd_io() {
   const std::string dir = g_dir + "/";
   CloudT c;
@@ -998,6 +1181,7 @@ int main(int argc, char** argv) {
       {"CylinderTest.FromCylinderFit", false, cylinder_from_fit},
       {"MapManager", false, map_manager},
       {"PCD.ReadWrite", false, pcd_io},
+      {"ROSMsg.WireFormat", false, ros_messages},
       {"PlaneTest.Initalizes", true, plane_initializes},
       {"PlaneTest.DistanceToFeature", true, plane_distance_to_feature},
       {"PlaneTest.TranslateModel", true, plane_translate_model},
