@@ -148,6 +148,22 @@ def bench_pose(ctx, args):
     eval_bytes = 16.0 * n_res + 32.0 * len(prob["cyl"]) + 16.0 * len(prob["planes"])
     res = {k: v.cpu().numpy() for k, v in out.items()}
     dt, dr = synth.pose_error(res["pose"], prob["pose_true"])
+    # the batched loop (2 launches per iteration) replayed as one HIP graph,
+    # which removes the per-launch gaps between the iterations
+    ctx.graph_begin()
+    plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+              t["pose_init"], out)
+    graph = ctx.graph_end()
+    graph.launch()
+    ctx.sync()
+    graph_same = all(np.array_equal(res[k], v.cpu().numpy()) for k, v in out.items())
+    e0.record(stream)
+    for _ in range(args.pose_steps):
+        graph.launch()
+    e1.record(stream)
+    ctx.sync()
+    graph_ms = e0.elapsed_time(e1) / args.pose_steps
+    graph.close()
     # the same solves in the fused mode (one workgroup per problem runs all
     # iterations in one launch), for the comparison of the two modes
     plan.set_mode(2)
@@ -208,6 +224,9 @@ def bench_pose(ctx, args):
         "eval_hbm_gbs": eval_bytes / (eval_ms / 1e3) / 1e9,
         "eval_hbm_frac": eval_bytes / (eval_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "pose_err_vs_truth": {"max_m": float(dt.max()), "max_deg": float(np.degrees(dr.max()))},
+        "graph_ms_per_solve": graph_ms,
+        "graph_lm_iters_per_s": 64 * iters / (graph_ms / 1e3),
+        "graph_bitwise_equal": bool(graph_same),
         "fused_ms_per_solve": fused_ms,
         "fused_bitwise_equal": bool(fused_same),
         "keyframe_size_solve_ms": small_ms,

@@ -85,7 +85,11 @@ int sloam_ctx_reset_timing(sloam_ctx* ctx);
  * the same context that would need a larger scratch buffer than the one the
  * graph holds fails with SLOAM_ERR_INVALID_ARG (sloam_ctx_last_error names
  * the buffer) instead of moving memory the graph still writes.  Run the
- * largest inputs before capturing, or give them a context of their own.    */
+ * largest inputs before capturing, or give them a context of their own.
+ * Destroying the context first releases the graph's HIP objects: the graph
+ * is then orphaned (sloam_graph_launch returns SLOAM_ERR_INVALID_ARG) and
+ * sloam_graph_destroy only frees its handle.  Plans, front ends and
+ * estimators created from a context must be destroyed before it.          */
 typedef struct sloam_graph sloam_graph;
 int sloam_graph_begin(sloam_ctx* ctx);
 int sloam_graph_end(sloam_ctx* ctx, sloam_graph** out);
@@ -146,6 +150,23 @@ void sloam_cyl_plan_destroy(sloam_cyl_plan* plan);
  * does, reusing the plan's device buffers (they only grow).  Not allowed
  * during graph capture.  On error the plan holds an empty batch.          */
 int sloam_cyl_plan_reset(sloam_cyl_plan* plan, const int32_t* h_offsets, int32_t n_clusters);
+/* Counted plan (no counterpart in the reference): the batch's size is read
+ * from device memory at every exec (*d_n_clusters clusters, their offsets in
+ * the exec's d_offsets), so that an exec needs no host-side sizes and can be
+ * captured in a HIP graph whose replays fit other batches, e.g. a
+ * keyframe's trees straight from its front end.  The layout (work list,
+ * refit chunks) is made on the device by the exec's first kernel; clusters
+ * run in index order, and every result equals a host-planned plan's.  At
+ * most cap_clusters clusters and cap_points points: a larger batch sets the
+ * overflow word of sloam_cyl_plan_counts and fits nothing.  Exec without
+ * d_mask; not with sloam_cyl_plan_reset.                                    */
+int sloam_cyl_plan_create_counted(sloam_ctx* ctx, int32_t cap_clusters, int64_t cap_points,
+                                  const int32_t* d_n_clusters, const sloam_cyl_fit_cfg* cfg,
+                                  sloam_cyl_plan** out);
+/* Device pointer to 4 int32 of a counted plan, written by each exec:
+ * clusters, work items, refit chunks, overflow (1: the batch exceeded the
+ * capacities).                                                              */
+int sloam_cyl_plan_counts(const sloam_cyl_plan* plan, const int32_t** d_counts);
 /* Run a planned batch.  d_offsets must hold the same offsets as h_offsets.
  * Optional outputs may be NULL:
  *   d_counts  [n_clusters * n_hyp] int32 inlier count per hypothesis
@@ -516,11 +537,99 @@ int sloam_frontend_create(sloam_ctx* ctx, const sloam_frontend_cfg* cfg, sloam_f
 void sloam_frontend_destroy(sloam_frontend* f);
 int sloam_frontend_buffers(const sloam_frontend* f, sloam_frontend_out* out);
 int sloam_frontend_exec(sloam_frontend* f);
+/* The front end's sequence enqueued on its context stream as eager
+ * launches, never captured by the plan itself: for callers that capture a
+ * larger sequence (a whole keyframe) into a graph of their own.  Run it once
+ * before such a capture (it sizes the context's scratch).                   */
+int sloam_frontend_enqueue(sloam_frontend* f);
 /* Host view of the results of the last exec (valid once its work has
  * completed, e.g. after sloam_sync); SLOAM_ERR_INVALID_ARG unless the plan was
  * created with host_results.  Counts past the capacities (overflow flag
  * trellis_counts[4]) are clamped to the capacities in the copied arrays. */
 int sloam_frontend_host_view(const sloam_frontend* f, sloam_frontend_host* out);
+
+/* ---- keyframe estimator on the device ----------------------------------------
+ * Replaces: sloam::sloam::RunSloam (sloam/src/core/sloam.cpp:453-532) with
+ * computeModels (:388-436), matchFeatures / matchModels (:258-328),
+ * TwoStepOptimizePose / OptimizePose (:33-255) and projectModels (:438-451),
+ * for trees modelled by cylinder fits of their points (the KeyframeRunner's
+ * plain and learned tree models).  The front end's device outputs, the tree
+ * fits (sloam_cyl_plan model records, one per tree in the front end's tree
+ * order), the pose estimate and the map trees go in; the pose, the
+ * landmarks projected at it and their map matches come out, in pinned host
+ * memory, without a host round trip between the front end and the pose.  The
+ * estimator keeps the reference's per-process state (firstScan_,
+ * prevGPlanes_) on the device; sloam_kf_estimator_reset starts over.  The
+ * model arithmetic is the drop-in classes' (same functions), so a keyframe
+ * gives the host RunSloam's decisions and pose.
+ * Capacities: cap_trees landmarks, cap_cells ground cells, cap_map map trees;
+ * beyond them the result header's overflow word is 1 and the outputs are
+ * truncated.                                                                */
+typedef struct sloam_kf_params {
+  double max_lidar_dist;         /* maxLidarDist                              */
+  double default_tree_radius;    /* defaultTreeRadius                         */
+  double max_tree_radius;        /* maxTreeRadius                             */
+  double max_axis_theta;         /* maxAxisTheta (degrees)                    */
+  double tree_match_thresh;      /* treeMatchThresh                           */
+  double add_new_tree_thresh_dist; /* AddNewTreeThreshDist                    */
+  double min_tree_models;        /* minTreeModels                             */
+  double min_ground_models;      /* minGroundModels                           */
+  double min_planes;             /* radii bins x theta bins x 0.1 (sloam.h)   */
+  int32_t features_per_tree;     /* featuresPerTree                           */
+  int32_t num_ground_features;   /* numGroundFeatures                         */
+  int32_t two_step;              /* twoStepOptim                              */
+  int32_t reserved;
+} sloam_kf_params;
+
+typedef struct sloam_kf_inputs { /* device pointers */
+  const int32_t* trellis_counts; /* sloam_frontend_out.trellis_counts ...     */
+  const int32_t* tree_offsets;
+  const int32_t* vertex_int;
+  const double* vertex_f64;
+  const int32_t* point_offsets;
+  const float* points;
+  int32_t n_cells;               /* ground cells (sloam_frontend_out.n_cells) */
+  int32_t num_features;          /* must equal num_ground_features            */
+  const int32_t* cell_info;
+  const double* cell_model;
+  const float* cell_features;
+  const double* tree_fits;       /* 8 doubles per tree: root, ray, radius, inliers */
+  const double* pose;            /* 7: t, q (x y z w), a unit quaternion       */
+  const double* map;             /* cap_map x 8: root, ray, radius, 0          */
+  const int32_t* n_map;          /* map trees in use                          */
+} sloam_kf_inputs;
+
+typedef struct sloam_kf_result_view { /* pinned host memory */
+  /* 32 int32: mode (0 first scan, 1 estimated, 2 returned false), planes,
+   * landmarks, tree check, ground check, tree feature matches, ground feature
+   * matches, tree solve index, ground solve index, solves, overflow, trees,
+   * matched landmarks, matched planes, cylinder rows, plane rows, success,
+   * ran (1: outputs written) */
+  const int32_t* head;
+  const double* pose;      /* 7: T_Map_Curr (t, q x y z w)                   */
+  const double* delta;     /* 7: T_Delta                                     */
+  const int32_t* lm_tree;  /* per landmark: the front end's tree index       */
+  const int32_t* lm_id;    /* per landmark: tree id (vertex [2]'s treeId)    */
+  const int32_t* lm_match; /* per landmark: matchModels' map index or -1     */
+  const double* lm_model;  /* per landmark: 8 doubles, root, ray, radius, 0  */
+} sloam_kf_result_view;
+
+typedef struct sloam_kf_estimator sloam_kf_estimator;
+int sloam_kf_estimator_create(sloam_ctx* ctx, const sloam_kf_params* params, int32_t cap_trees,
+                              int32_t cap_cells, int32_t cap_map, sloam_kf_estimator** out);
+void sloam_kf_estimator_destroy(sloam_kf_estimator* e);
+int sloam_kf_estimator_reset(sloam_kf_estimator* e);
+int sloam_kf_estimator_exec(sloam_kf_estimator* e, const sloam_kf_inputs* in);
+/* Results of the last exec (valid once its work has completed).           */
+int sloam_kf_estimator_result(const sloam_kf_estimator* e, sloam_kf_result_view* out);
+/* The front end's trees as a cluster batch for the cylinder fits, on the
+ * device: d_offsets (cap_trees + 1) relative to the first tree's first
+ * point, d_xyz (cap_points x 3 floats) their points, *d_n the trees in use
+ * (at most cap_trees; the points past cap_points are not copied, which a
+ * counted plan of the same capacities reports as overflow).  No counterpart
+ * in the reference, whose trees are PCL clouds.                            */
+int sloam_kf_tree_points(sloam_ctx* ctx, const sloam_frontend_out* fe, int32_t cap_trees,
+                         int64_t cap_points, int32_t* d_offsets, float* d_xyz, int32_t* d_n);
 
 /* ---- learned cylinder proposal (PointNet, BF16 MFMA) -----------------------
  * No counterpart in the reference.  It proposes a cylinder per cluster, so that
@@ -548,6 +657,11 @@ int sloam_frontend_host_view(const sloam_frontend* f, sloam_frontend_host* out);
  * (unit), centre[3] (a point on the axis), radius, valid (1 / 0).          */
 int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
                    const int32_t* d_offsets, int32_t n_clusters, float* d_proposal);
+/* The same for *d_n_clusters clusters (device memory, at most cap_clusters):
+ * no host-side size, for graph capture (no counterpart in the reference).  */
+int sloam_pn_infer_counted(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                           const int32_t* d_offsets, int32_t cap_clusters,
+                           const int32_t* d_n_clusters, float* d_proposal);
 
 /* Training loss against cylinder models (SLOAM_CYL_MODEL_STRIDE doubles per
  * cluster, e.g. the plain fit's output; clusters whose radius is <= 0 carry

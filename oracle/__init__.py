@@ -58,7 +58,8 @@ def _declare(lib):
     lib.oracle_cyl_hypothesis.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                           C.c_uint64, C.c_int, P, P, P]
     lib.oracle_cyl_fit_batch.argtypes = [P, P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
-                                         C.c_int, C.c_uint64, C.c_int, P, P, P, P, P, P, C.c_int]
+                                         C.c_int, C.c_uint64, C.c_int, P, P, P, P, P, P, C.c_int,
+                                         C.c_int]
     lib.oracle_cyl_distances_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float,
                                               C.c_float, C.c_uint64, C.c_int, P]
     lib.oracle_pose_residuals.argtypes = [P, C.c_int, P, P, P, P, P]
@@ -136,7 +137,10 @@ def cyl_hypothesis(xyz, cluster, h, thr, max_radius, seed, tag=0):
 
 def cyl_fit_batch(xyz, offsets, n_hyp, thr=0.05, max_radius=1.0, refit_rounds=3, refit_iters=2,
                   seed=0x5EED,
-                  tag=0, counts=True, samples=False, mask=True, threads=1, proposal=None):
+                  tag=0, counts=True, samples=False, mask=True, threads=1, proposal=None,
+                  fp64_refit=False):
+    """The batched cylinder fit (oracle_cyl_fit_batch).  The refit runs in the
+    GPU's FP32 arithmetic and summation order, or in FP64 (fp64_refit)."""
     x = np.ascontiguousarray(xyz, np.float32)
     off = np.ascontiguousarray(offsets, np.int32)
     prop = None if proposal is None else np.ascontiguousarray(proposal, np.float32)
@@ -158,7 +162,7 @@ def cyl_fit_batch(xyz, offsets, n_hyp, thr=0.05, max_radius=1.0, refit_rounds=3,
                                      _p(prop),
                                      _p(out.get("counts")), _p(out.get("samples")),
                                      _p(out.get("mask")), _p(out["best"]), _p(out["model"]),
-                                     threads)
+                                     threads, int(bool(fp64_refit)))
     if st:
         raise ValueError("oracle_cyl_fit_batch failed")
     return out

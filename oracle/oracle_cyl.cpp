@@ -160,7 +160,8 @@ inline bool inlier(float u, float v, float s, const Hyp& h) {
   return t > h.L && t < h.U;
 }
 
-// LO-RANSAC refit (FP64) of the best hypothesis h.
+// LO-RANSAC refit of the best hypothesis h, all in FP64 (refit_fp64; the
+// error figure of the FP32 refit below is taken against it).
 // `rounds` rounds of `iters` Gauss-Newton steps of the 5-parameter geometric
 // cylinder fit
 //   d_i = ||(p_i - c) - ((p_i - c).a) a|| - r
@@ -185,8 +186,8 @@ inline double cyl_dist(const CylModel& M, const double p[3], double w[3], double
   return std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) - M.r;
 }
 
-void refit(const float* P, int n, const Frame& f, const Hyp& h, int rounds, int iters, float thr,
-           double out[8]) {
+void refit_fp64(const float* P, int n, const Frame& f, const Hyp& h, int rounds, int iters,
+                float thr, double out[8]) {
   auto invalid = [&](double cnt) {
     for (int i = 0; i < 6; ++i) out[i] = 0.0;
     out[5] = 1.0;
@@ -302,6 +303,249 @@ void refit(const float* P, int n, const Frame& f, const Hyp& h, int rounds, int 
   out[7] = (double)k;
 }
 
+// ---------------------------------------------------------------------------
+// LO-RANSAC refit with FP32 residuals and Jacobians (refit_fp32): the same
+// rounds, Gauss-Newton steps, re-selection and output as refit_fp64, with
+//   - the point coordinates x = p - m (m the seed frame's centre) and the
+//     model taken to FP32 for each step (axis a, axis point c relative to m,
+//     radius r, and the FP64 basis e1, e2 of the plane orthogonal to a);
+//   - per point, in FP32 with explicit FMAs:
+//       q = x - c, h = q.a, w = q - h a, rho = sqrt(w.w), d = rho - r,
+//       n1 = (w.e1) / rho, n2 = (w.e2) / rho, J = (-h n1, -h n2, -n1, -n2, -1);
+//   - the 20 sums of J J^T (upper triangle) and J d over the round's inliers
+//     accumulated in a fixed order: the cluster is cut into chunks of
+//     kRefitChunk points, thread t of a chunk's kRefitBlock threads adds the
+//     products of points t, t + kRefitBlock, ... with FMAs into an FP32
+//     value, the chunk's kRefitBlock values are added by TwoSum (an error-free
+//     sum and its rounding error) in groups of kRefitGroup threads in thread
+//     order and the groups in order, and the chunks' (sum, error) pairs are
+//     added in chunk order the same way; the total is sum + error in FP64;
+//   - the 5 x 5 normal equations solved in FP64 (Cholesky), the model updated
+//     in FP64.
+// The final inliers (|d| < thr) and the centroid of their x use the same
+// per-point FP32 distance and the same summation order.  This is the
+// arithmetic of the GPU's chunked refit (sloam_amd/csrc/cylfit.hip), which it
+// matches bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kRefitBlock = 256;
+constexpr int kRefitPerThread = 8;
+constexpr int kRefitChunk = kRefitBlock * kRefitPerThread;
+constexpr int kRefitGroup = 32;
+
+struct Comp {
+  float s = 0.0f, e = 0.0f;
+};
+
+// s + e = a + b exactly (Knuth's TwoSum)
+inline void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+inline void comp_add(Comp& acc, float x) {
+  float s, e;
+  two_sum(acc.s, x, s, e);
+  acc.s = s;
+  acc.e = acc.e + e;
+}
+
+inline void comp_add(Comp& acc, const Comp& x) {
+  float s, e;
+  two_sum(acc.s, x.s, s, e);
+  acc.s = s;
+  acc.e = acc.e + (e + x.e);
+}
+
+// Chunk partial of K per-thread values v[t][k] (t < kRefitBlock).
+template <int K>
+void chunk_reduce(const float (*v)[K], Comp* out) {
+  constexpr int G = kRefitBlock / kRefitGroup;
+  for (int k = 0; k < K; ++k) {
+    Comp tot;
+    for (int g = 0; g < G; ++g) {
+      Comp gs;
+      for (int t = g * kRefitGroup; t < (g + 1) * kRefitGroup; ++t) comp_add(gs, v[t][k]);
+      comp_add(tot, gs);
+    }
+    out[k] = tot;
+  }
+}
+
+struct ModelF {
+  float a[3], c[3], r, e1[3], e2[3];
+};
+
+inline ModelF model_f32(const CylModel& M) {
+  ModelF F;
+  double e1[3], e2[3];
+  basis(M.a, e1, e2);
+  for (int i = 0; i < 3; ++i) {
+    F.a[i] = (float)M.a[i];
+    F.c[i] = (float)M.c[i];
+    F.e1[i] = (float)e1[i];
+    F.e2[i] = (float)e2[i];
+  }
+  F.r = (float)M.r;
+  return F;
+}
+
+// FP32 distance of x (relative to the frame centre) to the model; w, h, rho out.
+inline float dist_f32(const ModelF& F, const float x[3], float w[3], float& h, float& rho) {
+  const float q0 = x[0] - F.c[0], q1 = x[1] - F.c[1], q2 = x[2] - F.c[2];
+  h = std::fmaf(q2, F.a[2], std::fmaf(q1, F.a[1], q0 * F.a[0]));
+  w[0] = std::fmaf(-h, F.a[0], q0);
+  w[1] = std::fmaf(-h, F.a[1], q1);
+  w[2] = std::fmaf(-h, F.a[2], q2);
+  rho = std::sqrt(std::fmaf(w[2], w[2], std::fmaf(w[1], w[1], w[0] * w[0])));
+  return rho - F.r;
+}
+
+void refit_fp32(const float* P, int n, const Frame& f, const Hyp& h, int k0, int rounds,
+                int iters, float thr, double out[8]) {
+  auto invalid = [&](double cnt) {
+    for (int i = 0; i < 6; ++i) out[i] = 0.0;
+    out[5] = 1.0;
+    out[6] = -1.0;
+    out[7] = cnt;
+  };
+  if (k0 < 3 || !h.valid) {
+    invalid((double)k0);
+    return;
+  }
+  std::vector<float> X(3 * (size_t)n);
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) X[3 * i + c] = P[3 * i + c] - f.m[c];
+  std::vector<char> in((size_t)n, 0);
+  for (int i = 0; i < n; ++i) {
+    float u, v, s;
+    to_local(f, P + 3 * i, u, v, s);
+    in[i] = inlier(u, v, s, h) ? 1 : 0;
+  }
+  CylModel M;
+  for (int c = 0; c < 3; ++c) {
+    M.a[c] = f.axis[c];
+    M.c[c] = (double)h.cu * (double)f.e1[c] + (double)h.cv * (double)f.e2[c];
+  }
+  M.r = (double)h.r;
+  const int n_chunks = std::max(1, (n + kRefitChunk - 1) / kRefitChunk);
+  static thread_local std::vector<float> vbuf;
+  vbuf.assign((size_t)kRefitBlock * 20, 0.0f);
+  float(*v)[20] = reinterpret_cast<float(*)[20]>(vbuf.data());
+  bool stop = false;
+  for (int round = 0; round < rounds && !stop; ++round) {
+    for (int it = 0; it < iters; ++it) {
+      const ModelF F = model_f32(M);
+      const bool reselect = (round > 0 && it == 0);
+      Comp tot[20];
+      long long cnt = 0;
+      for (int k = 0; k < n_chunks; ++k) {
+        for (int t = 0; t < kRefitBlock; ++t) {
+          float* g = v[t];
+          for (int j = 0; j < 20; ++j) g[j] = 0.0f;
+          for (int j = 0; j < kRefitPerThread; ++j) {
+            const int i = k * kRefitChunk + t + kRefitBlock * j;
+            if (i >= n) break;
+            float w[3], hh, rho;
+            const float d = dist_f32(F, &X[3 * (size_t)i], w, hh, rho);
+            if (reselect) in[i] = (d < thr && d > -thr) ? 1 : 0;
+            if (!in[i]) continue;
+            ++cnt;
+            if (!(rho > 0.0f)) continue;
+            const float inv = 1.0f / rho;
+            const float n1 = std::fmaf(w[2], F.e1[2], std::fmaf(w[1], F.e1[1], w[0] * F.e1[0])) * inv;
+            const float n2 = std::fmaf(w[2], F.e2[2], std::fmaf(w[1], F.e2[1], w[0] * F.e2[0])) * inv;
+            const float J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0f};
+            int q = 0;
+            for (int a = 0; a < 5; ++a)
+              for (int b = a; b < 5; ++b, ++q) g[q] = std::fmaf(J[a], J[b], g[q]);
+            for (int a = 0; a < 5; ++a) g[15 + a] = std::fmaf(J[a], d, g[15 + a]);
+          }
+        }
+        Comp part[20];
+        chunk_reduce<20>(v, part);
+        for (int q = 0; q < 20; ++q) comp_add(tot[q], part[q]);
+      }
+      if (cnt < 5) {
+        stop = true;
+        break;
+      }
+      double gs[20];
+      for (int q = 0; q < 20; ++q) gs[q] = (double)tot[q].s + (double)tot[q].e;
+      double A[5][5], rhs[5], dl[5];
+      int q = 0;
+      for (int a = 0; a < 5; ++a)
+        for (int b = a; b < 5; ++b, ++q) {
+          A[a][b] = gs[q];
+          A[b][a] = gs[q];
+        }
+      for (int a = 0; a < 5; ++a) rhs[a] = -gs[15 + a];
+      if (!cholesky_solve5(A, rhs, dl)) break;
+      double e1[3], e2[3];
+      basis(M.a, e1, e2);
+      double na[3];
+      for (int c = 0; c < 3; ++c) na[c] = M.a[c] + dl[0] * e1[c] + dl[1] * e2[c];
+      const double nl = std::sqrt(na[0] * na[0] + na[1] * na[1] + na[2] * na[2]);
+      for (int c = 0; c < 3; ++c) {
+        M.a[c] = na[c] / nl;
+        M.c[c] = M.c[c] + dl[2] * e1[c] + dl[3] * e2[c];
+      }
+      M.r = M.r + dl[4];
+      if (!(M.r > 0.0)) {
+        stop = true;
+        break;
+      }
+    }
+  }
+  // final inliers under the final model: count and centroid of x
+  long long k = 0;
+  Comp mu[3];
+  if (M.r > 0.0) {
+    const ModelF F = model_f32(M);
+    float (*v3)[3] = reinterpret_cast<float(*)[3]>(vbuf.data());
+    for (int ck = 0; ck < n_chunks; ++ck) {
+      for (int t = 0; t < kRefitBlock; ++t) {
+        float* g = v3[t];
+        g[0] = g[1] = g[2] = 0.0f;
+        for (int j = 0; j < kRefitPerThread; ++j) {
+          const int i = ck * kRefitChunk + t + kRefitBlock * j;
+          if (i >= n) break;
+          float w[3], hh, rho;
+          const float d = dist_f32(F, &X[3 * (size_t)i], w, hh, rho);
+          if (d < thr && d > -thr) {
+            for (int c = 0; c < 3; ++c) g[c] = g[c] + X[3 * (size_t)i + c];
+            ++k;
+          }
+        }
+      }
+      Comp part[3];
+      chunk_reduce<3>(v3, part);
+      for (int c = 0; c < 3; ++c) comp_add(mu[c], part[c]);
+    }
+  }
+  if (!(M.r > 0.0) || k < 3) {
+    invalid((double)k);
+    return;
+  }
+  const double kd = (double)k;
+  double mean[3];
+  for (int c = 0; c < 3; ++c) mean[c] = ((double)mu[c].s + (double)mu[c].e) / kd;
+  const double ht = ((mean[0] - M.c[0]) * M.a[0] + (mean[1] - M.c[1]) * M.a[1] +
+                     (mean[2] - M.c[2]) * M.a[2]);
+  double ray[3] = {M.a[0], M.a[1], M.a[2]};
+  int big = 0;
+  for (int c = 1; c < 3; ++c)
+    if (std::fabs(ray[c]) > std::fabs(ray[big])) big = c;
+  if (ray[big] < 0)
+    for (int c = 0; c < 3; ++c) ray[c] = -ray[c];
+  for (int c = 0; c < 3; ++c) {
+    out[c] = (double)f.m[c] + M.c[c] + ht * M.a[c];
+    out[3 + c] = ray[c];
+  }
+  out[6] = M.r;
+  out[7] = kd;
+}
+
 }  // namespace
 }  // namespace oracle
 
@@ -366,10 +610,13 @@ int oracle_cyl_hypothesis(const float* xyz, int n, int cluster, int h, float thr
 // clusters in parallel (OpenMP); results do not depend on it.
 // proposal (nullable): 8 floats per cluster (axis, centre, radius, valid);
 // a valid one is hypothesis 0.
+// fp64_refit: refit in FP64 (refit_fp64) instead of the GPU's FP32
+// arithmetic (refit_fp32).
 int oracle_cyl_fit_batch(const float* xyz, const int32_t* offsets, int n_clusters, int n_hyp,
                          float thr, float max_radius, int refit_rounds, int refit_iters, uint64_t seed,
                          int tag, const float* proposal, int32_t* counts, int32_t* samples,
-                         uint32_t* mask, int32_t* best_out, double* model, int n_threads) {
+                         uint32_t* mask, int32_t* best_out, double* model, int n_threads,
+                         int fp64_refit) {
   if (n_clusters < 0 || n_hyp < 1) return -1;
   const int64_t n_points = offsets[n_clusters];
   if (mask) std::memset(mask, 0, sizeof(uint32_t) * (size_t)((n_points + 31) / 32));
@@ -421,7 +668,10 @@ int oracle_cyl_fit_batch(const float* xyz, const int32_t* offsets, int n_cluster
         mask[g >> 5] |= (1u << (g & 31));
       }
     }
-    refit(P, n, f, hb, refit_rounds, refit_iters, thr, out);
+    if (fp64_refit)
+      refit_fp64(P, n, f, hb, refit_rounds, refit_iters, thr, out);
+    else
+      refit_fp32(P, n, f, hb, best_count, refit_rounds, refit_iters, thr, out);
   }
   return 0;
 }

@@ -150,6 +150,28 @@ class FrontendOut(C.Structure):
                                            "line_stats")]
 
 
+class KfParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "max_lidar_dist", "default_tree_radius", "max_tree_radius", "max_axis_theta",
+        "tree_match_thresh", "add_new_tree_thresh_dist", "min_tree_models", "min_ground_models",
+        "min_planes")]
+    _fields_ += [(n, C.c_int32) for n in ("features_per_tree", "num_ground_features", "two_step",
+                                          "reserved")]
+
+
+class KfInputs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("trellis_counts", "tree_offsets", "vertex_int",
+                                          "vertex_f64", "point_offsets", "points")]
+    _fields_ += [("n_cells", C.c_int32), ("num_features", C.c_int32)]
+    _fields_ += [(n, C.c_void_p) for n in ("cell_info", "cell_model", "cell_features",
+                                           "tree_fits", "pose", "map", "n_map")]
+
+
+class KfResultView(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("head", "pose", "delta", "lm_tree", "lm_id",
+                                          "lm_match", "lm_model")]
+
+
 _P = C.c_void_p
 _I32 = C.c_int32
 _I64 = C.c_int64
@@ -157,6 +179,18 @@ _I64 = C.c_int64
 # name -> (restype, argtypes); every exported symbol of include/sloam_gpu.h
 PROTOTYPES = {
     "sloam_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "sloam_cyl_plan_create_counted": (C.c_int, [_P, _I32, _I64, _P, C.POINTER(CylFitCfg),
+                                                C.POINTER(_P)]),
+    "sloam_cyl_plan_counts": (C.c_int, [_P, C.POINTER(_P)]),
+    "sloam_pn_infer_counted": (C.c_int, [_P, _P, _P, _P, _I32, _P, _P]),
+    "sloam_frontend_enqueue": (C.c_int, [_P]),
+    "sloam_kf_tree_points": (C.c_int, [_P, C.POINTER(FrontendOut), _I32, _I64, _P, _P, _P]),
+    "sloam_kf_estimator_create": (C.c_int, [_P, C.POINTER(KfParams), _I32, _I32, _I32,
+                                            C.POINTER(_P)]),
+    "sloam_kf_estimator_destroy": (None, [_P]),
+    "sloam_kf_estimator_reset": (C.c_int, [_P]),
+    "sloam_kf_estimator_exec": (C.c_int, [_P, C.POINTER(KfInputs)]),
+    "sloam_kf_estimator_result": (C.c_int, [_P, C.POINTER(KfResultView)]),
     "sloam_ctx_destroy": (None, [_P]),
     "sloam_sync": (C.c_int, [_P]),
     "sloam_strerror": (C.c_char_p, [C.c_int]),

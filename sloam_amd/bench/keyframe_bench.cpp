@@ -6,7 +6,8 @@
 //   keyframe_bench <sequence file> [passes] [use_graph 0/1]
 //                  [--tree-model lines|plain|learned] [--weights <npy>]
 //                  [--traj-out <file>] [--log] [--pinned] [--pipelined]
-//                  [--shadow lines|plain|learned]
+//                  [--shadow lines|plain|learned] [--device-estimator]
+//                  [--compare-device]
 //
 // --tree-model picks how each tree's model is made (KeyframeRunner's
 // TreeModel; default lines, the reference's); --traj-out writes the first
@@ -29,7 +30,12 @@
 // runner hands it to RunSloam), its state before the call and its output
 // (format: write_dump_* below), so that another implementation of RunSloam
 // can be run on the same inputs.  --max-tree-radius and --tree-match-thresh
-// replace those two estimator parameters (sloam.yaml: 0.3 and 0.5).  The per-keyframe
+// replace those two estimator parameters (sloam.yaml: 0.3 and 0.5).
+// --device-estimator runs the estimator on the GPU (sloam_kf_estimator)
+// instead of the host sloam::sloam; --compare-device runs, in the first pass,
+// a shadow with the same tree model and the device estimator from its own
+// state, and counts the keyframes where its pose, landmarks and step
+// decisions equal the host estimator's bit for bit.  The per-keyframe
 // latencies never include the shadow's work, but with a single pass that
 // pass's throughput does: give two passes or more to time with --shadow.
 //
@@ -78,6 +84,18 @@ bool parse_tree_model(const std::string& name, TreeModel* m) {
     return false;
   }
   return true;
+}
+
+bool same_se3(const SE3& a, const SE3& b) {
+  const Quat &qa = a.unit_quaternion(), &qb = b.unit_quaternion();
+  return std::memcmp(&qa, &qb, sizeof(Quat)) == 0 &&
+         std::memcmp(&a.translation(), &b.translation(), sizeof(Vector3)) == 0;
+}
+
+bool same_model(const CylinderParameters& a, const CylinderParameters& b) {
+  return std::memcmp(&a.root, &b.root, sizeof(Vector3)) == 0 &&
+         std::memcmp(&a.ray, &b.ray, sizeof(Vector3)) == 0 &&
+         std::memcmp(&a.radius, &b.radius, sizeof(double)) == 0;
 }
 
 // translation (m) and rotation (deg) between two poses
@@ -207,11 +225,15 @@ int main(int argc, char** argv) {
   std::vector<std::string> pos;
   std::string model_name = "lines", weights, traj_out, shadow_name, dump_path;
   double max_tree_radius = -1.0, tree_match_thresh = -1.0;
-  bool log = false, pinned = false, pipelined = false;
+  bool log = false, pinned = false, pipelined = false, device_est = false, compare_device = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if (a == "--log") {
       log = true;
+    } else if (a == "--device-estimator") {
+      device_est = true;
+    } else if (a == "--compare-device") {
+      compare_device = true;
     } else if (a == "--pinned") {
       pinned = true;
     } else if (a == "--pipelined") {
@@ -247,6 +269,15 @@ int main(int argc, char** argv) {
   if (!parse_tree_model(model_name, &tree_model)) {
     std::fprintf(stderr, "unknown tree model %s\n", model_name.c_str());
     return 2;
+  }
+  if (compare_device) {
+    // the shadow: the same tree model with the estimator on the device, from
+    // its own state (which evolves as the host estimator's does)
+    if (!shadow_name.empty() || device_est) {
+      std::fprintf(stderr, "--compare-device runs its own shadow on the host estimator's run\n");
+      return 2;
+    }
+    shadow_name = model_name;
   }
   const bool use_shadow = !shadow_name.empty();
   if (use_shadow && !parse_tree_model(shadow_name, &shadow_model)) {
@@ -310,6 +341,12 @@ int main(int argc, char** argv) {
   // --shadow: per keyframe, the second runner's pose against the first's
   std::vector<double> sh_m, sh_deg;
   int sh_ok_differs = 0;
+  // --compare-device: keyframes compared, where the two estimators' success
+  // differs, and where the pose (bits), the landmarks (models, ids, matches)
+  // and the step decisions agree; the first keyframe that differs
+  int dc_calls = 0, dc_ok_differs = 0, dc_pose_eq = 0, dc_models_eq = 0, dc_step_eq = 0,
+      dc_first_diff = -1;
+  double dc_max_m = 0, dc_max_deg = 0;
   // per compared keyframe: m, deg, k, then each runner's trees, map
   // matches, tree step run, tree feature matches; largest radius difference
   // of the landmarks both kept in the same order
@@ -338,10 +375,12 @@ int main(int argc, char** argv) {
     if (use_shadow) {
       shadow.reset(new KeyframeRunner(H, W, fm, use_graph, 22.5f, -22.5f, false));
       shadow->setTreeModel(shadow_model, weights);
+      if (compare_device) shadow->setDeviceEstimator(true);
     }
     for (int pass = 0; pass < passes; ++pass) {
       KeyframeRunner kf(H, W, fm, use_graph, 22.5f, -22.5f, pipelined);
       kf.setTreeModel(tree_model, weights);
+      if (device_est) kf.setDeviceEstimator(true);
       int dump_k = 0, dump_calls = 0;
       if (dump.f && pass == 0)
         kf.setInputRecorder([&](const SloamInput& in, const MapView& m) {
@@ -406,19 +445,44 @@ int main(int argc, char** argv) {
           ok = pipelined ? kf.finish(estimate, map, out, &t)
                          : kf.run(xyzi, label, estimate, map, out, &t);
           if (dump.f && pass == 0)
-            dump.output(ok, kf.estimator().lastStep(), out, kf.estimator().getPrevGroundModel());
+            dump.output(ok, kf.lastStep(), out, kf.estimator().getPrevGroundModel());
           if (pass == 0 && !nodeFirst) {
-            const SloamStepInfo st = kf.estimator().lastStep();
+            const SloamStepInfo st = kf.lastStep();
             ++step_calls;
             tree_steps += st.treeCheck ? 1 : 0;
             ground_steps += st.groundCheck ? 1 : 0;
           }
           if (shadow && pass == 0) {
-            shadow->estimator().setPrevGroundModel(std::move(prevPlanes));
+            if (!compare_device) shadow->estimator().setPrevGroundModel(std::move(prevPlanes));
             SloamOutput so;
             const bool sok =
                 shadow->run(seq[k].xyzi.data(), seq[k].label.data(), before, map, so, nullptr);
-            if (k > 0 && ok && sok) {
+            if (compare_device) {
+              // bit-level comparison of the two estimators' outputs
+              ++dc_calls;
+              if (ok != sok) ++dc_ok_differs;
+              if (ok && sok) {
+                const bool pose_eq = same_se3(out.T_Map_Curr, so.T_Map_Curr);
+                bool models_eq = out.tm.size() == so.tm.size() && out.matches == so.matches;
+                for (size_t i = 0; models_eq && i < out.tm.size(); ++i)
+                  models_eq = same_model(out.tm[i].getModel(), so.tm[i].getModel()) &&
+                              out.tm[i].id == so.tm[i].id;
+                const SloamStepInfo a = kf.lastStep(), b = shadow->lastStep();
+                const bool step_eq = a.treeCheck == b.treeCheck && a.groundCheck == b.groundCheck &&
+                                     a.treeMatches == b.treeMatches &&
+                                     a.planeMatches == b.planeMatches;
+                dc_pose_eq += pose_eq ? 1 : 0;
+                dc_models_eq += models_eq ? 1 : 0;
+                dc_step_eq += step_eq ? 1 : 0;
+                double m, deg;
+                pose_diff(out.T_Map_Curr, so.T_Map_Curr, &m, &deg);
+                dc_max_m = std::max(dc_max_m, m);
+                dc_max_deg = std::max(dc_max_deg, deg);
+                if ((!pose_eq || !models_eq || !step_eq) && dc_first_diff < 0) dc_first_diff = k;
+              }
+            }
+            if (compare_device) {
+            } else if (k > 0 && ok && sok) {
               double m, deg;
               pose_diff(out.T_Map_Curr, so.T_Map_Curr, &m, &deg);
               sh_m.push_back(m);
@@ -428,8 +492,8 @@ int main(int argc, char** argv) {
                 for (int x : o.matches) c += x >= 0 ? 1 : 0;
                 return (double)c;
               };
-              const SloamStepInfo a = kf.estimator().lastStep();
-              const SloamStepInfo b = shadow->estimator().lastStep();
+              const SloamStepInfo a = kf.lastStep();
+              const SloamStepInfo b = shadow->lastStep();
               for (const Cylinder& a_c : out.tm)
                 for (const Cylinder& b_c : so.tm)
                   if (a_c.id == b_c.id) {
@@ -551,9 +615,16 @@ int main(int argc, char** argv) {
   std::printf(
       "{\"tree_model\": \"%s\", \"pinned_input\": %s, \"tree_fit_p50_ms\": %.4f, "
       "\"trees_fitted\": %lld, \"trees_fallback\": %lld, \"discarded\": %d, "
-      "\"pipelined\": %s, \"throughput_kf_per_s\": %.1f, ",
+      "\"pipelined\": %s, \"throughput_kf_per_s\": %.1f, \"device_estimator\": %s, ",
       model_name.c_str(), pinned ? "true" : "false", pct(fit, 0.5), fitted, fallback, discarded,
-      pipelined ? "true" : "false", pct(pass_rate, 0.5));
+      pipelined ? "true" : "false", pct(pass_rate, 0.5), device_est ? "true" : "false");
+  if (compare_device)
+    std::printf(
+        "\"device_compare\": {\"calls\": %d, \"success_differs\": %d, \"pose_bit_equal\": %d, "
+        "\"landmarks_equal\": %d, \"steps_equal\": %d, \"first_difference\": %d, "
+        "\"max_m\": %.3e, \"max_deg\": %.3e}, ",
+        dc_calls, dc_ok_differs, dc_pose_eq, dc_models_eq, dc_step_eq, dc_first_diff, dc_max_m,
+        dc_max_deg);
   std::printf(
       "\"params\": {\"max_tree_radius\": %.4f, \"tree_match_thresh\": %.4f, "
       "\"max_axis_theta\": %.2f, \"two_step\": %s}, \"steps\": {\"calls\": %d, "
@@ -581,7 +652,7 @@ int main(int argc, char** argv) {
       samples ? trees / samples : 0.0, samples ? planes / samples : 0.0,
       samples ? (double)ok_count / samples : 0.0, final_drift, max_drift, max_rot,
       pct(first, 0.5), first.empty() ? 0.0 : *std::max_element(first.begin(), first.end()));
-  if (use_shadow) {
+  if (use_shadow && !compare_device) {
     // all compared keyframes, then those where both runners kept the same
     // number of trees and took the same XY-yaw decision
     std::vector<double> same_m, same_deg;

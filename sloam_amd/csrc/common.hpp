@@ -25,8 +25,14 @@ struct SloamTimer {
   int64_t launches = 0;
 };
 
+struct sloam_graph;
+
 struct sloam_ctx {
   int device = 0;
+  // graphs captured on this context and not yet destroyed: destroying the
+  // context releases their HIP graphs and leaves them orphaned (a later
+  // sloam_graph_destroy only frees the handle)
+  std::vector<sloam_graph*> graphs;
   hipStream_t stream = nullptr;
   std::string last_error;
   bool timing = false;
@@ -64,6 +70,22 @@ int sloam_ctx_join(sloam_ctx* ctx, int i);
 int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out);
 // Adds `delta` (+1 / -1) to the hold count of every slot in the bit set.
 void sloam_ctx_pin_slots(sloam_ctx* ctx, uint32_t slots, int delta);
+
+// Pose solves whose problem count is in device memory (*d_n_problems <=
+// max_problems): LM state set-up and the fused LM kernel of
+// sloam_pose_plan_exec over a work list built on the device (d_work: int4
+// {problem, first residual, end residual, 0} per chunk of at most 2048
+// residuals, d_first: first chunk of each problem).  d_state holds
+// max_problems LM states (sloam_pose_state_bytes each); d_cylf / d_plf the
+// FP32 landmark tables (8 / 4 floats per cylinder / plane).
+size_t sloam_pose_state_bytes();
+int sloam_pose_fused_device(sloam_ctx* ctx, int max_problems, const int32_t* d_n_problems,
+                            const float* d_points, const int32_t* d_work, const int32_t* d_first,
+                            const double* d_cyl, const int32_t* d_cyl_off, const double* d_planes,
+                            const int32_t* d_pl_off, const double* d_pose_in,
+                            const int32_t* d_free_mask, const sloam_pose_cfg* cfg, void* d_state,
+                            float* d_cylf, float* d_plf, double* d_pose_out,
+                            double* d_params_out, int32_t* d_status, double* d_cost);
 
 // Records a HIP failure in the context and returns SLOAM_ERR_HIP.
 int sloam_record_hip_error(sloam_ctx* ctx, hipError_t e, const char* what);

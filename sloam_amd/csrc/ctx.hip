@@ -175,10 +175,15 @@ int sloam_ctx_create(int device, sloam_ctx** out) {
   return SLOAM_OK;
 }
 
+static void graph_release(sloam_graph* g);
+
 void sloam_ctx_destroy(sloam_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  // graphs still alive lose their HIP objects and their context
+  for (sloam_graph* g : ctx->graphs) graph_release(g);
+  ctx->graphs.clear();
   for (int i = 0; i < 2; ++i) {
     if (ctx->aux[i]) {
       (void)hipStreamSynchronize(ctx->aux[i]);
@@ -224,11 +229,21 @@ const char* sloam_ctx_last_error(const sloam_ctx* ctx) {
 void* sloam_ctx_stream(sloam_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 struct sloam_graph {
-  sloam_ctx* ctx = nullptr;
+  sloam_ctx* ctx = nullptr;  // null once the context is destroyed (orphaned)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t slots = 0;  // scratch slots the graph's kernels use (held until destroyed)
 };
+
+// Releases g's HIP objects and detaches it from its context (whose stream
+// has completed the graph's work).
+static void graph_release(sloam_graph* g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->exec = nullptr;
+  g->graph = nullptr;
+  g->ctx = nullptr;
+}
 
 int sloam_graph_begin(sloam_ctx* ctx) {
   if (!ctx || ctx->capturing) return SLOAM_ERR_INVALID_ARG;
@@ -273,12 +288,13 @@ int sloam_graph_end(sloam_ctx* ctx, sloam_graph** out) {
   }
   r->slots = ctx->capture_slots;
   sloam_ctx_pin_slots(ctx, r->slots, +1);
+  ctx->graphs.push_back(r);
   *out = r;
   return SLOAM_OK;
 }
 
 int sloam_graph_launch(sloam_graph* g) {
-  if (!g) return SLOAM_ERR_INVALID_ARG;
+  if (!g || !g->ctx) return SLOAM_ERR_INVALID_ARG;
   sloam_ctx* ctx = g->ctx;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
   SloamKernelScope ks(ctx, "graph");
@@ -288,10 +304,18 @@ int sloam_graph_launch(sloam_graph* g) {
 
 void sloam_graph_destroy(sloam_graph* g) {
   if (!g) return;
-  if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
-  if (g->exec) (void)hipGraphExecDestroy(g->exec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
-  sloam_ctx_pin_slots(g->ctx, g->slots, -1);
+  sloam_ctx* ctx = g->ctx;
+  if (ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    sloam_ctx_pin_slots(ctx, g->slots, -1);
+    auto& v = ctx->graphs;
+    for (size_t i = 0; i < v.size(); ++i)
+      if (v[i] == g) {
+        v.erase(v.begin() + (long)i);
+        break;
+      }
+    graph_release(g);
+  }
   delete g;
 }
 

@@ -39,24 +39,6 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kHB = 8;  // hypotheses held in registers per pass over a tile
 constexpr int kMaxLds = 65536;  // LDS bytes per workgroup this kernel asks for
-// Refit tiers: a cluster of n points is refitted by one workgroup of
-// kTierBlock[t] threads, t the first tier with n <= the tier's bound.  The
-// tiers run concurrently (the context stream and its auxiliary streams).
-// When the batch fills every SIMD, a cluster's time is set by its points per
-// SIMD whatever the workgroup size: measured on the 2000-cluster batch, 64 /
-// 256 threads split at 768 points beat three tiers with 512-thread
-// workgroups and beat staging the points in LDS (which lowers occupancy).  A
-// batch with fewer clusters than the device has SIMDs leaves SIMDs idle; a
-// 256-thread workgroup then spreads its cluster over the four SIMDs of its CU
-// (a quarter of the serial FP64 work per SIMD), so only clusters of up to
-// kTierMaxSmall points keep one wave.
-constexpr int kTiers = 2;
-constexpr int kTierMax[kTiers] = {768, 1 << 30};
-constexpr int kTierMaxSmall[kTiers] = {256, 1 << 30};
-constexpr int kTierBlock[kTiers] = {64, 256};
-static_assert(kTierBlock[0] == 64 && kTierBlock[kTiers - 1] == 256,
-              "launch_refit instantiates the refit kernel for 64 and 256 threads");
-static_assert(kTiers >= 1 && kTiers <= 3, "one stream per tier: the context stream and two auxiliary ones");
 
 struct CylFrame {
   float m[3];
@@ -177,11 +159,12 @@ __global__ __launch_bounds__(kBlock) void cyl_seed_kernel(const float* __restric
                                                           const int32_t* __restrict__ offsets,
                                                           int n_clusters,
                                                           const int32_t* __restrict__ active,
+                                                          const int32_t* __restrict__ bcnt,
                                                           CylFrame* __restrict__ frames) {
   __shared__ float red_f[kWaves][6];
   __shared__ long long red_i[kWaves][9];
   const int c = blockIdx.x;
-  if (c >= n_clusters) return;
+  if (c >= n_clusters || (bcnt && c >= bcnt[0])) return;
   if (active && !active[c]) return;
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
@@ -299,8 +282,10 @@ __global__ __launch_bounds__(kBlock) void cyl_score_kernel(
     const CylFrame* __restrict__ frames, const int4* __restrict__ work, int n_hyp, int hcap,
     int tile, float thr, float max_radius, uint32_t k0, uint32_t k1, uint32_t tag,
     const float* __restrict__ prop, const int32_t* __restrict__ active,
-    int32_t* __restrict__ counts, int32_t* __restrict__ samples) {
+    const int32_t* __restrict__ bcnt, int32_t* __restrict__ counts,
+    int32_t* __restrict__ samples) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (bcnt && (int)blockIdx.x >= bcnt[1]) return;  // past the batch's work items
   float4* hp = reinterpret_cast<float4*>(smem_raw);
   int* cnt = reinterpret_cast<int*>(hp + hcap);
   float* su = reinterpret_cast<float*>(cnt + hcap);
@@ -397,42 +382,154 @@ __global__ __launch_bounds__(kBlock) void cyl_score_kernel(
   for (int t = threadIdx.x; t < hc; t += kBlock) counts[(size_t)c * n_hyp + h0 + t] = cnt[t];
 }
 
-struct CylModel {
-  double a[3], c[3], r;
+// ---------------------------------------------------------------------------
+// refit: best hypothesis, mask, LO-RANSAC cylinder refit with FP32 residuals
+// and Jacobians, compensated FP32 sums and an FP64 5x5 solve.
+//
+// Rounds of Gauss-Newton steps on the 5-parameter geometric cylinder residual
+//   d_i = ||(p_i - c) - ((p_i - c).a) a|| - r
+// over the round's inliers: the FP32 inliers of the best hypothesis in round
+// 0, |d_i| < thr under the model at the start of the round afterwards.
+// Output record: root[3], ray[3], radius, inliers.
+//
+// Every cluster is cut into chunks of kRefitChunk points; one workgroup of
+// kRefitBlock threads takes one chunk, so a 5000-point cluster spreads over
+// three workgroups instead of serialising on one.  One launch per pass:
+//   cyl_refit_init   best hypothesis (max count, lowest index), inlier mask,
+//                    round-0 membership bits, the initial model;
+//   cyl_refit_step   one Gauss-Newton step: per point, in FP32, w, h,
+//                    rho = ||w||, d = rho - r, n = w / rho and
+//                    J = (-h n.e1, -h n.e2, -n.e1, -n.e2, -1); each thread
+//                    adds its points' J J^T and J d terms with FMAs, the
+//                    workgroup adds the threads' values with TwoSum in groups
+//                    of kRefitGroup threads (thread order, then group order)
+//                    into one (sum, error) pair per term and chunk;
+//   cyl_refit_solve  one thread per cluster: the chunks' pairs added in chunk
+//                    order, the normal equations solved in FP64, the next
+//                    model;
+//   cyl_refit_final  final inliers and their centroid's chunk sums the same
+//   cyl_refit_finish way, and the output record.
+// The per-cluster kernels read the chunk kernels' results across a kernel
+// boundary (no fences between workgroups inside a launch).  The order of
+// every addition is fixed (no atomics on values), so results do not depend
+// on scheduling; the oracle restates the same arithmetic.
+// ---------------------------------------------------------------------------
+constexpr int kRefitBlock = 256;
+constexpr int kRefitPerThread = 8;
+constexpr int kRefitChunk = kRefitBlock * kRefitPerThread;
+constexpr int kRefitGroup = 32;
+constexpr int kRefitGroups = kRefitBlock / kRefitGroup;
+constexpr int kRefitTerms = 20;  // 15 of J J^T (upper triangle), 5 of J d
+
+enum RefitStatus : int32_t {
+  kRefitActive = 0,    // Gauss-Newton steps continue
+  kRefitStopped = 1,   // no more steps (too few inliers, radius <= 0)
+  kRefitInvalid = 2,   // record already written as invalid
+  kRefitInactive = 3,  // cluster not part of this run
 };
 
-// As cyl_dist, and 1 / ||w|| from one reciprocal square root refined by two
-// Newton steps (no FP64 divide in the Gauss-Newton loop). A point on the axis
-// (||w|| = 0) keeps cyl_dist's answer: distance -r and a zero reciprocal.
-__device__ inline double cyl_dist_inv(const CylModel& M, const double p[3], double w[3],
-                                      double* h, double* inv) {
-  const double q[3] = {p[0] - M.c[0], p[1] - M.c[1], p[2] - M.c[2]};
-  const double hh = q[0] * M.a[0] + q[1] * M.a[1] + q[2] * M.a[2];
-  w[0] = q[0] - hh * M.a[0];
-  w[1] = q[1] - hh * M.a[1];
-  w[2] = q[2] - hh * M.a[2];
-  *h = hh;
-  const double s2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
-  if (!(s2 > 0.0)) {
-    *inv = 0.0;
-    return -M.r;
-  }
-  double y = __builtin_amdgcn_rsq(s2);
-  const double hs = 0.5 * s2;
-  y = y * __builtin_fma(-hs * y, y, 1.5);
-  y = y * __builtin_fma(-hs * y, y, 1.5);
-  *inv = y;
-  return s2 * y - M.r;
+// Per-cluster refit state.  The FP32 copy of the model and the basis of the
+// plane orthogonal to its axis are made whenever the FP64 model changes.
+struct RefitState {
+  double a[3], c[3], r;  // axis, axis point (relative to the frame centre), radius
+  float af[3], cf[3], rf, e1f[3], e2f[3];
+  int32_t status;
+  int32_t skip_round;  // round whose remaining steps are skipped (failed solve), -1
+  int32_t rounds, iters;
+};
+
+struct Comp2 {
+  float s, e;  // value s + e
+};
+
+// s + e = a + b exactly (TwoSum)
+__device__ inline void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
 }
 
-__device__ inline double cyl_dist(const CylModel& M, const double p[3], double w[3], double* h) {
-  const double q[3] = {p[0] - M.c[0], p[1] - M.c[1], p[2] - M.c[2]};
-  const double hh = q[0] * M.a[0] + q[1] * M.a[1] + q[2] * M.a[2];
-  w[0] = q[0] - hh * M.a[0];
-  w[1] = q[1] - hh * M.a[1];
-  w[2] = q[2] - hh * M.a[2];
-  *h = hh;
-  return sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) - M.r;
+__device__ inline void comp_add(Comp2& acc, float x) {
+  float s, e;
+  two_sum(acc.s, x, s, e);
+  acc.s = s;
+  acc.e = acc.e + e;
+}
+
+__device__ inline void comp_add(Comp2& acc, const Comp2& x) {
+  float s, e;
+  two_sum(acc.s, x.s, s, e);
+  acc.s = s;
+  acc.e = acc.e + (e + x.e);
+}
+
+template <int K>
+struct RefitRed {
+  float v[K][kRefitGroups][kRefitGroup + 1];  // +1: group chains read distinct banks
+  Comp2 grp[K][kRefitGroups];
+};
+
+// The chunk's (sum, error) pair of each of K per-thread values, into out[K]
+// (global): TwoSum chains over kRefitGroup threads, then over the groups.
+template <int K>
+__device__ inline void chunk_partial(const float (&g)[K], RefitRed<K>& r, Comp2* out) {
+  const int t = threadIdx.x;
+  const int tg = t / kRefitGroup, tl = t % kRefitGroup;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.v[k][tg][tl] = g[k];
+  __syncthreads();
+  if (t < K * kRefitGroups) {
+    const int k = t / kRefitGroups, q = t % kRefitGroups;
+    Comp2 s = {0.0f, 0.0f};
+    for (int j = 0; j < kRefitGroup; ++j) comp_add(s, r.v[k][q][j]);
+    r.grp[k][q] = s;
+  }
+  __syncthreads();
+  if (t < K) {
+    Comp2 s = {0.0f, 0.0f};
+#pragma unroll
+    for (int q = 0; q < kRefitGroups; ++q) comp_add(s, r.grp[t][q]);
+    out[t] = s;
+  }
+}
+
+// Sum of one int per thread over the workgroup (every thread gets it).
+__device__ inline int block_sum_int(int v, int* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int s = 0;
+#pragma unroll
+  for (int w = 0; w < kRefitBlock / 64; ++w) s += lds[w];
+  return s;
+}
+
+// FP32 copy of the FP64 model and its basis.
+__device__ inline void refit_model_f32(RefitState& S) {
+  double e1[3], e2[3];
+  sl_basis(S.a, e1, e2);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    S.af[i] = (float)S.a[i];
+    S.cf[i] = (float)S.c[i];
+    S.e1f[i] = (float)e1[i];
+    S.e2f[i] = (float)e2[i];
+  }
+  S.rf = (float)S.r;
+}
+
+// FP32 distance of x (relative to the frame centre) to the model; w, h, rho out.
+__device__ inline float refit_dist(const RefitState& S, const float x[3], float w[3], float& h,
+                                   float& rho) {
+  const float q0 = x[0] - S.cf[0], q1 = x[1] - S.cf[1], q2 = x[2] - S.cf[2];
+  h = __builtin_fmaf(q2, S.af[2], __builtin_fmaf(q1, S.af[1], q0 * S.af[0]));
+  w[0] = __builtin_fmaf(-h, S.af[0], q0);
+  w[1] = __builtin_fmaf(-h, S.af[1], q1);
+  w[2] = __builtin_fmaf(-h, S.af[2], q2);
+  rho = sqrtf(__builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0])));
+  return rho - S.rf;
 }
 
 __device__ inline void write_invalid(double* out, double k) {
@@ -483,127 +580,51 @@ __device__ inline bool cholesky_solve(const double (&A)[N][N], const double (&b)
   return ok;
 }
 
-// Sums K doubles over a block of BLOCK threads; every thread gets the totals.
-// lds holds (BLOCK / 64 + 1) * K doubles: the wave sums, then the totals,
-// added in wave order by K threads.
-template <int BLOCK, int K>
-__device__ inline void bsum(double (&v)[K], double* lds) {
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = sl_wave_sum(v[k]);
-  if (BLOCK == 64) return;
-  constexpr int W = BLOCK / 64;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
-  }
-  __syncthreads();
-  if (threadIdx.x < K) {
-    const int k = threadIdx.x;
-    double t = lds[k];
-#pragma unroll
-    for (int w = 1; w < W; ++w) t += lds[w * K + k];
-    lds[W * K + k] = t;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = lds[W * K + k];
-}
+// Refit chunk ck = {cluster, first point (cluster-relative), points, chunk
+// index within the cluster}; cluster record cl = {first chunk id, chunks,
+// first membership word, 0}.  Membership words: two per 64 points of the
+// cluster, written by the wave that owns those 64 points.
 
-// Sum over the workgroup of K doubles per thread, every thread getting the
-// totals, through LDS: each thread's values are stored, P threads per value
-// add BLOCK / P threads' values each in a fixed order, and K threads add the
-// P partials (the wave shuffles of bsum move each double through two LDS
-// permutes per step, six steps per value).
-template <int BLOCK, int K>
-struct BsumLds {
-  static constexpr int P = (BLOCK / K) < 16 ? (BLOCK / K) : 16;  // threads per value
-  static constexpr int L = (BLOCK + P - 1) / P;                   // values per thread
-  double x[K][BLOCK + 1];
-  double part[K][P];
-  double tot[K];
-};
-
-template <int BLOCK, int K>
-__device__ inline void bsum_lds(double (&v)[K], BsumLds<BLOCK, K>& r) {
-  constexpr int P = BsumLds<BLOCK, K>::P;
-  constexpr int L = BsumLds<BLOCK, K>::L;
-  static_assert(P >= 1 && K * P <= BLOCK, "bsum_lds layout");
-  const int t = threadIdx.x;
-  __syncthreads();  // r may still be read from the previous call
-#pragma unroll
-  for (int k = 0; k < K; ++k) r.x[k][t] = v[k];
-  __syncthreads();
-  if (t < K * P) {
-    const int k = t % K, p = t / K;
-    const int b = p * L, e = min(BLOCK, b + L);
-    double s = 0.0;
-    for (int j = b; j < e; ++j) s += r.x[k][j];
-    r.part[k][p] = s;
-  }
-  __syncthreads();
-  if (t < K) {
-    double s = r.part[t][0];
-#pragma unroll
-    for (int p = 1; p < P; ++p) s += r.part[t][p];
-    r.tot[t] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = r.tot[k];
-}
-
-// ---------------------------------------------------------------------------
-// refit: best hypothesis, mask, FP64 LO-RANSAC cylinder refit.
-// One workgroup per cluster, its size by the cluster's size (refit tiers).
-// Rounds of Gauss-Newton steps on the 5-parameter geometric cylinder residual
-//   d_i = ||(p_i - c) - ((p_i - c).a) a|| - r
-// over the round's inliers: the FP32 inliers of the best hypothesis in round
-// 0, |d_i| < thr under the model at the start of the round afterwards.  The
-// membership of a round is computed in its first pass and kept as a bit mask
-// in LDS.  Output record: root[3], ray[3], radius, inliers.
-// ---------------------------------------------------------------------------
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
+// Round 0: best hypothesis, global mask bits, membership bits, initial model.
+// A flagged cluster (alt_flags) takes its hypotheses from the alternate set
+// (its own counts, hypothesis count and sample stream, no proposal) and
+// refits with the alternate round and step counts.
+__global__ __launch_bounds__(kRefitBlock) void cyl_refit_init_kernel(
     const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
     const CylFrame* __restrict__ frames, const int32_t* __restrict__ counts,
-    const int32_t* __restrict__ list, int n_hyp, float thr, float max_radius, uint32_t k0,
-    uint32_t k1, uint32_t tag, int refit_rounds, int refit_iters,
+    const int4* __restrict__ rchunks, const int4* __restrict__ rclus, int n_hyp, float thr,
+    float max_radius, uint32_t k0, uint32_t k1, uint32_t tag, int refit_rounds, int refit_iters,
     const float* __restrict__ prop, const int32_t* __restrict__ active,
     const int32_t* __restrict__ alt_flags, const int32_t* __restrict__ alt_counts, int alt_n_hyp,
-    uint32_t alt_tag, int alt_rounds, int alt_iters, uint32_t* __restrict__ mask,
+    uint32_t alt_tag, int alt_rounds, int alt_iters, const int32_t* __restrict__ bcnt,
+    RefitState* __restrict__ state, uint32_t* __restrict__ member, uint32_t* __restrict__ mask,
     int32_t* __restrict__ best_out, double* __restrict__ model) {
-  constexpr int W = BLOCK / 64;
-  __shared__ double red[(W + 1) * 21];  // bsum of up to 21 values
-  __shared__ BsumLds<BLOCK, 21> red21;   // the Gauss-Newton sums
+  constexpr int W = kRefitBlock / 64;
   __shared__ long long redk[W];
   __shared__ HypOut best_h;
-  extern __shared__ uint32_t bits[];  // membership of the current round
-  const int c = list[blockIdx.x];
-  if (active && !active[c]) return;
+  if (bcnt && (int)blockIdx.x >= bcnt[2]) return;  // past the batch's chunks
+  const int4 ck = rchunks[blockIdx.x];
+  const int c = ck.x;
+  if (c < 0) return;  // padding of the per-XCD interleave
+  if (active && !active[c]) {
+    if (ck.w == 0 && threadIdx.x == 0) state[c].status = kRefitInactive;
+    return;
+  }
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
   const float* P = xyz + 3 * (size_t)off;
   const CylFrame f = frames[c];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  double* out = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
-  // a flagged cluster takes its hypotheses from the alternate set (its own
-  // counts, hypothesis count and sample stream, no proposal) and refits with
-  // the alternate round and step counts
   const bool alt = alt_flags && alt_flags[c];
   const int32_t* cnts = alt ? alt_counts : counts;
   const int nh = alt ? alt_n_hyp : n_hyp;
   const uint32_t tg = alt ? alt_tag : tag;
-  const int rounds = alt ? alt_rounds : refit_rounds;
-  const int iters = alt ? alt_iters : refit_iters;
   const float* pc = alt ? nullptr : cluster_prop(prop, c);
 
-  // 1. best hypothesis: key = count * 2^32 + (2^32 - 1 - index), max wins
+  // best hypothesis: key = count * 2^32 + (2^32 - 1 - index), max wins
   long long key = -1;
-  for (int h = threadIdx.x; h < nh; h += BLOCK) {
+  for (int h = threadIdx.x; h < nh; h += kRefitBlock) {
     const long long kk = ((long long)cnts[(size_t)c * nh + h] << 32) |
                          (long long)(0xFFFFFFFFu - (uint32_t)h);
     key = kk > key ? kk : key;
@@ -613,13 +634,11 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
     const long long other = __shfl_xor(key, o, 64);
     key = other > key ? other : key;
   }
-  if (BLOCK > 64) {
-    if (lane == 0) redk[wave] = key;
-    __syncthreads();
-    key = redk[0];
+  if (lane == 0) redk[wave] = key;
+  __syncthreads();
+  key = redk[0];
 #pragma unroll
-    for (int w = 1; w < W; ++w) key = redk[w] > key ? redk[w] : key;
-  }
+  for (int w = 1; w < W; ++w) key = redk[w] > key ? redk[w] : key;
   const int best = (n >= 3 && nh > 0) ? (int)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFll)) : -1;
   if (threadIdx.x == 0) {
     if (best >= 0) {
@@ -632,40 +651,39 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
       best_h.U = -INFINITY;
       best_h.cu = best_h.cv = best_h.r = 0.f;
     }
-    if (best_out) best_out[c] = best;
   }
   __syncthreads();
   const HypOut h = best_h;
 
-  // 2. round-0 membership = FP32 inliers of the best hypothesis: global mask
-  //    output and LDS bits (64 consecutive points per wave, 64-aligned)
-  double k0cnt[1] = {0.0};
-  for (int base = 0; base < n; base += BLOCK) {
-    const int i = base + (int)threadIdx.x;
+  // membership bits of the chunk (cluster-relative, 64-aligned per wave)
+  const int4 cl = rclus[c];
+  uint32_t* mem = member + cl.z;
+  const int end = ck.y + ck.z;
+#pragma unroll
+  for (int j = 0; j < kRefitPerThread; ++j) {
+    const int gbase = ck.y + kRefitBlock * j + 64 * wave;  // first point of this wave's group
+    const int i = gbase + lane;
     bool in = false;
-    if (i < n) {
+    if (i < end) {
       const float* q = P + 3 * (size_t)i;
       float u, v, s;
       local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
       in = is_inlier(u, v, s, h);
     }
     const uint64_t b = __builtin_amdgcn_ballot_w64(in);
-    if (lane == 0) {
-      const int wbase = (base + wave * 64) >> 5;
-      bits[wbase] = (uint32_t)b;
-      bits[wbase + 1] = (uint32_t)(b >> 32);
+    if (lane == 0 && gbase < end) {
+      mem[gbase >> 5] = (uint32_t)b;
+      mem[(gbase >> 5) + 1] = (uint32_t)(b >> 32);
     }
-    if (in) k0cnt[0] += 1.0;
   }
   if (mask) {
-    // global mask: iterate 64-aligned global indices so that each wave owns
-    // two whole 32-bit words (or-ed atomically: neighbouring clusters share
-    // boundary words)
-    const int gend = off + n;
-    for (int gb = (off & ~63); gb < gend; gb += BLOCK) {
+    // global mask: 64-aligned global indices, so that each wave owns two whole
+    // 32-bit words (or-ed atomically: neighbouring chunks share boundary words)
+    const int gs = off + ck.y, ge = off + end;
+    for (int gb = (gs & ~63); gb < ge; gb += kRefitBlock) {
       const int g = gb + (int)threadIdx.x;
       bool in = false;
-      if (g >= off && g < gend) {
+      if (g >= gs && g < ge) {
         const float* q = xyz + 3 * (size_t)g;
         float u, v, s;
         local_uvs(f, q[0], q[1], q[2], &u, &v, &s);
@@ -679,143 +697,255 @@ __global__ __launch_bounds__(BLOCK) void cyl_refit_kernel(
       }
     }
   }
-  bsum<BLOCK, 1>(k0cnt, red);
-  if (k0cnt[0] < 3.0 || !h.valid) {
-    if (threadIdx.x == 0) write_invalid(out, k0cnt[0]);
-    return;
+  if (ck.w != 0 || threadIdx.x != 0) return;
+  // the cluster's first chunk: initial model (the inlier count of the best
+  // hypothesis is its score)
+  if (best_out) best_out[c] = best;
+  RefitState S;
+  S.status = kRefitActive;
+  S.skip_round = -1;
+  S.rounds = alt ? alt_rounds : refit_rounds;
+  S.iters = alt ? alt_iters : refit_iters;
+  const int kin = best >= 0 ? cnts[(size_t)c * nh + best] : 0;
+  if (kin < 3 || !h.valid) {
+    S.status = kRefitInvalid;
+    write_invalid(model + (size_t)c * SLOAM_CYL_MODEL_STRIDE, (double)kin);
   }
-
-  // 3. LO refit
-  CylModel M;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    M.a[i] = f.axis[i];
-    M.c[i] = (double)h.cu * (double)f.e1[i] + (double)h.cv * (double)f.e2[i];
+    S.a[i] = f.axis[i];
+    S.c[i] = (double)h.cu * (double)f.e1[i] + (double)h.cv * (double)f.e2[i];
   }
-  M.r = (double)h.r;
-  const double dthr = (double)thr;
-  bool stop = false;
-  for (int round = 0; round < rounds && !stop; ++round) {
-    for (int it = 0; it < iters; ++it) {
-      double e1[3], e2[3];
-      sl_basis(M.a, e1, e2);
-      double g[21];
+  S.r = (double)h.r;
+  refit_model_f32(S);
+  state[c] = S;
+}
+
+// One Gauss-Newton step (launch `step` of the refit: round step / iters,
+// iteration step % iters of each cluster's own schedule).
+__global__ __launch_bounds__(kRefitBlock) void cyl_refit_step_kernel(
+    const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
+    const CylFrame* __restrict__ frames, const int4* __restrict__ rchunks,
+    const int4* __restrict__ rclus, RefitState* __restrict__ state, uint32_t* __restrict__ member,
+    Comp2* __restrict__ partial, int32_t* __restrict__ pcount, int step, float thr,
+    const int32_t* __restrict__ bcnt) {
+  __shared__ RefitRed<kRefitTerms> red;
+  __shared__ int cnt_w[kRefitBlock / 64];
+  if (bcnt && (int)blockIdx.x >= bcnt[2]) return;
+  const int4 ck = rchunks[blockIdx.x];
+  const int c = ck.x;
+  if (c < 0) return;
+  const RefitState S = state[c];
+  if (S.status != kRefitActive) return;
+  if (step >= S.rounds * S.iters) return;
+  const int round = step / S.iters, it = step % S.iters;
+  if (S.skip_round == round) return;
+  const bool reselect = round > 0 && it == 0;
+  const int off = offsets[c];
+  const float* P = xyz + 3 * (size_t)off;
+  const CylFrame f = frames[c];
+  const int4 cl = rclus[c];
+  uint32_t* mem = member + cl.z;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int end = ck.y + ck.z;
+  float g[kRefitTerms];
 #pragma unroll
-      for (int t = 0; t < 21; ++t) g[t] = 0.0;
-      const bool reselect = (round > 0 && it == 0);
-      for (int base = 0; base < n; base += BLOCK) {
-        const int i = base + (int)threadIdx.x;
-        bool in = false;
-        double dd = 0.0, hh = 0.0, w[3] = {0.0, 0.0, 0.0}, iw = 0.0;
-        if (i < n) {
-          const float* q = P + 3 * (size_t)i;
-          const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
-                               (double)q[2] - (double)f.m[2]};
-          if (reselect) {
-            dd = cyl_dist_inv(M, x, w, &hh, &iw);
-            in = dd < dthr && dd > -dthr;
-          } else {
-            in = (bits[i >> 5] >> (i & 31)) & 1u;
-            if (in) dd = cyl_dist_inv(M, x, w, &hh, &iw);
-          }
-        }
-        if (reselect) {
-          // bits of point i are written and read by the same wave only
-          const uint64_t b = __builtin_amdgcn_ballot_w64(in);
-          if (lane == 0) {
-            const int wbase = (base + wave * 64) >> 5;
-            bits[wbase] = (uint32_t)b;
-            bits[wbase + 1] = (uint32_t)(b >> 32);
-          }
-        }
-        if (in) {
-          g[20] += 1.0;
-          const double rho = dd + M.r;
-          if (rho > 0.0) {
-            const double inv = iw;
-            const double n1 = (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]) * inv;
-            const double n2 = (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]) * inv;
-            const double J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0};
-            int t = 0;
-#pragma unroll
-            for (int a = 0; a < 5; ++a)
-#pragma unroll
-              for (int bb = a; bb < 5; ++bb) g[t++] += J[a] * J[bb];
-#pragma unroll
-            for (int a = 0; a < 5; ++a) g[15 + a] += J[a] * dd;
-          }
-        }
+  for (int q = 0; q < kRefitTerms; ++q) g[q] = 0.0f;
+  int cnt = 0;
+#pragma unroll 2
+  for (int j = 0; j < kRefitPerThread; ++j) {
+    const int gbase = ck.y + kRefitBlock * j + 64 * wave;
+    const int i = gbase + lane;
+    bool in = false;
+    float d = 0.f, hh = 0.f, rho = 0.f, w[3] = {0.f, 0.f, 0.f};
+    if (i < end) {
+      const float* q = P + 3 * (size_t)i;
+      const float x[3] = {q[0] - f.m[0], q[1] - f.m[1], q[2] - f.m[2]};
+      d = refit_dist(S, x, w, hh, rho);
+      in = reselect ? (d < thr && d > -thr) : ((mem[i >> 5] >> (i & 31)) & 1u) != 0u;
+    }
+    if (reselect) {
+      const uint64_t b = __builtin_amdgcn_ballot_w64(in);
+      if (lane == 0 && gbase < end) {
+        mem[gbase >> 5] = (uint32_t)b;
+        mem[(gbase >> 5) + 1] = (uint32_t)(b >> 32);
       }
-      bsum_lds<BLOCK, 21>(g, red21);
-      if (g[20] < 5.0) {
-        stop = true;
-        break;
-      }
-      double A[5][5], rhs[5], dl[5];
-      {
+    }
+    if (in) {
+      ++cnt;
+      if (rho > 0.0f) {
+        const float inv = 1.0f / rho;
+        const float n1 =
+            __builtin_fmaf(w[2], S.e1f[2], __builtin_fmaf(w[1], S.e1f[1], w[0] * S.e1f[0])) * inv;
+        const float n2 =
+            __builtin_fmaf(w[2], S.e2f[2], __builtin_fmaf(w[1], S.e2f[1], w[0] * S.e2f[0])) * inv;
+        const float J[5] = {-hh * n1, -hh * n2, -n1, -n2, -1.0f};
         int t = 0;
 #pragma unroll
         for (int a = 0; a < 5; ++a)
 #pragma unroll
-          for (int bb = a; bb < 5; ++bb) {
-            A[a][bb] = g[t];
-            A[bb][a] = g[t];
+          for (int b = a; b < 5; ++b) {
+            g[t] = __builtin_fmaf(J[a], J[b], g[t]);
             ++t;
           }
 #pragma unroll
-        for (int a = 0; a < 5; ++a) rhs[a] = -g[15 + a];
-      }
-      if (!cholesky_solve<5>(A, rhs, dl)) break;
-      double na[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) na[i] = M.a[i] + dl[0] * e1[i] + dl[1] * e2[i];
-      const double nl = sqrt(na[0] * na[0] + na[1] * na[1] + na[2] * na[2]);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        M.a[i] = na[i] / nl;
-        M.c[i] = M.c[i] + dl[2] * e1[i] + dl[3] * e2[i];
-      }
-      M.r = M.r + dl[4];
-      if (!(M.r > 0.0)) {
-        stop = true;
-        break;
+        for (int a = 0; a < 5; ++a) g[15 + a] = __builtin_fmaf(J[a], d, g[15 + a]);
       }
     }
   }
+  const int chunk_id = cl.x + ck.w;
+  chunk_partial<kRefitTerms>(g, red, partial + (size_t)chunk_id * kRefitTerms);
+  const int bc = block_sum_int(cnt, cnt_w);
+  if (threadIdx.x == 0) pcount[chunk_id] = bc;
+}
 
-  // 4. final inliers under the final model: count + centroid
-  double fin[4] = {0.0, 0.0, 0.0, 0.0};
-  if (M.r > 0.0) {
-    for (int i = threadIdx.x; i < n; i += BLOCK) {
-      const float* q = P + 3 * (size_t)i;
-      const double x[3] = {(double)q[0] - (double)f.m[0], (double)q[1] - (double)f.m[1],
-                           (double)q[2] - (double)f.m[2]};
-      double w[3], hh;
-      const double dd = cyl_dist(M, x, w, &hh);
-      if (dd < dthr && dd > -dthr) {
-        fin[0] += x[0];
-        fin[1] += x[1];
-        fin[2] += x[2];
-        fin[3] += 1.0;
+// The Gauss-Newton step's solve, one thread per cluster, after the step's
+// chunk partials: the chunks' (sum, error) pairs added in chunk order, the
+// FP64 5x5 normal equations, the next model.
+__global__ __launch_bounds__(64) void cyl_refit_solve_kernel(
+    int n_clusters, const int4* __restrict__ rclus, RefitState* __restrict__ state,
+    const Comp2* __restrict__ partial, const int32_t* __restrict__ pcount, int step,
+    const int32_t* __restrict__ bcnt) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= n_clusters || (bcnt && c >= bcnt[0])) return;
+  const RefitState S = state[c];
+  if (S.status != kRefitActive) return;
+  if (step >= S.rounds * S.iters) return;
+  const int round = step / S.iters;
+  if (S.skip_round == round) return;
+  const int4 cl = rclus[c];
+  int tot_cnt = 0;
+  for (int k = 0; k < cl.y; ++k) tot_cnt += pcount[cl.x + k];
+  if (tot_cnt < 5) {
+    state[c].status = kRefitStopped;
+    return;
+  }
+  double gs[kRefitTerms];
+  for (int q = 0; q < kRefitTerms; ++q) {
+    Comp2 t = {0.0f, 0.0f};
+    for (int k = 0; k < cl.y; ++k) comp_add(t, partial[(size_t)(cl.x + k) * kRefitTerms + q]);
+    gs[q] = (double)t.s + (double)t.e;
+  }
+  RefitState N = S;
+  double A[5][5], rhs[5], dl[5];
+  {
+    int t = 0;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+      for (int bb = a; bb < 5; ++bb) {
+        A[a][bb] = gs[t];
+        A[bb][a] = gs[t];
+        ++t;
+      }
+#pragma unroll
+    for (int a = 0; a < 5; ++a) rhs[a] = -gs[15 + a];
+  }
+  if (!cholesky_solve<5>(A, rhs, dl)) {
+    state[c].skip_round = round;
+    return;
+  }
+  double e1[3], e2[3];
+  sl_basis(S.a, e1, e2);
+  double na[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) na[i] = S.a[i] + dl[0] * e1[i] + dl[1] * e2[i];
+  const double nl = sqrt(na[0] * na[0] + na[1] * na[1] + na[2] * na[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    N.a[i] = na[i] / nl;
+    N.c[i] = S.c[i] + dl[2] * e1[i] + dl[3] * e2[i];
+  }
+  N.r = S.r + dl[4];
+  if (!(N.r > 0.0)) N.status = kRefitStopped;
+  refit_model_f32(N);
+  state[c] = N;
+}
+
+// Final inliers under the final model, their centroid, and the output record.
+__global__ __launch_bounds__(kRefitBlock) void cyl_refit_final_kernel(
+    const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
+    const CylFrame* __restrict__ frames, const int4* __restrict__ rchunks,
+    const int4* __restrict__ rclus, const RefitState* __restrict__ state,
+    Comp2* __restrict__ partial, int32_t* __restrict__ pcount, float thr,
+    const int32_t* __restrict__ bcnt) {
+  __shared__ RefitRed<3> red;
+  __shared__ int cnt_w[kRefitBlock / 64];
+  if (bcnt && (int)blockIdx.x >= bcnt[2]) return;
+  const int4 ck = rchunks[blockIdx.x];
+  const int c = ck.x;
+  if (c < 0) return;
+  const RefitState S = state[c];
+  if (S.status == kRefitInvalid || S.status == kRefitInactive) return;
+  const int off = offsets[c];
+  const float* P = xyz + 3 * (size_t)off;
+  const CylFrame f = frames[c];
+  const int4 cl = rclus[c];
+  const int end = ck.y + ck.z;
+  const bool pos = S.r > 0.0;
+  float g[3] = {0.0f, 0.0f, 0.0f};
+  int cnt = 0;
+  if (pos) {
+#pragma unroll 2
+    for (int j = 0; j < kRefitPerThread; ++j) {
+      const int i = ck.y + kRefitBlock * j + (int)threadIdx.x;
+      if (i < end) {
+        const float* q = P + 3 * (size_t)i;
+        const float x[3] = {q[0] - f.m[0], q[1] - f.m[1], q[2] - f.m[2]};
+        float w[3], hh, rho;
+        const float d = refit_dist(S, x, w, hh, rho);
+        if (d < thr && d > -thr) {
+          g[0] = g[0] + x[0];
+          g[1] = g[1] + x[1];
+          g[2] = g[2] + x[2];
+          ++cnt;
+        }
       }
     }
   }
-  bsum<BLOCK, 4>(fin, red);
-  const double k = fin[3];
-  if (threadIdx.x != 0) return;
-  if (!(M.r > 0.0) || k < 3.0) {
+  const int chunk_id = cl.x + ck.w;
+  chunk_partial<3>(g, red, partial + (size_t)chunk_id * kRefitTerms);
+  const int bc = block_sum_int(cnt, cnt_w);
+  if (threadIdx.x == 0) pcount[chunk_id] = bc;
+}
+
+// The output record of each cluster from the final pass's chunk partials
+// (one thread per cluster).
+__global__ __launch_bounds__(64) void cyl_refit_finish_kernel(
+    int n_clusters, const CylFrame* __restrict__ frames, const int4* __restrict__ rclus,
+    const RefitState* __restrict__ state, const Comp2* __restrict__ partial,
+    const int32_t* __restrict__ pcount, const int32_t* __restrict__ bcnt,
+    double* __restrict__ model) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= n_clusters || (bcnt && c >= bcnt[0])) return;
+  const RefitState S = state[c];
+  if (S.status == kRefitInvalid || S.status == kRefitInactive) return;
+  const int4 cl = rclus[c];
+  int tot_cnt = 0;
+  for (int k = 0; k < cl.y; ++k) tot_cnt += pcount[cl.x + k];
+  double* out = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
+  const double k = (double)tot_cnt;
+  if (!(S.r > 0.0) || tot_cnt < 3) {
     write_invalid(out, k);
     return;
   }
-  const double ht = (fin[0] / k - M.c[0]) * M.a[0] + (fin[1] / k - M.c[1]) * M.a[1] +
-                    (fin[2] / k - M.c[2]) * M.a[2];
-  double ray[3] = {M.a[0], M.a[1], M.a[2]};
+  double mean[3];
+  for (int i = 0; i < 3; ++i) {
+    Comp2 t = {0.0f, 0.0f};
+    for (int j = 0; j < cl.y; ++j) comp_add(t, partial[(size_t)(cl.x + j) * kRefitTerms + i]);
+    mean[i] = ((double)t.s + (double)t.e) / k;
+  }
+  const CylFrame f = frames[c];
+  const double ht =
+      (mean[0] - S.c[0]) * S.a[0] + (mean[1] - S.c[1]) * S.a[1] + (mean[2] - S.c[2]) * S.a[2];
+  double ray[3] = {S.a[0], S.a[1], S.a[2]};
   sl_sign_normalise(ray);
   for (int i = 0; i < 3; ++i) {
-    out[i] = (double)f.m[i] + M.c[i] + ht * M.a[i];
+    out[i] = (double)f.m[i] + S.c[i] + ht * S.a[i];
     out[3 + i] = ray[i];
   }
-  out[6] = M.r;
+  out[6] = S.r;
   out[7] = k;
 }
 
@@ -833,72 +963,176 @@ struct sloam_cyl_plan {
   int tile = 0;
   int n_work = 0;
   int4* d_work = nullptr;
-  int32_t* d_lists = nullptr;   // small-cluster list then large-cluster list
-  int tier_n[kTiers] = {};      // clusters per refit tier
-  int tier_off[kTiers] = {};    // first list entry of each tier
-  int tier_bits[kTiers] = {};   // LDS bytes of the membership bits
+  // refit: chunks {cluster, first point, points, chunk index} in launch
+  // order, and per cluster {first chunk id, chunks, first membership word, 0}
+  int n_rchunks = 0;
+  int4* d_rchunks = nullptr;
+  int4* d_rclus = nullptr;
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
+  RefitState* d_state = nullptr;
+  uint32_t* d_member = nullptr;  // membership bits of the current round
+  Comp2* d_partial = nullptr;    // per chunk: kRefitTerms (sum, error) pairs
+  int32_t* d_pcount = nullptr;   // per chunk: inliers counted
   int n_xcd = 1;
-  int n_simd = 1024;  // SIMDs of the device (4 per CU)
-  // d_work and d_lists share one allocation (the work list, then the refit
-  // lists), uploaded in one copy; allocated sizes (bytes for the tables,
-  // elements otherwise) only grow across sloam_cyl_plan_reset
+  // counted plans: the layout is made on the device at every exec from the
+  // offsets and the cluster count at d_n (at most cap clusters and
+  // cap_points points); n_clusters, n_work and n_rchunks are then the grid
+  // sizes, d_cnt the batch's own {clusters, work items, refit chunks,
+  // overflow}
+  bool counted = false;
+  const int32_t* d_n = nullptr;
+  int64_t cap_points = 0;
+  int cpc = 0;  // work items per cluster
+  int32_t* d_cnt = nullptr;
+  // d_work, d_rchunks and d_rclus share one allocation, uploaded in one copy;
+  // allocated sizes (bytes for the tables, elements otherwise) only grow
+  // across sloam_cyl_plan_reset
   void* d_tables = nullptr;
-  size_t cap_tables = 0, cap_frames = 0, cap_counts = 0;
-  // pinned staging of the work and refit lists (bytes): their uploads are
-  // asynchronous on the context stream, ahead of the runs that read them
+  size_t cap_tables = 0, cap_frames = 0, cap_counts = 0, cap_state = 0, cap_member = 0,
+         cap_partial = 0;
+  // pinned staging of the tables (bytes): their uploads are asynchronous on
+  // the context stream, ahead of the runs that read them
   void* h_stage = nullptr;
   size_t cap_stage = 0;
 };
 
 static int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// active[c] = 1 when cluster c's model keeps fewer than min_ratio of its
-// points as inliers (or is invalid); *n_active counts them
-// Launches the refit tiers of plan p: concurrently, the larger tiers on
-// auxiliary streams forked from and joined back into the context stream.
+// Launches the refit of plan p: the init pass, one launch per Gauss-Newton
+// step (as many as the longest schedule among the clusters), the final pass.
 // alt_flags (nullable) marks the clusters that take their best hypothesis
 // from alt_counts (alt_n_hyp hypotheses of sample stream alt_tag, no
-// proposal) instead of counts.
+// proposal) and refit with alt_rounds x alt_iters steps, instead of counts.
 static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_offsets,
-                 const int32_t* counts, const float* d_proposal, const int32_t* d_active,
-                 const int32_t* alt_flags, const int32_t* alt_counts, int alt_n_hyp,
-                 uint32_t alt_tag, int alt_rounds, int alt_iters, uint32_t* d_mask,
-                 int32_t* d_best, double* d_model) {
+                        const int32_t* counts, const float* d_proposal, const int32_t* d_active,
+                        const int32_t* alt_flags, const int32_t* alt_counts, int alt_n_hyp,
+                        uint32_t alt_tag, int alt_rounds, int alt_iters, uint32_t* d_mask,
+                        int32_t* d_best, double* d_model) {
   sloam_ctx* ctx = p->ctx;
   const sloam_cyl_fit_cfg& cfg = p->cfg;
+  if (p->n_rchunks == 0) return SLOAM_OK;
   const uint32_t k0 = (uint32_t)cfg.seed;
   const uint32_t k1 = (uint32_t)(cfg.seed >> 32);
   const uint32_t tag = (uint32_t)cfg.stream_tag;
-  int st = SLOAM_OK;
-  for (int t = kTiers - 1; t >= 0; --t) {
-    if (p->tier_n[t] == 0) continue;
-    hipStream_t s = ctx->stream;
-    if (t > 0) {
-      st = sloam_ctx_fork(ctx, t - 1);
-      if (st) return st;
-      s = ctx->aux[t - 1];
-    }
-    const int32_t* list = p->d_lists + p->tier_off[t];
-    const dim3 g(p->tier_n[t]);
-    const size_t lds = p->tier_bits[t];
-#define SLOAM_REFIT_ARGS                                                                      \
-  d_xyz, d_offsets, p->d_frames, counts, list, cfg.n_hyp, cfg.thr, cfg.max_radius, k0, k1, tag, \
-      cfg.refit_rounds, cfg.refit_iters, d_proposal, d_active, alt_flags, alt_counts, alt_n_hyp, \
-      alt_tag, alt_rounds, alt_iters, d_mask, d_best, d_model
-    if (kTierBlock[t] == 64)
-      hipLaunchKernelGGL(cyl_refit_kernel<64>, g, dim3(64), lds, s, SLOAM_REFIT_ARGS);
-    else
-      hipLaunchKernelGGL(cyl_refit_kernel<256>, g, dim3(256), lds, s, SLOAM_REFIT_ARGS);
-#undef SLOAM_REFIT_ARGS
+  const dim3 grid(p->n_rchunks), block(kRefitBlock);
+  const int32_t* cnt = p->counted ? p->d_cnt : nullptr;
+  hipLaunchKernelGGL(cyl_refit_init_kernel, grid, block, 0, ctx->stream, d_xyz, d_offsets,
+                     p->d_frames, counts, p->d_rchunks, p->d_rclus, cfg.n_hyp, cfg.thr,
+                     cfg.max_radius, k0, k1, tag, cfg.refit_rounds, cfg.refit_iters, d_proposal,
+                     d_active, alt_flags, alt_counts, alt_n_hyp, alt_tag, alt_rounds, alt_iters,
+                     cnt, p->d_state, p->d_member, d_mask, d_best, d_model);
+  SLOAM_HIP(ctx, hipGetLastError());
+  const dim3 cgrid((p->n_clusters + 63) / 64), cblock(64);
+  int steps = cfg.refit_rounds * cfg.refit_iters;
+  if (alt_flags) steps = std::max(steps, alt_rounds * alt_iters);
+  for (int s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(cyl_refit_step_kernel, grid, block, 0, ctx->stream, d_xyz, d_offsets,
+                       p->d_frames, p->d_rchunks, p->d_rclus, p->d_state, p->d_member,
+                       p->d_partial, p->d_pcount, s, cfg.thr, cnt);
+    SLOAM_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(cyl_refit_solve_kernel, cgrid, cblock, 0, ctx->stream, p->n_clusters,
+                       p->d_rclus, p->d_state, p->d_partial, p->d_pcount, s, cnt);
     SLOAM_HIP(ctx, hipGetLastError());
   }
-  for (int t = 1; t < kTiers; ++t)
-    if (p->tier_n[t] > 0) {
-      st = sloam_ctx_join(ctx, t - 1);
-      if (st) return st;
+  hipLaunchKernelGGL(cyl_refit_final_kernel, grid, block, 0, ctx->stream, d_xyz, d_offsets,
+                     p->d_frames, p->d_rchunks, p->d_rclus, p->d_state, p->d_partial,
+                     p->d_pcount, cfg.thr, cnt);
+  SLOAM_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(cyl_refit_finish_kernel, cgrid, cblock, 0, ctx->stream, p->n_clusters,
+                     p->d_frames, p->d_rclus, p->d_state, p->d_partial, p->d_pcount, cnt, d_model);
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
+// Layout of a counted plan, made on the device: the work list (cluster c's
+// hypothesis chunks at c * cpc .. c * cpc + cpc - 1, clusters in index order),
+// the refit chunks and per-cluster records (prefix sums over the clusters),
+// and cnt = {clusters, work items, refit chunks, overflow}.  A batch beyond
+// the capacities sets overflow and zero counts (nothing runs).  One
+// workgroup of 256 threads.
+__global__ __launch_bounds__(256) void cyl_layout_kernel(
+    const int32_t* __restrict__ offsets, const int32_t* __restrict__ d_n, int cap,
+    long long cap_points, int n_hyp, int hcap, int cpc, int cap_rchunks, long long cap_words,
+    int4* __restrict__ work, int4* __restrict__ rchunks, int4* __restrict__ rclus,
+    int32_t* __restrict__ cnt) {
+  __shared__ int wck[4], wwd[4];
+  __shared__ long long s_ck, s_wd;
+  __shared__ int s_over;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nin = *d_n;
+  const int n = min(max(nin, 0), cap);
+  if (tid == 0) {
+    s_ck = 0;
+    s_wd = 0;
+    s_over = (nin > cap || nin < 0) ? 1 : 0;
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += 256) {
+    const int c = c0 + tid;
+    int len = 0, nck = 0, nw = 0;
+    if (c < n) {
+      len = offsets[c + 1] - offsets[c];
+      nck = max(1, (len + kRefitChunk - 1) / kRefitChunk);
+      nw = 2 * ((len + 63) / 64);
     }
+    // inclusive wave scans, then the waves' totals
+    int sc = nck, sw = nw;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int a = __shfl_up(sc, o, 64), b = __shfl_up(sw, o, 64);
+      if (lane >= o) {
+        sc += a;
+        sw += b;
+      }
+    }
+    if (lane == 63) {
+      wck[wave] = sc;
+      wwd[wave] = sw;
+    }
+    __syncthreads();
+    long long cb = s_ck, wb = s_wd;
+    for (int w = 0; w < wave; ++w) {
+      cb += wck[w];
+      wb += wwd[w];
+    }
+    cb += sc - nck;
+    wb += sw - nw;
+    if (c < n) {
+      rclus[c] = make_int4((int)cb, nck, (int)wb, 0);
+      for (int k = 0; k < nck; ++k)
+        if (cb + k < cap_rchunks)
+          rchunks[cb + k] = make_int4(c, k * kRefitChunk, min(kRefitChunk, len - k * kRefitChunk), k);
+      for (int j = 0; j < cpc; ++j)
+        work[(size_t)c * cpc + j] = make_int4(c, j * hcap, min(hcap, n_hyp - j * hcap), len);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 0; w < 4; ++w) {
+        s_ck += wck[w];
+        s_wd += wwd[w];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const bool over = s_over || s_ck > cap_rchunks || s_wd > cap_words ||
+                      (long long)offsets[n] - (long long)offsets[0] > cap_points;
+    cnt[0] = over ? 0 : n;
+    cnt[1] = over ? 0 : n * cpc;
+    cnt[2] = over ? 0 : (int)s_ck;
+    cnt[3] = over ? 1 : 0;
+  }
+}
+
+// Enqueues the device layout of a counted plan (nothing for other plans).
+static int layout_counted(sloam_cyl_plan* p, const int32_t* d_offsets) {
+  if (!p->counted) return SLOAM_OK;
+  sloam_ctx* ctx = p->ctx;
+  const long long cap_words = (long long)p->cap_member;
+  hipLaunchKernelGGL(cyl_layout_kernel, dim3(1), dim3(256), 0, ctx->stream, d_offsets, p->d_n,
+                     p->n_clusters, (long long)p->cap_points, p->cfg.n_hyp, p->hcap, p->cpc,
+                     p->n_rchunks, cap_words, p->d_work, p->d_rchunks, p->d_rclus, p->d_cnt);
+  SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
 }
 
@@ -909,11 +1143,12 @@ __global__ __launch_bounds__(256) void cyl_decide_kernel(const int32_t* __restri
                                                          int n_clusters,
                                                          const int32_t* __restrict__ counts,
                                                          int n_hyp, float min_ratio,
+                                                         const int32_t* __restrict__ bcnt,
                                                          int32_t* __restrict__ flags,
                                                          int32_t* __restrict__ n_flagged) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   bool a = false;
-  if (c < n_clusters) {
+  if (c < n_clusters && (!bcnt || c < bcnt[0])) {
     const int n = offsets[c + 1] - offsets[c];
     int best = 0;
     for (int h = 0; h < n_hyp; ++h) best = max(best, counts[(size_t)c * n_hyp + h]);
@@ -956,7 +1191,7 @@ static int grow(sloam_ctx* ctx, void** ptr, size_t* cap, size_t n, size_t elem) 
 }
 
 // Lays the batch h_offsets (n_clusters + 1 entries) out on plan p, whose cfg,
-// hcap and tile are set: the scoring work list and the refit tier lists,
+// hcap and tile are set: the scoring work list and the refit chunks,
 // uploaded into buffers that only grow.
 static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_clusters) {
   sloam_ctx* ctx = p->ctx;
@@ -1002,68 +1237,98 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
     for (int x = 0; x < n_xcd; ++x)
       work.push_back(k < bins[x].size() ? bins[x][k] : make_int4(-1, 0, 0, 0));
   p->n_work = (int)work.size();
-  // refit lists, one per tier, each in decreasing size
-  const int* tier_max = n_clusters < p->n_simd ? kTierMaxSmall : kTierMax;
-  std::vector<int32_t> tier_list[kTiers];
-  int tier_maxlen[kTiers] = {};
+  // refit chunks: every cluster (even an empty one) has at least one; the
+  // clusters are dealt to per-XCD bins as above and the bins interleaved, so
+  // all chunks of a cluster run on one XCD
+  std::vector<std::vector<int4>> rbins(n_xcd);
+  std::vector<long long> rload(n_xcd, 0);
+  std::vector<int4> rclus(n_clusters);
+  int n_chunk_ids = 0;
+  size_t n_words = 0;
+  for (int c = 0; c < n_clusters; ++c) {
+    const int len = h_offsets[c + 1] - h_offsets[c];
+    const int nck = std::max(1, (len + kRefitChunk - 1) / kRefitChunk);
+    rclus[c] = make_int4(n_chunk_ids, nck, (int)n_words, 0);
+    n_chunk_ids += nck;
+    n_words += 2 * (size_t)((len + 63) / 64);
+  }
   for (int c : order) {
     const int len = h_offsets[c + 1] - h_offsets[c];
-    int t = 0;
-    while (t + 1 < kTiers && len > tier_max[t]) ++t;
-    tier_list[t].push_back(c);
-    tier_maxlen[t] = std::max(tier_maxlen[t], len);
+    const int x = (int)(std::min_element(rload.begin(), rload.end()) - rload.begin());
+    for (int k = 0; k < rclus[c].y; ++k) {
+      const int first = k * kRefitChunk;
+      rbins[x].push_back(make_int4(c, first, std::min(kRefitChunk, len - first), k));
+    }
+    rload[x] += std::max(len, 1);
   }
-  std::vector<int32_t> lists;
-  for (int t = 0; t < kTiers; ++t) {
-    p->tier_n[t] = (int)tier_list[t].size();
-    p->tier_off[t] = (int)lists.size();
-    p->tier_bits[t] = round_up(std::max(tier_maxlen[t], 1), kTierBlock[t]) / 8;
-    lists.insert(lists.end(), tier_list[t].begin(), tier_list[t].end());
-  }
-  const size_t wb = sizeof(int4) * work.size(), lb = sizeof(int32_t) * lists.size();
-  int st = grow(ctx, &p->d_tables, &p->cap_tables, wb + lb, 1);
+  size_t rlongest = 0;
+  for (const auto& b : rbins) rlongest = std::max(rlongest, b.size());
+  std::vector<int4> rchunks;
+  rchunks.reserve(rlongest * n_xcd);
+  for (size_t k = 0; k < rlongest; ++k)
+    for (int x = 0; x < n_xcd; ++x)
+      rchunks.push_back(k < rbins[x].size() ? rbins[x][k] : make_int4(-1, 0, 0, 0));
+  p->n_rchunks = (int)rchunks.size();
+  const size_t wb = sizeof(int4) * work.size(), cb = sizeof(int4) * rchunks.size(),
+               lb = sizeof(int4) * rclus.size();
+  int st = grow(ctx, &p->d_tables, &p->cap_tables, wb + cb + lb, 1);
   if (!st) st = grow(ctx, (void**)&p->d_frames, &p->cap_frames, n_clusters, sizeof(CylFrame));
   if (!st)
     st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)n_clusters * cfg.n_hyp,
               sizeof(int32_t));
+  if (!st) st = grow(ctx, (void**)&p->d_state, &p->cap_state, n_clusters, sizeof(RefitState));
+  if (!st) st = grow(ctx, (void**)&p->d_member, &p->cap_member, std::max<size_t>(n_words, 1),
+                     sizeof(uint32_t));
+  if (!st && (size_t)n_chunk_ids > p->cap_partial) {
+    sloam_dev_free(p->d_pcount);
+    p->d_pcount = nullptr;
+    st = grow(ctx, (void**)&p->d_partial, &p->cap_partial, n_chunk_ids,
+              sizeof(Comp2) * kRefitTerms);
+    if (!st) st = sloam_dev_alloc(ctx, (void**)&p->d_pcount, sizeof(int32_t) * p->cap_partial);
+  }
   if (st) return st;
   p->d_work = static_cast<int4*>(p->d_tables);
-  p->d_lists = reinterpret_cast<int32_t*>(static_cast<char*>(p->d_tables) + wb);
-  // both lists through the pinned staging buffer, in one copy (the caller has
-  // made sure that no earlier upload from it is pending)
-  if (wb + lb > p->cap_stage) {
+  p->d_rchunks = reinterpret_cast<int4*>(static_cast<char*>(p->d_tables) + wb);
+  p->d_rclus = reinterpret_cast<int4*>(static_cast<char*>(p->d_tables) + wb + cb);
+  // the tables through the pinned staging buffer, in one copy (the caller
+  // has made sure that no earlier upload from it is pending)
+  const size_t tb = wb + cb + lb;
+  if (tb > p->cap_stage) {
     if (p->h_stage) (void)hipHostFree(p->h_stage);
     p->h_stage = nullptr;
     p->cap_stage = 0;
-    const size_t sz = (wb + lb) + (wb + lb) / 2;
+    const size_t sz = tb + tb / 2;
     hipError_t e = hipHostMalloc(&p->h_stage, sz);
     if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipHostMalloc(plan staging)");
     p->cap_stage = sz;
   }
   char* stage = static_cast<char*>(p->h_stage);
   if (wb) std::memcpy(stage, work.data(), wb);
-  if (lb) std::memcpy(stage + wb, lists.data(), lb);
-  if (wb + lb)
-    SLOAM_HIP(ctx, hipMemcpyAsync(p->d_tables, stage, wb + lb, hipMemcpyHostToDevice, ctx->stream));
+  if (cb) std::memcpy(stage + wb, rchunks.data(), cb);
+  if (lb) std::memcpy(stage + wb + cb, rclus.data(), lb);
+  if (tb) SLOAM_HIP(ctx, hipMemcpyAsync(p->d_tables, stage, tb, hipMemcpyHostToDevice, ctx->stream));
   return SLOAM_OK;
 }
 
 extern "C" {
 
-int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
-                          const sloam_cyl_fit_cfg* cfg, sloam_cyl_plan** out) {
-  if (!ctx || !h_offsets || !cfg || !out || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
-  *out = nullptr;
+}  // extern "C"
+
+// A plan with cfg's launch shapes and no batch yet (nullptr when cfg is
+// invalid; *st says why).
+static sloam_cyl_plan* plan_new(sloam_ctx* ctx, const sloam_cyl_fit_cfg* cfg, int* st) {
+  *st = SLOAM_ERR_INVALID_ARG;
   if (cfg->n_hyp < 1 || cfg->n_hyp > (1 << 20) || !(cfg->thr > 0.0f) ||
-      !(cfg->max_radius > 0.0f) || cfg->refit_iters < 0 || cfg->refit_rounds < 0 || cfg->hyp_chunk < 0 ||
-      cfg->tile_points < 0)
-    return SLOAM_ERR_INVALID_ARG;
-  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+      !(cfg->max_radius > 0.0f) || cfg->refit_iters < 0 || cfg->refit_rounds < 0 ||
+      cfg->hyp_chunk < 0 || cfg->tile_points < 0)
+    return nullptr;
   sloam_cyl_plan* p = new (std::nothrow) sloam_cyl_plan();
-  if (!p) return SLOAM_ERR_ALLOC;
+  if (!p) {
+    *st = SLOAM_ERR_ALLOC;
+    return nullptr;
+  }
   p->ctx = ctx;
   p->cfg = *cfg;
-
   // hypotheses per workgroup: a multiple of kWaves * kHB
   int chunk = cfg->hyp_chunk > 0 ? cfg->hyp_chunk : 256;
   chunk = std::min(chunk, round_up(cfg->n_hyp, kWaves * kHB));
@@ -1073,7 +1338,8 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
   const int hyp_bytes = p->hcap * (int)(sizeof(float4) + sizeof(int));
   if (hyp_bytes > kMaxLds - 64 * 12) {
     delete p;
-    return SLOAM_ERR_TOO_LARGE;
+    *st = SLOAM_ERR_TOO_LARGE;
+    return nullptr;
   }
   p->tile = std::min(p->tile, ((kMaxLds - hyp_bytes) / 12) / 64 * 64);
   int n_xcd = 1;
@@ -1082,14 +1348,21 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
     n_xcd = 1;
   (void)hipGetLastError();
   p->n_xcd = n_xcd;
-  int n_cu = 256;
-  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
-          hipSuccess ||
-      n_cu < 1)
-    n_cu = 256;
-  (void)hipGetLastError();
-  p->n_simd = 4 * n_cu;
-  const int st = plan_layout(p, h_offsets, n_clusters);
+  *st = SLOAM_OK;
+  return p;
+}
+
+extern "C" {
+
+int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_clusters,
+                          const sloam_cyl_fit_cfg* cfg, sloam_cyl_plan** out) {
+  if (!ctx || !h_offsets || !cfg || !out || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
+  *out = nullptr;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  int st = SLOAM_OK;
+  sloam_cyl_plan* p = plan_new(ctx, cfg, &st);
+  if (!p) return st;
+  st = plan_layout(p, h_offsets, n_clusters);
   if (st) {
     sloam_cyl_plan_destroy(p);
     return st;
@@ -1098,8 +1371,58 @@ int sloam_cyl_plan_create(sloam_ctx* ctx, const int32_t* h_offsets, int32_t n_cl
   return SLOAM_OK;
 }
 
+int sloam_cyl_plan_create_counted(sloam_ctx* ctx, int32_t cap_clusters, int64_t cap_points,
+                                  const int32_t* d_n_clusters, const sloam_cyl_fit_cfg* cfg,
+                                  sloam_cyl_plan** out) {
+  if (!ctx || !cfg || !out || !d_n_clusters || cap_clusters < 1 || cap_points < 0 ||
+      cap_points > ((int64_t)1 << 31) - 1 || cap_clusters > (1 << 20))
+    return SLOAM_ERR_INVALID_ARG;
+  *out = nullptr;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  int st = SLOAM_OK;
+  sloam_cyl_plan* p = plan_new(ctx, cfg, &st);
+  if (!p) return st;
+  p->counted = true;
+  p->d_n = d_n_clusters;
+  p->cap_points = cap_points;
+  p->cpc = (cfg->n_hyp + p->hcap - 1) / p->hcap;
+  p->n_clusters = cap_clusters;
+  p->n_points = cap_points;
+  p->n_work = cap_clusters * p->cpc;
+  p->n_rchunks = (int)(cap_clusters + cap_points / kRefitChunk + 1);
+  const size_t words = (size_t)(cap_points / 32) + 2 * (size_t)cap_clusters + 2;
+  const size_t tb = sizeof(int4) * ((size_t)p->n_work + p->n_rchunks + cap_clusters);
+  st = grow(ctx, &p->d_tables, &p->cap_tables, tb, 1);
+  if (!st) st = grow(ctx, (void**)&p->d_frames, &p->cap_frames, cap_clusters, sizeof(CylFrame));
+  if (!st)
+    st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)cap_clusters * cfg->n_hyp,
+              sizeof(int32_t));
+  if (!st) st = grow(ctx, (void**)&p->d_state, &p->cap_state, cap_clusters, sizeof(RefitState));
+  if (!st) st = grow(ctx, (void**)&p->d_member, &p->cap_member, words, sizeof(uint32_t));
+  if (!st)
+    st = grow(ctx, (void**)&p->d_partial, &p->cap_partial, p->n_rchunks,
+              sizeof(Comp2) * kRefitTerms);
+  if (!st) st = sloam_dev_alloc(ctx, (void**)&p->d_pcount, sizeof(int32_t) * p->n_rchunks);
+  if (!st) st = sloam_dev_alloc(ctx, (void**)&p->d_cnt, sizeof(int32_t) * 4);
+  if (st) {
+    sloam_cyl_plan_destroy(p);
+    return st;
+  }
+  p->d_work = static_cast<int4*>(p->d_tables);
+  p->d_rchunks = p->d_work + p->n_work;
+  p->d_rclus = p->d_rchunks + p->n_rchunks;
+  *out = p;
+  return SLOAM_OK;
+}
+
+int sloam_cyl_plan_counts(const sloam_cyl_plan* p, const int32_t** d_counts) {
+  if (!p || !d_counts || !p->counted) return SLOAM_ERR_INVALID_ARG;
+  *d_counts = p->d_cnt;
+  return SLOAM_OK;
+}
+
 int sloam_cyl_plan_reset(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_clusters) {
-  if (!p || !h_offsets || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
+  if (!p || !h_offsets || n_clusters < 0 || p->counted) return SLOAM_ERR_INVALID_ARG;
   sloam_ctx* ctx = p->ctx;
   if (ctx->capturing) return SLOAM_ERR_INVALID_ARG;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
@@ -1111,7 +1434,7 @@ int sloam_cyl_plan_reset(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_
     p->n_clusters = 0;
     p->n_points = 0;
     p->n_work = 0;
-    for (int t = 0; t < kTiers; ++t) p->tier_n[t] = 0;
+    p->n_rchunks = 0;
   }
   return st;
 }
@@ -1159,11 +1482,18 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
   int32_t* cv = d_counts_verify ? d_counts_verify : verify->d_counts;
   int32_t* cp = d_counts_plain ? d_counts_plain : plain->d_counts;
   const int S = verify->n_clusters;
+  if (verify->counted != plain->counted ||
+      (verify->counted && (verify->d_n != plain->d_n || verify->cap_points != plain->cap_points)))
+    return SLOAM_ERR_INVALID_ARG;
+  int st0 = layout_counted(verify, d_offsets);
+  if (!st0) st0 = layout_counted(plain, d_offsets);
+  if (st0) return st0;
+  const int32_t* vcnt = verify->counted ? verify->d_cnt : nullptr;
   {
     // one axis seed per cluster serves both hypothesis sets
     SloamKernelScope ks(ctx, "cyl_seed");
     hipLaunchKernelGGL(cyl_seed_kernel, dim3(S), dim3(kBlock), 0, ctx->stream, d_xyz, d_offsets,
-                       S, nullptr, verify->d_frames);
+                       S, nullptr, vcnt, verify->d_frames);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (verify->n_work > 0) {
@@ -1173,7 +1503,7 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
     hipLaunchKernelGGL(cyl_score_kernel, dim3(verify->n_work), dim3(kBlock), lds, ctx->stream,
                        d_xyz, d_offsets, verify->d_frames, verify->d_work, vc.n_hyp, verify->hcap,
                        verify->tile, vc.thr, vc.max_radius, k0, k1, (uint32_t)vc.stream_tag,
-                       d_proposal, nullptr, cv, nullptr);
+                       d_proposal, nullptr, vcnt, cv, nullptr);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {
@@ -1183,7 +1513,8 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
       if (st) return st;
     }
     hipLaunchKernelGGL(cyl_decide_kernel, dim3((S + 255) / 256), dim3(256), 0, ctx->stream,
-                       d_offsets, S, cv, vc.n_hyp, min_inlier_ratio, d_fallback, d_n_fallback);
+                       d_offsets, S, cv, vc.n_hyp, min_inlier_ratio, vcnt, d_fallback,
+                       d_n_fallback);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (plain->n_work > 0) {
@@ -1194,7 +1525,7 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
     hipLaunchKernelGGL(cyl_score_kernel, dim3(plain->n_work), dim3(kBlock), lds, ctx->stream,
                        d_xyz, d_offsets, verify->d_frames, plain->d_work, pc.n_hyp, plain->hcap,
                        plain->tile, pc.thr, pc.max_radius, k0, k1, (uint32_t)pc.stream_tag,
-                       nullptr, d_fallback, cp, nullptr);
+                       nullptr, d_fallback, plain->counted ? plain->d_cnt : nullptr, cp, nullptr);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {
@@ -1287,6 +1618,11 @@ void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
   sloam_dev_free(p->d_tables);
   sloam_dev_free(p->d_frames);
   sloam_dev_free(p->d_counts);
+  sloam_dev_free(p->d_state);
+  sloam_dev_free(p->d_member);
+  sloam_dev_free(p->d_partial);
+  sloam_dev_free(p->d_pcount);
+  sloam_dev_free(p->d_cnt);
   delete p;
 }
 
@@ -1311,6 +1647,10 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
   const uint32_t k1 = (uint32_t)(cfg.seed >> 32);
   const uint32_t tag = (uint32_t)cfg.stream_tag;
   int32_t* counts = d_counts ? d_counts : p->d_counts;
+  if (p->counted && d_mask) return SLOAM_ERR_INVALID_ARG;  // its size is the batch's
+  int st0 = layout_counted(p, d_offsets);
+  if (st0) return st0;
+  const int32_t* cnt = p->counted ? p->d_cnt : nullptr;
   if (d_mask) {
     int st = sloam_fill_u32(ctx, d_mask, (size_t)((p->n_points + 31) / 32), 0u);
     if (st) return st;
@@ -1318,7 +1658,7 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
   {
     SloamKernelScope ks(ctx, "cyl_seed");
     hipLaunchKernelGGL(cyl_seed_kernel, dim3(p->n_clusters), dim3(kBlock), 0, ctx->stream, d_xyz,
-                       d_offsets, p->n_clusters, d_active, p->d_frames);
+                       d_offsets, p->n_clusters, d_active, cnt, p->d_frames);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (p->n_work > 0) {
@@ -1326,7 +1666,7 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
     SloamKernelScope ks(ctx, "cyl_score");
     hipLaunchKernelGGL(cyl_score_kernel, dim3(p->n_work), dim3(kBlock), lds, ctx->stream, d_xyz,
                        d_offsets, p->d_frames, p->d_work, cfg.n_hyp, p->hcap, p->tile, cfg.thr,
-                       cfg.max_radius, k0, k1, tag, d_proposal, d_active, counts, d_samples);
+                       cfg.max_radius, k0, k1, tag, d_proposal, d_active, cnt, counts, d_samples);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {

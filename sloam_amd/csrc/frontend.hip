@@ -257,6 +257,14 @@ int sloam_frontend_host_view(const sloam_frontend* f, sloam_frontend_host* out) 
   return SLOAM_OK;
 }
 
+int sloam_frontend_enqueue(sloam_frontend* f) {
+  if (!f) return SLOAM_ERR_INVALID_ARG;
+  SLOAM_HIP(f->ctx, hipSetDevice(f->ctx->device));
+  const int st = enqueue(f);
+  if (!st) ++f->runs;
+  return st;
+}
+
 int sloam_frontend_exec(sloam_frontend* f) {
   if (!f) return SLOAM_ERR_INVALID_ARG;
   sloam_ctx* ctx = f->ctx;

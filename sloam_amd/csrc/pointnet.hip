@@ -100,13 +100,20 @@ __device__ inline bf16x8 frag_from_lds(const unsigned short* p) {
 }
 
 // ---------------------------------------------------------------------------
+// d_n (nullable): the cluster count in device memory, at most n_clusters
+// (the grid's size); likewise in pn_fwd and pn_head
+__device__ inline int pn_count(int n_clusters, const int32_t* d_n) {
+  return d_n ? min(n_clusters, max(*d_n, 0)) : n_clusters;
+}
+
 __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ xyz,
                                                       const int32_t* __restrict__ offsets,
-                                                      int n_clusters, float4* __restrict__ X,
+                                                      int n_clusters, const int32_t* __restrict__ d_n,
+                                                      float4* __restrict__ X,
                                                       PnMeta* __restrict__ meta) {
   __shared__ float red[kFB / 64][6];
   const int c = blockIdx.x;
-  if (c >= n_clusters) return;
+  if (c >= pn_count(n_clusters, d_n)) return;
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
   const int m = n < kP ? n : kP;
@@ -181,8 +188,8 @@ __global__ __launch_bounds__(kFB) void pn_prep_kernel(const float* __restrict__ 
 template <bool kTrain>
 __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
     const float* __restrict__ prm, const float4* __restrict__ X, const PnMeta* __restrict__ meta,
-    int n_clusters, float* __restrict__ act, unsigned short* __restrict__ H1,
-    unsigned short* __restrict__ H2) {
+    int n_clusters, const int32_t* __restrict__ d_n, float* __restrict__ act,
+    unsigned short* __restrict__ H1, unsigned short* __restrict__ H2) {
   __shared__ __align__(16) unsigned short h1s[kChunk * kS1];
   __shared__ __align__(16) unsigned short h2s[kChunk * kS2];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -218,7 +225,8 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
   }
 
   // persistent over clusters: the weight fragments stay in registers
-  for (int c = blockIdx.x; c < n_clusters; c += gridDim.x) {
+  const int n_used = pn_count(n_clusters, d_n);
+  for (int c = blockIdx.x; c < n_used; c += gridDim.x) {
   const int m = meta[c].m;
   // training: biased outputs, ReLU floor 0; inference: raw products
   float best[2] = {kTrain ? 0.f : -INFINITY, kTrain ? 0.f : -INFINITY};
@@ -372,14 +380,17 @@ __global__ __launch_bounds__(kFW) void pn_fwd_kernel(
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kFB) void pn_head_kernel(const float* __restrict__ prm,
                                                       const PnMeta* __restrict__ meta,
-                                                      int n_clusters, float* __restrict__ act,
+                                                      int n_clusters,
+                                                      const int32_t* __restrict__ d_n,
+                                                      float* __restrict__ act,
                                                       float* __restrict__ prop) {
   __shared__ float fs[kHC][kC3];
   __shared__ float h4s[kHC][kC4];
   __shared__ float outs[kHC][8];
   const int t = threadIdx.x;
   const int c0 = blockIdx.x * kHC;
-  const int nc = min(kHC, n_clusters - c0);
+  const int nc = min(kHC, pn_count(n_clusters, d_n) - c0);
+  if (nc <= 0) return;
   for (int i = t; i < nc * kC3; i += kFB)
     fs[i / kC3][i % kC3] = act[(size_t)(c0 + i / kC3) * kAct + kAF + i % kC3];
   __syncthreads();
@@ -754,14 +765,14 @@ int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   {
     SloamKernelScope ks(ctx, "pn_prep");
     hipLaunchKernelGGL(pn_prep_kernel, dim3(B), dim3(kFB), 0, ctx->stream, d_xyz, d_offsets, B,
-                       b->X, b->meta);
+                       (const int32_t*)nullptr, b->X, b->meta);
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
     hipLaunchKernelGGL(pn_fwd_kernel<true>, dim3(fwd_grid(ctx, B)), dim3(kFW), 0, ctx->stream,
-                       d_params, b->X, b->meta, B, b->act, b->H1, b->H2);
+                       d_params, b->X, b->meta, B, (const int32_t*)nullptr, b->act, b->H1, b->H2);
     hipLaunchKernelGGL(pn_head_kernel, dim3((B + kHC - 1) / kHC), dim3(kFB), 0, ctx->stream,
-                       d_params, b->meta, B, b->act, (float*)nullptr);
+                       d_params, b->meta, B, (const int32_t*)nullptr, b->act, (float*)nullptr);
   }
   {
     SloamKernelScope ks(ctx, "pn_bwd");
@@ -782,8 +793,9 @@ int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
 
 extern "C" {
 
-int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
-                   const int32_t* d_offsets, int32_t n_clusters, float* d_proposal) {
+static int pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                    const int32_t* d_offsets, int32_t n_clusters, const int32_t* d_n,
+                    float* d_proposal) {
   if (!ctx || n_clusters < 0) return SLOAM_ERR_INVALID_ARG;
   if (n_clusters == 0) return SLOAM_OK;
   if (!d_params || !d_xyz || !d_offsets || !d_proposal) return SLOAM_ERR_INVALID_ARG;
@@ -795,21 +807,33 @@ int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   {
     SloamKernelScope ks(ctx, "pn_prep");
     hipLaunchKernelGGL(pn_prep_kernel, dim3(n_clusters), dim3(kFB), 0, ctx->stream, d_xyz,
-                       d_offsets, n_clusters, b.X, b.meta);
+                       d_offsets, n_clusters, d_n, b.X, b.meta);
   }
   {
     SloamKernelScope ks(ctx, "pn_fwd");
     hipLaunchKernelGGL(pn_fwd_kernel<false>, dim3(fwd_grid(ctx, n_clusters)), dim3(kFW), 0,
-                       ctx->stream, d_params, b.X, b.meta, n_clusters, b.act,
+                       ctx->stream, d_params, b.X, b.meta, n_clusters, d_n, b.act,
                        (unsigned short*)nullptr, (unsigned short*)nullptr);
   }
   {
     SloamKernelScope ks(ctx, "pn_head");
     hipLaunchKernelGGL(pn_head_kernel, dim3((n_clusters + kHC - 1) / kHC), dim3(kFB), 0,
-                       ctx->stream, d_params, b.meta, n_clusters, b.act, d_proposal);
+                       ctx->stream, d_params, b.meta, n_clusters, d_n, b.act, d_proposal);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
+}
+
+int sloam_pn_infer(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                   const int32_t* d_offsets, int32_t n_clusters, float* d_proposal) {
+  return pn_infer(ctx, d_params, d_xyz, d_offsets, n_clusters, nullptr, d_proposal);
+}
+
+int sloam_pn_infer_counted(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
+                           const int32_t* d_offsets, int32_t cap_clusters,
+                           const int32_t* d_n_clusters, float* d_proposal) {
+  if (!d_n_clusters) return SLOAM_ERR_INVALID_ARG;
+  return pn_infer(ctx, d_params, d_xyz, d_offsets, cap_clusters, d_n_clusters, d_proposal);
 }
 
 int sloam_pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,

@@ -215,14 +215,16 @@ __device__ inline void two_sum(float a, float b, float* s, float* e) {
 // --------------------------------------------------------------------------
 // prep: LM state init + FP32 landmark tables relative to the anchor
 // --------------------------------------------------------------------------
+// d_n_problems (nullable): the problem count in device memory (a grid of
+// n_problems workgroups then serves any count up to n_problems)
 __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
-    int n_problems, const double* __restrict__ pose_in, const int32_t* __restrict__ free_mask,
-    const double* __restrict__ cyl,
+    int n_problems, const int32_t* __restrict__ d_n_problems, const double* __restrict__ pose_in,
+    const int32_t* __restrict__ free_mask, const double* __restrict__ cyl,
     const int32_t* __restrict__ cyl_off, const double* __restrict__ pl,
     const int32_t* __restrict__ pl_off, PoseCfgDev cfg, LMState* __restrict__ st,
     float4* __restrict__ cylf, float4* __restrict__ plf) {
   const int k = blockIdx.x;
-  if (k >= n_problems) return;
+  if (k >= n_problems || (d_n_problems && k >= *d_n_problems)) return;
   const double* x0 = pose_in + 7 * (size_t)k;
   const double ax = x0[0], ay = x0[1], az = x0[2];
   for (int i = cyl_off[k] + (int)threadIdx.x; i < cyl_off[k + 1]; i += kPBlock) {
@@ -317,11 +319,37 @@ __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e
   float acc[kSums];
 #pragma unroll
   for (int i = 0; i < kSums; ++i) acc[i] = 0.f;
+  // the thread's residual loads are all issued before any arithmetic, and
+  // each residual's landmark load one residual ahead of its use, so that a
+  // wave keeps several loads in flight instead of waiting on one dependent
+  // pair per residual
+  float4 pv[kRpt];
+#pragma unroll
   for (int j = 0; j < kRpt; ++j) {
     const int i = b + j * kPBlock + (int)threadIdx.x;
-    if (i >= e) break;
-    const float4 p = pts[i];
+    pv[j] = i < e ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const int n_mine = e > b + (int)threadIdx.x ? (e - b - (int)threadIdx.x + kPBlock - 1) / kPBlock
+                                               : 0;
+  auto gather = [&](int j, float4& A, float4& B) {
+    const int id = __float_as_int(pv[j].w);
+    if (id >= 0) {
+      A = cylf[2 * (size_t)id];
+      B = cylf[2 * (size_t)id + 1];
+    } else {
+      A = plf[-id - 1];
+      B = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nb = na;
+  if (n_mine > 0) gather(0, na, nb);
+#pragma unroll
+  for (int j = 0; j < kRpt; ++j) {
+    if (j >= n_mine) break;
+    const float4 p = pv[j];
     const int id = __float_as_int(p.w);
+    const float4 L0 = na, L1 = nb;
+    if (j + 1 < n_mine) gather(j + 1, na, nb);
     float rp[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -329,8 +357,6 @@ __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e
     const float lp[3] = {rp[0] + F.t[0], rp[1] + F.t[1], rp[2] + F.t[2]};
     float dir[3], r;
     if (id >= 0) {
-      const float4 L0 = cylf[2 * (size_t)id];
-      const float4 L1 = cylf[2 * (size_t)id + 1];
       const float v0 = lp[0] - L0.x, v1 = lp[1] - L0.y, v2 = lp[2] - L0.z;
       const float h = __builtin_fmaf(v0, L1.x, __builtin_fmaf(v1, L1.y, v2 * L1.z));
       const float w0 = __builtin_fmaf(-h, L1.x, v0);
@@ -343,7 +369,7 @@ __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e
       dir[1] = w1 * inv;
       dir[2] = w2 * inv;
     } else {
-      const float4 L = plf[-id - 1];
+      const float4 L = L0;
       const float s = __builtin_fmaf(L.x, lp[0], __builtin_fmaf(L.y, lp[1],
                                                                __builtin_fmaf(L.z, lp[2], L.w)));
       r = fabsf(s);
@@ -620,16 +646,16 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
 // fused: every iteration of one problem in one workgroup
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(kPBlock) void pose_fused_kernel(
-    int n_problems, const float4* __restrict__ pts, const int4* __restrict__ work,
-    const int32_t* __restrict__ first_work, const float4* __restrict__ cylf,
-    const float4* __restrict__ plf, PoseCfgDev cfg, float delta, LMState* __restrict__ st,
-    double* __restrict__ pose_out, int32_t* __restrict__ status, double* __restrict__ cost,
-    double* __restrict__ params_out) {
+    int n_problems, const int32_t* __restrict__ d_n_problems, const float4* __restrict__ pts,
+    const int4* __restrict__ work, const int32_t* __restrict__ first_work,
+    const float4* __restrict__ cylf, const float4* __restrict__ plf, PoseCfgDev cfg, float delta,
+    LMState* __restrict__ st, double* __restrict__ pose_out, int32_t* __restrict__ status,
+    double* __restrict__ cost, double* __restrict__ params_out) {
   __shared__ ChunkRed red;
   __shared__ double tot[kSums];
   __shared__ LMState sS;
   const int k = blockIdx.x;
-  if (k >= n_problems) return;
+  if (k >= n_problems || (d_n_problems && k >= *d_n_problems)) return;
   if (threadIdx.x == 0) sS = st[k];
   __syncthreads();
   const int w0 = first_work[k], w1 = first_work[k + 1];
@@ -757,6 +783,46 @@ bool valid_cfg(const sloam_pose_cfg* cfg) {
 
 }  // namespace
 
+static PoseCfgDev pose_cfg_dev(const sloam_pose_cfg& c) {
+  PoseCfgDev dc;
+  dc.max_iters = c.max_iters;
+  dc.stop_conv = c.stop_on_convergence;
+  dc.param = c.parametrization;
+  dc.delta = c.huber_delta;
+  dc.ftol = c.function_tolerance;
+  dc.gtol = c.gradient_tolerance;
+  dc.ptol = c.parameter_tolerance;
+  dc.radius0 = c.initial_radius;
+  dc.min_rel = c.min_relative_decrease;
+  return dc;
+}
+
+size_t sloam_pose_state_bytes() { return sizeof(LMState); }
+
+int sloam_pose_fused_device(sloam_ctx* ctx, int max_problems, const int32_t* d_n_problems,
+                            const float* d_points, const int32_t* d_work, const int32_t* d_first,
+                            const double* d_cyl, const int32_t* d_cyl_off, const double* d_planes,
+                            const int32_t* d_pl_off, const double* d_pose_in,
+                            const int32_t* d_free_mask, const sloam_pose_cfg* cfg, void* d_state,
+                            float* d_cylf, float* d_plf, double* d_pose_out,
+                            double* d_params_out, int32_t* d_status, double* d_cost) {
+  if (max_problems <= 0) return SLOAM_OK;
+  const PoseCfgDev dc = pose_cfg_dev(*cfg);
+  LMState* st = static_cast<LMState*>(d_state);
+  hipLaunchKernelGGL(pose_prep_kernel, dim3(max_problems), dim3(kPBlock), 0, ctx->stream,
+                     max_problems, d_n_problems, d_pose_in, d_free_mask, d_cyl, d_cyl_off,
+                     d_planes, d_pl_off, dc, st, reinterpret_cast<float4*>(d_cylf),
+                     reinterpret_cast<float4*>(d_plf));
+  SLOAM_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(pose_fused_kernel, dim3(max_problems), dim3(kPBlock), 0, ctx->stream,
+                     max_problems, d_n_problems, reinterpret_cast<const float4*>(d_points),
+                     reinterpret_cast<const int4*>(d_work), d_first,
+                     reinterpret_cast<const float4*>(d_cylf), reinterpret_cast<const float4*>(d_plf),
+                     dc, (float)cfg->huber_delta, st, d_pose_out, d_status, d_cost, d_params_out);
+  SLOAM_HIP(ctx, hipGetLastError());
+  return SLOAM_OK;
+}
+
 extern "C" {
 
 int sloam_pose_plan_create(sloam_ctx* ctx, const int32_t* h_res_offsets, int32_t n_problems,
@@ -843,14 +909,14 @@ int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double
   {
     SloamKernelScope ks(ctx, "pose_prep");
     hipLaunchKernelGGL(pose_prep_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
-                       p->n_problems, d_pose_in, d_free_mask, d_cyl, d_cyl_offsets, d_planes,
+                       p->n_problems, nullptr, d_pose_in, d_free_mask, d_cyl, d_cyl_offsets, d_planes,
                        d_plane_offsets, dc, p->d_state, p->d_cylf, p->d_plf);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (sloam_pose_plan_uses_fused(p)) {
     SloamKernelScope ks(ctx, "pose_fused");
     hipLaunchKernelGGL(pose_fused_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
-                       p->n_problems, reinterpret_cast<const float4*>(d_points), p->d_work,
+                       p->n_problems, nullptr, reinterpret_cast<const float4*>(d_points), p->d_work,
                        p->d_first, p->d_cylf, p->d_plf, dc, (float)c.huber_delta, p->d_state,
                        d_pose_out, d_status, d_cost, d_params_out);
     SLOAM_HIP(ctx, hipGetLastError());

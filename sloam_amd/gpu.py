@@ -155,7 +155,10 @@ class CylinderFitPlan:
 
     def close(self):
         if self.handle:
-            self.lib.sloam_cyl_plan_destroy(self.handle)
+            # a plan outliving its context is left to the process's end
+            # (its destroy would wait on the context's freed stream)
+            if self.ctx.handle:
+                self.lib.sloam_cyl_plan_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -284,7 +287,10 @@ class PosePlan:
 
     def close(self):
         if self.handle:
-            self.lib.sloam_pose_plan_destroy(self.handle)
+            # a plan outliving its context is left to the process's end
+            # (its destroy would wait on the context's freed stream)
+            if self.ctx.handle:
+                self.lib.sloam_pose_plan_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -421,7 +427,10 @@ class PlaneFitPlan:
 
     def close(self):
         if self.handle:
-            self.lib.sloam_plane_plan_destroy(self.handle)
+            # a plan outliving its context is left to the process's end
+            # (its destroy would wait on the context's freed stream)
+            if self.ctx.handle:
+                self.lib.sloam_plane_plan_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -581,7 +590,10 @@ class Frontend:
 
     def close(self):
         if self.handle:
-            self.lib.sloam_frontend_destroy(self.handle)
+            # a plan outliving its context is left to the process's end
+            # (its destroy would wait on the context's freed stream)
+            if self.ctx.handle:
+                self.lib.sloam_frontend_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <utility>
 
+#include "geometry.h"
 #include "gpu_bridge.h"
 
 sloam_line_fit_cfg Cylinder::lineFitCfg() {
@@ -19,27 +21,15 @@ sloam_line_fit_cfg Cylinder::lineFitCfg() {
   return c;
 }
 
-namespace {
-
-float squared_distance(const PointT& a, const PointT& b) {
-  const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-  return dx * dx + dy * dy + dz * dz;
-}
-
-}  // namespace
-
 bool Cylinder::withinRange(const std::vector<TreeVertex>& vertices, const Plane& gplane,
                            const FeatureModelParams& fmParams) {
-  PointT centroid;
-  centroid.x = (float)gplane.model.centroid(0);
-  centroid.y = (float)gplane.model.centroid(1);
-  centroid.z = (float)gplane.model.centroid(2);
-  return euclideanDist2D(vertices[1].coords, centroid) < fmParams.maxLidarDist;
+  return sloam_geom::cyl_within_range(vertices[1].coords, gplane.model.centroid,
+                                      fmParams.maxLidarDist);
 }
 
 bool Cylinder::needsLineFit(const std::vector<TreeVertex>& v) {
   if (v.size() < 3) return false;
-  return !(squared_distance(v[1].coords, v[v.size() - 2].coords) < 1.5f);
+  return sloam_geom::cyl_needs_line_fit((int)v.size(), v[1].coords, v[v.size() - 2].coords);
 }
 
 Cylinder::Cylinder(const std::vector<TreeVertex> vertices, const Plane& gplane,
@@ -64,6 +54,13 @@ Cylinder::Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
 Cylinder::Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
                    const FeatureModelParams& fmParams, const CylinderFit& fit) {
   build(vertices, gplane, fmParams, nullptr, &fit);
+}
+
+Cylinder::Cylinder(const CylinderParameters& m, size_t treeId, VectorType feats) {
+  model = m;
+  id = treeId;
+  features = std::move(feats);
+  isValid = true;
 }
 
 void Cylinder::build(const std::vector<TreeVertex>& vertices, const Plane& gplane,
@@ -92,22 +89,15 @@ PointT Cylinder::asPoint(const Scalar height) {
 }
 
 bool Cylinder::groundBasedRoot(const Plane& gplane) {
-  const Vector4& g = gplane.model.plane;
-  const float dist = (float)(std::fabs(g[0] * model.root(0) + g[1] * model.root(1) +
-                                       g[2] * model.root(2) + g[3]) /
-                             g.head3().norm());
-  if (dist < 2.0) {
-    model.root = rayPlaneIntersection(gplane.model.centroid, g.head3(), model.root, model.ray);
-    return true;
-  }
-  return false;
+  sloam_geom::CylModel m{model.root, model.ray, model.radius};
+  const bool near = sloam_geom::cyl_ground_based_root(gplane.model.plane, gplane.model.centroid, m);
+  model.root = m.root;
+  return near;
 }
 
 bool Cylinder::filter(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta, const Plane& gplane) {
-  if (model.radius == -1) return false;
-  const Vector3 up = gplane.model.plane.head3();
-  const Scalar theta = (180 / PIDEF) * std::acos(model.ray.dot(up) / (model.ray.norm() * up.norm()));
-  return (model.radius < maxTreeRadius) && ((theta <= maxAxisTheta) || (theta >= 180 - maxAxisTheta));
+  const sloam_geom::CylModel m{model.root, model.ray, model.radius};
+  return sloam_geom::cyl_filter(m, maxTreeRadius, maxAxisTheta, gplane.model.plane);
 }
 
 void Cylinder::computeModel(const std::vector<TreeVertex>& landmarkVtxs,
@@ -141,19 +131,22 @@ void Cylinder::computeModel(const std::vector<TreeVertex>& landmarkVtxs,
   }
   if (cfit) {
     // axis, root and radius of the cylinder fit; the radius floor of the
-    // medians' model
-    if (!cfit->ok || !(cfit->radius > 0.0)) {
-      model.radius = -1;
-      return;
+    // medians' model.  The root is the foot on the fitted axis of the bottom
+    // vertex, which the medians' model takes as its root (groundBasedRoot
+    // then keeps it: the axis points up, away from the ground)
+    sloam_geom::CylFitRecord f;
+    f.ok = cfit->ok;
+    for (int i = 0; i < 3; ++i) {
+      f.root[i] = cfit->root[i];
+      f.ray[i] = cfit->ray[i];
     }
-    const double sgn = cfit->ray[2] < 0.0 ? -1.0 : 1.0;  // axis pointing up
-    model.ray = Vector3(sgn * cfit->ray[0], sgn * cfit->ray[1], sgn * cfit->ray[2]);
-    // root: the foot on the fitted axis of the bottom vertex, which the
-    // medians' model takes as its root (groundBasedRoot then keeps it: the
-    // axis points up, away from the ground)
-    const Vector3 c(cfit->root[0], cfit->root[1], cfit->root[2]);
-    model.root = c + (model.root - c).dot(model.ray) * model.ray;
-    model.radius = cfit->radius < defaultTreeRadius ? defaultTreeRadius : cfit->radius;
+    f.radius = cfit->radius;
+    sloam_geom::CylModel m{model.root, model.ray, model.radius};
+    sloam_geom::cyl_model_from_fit(bottom, f, defaultTreeRadius, m);
+    model.root = m.root;
+    model.ray = m.ray;
+    model.radius = m.radius;
+    if (model.radius == -1) return;
     model.vertices = landmarkVtxs;
     model.lambda = 1.0;
     features.resize((size_t)featuresPerTree);
@@ -185,16 +178,7 @@ void Cylinder::computeModel(const std::vector<TreeVertex>& landmarkVtxs,
 }
 
 Scalar Cylinder::distance(const CylinderParameters& tgt) const {
-  const Scalar heights[3] = {0.0, 3.0, 6.0};
-  Scalar d = 0.0;
-  for (const Scalar h : heights) {
-    const Scalar st = (h - model.root[2]) / model.ray[2];
-    const Vector3 mp = model.root + st * model.ray;
-    const Scalar tt = (h - tgt.root[2]) / tgt.ray[2];
-    const Vector3 tp = tgt.root + tt * tgt.ray;
-    d += (mp - tp).norm();
-  }
-  return d / 3.0;
+  return sloam_geom::cyl_model_distance(model.root, model.ray, tgt.root, tgt.ray);
 }
 
 Scalar Cylinder::distance(const PointT& point) const {
@@ -207,10 +191,7 @@ CylinderParameters Cylinder::projectModel(const SE3& tf) const {
   CylinderParameters m;
   m.radius = model.radius;
   m.lambda = model.lambda;
-  Vector3 other = model.root + model.ray;
-  m.root = tf * model.root;
-  other = tf * other;
-  m.ray = other - m.root;
+  sloam_geom::cyl_project(tf, model.root, model.ray, m.root, m.ray);
   return m;
 }
 

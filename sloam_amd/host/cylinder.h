@@ -51,6 +51,10 @@ class Cylinder : public SemanticObject<CylinderParameters> {
   // and validity rules of the line-fit model otherwise.
   Cylinder(const std::vector<TreeVertex>& vertices, const Plane& gplane,
            const FeatureModelParams& fmParams, const CylinderFit& fit);
+  // A valid landmark with a model computed elsewhere (the GPU keyframe
+  // estimator): root, ray, radius, id and features as given, no vertices.
+  // No counterpart in the reference.
+  Cylinder(const CylinderParameters& model, size_t id, VectorType features);
   Scalar distance(const CylinderParameters& model) const override;
   Scalar distance(const PointT& point) const override;
   void project(const SE3& tf) override;

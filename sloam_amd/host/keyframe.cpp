@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -100,6 +101,8 @@ uint8_t* KeyframeRunner::pinnedLabels() {
 }
 
 void KeyframeRunner::setTreeModel(TreeModel mode, const std::string& weights_npy) {
+  if (device_est_)
+    throw std::runtime_error("keyframe: set the tree model before switching to the device keyframe");
   if (mode == TreeModel::Learned) {
     if (weights_npy.empty()) throw std::runtime_error("keyframe: the learned model needs weights");
     const std::vector<float> w = load_npy_f32(weights_npy, SLOAM_PN_PARAMS);
@@ -201,6 +204,229 @@ void KeyframeRunner::fitTreesLaunch(const Slot& sl, int nt) {
         "download models");
 }
 
+void KeyframeRunner::setDeviceEstimator(bool on) {
+  if (on && mode_ == TreeModel::MedianLines)
+    throw std::runtime_error("keyframe: the device keyframe serves the cylinder-fit tree models");
+  if (on && slots_.size() != 1)
+    throw std::runtime_error("keyframe: the device keyframe runs one scan at a time");
+  sloam_ctx* ctx = sloam_gpu::context();
+  if (on && !kf_est_) {
+    const sloam_frontend_out& fo = slots_[0].buf;
+    kf_cap_trees_ = std::min(fo.cap_trees, 512);
+    kf_cap_points_ = fo.cap_points;
+    kf_cap_map_ = 256;
+    sloam_kf_params p{};
+    p.max_lidar_dist = fm_.maxLidarDist;
+    p.default_tree_radius = fm_.defaultTreeRadius;
+    p.max_tree_radius = fm_.maxTreeRadius;
+    p.max_axis_theta = fm_.maxAxisTheta;
+    p.tree_match_thresh = fm_.treeMatchThresh;
+    p.add_new_tree_thresh_dist = fm_.AddNewTreeThreshDist;
+    p.min_tree_models = fm_.minTreeModels;
+    p.min_ground_models = fm_.minGroundModels;
+    p.min_planes = (fm_.groundRadiiBins * fm_.groundThetaBins) * 0.1;  // sloam.h's minPlanes_
+    p.features_per_tree = fm_.featuresPerTree;
+    p.num_ground_features = fm_.numGroundFeatures;
+    p.two_step = fm_.twoStepOptim ? 1 : 0;
+    check(sloam_kf_estimator_create(ctx, &p, kf_cap_trees_, std::max(fo.n_cells, 1), kf_cap_map_,
+                                    &kf_est_),
+          "sloam_kf_estimator_create");
+    const size_t in_bytes = sizeof(double) * (8 + 8 * (size_t)kf_cap_map_);
+    d_kf_in_.ensure(in_bytes);
+    hip_check(hipHostMalloc(&h_kf_in_, in_bytes), "hipHostMalloc");
+    std::memset(h_kf_in_, 0, in_bytes);
+    d_kt_off_.ensure(sizeof(int32_t) * ((size_t)kf_cap_trees_ + 1));
+    d_kt_xyz_.ensure(sizeof(float) * 3 * (size_t)kf_cap_points_);
+    d_kt_n_.ensure(sizeof(int32_t) * 4);
+    d_kt_prop_.ensure(sizeof(float) * SLOAM_PN_PROPOSAL_STRIDE * (size_t)kf_cap_trees_);
+    d_kt_flags_.ensure(sizeof(int32_t) * ((size_t)kf_cap_trees_ + 1));
+    d_kt_model_.ensure(sizeof(double) * SLOAM_CYL_MODEL_STRIDE * (size_t)kf_cap_trees_);
+  }
+  if (on && (!plain_c_ || (mode_ == TreeModel::Learned && !verify_c_))) {
+    // the settings of fitTreesLaunch's plans, on counted plans
+    sloam_cyl_fit_cfg cfg{};
+    cfg.n_hyp = 1024;
+    cfg.thr = 0.05f;
+    cfg.max_radius = 1.0f;
+    cfg.refit_iters = 2;
+    cfg.refit_rounds = 3;
+    cfg.seed = 0x5EED;
+    cfg.stream_tag = 0;
+    cfg.hyp_chunk = 128;
+    cfg.tile_points = 1024;
+    const int32_t* d_n = static_cast<const int32_t*>(d_kt_n_.p);
+    if (!plain_c_)
+      check(sloam_cyl_plan_create_counted(ctx, kf_cap_trees_, kf_cap_points_, d_n, &cfg, &plain_c_),
+            "sloam_cyl_plan_create_counted");
+    if (mode_ == TreeModel::Learned && !verify_c_) {
+      sloam_cyl_fit_cfg vc = cfg;
+      vc.n_hyp = 16;
+      vc.stream_tag = 1;
+      vc.hyp_chunk = 16;
+      check(sloam_cyl_plan_create_counted(ctx, kf_cap_trees_, kf_cap_points_, d_n, &vc, &verify_c_),
+            "sloam_cyl_plan_create_counted");
+    }
+  }
+  if (on && !device_est_) check(sloam_kf_estimator_reset(kf_est_), "sloam_kf_estimator_reset");
+  // a captured keyframe holds the tree model it was captured with
+  if (kf_graph_) {
+    sloam_graph_destroy(kf_graph_);
+    kf_graph_ = nullptr;
+  }
+  device_est_ = on;
+}
+
+void KeyframeRunner::enqueueDevice(Slot& sl) {
+  sloam_ctx* ctx = sloam_gpu::context();
+  const size_t N = (size_t)H_ * W_;
+  check(sloam_memcpy_async(ctx, sl.buf.xyzi, pinnedXyzi(), 16 * N), "upload xyzi");
+  check(sloam_memcpy_async(ctx, sl.buf.label, pinnedLabels(), N), "upload labels");
+  check(sloam_memcpy_async(ctx, d_kf_in_.p, h_kf_in_, sizeof(double) * (8 + 8 * (size_t)kf_cap_map_)),
+        "upload pose and map");
+  check(sloam_frontend_enqueue(sl.fe), "sloam_frontend_enqueue");
+  int32_t* off = static_cast<int32_t*>(d_kt_off_.p);
+  float* xyz = static_cast<float*>(d_kt_xyz_.p);
+  int32_t* d_n = static_cast<int32_t*>(d_kt_n_.p);
+  double* model = static_cast<double*>(d_kt_model_.p);
+  check(sloam_kf_tree_points(ctx, &sl.buf, kf_cap_trees_, kf_cap_points_, off, xyz, d_n),
+        "sloam_kf_tree_points");
+  if (mode_ == TreeModel::Learned) {
+    float* prop = static_cast<float*>(d_kt_prop_.p);
+    int32_t* flags = static_cast<int32_t*>(d_kt_flags_.p);
+    check(sloam_pn_infer_counted(ctx, static_cast<const float*>(d_params_.p), xyz, off,
+                                 kf_cap_trees_, d_n, prop),
+          "sloam_pn_infer_counted");
+    check(sloam_cyl_learned_exec(verify_c_, plain_c_, xyz, off, prop, 0.5f, nullptr, nullptr, flags,
+                                 flags + kf_cap_trees_, nullptr, model),
+          "sloam_cyl_learned_exec");
+  } else {
+    check(sloam_cyl_plan_exec(plain_c_, xyz, off, nullptr, nullptr, nullptr, nullptr, model),
+          "sloam_cyl_plan_exec");
+  }
+  const double* d = static_cast<const double*>(d_kf_in_.p);
+  sloam_kf_inputs in{};
+  in.trellis_counts = sl.buf.trellis_counts;
+  in.tree_offsets = sl.buf.tree_offsets;
+  in.vertex_int = sl.buf.vertex_int;
+  in.vertex_f64 = sl.buf.vertex_f64;
+  in.point_offsets = sl.buf.point_offsets;
+  in.points = sl.buf.points;
+  in.n_cells = sl.buf.n_cells;
+  in.num_features = fm_.numGroundFeatures;
+  in.cell_info = sl.buf.cell_info;
+  in.cell_model = sl.buf.cell_model;
+  in.cell_features = sl.buf.cell_features;
+  in.tree_fits = model;
+  in.pose = d;
+  in.n_map = reinterpret_cast<const int32_t*>(d + 7);
+  in.map = d + 8;
+  check(sloam_kf_estimator_exec(kf_est_, &in), "sloam_kf_estimator_exec");
+}
+
+bool KeyframeRunner::runDevice(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
+                               const MapView& mapModels, SloamOutput& out,
+                               KeyframeTiming* timing) {
+  sloam_ctx* ctx = sloam_gpu::context();
+  Slot& sl = slots_[0];
+  const size_t N = (size_t)H_ * W_;
+  const auto t0 = Clock::now();
+  // the scan, the pose (t, q x y z w), the map count (first word of slot 7)
+  // and the map trees, staged in pinned memory the sequence uploads from
+  if (xyzi != pinnedXyzi()) std::memcpy(pinnedXyzi(), xyzi, 16 * N);
+  if (labels != pinnedLabels()) std::memcpy(pinnedLabels(), labels, N);
+  const int K = (int)mapModels.size();
+  if (K > kf_cap_map_) throw std::runtime_error("keyframe: more map trees than the estimator holds");
+  double* h = static_cast<double*>(h_kf_in_);
+  {
+    const Vector3& t = poseEstimate.translation();
+    const Quat& q = poseEstimate.unit_quaternion();
+    const double x[7] = {t[0], t[1], t[2], q.x, q.y, q.z, q.w};
+    std::memcpy(h, x, sizeof(x));
+    const int32_t k32 = K;
+    h[7] = 0.0;
+    std::memcpy(h + 7, &k32, sizeof(k32));
+    for (int i = 0; i < K; ++i) {
+      const CylinderParameters& m = mapModels[i]->model;
+      double* row = h + 8 + 8 * (size_t)i;
+      for (int c = 0; c < 3; ++c) {
+        row[c] = m.root[c];
+        row[3 + c] = m.ray[c];
+      }
+      row[6] = m.radius;
+      row[7] = 0.0;
+    }
+  }
+  const auto t1 = Clock::now();
+  if (kf_graph_) {
+    check(sloam_graph_launch(kf_graph_), "sloam_graph_launch");
+  } else {
+    // the first keyframe runs eagerly (sizing every scratch buffer), then
+    // the sequence is captured for the next ones
+    enqueueDevice(sl);
+    if (fe_cfg_.use_graph) {
+      check(sloam_sync(ctx), "sloam_sync");
+      check(sloam_graph_begin(ctx), "sloam_graph_begin");
+      enqueueDevice(sl);
+      sloam_graph* g = nullptr;
+      check(sloam_graph_end(ctx, &g), "sloam_graph_end");
+      kf_graph_ = g;
+      // the captured copy of the sequence has not run: the eager one has
+    }
+  }
+  check(sloam_sync(ctx), "sloam_sync");
+  const auto t2 = Clock::now();
+  sloam_kf_result_view r{};
+  check(sloam_kf_estimator_result(kf_est_, &r), "sloam_kf_estimator_result");
+  const int32_t* hd = r.head;
+  // header words: see sloam_kf_result_view
+  if (hd[10]) throw std::runtime_error("keyframe: a device keyframe capacity was exceeded");
+  dev_step_ = SloamStepInfo{hd[3] != 0, hd[4] != 0, hd[5], hd[6]};
+  bool ok = false;
+  if (hd[17]) {
+    out.T_Map_Curr = SE3::fromUnitQuaternion(Quat(r.pose[6], r.pose[3], r.pose[4], r.pose[5]),
+                                             Vector3(r.pose[0], r.pose[1], r.pose[2]));
+    out.T_Delta = SE3::fromUnitQuaternion(Quat(r.delta[6], r.delta[3], r.delta[4], r.delta[5]),
+                                          Vector3(r.delta[0], r.delta[1], r.delta[2]));
+    const int n_lm = hd[2];
+    out.matches.assign(r.lm_match, r.lm_match + n_lm);
+    out.tm.clear();
+    out.tm.reserve(n_lm);
+    const int32_t *to = sl.host.tree_offsets, *po = sl.host.point_offsets;
+    const float* pts = sl.host.points;
+    const int fpt = fm_.featuresPerTree;
+    for (int l = 0; l < n_lm; ++l) {
+      CylinderParameters m;
+      const double* q = r.lm_model + 8 * (size_t)l;
+      m.root = Vector3(q[0], q[1], q[2]);
+      m.ray = Vector3(q[3], q[4], q[5]);
+      m.radius = q[6];
+      m.lambda = 1.0;
+      // the tree's first featuresPerTree points (zero points past its last),
+      // intensity = tree id (cylinder.cpp:86-96, 172)
+      const int t = r.lm_tree[l];
+      const int pb = po[to[t]], pe = po[to[t + 1]];
+      VectorType f((size_t)fpt);
+      for (int j = 0; j < fpt && pb + j < pe; ++j) {
+        const float* p = pts + 4 * (size_t)(pb + j);
+        f[j] = PointT{p[0], p[1], p[2], (float)r.lm_id[l]};
+      }
+      out.tm.emplace_back(m, (size_t)r.lm_id[l], std::move(f));
+    }
+    ok = hd[16] != 0;
+  }
+  if (timing) {
+    *timing = KeyframeTiming();
+    timing->upload_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    timing->frontend_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    timing->estimator_ms = ms_since(t2);
+    timing->total_ms = ms_since(t0);
+    timing->trees = (int)out.tm.size();
+    timing->planes = hd[1];
+    timing->fitted = hd[11];
+  }
+  return ok;
+}
+
 void KeyframeRunner::fitTreesCollect(int nt, std::vector<CylinderFit>& fits, double launch_ms,
                                      KeyframeTiming* timing) {
   const auto t0 = Clock::now();
@@ -292,6 +518,11 @@ void KeyframeRunner::makeFrontend() {
 }
 
 KeyframeRunner::~KeyframeRunner() {
+  sloam_graph_destroy(kf_graph_);
+  sloam_cyl_plan_destroy(plain_c_);
+  sloam_cyl_plan_destroy(verify_c_);
+  sloam_kf_estimator_destroy(kf_est_);
+  if (h_kf_in_) (void)hipHostFree(h_kf_in_);
   sloam_cyl_plan_destroy(plain_);
   sloam_cyl_plan_destroy(verify_);
   if (h_model_) (void)hipHostFree(h_model_);
@@ -329,6 +560,7 @@ bool KeyframeRunner::finish(const SE3& poseEstimate, const std::vector<Cylinder>
 bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
                          const MapView& mapModels, SloamOutput& out, KeyframeTiming* timing) {
   if (pending_) throw std::runtime_error("keyframe: run() with submitted scans pending");
+  if (device_est_) return runDevice(xyzi, labels, poseEstimate, mapModels, out, timing);
   Slot& sl = slots_[next_];
   next_ = (next_ + 1) % (int)slots_.size();
   const size_t N = (size_t)H_ * W_;
@@ -343,6 +575,7 @@ bool KeyframeRunner::run(const float* xyzi, const uint8_t* labels, const SE3& po
 }
 
 void KeyframeRunner::submit(const float* xyzi, const uint8_t* labels) {
+  if (device_est_) throw std::runtime_error("keyframe: the device keyframe runs through run()");
   if (pending_ == (int)slots_.size())
     throw std::runtime_error("keyframe: every front-end slot is busy (finish() first)");
   Slot& sl = slots_[next_];

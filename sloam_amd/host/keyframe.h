@@ -83,6 +83,24 @@ class KeyframeRunner {
 
   sloam::sloam& estimator() { return sloam_; }
 
+  // Runs the whole keyframe on the GPU, for the cylinder-fit tree models
+  // (Plain, Learned): the scan's upload, the front end, the tree fits on
+  // counted plans (sizes read on the device), the estimator
+  // (sloam_kf_estimator instead of the host sloam::sloam: the same
+  // decisions, pose and landmarks) and the results into pinned memory, with
+  // no host step in between; with use_graph the sequence is captured once
+  // and every later keyframe is one HIP graph launch.  The returned
+  // landmarks carry their model, id and features (no vertices).  The device
+  // keeps its own first-scan flag and previous planes: switching on starts
+  // from a first scan, and estimator() is not used meanwhile.  run() only
+  // (not submit / finish); at most 512 trees per scan.
+  void setDeviceEstimator(bool on);
+  bool deviceEstimator() const { return device_est_; }
+  // what the last keyframe's estimator decided (either estimator)
+  SloamStepInfo lastStep() const {
+    return device_est_ ? dev_step_ : sloam_.lastStep();
+  }
+
   // Called with each keyframe's estimator input right before RunSloam (its
   // landmarks, ground cells and tree fits as the front end made them, and
   // the map trees): for recording keyframes, e.g. to replay them elsewhere.
@@ -134,6 +152,12 @@ class KeyframeRunner {
   void makeFrontend();
   // the estimator on slot s's front-end results; t0..t2: submit, upload done,
   // front end done
+  // the device keyframe (setDeviceEstimator): stages the scan, the pose and
+  // the map trees in pinned memory, enqueues (or replays) the sequence,
+  // unpacks the results into out
+  bool runDevice(const float* xyzi, const uint8_t* labels, const SE3& poseEstimate,
+                 const MapView& mapModels, SloamOutput& out, KeyframeTiming* timing);
+  void enqueueDevice(Slot& s);
   bool estimate(const Slot& s, const SE3& poseEstimate, const MapView& mapModels,
                 SloamOutput& out, KeyframeTiming* timing, std::chrono::steady_clock::time_point t0,
                 std::chrono::steady_clock::time_point t1,
@@ -155,4 +179,19 @@ class KeyframeRunner {
   DevBuf d_params_, d_xyz_, d_off_, d_model_, d_prop_, d_flags_;
   double* h_model_ = nullptr;   // pinned: models of the last fit + fallback count
   size_t h_model_cap_ = 0;      // trees
+  // device estimator: pose, map count and map trees staged in pinned memory
+  // and uploaded in one copy
+  bool device_est_ = false;
+  sloam_kf_estimator* kf_est_ = nullptr;
+  int kf_cap_trees_ = 0, kf_cap_map_ = 0;
+  int64_t kf_cap_points_ = 0;
+  DevBuf d_kf_in_;
+  void* h_kf_in_ = nullptr;
+  SloamStepInfo dev_step_;
+  // the device keyframe's tree batch (offsets, xyz, count), counted fit
+  // plans, proposals, fallback flags and models, and its captured graph
+  sloam_cyl_plan* plain_c_ = nullptr;
+  sloam_cyl_plan* verify_c_ = nullptr;
+  DevBuf d_kt_off_, d_kt_xyz_, d_kt_n_, d_kt_prop_, d_kt_flags_, d_kt_model_;
+  sloam_graph* kf_graph_ = nullptr;
 };

@@ -5,6 +5,7 @@
 #include <limits>
 #include <utility>
 
+#include "geometry.h"
 #include "gpu_bridge.h"
 
 Plane::Plane(const VectorType& points, const FeatureModelParams& fmParams) {
@@ -38,13 +39,11 @@ Plane::Plane(const PlaneParameters& fitted, VectorType feats, bool valid) {
 }
 
 Scalar Plane::distance(const PlaneParameters& tgt) const {
-  return (model.centroid - tgt.centroid).norm();
+  return sloam_geom::plane_model_distance(model.centroid, tgt.centroid);
 }
 
 Scalar Plane::distance(const PointT& p) const {
-  return std::fabs(model.plane[0] * p.x + model.plane[1] * p.y + model.plane[2] * p.z +
-                   model.plane[3]) /
-         model.plane.head3().norm();
+  return sloam_geom::plane_point_distance(model.plane, p);
 }
 
 void Plane::project(const SE3& tf) {
@@ -62,7 +61,6 @@ void Plane::project(const SE3& tf) {
 
 PlaneParameters Plane::projectModel(const SE3& tf) const {
   PlaneParameters m = model;
-  m.plane = tf.matrix().inverse().transpose() * model.plane;
-  m.centroid = tf * model.centroid;
+  sloam_geom::plane_project(tf, model.plane, model.centroid, m.plane, m.centroid);
   return m;
 }

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <utility>
 
+#include "geometry.h"
 #include "gpu_bridge.h"
 
 namespace sloam {
@@ -390,24 +391,14 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
     }
   }
   const std::vector<Plane>& cells = in.groundCells.empty() ? fittedCells : in.groundCells;
-  const Vector3 src_vec(0, 0, 1);
   const Matrix4 tfm = in.poseEstimate.inverse().matrix().transpose();
   for (const Plane& ground : cells) {
     if (!ground.isValid) continue;
-    // roughly horizontal in the map frame (sloam.cpp:401-406)
-    const Vector4 normal = tfm * ground.model.plane;
-    const Quat q_rot = Quat::FromTwoVectors(src_vec, normal.head3());
-    double R[3][3];
-    q_rot.toRotationMatrix(R);
-    const Vector3 angles = eulerAngles012(R);
-    const bool angleCheck =
-        (angles[0] < 0.1 && angles[1] < 0.1 && angles[2] < 0.1) ||
-        (M_PI - std::fabs(angles[0]) < 0.1 && M_PI - std::fabs(angles[1]) < 0.1 &&
-         M_PI - std::fabs(angles[2]) < 0.1);
-    // under the robot (sloam.cpp:408)
-    const bool heightCheck =
-        (in.poseEstimate * ground.model.centroid)[2] < in.poseEstimate.translation()[2];
-    if (angleCheck && heightCheck) planes.push_back(ground);
+    // roughly horizontal in the map frame and under the robot
+    // (sloam.cpp:401-408)
+    if (sloam_geom::plane_accepted(in.poseEstimate, tfm, ground.model.plane,
+                                   ground.model.centroid))
+      planes.push_back(ground);
   }
   if (planes.empty()) return;
 

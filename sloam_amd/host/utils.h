@@ -7,10 +7,10 @@
 #include "definitions.h"
 
 // squares in double, result rounded to float (utils.h:9)
-inline float pow_2(const Scalar& x) { return (float)(x * x); }
+SLOAM_HD inline float pow_2(const Scalar& x) { return (float)(x * x); }
 
-inline float euclideanDist2D(const PointT& a, const PointT& b) {
-  return std::sqrt(pow_2(a.x - b.x) + pow_2(a.y - b.y));
+SLOAM_HD inline float euclideanDist2D(const PointT& a, const PointT& b) {
+  return sqrtf(pow_2(a.x - b.x) + pow_2(a.y - b.y));
 }
 
 // float running sums, divided by the count in double (utils.h:14-28)
@@ -29,10 +29,11 @@ inline PointT computeCentroid(const VectorType& features) {
 
 // intersection of the ray root + t ray with the plane (utils.h:41-52): the
 // denominator and t are floats, t must be >= 0.001
-inline Vector3 rayPlaneIntersection(const Vector3& planeCentroid, const Vector3& planeNormal,
-                                    const Vector3& rayOrigin, const Vector3& rayDirection) {
+SLOAM_HD inline Vector3 rayPlaneIntersection(const Vector3& planeCentroid,
+                                             const Vector3& planeNormal, const Vector3& rayOrigin,
+                                             const Vector3& rayDirection) {
   const float denom = (float)planeNormal.dot(rayDirection);
-  if (std::fabs(denom) > 0.001f) {
+  if (fabsf(denom) > 0.001f) {
     const float t = (float)((planeCentroid - rayOrigin).dot(planeNormal) / (double)denom);
     if (t >= 0.001f) return rayOrigin + (double)t * rayDirection;
   }

@@ -50,7 +50,7 @@ def make_chunk(ctx, seed, n, batch, outlier_frac=0.2):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--chunks", type=int, default=60)
+    ap.add_argument("--chunks", type=int, default=100)
     ap.add_argument("--chunk-size", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=250)
     ap.add_argument("--epochs", type=int, default=6)

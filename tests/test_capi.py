@@ -59,6 +59,9 @@ STRUCTS = {
     "sloam_frontend_out": "FrontendOut",
     "sloam_frontend_host": "FrontendHost",
     "sloam_pn_train_cfg": "PnTrainCfg",
+    "sloam_kf_params": "KfParams",
+    "sloam_kf_inputs": "KfInputs",
+    "sloam_kf_result_view": "KfResultView",
 }
 
 

@@ -1,9 +1,13 @@
 """GPU parity of batched cylinder fitting (C ABI sloam_cyl_plan_*) against the
 CPU oracle on the same seeded inputs.
 
-Bar: sampled indices, per-hypothesis inlier counts, best hypothesis and inlier
-masks bit-exact; refitted model (FP64 on both sides, different summation
-order) within 1e-6 m / rad -- the stated requirement is 1e-4."""
+Bar: sampled indices, per-hypothesis inlier counts, best hypothesis, inlier
+masks and the refitted models bit-exact against the oracle's restatement of
+the GPU arithmetic (FP32 residuals and Jacobians, TwoSum sums in the GPU's
+fixed order, FP64 5x5 solve).  Against the oracle's all-FP64 refit the models
+agree within 1e-4 m / rad wherever both keep the same final inliers (a point
+within FP32 rounding of the 5 cm band may change sides); every error is
+printed."""
 import numpy as np
 import pytest
 import torch
@@ -14,7 +18,7 @@ from sloam_amd.gpu import CylinderFitPlan, fit_cylinders
 
 pytestmark = pytest.mark.gpu
 
-MODEL_TOL = 1e-6
+FP64_TOL = 1e-4
 
 
 def _compare(g, o, n_points):
@@ -26,11 +30,23 @@ def _compare(g, o, n_points):
     if "mask" in g:
         assert np.array_equal(g["mask"], o["mask"])
     gm, om = g["model"], o["model"]
-    valid = om[:, 6] > 0
-    assert np.array_equal(gm[:, 6] > 0, valid)
-    assert np.allclose(gm[valid, :7], om[valid, :7], atol=MODEL_TOL, rtol=0)
-    # final inlier counts may differ only by points within rounding of thr
-    assert np.all(np.abs(gm[:, 7] - om[:, 7]) <= 1)
+    bad = np.flatnonzero(np.any(gm.view(np.uint64) != om.view(np.uint64), axis=1))
+    assert bad.size == 0, (bad[:8], gm[bad[:2]], om[bad[:2]])
+
+
+def _vs_fp64(g, d, n_hyp, **kw):
+    """Error of the GPU's models against the oracle's FP64 refit."""
+    o = oracle.cyl_fit_batch(d["xyz"], d["offsets"], n_hyp, threads=16, fp64_refit=True,
+                             counts=False, mask=False, **kw)
+    gm, om = g["model"], o["model"]
+    both = (gm[:, 6] > 0) & (om[:, 6] > 0)
+    same = both & (gm[:, 7] == om[:, 7])
+    err = np.abs(gm[same, :7] - om[same, :7]).max(axis=1) if same.any() else np.zeros(1)
+    print(f"[cylfit] vs FP64 refit: {int(same.sum())} of {int(both.sum())} valid fits keep the "
+          f"same inliers; largest difference {err.max():.2e}, median {np.median(err):.2e}; "
+          f"validity differs on {int(np.sum((gm[:, 6] > 0) != (om[:, 6] > 0)))}", flush=True)
+    assert err.max() <= FP64_TOL
+    assert same.sum() >= 0.99 * both.sum()
 
 
 def _run_both(d, n_hyp, **kw):
@@ -47,6 +63,7 @@ def test_small_batch_bit_exact(ctx, n_hyp):
                       mask=True, best=True)
     o = oracle.cyl_fit_batch(d["xyz"], d["offsets"], n_hyp, samples=True, threads=8)
     _compare(g, o, d["offsets"][-1])
+    _vs_fp64(g, d, n_hyp)
 
 
 def test_edge_clusters(ctx):
@@ -85,6 +102,7 @@ def test_full_batch_2000_clusters(ctx):
     g = fit_cylinders(ctx, d["xyz"], d["offsets"], n_hyp=1024, counts=True, mask=True, best=True)
     o = oracle.cyl_fit_batch(d["xyz"], d["offsets"], 1024, threads=16)
     _compare(g, o, d["offsets"][-1])
+    _vs_fp64(g, d, 1024)
     # size-independent sanity: every count <= cluster size, mask popcount per
     # cluster == count of the best hypothesis
     sizes = np.diff(d["offsets"])
@@ -207,3 +225,125 @@ def test_autotune_choice_is_valid_and_cached(ctx):
         p.close()
     for k in outs[0]:
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def _read_counts(ctx, plan_handle):
+    """The 4 device counts of a counted plan: clusters, work items, refit
+    chunks, overflow."""
+    import ctypes as C
+    p = C.c_void_p()
+    assert ctx.lib.sloam_cyl_plan_counts(plan_handle, C.byref(p)) == 0
+    host = np.zeros(4, np.int32)
+    assert ctx.lib.sloam_memcpy_async(ctx.handle, host.ctypes.data_as(C.c_void_p), p, 16) == 0
+    ctx.sync()
+    return host
+
+
+def _counted(ctx, cap, cap_points, d_n, n_hyp, tag=0, hyp_chunk=0):
+    import ctypes as C
+    from sloam_amd._lib import CylFitCfg, check
+    from sloam_amd.gpu import _ptr
+    cfg = CylFitCfg(n_hyp=n_hyp, thr=0.05, max_radius=1.0, refit_iters=2, seed=0x5EED,
+                    stream_tag=tag, hyp_chunk=hyp_chunk, tile_points=0, refit_rounds=3)
+    h = C.c_void_p()
+    check(ctx.lib.sloam_cyl_plan_create_counted(ctx.handle, cap, cap_points, _ptr(d_n),
+                                                C.byref(cfg), C.byref(h)), ctx.handle)
+    return h
+
+
+def test_counted_plan_matches_host_plan(ctx):
+    """A counted plan (batch size read on the device, layout made on the
+    device) gives a host-planned plan's counts, best hypotheses and models
+    bit for bit, on two batches in a row (the second smaller); a batch past
+    its capacity sets the overflow word and fits nothing."""
+    import ctypes as C
+    from sloam_amd._lib import check
+    from sloam_amd.gpu import _ptr
+    dev = torch.device("cuda", 0)
+    cap, cap_points, n_hyp = 80, 300000, 512
+    d_n = torch.zeros(1, dtype=torch.int32, device=dev)
+    plan = _counted(ctx, cap, cap_points, d_n, n_hyp)
+    try:
+        for seed, n in ((41, 60), (42, 25)):
+            d = synth.make_clusters(n, seed=seed)
+            print(f"[cylfit] counted plan: {n} clusters, {d['offsets'][-1]} points", flush=True)
+            x = torch.as_tensor(d["xyz"]).to(dev)
+            off = torch.as_tensor(d["offsets"]).to(dev)
+            d_n.fill_(n)
+            counts = torch.full((cap, n_hyp), -1, dtype=torch.int32, device=dev)
+            best = torch.full((cap,), -9, dtype=torch.int32, device=dev)
+            model = torch.zeros((cap, 8), dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()
+            check(ctx.lib.sloam_cyl_plan_exec(plan, _ptr(x), _ptr(off), _ptr(counts), None, None,
+                                              _ptr(best), _ptr(model)), ctx.handle)
+            ctx.sync()
+            ref = fit_cylinders(ctx, d["xyz"], d["offsets"], n_hyp=n_hyp, counts=True,
+                                mask=False, best=True)
+            assert np.array_equal(counts.cpu().numpy()[:n], ref["counts"])
+            assert np.array_equal(best.cpu().numpy()[:n], ref["best"])
+            assert np.array_equal(model.cpu().numpy()[:n].view(np.uint64),
+                                  ref["model"].view(np.uint64))
+            k = _read_counts(ctx, plan)
+            assert k[0] == n and k[1] > 0 and k[2] >= n and k[3] == 0
+        # past the capacity: overflow, nothing fitted
+        d = synth.make_clusters(cap + 5, seed=43)
+        x = torch.as_tensor(d["xyz"]).to(dev)
+        off = torch.as_tensor(d["offsets"]).to(dev)
+        d_n.fill_(cap + 5)
+        model = torch.full((cap, 8), 7.0, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        check(ctx.lib.sloam_cyl_plan_exec(plan, _ptr(x), _ptr(off), None, None, None, None,
+                                          _ptr(model)), ctx.handle)
+        ctx.sync()
+        k = _read_counts(ctx, plan)
+        print(f"[cylfit] counted plan past its capacity: counts {k.tolist()}", flush=True)
+        assert k[3] == 1 and k[0] == 0 and k[2] == 0
+        assert np.all(model.cpu().numpy() == 7.0)
+    finally:
+        ctx.lib.sloam_cyl_plan_destroy(plan)
+
+
+def test_counted_learned_fit_matches_host_plans(ctx):
+    """The learned fit on counted plans with the counted proposal inference
+    (sloam_pn_infer_counted) equals the host-planned learned fit."""
+    import ctypes as C
+    from sloam_amd import proposal
+    from sloam_amd._lib import check
+    from sloam_amd.gpu import _ptr
+    dev = torch.device("cuda", 0)
+    d = synth.make_clusters(48, seed=44)
+    n = len(d["offsets"]) - 1
+    cap, cap_points = 64, 300000
+    x = torch.as_tensor(d["xyz"]).to(dev)
+    off = torch.as_tensor(d["offsets"]).to(dev)
+    net = proposal.ProposalNet(ctx, proposal.load_weights(proposal.DEFAULT_WEIGHTS))
+    lf = proposal.LearnedCylinderFit(ctx, net, d["offsets"])
+    ref_out = lf.alloc()
+    torch.cuda.synchronize()
+    lf.exec(x, off, ref_out)
+    ctx.sync()
+    ref = ref_out["model"].cpu().numpy()
+    d_n = torch.full((1,), n, dtype=torch.int32, device=dev)
+    verify = _counted(ctx, cap, cap_points, d_n, 16, tag=1, hyp_chunk=0)
+    plain = _counted(ctx, cap, cap_points, d_n, 1024, tag=0)
+    try:
+        prop = torch.zeros((cap, 8), dtype=torch.float32, device=dev)
+        flags = torch.zeros((cap,), dtype=torch.int32, device=dev)
+        n_fb = torch.zeros((1,), dtype=torch.int32, device=dev)
+        model = torch.zeros((cap, 8), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        check(ctx.lib.sloam_pn_infer_counted(ctx.handle, _ptr(net.params), _ptr(x), _ptr(off),
+                                             cap, _ptr(d_n), _ptr(prop)), ctx.handle)
+        check(ctx.lib.sloam_cyl_learned_exec(verify, plain, _ptr(x), _ptr(off), _ptr(prop),
+                                             C.c_float(0.5), None, None, _ptr(flags),
+                                             _ptr(n_fb), None, _ptr(model)), ctx.handle)
+        ctx.sync()
+        got = model.cpu().numpy()[:n]
+        print(f"[cylfit] counted learned fit: {int(n_fb.cpu()[0])} fallbacks, host "
+              f"{int(ref_out['n_fallback'].cpu()[0])}", flush=True)
+        assert np.array_equal(got.view(np.uint64), ref[:n].view(np.uint64))
+        assert int(n_fb.cpu()[0]) == int(ref_out["n_fallback"].cpu()[0])
+    finally:
+        ctx.lib.sloam_cyl_plan_destroy(verify)
+        ctx.lib.sloam_cyl_plan_destroy(plain)
+        lf.close()

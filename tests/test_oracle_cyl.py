@@ -192,3 +192,32 @@ def test_edge_sizes():
     assert np.all(res["samples"][:3] == -1)
     assert np.all(res["model"][:3, 6] == -1)
     assert np.all(res["model"][4, 6] == -1)  # collinear: no circle
+
+
+def test_fp32_refit_agrees_with_fp64_refit():
+    """The refit in FP32 residuals / compensated FP32 sums (the GPU's
+    arithmetic) against the same refit in FP64, on clusters of every size
+    class including ones of several 2048-point chunks: equal inlier counts,
+    models within 1e-6 m / rad."""
+    d = synth.make_clusters(120, seed=21, n_min=50, n_max=9000)
+    a = oracle.cyl_fit_batch(d["xyz"], d["offsets"], 256, threads=8, mask=False)
+    b = oracle.cyl_fit_batch(d["xyz"], d["offsets"], 256, threads=8, mask=False, fp64_refit=True)
+    assert np.array_equal(a["best"], b["best"])
+    assert np.array_equal(a["model"][:, 7], b["model"][:, 7])
+    ok = b["model"][:, 6] > 0
+    assert np.array_equal(a["model"][:, 6] > 0, ok) and ok.mean() > 0.9
+    assert np.abs(a["model"][ok, :7] - b["model"][ok, :7]).max() < 1e-6
+    assert np.diff(d["offsets"]).max() > 3 * 2048
+
+
+def test_fp32_refit_noise_free_cylinder_exact():
+    """A noise-free cylinder of 7000 points (four refit chunks) is recovered
+    by the FP32 refit to FP32 accuracy."""
+    pts = _grid_cylinder((5.0, 4.0, -1.0), 0.35, 6.0, n_h=175, n_a=40)
+    off = np.array([0, len(pts)], np.int32)
+    assert len(pts) == 7000
+    m = oracle.cyl_fit_batch(pts, off, 64, thr=0.05)["model"][0]
+    assert abs(m[6] - 0.35) < 2e-6
+    assert abs(abs(m[5]) - 1.0) < 1e-9
+    assert np.allclose(m[:2], [5.0, 4.0], atol=2e-6)
+    assert m[7] == len(pts)
