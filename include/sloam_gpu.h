@@ -333,6 +333,22 @@ int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* 
                             int32_t* d_winner, uint8_t* d_mask, float* d_ground,
                             int32_t* d_n_ground, float* d_tree);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, outputs and errors as
+ * sloam_project_partition, but its scratch (z-buffer keys, point
+ * projections, block counts) is a fixed array in device memory that each GPU
+ * holds for the life of the process (about 8.4 MB), so the call allocates and
+ * frees nothing, and it records no kernel timings.  Limits: height * width <=
+ * 524288 pixels and n <= 524288 points, else SLOAM_ERR_TOO_LARGE.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * contexts, or two host threads, with work in flight at once) race; calls
+ * ordered on one stream do not.                                             */
+int sloam_project_partition_noalloc(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* d_label,
+                                    int64_t n, const sloam_proj_cfg* cfg, int32_t* d_proj,
+                                    float* d_range, int32_t* d_winner, uint8_t* d_mask,
+                                    float* d_ground, int32_t* d_n_ground, float* d_tree);
+
 /* ---- ground planes: binning, bottom-k selection, per-cell plane fit -------
  * Replaces: sloam::sloam::binGroundPoints (sloam/src/core/sloam.cpp:330-386)
  * and Plane::Plane / Plane::computeModel (sloam/src/objects/plane.cpp:3-17,

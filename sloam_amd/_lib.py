@@ -216,6 +216,8 @@ PROTOTYPES = {
     "sloam_pose_plan_uses_fused": (C.c_int, [_P]),
     "sloam_project_partition": (C.c_int, [_P, _P, _P, _I64, C.POINTER(ProjCfg), _P, _P, _P, _P,
                                           _P, _P, _P]),
+    "sloam_project_partition_noalloc": (C.c_int, [_P, _P, _P, _I64, C.POINTER(ProjCfg), _P, _P,
+                                                  _P, _P, _P, _P, _P]),
     "sloam_ground_planes": (C.c_int, [_P, _P, _I32, _P, C.POINTER(GroundCfg), _P, _P, _P, _P,
                                       _P]),
     "sloam_plane_plan_create": (C.c_int, [_P, _P, _I32, C.POINTER(PlaneFitCfg), C.POINTER(_P)]),

@@ -71,6 +71,10 @@ int sloam_ctx_scratch(sloam_ctx* ctx, int slot, size_t bytes, void** out);
 // Adds `delta` (+1 / -1) to the hold count of every slot in the bit set.
 void sloam_ctx_pin_slots(sloam_ctx* ctx, uint32_t slots, int delta);
 
+// Finds this device's address of the scratch array of
+// sloam_project_partition_noalloc (once per device; the device is current).
+int sloam_project_noalloc_init(sloam_ctx* ctx);
+
 // Pose solves whose problem count is in device memory (*d_n_problems <=
 // max_problems): LM state set-up and the fused LM kernel of
 // sloam_pose_plan_exec over a work list built on the device (d_work: int4

@@ -171,6 +171,9 @@ int sloam_ctx_create(int device, sloam_ctx** out) {
     delete ctx;
     return SLOAM_ERR_HIP;
   }
+  // a failure here only disables sloam_project_partition_noalloc, which
+  // then reports it
+  (void)sloam_project_noalloc_init(ctx);
   *out = ctx;
   return SLOAM_OK;
 }

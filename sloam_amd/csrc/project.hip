@@ -22,6 +22,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <mutex>
 
 namespace {
 
@@ -214,34 +215,57 @@ __global__ __launch_bounds__(kPB) void part_scatter_kernel(
   }
 }
 
-}  // namespace
+// Scratch of sloam_project_partition_noalloc: the z-buffer keys, the point
+// projections (when the caller passes no d_proj) and the partition's block
+// counts of one call of at most kNoAllocMaxPixels pixels and kNoAllocMaxPoints
+// points.  Each device holds its own copy for the life of the process; the
+// host reaches it only through the address hipGetSymbolAddress returns for
+// that device (noalloc_ptr).
+constexpr int64_t kNoAllocMaxPixels = int64_t(1) << 19;
+constexpr int64_t kNoAllocMaxPoints = int64_t(1) << 19;
+constexpr size_t kNoAllocBytes = sizeof(unsigned long long) * kNoAllocMaxPixels +
+                                 sizeof(int2) * kNoAllocMaxPoints +
+                                 sizeof(int32_t) * (kNoAllocMaxPoints / kPartItems + 2);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
-extern "C" {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+hipError_t g_noalloc_err[kMaxDevices] = {};  // the lookup's failure, if any
 
-int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* d_label,
-                            int64_t n, const sloam_proj_cfg* cfg, int32_t* d_proj, float* d_range,
-                            int32_t* d_winner, uint8_t* d_mask, float* d_ground,
-                            int32_t* d_n_ground, float* d_tree) {
+// Bytes of scratch one call needs: keys (8 B / pixel), projections (8 B /
+// point) when the caller gives none, block counts (4 B / block + 1).
+size_t proj_scratch_bytes(int n_pix, int N, bool own_proj) {
+  const int n_blocks = (N + kPartItems - 1) / kPartItems;
+  return sizeof(unsigned long long) * (size_t)n_pix + (own_proj ? 0 : sizeof(int2) * (size_t)N) +
+         sizeof(int32_t) * (size_t)(n_blocks + 1);
+}
+
+int proj_check(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* d_label, int64_t n,
+               const sloam_proj_cfg* cfg, uint8_t* d_mask, float* d_ground, int32_t* d_n_ground) {
   if (!ctx || !cfg || !d_mask || !d_n_ground || n < 0) return SLOAM_ERR_INVALID_ARG;
   if (n > 0 && (!d_xyzi || !d_label || !d_ground)) return SLOAM_ERR_INVALID_ARG;
   if (cfg->height < 1 || cfg->width < 1 || cfg->height > 4096 || cfg->width > 65536)
     return SLOAM_ERR_INVALID_ARG;
   if (n > 0x7fffffff) return SLOAM_ERR_TOO_LARGE;
+  if (cfg->destagger && n != (int64_t)cfg->height * cfg->width) return SLOAM_ERR_INVALID_ARG;
+  return SLOAM_OK;
+}
+
+// Enqueues the projection and partition of one scan with its scratch at
+// `base` (proj_scratch_bytes bytes).  `timed`: bracket the kernels with the
+// context's timers (the no-allocation variant does not, since a timer's
+// first use creates events).
+int proj_run(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* d_label, int64_t n,
+             const sloam_proj_cfg* cfg, int32_t* d_proj, float* d_range, int32_t* d_winner,
+             uint8_t* d_mask, float* d_ground, int32_t* d_n_ground, float* d_tree, void* base,
+             bool timed) {
   const int H = cfg->height, W = cfg->width, N = (int)n;
-  if (cfg->destagger && N != H * W) return SLOAM_ERR_INVALID_ARG;
-  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
   const int n_pix = H * W;
   const int n_blocks = (N + kPartItems - 1) / kPartItems;
-  // scratch: keys (8 B / pixel), projections (8 B / point) if the caller gives
-  // none, block counts
-  const size_t need = sizeof(unsigned long long) * n_pix + (d_proj ? 0 : sizeof(int2) * (size_t)N) +
-                      sizeof(int32_t) * (size_t)(n_blocks + 1);
-  void* scratch = nullptr;
-  int st = sloam_ctx_scratch(ctx, 0, need, &scratch);
-  if (st) return st;
-  unsigned long long* keys = (unsigned long long*)scratch;
+  unsigned long long* keys = (unsigned long long*)base;
   int2* proj = d_proj ? (int2*)d_proj : (int2*)(keys + n_pix);
-  int32_t* counts = (int32_t*)((char*)scratch + sizeof(unsigned long long) * n_pix +
+  int32_t* counts = (int32_t*)((char*)base + sizeof(unsigned long long) * n_pix +
                                (d_proj ? 0 : sizeof(int2) * (size_t)N));
   ProjCfgDev dc;
   dc.H = H;
@@ -254,8 +278,9 @@ int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* 
     dc.fov = fabsf(fov_down) + fabsf(fov_up);
   }
   dc.destagger = cfg->destagger;
+  sloam_ctx* tctx = timed ? ctx : nullptr;  // a null context times nothing
   {
-    SloamKernelScope ks(ctx, "proj_zbuf");
+    SloamKernelScope ks(tctx, "proj_zbuf");
     hipLaunchKernelGGL(fill_u64_kernel, dim3((n_pix + kPB - 1) / kPB), dim3(kPB), 0, ctx->stream,
                        keys, n_pix, ~0ull);
     if (N > 0)
@@ -264,13 +289,13 @@ int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* 
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {
-    SloamKernelScope ks(ctx, "proj_pixels");
+    SloamKernelScope ks(tctx, "proj_pixels");
     hipLaunchKernelGGL(proj_pixels_kernel, dim3((n_pix + kPB - 1) / kPB), dim3(kPB), 0,
                        ctx->stream, keys, d_label, n_pix, d_range, d_winner, d_mask);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   {
-    SloamKernelScope ks(ctx, "partition");
+    SloamKernelScope ks(tctx, "partition");
     if (N > 0) {
       hipLaunchKernelGGL(part_count_kernel, dim3(n_blocks), dim3(kPB), 0, ctx->stream, proj,
                          d_mask, N, W, counts);
@@ -287,6 +312,67 @@ int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* 
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
+}
+
+}  // namespace
+
+// Looks up this device's copy of the no-allocation scratch once per device
+// (the device must be current).  Called when a context is created.
+int sloam_project_noalloc_init(sloam_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return SLOAM_OK;  // variant unavailable
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[ctx->device]) return SLOAM_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_noalloc_err[ctx->device] = e;
+  if (e != hipSuccess) return sloam_record_hip_error(ctx, e, "hipGetSymbolAddress(scratch)");
+  g_noalloc_ptr[ctx->device] = ptr;
+  return ptr ? SLOAM_OK : SLOAM_ERR_HIP;
+}
+
+extern "C" {
+
+int sloam_project_partition(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* d_label,
+                            int64_t n, const sloam_proj_cfg* cfg, int32_t* d_proj, float* d_range,
+                            int32_t* d_winner, uint8_t* d_mask, float* d_ground,
+                            int32_t* d_n_ground, float* d_tree) {
+  int st = proj_check(ctx, d_xyzi, d_label, n, cfg, d_mask, d_ground, d_n_ground);
+  if (st) return st;
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  void* base = nullptr;
+  st = sloam_ctx_scratch(ctx, 0, proj_scratch_bytes(cfg->height * cfg->width, (int)n, d_proj),
+                         &base);
+  if (st) return st;
+  return proj_run(ctx, d_xyzi, d_label, n, cfg, d_proj, d_range, d_winner, d_mask, d_ground,
+                  d_n_ground, d_tree, base, true);
+}
+
+int sloam_project_partition_noalloc(sloam_ctx* ctx, const float* d_xyzi, const uint8_t* d_label,
+                                    int64_t n, const sloam_proj_cfg* cfg, int32_t* d_proj,
+                                    float* d_range, int32_t* d_winner, uint8_t* d_mask,
+                                    float* d_ground, int32_t* d_n_ground, float* d_tree) {
+  int st = proj_check(ctx, d_xyzi, d_label, n, cfg, d_mask, d_ground, d_n_ground);
+  if (st) return st;
+  if ((int64_t)cfg->height * cfg->width > kNoAllocMaxPixels || n > kNoAllocMaxPoints) {
+    ctx->last_error = "sloam_project_partition_noalloc: at most 524288 pixels and 524288 points";
+    return SLOAM_ERR_TOO_LARGE;
+  }
+  void* base = nullptr;
+  hipError_t lookup = hipSuccess;
+  if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    base = g_noalloc_ptr[ctx->device];
+    lookup = g_noalloc_err[ctx->device];
+  }
+  if (!base) {
+    ctx->last_error = std::string("sloam_project_partition_noalloc: no scratch address for "
+                                  "this device (hipGetSymbolAddress: ") +
+                      hipGetErrorString(lookup) + ")";
+    return SLOAM_ERR_INTERNAL;
+  }
+  SLOAM_HIP(ctx, hipSetDevice(ctx->device));
+  return proj_run(ctx, d_xyzi, d_label, n, cfg, d_proj, d_range, d_winner, d_mask, d_ground,
+                  d_n_ground, d_tree, base, false);
 }
 
 }  // extern "C"
