@@ -697,19 +697,25 @@ def bench_keyframe_stream(args):
                        check=True, timeout=1200, cwd=ROOT)
         gen_s = time.perf_counter() - t0
         res, traj = {}, {}
-        for name, model, pipelined in (("learned", "learned", 0),
-                                       ("learned_pipelined", "learned", 1),
-                                       ("plain", "plain", 0)):
+        # name, tree model, pipelined, device estimator (the whole keyframe
+        # as one captured graph)
+        for name, model, pipelined, device in (("learned", "learned", 0, 0),
+                                               ("learned_pipelined", "learned", 1, 0),
+                                               ("plain", "plain", 0, 0),
+                                               ("learned_device", "learned", 0, 1),
+                                               ("plain_device", "plain", 0, 1)):
             print(f"[bench] long keyframe stream: {name}", file=sys.stderr, flush=True)
             res[name], traj[name] = run_keyframe_bench(
                 path, 2, model, 1, pipelined, os.path.join(d, f"traj_{name}.bin"),
-                shadow="plain" if name == "learned" else None)
+                shadow="plain" if name == "learned" else None,
+                extra=KF_PARAMS + (("--device-estimator",) if device else ()))
 
     def summary(r):
         return {k: r[k] for k in ("latency_ms", "throughput_kf_per_s", "stage_p50_ms",
                                   "estimator_p50_ms", "tree_fit_p50_ms", "trees_mean",
                                   "planes_mean", "success_rate", "discarded", "trees_fitted",
-                                  "trees_fallback", "drift", "map_p50_ms")}
+                                  "trees_fallback", "drift", "map_p50_ms",
+                                  "latency_with_map_ms")}
 
     g = summary(res["learned"])
     g["keyframes"] = n
@@ -720,13 +726,25 @@ def bench_keyframe_stream(args):
                       "trajectory_equal": bool(np.array_equal(traj["learned_pipelined"],
                                                               traj["learned"]))}
     g["plain"] = summary(res["plain"])
+    # the device estimator: the same keyframes, bit for bit, with the
+    # estimator on the GPU behind the front end in one graph per keyframe
+    g["device"] = {
+        name: {"latency_ms": res[f"{name}_device"]["latency_ms"],
+               "latency_with_map_ms": res[f"{name}_device"]["latency_with_map_ms"],
+               "throughput_kf_per_s": res[f"{name}_device"]["throughput_kf_per_s"],
+               "success_rate": res[f"{name}_device"]["success_rate"],
+               "trajectory_equal_to_host": bool(np.array_equal(traj[f"{name}_device"],
+                                                               traj[name]))}
+        for name in ("learned", "plain")}
     g["workload"] = (f"{n} keyframes of {args.kf_height}x2048 scans, {args.stream_spacing} m "
                      f"apart ({n * args.stream_spacing:.0f} m), 300 trees over "
                      f"{args.stream_forest_length:.0f} x 60 m, 40 % understory clutter, "
                      f"odometry noise {sigma_t:.4f} m / {sigma_yaw:.3f} deg per step; "
                      f"two passes, the second timed; learned tree model (sequential and "
-                     f"pipelined runner), plain cylinder fit beside it; pinned input "
-                     f"buffers; drift is the final/max keyframe position error against "
+                     f"pipelined runner), plain cylinder fit beside it, each also with "
+                     f"the estimator on the GPU (device: the keyframe as one HIP graph); "
+                     f"pinned input buffers; latency without the map query and update "
+                     f"(latency_with_map_ms with them); drift is the final/max keyframe position error against "
                      f"the true poses, drift_vs_plain against the plain trajectory "
                      f"(two maps built apart), step_vs_plain the per-keyframe pose "
                      f"difference of a plain runner fed the learned run's estimate and "
@@ -959,6 +977,10 @@ def main():
         "keyframe_throughput_kf_per_s": None if head is None else
         {"sequential": head["throughput_kf_per_s"],
          "pipelined": head["pipelined"]["throughput_kf_per_s"]},
+        "keyframe_device_latency_ms": None if head is None or "device" not in head else
+        {"p50": head["device"]["learned"]["latency_ms"]["p50"],
+         "p99": head["device"]["learned"]["latency_ms"]["p99"],
+         "trajectory_equal_to_host": head["device"]["learned"]["trajectory_equal_to_host"]},
         "keyframe_drift_vs_plain": None if head is None else head["drift_vs_plain"],
         "keyframe_step_vs_plain": None if head is None else head["step_vs_plain"],
         "keyframe_headline": head_name,

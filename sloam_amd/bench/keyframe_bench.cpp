@@ -335,6 +335,8 @@ int main(int argc, char** argv) {
   // (MapManager::getSubmap + updateMap, as SLOAMNode::run makes them around
   // each keyframe; counted in the throughput, not in the latency)
   std::vector<double> map_ms;
+  // the keyframe's latency with its map query and update added
+  std::vector<double> with_map;
   long long fitted = 0, fallback = 0;
   int discarded = 0;
   std::vector<double> pass_rate;  // keyframes per second of each timed pass
@@ -347,6 +349,7 @@ int main(int argc, char** argv) {
   int dc_calls = 0, dc_ok_differs = 0, dc_pose_eq = 0, dc_models_eq = 0, dc_step_eq = 0,
       dc_first_diff = -1;
   double dc_max_m = 0, dc_max_deg = 0;
+  int dc_printed = 0;  // differing keyframes printed to stderr
   // per compared keyframe: m, deg, k, then each runner's trees, map
   // matches, tree step run, tree feature matches; largest radius difference
   // of the landmarks both kept in the same order
@@ -479,6 +482,21 @@ int main(int argc, char** argv) {
                 dc_max_m = std::max(dc_max_m, m);
                 dc_max_deg = std::max(dc_max_deg, deg);
                 if ((!pose_eq || !models_eq || !step_eq) && dc_first_diff < 0) dc_first_diff = k;
+                if ((!pose_eq || !models_eq || !step_eq) && dc_printed < 10) {
+                  ++dc_printed;
+                  std::fprintf(stderr,
+                               "device differs at keyframe %d: pose %d (%.3e m %.3e deg), "
+                               "landmarks %d (%zu / %zu), steps %d (tree %d/%d ground %d/%d "
+                               "matches %d/%d %d/%d)\n",
+                               k, pose_eq ? 1 : 0, m, deg, models_eq ? 1 : 0, out.tm.size(),
+                               so.tm.size(), step_eq ? 1 : 0, a.treeCheck, b.treeCheck,
+                               a.groundCheck, b.groundCheck, a.treeMatches, b.treeMatches,
+                               a.planeMatches, b.planeMatches);
+                }
+              } else if (ok != sok && dc_printed < 10) {
+                ++dc_printed;
+                std::fprintf(stderr, "device differs at keyframe %d: success %d / %d\n", k,
+                             ok ? 1 : 0, sok ? 1 : 0);
               }
             }
             if (compare_device) {
@@ -568,6 +586,7 @@ int main(int argc, char** argv) {
         }
         total.push_back(t.total_ms);
         map_ms.push_back(kf_map_ms);
+        with_map.push_back(t.total_ms + kf_map_ms);
         slow.push_back({t.total_ms, (double)k, t.upload_ms, t.frontend_ms, t.estimator_ms,
                         t.phases.models, t.phases.pose});
         fe.push_back(t.frontend_ms);
@@ -640,7 +659,8 @@ int main(int argc, char** argv) {
       "\"stage_p50_ms\": {\"upload\": %.4f, \"frontend\": %.4f, \"download\": %.4f, "
       "\"estimator\": %.4f}, \"estimator_p50_ms\": {\"build\": %.4f, \"models\": %.4f, "
       "\"match\": %.4f, \"pose\": %.4f, \"finish\": %.4f}, "
-      "\"frontend_p99_ms\": %.4f, \"map_p50_ms\": %.4f, \"trees_mean\": %.2f, "
+      "\"frontend_p99_ms\": %.4f, \"map_p50_ms\": %.4f, \"latency_with_map_ms\": "
+      "{\"p50\": %.4f, \"p99\": %.4f}, \"trees_mean\": %.2f, "
       "\"planes_mean\": %.2f, \"success_rate\": %.4f, \"drift\": {\"final_m\": %.4f, "
       "\"max_m\": %.4f, \"max_deg\": %.4f}, \"warmup_pass_latency_ms\": {\"p50\": %.4f, "
       "\"max\": %.4f}, ",
@@ -649,6 +669,7 @@ int main(int argc, char** argv) {
       total.empty() ? 0.0 : *std::max_element(total.begin(), total.end()), pct(up, 0.5),
       pct(fe, 0.5), pct(down, 0.5), pct(est, 0.5), pct(build, 0.5), pct(ph_models, 0.5),
       pct(ph_match, 0.5), pct(ph_pose, 0.5), pct(ph_finish, 0.5), pct(fe, 0.99), pct(map_ms, 0.5),
+      pct(with_map, 0.5), pct(with_map, 0.99),
       samples ? trees / samples : 0.0, samples ? planes / samples : 0.0,
       samples ? (double)ok_count / samples : 0.0, final_drift, max_drift, max_rot,
       pct(first, 0.5), first.empty() ? 0.0 : *std::max_element(first.begin(), first.end()));

@@ -25,6 +25,7 @@
 // Coordinates are evaluated relative to a per-problem anchor (the initial
 // translation) so that FP32 residuals keep sub-micrometre resolution.
 #include "common.hpp"
+#include "detmath.h"
 
 #include <algorithm>
 #include <cmath>
@@ -88,11 +89,13 @@ __device__ inline void quat_left_update(const double q[4], const double dth[3], 
   const double ang = sqrt(dth[0] * dth[0] + dth[1] * dth[1] + dth[2] * dth[2]);
   double d[4];
   if (ang > 1e-300) {
-    const double s = sin(0.5 * ang) / ang;
+    double sh, ch;
+    sloam_dm::sincos(0.5 * ang, &sh, &ch);
+    const double s = sh / ang;
     d[0] = dth[0] * s;
     d[1] = dth[1] * s;
     d[2] = dth[2] * s;
-    d[3] = cos(0.5 * ang);
+    d[3] = ch;
   } else {
     d[0] = 0.5 * dth[0];
     d[1] = 0.5 * dth[1];
@@ -116,7 +119,8 @@ __device__ inline void quat_to_aa(const double qxyzw[4], double aa[3]) {
   const double s2 = q1 * q1 + q2 * q2 + q3 * q3;
   if (s2 > 0.0) {
     const double st = sqrt(s2);
-    const double two_theta = 2.0 * ((c < 0.0) ? atan2(-st, -c) : atan2(st, c));
+    const double two_theta =
+        2.0 * ((c < 0.0) ? sloam_dm::atan2(-st, -c) : sloam_dm::atan2(st, c));
     const double k = two_theta / st;
     aa[0] = q1 * k;
     aa[1] = q2 * k;
@@ -133,11 +137,13 @@ __device__ inline void aa_to_quat(const double aa[3], double qxyzw[4]) {
   if (t2 > 0.0) {
     const double th = sqrt(t2);
     const double h = 0.5 * th;
-    const double k = sin(h) / th;
+    double sh, ch;
+    sloam_dm::sincos(h, &sh, &ch);
+    const double k = sh / th;
     qxyzw[0] = aa[0] * k;
     qxyzw[1] = aa[1] * k;
     qxyzw[2] = aa[2] * k;
-    qxyzw[3] = cos(h);
+    qxyzw[3] = ch;
   } else {
     qxyzw[0] = 0.5 * aa[0];
     qxyzw[1] = 0.5 * aa[1];
@@ -155,8 +161,10 @@ __device__ inline void left_jacobian(const double w[3], double J[3][3]) {
     b = 1.0 / 6.0 - t2 / 120.0;
   } else {
     const double t = sqrt(t2);
-    a = (1.0 - cos(t)) / t2;
-    b = (t - sin(t)) / (t2 * t);
+    double st, ct;
+    sloam_dm::sincos(t, &st, &ct);
+    a = (1.0 - ct) / t2;
+    b = (t - st) / (t2 * t);
   }
   const double K[3][3] = {{0.0, -w[2], w[1]}, {w[2], 0.0, -w[0]}, {-w[1], w[0], 0.0}};
   for (int i = 0; i < 3; ++i)

@@ -13,12 +13,18 @@
 
 // The vector, quaternion and rigid-transform helpers below are also compiled
 // into the GPU keyframe estimator (sloam_amd/csrc/keyframe.hip), so that the
-// device evaluates the same expressions in the same order as the host.
+// device evaluates the same expressions in the same order as the host; their
+// sines, cosines and arc tangents are detmath.h's, which give the same bits
+// on both (the host's and the GPU's libm differ in the last place).
+#ifndef SLOAM_HD
 #if defined(__HIPCC__)
 #define SLOAM_HD __host__ __device__
 #else
 #define SLOAM_HD
 #endif
+#endif
+
+#include "detmath.h"
 
 using Scalar = double;
 
@@ -190,17 +196,18 @@ struct Quat {
 SLOAM_HD inline Vector3 eulerAngles012(const double R[3][3]) {
   // first angle from the last column; it is folded into (-pi, 0] before the
   // final sign flip, the second angle's cosine takes the matching sign
-  double a = atan2(R[1][2], R[2][2]);
+  double a = sloam_dm::atan2(R[1][2], R[2][2]);
   const double c2 = sqrt(R[0][0] * R[0][0] + R[0][1] * R[0][1]);
   double b;
   if (a > 0) {
     a -= M_PI;
-    b = atan2(-R[0][2], -c2);
+    b = sloam_dm::atan2(-R[0][2], -c2);
   } else {
-    b = atan2(-R[0][2], c2);
+    b = sloam_dm::atan2(-R[0][2], c2);
   }
-  const double s1 = sin(a), c1 = cos(a);
-  const double c = atan2(s1 * R[2][0] - c1 * R[1][0], c1 * R[1][1] - s1 * R[2][1]);
+  double s1, c1;
+  sloam_dm::sincos(a, &s1, &c1);
+  const double c = sloam_dm::atan2(s1 * R[2][0] - c1 * R[1][0], c1 * R[1][1] - s1 * R[2][1]);
   return {-a, -b, -c};
 }
 
@@ -211,8 +218,10 @@ SLOAM_HD inline void angleAxisToQuaternion(const double aa[3], double q[4]) {
   if (t2 > 0.0) {
     const double t = sqrt(t2);
     const double h = t * 0.5;
-    const double k = sin(h) / t;
-    q[0] = cos(h);
+    double sh, ch;
+    sloam_dm::sincos(h, &sh, &ch);
+    const double k = sh / t;
+    q[0] = ch;
     q[1] = aa[0] * k;
     q[2] = aa[1] * k;
     q[3] = aa[2] * k;
@@ -231,7 +240,8 @@ SLOAM_HD inline void quaternionToAngleAxis(const double q[4], double aa[3]) {
   if (s2 > 0.0) {
     const double s = sqrt(s2), c = q[0];
     // the angle of the shorter rotation, in [-pi, pi]
-    const double two_theta = 2.0 * (c < 0.0 ? atan2(-s, -c) : atan2(s, c));
+    const double two_theta =
+        2.0 * (c < 0.0 ? sloam_dm::atan2(-s, -c) : sloam_dm::atan2(s, c));
     k = two_theta / s;
   }
   aa[0] = q[1] * k;

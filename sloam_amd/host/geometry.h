@@ -125,7 +125,7 @@ SLOAM_HD inline bool cyl_filter(const CylModel& m, double maxTreeRadius, double 
                                 const Vector4& g) {
   if (m.radius == -1) return false;
   const Vector3 up = g.head3();
-  const double theta = (180 / PIDEF) * acos(m.ray.dot(up) / (m.ray.norm() * up.norm()));
+  const double theta = (180 / PIDEF) * sloam_dm::acos(m.ray.dot(up) / (m.ray.norm() * up.norm()));
   return (m.radius < maxTreeRadius) && ((theta <= maxAxisTheta) || (theta >= 180 - maxAxisTheta));
 }
 

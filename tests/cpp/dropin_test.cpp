@@ -29,6 +29,7 @@
 #include "mapManager.h"
 #include "pcdIO.h"
 #include "definitions.h"
+#include "detmath.h"
 #include "plane.h"
 #include "rosmsg.h"
 #include "segmentation.h"
@@ -232,6 +233,62 @@ void test_angle_axis() {
   double q[4];
   angleAxisToQuaternion(z, q);
   EXPECT_EQ(q[0], 1.0);
+}
+
+// error of a double result in units of the last place of the exact value,
+// the exact value taken from the long double function (64-bit significand)
+double ulp_error(double got, long double exact) {
+  const double e = (double)exact;
+  const double ulp = std::nextafter(std::fabs(e), INFINITY) - std::fabs(e);
+  return (double)(std::fabs((long double)got - exact) / (long double)ulp);
+}
+
+// sloam_dm's sine, cosine, arc tangent and arc cosine against the long double
+// libm over the ranges the pose and the model gates use and beyond; the
+// quadrants, signed zeros and NaN of atan2 as in C
+void test_detmath() {
+  std::mt19937_64 g(11);
+  double worst[4] = {0, 0, 0, 0};
+  std::uniform_real_distribution<double> small(-4.0, 4.0), wide(-1000.0, 1000.0), unit(-1.0, 1.0);
+  std::normal_distribution<double> n(0.0, 1.0);
+  std::uniform_real_distribution<double> mag(-30.0, 30.0);
+  for (int it = 0; it < 400000; ++it) {
+    const double x = (it & 1) ? small(g) : wide(g);
+    double s, c;
+    sloam_dm::sincos(x, &s, &c);
+    worst[0] = std::max(worst[0], ulp_error(s, sinl((long double)x)));
+    worst[1] = std::max(worst[1], ulp_error(c, cosl((long double)x)));
+    // atan2 over every quadrant and magnitude ratios from 1e-13 to 1e13
+    const double y2 = n(g) * std::pow(10.0, mag(g) / 5), x2 = n(g) * std::pow(10.0, mag(g) / 5);
+    worst[2] = std::max(worst[2],
+                        ulp_error(sloam_dm::atan2(y2, x2), atan2l((long double)y2, (long double)x2)));
+    const double a = unit(g);
+    worst[3] = std::max(worst[3], ulp_error(sloam_dm::acos(a), acosl((long double)a)));
+  }
+  std::printf("detmath worst ulp error: sin %.3f cos %.3f atan2 %.3f acos %.3f\n", worst[0],
+              worst[1], worst[2], worst[3]);
+  EXPECT_TRUE(worst[0] <= 0.8);
+  EXPECT_TRUE(worst[1] <= 0.8);
+  EXPECT_TRUE(worst[2] <= 1.3);
+  EXPECT_TRUE(worst[3] <= 1.8);
+  // exact points and the sign conventions of C's atan2
+  EXPECT_EQ(sloam_dm::atan2(0.0, 1.0), 0.0);
+  EXPECT_EQ(sloam_dm::atan2(0.0, -1.0), M_PI);
+  EXPECT_EQ(sloam_dm::atan2(-0.0, -1.0), -M_PI);
+  EXPECT_EQ(sloam_dm::atan2(1.0, 0.0), M_PI / 2);
+  EXPECT_EQ(sloam_dm::atan2(-1.0, 0.0), -M_PI / 2);
+  EXPECT_EQ(sloam_dm::atan2(1.0, 1.0), M_PI / 4);
+  EXPECT_NEAR(sloam_dm::atan2(-1.0, -1.0), -3 * M_PI / 4, 4.5e-16);
+  EXPECT_TRUE(std::signbit(sloam_dm::atan2(-0.0, 1.0)));
+  EXPECT_TRUE(std::isnan(sloam_dm::atan2(NAN, 1.0)));
+  EXPECT_TRUE(std::isnan(sloam_dm::acos(1.0 + 1e-15)));
+  EXPECT_EQ(sloam_dm::acos(1.0), 0.0);
+  EXPECT_EQ(sloam_dm::acos(-1.0), M_PI);
+  EXPECT_EQ(sloam_dm::sin(0.0), 0.0);
+  EXPECT_EQ(sloam_dm::cos(0.0), 1.0);
+  double s, c;
+  sloam_dm::sincos(2e6, &s, &c);
+  EXPECT_TRUE(std::isnan(s) && std::isnan(c));
 }
 
 void test_se3() {
@@ -1178,6 +1235,7 @@ int main(int argc, char** argv) {
       {"FromTwoVectors", false, test_from_two_vectors},
       {"AngleAxisQuaternion", false, test_angle_axis},
       {"SE3", false, test_se3},
+      {"DetMath.Accuracy", false, test_detmath},
       {"CylinderTest.FromCylinderFit", false, cylinder_from_fit},
       {"MapManager", false, map_manager},
       {"PCD.ReadWrite", false, pcd_io},

@@ -10,12 +10,13 @@ estimates and submaps), the same success, the same XY-yaw / Z-roll-pitch
 decisions and feature-match counts, the same pose bits, and the same
 landmarks (models, ids, map matches); over a whole run
 (--device-estimator), the same trajectory bits.  Both estimators evaluate
-the same model functions (host/geometry.h) and run the same LM kernel; only
-the libm of the two sides (atan2, acos, sin, cos in the gates) could differ,
-in a last bit, at a threshold."""
+the same model functions (host/geometry.h) with the same elementary
+functions (host/detmath.h: the two libms differ in last bits) and run the
+same LM kernel."""
 import json
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -53,11 +54,27 @@ def _run(seq, tmp, model, extra=(), tag=""):
     return res, np.fromfile(traj, dtype=np.float64).reshape(-1, 7)
 
 
-@pytest.mark.parametrize("which", ["short", "stream"])
+@pytest.fixture(scope="module")
+def long_stream(tmp_path_factory):
+    """The first 200 keyframes of the bench's long stream (0.2 m apart, a
+    260 m forest, odometry noise 0.0316 m / 0.316 deg per step), cast by
+    eight processes in a child (this process holds the GPU): keyframes whose
+    XY-yaw step is skipped compose the pose estimate's own yaw, the case
+    that needs the two sides' arc tangents to agree."""
+    path = str(tmp_path_factory.mktemp("kfdev_long") / "long.bin")
+    print("[kf-device] writing 200 keyframes of the long stream", flush=True)
+    subprocess.run([sys.executable, "-m", "sloam_amd.synth", path, "200", "0.2", "260",
+                    "0.0316227766", "0.316227766", "8", "128"], check=True, timeout=300, cwd=ROOT)
+    return path
+
+
+@pytest.mark.parametrize("which", ["short", "stream", "long"])
 @pytest.mark.parametrize("model", ["plain", "learned"])
-def test_device_estimator_matches_host_per_keyframe(sequences, tmp_path, model, which):
+def test_device_estimator_matches_host_per_keyframe(sequences, long_stream, tmp_path, model,
+                                                    which):
     assert os.path.exists(EXE), "build first"
-    res, _ = _run(sequences[which], tmp_path, model, extra=("--compare-device",), tag=which)
+    seq = long_stream if which == "long" else sequences[which]
+    res, _ = _run(seq, tmp_path, model, extra=("--compare-device",), tag=which)
     dc = res["device_compare"]
     print(f"[kf-device] {model} {which}: {dc}; steps {res['steps']}", flush=True)
     assert dc["calls"] >= 10
@@ -67,12 +84,13 @@ def test_device_estimator_matches_host_per_keyframe(sequences, tmp_path, model, 
     assert dc["steps_equal"] == dc["calls"], dc
 
 
-def test_device_estimator_trajectory(sequences, tmp_path):
+@pytest.mark.parametrize("which", ["stream", "long"])
+def test_device_estimator_trajectory(sequences, long_stream, tmp_path, which):
     """A whole run with the device estimator gives the host estimator's
     trajectory bit for bit (the map built from its own outputs)."""
-    host, t_host = _run(sequences["stream"], tmp_path, "plain", tag="host")
-    dev, t_dev = _run(sequences["stream"], tmp_path, "plain", extra=("--device-estimator",),
-                      tag="dev")
+    seq = long_stream if which == "long" else sequences[which]
+    host, t_host = _run(seq, tmp_path, "plain", tag="host")
+    dev, t_dev = _run(seq, tmp_path, "plain", extra=("--device-estimator",), tag="dev")
     print(f"[kf-device] host success {host['success_rate']}, device {dev['success_rate']}; "
           f"latency p50 host {host['latency_ms']['p50']} ms, device {dev['latency_ms']['p50']} ms",
           flush=True)
