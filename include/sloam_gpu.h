@@ -157,8 +157,10 @@ int sloam_cyl_plan_reset(sloam_cyl_plan* plan, const int32_t* h_offsets, int32_t
  * keyframe's trees straight from its front end.  The layout (work list,
  * refit chunks) is made on the device by the exec's first kernel; clusters
  * run in index order, and every result equals a host-planned plan's.  At
- * most cap_clusters clusters and cap_points points: a larger batch sets the
- * overflow word of sloam_cyl_plan_counts and fits nothing.  Exec without
+ * most cap_clusters clusters and cap_points points, and at most 131072
+ * points per cluster (the host-planned plans' bound, SLOAM_ERR_TOO_LARGE
+ * there): a larger batch sets the overflow word of sloam_cyl_plan_counts and
+ * fits nothing.  Exec without
  * d_mask; not with sloam_cyl_plan_reset.                                    */
 int sloam_cyl_plan_create_counted(sloam_ctx* ctx, int32_t cap_clusters, int64_t cap_points,
                                   const int32_t* d_n_clusters, const sloam_cyl_fit_cfg* cfg,

@@ -392,27 +392,28 @@ __global__ __launch_bounds__(kBlock) void cyl_score_kernel(
 // 0, |d_i| < thr under the model at the start of the round afterwards.
 // Output record: root[3], ray[3], radius, inliers.
 //
-// Every cluster is cut into chunks of kRefitChunk points; one workgroup of
-// kRefitBlock threads takes one chunk, so a 5000-point cluster spreads over
-// three workgroups instead of serialising on one.  One launch per pass:
-//   cyl_refit_init   best hypothesis (max count, lowest index), inlier mask,
-//                    round-0 membership bits, the initial model;
-//   cyl_refit_step   one Gauss-Newton step: per point, in FP32, w, h,
-//                    rho = ||w||, d = rho - r, n = w / rho and
-//                    J = (-h n.e1, -h n.e2, -n.e1, -n.e2, -1); each thread
-//                    adds its points' J J^T and J d terms with FMAs, the
-//                    workgroup adds the threads' values with TwoSum in groups
-//                    of kRefitGroup threads (thread order, then group order)
-//                    into one (sum, error) pair per term and chunk;
-//   cyl_refit_solve  one thread per cluster: the chunks' pairs added in chunk
-//                    order, the normal equations solved in FP64, the next
-//                    model;
-//   cyl_refit_final  final inliers and their centroid's chunk sums the same
-//   cyl_refit_finish way, and the output record.
-// The per-cluster kernels read the chunk kernels' results across a kernel
-// boundary (no fences between workgroups inside a launch).  The order of
-// every addition is fixed (no atomics on values), so results do not depend
-// on scheduling; the oracle restates the same arithmetic.
+// Every cluster is cut into chunks of kRefitChunk points; the sums of a step
+// are made per chunk by kRefitBlock threads: per point, in FP32, w, h,
+// rho = ||w||, d = rho - r, n = w / rho and J = (-h n.e1, -h n.e2, -n.e1,
+// -n.e2, -1); each thread adds its points' J J^T and J d terms with FMAs, the
+// chunk's threads' values are added with TwoSum in groups of kRefitGroup
+// threads (thread order, then group order) into one (sum, error) pair per
+// term; the solve adds the chunks' pairs in chunk order and solves the
+// normal equations in FP64.  The order of every addition is fixed (no
+// atomics on values), so results do not depend on scheduling; the oracle
+// restates the same arithmetic.  Two launch forms give the same bits:
+//   per step  (large batches)  cyl_refit_init (best hypothesis, inlier mask,
+//             round-0 membership bits, initial model), then per Gauss-Newton
+//             step cyl_refit_step (one workgroup per chunk) and
+//             cyl_refit_solve (one wave per cluster: lane q sums term q over
+//             the chunks, lane 0 solves), then cyl_refit_final (the final
+//             inliers' centroid sums per chunk) and cyl_refit_finish (the
+//             output record, one wave per cluster);
+//   fused     (small batches, e.g. a keyframe's trees)  cyl_refit_init, then
+//             cyl_refit_fused: one workgroup walks every step of one cluster
+//             over its chunks, its first wave solving between steps.
+// The kernels of the per-step form read the previous launch's results (no
+// fences between workgroups inside a launch).
 // ---------------------------------------------------------------------------
 constexpr int kRefitBlock = 256;
 constexpr int kRefitPerThread = 8;
@@ -420,6 +421,17 @@ constexpr int kRefitChunk = kRefitBlock * kRefitPerThread;
 constexpr int kRefitGroup = 32;
 constexpr int kRefitGroups = kRefitBlock / kRefitGroup;
 constexpr int kRefitTerms = 20;  // 15 of J J^T (upper triangle), 5 of J d
+// largest cluster the plans take (points), and its chunks
+constexpr int kRefitMaxPoints = 1 << 17;
+constexpr int kRefitFusedChunks = kRefitMaxPoints / kRefitChunk;
+// the fused refit (a workgroup walks all steps of one cluster) takes a
+// host-planned batch of at most this many clusters of at most this many
+// chunks: beyond, one launch per step (every chunk of every cluster in
+// parallel) is faster; counted plans (a keyframe's trees) always take it
+constexpr int kRefitFusedPlanClusters = 512;
+constexpr int kRefitFusedPlanChunks = 4;
+// clusters (one wave each) per workgroup of the per-cluster solve and finish
+constexpr int kRefitWaveClusters = 4;
 
 enum RefitStatus : int32_t {
   kRefitActive = 0,    // Gauss-Newton steps continue
@@ -721,47 +733,42 @@ __global__ __launch_bounds__(kRefitBlock) void cyl_refit_init_kernel(
   state[c] = S;
 }
 
-// One Gauss-Newton step (launch `step` of the refit: round step / iters,
-// iteration step % iters of each cluster's own schedule).
-__global__ __launch_bounds__(kRefitBlock) void cyl_refit_step_kernel(
-    const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
-    const CylFrame* __restrict__ frames, const int4* __restrict__ rchunks,
-    const int4* __restrict__ rclus, RefitState* __restrict__ state, uint32_t* __restrict__ member,
-    Comp2* __restrict__ partial, int32_t* __restrict__ pcount, int step, float thr,
-    const int32_t* __restrict__ bcnt) {
-  __shared__ RefitRed<kRefitTerms> red;
-  __shared__ int cnt_w[kRefitBlock / 64];
-  if (bcnt && (int)blockIdx.x >= bcnt[2]) return;
-  const int4 ck = rchunks[blockIdx.x];
-  const int c = ck.x;
-  if (c < 0) return;
-  const RefitState S = state[c];
-  if (S.status != kRefitActive) return;
-  if (step >= S.rounds * S.iters) return;
-  const int round = step / S.iters, it = step % S.iters;
-  if (S.skip_round == round) return;
-  const bool reselect = round > 0 && it == 0;
-  const int off = offsets[c];
-  const float* P = xyz + 3 * (size_t)off;
-  const CylFrame f = frames[c];
-  const int4 cl = rclus[c];
-  uint32_t* mem = member + cl.z;
+// One Gauss-Newton step's partials of the chunk [first, end) of a cluster
+// (points P, frame f, model S, membership bits mem), by all threads of the
+// workgroup: the chunk's (sum, error) pairs into out[kRefitTerms] and its
+// inlier count into *out_count.  In a re-selecting step the membership bits
+// of the chunk are rewritten from the model.
+__device__ inline void refit_step_chunk(const float* __restrict__ P, const CylFrame& f,
+                                        const RefitState& S, uint32_t* __restrict__ mem,
+                                        int first, int end, bool reselect, float thr,
+                                        RefitRed<kRefitTerms>& red, int* cnt_w, Comp2* out,
+                                        int32_t* out_count) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int end = ck.y + ck.z;
   float g[kRefitTerms];
 #pragma unroll
   for (int q = 0; q < kRefitTerms; ++q) g[q] = 0.0f;
   int cnt = 0;
+  // every point of the thread loaded before any is used (the loads overlap)
+  float px[kRefitPerThread][3];
+#pragma unroll
+  for (int j = 0; j < kRefitPerThread; ++j) {
+    const int i = first + kRefitBlock * j + (int)threadIdx.x;
+    if (i < end) {
+      const float* q = P + 3 * (size_t)i;
+      px[j][0] = q[0];
+      px[j][1] = q[1];
+      px[j][2] = q[2];
+    }
+  }
 #pragma unroll 2
   for (int j = 0; j < kRefitPerThread; ++j) {
-    const int gbase = ck.y + kRefitBlock * j + 64 * wave;
+    const int gbase = first + kRefitBlock * j + 64 * wave;
     const int i = gbase + lane;
     bool in = false;
     float d = 0.f, hh = 0.f, rho = 0.f, w[3] = {0.f, 0.f, 0.f};
     if (i < end) {
-      const float* q = P + 3 * (size_t)i;
-      const float x[3] = {q[0] - f.m[0], q[1] - f.m[1], q[2] - f.m[2]};
+      const float x[3] = {px[j][0] - f.m[0], px[j][1] - f.m[1], px[j][2] - f.m[2]};
       d = refit_dist(S, x, w, hh, rho);
       in = reselect ? (d < thr && d > -thr) : ((mem[i >> 5] >> (i & 31)) & 1u) != 0u;
     }
@@ -794,40 +801,29 @@ __global__ __launch_bounds__(kRefitBlock) void cyl_refit_step_kernel(
       }
     }
   }
-  const int chunk_id = cl.x + ck.w;
-  chunk_partial<kRefitTerms>(g, red, partial + (size_t)chunk_id * kRefitTerms);
+  chunk_partial<kRefitTerms>(g, red, out);
   const int bc = block_sum_int(cnt, cnt_w);
-  if (threadIdx.x == 0) pcount[chunk_id] = bc;
+  if (threadIdx.x == 0) *out_count = bc;
 }
 
-// The Gauss-Newton step's solve, one thread per cluster, after the step's
-// chunk partials: the chunks' (sum, error) pairs added in chunk order, the
-// FP64 5x5 normal equations, the next model.
-__global__ __launch_bounds__(64) void cyl_refit_solve_kernel(
-    int n_clusters, const int4* __restrict__ rclus, RefitState* __restrict__ state,
-    const Comp2* __restrict__ partial, const int32_t* __restrict__ pcount, int step,
-    const int32_t* __restrict__ bcnt) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  if (c >= n_clusters || (bcnt && c >= bcnt[0])) return;
-  const RefitState S = state[c];
-  if (S.status != kRefitActive) return;
-  if (step >= S.rounds * S.iters) return;
-  const int round = step / S.iters;
-  if (S.skip_round == round) return;
-  const int4 cl = rclus[c];
-  int tot_cnt = 0;
-  for (int k = 0; k < cl.y; ++k) tot_cnt += pcount[cl.x + k];
+// Term q of a step's sums over a cluster's nck chunks: the chunks' (sum,
+// error) pairs of term q added in chunk order, then sum + error in FP64.
+__device__ inline double refit_term(const Comp2* __restrict__ partial, int nck, int q) {
+  Comp2 t = {0.0f, 0.0f};
+  for (int k = 0; k < nck; ++k) comp_add(t, partial[(size_t)k * kRefitTerms + q]);
+  return (double)t.s + (double)t.e;
+}
+
+// The Gauss-Newton step's solve of one cluster (one thread) from its summed
+// terms gs and inlier count: the FP64 5x5 normal equations, the next model in
+// S.  Too few inliers stop the cluster; a failed factorisation skips the
+// rest of the round.
+__device__ inline void refit_solve_terms(RefitState& S, const double (&gs)[kRefitTerms],
+                                         int tot_cnt, int round) {
   if (tot_cnt < 5) {
-    state[c].status = kRefitStopped;
+    S.status = kRefitStopped;
     return;
   }
-  double gs[kRefitTerms];
-  for (int q = 0; q < kRefitTerms; ++q) {
-    Comp2 t = {0.0f, 0.0f};
-    for (int k = 0; k < cl.y; ++k) comp_add(t, partial[(size_t)(cl.x + k) * kRefitTerms + q]);
-    gs[q] = (double)t.s + (double)t.e;
-  }
-  RefitState N = S;
   double A[5][5], rhs[5], dl[5];
   {
     int t = 0;
@@ -843,7 +839,7 @@ __global__ __launch_bounds__(64) void cyl_refit_solve_kernel(
     for (int a = 0; a < 5; ++a) rhs[a] = -gs[15 + a];
   }
   if (!cholesky_solve<5>(A, rhs, dl)) {
-    state[c].skip_round = round;
+    S.skip_round = round;
     return;
   }
   double e1[3], e2[3];
@@ -852,6 +848,7 @@ __global__ __launch_bounds__(64) void cyl_refit_solve_kernel(
 #pragma unroll
   for (int i = 0; i < 3; ++i) na[i] = S.a[i] + dl[0] * e1[i] + dl[1] * e2[i];
   const double nl = sqrt(na[0] * na[0] + na[1] * na[1] + na[2] * na[2]);
+  RefitState N = S;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     N.a[i] = na[i] / nl;
@@ -860,10 +857,153 @@ __global__ __launch_bounds__(64) void cyl_refit_solve_kernel(
   N.r = S.r + dl[4];
   if (!(N.r > 0.0)) N.status = kRefitStopped;
   refit_model_f32(N);
-  state[c] = N;
+  S = N;
 }
 
-// Final inliers under the final model, their centroid, and the output record.
+// Sum of one int per lane over a wave (every lane gets it).
+__device__ inline int wave_sum_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// The solve by one wave: lane q < kRefitTerms sums term q over the chunks
+// (the loads of all terms in flight at once), lane 0 solves; S is lane 0's.
+__device__ inline void refit_solve_wave(RefitState& S, const Comp2* __restrict__ partial,
+                                        const int32_t* __restrict__ pcount, int nck, int round) {
+  const int lane = threadIdx.x & 63;
+  const double mine = lane < kRefitTerms ? refit_term(partial, nck, lane) : 0.0;
+  int cnt = 0;
+  for (int k = lane; k < nck; k += 64) cnt += pcount[k];
+  cnt = wave_sum_int(cnt);
+  double gs[kRefitTerms];
+#pragma unroll
+  for (int q = 0; q < kRefitTerms; ++q) gs[q] = __shfl(mine, q, 64);
+  if (lane == 0) refit_solve_terms(S, gs, cnt, round);
+}
+
+// The final pass over the chunk [first, end): the chunk sums of the final
+// inliers' coordinates (frame-relative) into out[3] and their count.
+__device__ inline void refit_final_chunk(const float* __restrict__ P, const CylFrame& f,
+                                         const RefitState& S, int first, int end, float thr,
+                                         RefitRed<3>& red, int* cnt_w, Comp2* out,
+                                         int32_t* out_count) {
+  const bool pos = S.r > 0.0;
+  float g[3] = {0.0f, 0.0f, 0.0f};
+  int cnt = 0;
+  if (pos) {
+    float px[kRefitPerThread][3];
+#pragma unroll
+    for (int j = 0; j < kRefitPerThread; ++j) {
+      const int i = first + kRefitBlock * j + (int)threadIdx.x;
+      if (i < end) {
+        const float* q = P + 3 * (size_t)i;
+        px[j][0] = q[0];
+        px[j][1] = q[1];
+        px[j][2] = q[2];
+      }
+    }
+#pragma unroll 2
+    for (int j = 0; j < kRefitPerThread; ++j) {
+      const int i = first + kRefitBlock * j + (int)threadIdx.x;
+      if (i < end) {
+        const float x[3] = {px[j][0] - f.m[0], px[j][1] - f.m[1], px[j][2] - f.m[2]};
+        float w[3], hh, rho;
+        const float d = refit_dist(S, x, w, hh, rho);
+        if (d < thr && d > -thr) {
+          g[0] = g[0] + x[0];
+          g[1] = g[1] + x[1];
+          g[2] = g[2] + x[2];
+          ++cnt;
+        }
+      }
+    }
+  }
+  chunk_partial<3>(g, red, out);
+  const int bc = block_sum_int(cnt, cnt_w);
+  if (threadIdx.x == 0) *out_count = bc;
+}
+
+// The output record of one cluster by one wave from its final pass's chunk
+// partials: lanes 0..2 sum the coordinates over the chunks, lane 0 writes
+// root (the final inliers' centroid projected on the axis), ray, radius,
+// inliers; invalid without a positive radius or three inliers.
+__device__ inline void refit_finish_wave(const RefitState& S, const CylFrame& f,
+                                         const Comp2* __restrict__ partial,
+                                         const int32_t* __restrict__ pcount, int nck,
+                                         double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const double mine = lane < 3 ? refit_term(partial, nck, lane) : 0.0;
+  int tot_cnt = 0;
+  for (int k = lane; k < nck; k += 64) tot_cnt += pcount[k];
+  tot_cnt = wave_sum_int(tot_cnt);
+  const double sum[3] = {__shfl(mine, 0, 64), __shfl(mine, 1, 64), __shfl(mine, 2, 64)};
+  if (lane != 0) return;
+  const double k = (double)tot_cnt;
+  if (!(S.r > 0.0) || tot_cnt < 3) {
+    write_invalid(out, k);
+    return;
+  }
+  double mean[3];
+  for (int i = 0; i < 3; ++i) mean[i] = sum[i] / k;
+  const double ht =
+      (mean[0] - S.c[0]) * S.a[0] + (mean[1] - S.c[1]) * S.a[1] + (mean[2] - S.c[2]) * S.a[2];
+  double ray[3] = {S.a[0], S.a[1], S.a[2]};
+  sl_sign_normalise(ray);
+  for (int i = 0; i < 3; ++i) {
+    out[i] = (double)f.m[i] + S.c[i] + ht * S.a[i];
+    out[3 + i] = ray[i];
+  }
+  out[6] = S.r;
+  out[7] = k;
+}
+
+// One Gauss-Newton step (launch `step` of the refit: round step / iters,
+// iteration step % iters of each cluster's own schedule), one workgroup per
+// chunk.
+__global__ __launch_bounds__(kRefitBlock) void cyl_refit_step_kernel(
+    const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
+    const CylFrame* __restrict__ frames, const int4* __restrict__ rchunks,
+    const int4* __restrict__ rclus, const RefitState* __restrict__ state,
+    uint32_t* __restrict__ member, Comp2* __restrict__ partial, int32_t* __restrict__ pcount,
+    int step, float thr, const int32_t* __restrict__ bcnt) {
+  __shared__ RefitRed<kRefitTerms> red;
+  __shared__ int cnt_w[kRefitBlock / 64];
+  if (bcnt && (int)blockIdx.x >= bcnt[2]) return;
+  const int4 ck = rchunks[blockIdx.x];
+  const int c = ck.x;
+  if (c < 0) return;
+  const RefitState S = state[c];
+  if (S.status != kRefitActive) return;
+  if (step >= S.rounds * S.iters) return;
+  const int round = step / S.iters, it = step % S.iters;
+  if (S.skip_round == round) return;
+  const int4 cl = rclus[c];
+  const int chunk_id = cl.x + ck.w;
+  refit_step_chunk(xyz + 3 * (size_t)offsets[c], frames[c], S, member + cl.z, ck.y, ck.y + ck.z,
+                   round > 0 && it == 0, thr, red, cnt_w,
+                   partial + (size_t)chunk_id * kRefitTerms, pcount + chunk_id);
+}
+
+// The Gauss-Newton step's solve after the step's chunk partials, one wave
+// per cluster (kRefitWaveClusters per workgroup).
+__global__ __launch_bounds__(64 * kRefitWaveClusters) void cyl_refit_solve_kernel(
+    int n_clusters, const int4* __restrict__ rclus, RefitState* __restrict__ state,
+    const Comp2* __restrict__ partial, const int32_t* __restrict__ pcount, int step,
+    const int32_t* __restrict__ bcnt) {
+  const int c = blockIdx.x * kRefitWaveClusters + (int)(threadIdx.x >> 6);
+  if (c >= n_clusters || (bcnt && c >= bcnt[0])) return;
+  RefitState S = state[c];
+  if (S.status != kRefitActive) return;
+  if (step >= S.rounds * S.iters) return;
+  const int round = step / S.iters;
+  if (S.skip_round == round) return;
+  const int4 cl = rclus[c];
+  refit_solve_wave(S, partial + (size_t)cl.x * kRefitTerms, pcount + cl.x, cl.y, round);
+  if ((threadIdx.x & 63) == 0) state[c] = S;
+}
+
+// Final inliers under the final model and their centroid's chunk sums.
 __global__ __launch_bounds__(kRefitBlock) void cyl_refit_final_kernel(
     const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
     const CylFrame* __restrict__ frames, const int4* __restrict__ rchunks,
@@ -878,75 +1018,87 @@ __global__ __launch_bounds__(kRefitBlock) void cyl_refit_final_kernel(
   if (c < 0) return;
   const RefitState S = state[c];
   if (S.status == kRefitInvalid || S.status == kRefitInactive) return;
-  const int off = offsets[c];
-  const float* P = xyz + 3 * (size_t)off;
-  const CylFrame f = frames[c];
   const int4 cl = rclus[c];
-  const int end = ck.y + ck.z;
-  const bool pos = S.r > 0.0;
-  float g[3] = {0.0f, 0.0f, 0.0f};
-  int cnt = 0;
-  if (pos) {
-#pragma unroll 2
-    for (int j = 0; j < kRefitPerThread; ++j) {
-      const int i = ck.y + kRefitBlock * j + (int)threadIdx.x;
-      if (i < end) {
-        const float* q = P + 3 * (size_t)i;
-        const float x[3] = {q[0] - f.m[0], q[1] - f.m[1], q[2] - f.m[2]};
-        float w[3], hh, rho;
-        const float d = refit_dist(S, x, w, hh, rho);
-        if (d < thr && d > -thr) {
-          g[0] = g[0] + x[0];
-          g[1] = g[1] + x[1];
-          g[2] = g[2] + x[2];
-          ++cnt;
-        }
-      }
-    }
-  }
   const int chunk_id = cl.x + ck.w;
-  chunk_partial<3>(g, red, partial + (size_t)chunk_id * kRefitTerms);
-  const int bc = block_sum_int(cnt, cnt_w);
-  if (threadIdx.x == 0) pcount[chunk_id] = bc;
+  refit_final_chunk(xyz + 3 * (size_t)offsets[c], frames[c], S, ck.y, ck.y + ck.z, thr, red,
+                    cnt_w, partial + (size_t)chunk_id * kRefitTerms, pcount + chunk_id);
 }
 
-// The output record of each cluster from the final pass's chunk partials
-// (one thread per cluster).
-__global__ __launch_bounds__(64) void cyl_refit_finish_kernel(
+// The output record of each cluster from the final pass's chunk partials,
+// one wave per cluster (kRefitWaveClusters per workgroup).
+__global__ __launch_bounds__(64 * kRefitWaveClusters) void cyl_refit_finish_kernel(
     int n_clusters, const CylFrame* __restrict__ frames, const int4* __restrict__ rclus,
     const RefitState* __restrict__ state, const Comp2* __restrict__ partial,
     const int32_t* __restrict__ pcount, const int32_t* __restrict__ bcnt,
     double* __restrict__ model) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
+  const int c = blockIdx.x * kRefitWaveClusters + (int)(threadIdx.x >> 6);
   if (c >= n_clusters || (bcnt && c >= bcnt[0])) return;
   const RefitState S = state[c];
   if (S.status == kRefitInvalid || S.status == kRefitInactive) return;
   const int4 cl = rclus[c];
-  int tot_cnt = 0;
-  for (int k = 0; k < cl.y; ++k) tot_cnt += pcount[cl.x + k];
-  double* out = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
-  const double k = (double)tot_cnt;
-  if (!(S.r > 0.0) || tot_cnt < 3) {
-    write_invalid(out, k);
-    return;
-  }
-  double mean[3];
-  for (int i = 0; i < 3; ++i) {
-    Comp2 t = {0.0f, 0.0f};
-    for (int j = 0; j < cl.y; ++j) comp_add(t, partial[(size_t)(cl.x + j) * kRefitTerms + i]);
-    mean[i] = ((double)t.s + (double)t.e) / k;
-  }
+  refit_finish_wave(S, frames[c], partial + (size_t)cl.x * kRefitTerms, pcount + cl.x, cl.y,
+                    model + (size_t)c * SLOAM_CYL_MODEL_STRIDE);
+}
+
+// Every Gauss-Newton step, the final pass and the output record of one
+// cluster in one workgroup: the workgroup of the cluster's first refit chunk
+// (the others exit) walks the cluster's chunks in order at every step, keeps
+// the chunk partials and counts in LDS and solves on its first thread
+// between steps.  Every sum is the same function over the same chunk as in
+// the launch-per-step sequence, so the records carry the same bits, without
+// its launches and per-cluster solve kernels.  Clusters of up to
+// kRefitFusedChunks chunks (every plan's bound).
+__global__ __launch_bounds__(kRefitBlock) void cyl_refit_fused_kernel(
+    const float* __restrict__ xyz, const int32_t* __restrict__ offsets,
+    const CylFrame* __restrict__ frames, const int4* __restrict__ rchunks,
+    const int4* __restrict__ rclus, RefitState* __restrict__ state, uint32_t* __restrict__ member,
+    float thr, const int32_t* __restrict__ bcnt, double* __restrict__ model) {
+  __shared__ RefitRed<kRefitTerms> red;
+  __shared__ RefitRed<3> red3;
+  __shared__ int cnt_w[kRefitBlock / 64];
+  __shared__ Comp2 part[kRefitFusedChunks * kRefitTerms];
+  __shared__ int32_t pc[kRefitFusedChunks];
+  __shared__ RefitState s_next;
+  if (bcnt && (int)blockIdx.x >= bcnt[2]) return;
+  const int4 ck = rchunks[blockIdx.x];
+  const int c = ck.x;
+  if (c < 0 || ck.w != 0) return;  // one workgroup per cluster
+  RefitState S = state[c];
+  if (S.status == kRefitInvalid || S.status == kRefitInactive) return;
+  const int4 cl = rclus[c];
+  if (cl.y > kRefitFusedChunks) return;  // excluded by the plans' cluster bound
+  const int off = offsets[c];
+  const int n = offsets[c + 1] - off;
+  const float* P = xyz + 3 * (size_t)off;
   const CylFrame f = frames[c];
-  const double ht =
-      (mean[0] - S.c[0]) * S.a[0] + (mean[1] - S.c[1]) * S.a[1] + (mean[2] - S.c[2]) * S.a[2];
-  double ray[3] = {S.a[0], S.a[1], S.a[2]};
-  sl_sign_normalise(ray);
-  for (int i = 0; i < 3; ++i) {
-    out[i] = (double)f.m[i] + S.c[i] + ht * S.a[i];
-    out[3 + i] = ray[i];
+  uint32_t* mem = member + cl.z;
+  const int steps = S.rounds * S.iters;
+  for (int step = 0; step < steps && S.status == kRefitActive; ++step) {
+    const int round = step / S.iters, it = step % S.iters;
+    if (S.skip_round == round) continue;
+    for (int k = 0; k < cl.y; ++k) {
+      const int first = k * kRefitChunk;
+      refit_step_chunk(P, f, S, mem, first, first + min(kRefitChunk, n - first),
+                       round > 0 && it == 0, thr, red, cnt_w, part + k * kRefitTerms, pc + k);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      RefitState N = S;
+      refit_solve_wave(N, part, pc, cl.y, round);
+      if (threadIdx.x == 0) s_next = N;
+    }
+    __syncthreads();
+    S = s_next;
   }
-  out[6] = S.r;
-  out[7] = k;
+  for (int k = 0; k < cl.y; ++k) {
+    const int first = k * kRefitChunk;
+    refit_final_chunk(P, f, S, first, first + min(kRefitChunk, n - first), thr, red3, cnt_w,
+                      part + k * kRefitTerms, pc + k);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) state[c] = S;
+  if (threadIdx.x < 64)
+    refit_finish_wave(S, f, part, pc, cl.y, model + (size_t)c * SLOAM_CYL_MODEL_STRIDE);
 }
 
 }  // namespace
@@ -981,6 +1133,9 @@ struct sloam_cyl_plan {
   // sizes, d_cnt the batch's own {clusters, work items, refit chunks,
   // overflow}
   bool counted = false;
+  // refit in one workgroup per cluster (cyl_refit_fused) instead of one
+  // launch per Gauss-Newton step
+  bool fused = true;
   const int32_t* d_n = nullptr;
   int64_t cap_points = 0;
   int cpc = 0;  // work items per cluster
@@ -999,8 +1154,9 @@ struct sloam_cyl_plan {
 
 static int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// Launches the refit of plan p: the init pass, one launch per Gauss-Newton
-// step (as many as the longest schedule among the clusters), the final pass.
+// Launches the refit of plan p: the init pass, then either the fused refit
+// (one workgroup per cluster) or one launch per Gauss-Newton step (as many as
+// the longest schedule among the clusters) and the final pass.
 // alt_flags (nullable) marks the clusters that take their best hypothesis
 // from alt_counts (alt_n_hyp hypotheses of sample stream alt_tag, no
 // proposal) and refit with alt_rounds x alt_iters steps, instead of counts.
@@ -1023,7 +1179,16 @@ static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
                      d_active, alt_flags, alt_counts, alt_n_hyp, alt_tag, alt_rounds, alt_iters,
                      cnt, p->d_state, p->d_member, d_mask, d_best, d_model);
   SLOAM_HIP(ctx, hipGetLastError());
-  const dim3 cgrid((p->n_clusters + 63) / 64), cblock(64);
+  if (p->fused) {
+    hipLaunchKernelGGL(cyl_refit_fused_kernel, grid, block, 0, ctx->stream, d_xyz, d_offsets,
+                       p->d_frames, p->d_rchunks, p->d_rclus, p->d_state, p->d_member, cfg.thr,
+                       cnt, d_model);
+    SLOAM_HIP(ctx, hipGetLastError());
+    return SLOAM_OK;
+  }
+  // one wave per cluster
+  const dim3 cgrid((std::max(p->n_clusters, 1) + kRefitWaveClusters - 1) / kRefitWaveClusters),
+      cblock(64 * kRefitWaveClusters);
   int steps = cfg.refit_rounds * cfg.refit_iters;
   if (alt_flags) steps = std::max(steps, alt_rounds * alt_iters);
   for (int s = 0; s < steps; ++s) {
@@ -1040,7 +1205,8 @@ static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
                      p->d_pcount, cfg.thr, cnt);
   SLOAM_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(cyl_refit_finish_kernel, cgrid, cblock, 0, ctx->stream, p->n_clusters,
-                     p->d_frames, p->d_rclus, p->d_state, p->d_partial, p->d_pcount, cnt, d_model);
+                     p->d_frames, p->d_rclus, p->d_state, p->d_partial, p->d_pcount, cnt,
+                     d_model);
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
 }
@@ -1075,6 +1241,7 @@ __global__ __launch_bounds__(256) void cyl_layout_kernel(
       len = offsets[c + 1] - offsets[c];
       nck = max(1, (len + kRefitChunk - 1) / kRefitChunk);
       nw = 2 * ((len + 63) / 64);
+      if (len > kRefitMaxPoints || len < 0) s_over = 1;  // the host plans' bound
     }
     // inclusive wave scans, then the waves' totals
     int sc = nck, sw = nw;
@@ -1200,7 +1367,7 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
   for (int i = 0; i < n_clusters; ++i) {
     const int64_t len = (int64_t)h_offsets[i + 1] - h_offsets[i];
     if (len < 0) return SLOAM_ERR_INVALID_ARG;
-    if (len > (1 << 17)) return SLOAM_ERR_TOO_LARGE;  // integer-exact covariance bound
+    if (len > kRefitMaxPoints) return SLOAM_ERR_TOO_LARGE;  // integer-exact covariance bound
   }
   p->n_clusters = n_clusters;
   p->n_points = h_offsets[n_clusters];
@@ -1269,6 +1436,9 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
     for (int x = 0; x < n_xcd; ++x)
       rchunks.push_back(k < rbins[x].size() ? rbins[x][k] : make_int4(-1, 0, 0, 0));
   p->n_rchunks = (int)rchunks.size();
+  int max_chunks = 0;
+  for (int c = 0; c < n_clusters; ++c) max_chunks = std::max(max_chunks, rclus[c].y);
+  p->fused = n_clusters <= kRefitFusedPlanClusters && max_chunks <= kRefitFusedPlanChunks;
   const size_t wb = sizeof(int4) * work.size(), cb = sizeof(int4) * rchunks.size(),
                lb = sizeof(int4) * rclus.size();
   int st = grow(ctx, &p->d_tables, &p->cap_tables, wb + cb + lb, 1);
@@ -1276,7 +1446,8 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
   if (!st)
     st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)n_clusters * cfg.n_hyp,
               sizeof(int32_t));
-  if (!st) st = grow(ctx, (void**)&p->d_state, &p->cap_state, n_clusters, sizeof(RefitState));
+  if (!st)
+    st = grow(ctx, (void**)&p->d_state, &p->cap_state, n_clusters, sizeof(RefitState));
   if (!st) st = grow(ctx, (void**)&p->d_member, &p->cap_member, std::max<size_t>(n_words, 1),
                      sizeof(uint32_t));
   if (!st && (size_t)n_chunk_ids > p->cap_partial) {
@@ -1383,6 +1554,7 @@ int sloam_cyl_plan_create_counted(sloam_ctx* ctx, int32_t cap_clusters, int64_t 
   sloam_cyl_plan* p = plan_new(ctx, cfg, &st);
   if (!p) return st;
   p->counted = true;
+  p->fused = true;
   p->d_n = d_n_clusters;
   p->cap_points = cap_points;
   p->cpc = (cfg->n_hyp + p->hcap - 1) / p->hcap;
@@ -1397,12 +1569,13 @@ int sloam_cyl_plan_create_counted(sloam_ctx* ctx, int32_t cap_clusters, int64_t 
   if (!st)
     st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)cap_clusters * cfg->n_hyp,
               sizeof(int32_t));
-  if (!st) st = grow(ctx, (void**)&p->d_state, &p->cap_state, cap_clusters, sizeof(RefitState));
+  if (!st)
+    st = grow(ctx, (void**)&p->d_state, &p->cap_state, cap_clusters, sizeof(RefitState));
   if (!st) st = grow(ctx, (void**)&p->d_member, &p->cap_member, words, sizeof(uint32_t));
   if (!st)
     st = grow(ctx, (void**)&p->d_partial, &p->cap_partial, p->n_rchunks,
               sizeof(Comp2) * kRefitTerms);
-  if (!st) st = sloam_dev_alloc(ctx, (void**)&p->d_pcount, sizeof(int32_t) * p->n_rchunks);
+  if (!st) st = sloam_dev_alloc(ctx, (void**)&p->d_pcount, sizeof(int32_t) * p->cap_partial);
   if (!st) st = sloam_dev_alloc(ctx, (void**)&p->d_cnt, sizeof(int32_t) * 4);
   if (st) {
     sloam_cyl_plan_destroy(p);

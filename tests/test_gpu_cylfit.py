@@ -84,6 +84,27 @@ def test_edge_clusters(ctx):
     _compare(g, o, off[-1])
 
 
+def test_large_cluster_takes_per_step_refit(ctx):
+    """A batch holding a 40000-point cluster (20 refit chunks) is refitted
+    with one launch per Gauss-Newton step; without it, the same small batch
+    takes the fused refit (one workgroup per cluster: at most 512 clusters of
+    at most 4 chunks).  Both give the oracle's records bit for bit (the
+    2000-cluster test covers the per-step form at full size)."""
+    rng = np.random.default_rng(12)
+    big, _ = synth.trunk_points(rng, 40000, np.array([4.0, -2.0, 0.0]), np.array([0, 0, 1.0]),
+                                0.3, 8.0, 0.02, 0.2)
+    d = synth.make_clusters(12, seed=13)
+    xyz = np.concatenate([d["xyz"], big.astype(np.float32)])
+    off = np.concatenate([d["offsets"], [d["offsets"][-1] + len(big)]]).astype(np.int32)
+    for n_c in (13, 12):
+        o_off = off[:n_c + 1]
+        o_xyz = xyz[:o_off[-1]]
+        _run_both({"xyz": o_xyz, "offsets": o_off}, 128)
+        g = fit_cylinders(ctx, o_xyz, o_off, n_hyp=128, counts=True, mask=True, best=True)
+        o = oracle.cyl_fit_batch(o_xyz, o_off, 128, threads=8)
+        _compare(g, o, o_off[-1])
+
+
 @pytest.mark.parametrize("hyp_chunk,tile", [(32, 64), (96, 512), (1024, 4096)])
 def test_launch_shapes_do_not_change_results(ctx, hyp_chunk, tile):
     d = synth.make_clusters(40, seed=9)
@@ -298,6 +319,21 @@ def test_counted_plan_matches_host_plan(ctx):
         k = _read_counts(ctx, plan)
         print(f"[cylfit] counted plan past its capacity: counts {k.tolist()}", flush=True)
         assert k[3] == 1 and k[0] == 0 and k[2] == 0
+        assert np.all(model.cpu().numpy() == 7.0)
+        # a cluster beyond 131072 points (the host plans' bound): overflow
+        rng = np.random.default_rng(44)
+        big, _ = synth.trunk_points(rng, 140000, np.array([3.0, 0.0, 0.0]),
+                                    np.array([0, 0, 1.0]), 0.3, 8.0, 0.02, 0.2)
+        x = torch.as_tensor(big.astype(np.float32)).to(dev)
+        off = torch.as_tensor(np.array([0, len(big)], np.int32)).to(dev)
+        d_n.fill_(1)
+        torch.cuda.synchronize()
+        check(ctx.lib.sloam_cyl_plan_exec(plan, _ptr(x), _ptr(off), None, None, None, None,
+                                          _ptr(model)), ctx.handle)
+        ctx.sync()
+        k = _read_counts(ctx, plan)
+        print(f"[cylfit] counted plan, a 140000-point cluster: counts {k.tolist()}", flush=True)
+        assert k[3] == 1 and k[0] == 0
         assert np.all(model.cpu().numpy() == 7.0)
     finally:
         ctx.lib.sloam_cyl_plan_destroy(plan)
