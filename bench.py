@@ -403,9 +403,30 @@ def bench_learned(ctx, args):
     ref = pout["model"].cpu().numpy()
     lm = lout["model"].cpu().numpy()
     nf = int(lout["n_fallback"].cpu().numpy()[0])
+    # the plain fit's own spread: the same clusters with another hypothesis
+    # stream (another seed), against the first
+    plain2 = CylinderFitPlan(ctx, d["offsets"], n_hyp=args.hyp, stream_tag=1)
+    pout2 = plain2.alloc_outputs(mask=False, best=False)
+    plain2.exec(xyz, off, pout2)
+    ctx.sync()
+    ref2 = pout2["model"].cpu().numpy()
+    plain2.close()
     bins = bench_learned_size_bins(ctx, args, net, timed)
     ok = (ref[:, 6] > 0) & (lm[:, 6] > 0)
     err = np.abs(lm[ok, 6] - ref[ok, 6]) * 100
+    ok2 = (ref[:, 6] > 0) & (ref2[:, 6] > 0)
+    err2 = np.abs(ref2[ok2, 6] - ref[ok2, 6]) * 100
+
+    def vs_truth(m):
+        v = m[:, 6] > 0
+        e = np.abs(m[v, 6] - d["radius"][v]) * 100
+        return {"median": float(np.median(e)), "p90": float(np.percentile(e, 90)),
+                "frac_over_1cm": float(np.mean(e > 1.0)), "valid": int(v.sum())}
+
+    def by_size(e, okm):
+        sz = n[okm]
+        return {f"{lo}-{hi - 1}": float(np.mean(e[(sz >= lo) & (sz < hi)] > 1.0))
+                for lo, hi in ((50, 256), (256, 1024), (1024, 5001))}
     fwd_s = kern["pn_fwd"] / 1e3
     plain.close()
     fit.close()
@@ -438,7 +459,15 @@ def bench_learned(ctx, args):
         "fallback_fraction": nf / S,
         "radius_diff_vs_plain_cm": {"median": float(np.median(err)),
                                     "p90": float(np.percentile(err, 90)),
-                                    "frac_over_1cm": float(np.mean(err > 1.0))},
+                                    "frac_over_1cm": float(np.mean(err > 1.0)),
+                                    "frac_over_1cm_by_size": by_size(err, ok)},
+        # the floor of the comparison: plain against plain with another seed
+        "plain_vs_plain_other_seed_cm": {"median": float(np.median(err2)),
+                                         "p90": float(np.percentile(err2, 90)),
+                                         "frac_over_1cm": float(np.mean(err2 > 1.0)),
+                                         "frac_over_1cm_by_size": by_size(err2, ok2)},
+        "radius_err_vs_truth_cm": {"learned": vs_truth(lm), "plain": vs_truth(ref),
+                                   "plain_other_seed": vs_truth(ref2)},
         "kernel_ms_per_batch": kern,
         "size_bins": bins,
         "cpu_baseline": cpu,
