@@ -744,7 +744,7 @@ def bench_keyframe_stream(args):
                                   "estimator_p50_ms", "tree_fit_p50_ms", "trees_mean",
                                   "planes_mean", "success_rate", "discarded", "trees_fitted",
                                   "trees_fallback", "drift", "map_p50_ms",
-                                  "latency_with_map_ms")}
+                                  "latency_with_map_ms", "odometry_drift", "steps", "params")}
 
     g = summary(res["learned"])
     g["keyframes"] = n

@@ -45,21 +45,11 @@ struct sloam_ctx {
   std::vector<int> scratch_pins;
   // slots requested since the current capture began (bit per slot)
   uint32_t capture_slots = 0;
-  // auxiliary streams (default priority) and their fork / join events, for
-  // independent launches that overlap; created on first use
-  hipStream_t aux[2] = {nullptr, nullptr};
-  hipEvent_t fork_ev = nullptr;
-  hipEvent_t join_ev[2] = {nullptr, nullptr};
   // sloam_graph_begin .. sloam_graph_end: the stream is being captured (no
   // synchronisation or allocation may happen); timing is off meanwhile
   bool capturing = false;
   bool timing_before_capture = false;
 };
-
-// Makes aux stream i wait for the work enqueued so far on ctx->stream.
-int sloam_ctx_fork(sloam_ctx* ctx, int i);
-// Makes ctx->stream wait for the work enqueued so far on aux stream i.
-int sloam_ctx_join(sloam_ctx* ctx, int i);
 
 // Scratch buffer `slot` (0..31) of at least `bytes` bytes (reallocated when
 // too small; the previous contents are not kept).  Callers own a slot for the

@@ -137,23 +137,6 @@ extern "C" int sloam_xyz_from_xyzi(sloam_ctx* ctx, const float* d_xyzi, int64_t 
   return SLOAM_OK;
 }
 
-int sloam_ctx_fork(sloam_ctx* ctx, int i) {
-  if (!ctx->aux[i]) {
-    SLOAM_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));
-    SLOAM_HIP(ctx, hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
-  }
-  if (!ctx->fork_ev) SLOAM_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-  SLOAM_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
-  SLOAM_HIP(ctx, hipStreamWaitEvent(ctx->aux[i], ctx->fork_ev, 0));
-  return SLOAM_OK;
-}
-
-int sloam_ctx_join(sloam_ctx* ctx, int i) {
-  SLOAM_HIP(ctx, hipEventRecord(ctx->join_ev[i], ctx->aux[i]));
-  SLOAM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0));
-  return SLOAM_OK;
-}
-
 extern "C" {
 
 int sloam_ctx_create(int device, sloam_ctx** out) {
@@ -187,14 +170,6 @@ void sloam_ctx_destroy(sloam_ctx* ctx) {
   // graphs still alive lose their HIP objects and their context
   for (sloam_graph* g : ctx->graphs) graph_release(g);
   ctx->graphs.clear();
-  for (int i = 0; i < 2; ++i) {
-    if (ctx->aux[i]) {
-      (void)hipStreamSynchronize(ctx->aux[i]);
-      (void)hipStreamDestroy(ctx->aux[i]);
-    }
-    if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
-  }
-  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   for (auto& kv : ctx->timers)
     for (auto& pr : kv.second.pending) {
       (void)hipEventDestroy(pr.first);
