@@ -397,7 +397,7 @@ def bench_learned(ctx, args):
     ms_two_pass = timed(lambda: fit.exec_two_pass(xyz, off, lout2))
     kern = {}
     for name in ("pn_prep", "pn_fwd", "pn_head", "cyl_seed", "cyl_score", "cyl_decide",
-                 "cyl_refit"):
+                 "cyl_refit", "cyl_rescue"):
         ms, k = ctx.kernel_time(name)
         kern[name] = ms / args.steps  # per batch (some kernels launch twice per batch)
     ref = pout["model"].cpu().numpy()
@@ -727,17 +727,20 @@ def bench_keyframe_stream(args):
         gen_s = time.perf_counter() - t0
         res, traj = {}, {}
         # name, tree model, pipelined, device estimator (the whole keyframe
-        # as one captured graph)
-        for name, model, pipelined, device in (("learned", "learned", 0, 0),
-                                               ("learned_pipelined", "learned", 1, 0),
-                                               ("plain", "plain", 0, 0),
-                                               ("learned_device", "learned", 0, 1),
-                                               ("plain_device", "plain", 0, 1)):
+        # as one captured graph), shadow model and options: learned against
+        # plain one keyframe at a time, and plain against plain with another
+        # hypothesis stream (the plain fit's own spread)
+        for name, model, pipelined, device, shadow, more in (
+                ("learned", "learned", 0, 0, "plain", ()),
+                ("learned_pipelined", "learned", 1, 0, None, ()),
+                ("plain", "plain", 0, 0, None, ()),
+                ("learned_device", "learned", 0, 1, None, ("--device-estimator",)),
+                ("plain_device", "plain", 0, 1, None, ("--device-estimator",)),
+                ("plain_seed", "plain", 0, 0, "plain", ("--shadow-seed", "0x1234"))):
             print(f"[bench] long keyframe stream: {name}", file=sys.stderr, flush=True)
             res[name], traj[name] = run_keyframe_bench(
                 path, 2, model, 1, pipelined, os.path.join(d, f"traj_{name}.bin"),
-                shadow="plain" if name == "learned" else None,
-                extra=KF_PARAMS + (("--device-estimator",) if device else ()))
+                shadow=shadow, extra=KF_PARAMS + more)
 
     def summary(r):
         return {k: r[k] for k in ("latency_ms", "throughput_kf_per_s", "stage_p50_ms",
@@ -750,6 +753,9 @@ def bench_keyframe_stream(args):
     g["keyframes"] = n
     g["drift_vs_plain"] = traj_diff(traj["learned"], traj["plain"])
     g["step_vs_plain"] = res["learned"]["shadow"]
+    # the floor of that comparison: plain against plain with another
+    # hypothesis stream, one keyframe at a time the same way
+    g["step_plain_vs_plain_other_seed"] = res["plain_seed"]["shadow"]
     g["pipelined"] = {"throughput_kf_per_s": res["learned_pipelined"]["throughput_kf_per_s"],
                       "latency_ms": res["learned_pipelined"]["latency_ms"],
                       "trajectory_equal": bool(np.array_equal(traj["learned_pipelined"],
@@ -777,7 +783,8 @@ def bench_keyframe_stream(args):
                      f"the true poses, drift_vs_plain against the plain trajectory "
                      f"(two maps built apart), step_vs_plain the per-keyframe pose "
                      f"difference of a plain runner fed the learned run's estimate and "
-                     f"submap (first pass); "
+                     f"submap (first pass), step_plain_vs_plain_other_seed the same for "
+                     f"a plain runner against plain with another hypothesis stream; "
                      f"sequence generated on the host in {gen_s:.1f} s (not timed)")
     return g
 
@@ -1012,6 +1019,8 @@ def main():
          "trajectory_equal_to_host": head["device"]["learned"]["trajectory_equal_to_host"]},
         "keyframe_drift_vs_plain": None if head is None else head["drift_vs_plain"],
         "keyframe_step_vs_plain": None if head is None else head["step_vs_plain"],
+        "keyframe_step_plain_vs_plain_other_seed": None if head is None else
+        head.get("step_plain_vs_plain_other_seed"),
         "keyframe_headline": head_name,
         "small_keyframe": small,
         "pose": pose,

@@ -200,7 +200,10 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* plan, const float* d_xyz, const int32
  * d_n_fallback, nullable) and scored with the `plain` plan's hypotheses
  * (no proposal); one LO refit of every cluster from its best hypothesis of
  * the set it was scored with, with that plan's refit_rounds / refit_iters.
- * A flagged cluster's model equals the plain plan's own fit of it, an
+ * Then a rescue: a cluster the verify stage kept whose refitted model is
+ * invalid or keeps fewer than min_inlier_ratio * n inliers is flagged too
+ * (d_fallback, d_n_fallback), scored with the plain hypotheses and refitted
+ * again.  A flagged cluster's model equals the plain plan's own fit of it, an
  * unflagged one the verify plan's.  Both plans must be built on the same
  * offsets with the same thr, max_radius and seed.  d_counts_verify /
  * d_counts_plain (nullable: plan scratch) hold n_clusters x n_hyp counts of

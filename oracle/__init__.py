@@ -586,8 +586,10 @@ def learned_fit(params, xyz, offsets, n_hyp=16, plain_hyp=1024, thr=0.05, max_ra
     sloam_cyl_learned_exec): proposals of the network in FP64, n_hyp
     verifying hypotheses with the proposal as hypothesis 0 and LO refit;
     the clusters whose best verifying hypothesis holds fewer than
-    min_inlier_ratio x n points are fitted again by the plain n_hyp-hypothesis
-    method."""
+    min_inlier_ratio x n points are fitted again by the plain
+    plain_hyp-hypothesis method, and so are those (n >= 3) whose verified
+    model is invalid or keeps fewer than min_inlier_ratio x n inliers after
+    its refit (the rescue of sloam_cyl_learned_exec)."""
     off = np.ascontiguousarray(offsets, np.int32)
     prop = pn_forward(params, xyz, off, emulate=False)["proposal"].astype(np.float32)
     v = cyl_fit_batch(xyz, off, n_hyp, thr=thr, max_radius=max_radius, seed=seed, tag=1,
@@ -595,7 +597,10 @@ def learned_fit(params, xyz, offsets, n_hyp=16, plain_hyp=1024, thr=0.05, max_ra
     n = np.diff(off)
     S = len(n)
     best_count = v["counts"][np.arange(S), v["best"]]
-    fb = np.nonzero(best_count < min_inlier_ratio * n)[0]
+    decided = (n >= 3) & (best_count < min_inlier_ratio * n)
+    vm = v["model"]
+    rescued = ~decided & (n >= 3) & ~((vm[:, 6] > 0) & (vm[:, 7] >= min_inlier_ratio * n))
+    fb = np.nonzero(decided | rescued)[0]
     model = v["model"].copy()
     if len(fb):
         pts = np.concatenate([xyz[off[c]:off[c + 1]] for c in fb])

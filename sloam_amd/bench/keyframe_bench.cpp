@@ -225,6 +225,9 @@ int main(int argc, char** argv) {
   std::vector<std::string> pos;
   std::string model_name = "lines", weights, traj_out, shadow_name, dump_path;
   double max_tree_radius = -1.0, tree_match_thresh = -1.0;
+  // --shadow-seed: the shadow's hypothesis key (another stream of the same
+  // tree model measures that model's spread against itself)
+  unsigned long long shadow_seed = 0;
   bool log = false, pinned = false, pipelined = false, device_est = false, compare_device = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -242,6 +245,8 @@ int main(int argc, char** argv) {
       (a == "--max-tree-radius" ? max_tree_radius : tree_match_thresh) = std::atof(argv[++i]);
     } else if (a == "--dump" && i + 1 < argc) {
       dump_path = argv[++i];
+    } else if (a == "--shadow-seed" && i + 1 < argc) {
+      shadow_seed = std::strtoull(argv[++i], nullptr, 0);
     } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out" ||
                 a == "--shadow") &&
                i + 1 < argc) {
@@ -356,6 +361,9 @@ int main(int argc, char** argv) {
   std::vector<std::array<double, 12>> sh_rows;
   // radius differences of the trees both runners kept (paired by tree id)
   std::vector<double> sh_dr;
+  // trees kept by one runner and rejected by the other, per rejecting rule
+  // (Cylinder::RejectReason bits 0..6; [7]: not among the other's rejects)
+  std::array<int, 8> flips_first_rejects{}, flips_shadow_rejects{};
   // first pass: RunSloam calls after the first, and how many of them ran the
   // XY-yaw (tree) and Z-roll-pitch (ground) steps
   int step_calls = 0, tree_steps = 0, ground_steps = 0;
@@ -378,6 +386,7 @@ int main(int argc, char** argv) {
     if (use_shadow) {
       shadow.reset(new KeyframeRunner(H, W, fm, use_graph, 22.5f, -22.5f, false));
       shadow->setTreeModel(shadow_model, weights);
+      if (shadow_seed) shadow->setHypothesisSeed(shadow_seed);
       if (compare_device) shadow->setDeviceEstimator(true);
     }
     for (int pass = 0; pass < passes; ++pass) {
@@ -512,6 +521,27 @@ int main(int argc, char** argv) {
               };
               const SloamStepInfo a = kf.lastStep();
               const SloamStepInfo b = shadow->lastStep();
+              // trees one runner keeps and the other rejects, by the rules
+              // that reject them
+              auto tally = [](const SloamOutput& keeper, const SloamOutput& other,
+                              const std::vector<std::pair<int, int>>& other_rejected,
+                              std::array<int, 8>& counts) {
+                for (const Cylinder& c : keeper.tm) {
+                  bool kept = false;
+                  for (const Cylinder& o : other.tm) kept = kept || o.id == c.id;
+                  if (kept) continue;
+                  int reason = -1;
+                  for (const auto& r : other_rejected)
+                    if ((size_t)r.first == c.id) reason = r.second;
+                  if (reason < 0) {
+                    ++counts[7];
+                    continue;
+                  }
+                  for (int bit = 0; bit < 7; ++bit) counts[bit] += (reason >> bit) & 1;
+                }
+              };
+              tally(out, so, shadow->estimator().lastRejectedTrees(), flips_shadow_rejects);
+              tally(so, out, kf.estimator().lastRejectedTrees(), flips_first_rejects);
               for (const Cylinder& a_c : out.tm)
                 for (const Cylinder& b_c : so.tm)
                   if (a_c.id == b_c.id) {
@@ -701,6 +731,18 @@ int main(int argc, char** argv) {
     stats(same_m, same_deg);
     int over = 0;
     for (double x : sh_dr) over += x > 0.01 ? 1 : 0;
+    static const char* kRules[8] = {"out_of_range",     "no_model",   "root_off_ground",
+                                    "root_at_origin",   "radius_not_positive",
+                                    "radius_cap",       "axis_tilt",  "unexplained"};
+    auto flips = [&](const std::array<int, 8>& c) {
+      std::printf("{");
+      for (int i = 0; i < 8; ++i) std::printf("%s\"%s\": %d", i ? ", " : "", kRules[i], c[i]);
+      std::printf("}");
+    };
+    std::printf("}, \"validity_flips\": {\"rejected_by_first\": ");
+    flips(flips_first_rejects);
+    std::printf(", \"rejected_by_shadow\": ");
+    flips(flips_shadow_rejects);
     std::printf(
         "}, \"tree_radius_diff_m\": {\"trees\": %zu, \"p50\": %.6f, \"p99\": %.6f, "
         "\"max\": %.6f, \"frac_over_1cm\": %.4f}}}\n",

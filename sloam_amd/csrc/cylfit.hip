@@ -1122,6 +1122,7 @@ struct sloam_cyl_plan {
   int4* d_rclus = nullptr;
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
+  int32_t* d_rescue = nullptr;  // per cluster: refitted again by the plain method
   RefitState* d_state = nullptr;
   uint32_t* d_member = nullptr;  // membership bits of the current round
   Comp2* d_partial = nullptr;    // per chunk: kRefitTerms (sum, error) pairs
@@ -1145,7 +1146,7 @@ struct sloam_cyl_plan {
   // across sloam_cyl_plan_reset
   void* d_tables = nullptr;
   size_t cap_tables = 0, cap_frames = 0, cap_counts = 0, cap_state = 0, cap_member = 0,
-         cap_partial = 0;
+         cap_partial = 0, cap_rescue = 0;
   // pinned staging of the tables (bytes): their uploads are asynchronous on
   // the context stream, ahead of the runs that read them
   void* h_stage = nullptr;
@@ -1164,7 +1165,7 @@ static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
                         const int32_t* counts, const float* d_proposal, const int32_t* d_active,
                         const int32_t* alt_flags, const int32_t* alt_counts, int alt_n_hyp,
                         uint32_t alt_tag, int alt_rounds, int alt_iters, uint32_t* d_mask,
-                        int32_t* d_best, double* d_model) {
+                        int32_t* d_best, double* d_model, bool force_fused = false) {
   sloam_ctx* ctx = p->ctx;
   const sloam_cyl_fit_cfg& cfg = p->cfg;
   if (p->n_rchunks == 0) return SLOAM_OK;
@@ -1179,7 +1180,7 @@ static int launch_refit(sloam_cyl_plan* p, const float* d_xyz, const int32_t* d_
                      d_active, alt_flags, alt_counts, alt_n_hyp, alt_tag, alt_rounds, alt_iters,
                      cnt, p->d_state, p->d_member, d_mask, d_best, d_model);
   SLOAM_HIP(ctx, hipGetLastError());
-  if (p->fused) {
+  if (p->fused || force_fused) {
     hipLaunchKernelGGL(cyl_refit_fused_kernel, grid, block, 0, ctx->stream, d_xyz, d_offsets,
                        p->d_frames, p->d_rchunks, p->d_rclus, p->d_state, p->d_member, cfg.thr,
                        cnt, d_model);
@@ -1344,6 +1345,33 @@ __global__ __launch_bounds__(256) void cyl_fallback_kernel(const int32_t* __rest
   if ((threadIdx.x & 63) == 0 && b && n_active) atomicAdd(n_active, __builtin_popcountll(b));
 }
 
+// Rescue flags of the learned fit, after its refit: a cluster (n >= 3) that
+// the verify stage kept is fitted again by the plain method when its refitted
+// model is invalid or keeps fewer than min_ratio * n inliers; it joins the
+// fallback flags (and their count).
+__global__ __launch_bounds__(256) void cyl_rescue_kernel(const int32_t* __restrict__ offsets,
+                                                         int n_clusters,
+                                                         const double* __restrict__ model,
+                                                         float min_ratio,
+                                                         const int32_t* __restrict__ bcnt,
+                                                         int32_t* __restrict__ fallback,
+                                                         int32_t* __restrict__ rescue,
+                                                         int32_t* __restrict__ n_flagged) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  bool a = false;
+  if (c < n_clusters) {
+    if (!bcnt || c < bcnt[0]) {
+      const int n = offsets[c + 1] - offsets[c];
+      const double* m = model + (size_t)c * SLOAM_CYL_MODEL_STRIDE;
+      a = fallback[c] == 0 && n >= 3 && !(m[6] > 0.0 && m[7] >= (double)min_ratio * (double)n);
+      if (a) fallback[c] = 1;
+    }
+    rescue[c] = a ? 1 : 0;
+  }
+  const uint64_t b = __builtin_amdgcn_ballot_w64(a);
+  if ((threadIdx.x & 63) == 0 && b && n_flagged) atomicAdd(n_flagged, __builtin_popcountll(b));
+}
+
 // Grows the device buffer *ptr (element size elem) to hold n elements;
 // contents are not kept.
 static int grow(sloam_ctx* ctx, void** ptr, size_t* cap, size_t n, size_t elem) {
@@ -1450,6 +1478,7 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
     st = grow(ctx, (void**)&p->d_state, &p->cap_state, n_clusters, sizeof(RefitState));
   if (!st) st = grow(ctx, (void**)&p->d_member, &p->cap_member, std::max<size_t>(n_words, 1),
                      sizeof(uint32_t));
+  if (!st) st = grow(ctx, (void**)&p->d_rescue, &p->cap_rescue, n_clusters, sizeof(int32_t));
   if (!st && (size_t)n_chunk_ids > p->cap_partial) {
     sloam_dev_free(p->d_pcount);
     p->d_pcount = nullptr;
@@ -1572,6 +1601,7 @@ int sloam_cyl_plan_create_counted(sloam_ctx* ctx, int32_t cap_clusters, int64_t 
   if (!st)
     st = grow(ctx, (void**)&p->d_state, &p->cap_state, cap_clusters, sizeof(RefitState));
   if (!st) st = grow(ctx, (void**)&p->d_member, &p->cap_member, words, sizeof(uint32_t));
+  if (!st) st = grow(ctx, (void**)&p->d_rescue, &p->cap_rescue, cap_clusters, sizeof(int32_t));
   if (!st)
     st = grow(ctx, (void**)&p->d_partial, &p->cap_partial, p->n_rchunks,
               sizeof(Comp2) * kRefitTerms);
@@ -1708,6 +1738,32 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
                                 pc.refit_iters, nullptr, d_best, d_model);
     if (st) return st;
   }
+  {
+    // rescue: the verified clusters whose refit failed or kept fewer than
+    // min_inlier_ratio * n inliers take the plain method's fit after all
+    // (scored with the plain hypotheses in the same seed frames, refitted
+    // alone in the fused form)
+    SloamKernelScope ks(ctx, "cyl_rescue");
+    hipLaunchKernelGGL(cyl_rescue_kernel, dim3((S + 255) / 256), dim3(256), 0, ctx->stream,
+                       d_offsets, S, d_model, min_inlier_ratio, vcnt, d_fallback,
+                       verify->d_rescue, d_n_fallback);
+    SLOAM_HIP(ctx, hipGetLastError());
+    if (plain->n_work > 0) {
+      const size_t lds =
+          (size_t)plain->hcap * (sizeof(float4) + sizeof(int)) + 3 * sizeof(float) * plain->tile;
+      hipLaunchKernelGGL(cyl_score_kernel, dim3(plain->n_work), dim3(kBlock), lds, ctx->stream,
+                         d_xyz, d_offsets, verify->d_frames, plain->d_work, pc.n_hyp,
+                         plain->hcap, plain->tile, pc.thr, pc.max_radius, k0, k1,
+                         (uint32_t)pc.stream_tag, nullptr, verify->d_rescue,
+                         plain->counted ? plain->d_cnt : nullptr, cp, nullptr);
+      SLOAM_HIP(ctx, hipGetLastError());
+    }
+    const int st = launch_refit(verify, d_xyz, d_offsets, cv, nullptr, verify->d_rescue,
+                                verify->d_rescue, cp, pc.n_hyp, (uint32_t)pc.stream_tag,
+                                pc.refit_rounds, pc.refit_iters, nullptr, d_best, d_model,
+                                /*force_fused=*/true);
+    if (st) return st;
+  }
   return SLOAM_OK;
 }
 
@@ -1795,6 +1851,7 @@ void sloam_cyl_plan_destroy(sloam_cyl_plan* p) {
   sloam_dev_free(p->d_member);
   sloam_dev_free(p->d_partial);
   sloam_dev_free(p->d_pcount);
+  sloam_dev_free(p->d_rescue);
   sloam_dev_free(p->d_cnt);
   delete p;
 }

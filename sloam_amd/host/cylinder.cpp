@@ -67,14 +67,23 @@ void Cylinder::build(const std::vector<TreeVertex>& vertices, const Plane& gplan
                      const FeatureModelParams& fmParams, const LineFit* fit,
                      const CylinderFit* cfit) {
   isValid = false;
+  rejectReason = kRejectNoModel;
   if (vertices.size() < 3) return;  // the reference indexes vertices [1] and [2]
   if (withinRange(vertices, gplane, fmParams)) {
     computeModel(vertices, fmParams.defaultTreeRadius, fmParams.featuresPerTree, fit, cfit);
     const bool validZ = groundBasedRoot(gplane);
     const bool validNorm = model.root.norm() > 0.01;
     const bool validRadius = model.radius > 0.0;
-    const bool validTree = filter(fmParams.maxTreeRadius, fmParams.maxAxisTheta, gplane);
+    const int fails = filterFails(fmParams.maxTreeRadius, fmParams.maxAxisTheta, gplane);
+    const bool validTree = fails == 0;
     isValid = validZ && validRadius && validTree && validNorm;
+    rejectReason = (validZ ? 0 : kRejectRootOffGround) | (validNorm ? 0 : kRejectRootAtOrigin) |
+                   (validRadius ? 0 : kRejectRadiusNotPositive) |
+                   ((fails & sloam_geom::kCylNoModel) ? kRejectNoModel : 0) |
+                   ((fails & sloam_geom::kCylRadiusCap) ? kRejectRadiusCap : 0) |
+                   ((fails & sloam_geom::kCylAxisTilt) ? kRejectAxisTilt : 0);
+  } else {
+    rejectReason = kRejectOutOfRange;
   }
 }
 
@@ -95,9 +104,10 @@ bool Cylinder::groundBasedRoot(const Plane& gplane) {
   return near;
 }
 
-bool Cylinder::filter(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta, const Plane& gplane) {
+int Cylinder::filterFails(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta,
+                          const Plane& gplane) {
   const sloam_geom::CylModel m{model.root, model.ray, model.radius};
-  return sloam_geom::cyl_filter(m, maxTreeRadius, maxAxisTheta, gplane.model.plane);
+  return sloam_geom::cyl_filter_fails(m, maxTreeRadius, maxAxisTheta, gplane.model.plane);
 }
 
 void Cylinder::computeModel(const std::vector<TreeVertex>& landmarkVtxs,

@@ -71,11 +71,29 @@ class Cylinder : public SemanticObject<CylinderParameters> {
   // the sample-consensus line settings of computeModel (cylinder.cpp:116-124)
   static sloam_line_fit_cfg lineFitCfg();
 
+  // Why construction left the tree invalid, as bits (0 when valid): vertex [1]
+  // out of range, no model (line or cylinder fit failed, tree too short), root
+  // more than 2 m from the ground plane, root at the origin, radius not
+  // positive, radius at or over the cap, axis tilted past maxAxisTheta.  No
+  // counterpart in the reference, which keeps only isValid.
+  enum RejectReason : int {
+    kRejectOutOfRange = 1,
+    kRejectNoModel = 2,
+    kRejectRootOffGround = 4,
+    kRejectRootAtOrigin = 8,
+    kRejectRadiusNotPositive = 16,
+    kRejectRadiusCap = 32,
+    kRejectAxisTilt = 64,
+  };
+  int rejectReason = 0;
+
  private:
   // exactly one of fit / cfit is given
   void build(const std::vector<TreeVertex>& vertices, const Plane& gplane,
              const FeatureModelParams& fmParams, const LineFit* fit, const CylinderFit* cfit);
-  bool filter(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta, const Plane& gplane);
+  // the rules of filter (cylinder.cpp:57-73) the model fails
+  // (sloam_geom::CylFilterFail bits; 0: it passes)
+  int filterFails(const Scalar& maxTreeRadius, const Scalar& maxAxisTheta, const Plane& gplane);
   void computeModel(const std::vector<TreeVertex>& landmarkVtxs, const Scalar defaultTreeRadius,
                     const Scalar featuresPerTree, const LineFit* fit, const CylinderFit* cfit);
   bool groundBasedRoot(const Plane& gplane);

@@ -119,14 +119,26 @@ SLOAM_HD inline bool cyl_ground_based_root(const Vector4& g, const Vector3& g_ce
   return false;
 }
 
+// The rules of Cylinder::filter a model fails (bits; 0 when it passes).
+enum CylFilterFail : int { kCylNoModel = 1, kCylRadiusCap = 2, kCylAxisTilt = 4 };
+
 // Cylinder::filter (cylinder.cpp:57-73): radius below the cap and the axis
-// within maxAxisTheta degrees of the ground normal (either direction)
-SLOAM_HD inline bool cyl_filter(const CylModel& m, double maxTreeRadius, double maxAxisTheta,
-                                const Vector4& g) {
-  if (m.radius == -1) return false;
+// within maxAxisTheta degrees of the ground normal (either direction); the
+// failing rules
+SLOAM_HD inline int cyl_filter_fails(const CylModel& m, double maxTreeRadius,
+                                     double maxAxisTheta, const Vector4& g) {
+  if (m.radius == -1) return kCylNoModel;
   const Vector3 up = g.head3();
   const double theta = (180 / PIDEF) * sloam_dm::acos(m.ray.dot(up) / (m.ray.norm() * up.norm()));
-  return (m.radius < maxTreeRadius) && ((theta <= maxAxisTheta) || (theta >= 180 - maxAxisTheta));
+  int fails = 0;
+  if (!(m.radius < maxTreeRadius)) fails |= kCylRadiusCap;
+  if (!((theta <= maxAxisTheta) || (theta >= 180 - maxAxisTheta))) fails |= kCylAxisTilt;
+  return fails;
+}
+
+SLOAM_HD inline bool cyl_filter(const CylModel& m, double maxTreeRadius, double maxAxisTheta,
+                                const Vector4& g) {
+  return cyl_filter_fails(m, maxTreeRadius, maxAxisTheta, g) == 0;
 }
 
 // The validity the Cylinder constructor leaves (cylinder.cpp:15-27) once the

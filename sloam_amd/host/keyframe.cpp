@@ -143,7 +143,7 @@ void KeyframeRunner::fitTreesLaunch(const Slot& sl, int nt) {
   cfg.max_radius = 1.0f;
   cfg.refit_iters = 2;
   cfg.refit_rounds = 3;
-  cfg.seed = 0x5EED;
+  cfg.seed = fit_seed_;
   cfg.stream_tag = 0;
   cfg.hyp_chunk = 128;
   cfg.tile_points = 1024;
@@ -204,6 +204,12 @@ void KeyframeRunner::fitTreesLaunch(const Slot& sl, int nt) {
         "download models");
 }
 
+void KeyframeRunner::setHypothesisSeed(uint64_t seed) {
+  if (plain_ || verify_ || plain_c_ || verify_c_)
+    throw std::runtime_error("keyframe: the hypothesis seed is set before the first keyframe");
+  fit_seed_ = seed;
+}
+
 void KeyframeRunner::setDeviceEstimator(bool on) {
   if (on && mode_ == TreeModel::MedianLines)
     throw std::runtime_error("keyframe: the device keyframe serves the cylinder-fit tree models");
@@ -250,7 +256,7 @@ void KeyframeRunner::setDeviceEstimator(bool on) {
     cfg.max_radius = 1.0f;
     cfg.refit_iters = 2;
     cfg.refit_rounds = 3;
-    cfg.seed = 0x5EED;
+    cfg.seed = fit_seed_;
     cfg.stream_tag = 0;
     cfg.hyp_chunk = 128;
     cfg.tile_points = 1024;

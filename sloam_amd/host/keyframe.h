@@ -121,6 +121,11 @@ class KeyframeRunner {
   // network's parameters (float32 .npy of SLOAM_PN_PARAMS values).
   void setTreeModel(TreeModel mode, const std::string& weights_npy = std::string());
   TreeModel treeModel() const { return mode_; }
+  // Key of the cylinder fits' hypothesis draws (default 0x5EED), before the
+  // first keyframe: another key draws another hypothesis stream, e.g. to
+  // measure how far the plain fit differs from itself.  No counterpart in
+  // the reference.
+  void setHypothesisSeed(uint64_t seed);
 
  private:
   // grow-only device buffer
@@ -171,6 +176,7 @@ class KeyframeRunner {
   FeatureModelParams fm_;
   sloam::sloam sloam_;
   TreeModel mode_ = TreeModel::MedianLines;
+  uint64_t fit_seed_ = 0x5EED;
   InputRecorder recorder_;
   sloam_cyl_plan* plain_ = nullptr;
   sloam_cyl_plan* verify_ = nullptr;

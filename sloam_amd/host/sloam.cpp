@@ -379,6 +379,7 @@ void sloam::binGroundPoints(const SE3 pose, const VectorType& points, GroundGrid
 
 void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
                           std::vector<Plane>& planes) {
+  rejected_.clear();
   // ground: all cells of the scan in one call (binned at the origin,
   // sloam.cpp:390-393), unless the caller brings them
   std::vector<Plane> fittedCells;
@@ -439,7 +440,10 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
       const CylinderFit none;
       Cylinder c(in.landmarks[k], planes[best[k]], fmParams_,
                  set_of[k] >= 0 ? in.treeCylinderFits[k] : none);
-      if (c.isValid) landmarks.push_back(std::move(c));
+      if (c.isValid)
+        landmarks.push_back(std::move(c));
+      else
+        rejected_.emplace_back(in.landmarks[k][2].treeId, c.rejectReason);
       continue;
     }
     LineFit fit;
@@ -454,7 +458,10 @@ void sloam::computeModels(SloamInput& in, std::vector<Cylinder>& landmarks,
       }
     }
     Cylinder c(in.landmarks[k], planes[best[k]], fmParams_, fit);
-    if (c.isValid) landmarks.push_back(std::move(c));
+    if (c.isValid)
+      landmarks.push_back(std::move(c));
+    else
+      rejected_.emplace_back(in.landmarks[k][2].treeId, c.rejectReason);
   }
 }
 

@@ -125,6 +125,9 @@ class sloam {
   CloudT getPrevGroundFeatures();
   const SloamPhaseTimes& lastPhaseTimes() const { return phase_; }
   const SloamStepInfo& lastStep() const { return step_; }
+  // The trees the last RunSloam call modelled but rejected: tree id and
+  // Cylinder::RejectReason bits.  No counterpart in the reference.
+  const std::vector<std::pair<int, int>>& lastRejectedTrees() const { return rejected_; }
 
  private:
   // ground binning / fit settings of fmParams_, binned around `pose`
@@ -139,6 +142,7 @@ class sloam {
   double minPlanes_ = 0.0;
   SloamPhaseTimes phase_;
   SloamStepInfo step_;
+  std::vector<std::pair<int, int>> rejected_;
 };
 
 }  // namespace sloam

@@ -136,8 +136,10 @@ class LearnedCylinderFit:
         """Enqueues the learned fit (asynchronous, sloam_cyl_learned_exec: one
         seed, the verifying hypotheses for every cluster, the plain ones for
         the clusters whose best verifying hypothesis holds fewer than
-        min_inlier_ratio x n inliers, one refit); out["active"] flags those
-        clusters and out["n_fallback"] counts them."""
+        min_inlier_ratio x n inliers, one refit; then the plain fit again for
+        the verified clusters whose refit failed or kept fewer than
+        min_inlier_ratio x n inliers); out["active"] flags the clusters that
+        took the plain fit and out["n_fallback"] counts them."""
         self.net.infer(xyz, offsets, out["proposal"])
         check(self.ctx.lib.sloam_cyl_learned_exec(
             self.verify.handle, self.plain.handle, _ptr(xyz), _ptr(offsets), _ptr(out["proposal"]),

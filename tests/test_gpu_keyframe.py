@@ -3,8 +3,11 @@ three tree models: the reference's median lines, the batched plain cylinder
 fit and the learned-proposal fit with its plain fallback.
 
 Bar: the learned trajectory within 1 cm / 0.1 deg of the plain one at every
-keyframe; the front end's HIP-graph replay gives the same trajectory as its
-eager launches, bit for bit."""
+keyframe of the short sequence; on the long stream, the learned fit within
+the bar as often as plain is against itself (another hypothesis stream),
+less 5 points, with every differing keyframe explained by a validity rule;
+the front end's HIP-graph replay gives the same trajectory as its eager
+launches, bit for bit."""
 import json
 import os
 import subprocess
@@ -76,6 +79,37 @@ def test_learned_vs_plain_one_keyframe(sequence, tmp_path):
     assert sh["m"]["max"] <= 0.01 and sh["deg"]["max"] <= 0.1
     rd = sh["tree_radius_diff_m"]
     assert rd["trees"] > 0 and rd["max"] <= 0.01
+
+
+def test_learned_vs_plain_on_the_long_stream(long_stream, tmp_path):
+    """200 keyframes of the long stream, one keyframe at a time (the shadow
+    starts every keyframe from the first runner's estimate, submap and
+    ground planes): learned against plain, and plain against plain with
+    another hypothesis stream, the plain fit's own spread.  Where both
+    runners keep the same trees and take the same XY-yaw decision, the
+    poses are equal bit for bit; every keyframe where they differ is
+    explained by a validity rule that one fit passes and the other fails
+    (rejected trees logged by rule); and the learned fit holds the
+    1 cm / 0.1 deg bar on at least as many keyframes as plain holds against
+    itself, less 5 points."""
+    params = ("--max-tree-radius", "0.45", "--tree-match-thresh", "1.0")
+    learned, _ = _run(long_stream, tmp_path, "learned", extra=("--shadow", "plain", *params))
+    plain, _ = _run(long_stream, tmp_path, "plain",
+                    extra=("--shadow", "plain", "--shadow-seed", "0x1234", *params))
+    for name, r in (("learned vs plain", learned), ("plain vs plain, other stream", plain)):
+        sh = r["shadow"]
+        print(f"[keyframe] long stream, {name}: within 1 cm / 0.1 deg "
+              f"{sh['within_1cm_0p1deg']}, same decisions {sh['same_decisions']['compared']} "
+              f"of {sh['compared']}, p99 {sh['m']['p99']} m {sh['deg']['p99']} deg; validity "
+              f"flips {sh['validity_flips']}", flush=True)
+        assert sh["compared"] >= 190 and sh["success_differs"] == 0
+        assert sh["same_decisions"]["m"]["max"] == 0.0
+        assert sh["same_decisions"]["deg"]["max"] == 0.0
+        for side in ("rejected_by_first", "rejected_by_shadow"):
+            assert sh["validity_flips"][side]["unexplained"] == 0
+    lw = learned["shadow"]["within_1cm_0p1deg"]
+    pw = plain["shadow"]["within_1cm_0p1deg"]
+    assert lw >= pw - 0.05 and lw >= 0.85
 
 
 def test_graph_replay_matches_eager(sequence, tmp_path):

@@ -16,7 +16,6 @@ same LM kernel."""
 import json
 import os
 import subprocess
-import sys
 
 import numpy as np
 import pytest
@@ -52,20 +51,6 @@ def _run(seq, tmp, model, extra=(), tag=""):
     assert r.returncode == 0, r.stderr[-2000:]
     res = json.loads(r.stdout.strip().splitlines()[-1])
     return res, np.fromfile(traj, dtype=np.float64).reshape(-1, 7)
-
-
-@pytest.fixture(scope="module")
-def long_stream(tmp_path_factory):
-    """The first 200 keyframes of the bench's long stream (0.2 m apart, a
-    260 m forest, odometry noise 0.0316 m / 0.316 deg per step), cast by
-    eight processes in a child (this process holds the GPU): keyframes whose
-    XY-yaw step is skipped compose the pose estimate's own yaw, the case
-    that needs the two sides' arc tangents to agree."""
-    path = str(tmp_path_factory.mktemp("kfdev_long") / "long.bin")
-    print("[kf-device] writing 200 keyframes of the long stream", flush=True)
-    subprocess.run([sys.executable, "-m", "sloam_amd.synth", path, "200", "0.2", "260",
-                    "0.0316227766", "0.316227766", "8", "128"], check=True, timeout=300, cwd=ROOT)
-    return path
 
 
 @pytest.mark.parametrize("which", ["short", "stream", "long"])
