@@ -187,6 +187,36 @@ __device__ inline bool same_row(const double* a, const double* b, int n) {
 // ---------------------------------------------------------------------------
 // computeModels, matchFeatures, the checks and the pose problem set
 // ---------------------------------------------------------------------------
+// The scan `best = 0, bestDist = init; for i < n: if (d(i) < bestDist)
+// {bestDist = d(i); best = i;}` by one wave: lane j scans i = j, j + 64, ...
+// and the lanes' (distance, index) pairs are combined by the least distance,
+// then the least index, which is the scan's choice.  Every lane returns best
+// and its distance.
+template <class Dist>
+__device__ inline int wave_scan_min(int n, double init, Dist dist, double* best_dist) {
+  const int lane = threadIdx.x & 63;
+  double bv = init;
+  int bi = -1;
+  for (int i = lane; i < n; i += 64) {
+    const double d = dist(i);
+    if (d < bv) {
+      bv = d;
+      bi = i;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(bv, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (oi >= 0 && (bi < 0 || ov < bv || (ov == bv && oi < bi))) {
+      bv = ov;
+      bi = oi;
+    }
+  }
+  *best_dist = bv;
+  return bi < 0 ? 0 : bi;
+}
+
 __global__ __launch_bounds__(kKB) void kf_models_kernel(KfBufs b, KfParamsDev P) {
   __shared__ int wcount[kKB / 64];
   __shared__ int base;
@@ -302,24 +332,23 @@ __global__ __launch_bounds__(kKB) void kf_models_kernel(KfBufs b, KfParamsDev P)
   }
   if (mode != 1) return;
 
-  // matchFeatures: trees against the map (sloam.cpp:477-480), planes against
-  // the previous planes with the fixed 1.0 threshold (sloam.cpp:482)
-  for (int l = tid; l < n_lm; l += kKB) {
+  // matchFeatures: trees against the map (sloam.cpp:477-480), one wave per
+  // landmark with the map trees across its lanes; planes against the previous
+  // planes with the fixed 1.0 threshold (sloam.cpp:482)
+  for (int l = tid >> 6; l < n_lm; l += kKB / 64) {
     const double* o = b.lm_model + 8 * (size_t)l;
     Vector3 pr, pd;
     sloam_geom::cyl_project(pose, Vector3(o[0], o[1], o[2]), Vector3(o[3], o[4], o[5]), pr, pd);
-    double bestDist = P.treeMatchThresh + 100;
-    int best = 0;
-    for (int i = 0; i < n_map; ++i) {
-      const double* m = b.map + 8 * (size_t)i;
-      const double d =
-          sloam_geom::cyl_model_distance(Vector3(m[0], m[1], m[2]), Vector3(m[3], m[4], m[5]), pr, pd);
-      if (d < bestDist) {
-        bestDist = d;
-        best = i;
-      }
-    }
-    b.lm_map[l] = bestDist < P.treeMatchThresh ? best : -1;
+    double bestDist;
+    const int best = wave_scan_min(
+        n_map, P.treeMatchThresh + 100,
+        [&](int i) {
+          const double* m = b.map + 8 * (size_t)i;
+          return sloam_geom::cyl_model_distance(Vector3(m[0], m[1], m[2]),
+                                                Vector3(m[3], m[4], m[5]), pr, pd);
+        },
+        &bestDist);
+    if ((tid & 63) == 0) b.lm_map[l] = bestDist < P.treeMatchThresh ? best : -1;
   }
   for (int p = tid; p < n_planes; p += kKB) {
     int match = -1;
@@ -591,26 +620,26 @@ __global__ __launch_bounds__(kKB) void kf_finish_kernel(KfBufs b, KfParamsDev P)
   const SE3 cur = load_pose(s_pose);
   const int n_lm = h[kHNLm], n_planes = h[kHNPlanes];
   const int n_map = min(max(*b.n_map, 0), b.cap_map);
-  // projectModels at the pose, then matchModels (sloam.cpp:299-328)
-  for (int l = tid; l < n_lm; l += kKB) {
+  // projectModels at the pose, then matchModels (sloam.cpp:299-328), one
+  // wave per landmark with the map trees across its lanes
+  for (int l = tid >> 6; l < n_lm; l += kKB / 64) {
     const double* o = b.lm_model + 8 * (size_t)l;
     Vector3 pr, pd;
     sloam_geom::cyl_project(cur, Vector3(o[0], o[1], o[2]), Vector3(o[3], o[4], o[5]), pr, pd);
     int match = -1;
     if (mode == 1 && n_map > 0) {
-      double bestDist = P.treeMatchThresh + 100;
-      int best = 0;
-      for (int i = 0; i < n_map; ++i) {
-        const double* m = b.map + 8 * (size_t)i;
-        const double d = sloam_geom::cyl_model_distance(Vector3(m[0], m[1], m[2]),
-                                                        Vector3(m[3], m[4], m[5]), pr, pd);
-        if (d < bestDist) {
-          bestDist = d;
-          best = i;
-        }
-      }
+      double bestDist;
+      const int best = wave_scan_min(
+          n_map, P.treeMatchThresh + 100,
+          [&](int i) {
+            const double* m = b.map + 8 * (size_t)i;
+            return sloam_geom::cyl_model_distance(Vector3(m[0], m[1], m[2]),
+                                                  Vector3(m[3], m[4], m[5]), pr, pd);
+          },
+          &bestDist);
       if (bestDist < P.addNewTreeThreshDist) match = best;
     }
+    if ((tid & 63) != 0) continue;
     double* ro = b.r_model + 8 * (size_t)l;
     for (int i = 0; i < 3; ++i) {
       ro[i] = pr[i];

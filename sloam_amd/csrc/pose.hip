@@ -330,7 +330,8 @@ __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e
   // the thread's residual loads are all issued before any arithmetic, and
   // each residual's landmark load one residual ahead of its use, so that a
   // wave keeps several loads in flight instead of waiting on one dependent
-  // pair per residual
+  // pair per residual (loading every landmark up front costs more in
+  // occupancy than it gains)
   float4 pv[kRpt];
 #pragma unroll
   for (int j = 0; j < kRpt; ++j) {
@@ -637,9 +638,19 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
   if (k >= n_problems) return;
   if (st[k].done) return;
   if (threadIdx.x < kSums) {
+    // the chunks' sums added in chunk order; loaded eight at a time so that
+    // the loads overlap
     double s = 0.0;
-    for (int wk = first_work[k]; wk < first_work[k + 1]; ++wk)
-      s += partials[(size_t)wk * kSums + threadIdx.x];
+    const int w0 = first_work[k], w1 = first_work[k + 1];
+    for (int wk = w0; wk < w1; wk += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        v[u] = wk + u < w1 ? partials[(size_t)(wk + u) * kSums + threadIdx.x] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (wk + u < w1) s += v[u];
+    }
     tot[threadIdx.x] = s;
   }
   __syncthreads();
