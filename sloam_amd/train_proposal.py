@@ -6,8 +6,11 @@ share drawn per chunk from --outliers), labelled by the plain method (1024-hypot
 sloam_cyl_plan) on the same device.  Optimiser: Adam on the FP32 master
 weights through sloam_pn_train_step, cosine-decayed step size.
 
-    python -m sloam_amd.train_proposal --chunks 60 --epochs 6 \\
+    python -m sloam_amd.train_proposal \\
         --out sloam_amd/weights/cylinder_proposal.npy
+
+(the defaults: 100 chunks of 2000 clusters = 200k clusters, 12 epochs; the
+committed weights come from this command, about 25 s on one MI355X)
 
 prints one progress line per chunk generated and every 100 steps."""
 import argparse
@@ -53,7 +56,7 @@ def main(argv=None):
     ap.add_argument("--chunks", type=int, default=100)
     ap.add_argument("--chunk-size", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=250)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-min", type=float, default=5e-5)
     ap.add_argument("--seed", type=int, default=13)
