@@ -246,9 +246,12 @@ int sloam_cyl_fallback(sloam_ctx* ctx, const int32_t* d_offsets, int32_t n_clust
  * d_status (nullable): 2 int32 per problem: termination (0 CONVERGENCE,
  *   1 NO_CONVERGENCE) and iterations; d_cost (nullable): 2 doubles per problem:
  *   initial and final cost 1/2 sum rho(r^2).
- * Residuals and Jacobians are FP32 (relative to the initial translation), the
- * normal equations are summed with error-free FP32 transformations and solved
- * in FP64.
+ * precision 0: residuals and Jacobians are FP32 (relative to the initial
+ *   translation), the normal equations are summed with error-free FP32
+ *   transformations and solved in FP64 (the throughput form);
+ * precision 1: residuals, Jacobians and sums are FP64, as Ceres evaluates them
+ *   (the keyframe solves use it: their termination tests then see Ceres'
+ *   costs).
  * parametrization 0 (OptimizePose, sloam.cpp:174-255): t + unit quaternion,
  *   rotation updated by left multiplication (Ceres' quaternion local
  *   parameterization); all six parameters free.
@@ -270,7 +273,7 @@ typedef struct sloam_pose_cfg {
   double initial_radius;         /* trust region (Ceres default 1e4)          */
   double min_relative_decrease;  /* step acceptance (Ceres default 1e-3)      */
   int32_t parametrization;       /* 0 quaternion / left, 1 angle-axis subset  */
-  int32_t reserved;
+  int32_t precision;             /* 0 FP32 compensated, 1 FP64 evaluation     */
 } sloam_pose_cfg;
 
 typedef struct sloam_pose_plan sloam_pose_plan;
@@ -625,7 +628,8 @@ typedef struct sloam_kf_result_view { /* pinned host memory */
    * landmarks, tree check, ground check, tree feature matches, ground feature
    * matches, tree solve index, ground solve index, solves, overflow, trees,
    * matched landmarks, matched planes, cylinder rows, plane rows, success,
-   * ran (1: outputs written) */
+   * ran (1: outputs written), LM iterations of the tree (or joint) solve and
+   * of the ground solve (-1: not run) */
   const int32_t* head;
   const double* pose;      /* 7: T_Map_Curr (t, q x y z w)                   */
   const double* delta;     /* 7: T_Delta                                     */

@@ -468,7 +468,7 @@ class SloamEstimator:
         i32 = lambda a: np.ascontiguousarray(a, np.int32)  # noqa: E731
         f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
         f32 = lambda a: np.ascontiguousarray(a, np.float32)  # noqa: E731
-        info = np.zeros(10, np.int32)
+        info = np.zeros(12, np.int32)
         pose = np.zeros(7)
         delta = np.zeros(7)
         tm_id = np.zeros(max(nt, 1), np.int32)
@@ -492,7 +492,8 @@ class SloamEstimator:
         return {"ok": bool(info[0]), "early": bool(info[1]), "tree_step": bool(info[2]),
                 "ground_step": bool(info[3]), "tree_matches": int(info[4]),
                 "plane_matches": int(info[5]), "first": bool(info[8]),
-                "converged": bool(info[9]), "pose": pose, "delta": delta,
+                "converged": bool(info[9]), "tree_iters": int(info[10]),
+                "ground_iters": int(info[11]), "pose": pose, "delta": delta,
                 "tm_id": tm_id[:k].copy(), "tm": tm[:k].copy(), "matches": matches[:k].copy(),
                 "tree_valid": valid[:nt].copy(), "planes": planes[:n_pl, :7].copy()}
 

@@ -384,7 +384,7 @@ void push(Problem& pr, const float p[3], int32_t id) {
 
 // one LM solve; mode 0: joint quaternion (all free), 1: angle-axis subset
 bool solve(const Problem& pr, const Pose& x0, int param, int32_t free_mask, double pose_out[7],
-           double params_out[6]) {
+           double params_out[6], int32_t* iterations) {
   const int32_t off[2] = {0, (int32_t)(pr.pts.size() / 4)};
   const int32_t cfg_i[3] = {50, 1, param};
   const double cfg_d[6] = {0.1, 1e-6, 1e-10, 1e-8, 1e4, 1e-3};
@@ -395,6 +395,7 @@ bool solve(const Problem& pr, const Pose& x0, int param, int32_t free_mask, doub
   oracle_pose_lm_batch(pr.pts.data(), off, 1, pr.cyl.empty() ? dummy : pr.cyl.data(),
                        pr.plane.empty() ? dummy : pr.plane.data(), pin, &free_mask, cfg_i, cfg_d,
                        pose_out, params_out, st, cost, 1);
+  *iterations = st[1];
   return st[0] == 0;
 }
 
@@ -462,9 +463,10 @@ int oracle_sloam_set_state(void* h, int first, int n_prev, const double* prev) {
 //          fits given (cyl_ok; cyl_fit: root[3], ray[3], radius)
 //   map: n_map cylinders, 7 doubles each (root, ray, radius)
 // Outputs:
-//   info[10]: returned value, early return (no state change), treeCheck,
+//   info[12]: returned value, early return (no state change), treeCheck,
 //          groundCheck, tree feature matches, plane feature matches, trees
-//          kept, planes kept, first scan, pose solve converged
+//          kept, planes kept, first scan, pose solve converged, LM iterations
+//          of the tree (or joint) solve and of the ground solve (-1: not run)
 //   pose_out[7] = T_Map_Curr, delta_out[7] = T_Delta
 //   tm_id / tm (7 doubles, projected) / matches per kept tree (capacity
 //          n_trees); tree_valid per input tree
@@ -474,12 +476,13 @@ int oracle_sloam_run(void* h, const double pose[7], int n_cells, const double* c
                      const int32_t* tree_off, const int32_t* vtx_int, const double* vtx_f64,
                      const int32_t* vtx_pt_off, const float* pts, int tree_mode,
                      const int32_t* line_int, const float* line_coeff, const int32_t* cyl_ok,
-                     const double* cyl_fit, int n_map, const double* map, int32_t info[10],
+                     const double* cyl_fit, int n_map, const double* map, int32_t info[12],
                      double pose_out[7], double delta_out[7], int32_t* tm_id, double* tm,
                      int32_t* matches, int32_t* tree_valid, double* planes_out) {
   State* s = static_cast<State*>(h);
   const Params& prm = s->p;
   for (int i = 0; i < 10; ++i) info[i] = 0;
+  info[10] = info[11] = -1;
   Pose est;
   std::memcpy(est.t, pose, 3 * sizeof(double));
   std::memcpy(est.q, pose + 3, 4 * sizeof(double));
@@ -662,7 +665,7 @@ int oracle_sloam_run(void* h, const double pose[7], int n_cells, const double* c
       Problem pr;
       tree_problem(pr);
       double po[7], pa[6];
-      const bool ok = solve(pr, est, 1, (1 << 0) | (1 << 1) | (1 << 5), po, pa);
+      const bool ok = solve(pr, est, 1, (1 << 0) | (1 << 1) | (1 << 5), po, pa, &info[10]);
       conv = conv && ok;
       if (ok) {
         treeOut[0] = pa[0];
@@ -674,7 +677,7 @@ int oracle_sloam_run(void* h, const double pose[7], int n_cells, const double* c
       Problem pr;
       ground_problem(pr);
       double po[7], pa[6];
-      const bool ok = solve(pr, est, 1, (1 << 2) | (1 << 3) | (1 << 4), po, pa);
+      const bool ok = solve(pr, est, 1, (1 << 2) | (1 << 3) | (1 << 4), po, pa, &info[11]);
       conv = conv && ok;
       if (ok) {
         groundOut[0] = pa[2];
@@ -694,7 +697,7 @@ int oracle_sloam_run(void* h, const double pose[7], int n_cells, const double* c
     tree_problem(pr);
     ground_problem(pr);
     double po[7], pa[6];
-    success = solve(pr, est, 0, 0x3f, po, pa);
+    success = solve(pr, est, 0, 0x3f, po, pa, &info[10]);
     info[9] = success;
     if (success) {
       std::memcpy(delta.t, po, 3 * sizeof(double));

@@ -116,7 +116,7 @@ class PoseCfg(C.Structure):
         ("initial_radius", C.c_double),
         ("min_relative_decrease", C.c_double),
         ("parametrization", C.c_int32),
-        ("reserved", C.c_int32),
+        ("precision", C.c_int32),
     ]
 
 class FrontendCfg(C.Structure):

@@ -206,6 +206,8 @@ struct DumpWriter {
     put((int32_t)(st.groundCheck ? 1 : 0));
     put((int32_t)st.treeMatches);
     put((int32_t)st.planeMatches);
+    put((int32_t)st.treeIters);
+    put((int32_t)st.groundIters);
     pose(out.T_Map_Curr);
     pose(out.T_Delta);
     put((int32_t)out.tm.size());
@@ -482,7 +484,9 @@ int main(int argc, char** argv) {
                 const SloamStepInfo a = kf.lastStep(), b = shadow->lastStep();
                 const bool step_eq = a.treeCheck == b.treeCheck && a.groundCheck == b.groundCheck &&
                                      a.treeMatches == b.treeMatches &&
-                                     a.planeMatches == b.planeMatches;
+                                     a.planeMatches == b.planeMatches &&
+                                     a.treeIters == b.treeIters &&
+                                     a.groundIters == b.groundIters;
                 dc_pose_eq += pose_eq ? 1 : 0;
                 dc_models_eq += models_eq ? 1 : 0;
                 dc_step_eq += step_eq ? 1 : 0;
@@ -496,11 +500,12 @@ int main(int argc, char** argv) {
                   std::fprintf(stderr,
                                "device differs at keyframe %d: pose %d (%.3e m %.3e deg), "
                                "landmarks %d (%zu / %zu), steps %d (tree %d/%d ground %d/%d "
-                               "matches %d/%d %d/%d)\n",
+                               "matches %d/%d %d/%d iterations %d/%d %d/%d)\n",
                                k, pose_eq ? 1 : 0, m, deg, models_eq ? 1 : 0, out.tm.size(),
                                so.tm.size(), step_eq ? 1 : 0, a.treeCheck, b.treeCheck,
                                a.groundCheck, b.groundCheck, a.treeMatches, b.treeMatches,
-                               a.planeMatches, b.planeMatches);
+                               a.planeMatches, b.planeMatches, a.treeIters, b.treeIters,
+                               a.groundIters, b.groundIters);
                 }
               } else if (ok != sok && dc_printed < 10) {
                 ++dc_printed;

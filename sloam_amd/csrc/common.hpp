@@ -70,15 +70,16 @@ int sloam_project_noalloc_init(sloam_ctx* ctx);
 // sloam_pose_plan_exec over a work list built on the device (d_work: int4
 // {problem, first residual, end residual, 0} per chunk of at most 2048
 // residuals, d_first: first chunk of each problem).  d_state holds
-// max_problems LM states (sloam_pose_state_bytes each); d_cylf / d_plf the
-// FP32 landmark tables (8 / 4 floats per cylinder / plane).
+// max_problems LM states (sloam_pose_state_bytes each); d_cyl_tab / d_pl_tab
+// the landmark tables (per cylinder / plane: 32 / 16 bytes at cfg->precision
+// 0, 64 / 32 bytes at precision 1).
 size_t sloam_pose_state_bytes();
 int sloam_pose_fused_device(sloam_ctx* ctx, int max_problems, const int32_t* d_n_problems,
                             const float* d_points, const int32_t* d_work, const int32_t* d_first,
                             const double* d_cyl, const int32_t* d_cyl_off, const double* d_planes,
                             const int32_t* d_pl_off, const double* d_pose_in,
                             const int32_t* d_free_mask, const sloam_pose_cfg* cfg, void* d_state,
-                            float* d_cylf, float* d_plf, double* d_pose_out,
+                            void* d_cyl_tab, void* d_pl_tab, double* d_pose_out,
                             double* d_params_out, int32_t* d_status, double* d_cost);
 
 // Records a HIP failure in the context and returns SLOAM_ERR_HIP.

@@ -57,6 +57,8 @@ enum : int {
   kHNPl,            // plane table rows
   kHSuccess,
   kHRan,            // RunSloam wrote its outputs (first scan or estimate)
+  kHTreeIters,      // LM iterations of the tree (or joint) solve, -1: not run
+  kHGroundIters,    // LM iterations of the ground solve, -1: not run
   kHWords = 32,
 };
 
@@ -559,6 +561,7 @@ __global__ __launch_bounds__(kKB) void kf_finish_kernel(KfBufs b, KfParamsDev P)
       for (int i = 0; i < kHWords; ++i) r[i] = h[i];
       r[kHRan] = 0;
       r[kHSuccess] = 0;
+      r[kHTreeIters] = r[kHGroundIters] = -1;
     }
     return;
   }
@@ -669,6 +672,17 @@ __global__ __launch_bounds__(kKB) void kf_finish_kernel(KfBufs b, KfParamsDev P)
     for (int i = 0; i < kHWords; ++i) r[i] = h[i];
     r[kHRan] = 1;
     r[kHSuccess] = s_success;
+    int ti = -1, gi = -1;
+    if (mode == 1) {
+      if (P.two_step) {
+        if (h[kHTreeK] >= 0) ti = b.status[2 * h[kHTreeK] + 1];
+        if (h[kHGroundK] >= 0) gi = b.status[2 * h[kHGroundK] + 1];
+      } else if (h[kHNProblems] == 1) {
+        ti = b.status[1];
+      }
+    }
+    r[kHTreeIters] = ti;
+    r[kHGroundIters] = gi;
     for (int i = 0; i < 7; ++i) {
       b.r_pose[i] = s_pose[i];
       b.r_delta[i] = s_delta[i];
@@ -721,8 +735,8 @@ struct sloam_kf_estimator {
   int32_t* status = nullptr;
   double* cost = nullptr;
   void* lm_state = nullptr;
-  float* cylf = nullptr;
-  float* plf = nullptr;
+  void* cyl_tab = nullptr;  // pose landmark tables, FP64 size
+  void* pl_tab = nullptr;
   size_t r_off[7] = {};  // byte offsets of the result regions
 };
 
@@ -764,8 +778,8 @@ int sloam_kf_estimator_create(sloam_ctx* ctx, const sloam_kf_params* p, int32_t 
                o_work = take(16 * (size_t)e->cap_work), o_first = take(4 * 4),
                o_np = take(4), o_pose_out = take(8 * 14), o_params = take(8 * 12),
                o_status = take(4 * 4), o_cost = take(8 * 4),
-               o_lm_state = take(2 * sloam_pose_state_bytes()), o_cylf = take(4 * 8 * T),
-               o_plf = take(4 * 4 * Cc);
+               o_lm_state = take(2 * sloam_pose_state_bytes()), o_cylf = take(8 * 8 * T),
+               o_plf = take(8 * 4 * Cc);
   int st = sloam_dev_alloc(ctx, &e->dev, off);
   if (st) {
     delete e;
@@ -824,8 +838,8 @@ int sloam_kf_estimator_create(sloam_ctx* ctx, const sloam_kf_params* p, int32_t 
   e->status = reinterpret_cast<int32_t*>(d + o_status);
   e->cost = reinterpret_cast<double*>(d + o_cost);
   e->lm_state = d + o_lm_state;
-  e->cylf = reinterpret_cast<float*>(d + o_cylf);
-  e->plf = reinterpret_cast<float*>(d + o_plf);
+  e->cyl_tab = d + o_cylf;
+  e->pl_tab = d + o_plf;
   b.pose_out = e->pose_out;
   b.params_out = e->params_out;
   b.status = e->status;
@@ -921,11 +935,12 @@ int sloam_kf_estimator_exec(sloam_kf_estimator* e, const sloam_kf_inputs* in) {
   c.initial_radius = 1e4;
   c.min_relative_decrease = 1e-3;
   c.parametrization = P.two_step ? 1 : 0;
+  c.precision = 1;  // FP64 evaluation, as Ceres'
   {
     SloamKernelScope ks(ctx, "kf_pose");
     const int st = sloam_pose_fused_device(
         ctx, 2, b.n_problems, b.res, b.work, b.first, b.cyl, b.cyl_off, b.pl, b.pl_off, b.pose_in,
-        b.free_mask, &c, e->lm_state, e->cylf, e->plf, e->pose_out, e->params_out, e->status,
+        b.free_mask, &c, e->lm_state, e->cyl_tab, e->pl_tab, e->pose_out, e->params_out, e->status,
         e->cost);
     if (st) return st;
   }

@@ -24,6 +24,14 @@
 //               2 * (max_iters + 1) launches cost more than the work.
 // Coordinates are evaluated relative to a per-problem anchor (the initial
 // translation) so that FP32 residuals keep sub-micrometre resolution.
+//
+// Precision FP64 (sloam_pose_cfg.precision 1) evaluates residuals, Jacobians
+// and sums in FP64 with the landmarks in FP64 and absolute coordinates, the
+// arithmetic Ceres does: the solves of one keyframe use it, so that their
+// termination tests (relative cost change against 1e-6) see the same costs as
+// Ceres'.  FP32 residuals carry rounding of ~1e-6 of the cost of a keyframe's
+// problem, the size of that tolerance, which moves the iteration at which an
+// LM run stops and, in a flat valley of the cost, its result by ~1e-4.
 #include "common.hpp"
 #include "detmath.h"
 
@@ -47,6 +55,12 @@ struct ChunkRed {
   float x[kSums][kPBlock + 1];
   float ps[kSums][kParts];
   float pc[kSums][kParts];
+};
+
+// the same in FP64 (plain sums in the same order)
+struct ChunkRed64 {
+  double x[kSums][kPBlock + 1];
+  double ps[kSums][kParts];
 };
 
 struct LMState {
@@ -221,16 +235,18 @@ __device__ inline void two_sum(float a, float b, float* s, float* e) {
 }
 
 // --------------------------------------------------------------------------
-// prep: LM state init + FP32 landmark tables relative to the anchor
+// prep: LM state init + landmark tables: FP32 relative to the anchor, or FP64
+// absolute (F64)
 // --------------------------------------------------------------------------
 // d_n_problems (nullable): the problem count in device memory (a grid of
 // n_problems workgroups then serves any count up to n_problems)
+template <bool F64>
 __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
     int n_problems, const int32_t* __restrict__ d_n_problems, const double* __restrict__ pose_in,
     const int32_t* __restrict__ free_mask, const double* __restrict__ cyl,
     const int32_t* __restrict__ cyl_off, const double* __restrict__ pl,
     const int32_t* __restrict__ pl_off, PoseCfgDev cfg, LMState* __restrict__ st,
-    float4* __restrict__ cylf, float4* __restrict__ plf) {
+    void* __restrict__ cyl_tab, void* __restrict__ pl_tab) {
   const int k = blockIdx.x;
   if (k >= n_problems || (d_n_problems && k >= *d_n_problems)) return;
   const double* x0 = pose_in + 7 * (size_t)k;
@@ -238,16 +254,26 @@ __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
   for (int i = cyl_off[k] + (int)threadIdx.x; i < cyl_off[k + 1]; i += kPBlock) {
     const double* c = cyl + 8 * (size_t)i;
     const double rn = sqrt(c[3] * c[3] + c[4] * c[4] + c[5] * c[5]);
-    cylf[2 * (size_t)i] = make_float4((float)(c[0] - ax), (float)(c[1] - ay), (float)(c[2] - az),
-                                      (float)c[6]);
-    cylf[2 * (size_t)i + 1] =
-        make_float4((float)(c[3] / rn), (float)(c[4] / rn), (float)(c[5] / rn), 0.f);
+    if constexpr (F64) {
+      double4* t = static_cast<double4*>(cyl_tab) + 2 * (size_t)i;
+      t[0] = make_double4(c[0], c[1], c[2], c[6]);
+      t[1] = make_double4(c[3] / rn, c[4] / rn, c[5] / rn, 0.0);
+    } else {
+      float4* t = static_cast<float4*>(cyl_tab) + 2 * (size_t)i;
+      t[0] = make_float4((float)(c[0] - ax), (float)(c[1] - ay), (float)(c[2] - az), (float)c[6]);
+      t[1] = make_float4((float)(c[3] / rn), (float)(c[4] / rn), (float)(c[5] / rn), 0.f);
+    }
   }
   for (int i = pl_off[k] + (int)threadIdx.x; i < pl_off[k + 1]; i += kPBlock) {
     const double* q = pl + 4 * (size_t)i;
     const double nn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-    const double dr = (q[3] + q[0] * ax + q[1] * ay + q[2] * az) / nn;
-    plf[i] = make_float4((float)(q[0] / nn), (float)(q[1] / nn), (float)(q[2] / nn), (float)dr);
+    if constexpr (F64) {
+      static_cast<double4*>(pl_tab)[i] = make_double4(q[0] / nn, q[1] / nn, q[2] / nn, q[3] / nn);
+    } else {
+      const double dr = (q[3] + q[0] * ax + q[1] * ay + q[2] * az) / nn;
+      static_cast<float4*>(pl_tab)[i] =
+          make_float4((float)(q[0] / nn), (float)(q[1] / nn), (float)(q[2] / nn), (float)dr);
+    }
   }
   if (threadIdx.x == 0) {
     LMState s;
@@ -462,18 +488,192 @@ __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e
   return out;
 }
 
+// FP64 evaluation: the pose in FP64, absolute coordinates; the residual,
+// Jacobian and Huber forms and the order of their products follow the oracle's
+// restatement of CylinderCost / PlaneCost and Ceres' corrector
+struct EvalFrame64 {
+  double R[3][3], t[3], Jl[3][3];
+  int fm, param;
+};
+
+__device__ inline void eval_frame64(const LMState& s, int param, EvalFrame64& F) {
+  double q[4] = {s.xe[3], s.xe[4], s.xe[5], s.xe[6]};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) F.Jl[i][j] = i == j ? 1.0 : 0.0;
+  if (param == 1) {
+    const double w[3] = {s.aae[3], s.aae[4], s.aae[5]};
+    aa_to_quat(w, q);
+    left_jacobian(w, F.Jl);
+  }
+  quat_to_R(q, F.R);
+  const double* tt = param == 1 ? s.aae : s.xe;
+  for (int i = 0; i < 3; ++i) F.t[i] = tt[i];
+  F.fm = s.free_mask;
+  F.param = param;
+}
+
+// One chunk as eval_chunk, in FP64; returns the chunk's 28 sums in threads
+// 0..27
+__device__ inline double eval_chunk64(const float4* __restrict__ pts, int b, int e,
+                                      const EvalFrame64& F, const double4* __restrict__ cyld,
+                                      const double4* __restrict__ pld, double delta,
+                                      ChunkRed64& red) {
+  const double d2 = delta * delta;
+  double acc[kSums];
+#pragma unroll
+  for (int i = 0; i < kSums; ++i) acc[i] = 0.0;
+  float4 pv[kRpt];
+#pragma unroll
+  for (int j = 0; j < kRpt; ++j) {
+    const int i = b + j * kPBlock + (int)threadIdx.x;
+    pv[j] = i < e ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const int n_mine = e > b + (int)threadIdx.x ? (e - b - (int)threadIdx.x + kPBlock - 1) / kPBlock
+                                               : 0;
+  auto gather = [&](int j, double4& A, double4& B) {
+    const int id = __float_as_int(pv[j].w);
+    if (id >= 0) {
+      A = cyld[2 * (size_t)id];
+      B = cyld[2 * (size_t)id + 1];
+    } else {
+      A = pld[-id - 1];
+      B = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+  };
+  double4 na = make_double4(0.0, 0.0, 0.0, 0.0), nb = na;
+  if (n_mine > 0) gather(0, na, nb);
+#pragma unroll
+  for (int j = 0; j < kRpt; ++j) {
+    if (j >= n_mine) break;
+    const float4 p = pv[j];
+    const int id = __float_as_int(p.w);
+    const double4 L0 = na, L1 = nb;
+    if (j + 1 < n_mine) gather(j + 1, na, nb);
+    const double P[3] = {(double)p.x, (double)p.y, (double)p.z};
+    double rp[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) rp[a] = F.R[a][0] * P[0] + F.R[a][1] * P[1] + F.R[a][2] * P[2];
+    const double lp[3] = {rp[0] + F.t[0], rp[1] + F.t[1], rp[2] + F.t[2]};
+    double dir[3], r;
+    if (id >= 0) {
+      const double v0 = lp[0] - L0.x, v1 = lp[1] - L0.y, v2 = lp[2] - L0.z;
+      const double h = v0 * L1.x + v1 * L1.y + v2 * L1.z;
+      const double w0 = v0 - h * L1.x, w1 = v1 - h * L1.y, w2 = v2 - h * L1.z;
+      const double rho = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+      r = rho - L0.w;
+      dir[0] = rho > 0.0 ? w0 / rho : 0.0;
+      dir[1] = rho > 0.0 ? w1 / rho : 0.0;
+      dir[2] = rho > 0.0 ? w2 / rho : 0.0;
+    } else {
+      const double sv = L0.x * lp[0] + L0.y * lp[1] + L0.z * lp[2] + L0.w;
+      r = fabs(sv);
+      const double sg = sv < 0.0 ? -1.0 : 1.0;
+      dir[0] = sg * L0.x;
+      dir[1] = sg * L0.y;
+      dir[2] = sg * L0.z;
+    }
+    double J[6];
+    J[0] = dir[0];
+    J[1] = dir[1];
+    J[2] = dir[2];
+    J[3] = rp[1] * dir[2] - rp[2] * dir[1];
+    J[4] = rp[2] * dir[0] - rp[0] * dir[2];
+    J[5] = rp[0] * dir[1] - rp[1] * dir[0];
+    if (F.param == 1) {
+      const double jt[3] = {J[3], J[4], J[5]};
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        J[3 + c] = jt[0] * F.Jl[0][c] + jt[1] * F.Jl[1][c] + jt[2] * F.Jl[2][c];
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+      if (!((F.fm >> a) & 1)) J[a] = 0.0;
+    const double s2 = r * r;
+    double wgt, rho;
+    if (s2 <= d2) {
+      wgt = 1.0;
+      rho = s2;
+    } else {
+      const double ar = sqrt(s2);
+      wgt = delta / ar;
+      rho = 2.0 * delta * ar - d2;
+    }
+    acc[27] += 0.5 * rho;
+    int tt = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      const double wa = wgt * J[a];
+#pragma unroll
+      for (int bb = a; bb < 6; ++bb) {
+        acc[tt] += wa * J[bb];
+        ++tt;
+      }
+      acc[21 + a] += wa * r;
+    }
+  }
+  // the reduction of eval_chunk with plain FP64 sums
+#pragma unroll
+  for (int i = 0; i < kSums; ++i) red.x[i][threadIdx.x] = acc[i];
+  __syncthreads();
+  if (threadIdx.x < kSums * kParts) {
+    const int i = threadIdx.x % kSums, part = threadIdx.x / kSums;
+    const double* row = &red.x[i][part * (kPBlock / kParts)];
+    double sm = row[0];
+    for (int j = 1; j < kPBlock / kParts; ++j) sm += row[j];
+    red.ps[i][part] = sm;
+  }
+  __syncthreads();
+  double out = 0.0;
+  if (threadIdx.x < kSums) {
+    const int i = threadIdx.x;
+    out = red.ps[i][0];
+    for (int v = 1; v < kParts; ++v) out += red.ps[i][v];
+  }
+  __syncthreads();  // red is reused by the next chunk
+  return out;
+}
+
+// the evaluation of one precision: frame, chunk, LDS and table types
+template <bool F64>
+struct PoseEval {
+  using Frame = EvalFrame;
+  using Red = ChunkRed;
+  __device__ static void frame(const LMState& s, int param, Frame& F) { eval_frame(s, param, F); }
+  __device__ static double chunk(const float4* pts, int b, int e, const Frame& F,
+                                 const void* cyl_tab, const void* pl_tab, double delta, Red& red) {
+    return eval_chunk(pts, b, e, F, static_cast<const float4*>(cyl_tab),
+                      static_cast<const float4*>(pl_tab), (float)delta, red);
+  }
+};
+
+template <>
+struct PoseEval<true> {
+  using Frame = EvalFrame64;
+  using Red = ChunkRed64;
+  __device__ static void frame(const LMState& s, int param, Frame& F) {
+    eval_frame64(s, param, F);
+  }
+  __device__ static double chunk(const float4* pts, int b, int e, const Frame& F,
+                                 const void* cyl_tab, const void* pl_tab, double delta, Red& red) {
+    return eval_chunk64(pts, b, e, F, static_cast<const double4*>(cyl_tab),
+                        static_cast<const double4*>(pl_tab), delta, red);
+  }
+};
+
 // work item = {problem, first residual, end residual, unused}
+template <bool F64>
 __global__ __launch_bounds__(kPBlock) void pose_eval_kernel(
     const float4* __restrict__ pts, const int4* __restrict__ work, const LMState* __restrict__ st,
-    const float4* __restrict__ cylf, const float4* __restrict__ plf, float delta, int param,
+    const void* __restrict__ cyl_tab, const void* __restrict__ pl_tab, double delta, int param,
     double* __restrict__ partials) {
-  __shared__ ChunkRed red;
+  using E = PoseEval<F64>;
+  __shared__ typename E::Red red;
   const int4 w = work[blockIdx.x];
   const int k = w.x;
   if (st[k].done) return;
-  EvalFrame F;
-  eval_frame(st[k], param, F);
-  const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red);
+  typename E::Frame F;
+  E::frame(st[k], param, F);
+  const double p = E::chunk(pts, w.y, w.z, F, cyl_tab, pl_tab, delta, red);
   if (threadIdx.x < kSums) partials[(size_t)blockIdx.x * kSums + threadIdx.x] = p;
 }
 
@@ -664,13 +864,15 @@ __global__ __launch_bounds__(64) void pose_solve_kernel(
 // --------------------------------------------------------------------------
 // fused: every iteration of one problem in one workgroup
 // --------------------------------------------------------------------------
+template <bool F64>
 __global__ __launch_bounds__(kPBlock) void pose_fused_kernel(
     int n_problems, const int32_t* __restrict__ d_n_problems, const float4* __restrict__ pts,
     const int4* __restrict__ work, const int32_t* __restrict__ first_work,
-    const float4* __restrict__ cylf, const float4* __restrict__ plf, PoseCfgDev cfg, float delta,
+    const void* __restrict__ cyl_tab, const void* __restrict__ pl_tab, PoseCfgDev cfg,
     LMState* __restrict__ st, double* __restrict__ pose_out, int32_t* __restrict__ status,
     double* __restrict__ cost, double* __restrict__ params_out) {
-  __shared__ ChunkRed red;
+  using E = PoseEval<F64>;
+  __shared__ typename E::Red red;
   __shared__ double tot[kSums];
   __shared__ LMState sS;
   const int k = blockIdx.x;
@@ -681,12 +883,12 @@ __global__ __launch_bounds__(kPBlock) void pose_fused_kernel(
   bool ran = false;
   for (int it = 0; it <= cfg.max_iters; ++it) {
     if (sS.done) break;
-    EvalFrame F;
-    eval_frame(sS, cfg.param, F);
+    typename E::Frame F;
+    E::frame(sS, cfg.param, F);
     double acc = 0.0;
     for (int wk = w0; wk < w1; ++wk) {
       const int4 w = work[wk];
-      const double p = eval_chunk(pts, w.y, w.z, F, cylf, plf, delta, red);
+      const double p = E::chunk(pts, w.y, w.z, F, cyl_tab, pl_tab, cfg.delta, red);
       if (threadIdx.x < kSums) acc += p;
     }
     if (threadIdx.x < kSums) tot[threadIdx.x] = acc;
@@ -724,8 +926,8 @@ struct sloam_pose_plan {
   int32_t* d_first = nullptr;
   LMState* d_state = nullptr;
   double* d_partials = nullptr;
-  float4* d_cylf = nullptr;
-  float4* d_plf = nullptr;
+  void* d_cyl_tab = nullptr;  // landmark tables (pose_prep_kernel), bytes
+  void* d_pl_tab = nullptr;
   size_t cap_work = 0, cap_first = 0, cap_state = 0, cap_partials = 0, cap_cyl = 0, cap_pl = 0;
 };
 
@@ -781,8 +983,10 @@ int plan_build(sloam_pose_plan* p, const int32_t* h_res_offsets, int32_t n_probl
   if (!st)
     st = grow(ctx, (void**)&p->d_partials, &p->cap_partials, (size_t)kSums * p->h_work.size(),
               sizeof(double));
-  if (!st) st = grow(ctx, (void**)&p->d_cylf, &p->cap_cyl, 2 * (size_t)n_cylinders, sizeof(float4));
-  if (!st) st = grow(ctx, (void**)&p->d_plf, &p->cap_pl, (size_t)n_planes, sizeof(float4));
+  // landmark table entries: 2 float4 / 2 double4 per cylinder, 1 per plane
+  const size_t tab = p->cfg.precision == 1 ? sizeof(double4) : sizeof(float4);
+  if (!st) st = grow(ctx, &p->d_cyl_tab, &p->cap_cyl, 2 * tab * (size_t)n_cylinders, 1);
+  if (!st) st = grow(ctx, &p->d_pl_tab, &p->cap_pl, tab * (size_t)n_planes, 1);
   if (st) return st;
   // the host vectors stay alive in the plan until the next build, which
   // synchronises the stream first
@@ -797,7 +1001,8 @@ int plan_build(sloam_pose_plan* p, const int32_t* h_res_offsets, int32_t n_probl
 
 bool valid_cfg(const sloam_pose_cfg* cfg) {
   return cfg->max_iters >= 0 && cfg->huber_delta > 0.0 && cfg->initial_radius > 0.0 &&
-         cfg->parametrization >= 0 && cfg->parametrization <= 1;
+         cfg->parametrization >= 0 && cfg->parametrization <= 1 && cfg->precision >= 0 &&
+         cfg->precision <= 1;
 }
 
 }  // namespace
@@ -818,26 +1023,57 @@ static PoseCfgDev pose_cfg_dev(const sloam_pose_cfg& c) {
 
 size_t sloam_pose_state_bytes() { return sizeof(LMState); }
 
+namespace {
+
+// pose_prep + pose_fused of one precision
+template <bool F64>
+void launch_prep(sloam_ctx* ctx, int grid, int n_problems, const int32_t* d_n_problems,
+                 const double* d_pose_in, const int32_t* d_free_mask, const double* d_cyl,
+                 const int32_t* d_cyl_off, const double* d_planes, const int32_t* d_pl_off,
+                 const PoseCfgDev& dc, LMState* st, void* cyl_tab, void* pl_tab) {
+  hipLaunchKernelGGL(pose_prep_kernel<F64>, dim3(grid), dim3(kPBlock), 0, ctx->stream,
+                     n_problems, d_n_problems, d_pose_in, d_free_mask, d_cyl, d_cyl_off, d_planes,
+                     d_pl_off, dc, st, cyl_tab, pl_tab);
+}
+
+template <bool F64>
+void launch_fused(sloam_ctx* ctx, int grid, int n_problems, const int32_t* d_n_problems,
+                  const float* d_points, const int4* d_work, const int32_t* d_first,
+                  const void* cyl_tab, const void* pl_tab, const PoseCfgDev& dc, LMState* st,
+                  double* d_pose_out, int32_t* d_status, double* d_cost, double* d_params_out) {
+  hipLaunchKernelGGL(pose_fused_kernel<F64>, dim3(grid), dim3(kPBlock), 0, ctx->stream,
+                     n_problems, d_n_problems, reinterpret_cast<const float4*>(d_points), d_work,
+                     d_first, cyl_tab, pl_tab, dc, st, d_pose_out, d_status, d_cost,
+                     d_params_out);
+}
+
+}  // namespace
+
 int sloam_pose_fused_device(sloam_ctx* ctx, int max_problems, const int32_t* d_n_problems,
                             const float* d_points, const int32_t* d_work, const int32_t* d_first,
                             const double* d_cyl, const int32_t* d_cyl_off, const double* d_planes,
                             const int32_t* d_pl_off, const double* d_pose_in,
                             const int32_t* d_free_mask, const sloam_pose_cfg* cfg, void* d_state,
-                            float* d_cylf, float* d_plf, double* d_pose_out,
+                            void* d_cyl_tab, void* d_pl_tab, double* d_pose_out,
                             double* d_params_out, int32_t* d_status, double* d_cost) {
   if (max_problems <= 0) return SLOAM_OK;
+  if (!valid_cfg(cfg)) return SLOAM_ERR_INVALID_ARG;
   const PoseCfgDev dc = pose_cfg_dev(*cfg);
   LMState* st = static_cast<LMState*>(d_state);
-  hipLaunchKernelGGL(pose_prep_kernel, dim3(max_problems), dim3(kPBlock), 0, ctx->stream,
-                     max_problems, d_n_problems, d_pose_in, d_free_mask, d_cyl, d_cyl_off,
-                     d_planes, d_pl_off, dc, st, reinterpret_cast<float4*>(d_cylf),
-                     reinterpret_cast<float4*>(d_plf));
-  SLOAM_HIP(ctx, hipGetLastError());
-  hipLaunchKernelGGL(pose_fused_kernel, dim3(max_problems), dim3(kPBlock), 0, ctx->stream,
-                     max_problems, d_n_problems, reinterpret_cast<const float4*>(d_points),
-                     reinterpret_cast<const int4*>(d_work), d_first,
-                     reinterpret_cast<const float4*>(d_cylf), reinterpret_cast<const float4*>(d_plf),
-                     dc, (float)cfg->huber_delta, st, d_pose_out, d_status, d_cost, d_params_out);
+  const int4* work = reinterpret_cast<const int4*>(d_work);
+  if (cfg->precision == 1) {
+    launch_prep<true>(ctx, max_problems, max_problems, d_n_problems, d_pose_in, d_free_mask, d_cyl,
+                      d_cyl_off, d_planes, d_pl_off, dc, st, d_cyl_tab, d_pl_tab);
+    SLOAM_HIP(ctx, hipGetLastError());
+    launch_fused<true>(ctx, max_problems, max_problems, d_n_problems, d_points, work, d_first,
+                       d_cyl_tab, d_pl_tab, dc, st, d_pose_out, d_status, d_cost, d_params_out);
+  } else {
+    launch_prep<false>(ctx, max_problems, max_problems, d_n_problems, d_pose_in, d_free_mask,
+                       d_cyl, d_cyl_off, d_planes, d_pl_off, dc, st, d_cyl_tab, d_pl_tab);
+    SLOAM_HIP(ctx, hipGetLastError());
+    launch_fused<false>(ctx, max_problems, max_problems, d_n_problems, d_points, work, d_first,
+                        d_cyl_tab, d_pl_tab, dc, st, d_pose_out, d_status, d_cost, d_params_out);
+  }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
 }
@@ -897,8 +1133,8 @@ void sloam_pose_plan_destroy(sloam_pose_plan* p) {
   sloam_dev_free(p->d_first);
   sloam_dev_free(p->d_state);
   sloam_dev_free(p->d_partials);
-  sloam_dev_free(p->d_cylf);
-  sloam_dev_free(p->d_plf);
+  sloam_dev_free(p->d_cyl_tab);
+  sloam_dev_free(p->d_pl_tab);
   delete p;
 }
 
@@ -915,38 +1151,41 @@ int sloam_pose_plan_exec(sloam_pose_plan* p, const float* d_points, const double
   if (p->n_problems == 0) return SLOAM_OK;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
   const sloam_pose_cfg& c = p->cfg;
-  PoseCfgDev dc;
-  dc.max_iters = c.max_iters;
-  dc.stop_conv = c.stop_on_convergence;
-  dc.param = c.parametrization;
-  dc.delta = c.huber_delta;
-  dc.ftol = c.function_tolerance;
-  dc.gtol = c.gradient_tolerance;
-  dc.ptol = c.parameter_tolerance;
-  dc.radius0 = c.initial_radius;
-  dc.min_rel = c.min_relative_decrease;
+  const PoseCfgDev dc = pose_cfg_dev(c);
+  const bool f64 = c.precision == 1;
   {
     SloamKernelScope ks(ctx, "pose_prep");
-    hipLaunchKernelGGL(pose_prep_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
-                       p->n_problems, nullptr, d_pose_in, d_free_mask, d_cyl, d_cyl_offsets, d_planes,
-                       d_plane_offsets, dc, p->d_state, p->d_cylf, p->d_plf);
+    if (f64)
+      launch_prep<true>(ctx, p->n_problems, p->n_problems, nullptr, d_pose_in, d_free_mask, d_cyl,
+                        d_cyl_offsets, d_planes, d_plane_offsets, dc, p->d_state, p->d_cyl_tab,
+                        p->d_pl_tab);
+    else
+      launch_prep<false>(ctx, p->n_problems, p->n_problems, nullptr, d_pose_in, d_free_mask,
+                         d_cyl, d_cyl_offsets, d_planes, d_plane_offsets, dc, p->d_state,
+                         p->d_cyl_tab, p->d_pl_tab);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (sloam_pose_plan_uses_fused(p)) {
     SloamKernelScope ks(ctx, "pose_fused");
-    hipLaunchKernelGGL(pose_fused_kernel, dim3(p->n_problems), dim3(kPBlock), 0, ctx->stream,
-                       p->n_problems, nullptr, reinterpret_cast<const float4*>(d_points), p->d_work,
-                       p->d_first, p->d_cylf, p->d_plf, dc, (float)c.huber_delta, p->d_state,
-                       d_pose_out, d_status, d_cost, d_params_out);
+    if (f64)
+      launch_fused<true>(ctx, p->n_problems, p->n_problems, nullptr, d_points, p->d_work,
+                         p->d_first, p->d_cyl_tab, p->d_pl_tab, dc, p->d_state, d_pose_out,
+                         d_status, d_cost, d_params_out);
+    else
+      launch_fused<false>(ctx, p->n_problems, p->n_problems, nullptr, d_points, p->d_work,
+                          p->d_first, p->d_cyl_tab, p->d_pl_tab, dc, p->d_state, d_pose_out,
+                          d_status, d_cost, d_params_out);
     SLOAM_HIP(ctx, hipGetLastError());
     return SLOAM_OK;
   }
   for (int it = 0; it <= c.max_iters; ++it) {
     {
       SloamKernelScope ks(ctx, "pose_eval");
-      hipLaunchKernelGGL(pose_eval_kernel, dim3(p->n_work), dim3(kPBlock), 0, ctx->stream,
+      auto kern = f64 ? pose_eval_kernel<true> : pose_eval_kernel<false>;
+      hipLaunchKernelGGL(kern, dim3(p->n_work), dim3(kPBlock), 0, ctx->stream,
                          reinterpret_cast<const float4*>(d_points), p->d_work, p->d_state,
-                         p->d_cylf, p->d_plf, (float)c.huber_delta, c.parametrization,
+                         static_cast<const void*>(p->d_cyl_tab),
+                         static_cast<const void*>(p->d_pl_tab), c.huber_delta, c.parametrization,
                          p->d_partials);
     }
     SLOAM_HIP(ctx, hipGetLastError());

@@ -244,7 +244,7 @@ class PosePlan:
     def __init__(self, ctx, res_offsets, n_cylinders, n_planes, max_iters=10,
                  stop_on_convergence=False, huber_delta=0.1, function_tolerance=1e-6,
                  gradient_tolerance=1e-10, parameter_tolerance=1e-8, initial_radius=1e4,
-                 min_relative_decrease=1e-3, parametrization=0, mode=0):
+                 min_relative_decrease=1e-3, parametrization=0, mode=0, precision=0):
         self.ctx = ctx
         self.lib = ctx.lib
         off = np.ascontiguousarray(res_offsets, dtype=np.int32)
@@ -256,7 +256,7 @@ class PosePlan:
                            parameter_tolerance=parameter_tolerance,
                            initial_radius=initial_radius,
                            min_relative_decrease=min_relative_decrease,
-                           parametrization=parametrization, reserved=0)
+                           parametrization=parametrization, precision=precision)
         h = C.c_void_p()
         check(self.lib.sloam_pose_plan_create(ctx.handle, off.ctypes.data_as(C.c_void_p),
                                               self.n_problems, int(n_cylinders), int(n_planes),

@@ -386,7 +386,7 @@ bool KeyframeRunner::runDevice(const float* xyzi, const uint8_t* labels, const S
   const int32_t* hd = r.head;
   // header words: see sloam_kf_result_view
   if (hd[10]) throw std::runtime_error("keyframe: a device keyframe capacity was exceeded");
-  dev_step_ = SloamStepInfo{hd[3] != 0, hd[4] != 0, hd[5], hd[6]};
+  dev_step_ = SloamStepInfo{hd[3] != 0, hd[4] != 0, hd[5], hd[6], hd[18], hd[19]};
   bool ok = false;
   if (hd[17]) {
     out.T_Map_Curr = SE3::fromUnitQuaternion(Quat(r.pose[6], r.pose[3], r.pose[4], r.pose[5]),

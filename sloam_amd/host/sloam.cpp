@@ -29,6 +29,7 @@ sloam_pose_cfg reference_pose_cfg(int parametrization) {
   c.initial_radius = 1e4;
   c.min_relative_decrease = 1e-3;
   c.parametrization = parametrization;
+  c.precision = 1;  // FP64 evaluation, as Ceres'
   return c;
 }
 
@@ -179,7 +180,8 @@ void pose_params(const SE3& pose, double params[6]) {
 // values (sloam.cpp:99-110, 151-161).
 template <typename TV, typename GV>
 void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround, const TV& allTMatch,
-              const GV& allGMatch, double treeOut[3], double groundOut[3]) {
+              const GV& allGMatch, double treeOut[3], double groundOut[3],
+              int* iters = nullptr) {
   double p0[6];
   pose_params(poseEstimate, p0);
   treeOut[0] = p0[0];
@@ -207,8 +209,13 @@ void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround, const 
     ground_k = (int)probs.size();
     probs.push_back(std::move(pr));
   }
+  if (iters) iters[0] = iters[1] = -1;
   if (probs.empty()) return;
   const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(1));
+  if (iters) {
+    if (tree_k >= 0) iters[0] = res[tree_k].iterations;
+    if (ground_k >= 0) iters[1] = res[ground_k].iterations;
+  }
   if (tree_k >= 0 && res[tree_k].termination == 0) {
     treeOut[0] = res[tree_k].params[0];
     treeOut[1] = res[tree_k].params[1];
@@ -224,9 +231,10 @@ void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround, const 
 // TwoStepOptimizePose (sloam.cpp:33-53) over either match form
 template <typename TV, typename GV>
 bool two_step_pose(const SE3& poseEstimate, bool optimTrees, bool optimGround, const TV& allTMatch,
-                   const GV& allGMatch, SE3& tf) {
+                   const GV& allGMatch, SE3& tf, int* iters = nullptr) {
   double treeOut[3], groundOut[3];
-  two_step(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, treeOut, groundOut);
+  two_step(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, treeOut, groundOut,
+           iters);
   // roll, pitch from the ground step, yaw from the tree step (sloam.cpp:41-51)
   const double angleAxis[3] = {groundOut[1], groundOut[2], treeOut[2]};
   double q[4];
@@ -240,7 +248,8 @@ bool two_step_pose(const SE3& poseEstimate, bool optimTrees, bool optimGround, c
 
 // OptimizePose (sloam.cpp:174-255) over either match form
 template <typename TV, typename GV>
-bool joint_pose(const SE3& poseEstimate, const TV& allTMatch, const GV& allGMatch, SE3& tf) {
+bool joint_pose(const SE3& poseEstimate, const TV& allTMatch, const GV& allGMatch, SE3& tf,
+                int* iters = nullptr) {
   std::vector<sloam_gpu::PoseProblem> probs(1);
   set_initial_pose(poseEstimate, probs[0]);
   add_tree_matches(allTMatch, probs[0]);
@@ -255,6 +264,7 @@ bool joint_pose(const SE3& poseEstimate, const TV& allTMatch, const GV& allGMatc
     const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(0));
     success = res[0].termination == 0;
     std::memcpy(x, res[0].pose, sizeof(x));
+    if (iters) iters[0] = res[0].iterations;
   }
   // the output is written only on convergence (sloam.cpp:238-244)
   if (success) {
@@ -529,13 +539,16 @@ bool sloam::RunSloam(SloamInput& in, const MapView& mapModels, SloamOutput& out)
   const bool groundCheck =
       planeMatches.size() > fmParams_.minGroundModels && planes.size() > minPlanes_;
   step_ = {treeCheck, groundCheck, (int)treeMatches.size(), (int)planeMatches.size()};
+  int iters[2] = {-1, -1};
   if (fmParams_.twoStepOptim) {
     success = two_step_pose(in.poseEstimate, treeCheck, groundCheck, treeMatches, planeMatches,
-                            currPose);
+                            currPose, iters);
   } else if (treeCheck && groundCheck) {
-    success = joint_pose(in.poseEstimate, treeMatches, planeMatches, T_Delta);
+    success = joint_pose(in.poseEstimate, treeMatches, planeMatches, T_Delta, iters);
     if (success) currPose = T_Delta;
   }
+  step_.treeIters = iters[0];
+  step_.groundIters = iters[1];
   const auto p3 = PhaseClock::now();
   phase_.pose = ms_between(p2, p3);
 

@@ -66,10 +66,13 @@ struct SloamPhaseTimes {
 
 // what the last RunSloam call decided: whether its XY-yaw (tree) and
 // Z-roll-pitch (ground) steps had enough matched features to run, and how
-// many tree and ground feature matches it had (zero before association)
+// many tree and ground feature matches it had (zero before association);
+// the LM iterations of its tree (or joint) and ground solves, -1 for a solve
+// that did not run
 struct SloamStepInfo {
   bool treeCheck = false, groundCheck = false;
   int treeMatches = 0, planeMatches = 0;
+  int treeIters = -1, groundIters = -1;
 };
 
 namespace sloam {

@@ -76,7 +76,8 @@ def read(path):
         r["prev_planes"] = c.take("<f8", 7 * npv).reshape(npv, 7).copy()
         # output of the call
         o = {"ok": bool(c.i32()), "tree_step": bool(c.i32()), "ground_step": bool(c.i32()),
-             "tree_matches": c.i32(), "plane_matches": c.i32()}
+             "tree_matches": c.i32(), "plane_matches": c.i32(), "tree_iters": c.i32(),
+             "ground_iters": c.i32()}
         o["pose"] = c.take("<f8", 7).copy()
         o["delta"] = c.take("<f8", 7).copy()
         ntm = c.i32()

@@ -126,15 +126,18 @@ FUSED_CASES = [
 ]
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("name,make,kw", FUSED_CASES, ids=[c[0] for c in FUSED_CASES])
-def test_fused_matches_batched_bitwise(ctx, name, make, kw):
+def test_fused_matches_batched_bitwise(ctx, name, make, kw, precision):
     """The one-launch LM kernel evaluates the same chunks in the same order and
-    runs the same step: every output equals the two-kernel loop's bit for bit."""
+    runs the same step: every output equals the two-kernel loop's bit for bit,
+    in either evaluation precision."""
     prob = make()
-    print(f"[pose] fused vs batched: {name}, {len(prob['res_offsets']) - 1} problems, "
-          f"{prob['res_offsets'][-1]} residuals", flush=True)
-    b = solve_poses(ctx, prob, mode=1, **kw)
-    f = solve_poses(ctx, prob, mode=2, **kw)
+    print(f"[pose] fused vs batched: {name}, precision {precision}, "
+          f"{len(prob['res_offsets']) - 1} problems, {prob['res_offsets'][-1]} residuals",
+          flush=True)
+    b = solve_poses(ctx, prob, mode=1, precision=precision, **kw)
+    f = solve_poses(ctx, prob, mode=2, precision=precision, **kw)
     for k in ("pose", "params", "status", "cost"):
         assert np.array_equal(b[k], f[k]), k
 
@@ -183,3 +186,35 @@ def test_plan_reset_reuses_buffers(ctx):
         for k in ("pose", "params", "status", "cost"):
             assert np.array_equal(got[k], ref[k]), (i, k)
     plan.close()
+
+
+FP64_CASES = [c for c in FUSED_CASES if c[0] != "empty-problem"] + [
+    # a keyframe-sized two-step solve: 120 tree features on 6 trees, 100 ground
+    ("keyframe-two-step", lambda: synth.split_two_step(synth.make_pose_problems(
+        1, n_cyl=6, n_planes=10, pts_per_cyl=20, pts_per_plane=10, seed=36)),
+     dict(max_iters=50, stop_on_convergence=True, parametrization=1)),
+]
+
+
+@pytest.mark.parametrize("name,make,kw", FP64_CASES, ids=[c[0] for c in FP64_CASES])
+def test_fp64_evaluation_follows_the_oracle(ctx, name, make, kw):
+    """Precision 1 evaluates in FP64 as Ceres does: every LM decision is the
+    oracle's (the same iteration counts and terminations) and the poses agree
+    to 1e-8 (metres, quaternion components), four orders below the bar of the
+    FP32 form.  The only differences left come from the last bits of the sine /
+    cosine of the pose composition (the oracle calls the host libm) and from
+    the summation order.  (The quaternions are compared by component: the
+    angle between two rotations, acos of a dot product, cannot resolve less
+    than ~3e-8 rad.)"""
+    prob = make()
+    g = solve_poses(ctx, prob, precision=1, **kw)
+    o = oracle.pose_lm_batch(prob, threads=8, **kw)
+    dt = np.abs(g["pose"][:, :3] - o["pose"][:, :3]).max()
+    qg, qo = g["pose"][:, 3:], o["pose"][:, 3:]
+    dq = np.minimum(np.abs(qg - qo).max(axis=1), np.abs(qg + qo).max(axis=1)).max()
+    print(f"[pose] fp64 {name}: |dt| max {dt:.3e} m, |dq| max {dq:.3e}, "
+          f"iterations {g['status'][:, 1].tolist()} / oracle {o['status'][:, 1].tolist()}",
+          flush=True)
+    assert np.array_equal(g["status"], o["status"])
+    assert dt < 1e-8 and dq < 1e-8
+    assert np.allclose(g["cost"], o["cost"], rtol=1e-9, atol=0)
