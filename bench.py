@@ -178,15 +178,39 @@ def bench_pose(ctx, args):
     ctx.sync()
     fused_ms = e0.elapsed_time(e1) / args.pose_steps
     fused_same = all(np.array_equal(res[k], v.cpu().numpy()) for k, v in out.items())
+    plan.close()
+    # the same 64 solves with FP64 evaluation (the keyframe solves' form)
+    plan = PosePlan(ctx, prob["res_offsets"], len(prob["cyl"]), len(prob["planes"]),
+                    max_iters=iters, precision=1, mode=1)
+    plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+              t["pose_init"], out)
+    ctx.sync()
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    e0.record(stream)
+    for _ in range(args.pose_steps):
+        plan.exec(t["points"], t["cyl"], t["cyl_offsets"], t["planes"], t["plane_offsets"],
+                  t["pose_init"], out)
+    e1.record(stream)
+    ctx.sync()
+    ctx.enable_timing(False)
+    f64_ms = e0.elapsed_time(e1) / args.pose_steps
+    ev64_ms, ev64_n = ctx.kernel_time("pose_eval")
+    eval64_ms = ev64_ms / max(ev64_n, 1)
+    eval64_bytes = 16.0 * n_res + 64.0 * len(prob["cyl"]) + 32.0 * len(prob["planes"])
+    res64 = {k: v.cpu().numpy() for k, v in out.items()}
+    dt64 = np.abs(res64["pose"][:, :3] - res["pose"][:, :3]).max()
+    plan.close()
     # one keyframe-sized problem (2 x 3250 residuals, 50 iterations, stop at
-    # convergence): fused against batched
+    # convergence): fused against batched, FP32 and FP64 evaluation
     small = synth.make_pose_problems(2, n_cyl=50, n_planes=5, seed=36)
     ts = {k: torch.as_tensor(np.ascontiguousarray(small[k])).to(dev)
           for k in ("points", "cyl", "cyl_offsets", "planes", "plane_offsets", "pose_init")}
     small_ms = {}
-    for mode, name in ((1, "batched"), (2, "fused")):
+    for mode, precision, name in ((1, 0, "batched"), (2, 0, "fused"), (1, 1, "batched_fp64"),
+                                  (2, 1, "fused_fp64")):
         sp = PosePlan(ctx, small["res_offsets"], len(small["cyl"]), len(small["planes"]),
-                      max_iters=50, stop_on_convergence=True, mode=mode)
+                      max_iters=50, stop_on_convergence=True, mode=mode, precision=precision)
         so = sp.alloc_outputs()
         torch.cuda.synchronize()
         sp.exec(ts["points"], ts["cyl"], ts["cyl_offsets"], ts["planes"], ts["plane_offsets"],
@@ -200,7 +224,6 @@ def bench_pose(ctx, args):
         ctx.sync()
         small_ms[name] = e0.elapsed_time(e1) / args.pose_steps
         sp.close()
-    plan.close()
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -229,6 +252,9 @@ def bench_pose(ctx, args):
         "graph_bitwise_equal": bool(graph_same),
         "fused_ms_per_solve": fused_ms,
         "fused_bitwise_equal": bool(fused_same),
+        "fp64": {"ms_per_solve": f64_ms, "eval_kernel_ms": eval64_ms,
+                 "eval_hbm_frac": eval64_bytes / (eval64_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                 "max_translation_diff_vs_fp32_m": float(dt64)},
         "keyframe_size_solve_ms": small_ms,
         "cpu_baseline": cpu,
     }

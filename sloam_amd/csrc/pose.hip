@@ -313,9 +313,10 @@ __global__ __launch_bounds__(kPBlock) void pose_prep_kernel(
 // --------------------------------------------------------------------------
 // pose to evaluate, relative to the anchor, in FP32; for the angle-axis
 // parametrization the rotation columns of the Jacobian are mapped by the left
-// Jacobian Jl(w) and fixed parameters are masked
+// Jacobian Jl(w) and fixed parameters are masked (sub: either applies)
 struct EvalFrame {
   float R[3][3], t[3], Jl[3][3], msk[6];
+  int sub;
 };
 
 __device__ inline void eval_frame(const LMState& s, int param, EvalFrame& F) {
@@ -341,11 +342,21 @@ __device__ inline void eval_frame(const LMState& s, int param, EvalFrame& F) {
   }
 #pragma unroll
   for (int i = 0; i < 6; ++i) F.msk[i] = ((fm >> i) & 1) ? 1.0f : 0.0f;
+  F.sub = param == 1 || fm != 0x3f;
 }
 
 // One chunk [b, e) of at most kPBlock * kRpt residuals over a workgroup of
 // kPBlock threads: thread t takes residuals b + j * kPBlock + t.  Returns the
 // chunk's 28 sums as FP64 (sum + compensation) in threads 0..27.
+//
+// The kernel is bound by VALU issue (~160 instructions per residual against
+// 16 bytes of HBM).  kSub false (the quaternion parametrization, all
+// parameters free) drops the left-Jacobian map and the masks, whose identity
+// and unit factors change no bit.  The divides and the square root stay
+// correctly rounded: the hardware reciprocals would save ~35 instructions per
+// residual but move the LM path of small ill-conditioned problems by more
+// than 1e-4 within a fixed iteration count.
+template <bool kSub>
 __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e,
                                     const EvalFrame& F, const float4* __restrict__ cylf,
                                     const float4* __restrict__ plf, float delta, ChunkRed& red) {
@@ -414,18 +425,27 @@ __device__ inline double eval_chunk(const float4* __restrict__ pts, int b, int e
       dir[2] = sg * L.z;
     }
     float J[6];
-    J[0] = dir[0] * F.msk[0];
-    J[1] = dir[1] * F.msk[1];
-    J[2] = dir[2] * F.msk[2];
     {
       const float jt0 = __builtin_fmaf(rp[1], dir[2], -rp[2] * dir[1]);
       const float jt1 = __builtin_fmaf(rp[2], dir[0], -rp[0] * dir[2]);
       const float jt2 = __builtin_fmaf(rp[0], dir[1], -rp[1] * dir[0]);
+      if constexpr (kSub) {
+        J[0] = dir[0] * F.msk[0];
+        J[1] = dir[1] * F.msk[1];
+        J[2] = dir[2] * F.msk[2];
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        J[3 + c] =
-            __builtin_fmaf(jt0, F.Jl[0][c], __builtin_fmaf(jt1, F.Jl[1][c], jt2 * F.Jl[2][c])) *
-            F.msk[3 + c];
+        for (int c = 0; c < 3; ++c)
+          J[3 + c] =
+              __builtin_fmaf(jt0, F.Jl[0][c], __builtin_fmaf(jt1, F.Jl[1][c], jt2 * F.Jl[2][c])) *
+              F.msk[3 + c];
+      } else {
+        J[0] = dir[0];
+        J[1] = dir[1];
+        J[2] = dir[2];
+        J[3] = jt0;
+        J[4] = jt1;
+        J[5] = jt2;
+      }
     }
     const float s2 = r * r;
     float wgt, rho;
@@ -641,8 +661,10 @@ struct PoseEval {
   __device__ static void frame(const LMState& s, int param, Frame& F) { eval_frame(s, param, F); }
   __device__ static double chunk(const float4* pts, int b, int e, const Frame& F,
                                  const void* cyl_tab, const void* pl_tab, double delta, Red& red) {
-    return eval_chunk(pts, b, e, F, static_cast<const float4*>(cyl_tab),
-                      static_cast<const float4*>(pl_tab), (float)delta, red);
+    const float4* ct = static_cast<const float4*>(cyl_tab);
+    const float4* pt = static_cast<const float4*>(pl_tab);
+    return F.sub ? eval_chunk<true>(pts, b, e, F, ct, pt, (float)delta, red)
+                 : eval_chunk<false>(pts, b, e, F, ct, pt, (float)delta, red);
   }
 };
 
