@@ -602,7 +602,7 @@ typedef struct sloam_kf_params {
   int32_t features_per_tree;     /* featuresPerTree                           */
   int32_t num_ground_features;   /* numGroundFeatures                         */
   int32_t two_step;              /* twoStepOptim                              */
-  int32_t reserved;
+  int32_t pose_precision;        /* as sloam_pose_cfg.precision (1: FP64)     */
 } sloam_kf_params;
 
 typedef struct sloam_kf_inputs { /* device pointers */

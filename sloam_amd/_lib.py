@@ -156,7 +156,7 @@ class KfParams(C.Structure):
         "tree_match_thresh", "add_new_tree_thresh_dist", "min_tree_models", "min_ground_models",
         "min_planes")]
     _fields_ += [(n, C.c_int32) for n in ("features_per_tree", "num_ground_features", "two_step",
-                                          "reserved")]
+                                          "pose_precision")]
 
 
 class KfInputs(C.Structure):

@@ -35,7 +35,9 @@
 // instead of the host sloam::sloam; --compare-device runs, in the first pass,
 // a shadow with the same tree model and the device estimator from its own
 // state, and counts the keyframes where its pose, landmarks and step
-// decisions equal the host estimator's bit for bit.  The per-keyframe
+// decisions equal the host estimator's bit for bit.  --pose-precision 0
+// runs the pose solves of both runners with FP32 residuals and compensated
+// FP32 sums instead of FP64 (KeyframeRunner::setPosePrecision).  The per-keyframe
 // latencies never include the shadow's work, but with a single pass that
 // pass's throughput does: give two passes or more to time with --shadow.
 //
@@ -230,6 +232,7 @@ int main(int argc, char** argv) {
   // --shadow-seed: the shadow's hypothesis key (another stream of the same
   // tree model measures that model's spread against itself)
   unsigned long long shadow_seed = 0;
+  int pose_precision = 1;
   bool log = false, pinned = false, pipelined = false, device_est = false, compare_device = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -247,6 +250,8 @@ int main(int argc, char** argv) {
       (a == "--max-tree-radius" ? max_tree_radius : tree_match_thresh) = std::atof(argv[++i]);
     } else if (a == "--dump" && i + 1 < argc) {
       dump_path = argv[++i];
+    } else if (a == "--pose-precision" && i + 1 < argc) {
+      pose_precision = std::atoi(argv[++i]);
     } else if (a == "--shadow-seed" && i + 1 < argc) {
       shadow_seed = std::strtoull(argv[++i], nullptr, 0);
     } else if ((a == "--tree-model" || a == "--weights" || a == "--traj-out" ||
@@ -268,7 +273,7 @@ int main(int argc, char** argv) {
                  "usage: %s <sequence> [passes] [use_graph] [--tree-model lines|plain|learned] "
                  "[--weights npy] [--traj-out file] [--log] [--pinned] [--pipelined] "
                  "[--shadow lines|plain|learned] [--dump file] [--max-tree-radius m] "
-                 "[--tree-match-thresh m]\n",
+                 "[--tree-match-thresh m] [--pose-precision 0|1]\n",
                  argv[0]);
     return 2;
   }
@@ -389,11 +394,13 @@ int main(int argc, char** argv) {
       shadow.reset(new KeyframeRunner(H, W, fm, use_graph, 22.5f, -22.5f, false));
       shadow->setTreeModel(shadow_model, weights);
       if (shadow_seed) shadow->setHypothesisSeed(shadow_seed);
+      shadow->setPosePrecision(pose_precision);
       if (compare_device) shadow->setDeviceEstimator(true);
     }
     for (int pass = 0; pass < passes; ++pass) {
       KeyframeRunner kf(H, W, fm, use_graph, 22.5f, -22.5f, pipelined);
       kf.setTreeModel(tree_model, weights);
+      kf.setPosePrecision(pose_precision);
       if (device_est) kf.setDeviceEstimator(true);
       int dump_k = 0, dump_calls = 0;
       if (dump.f && pass == 0)

@@ -747,7 +747,8 @@ int sloam_kf_estimator_create(sloam_ctx* ctx, const sloam_kf_params* p, int32_t 
   if (!ctx || !p || !out || cap_trees < 1 || cap_cells < 1 || cap_map < 1)
     return SLOAM_ERR_INVALID_ARG;
   if (p->features_per_tree < 1 || p->num_ground_features < 1 || cap_trees > (1 << 16) ||
-      cap_cells > (1 << 16) || cap_map > (1 << 20))
+      cap_cells > (1 << 16) || cap_map > (1 << 20) || p->pose_precision < 0 ||
+      p->pose_precision > 1)
     return SLOAM_ERR_INVALID_ARG;
   *out = nullptr;
   SLOAM_HIP(ctx, hipSetDevice(ctx->device));
@@ -935,7 +936,7 @@ int sloam_kf_estimator_exec(sloam_kf_estimator* e, const sloam_kf_inputs* in) {
   c.initial_radius = 1e4;
   c.min_relative_decrease = 1e-3;
   c.parametrization = P.two_step ? 1 : 0;
-  c.precision = 1;  // FP64 evaluation, as Ceres'
+  c.precision = e->params.pose_precision;
   {
     SloamKernelScope ks(ctx, "kf_pose");
     const int st = sloam_pose_fused_device(

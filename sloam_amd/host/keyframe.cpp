@@ -210,6 +210,12 @@ void KeyframeRunner::setHypothesisSeed(uint64_t seed) {
   fit_seed_ = seed;
 }
 
+void KeyframeRunner::setPosePrecision(int precision) {
+  if (kf_est_ && precision != sloam_.posePrecision())
+    throw std::runtime_error("keyframe: the pose precision is set before the device estimator");
+  sloam_.setPosePrecision(precision);
+}
+
 void KeyframeRunner::setDeviceEstimator(bool on) {
   if (on && mode_ == TreeModel::MedianLines)
     throw std::runtime_error("keyframe: the device keyframe serves the cylinder-fit tree models");
@@ -234,6 +240,7 @@ void KeyframeRunner::setDeviceEstimator(bool on) {
     p.features_per_tree = fm_.featuresPerTree;
     p.num_ground_features = fm_.numGroundFeatures;
     p.two_step = fm_.twoStepOptim ? 1 : 0;
+    p.pose_precision = sloam_.posePrecision();
     check(sloam_kf_estimator_create(ctx, &p, kf_cap_trees_, std::max(fo.n_cells, 1), kf_cap_map_,
                                     &kf_est_),
           "sloam_kf_estimator_create");

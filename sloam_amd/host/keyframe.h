@@ -126,6 +126,9 @@ class KeyframeRunner {
   // measure how far the plain fit differs from itself.  No counterpart in
   // the reference.
   void setHypothesisSeed(uint64_t seed);
+  // Arithmetic of the pose solves (sloam::sloam::setPosePrecision), for both
+  // estimators; set before the device estimator is switched on.
+  void setPosePrecision(int precision);
 
  private:
   // grow-only device buffer

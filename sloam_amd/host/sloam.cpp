@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 #include <utility>
 
 #include "geometry.h"
@@ -17,8 +18,9 @@ namespace sloam {
 namespace {
 
 // Ceres settings of the reference's solves (sloam.cpp:88-97, 219-226):
-// LM, HuberLoss(0.1), 50 iterations, Ceres' default tolerances.
-sloam_pose_cfg reference_pose_cfg(int parametrization) {
+// LM, HuberLoss(0.1), 50 iterations, Ceres' default tolerances; precision as
+// sloam_pose_cfg.precision.
+sloam_pose_cfg reference_pose_cfg(int parametrization, int precision) {
   sloam_pose_cfg c{};
   c.max_iters = 50;
   c.stop_on_convergence = 1;
@@ -29,7 +31,7 @@ sloam_pose_cfg reference_pose_cfg(int parametrization) {
   c.initial_radius = 1e4;
   c.min_relative_decrease = 1e-3;
   c.parametrization = parametrization;
-  c.precision = 1;  // FP64 evaluation, as Ceres'
+  c.precision = precision;
   return c;
 }
 
@@ -180,7 +182,7 @@ void pose_params(const SE3& pose, double params[6]) {
 // values (sloam.cpp:99-110, 151-161).
 template <typename TV, typename GV>
 void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround, const TV& allTMatch,
-              const GV& allGMatch, double treeOut[3], double groundOut[3],
+              const GV& allGMatch, double treeOut[3], double groundOut[3], int precision,
               int* iters = nullptr) {
   double p0[6];
   pose_params(poseEstimate, p0);
@@ -211,7 +213,7 @@ void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround, const 
   }
   if (iters) iters[0] = iters[1] = -1;
   if (probs.empty()) return;
-  const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(1));
+  const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(1, precision));
   if (iters) {
     if (tree_k >= 0) iters[0] = res[tree_k].iterations;
     if (ground_k >= 0) iters[1] = res[ground_k].iterations;
@@ -231,10 +233,10 @@ void two_step(const SE3& poseEstimate, bool optimTrees, bool optimGround, const 
 // TwoStepOptimizePose (sloam.cpp:33-53) over either match form
 template <typename TV, typename GV>
 bool two_step_pose(const SE3& poseEstimate, bool optimTrees, bool optimGround, const TV& allTMatch,
-                   const GV& allGMatch, SE3& tf, int* iters = nullptr) {
+                   const GV& allGMatch, SE3& tf, int precision, int* iters = nullptr) {
   double treeOut[3], groundOut[3];
   two_step(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, treeOut, groundOut,
-           iters);
+           precision, iters);
   // roll, pitch from the ground step, yaw from the tree step (sloam.cpp:41-51)
   const double angleAxis[3] = {groundOut[1], groundOut[2], treeOut[2]};
   double q[4];
@@ -249,7 +251,7 @@ bool two_step_pose(const SE3& poseEstimate, bool optimTrees, bool optimGround, c
 // OptimizePose (sloam.cpp:174-255) over either match form
 template <typename TV, typename GV>
 bool joint_pose(const SE3& poseEstimate, const TV& allTMatch, const GV& allGMatch, SE3& tf,
-                int* iters = nullptr) {
+                int precision, int* iters = nullptr) {
   std::vector<sloam_gpu::PoseProblem> probs(1);
   set_initial_pose(poseEstimate, probs[0]);
   add_tree_matches(allTMatch, probs[0]);
@@ -261,7 +263,7 @@ bool joint_pose(const SE3& poseEstimate, const TV& allTMatch, const GV& allGMatc
     success = true;
     std::memcpy(x, probs[0].pose, sizeof(x));
   } else {
-    const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(0));
+    const auto res = sloam_gpu::solvePoses(probs, reference_pose_cfg(0, precision));
     success = res[0].termination == 0;
     std::memcpy(x, res[0].pose, sizeof(x));
     if (iters) iters[0] = res[0].iterations;
@@ -284,6 +286,12 @@ sloam::sloam() {
 
 std::vector<Plane> sloam::getPrevGroundModel() { return prevGPlanes_; }
 
+void sloam::setPosePrecision(int precision) {
+  if (precision != 0 && precision != 1)
+    throw std::invalid_argument("sloam: pose precision is 0 (FP32) or 1 (FP64)");
+  posePrecision_ = precision;
+}
+
 CloudT sloam::getPrevGroundFeatures() {
   CloudT pc;
   for (const auto& ground : prevGPlanes_)
@@ -297,27 +305,28 @@ bool sloam::TwoStepOptimizePose(const SE3& poseEstimate, const bool optimTrees,
                                 const bool optimGround,
                                 const std::vector<ObjectMatch<Cylinder>>& allTMatch,
                                 const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
-  return two_step_pose(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, tf);
+  return two_step_pose(poseEstimate, optimTrees, optimGround, allTMatch, allGMatch, tf,
+                       posePrecision_);
 }
 
 void sloam::OptimizeXYYaw(const SE3& poseEstimate, const bool optimize,
                           const std::vector<ObjectMatch<Cylinder>>& allTMatch, double* out) {
   double ground[3];
   two_step(poseEstimate, optimize, false, allTMatch, std::vector<ObjectMatch<Plane>>(), out,
-           ground);
+           ground, posePrecision_);
 }
 
 void sloam::OptimizeZRollPitch(const SE3& poseEstimate, const bool optimize,
                                const std::vector<ObjectMatch<Plane>>& allGMatch, double* out) {
   double tree[3];
   two_step(poseEstimate, false, optimize, std::vector<ObjectMatch<Cylinder>>(), allGMatch, tree,
-           out);
+           out, posePrecision_);
 }
 
 bool sloam::OptimizePose(const SE3& poseEstimate,
                          const std::vector<ObjectMatch<Cylinder>>& allTMatch,
                          const std::vector<ObjectMatch<Plane>>& allGMatch, SE3& tf) {
-  return joint_pose(poseEstimate, allTMatch, allGMatch, tf);
+  return joint_pose(poseEstimate, allTMatch, allGMatch, tf, posePrecision_);
 }
 
 template <typename T>
@@ -542,9 +551,10 @@ bool sloam::RunSloam(SloamInput& in, const MapView& mapModels, SloamOutput& out)
   int iters[2] = {-1, -1};
   if (fmParams_.twoStepOptim) {
     success = two_step_pose(in.poseEstimate, treeCheck, groundCheck, treeMatches, planeMatches,
-                            currPose, iters);
+                            currPose, posePrecision_, iters);
   } else if (treeCheck && groundCheck) {
-    success = joint_pose(in.poseEstimate, treeMatches, planeMatches, T_Delta, iters);
+    success = joint_pose(in.poseEstimate, treeMatches, planeMatches, T_Delta, posePrecision_,
+                         iters);
     if (success) currPose = T_Delta;
   }
   step_.treeIters = iters[0];

@@ -131,6 +131,12 @@ class sloam {
   // The trees the last RunSloam call modelled but rejected: tree id and
   // Cylinder::RejectReason bits.  No counterpart in the reference.
   const std::vector<std::pair<int, int>>& lastRejectedTrees() const { return rejected_; }
+  // Arithmetic of the residuals, Jacobians and normal equations of the pose
+  // solves: 1 (default) FP64, as Ceres evaluates them; 0 FP32 with
+  // compensated FP32 sums (sloam_pose_cfg.precision).  No counterpart in the
+  // reference.
+  void setPosePrecision(int precision);
+  int posePrecision() const { return posePrecision_; }
 
  private:
   // ground binning / fit settings of fmParams_, binned around `pose`
@@ -146,6 +152,7 @@ class sloam {
   SloamPhaseTimes phase_;
   SloamStepInfo step_;
   std::vector<std::pair<int, int>> rejected_;
+  int posePrecision_ = 1;
 };
 
 }  // namespace sloam

@@ -58,10 +58,16 @@ def _dump(seq, tmp, model, extra=()):
     return kf_dump.read(path)
 
 
-def _compare(recs, params, tree_mode=None):
+def _compare(recs, params, tree_mode=None, pose_tol=1e-4, same_iterations=True, stats=None):
+    """Every record against the oracle; returns the largest pose difference
+    and the number of keyframes whose XY-yaw step ran.  stats (a dict)
+    collects the pose differences and the keyframes whose LM iteration
+    counts differ from the oracle's."""
     est = oracle.SloamEstimator(**params)
     worst = 0.0
     steps = 0
+    if stats is not None:
+        stats.update(diffs=[], iteration_flips=[])
     for i, rec in enumerate(recs):
         est.set_state(rec["first"], rec["prev_planes"])
         o = est.run(rec, tree_mode=tree_mode)
@@ -82,8 +88,14 @@ def _compare(recs, params, tree_mode=None):
         qo, qg = o["pose"][3:], g["pose"][3:]
         dq = min(np.abs(qo - qg).max(), np.abs(qo + qg).max())
         worst = max(worst, dp, dq)
-        assert (o["tree_iters"], o["ground_iters"]) == (g["tree_iters"], g["ground_iters"]), k
-        assert dp <= 1e-4 and dq <= 1e-4, (k, dp, dq)
+        same = (o["tree_iters"], o["ground_iters"]) == (g["tree_iters"], g["ground_iters"])
+        if stats is not None:
+            stats["diffs"].append(max(dp, dq))
+            if not same:
+                stats["iteration_flips"].append(k)
+        if same_iterations:
+            assert same, k
+        assert dp <= pose_tol and dq <= pose_tol, (k, dp, dq)
         # models in the sensor frame (each taken back by its own pose)
         to = kf_dump.unproject_trees(o["tm"], o["pose"])
         tg = kf_dump.unproject_trees(g["tm"], g["pose"])
@@ -127,3 +139,27 @@ def test_estimator_matches_oracle_stream(sequences, tmp_path, model):
     print(f"[estimator] stream {model}: {len(recs)} keyframes, tree steps {steps}, "
           f"largest pose difference {worst:.2e}", flush=True)
     assert steps >= len(recs) // 2
+
+
+@pytest.mark.parametrize("precision", [1, 0], ids=["fp64", "fp32"])
+def test_estimator_pose_precision_on_the_long_stream(long_stream, tmp_path, precision):
+    """200 keyframes of the long stream, plain tree model, both arithmetic
+    forms of the pose solves.  FP64 (the default, Ceres' arithmetic): every
+    LM run stops at the oracle's iteration and every pose agrees to 1e-9.
+    FP32 residuals with compensated FP32 sums: the cost's rounding is of the
+    size of Ceres' function tolerance (1e-6 of the cost), so LM runs stop at
+    other iterations than the oracle's and, in flat valleys of the cost,
+    elsewhere; measured here, and bounded by 1e-3."""
+    recs = _dump(long_stream, tmp_path, "plain",
+                 extra=("--max-tree-radius", "0.45", "--tree-match-thresh", "1.0",
+                        "--pose-precision", str(precision)))
+    st = {}
+    worst, steps = _compare(recs, {"maxTreeRadius": 0.45, "treeMatchThresh": 1.0},
+                            pose_tol=1e-9 if precision == 1 else 1e-3,
+                            same_iterations=precision == 1, stats=st)
+    d = np.array(st["diffs"])
+    print(f"[estimator] long stream, pose precision {precision}: {len(recs)} keyframes, tree "
+          f"steps {steps}, LM iteration counts differ on {len(st['iteration_flips'])}, pose "
+          f"difference median {np.median(d):.2e} max {d.max():.2e}, over 1e-4 on "
+          f"{int((d > 1e-4).sum())}", flush=True)
+    assert len(recs) >= 190 and steps >= len(recs) // 2
