@@ -20,7 +20,10 @@
 //   pn_bwd      one workgroup per cluster: loss, head gradients, and the
 //               max-pool gradient routed to its argmax points only (at most
 //               256 of them), back through layers 3, 2, 1 from the
-//               activations the forward pass saved
+//               activations the forward pass saved; the layer-2 products
+//               (dW2 = dz2' H1 and dz1 = dz2 W2) on v_mfma_f32_16x16x32_bf16
+//               over tiles of 32 active points, dz2 entering as two BF16
+//               parts
 //   pn_reduce   batch mean of the per-cluster gradients (fixed order)
 //   pn_adam     Adam update of the FP32 master weights
 #include "common.hpp"
@@ -481,21 +484,41 @@ __global__ __launch_bounds__(kFB) void pn_head_kernel(const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// backward.  G: per-cluster gradients [B][kNP]; dz2g / dz1g: per-cluster
-// scratch [B][256][128] / [B][256][64] for the active points' gradients.
+// backward.  G: per-cluster gradients [B][kNP].
+//
+// The layer-2 products run on BF16 MFMA per tile of kQT active points.  dz2
+// is FP32; it enters as hi = bf16(dz2) plus lo = bf16(dz2 - hi), 16
+// significant bits (relative error below 2^-16), against H1 and the
+// BF16-rounded W2, which are BF16 values already, so the products keep
+// FP32-level accuracy with FP32 accumulation.
 // ---------------------------------------------------------------------------
+constexpr int kQT = 32;        // active points per tile (the K of one MFMA)
+constexpr int kSQ = kQT + 8;   // row stride (bf16) of the [channel][point] tiles
+constexpr int kSJ = kC2 + 8;   // row stride (bf16) of the [point][channel] tiles
+// four waves: wave w owns dz1 channels 16w.. and dW2 rows 32w..; two
+// threads per layer-2 channel fill the dz2 tile
+static_assert(kFB == 256 && kC1 == 4 * 16 && kC2 == 4 * 32 && kFB == 2 * kC2 && kQT == 32,
+              "backward tile layout");
+
 __global__ __launch_bounds__(kFB) void pn_bwd_kernel(
     const float* __restrict__ prm, const float4* __restrict__ X, const PnMeta* __restrict__ meta,
     const float* __restrict__ act, const unsigned short* __restrict__ H1,
     const unsigned short* __restrict__ H2, const double* __restrict__ targets, LossCfg lc,
-    int n_clusters, float* __restrict__ G, float* __restrict__ dz2g, float* __restrict__ dz1g,
-    float* __restrict__ loss_out) {
+    int n_clusters, float* __restrict__ G, float* __restrict__ loss_out) {
   __shared__ float f[kC3], dz3[kC3], h4[kC4], dz4[kC4], dout[8];
   __shared__ int fidx[kC3];
   __shared__ int slot_of[kP];
   __shared__ int plist[kC3];
   __shared__ int ccount[kC3 + 1], cstart[kC3 + 1], corder[kC3];
   __shared__ int nact;
+  // one tile of active points: dz2 as [point][channel] (A of dz1 = dz2 W2)
+  // and [channel][point] (A of dW2 = dz2' H1), hi and lo parts; H1 as
+  // [channel][point] (B of dW2); the points
+  __shared__ __align__(16) unsigned short zq_hi[kQT * kSJ], zq_lo[kQT * kSJ];
+  __shared__ __align__(16) unsigned short zj_hi[kC2 * kSQ], zj_lo[kC2 * kSQ];
+  __shared__ __align__(16) unsigned short h1t[kC1 * kSQ];
+  __shared__ float4 xq[kQT];
+  __shared__ float db2s[kFB];
   const int c = blockIdx.x;
   if (c >= n_clusters) return;
   const int t = threadIdx.x;
@@ -504,8 +527,6 @@ __global__ __launch_bounds__(kFB) void pn_bwd_kernel(
   const unsigned short* H1c = H1 + (size_t)c * kP * kC1;
   const unsigned short* H2c = H2 + (size_t)c * kP * kC2;
   const float4* Xc = X + (size_t)c * kP;
-  float* dz2 = dz2g + (size_t)c * kC3 * kC2;
-  float* dz1 = dz1g + (size_t)c * kC3 * kC1;
   for (int k = t; k < kC3; k += kFB) {
     f[k] = a[kAF + k];
     fidx[k] = __float_as_int(a[kAI + k]);
@@ -614,54 +635,127 @@ __global__ __launch_bounds__(kFB) void pn_bwd_kernel(
     Gc[kW3 + i] = fidx[ch] >= 0 ? dz3[ch] * bf_val(H2c[(size_t)fidx[ch] * kC2 + k]) : 0.f;
   }
   if (t < kC3) Gc[kB3 + t] = dz3[t];
-  // dz2 of the active points
-  for (int i = t; i < P * kC2; i += kFB) {
-    const int q = i / kC2, j = i % kC2;
-    float s = 0.f;
-    for (int r = cstart[q]; r < cstart[q + 1]; ++r) {
-      const int ch = corder[r];
-      s += bf_round(prm[kW3 + ch * kC2 + j]) * dz3[ch];
+  // layer-2 weight fragments for dz1 = dz2 W2 (B operand: column k = 16 wave
+  // + lr, rows j = 32 ks + 8 lg + e), BF16-rounded as in the forward pass
+  const int lane = t & 63, wave = t >> 6, lr = lane & 15, lg = lane >> 4;
+  bf16x8 w2b[kC2 / 32];
+#pragma unroll
+  for (int ks = 0; ks < kC2 / 32; ++ks) {
+    unsigned short h[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      h[e] = bf_bits(prm[kW2 + (32 * ks + 8 * lg + e) * kC1 + 16 * wave + lr]);
+    u32x4 u;
+    u.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
+    u.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+    u.z = (unsigned)h[4] | ((unsigned)h[5] << 16);
+    u.w = (unsigned)h[6] | ((unsigned)h[7] << 16);
+    w2b[ks] = __builtin_bit_cast(bf16x8, u);
+  }
+  // dW2 accumulators: rows j = 32 wave + 16 jt + 4 lg + e, columns 16 kt + lr
+  f32x4 dw2[2][kC1 / 16];
+#pragma unroll
+  for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+    for (int kt = 0; kt < kC1 / 16; ++kt) dw2[jt][kt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // layer-1 sums (x, y, z, bias) of channel 16 wave + lr over this lane's rows
+  float s1[4] = {0.f, 0.f, 0.f, 0.f};
+  // dB2 of channel jj over the tile rows of parity qpar
+  const int jj = t & (kC2 - 1), qpar = t >> 7;
+  float db2 = 0.f;
+  for (int q0 = 0; q0 < P; q0 += kQT) {
+    // dz2 of the tile's points (zero beyond P): the max-pool's channels of
+    // each point back through W3, then ReLU'
+    for (int qi = qpar; qi < kQT; qi += kFB / kC2) {
+      const int q = q0 + qi;
+      float v = 0.f;
+      if (q < P) {
+        float sm = 0.f;
+        for (int r = cstart[q]; r < cstart[q + 1]; ++r) {
+          const int ch = corder[r];
+          sm += bf_round(prm[kW3 + ch * kC2 + jj]) * dz3[ch];
+        }
+        const float h2 = bf_val(H2c[(size_t)plist[q] * kC2 + jj]);
+        v = h2 > 0.f ? sm : 0.f;
+      }
+      db2 += v;
+      const unsigned short hi = bf_bits(v);
+      const unsigned short lo = bf_bits(v - bf_val(hi));
+      zq_hi[qi * kSJ + jj] = hi;
+      zq_lo[qi * kSJ + jj] = lo;
+      zj_hi[jj * kSQ + qi] = hi;
+      zj_lo[jj * kSQ + qi] = lo;
     }
-    const float h2 = bf_val(H2c[(size_t)plist[q] * kC2 + j]);
-    dz2[q * kC2 + j] = h2 > 0.f ? s : 0.f;
+    for (int i = t; i < kQT * kC1; i += kFB) {
+      const int qi = i / kC1, k = i % kC1, q = q0 + qi;
+      h1t[k * kSQ + qi] = q < P ? H1c[(size_t)plist[q] * kC1 + k] : (unsigned short)0;
+    }
+    if (t < kQT) xq[t] = q0 + t < P ? Xc[plist[q0 + t]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    // dW2[j][k] += sum_q dz2[q][j] h1[q][k]
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt) {
+      const int j0 = 32 * wave + 16 * jt;
+      const bf16x8 ah = frag_from_lds(zj_hi + (j0 + lr) * kSQ + 8 * lg);
+      const bf16x8 al = frag_from_lds(zj_lo + (j0 + lr) * kSQ + 8 * lg);
+#pragma unroll
+      for (int kt = 0; kt < kC1 / 16; ++kt) {
+        const bf16x8 bh = frag_from_lds(h1t + (16 * kt + lr) * kSQ + 8 * lg);
+        dw2[jt][kt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, dw2[jt][kt], 0, 0, 0);
+        dw2[jt][kt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, dw2[jt][kt], 0, 0, 0);
+      }
+    }
+    // dz1[q][k] = relu'(h1[q][k]) sum_j dz2[q][j] W2[j][k] for this wave's
+    // channels k, then the layer-1 sums over the tile's points
+#pragma unroll
+    for (int qt = 0; qt < kQT / 16; ++qt) {
+      f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kC2 / 32; ++ks) {
+        const bf16x8 ah = frag_from_lds(zq_hi + (16 * qt + lr) * kSJ + 32 * ks + 8 * lg);
+        const bf16x8 al = frag_from_lds(zq_lo + (16 * qt + lr) * kSJ + 32 * ks + 8 * lg);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, w2b[ks], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, w2b[ks], d, 0, 0, 0);
+      }
+      const int k = 16 * wave + lr;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int qi = 16 * qt + 4 * lg + e;
+        const float g = bf_val(h1t[k * kSQ + qi]) > 0.f ? d[e] : 0.f;
+        const float4 x = xq[qi];
+        s1[0] += g * x.x;
+        s1[1] += g * x.y;
+        s1[2] += g * x.z;
+        s1[3] += g;
+      }
+    }
+    __syncthreads();  // the next tile overwrites the tiles
+  }
+#pragma unroll
+  for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+    for (int kt = 0; kt < kC1 / 16; ++kt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        Gc[kW2 + (32 * wave + 16 * jt + 4 * lg + e) * kC1 + 16 * kt + lr] = dw2[jt][kt][e];
+  db2s[t] = db2;
+  // layer 1: the four lane groups holding channel k, added in a fixed tree
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) {
+    float v = s1[cc];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    s1[cc] = v;
+  }
+  if (lg == 0) {
+    const int k = 16 * wave + lr;
+    Gc[kW1 + 3 * k] = s1[0];
+    Gc[kW1 + 3 * k + 1] = s1[1];
+    Gc[kW1 + 3 * k + 2] = s1[2];
+    Gc[kB1 + k] = s1[3];
   }
   __syncthreads();
-  // layer 2 weights and dz1
-  for (int i = t; i < kC2 * kC1; i += kFB) {
-    const int j = i / kC1, k = i % kC1;
-    float s = 0.f;
-    for (int q = 0; q < P; ++q) s += dz2[q * kC2 + j] * bf_val(H1c[(size_t)plist[q] * kC1 + k]);
-    Gc[kW2 + i] = s;
-  }
-  if (t < kC2) {
-    float s = 0.f;
-    for (int q = 0; q < P; ++q) s += dz2[q * kC2 + t];
-    Gc[kB2 + t] = s;
-  }
-  for (int i = t; i < P * kC1; i += kFB) {
-    const int q = i / kC1, k = i % kC1;
-    float s = 0.f;
-    for (int j = 0; j < kC2; ++j) s += bf_round(prm[kW2 + j * kC1 + k]) * dz2[q * kC2 + j];
-    const float h1 = bf_val(H1c[(size_t)plist[q] * kC1 + k]);
-    dz1[q * kC1 + k] = h1 > 0.f ? s : 0.f;
-  }
-  __syncthreads();
-  // layer 1
-  if (t < kC1) {
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, sb = 0.f;
-    for (int q = 0; q < P; ++q) {
-      const float d = dz1[q * kC1 + t];
-      const float4 x = Xc[plist[q]];
-      s0 += d * x.x;
-      s1 += d * x.y;
-      s2 += d * x.z;
-      sb += d;
-    }
-    Gc[kW1 + 3 * t] = s0;
-    Gc[kW1 + 3 * t + 1] = s1;
-    Gc[kW1 + 3 * t + 2] = s2;
-    Gc[kB1 + t] = sb;
-  }
+  if (t < kC2) Gc[kB2 + t] = db2s[t] + db2s[t + kC2];
 }
 
 // sum over the batch of the per-cluster gradients (already scaled by 1/B)
@@ -709,8 +803,6 @@ struct PnBuffers {
   unsigned short* H1;
   unsigned short* H2;
   float* G;
-  float* dz2;
-  float* dz1;
   float* grad;
 };
 
@@ -722,7 +814,6 @@ int pn_buffers(sloam_ctx* ctx, int B, bool train, PnBuffers* b) {
   if (train)
     need += al(2 * (size_t)kP * kC1 * nB) +
             al(2 * (size_t)kP * kC2 * nB) + al(sizeof(float) * kNP * nB) +
-            al(sizeof(float) * kC3 * kC2 * nB) + al(sizeof(float) * kC3 * kC1 * nB) +
             al(sizeof(float) * kNP);
   void* base = nullptr;
   const int st = sloam_ctx_scratch(ctx, 3, need, &base);
@@ -741,8 +832,6 @@ int pn_buffers(sloam_ctx* ctx, int B, bool train, PnBuffers* b) {
     b->H1 = (unsigned short*)take(2 * (size_t)kP * kC1 * nB);
     b->H2 = (unsigned short*)take(2 * (size_t)kP * kC2 * nB);
     b->G = (float*)take(sizeof(float) * kNP * nB);
-    b->dz2 = (float*)take(sizeof(float) * kC3 * kC2 * nB);
-    b->dz1 = (float*)take(sizeof(float) * kC3 * kC1 * nB);
     b->grad = (float*)take(sizeof(float) * kNP);
   }
   return SLOAM_OK;
@@ -777,8 +866,7 @@ int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   {
     SloamKernelScope ks(ctx, "pn_bwd");
     hipLaunchKernelGGL(pn_bwd_kernel, dim3(B), dim3(kFB), 0, ctx->stream, d_params, b->X,
-                       b->meta, b->act, b->H1, b->H2, d_targets, lc, B, b->G, b->dz2, b->dz1,
-                       d_loss);
+                       b->meta, b->act, b->H1, b->H2, d_targets, lc, B, b->G, d_loss);
   }
   {
     SloamKernelScope ks(ctx, "pn_reduce");
