@@ -180,6 +180,26 @@ __device__ inline int tile_compact(bool f, int* wcount, int* base) {
   return idx;
 }
 
+// The same over the whole workgroup for every thread: the number of flagged
+// items before this thread's (over all tiles so far), flagged or not.
+__device__ inline int tile_scan(bool f, int* wcount, int* base) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t bits = __builtin_amdgcn_ballot_w64(f);
+  const int before = __builtin_popcountll(bits & ((1ull << lane) - 1ull));
+  if (lane == 0) wcount[wave] = __builtin_popcountll(bits);
+  __syncthreads();
+  int off = *base;
+  for (int w = 0; w < wave; ++w) off += wcount[w];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kKB / 64; ++w) t += wcount[w];
+    *base += t;
+  }
+  __syncthreads();
+  return off + before;
+}
+
 __device__ inline bool same_row(const double* a, const double* b, int n) {
   for (int i = 0; i < n; ++i)
     if (!(a[i] == b[i])) return false;
@@ -372,38 +392,83 @@ __global__ __launch_bounds__(kKB) void kf_models_kernel(KfBufs b, KfParamsDev P)
   }
   __syncthreads();
 
-  // checks, landmark tables (consecutive matches of one landmark share a
-  // row), problem layout (sloam.cpp:497-532; host/sloam.cpp add_*_matches)
+  // landmark tables (consecutive matches of one landmark share a row;
+  // host/sloam.cpp add_*_matches), built by the whole workgroup: the matched
+  // items in order, then a new row wherever an item's row differs from the
+  // previous matched item's (equal rows are equal to the run's first row, so
+  // this is the sequential scan's outcome)
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int i0 = 0; i0 < n_lm; i0 += kKB) {
+    const int l = i0 + tid;
+    const int idx = tile_compact(l < n_lm && b.lm_map[l] >= 0, wcount, &base);
+    if (idx >= 0) b.mlm[idx] = l;
+  }
+  const int nml = base;
+  __syncthreads();
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int i0 = 0; i0 < nml; i0 += kKB) {
+    const int i = i0 + tid;
+    bool fresh = false;
+    int j = 0;
+    if (i < nml) {
+      j = b.lm_map[b.mlm[i]];
+      fresh = i == 0 || !same_row(b.map + 8 * (size_t)b.lm_map[b.mlm[i - 1]],
+                                  b.map + 8 * (size_t)j, 7);
+    }
+    const int before = tile_scan(fresh, wcount, &base);
+    if (i < nml) {
+      const int row = fresh ? before : before - 1;
+      b.lm_tab[b.mlm[i]] = row;
+      if (fresh) {
+        double* dst = b.cyl + 8 * (size_t)row;
+        const double* src = b.map + 8 * (size_t)j;
+        for (int c = 0; c < 7; ++c) dst[c] = src[c];
+        dst[7] = 0.0;
+      }
+    }
+  }
+  const int ncyl = base;
+  __syncthreads();
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int i0 = 0; i0 < n_planes; i0 += kKB) {
+    const int p = i0 + tid;
+    const int idx = tile_compact(p < n_planes && b.pl_match[p] >= 0, wcount, &base);
+    if (idx >= 0) b.mpl[idx] = p;
+  }
+  const int nmp = base;
+  __syncthreads();
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int i0 = 0; i0 < nmp; i0 += kKB) {
+    const int i = i0 + tid;
+    bool fresh = false;
+    int j = 0;
+    if (i < nmp) {
+      j = b.pl_match[b.mpl[i]];
+      fresh = i == 0 || !same_row(b.prev + 8 * (size_t)b.pl_match[b.mpl[i - 1]],
+                                  b.prev + 8 * (size_t)j, 4);
+    }
+    const int before = tile_scan(fresh, wcount, &base);
+    if (i < nmp) {
+      const int row = fresh ? before : before - 1;
+      b.pl_tab[b.mpl[i]] = row;
+      if (fresh) {
+        double* dst = b.pl + 4 * (size_t)row;
+        const double* src = b.prev + 8 * (size_t)j;
+        for (int c = 0; c < 4; ++c) dst[c] = src[c];
+      }
+    }
+  }
+  const int npl = base;
+  __syncthreads();
+
+  // checks and problem layout (sloam.cpp:497-532)
   __shared__ int s_res_t, s_res_g;
   if (tid == 0) {
     int* h = b.head;
-    int nml = 0, ncyl = 0;
-    for (int l = 0; l < n_lm; ++l) {
-      const int j = b.lm_map[l];
-      if (j < 0) continue;
-      const double* row = b.map + 8 * (size_t)j;
-      if (ncyl == 0 || !same_row(b.cyl + 8 * (size_t)(ncyl - 1), row, 7)) {
-        double* dst = b.cyl + 8 * (size_t)ncyl;
-        for (int i = 0; i < 7; ++i) dst[i] = row[i];
-        dst[7] = 0.0;
-        ++ncyl;
-      }
-      b.lm_tab[l] = ncyl - 1;
-      b.mlm[nml++] = l;
-    }
-    int nmp = 0, npl = 0;
-    for (int p = 0; p < n_planes; ++p) {
-      const int j = b.pl_match[p];
-      if (j < 0) continue;
-      const double* row = b.prev + 8 * (size_t)j;
-      if (npl == 0 || !same_row(b.pl + 4 * (size_t)(npl - 1), row, 4)) {
-        double* dst = b.pl + 4 * (size_t)npl;
-        for (int i = 0; i < 4; ++i) dst[i] = row[i];
-        ++npl;
-      }
-      b.pl_tab[p] = npl - 1;
-      b.mpl[nmp++] = p;
-    }
     const int tm = nml * P.fpt, pm = nmp * P.nf;
     const bool treeCheck = (double)n_lm > P.minTreeModels && tm > 5 * P.fpt;
     const bool groundCheck = (double)pm > P.minGroundModels && (double)n_planes > P.minPlanes;
