@@ -38,8 +38,10 @@ constexpr int kGB = 256;
 constexpr double kPiDef = 3.14159265;  // PIDEF, sloam/include/helpers/definitions.h:28
 constexpr int kCentroidTile = 2048;
 
+// unsigned key in the order of the floats' <, -0 and +0 equal (so that their
+// ties keep input order, as std::stable_sort with < does)
 __device__ inline uint32_t orderable(float f) {
-  const uint32_t u = __float_as_uint(f);
+  const uint32_t u = __float_as_uint(f == 0.0f ? 0.0f : f);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 

@@ -74,3 +74,31 @@ def test_plane_ransac_bit_exact(ctx):
         assert np.array_equal(g[k], o[k]), k
     assert np.allclose(g["model"][:, :7], o["model"][:, :7], atol=1e-9, rtol=0)
     assert np.all(np.abs(g["model"][:, 7] - o["model"][:, 7]) <= 1)
+
+
+@pytest.mark.parametrize("n,bins", [(3000, (1, 1)), (30000, (1, 4)), (40000, (1, 1))],
+                         ids=["3000-one-cell", "30000-four-cells", "40000-one-cell"])
+def test_ground_planes_bottom_k_order(ctx, n, bins):
+    """The bottom-k order by (z, input order) of the (cell, z) radix sort.  z
+    is quantised to 1 cm so that many points tie, -0 and +0 among them (equal
+    under <, so input order decides); the kept points and the features must
+    come out in the oracle's order."""
+    rng = np.random.default_rng(n)
+    r = 25.0 * np.sqrt(rng.uniform(0.0, 1.0, n))
+    th = rng.uniform(-np.pi, np.pi, n)
+    pts = np.zeros((n, 4), np.float32)
+    pts[:, 0] = r * np.cos(th)
+    pts[:, 1] = r * np.sin(th)
+    pts[:, 2] = np.round(0.02 * pts[:, 0] + rng.normal(0.0, 0.05, n), 2)
+    prm = dict(radii_bins=bins[0], theta_bins=bins[1], min_dist=0.0, max_dist=30.0,
+               retain_thresh=0.05, num_features=60)
+    g = ground_planes(ctx, pts, **prm)
+    o = oracle.ground_planes(pts, **prm)
+    print(f"[ground] {n} points in {bins[0]} x {bins[1]} cells: kept "
+          f"{o['info'][:, 1].tolist()}", flush=True)
+    assert np.array_equal(g["info"], o["info"])
+    valid = o["info"][:, 0] == 1
+    assert valid.all()
+    assert np.array_equal(g["model"][valid, 4:7], o["model"][valid, 4:7])
+    assert np.allclose(g["model"][valid, :4], o["model"][valid, :4], atol=1e-9, rtol=0)
+    assert np.array_equal(g["features"][valid, :60], o["features"][valid, :60])
