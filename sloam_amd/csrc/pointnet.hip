@@ -758,13 +758,38 @@ __global__ __launch_bounds__(kFB) void pn_bwd_kernel(
   if (t < kC2) Gc[kB2 + t] = db2s[t] + db2s[t + kC2];
 }
 
-// sum over the batch of the per-cluster gradients (already scaled by 1/B)
-__global__ __launch_bounds__(256) void pn_reduce_kernel(const float* __restrict__ G, int B,
+// sum over the batch of the per-cluster gradients (already scaled by 1/B),
+// in two fixed-order levels: kRedSlices consecutive slices of the batch
+// (grid y), each summed in cluster order with four loads in flight, then the
+// slices in order
+constexpr int kRedSlices = 8;
+
+__global__ __launch_bounds__(256) void pn_reduce_part_kernel(const float* __restrict__ G, int B,
+                                                             float* __restrict__ part) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kNP) return;
+  const int sl = blockIdx.y;
+  const int c0 = (int)((int64_t)B * sl / kRedSlices), c1 = (int)((int64_t)B * (sl + 1) / kRedSlices);
+  float s = 0.f;
+  int c = c0;
+  for (; c + 4 <= c1; c += 4) {
+    const float v0 = G[(size_t)c * kNP + i], v1 = G[(size_t)(c + 1) * kNP + i];
+    const float v2 = G[(size_t)(c + 2) * kNP + i], v3 = G[(size_t)(c + 3) * kNP + i];
+    s += v0;
+    s += v1;
+    s += v2;
+    s += v3;
+  }
+  for (; c < c1; ++c) s += G[(size_t)c * kNP + i];
+  part[(size_t)sl * kNP + i] = s;
+}
+
+__global__ __launch_bounds__(256) void pn_reduce_kernel(const float* __restrict__ part,
                                                         float* __restrict__ grad) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= kNP) return;
   float s = 0.f;
-  for (int c = 0; c < B; ++c) s += G[(size_t)c * kNP + i];
+  for (int sl = 0; sl < kRedSlices; ++sl) s += part[(size_t)sl * kNP + i];
   grad[i] = s;
 }
 
@@ -803,6 +828,7 @@ struct PnBuffers {
   unsigned short* H1;
   unsigned short* H2;
   float* G;
+  float* part;  // kRedSlices partial sums of the gradient
   float* grad;
 };
 
@@ -814,7 +840,7 @@ int pn_buffers(sloam_ctx* ctx, int B, bool train, PnBuffers* b) {
   if (train)
     need += al(2 * (size_t)kP * kC1 * nB) +
             al(2 * (size_t)kP * kC2 * nB) + al(sizeof(float) * kNP * nB) +
-            al(sizeof(float) * kNP);
+            al(sizeof(float) * kNP * kRedSlices) + al(sizeof(float) * kNP);
   void* base = nullptr;
   const int st = sloam_ctx_scratch(ctx, 3, need, &base);
   if (st) return st;
@@ -832,6 +858,7 @@ int pn_buffers(sloam_ctx* ctx, int B, bool train, PnBuffers* b) {
     b->H1 = (unsigned short*)take(2 * (size_t)kP * kC1 * nB);
     b->H2 = (unsigned short*)take(2 * (size_t)kP * kC2 * nB);
     b->G = (float*)take(sizeof(float) * kNP * nB);
+    b->part = (float*)take(sizeof(float) * kNP * kRedSlices);
     b->grad = (float*)take(sizeof(float) * kNP);
   }
   return SLOAM_OK;
@@ -870,8 +897,10 @@ int pn_loss_grad(sloam_ctx* ctx, const float* d_params, const float* d_xyz,
   }
   {
     SloamKernelScope ks(ctx, "pn_reduce");
+    hipLaunchKernelGGL(pn_reduce_part_kernel, dim3((kNP + 255) / 256, kRedSlices), dim3(256), 0,
+                       ctx->stream, b->G, B, b->part);
     hipLaunchKernelGGL(pn_reduce_kernel, dim3((kNP + 255) / 256), dim3(256), 0, ctx->stream,
-                       b->G, B, d_grad ? d_grad : b->grad);
+                       b->part, d_grad ? d_grad : b->grad);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
