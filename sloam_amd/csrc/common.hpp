@@ -110,13 +110,17 @@ int sloam_fill_u32(sloam_ctx* ctx, uint32_t* d, size_t n, uint32_t value);
 
 // Stable LSD radix sort (radix.hip) of n (key, value) pairs by key bits
 // [begin_bit, end_bit), on the context stream, without memsets (graph-capturable).
-// keys / vals (nullable) hold the input and receive the result; keys_tmp /
-// vals_tmp are same-sized scratch; hist holds sloam_radix_hist_ints(n) ints.
-// Sorts by the key bits [begin_bit, end_bit); d_n (nullable) is a device-side
-// count that bounds the sort below n (the capacity the launches are sized for).
+// keys / vals (nullable) hold the input; keys_tmp / vals_tmp are same-sized
+// scratch; hist holds sloam_radix_hist_ints(n) ints.  Sorts by the key bits
+// [begin_bit, end_bit); d_n (nullable) is a device-side count that bounds the
+// sort below n (the capacity the launches are sized for).  With keys_out
+// (vals_out) given, they receive the buffers that hold the result, keys or
+// keys_tmp by the pass count's parity, and no copy is made; without, the
+// result is copied back into keys / vals.
 int sloam_radix_sort(sloam_ctx* ctx, unsigned long long* keys, int32_t* vals,
                      unsigned long long* keys_tmp, int32_t* vals_tmp, int n, int begin_bit,
-                     int end_bit, const int32_t* d_n, int32_t* hist);
+                     int end_bit, const int32_t* d_n, int32_t* hist,
+                     unsigned long long** keys_out = nullptr, int32_t** vals_out = nullptr);
 size_t sloam_radix_hist_ints(int n);
 
 // sloam_fit_lines over n_sets sets of which the first *d_n_sets (device,

@@ -663,14 +663,14 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
   dc.ox = cfg->origin[0];
   dc.oy = cfg->origin[1];
   const int cap = std::max(capacity, 1);
-  // scratch: 2 key arrays + their sort buffer, 2 value arrays + sort buffer,
-  // counts, offsets, sort histogram
+  // scratch: 2 key arrays + a sort buffer each, 2 value arrays + a sort
+  // buffer each, counts, offsets, sort histogram
   const size_t kb = sizeof(unsigned long long) * (size_t)cap;
   const size_t vb = sizeof(int32_t) * (size_t)cap;
   const size_t cb = sizeof(int32_t) * (size_t)(n_cells + 1);
   const size_t hb = sizeof(int32_t) * sloam_radix_hist_ints(cap);
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
-  const size_t need = 3 * al(kb) + 3 * al(vb) + 2 * al(cb) + al(hb);
+  const size_t need = 4 * al(kb) + 4 * al(vb) + 2 * al(cb) + al(hb);
   void* base = nullptr;
   int st = sloam_ctx_scratch(ctx, 1, need, &base);
   if (st) return st;
@@ -683,9 +683,14 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
   unsigned long long* kA = (unsigned long long*)take(kb);
   unsigned long long* kB = (unsigned long long*)take(kb);
   unsigned long long* kT = (unsigned long long*)take(kb);
+  unsigned long long* kT2 = (unsigned long long*)take(kb);
   int32_t* vA = (int32_t*)take(vb);
   int32_t* vB = (int32_t*)take(vb);
   int32_t* vT = (int32_t*)take(vb);
+  int32_t* vT2 = (int32_t*)take(vb);
+  // the two sorted value arrays (where each sort's last pass leaves them)
+  int32_t* by_idx = vA;
+  int32_t* by_z = vB;
   int32_t* counts = (int32_t*)take(cb);
   int32_t* offs = (int32_t*)take(cb);
   int32_t* hist = (int32_t*)take(hb);
@@ -705,9 +710,13 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
     // (cell, z) keys hold the orderable z in the low 32 bits
     int cell_bits = 1;
     while ((1 << cell_bits) <= n_cells) ++cell_bits;
-    st = sloam_radix_sort(ctx, kA, vA, kT, vT, capacity, 0, cell_bits, d_count, hist);
+    unsigned long long* sorted_keys = nullptr;
+    st = sloam_radix_sort(ctx, kA, vA, kT, vT, capacity, 0, cell_bits, d_count, hist, &sorted_keys,
+                          &by_idx);
     if (st) return st;
-    st = sloam_radix_sort(ctx, kB, vB, kT, vT, capacity, 0, 32 + cell_bits, d_count, hist);
+    st = sloam_radix_sort(ctx, kB, vB, kT2, vT2, capacity, 0, 32 + cell_bits, d_count, hist,
+                          &sorted_keys,
+                          &by_z);
     if (st) return st;
   }
   {
@@ -715,7 +724,7 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
     hipLaunchKernelGGL(ground_offsets_kernel, dim3(1), dim3(kGB), 0, ctx->stream, counts, n_cells,
                        offs);
     hipLaunchKernelGGL(ground_fit_kernel, dim3(n_cells), dim3(kGB), 0, ctx->stream,
-                       reinterpret_cast<const float4*>(d_points), offs, vA, vB, dc,
+                       reinterpret_cast<const float4*>(d_points), offs, by_idx, by_z, dc,
                        reinterpret_cast<int4*>(d_cell_info), d_cell_model,
                        reinterpret_cast<float4*>(d_features), d_kept_index, d_kept_offsets);
   }

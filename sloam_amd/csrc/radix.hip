@@ -281,7 +281,10 @@ size_t sloam_radix_hist_ints(int n) {
 
 int sloam_radix_sort(sloam_ctx* ctx, unsigned long long* keys, int32_t* vals,
                      unsigned long long* keys_tmp, int32_t* vals_tmp, int n, int begin_bit,
-                     int end_bit, const int32_t* d_n, int32_t* hist) {
+                     int end_bit, const int32_t* d_n, int32_t* hist,
+                     unsigned long long** keys_out, int32_t** vals_out) {
+  if (keys_out) *keys_out = keys;
+  if (vals_out) *vals_out = vals;
   if (n <= 1 || end_bit <= begin_bit) return SLOAM_OK;
   if (begin_bit < 0 || end_bit > 64 || !keys || !keys_tmp || !hist || (vals && !vals_tmp))
     return SLOAM_ERR_INVALID_ARG;
@@ -312,9 +315,14 @@ int sloam_radix_sort(sloam_ctx* ctx, unsigned long long* keys, int32_t* vals,
       std::swap(va, vb);
     }
   }
-  if (passes & 1)
+  if (keys_out) {
+    // the caller reads the sorted pairs where the last pass left them
+    *keys_out = ka;
+    if (vals_out) *vals_out = vals ? va : nullptr;
+  } else if (passes & 1) {
     hipLaunchKernelGGL(rs_copy_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, ka,
                        vals ? va : nullptr, d_n, n, keys, vals);
+  }
   SLOAM_HIP(ctx, hipGetLastError());
   return SLOAM_OK;
 }

@@ -649,6 +649,7 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
   (void)take(ib);
   unsigned long long* keys = (unsigned long long*)take(kb);
   unsigned long long* keys2 = (unsigned long long*)take(kb);
+  unsigned long long* skeys = keys;  // the sorted keys (keys or keys2)
   SegOut* seg_out = (SegOut*)take(sizeof(SegOut) * (size_t)n1);
   float4* tmp = (float4*)take(sizeof(float4) * (size_t)n1);
   int* bc = (int*)take(sizeof(int) * (n_blocks + 1));
@@ -683,28 +684,28 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
     {
       SloamKernelScope ks(ctx, "trellis_sort");
       st = sloam_radix_sort(ctx, keys, nullptr, keys2, nullptr, N, dc.rb + dc.cb, dc.key_bits,
-                            nullptr, (int32_t*)sort_buf);
+                            nullptr, (int32_t*)sort_buf, &skeys);
       if (st) return st;
     }
     {
       SloamKernelScope ks(ctx, "trellis_segments");
-      hipLaunchKernelGGL(tr_segcount_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, keys, N, dc,
+      hipLaunchKernelGGL(tr_segcount_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, skeys, N, dc,
                          bc);
       hipLaunchKernelGGL(tr_scan_kernel, dim3(1), dim3(kTB), 0, ctx->stream, bc, n_blocks,
                          counters + kCntSegs);
-      hipLaunchKernelGGL(tr_segscatter_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, keys,
+      hipLaunchKernelGGL(tr_segscatter_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, skeys,
                          N, dc, bc, seg_start);
     }
     {
       SloamKernelScope ks(ctx, "trellis_vertex");
       hipLaunchKernelGGL(tr_vertex_kernel, dim3(kVtxBlocks), dim3(64), 0, ctx->stream, cloud,
-                         keys, seg_start, counters, dc, seg_out, tmp);
+                         skeys, seg_start, counters, dc, seg_out, tmp);
     }
     SLOAM_HIP(ctx, hipGetLastError());
   }
   {
     SloamKernelScope ks(ctx, "trellis_assemble");
-    hipLaunchKernelGGL(tr_assemble_kernel, dim3(1), dim3(kAsm), 0, ctx->stream, keys, seg_start,
+    hipLaunchKernelGGL(tr_assemble_kernel, dim3(1), dim3(kAsm), 0, ctx->stream, skeys, seg_start,
                        seg_out, counters, dc, segP, compbase, comptotal, seg_vi, seg_po,
                        d_tree_offsets, d_vertex_int, d_vertex_f64, d_point_offsets, d_counts);
     if (N > 0)
