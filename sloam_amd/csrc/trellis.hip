@@ -92,10 +92,16 @@ __device__ inline float dist3(float ax, float ay, float az, float bx, float by, 
   return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
+// also zeroes the component sizes and the counters (no fill launches)
 __global__ __launch_bounds__(kTB) void tr_init_kernel(const float4* __restrict__ cloud, int N,
-                                                      int* __restrict__ L) {
+                                                      int* __restrict__ L, int* __restrict__ size,
+                                                      int* __restrict__ counters) {
   const int p = blockIdx.x * kTB + threadIdx.x;
-  if (p < N) L[p] = isfinite(cloud[p].x) ? p : -1;
+  if (p < N) {
+    L[p] = isfinite(cloud[p].x) ? p : -1;
+    size[p] = 0;
+  }
+  if (p < kCntN) counters[p] = 0;
 }
 
 __global__ __launch_bounds__(kTB) void tr_union_kernel(const float4* __restrict__ cloud,
@@ -659,15 +665,16 @@ int sloam_trellis(sloam_ctx* ctx, const float* d_cloud, int32_t height, int32_t 
   (void)bc2;
   int* label = d_labels ? d_labels : label_s;
   const float4* cloud = reinterpret_cast<const float4*>(d_cloud);
-  st = sloam_fill_u32(ctx, (uint32_t*)counters, kCntN, 0u);
-  if (st) return st;
-  if (N > 0) {
-    st = sloam_fill_u32(ctx, (uint32_t*)size, (size_t)N, 0u);
+  if (N == 0) {
+    st = sloam_fill_u32(ctx, (uint32_t*)counters, kCntN, 0u);
     if (st) return st;
+  }
+  if (N > 0) {
     const dim3 g((N + kTB - 1) / kTB);
     {
       SloamKernelScope ks(ctx, "trellis_cc");
-      hipLaunchKernelGGL(tr_init_kernel, g, dim3(kTB), 0, ctx->stream, cloud, N, L);
+      hipLaunchKernelGGL(tr_init_kernel, g, dim3(kTB), 0, ctx->stream, cloud, N, L, size,
+                         counters);
       hipLaunchKernelGGL(tr_union_kernel, g, dim3(kTB), 0, ctx->stream, cloud, dc, L);
       hipLaunchKernelGGL(tr_flatten_kernel, g, dim3(kTB), 0, ctx->stream, N, L, root);
       hipLaunchKernelGGL(tr_count_roots_kernel, dim3(n_blocks), dim3(kTB), 0, ctx->stream, root,
