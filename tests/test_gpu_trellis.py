@@ -69,6 +69,33 @@ def test_long_beams_and_ties(ctx):
     _same(g, o)
 
 
+@pytest.mark.parametrize("length", [161, 300, 512])
+def test_medians_of_long_beams_with_ties(ctx, length):
+    """Beams of 161-512 points staged in LDS: coordinates quantised to 1 cm so
+    that many values tie (index order decides the median), y crossing zero
+    and the lowest beam's z around zero, with both signs of zero among the
+    ties (equal under <)."""
+    H, W = 24, 2048  # more than min_vertices (16) beams make a tree
+    rng = np.random.default_rng(length)
+    cloud = np.full((H * W, 4), np.nan, np.float32)
+    for r in range(H):
+        c = np.arange(100, 100 + length)
+        x = np.round(6.0 + 0.0005 * c + rng.normal(0.0, 0.005, len(c)), 2)
+        y = np.round(0.01 * (c - c.mean()) / 10.0, 2)
+        z = np.round(rng.normal(0.1 * r, 0.02, len(c)), 2)
+        y[::7] = -0.0
+        if r == 0:
+            z[::5] = -0.0
+        idx = r * W + c
+        cloud[idx, 0], cloud[idx, 1], cloud[idx, 2] = x, y, z
+        cloud[idx, 3] = r
+    print(f"[trellis] {H} beams of {length} points", flush=True)
+    g = trellis(ctx, cloud, H, W, max_dist_to_centroid=0.5)
+    o = oracle.trellis(cloud, H, W, max_dist=0.5)
+    _same(g, o)
+    assert len(g["tree_offsets"]) > 1
+
+
 def test_empty_and_overflow(ctx):
     H, W = 8, 64
     cloud = np.full((H * W, 4), np.nan, np.float32)
