@@ -437,6 +437,23 @@ def bench_learned(ctx, args):
     ctx.sync()
     ref2 = pout2["model"].cpu().numpy()
     plain2.close()
+    # one training step of the network on the first 250 clusters with the
+    # plain fits as targets (forward saving activations, backward on MFMA,
+    # batch gradient, Adam), on a copy of the weights
+    B = min(250, S)
+    tnet = proposal.ProposalNet(ctx, proposal.load_weights())
+    txyz, toff, tgt = xyz[:int(d["offsets"][B])], off[:B + 1], pout["model"][:B].contiguous()
+    tloss = torch.empty(B, dtype=torch.float32, device=dev)
+    tcfg = proposal.train_cfg(lr=1e-4)
+    tnet.train_step(txyz, toff, tgt, tcfg, tloss)
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    ms_train = timed(lambda: tnet.train_step(txyz, toff, tgt, tcfg, tloss))
+    ctx.enable_timing(False)
+    train = {"clusters": B, "ms_per_step": ms_train}
+    for name in ("pn_prep", "pn_fwd", "pn_bwd", "pn_reduce"):
+        ms, k = ctx.kernel_time(name)
+        train[name + "_ms"] = ms / args.steps
     bins = bench_learned_size_bins(ctx, args, net, timed)
     ok = (ref[:, 6] > 0) & (lm[:, 6] > 0)
     err = np.abs(lm[ok, 6] - ref[ok, 6]) * 100
@@ -495,6 +512,7 @@ def bench_learned(ctx, args):
         "radius_err_vs_truth_cm": {"learned": vs_truth(lm), "plain": vs_truth(ref),
                                    "plain_other_seed": vs_truth(ref2)},
         "kernel_ms_per_batch": kern,
+        "train_step": train,
         "size_bins": bins,
         "cpu_baseline": cpu,
         "net_mfma": {"achieved": net_flop / fwd_s / 1e12, "peak": BF16_PEAK_TFLOPS,
