@@ -155,16 +155,21 @@ __device__ inline const float* cluster_prop(const float* prop, int c) {
   return q[7] > 0.0f ? q : nullptr;
 }
 
+// order (nullable): the cluster of each workgroup, -1 for padding (the plans'
+// largest-first order dealt to the XCDs as the scoring work list deals them,
+// so a cluster's points are read through the L2 its scoring will read them
+// through); without it workgroup b seeds cluster b
 __global__ __launch_bounds__(kBlock) void cyl_seed_kernel(const float* __restrict__ xyz,
                                                           const int32_t* __restrict__ offsets,
                                                           int n_clusters,
                                                           const int32_t* __restrict__ active,
                                                           const int32_t* __restrict__ bcnt,
-                                                          CylFrame* __restrict__ frames) {
+                                                          CylFrame* __restrict__ frames,
+                                                          const int32_t* __restrict__ order) {
   __shared__ float red_f[kWaves][6];
   __shared__ long long red_i[kWaves][9];
-  const int c = blockIdx.x;
-  if (c >= n_clusters || (bcnt && c >= bcnt[0])) return;
+  const int c = order ? order[blockIdx.x] : (int)blockIdx.x;
+  if (c < 0 || c >= n_clusters || (bcnt && c >= bcnt[0])) return;
   if (active && !active[c]) return;
   const int off = offsets[c];
   const int n = offsets[c + 1] - off;
@@ -1120,6 +1125,8 @@ struct sloam_cyl_plan {
   int n_rchunks = 0;
   int4* d_rchunks = nullptr;
   int4* d_rclus = nullptr;
+  int n_seed = 0;                    // seed workgroups (clusters + padding)
+  int32_t* d_seed_order = nullptr;   // their clusters (in d_tables)
   CylFrame* d_frames = nullptr;
   int32_t* d_counts = nullptr;  // used when the caller passes no counts buffer
   int32_t* d_rescue = nullptr;  // per cluster: refitted again by the plain method
@@ -1424,6 +1431,18 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
     }
     load[x] += (long long)std::max(len, 1) * chunks;
   }
+  // the seed: one workgroup per cluster, in each bin's order, bins interleaved
+  std::vector<std::vector<int32_t>> sbins(n_xcd);
+  for (int x = 0; x < n_xcd; ++x)
+    for (const int4& w : bins[x])
+      if (sbins[x].empty() || sbins[x].back() != w.x) sbins[x].push_back(w.x);
+  size_t slongest = 0;
+  for (const auto& b : sbins) slongest = std::max(slongest, b.size());
+  std::vector<int32_t> seed;
+  seed.reserve(slongest * n_xcd);
+  for (size_t k = 0; k < slongest; ++k)
+    for (int x = 0; x < n_xcd; ++x) seed.push_back(k < sbins[x].size() ? sbins[x][k] : -1);
+  p->n_seed = (int)seed.size();
   size_t longest = 0;
   for (const auto& b : bins) longest = std::max(longest, b.size());
   std::vector<int4> work;
@@ -1468,8 +1487,8 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
   for (int c = 0; c < n_clusters; ++c) max_chunks = std::max(max_chunks, rclus[c].y);
   p->fused = n_clusters <= kRefitFusedPlanClusters && max_chunks <= kRefitFusedPlanChunks;
   const size_t wb = sizeof(int4) * work.size(), cb = sizeof(int4) * rchunks.size(),
-               lb = sizeof(int4) * rclus.size();
-  int st = grow(ctx, &p->d_tables, &p->cap_tables, wb + cb + lb, 1);
+               lb = sizeof(int4) * rclus.size(), sb = sizeof(int32_t) * seed.size();
+  int st = grow(ctx, &p->d_tables, &p->cap_tables, wb + cb + lb + sb, 1);
   if (!st) st = grow(ctx, (void**)&p->d_frames, &p->cap_frames, n_clusters, sizeof(CylFrame));
   if (!st)
     st = grow(ctx, (void**)&p->d_counts, &p->cap_counts, (size_t)n_clusters * cfg.n_hyp,
@@ -1490,9 +1509,10 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
   p->d_work = static_cast<int4*>(p->d_tables);
   p->d_rchunks = reinterpret_cast<int4*>(static_cast<char*>(p->d_tables) + wb);
   p->d_rclus = reinterpret_cast<int4*>(static_cast<char*>(p->d_tables) + wb + cb);
+  p->d_seed_order = reinterpret_cast<int32_t*>(static_cast<char*>(p->d_tables) + wb + cb + lb);
   // the tables through the pinned staging buffer, in one copy (the caller
   // has made sure that no earlier upload from it is pending)
-  const size_t tb = wb + cb + lb;
+  const size_t tb = wb + cb + lb + sb;
   if (tb > p->cap_stage) {
     if (p->h_stage) (void)hipHostFree(p->h_stage);
     p->h_stage = nullptr;
@@ -1506,6 +1526,7 @@ static int plan_layout(sloam_cyl_plan* p, const int32_t* h_offsets, int32_t n_cl
   if (wb) std::memcpy(stage, work.data(), wb);
   if (cb) std::memcpy(stage + wb, rchunks.data(), cb);
   if (lb) std::memcpy(stage + wb + cb, rclus.data(), lb);
+  if (sb) std::memcpy(stage + wb + cb + lb, seed.data(), sb);
   if (tb) SLOAM_HIP(ctx, hipMemcpyAsync(p->d_tables, stage, tb, hipMemcpyHostToDevice, ctx->stream));
   return SLOAM_OK;
 }
@@ -1695,8 +1716,10 @@ int sloam_cyl_learned_exec(sloam_cyl_plan* verify, sloam_cyl_plan* plain, const 
   {
     // one axis seed per cluster serves both hypothesis sets
     SloamKernelScope ks(ctx, "cyl_seed");
-    hipLaunchKernelGGL(cyl_seed_kernel, dim3(S), dim3(kBlock), 0, ctx->stream, d_xyz, d_offsets,
-                       S, nullptr, vcnt, verify->d_frames);
+    const bool ord = !verify->counted && verify->n_seed > 0;
+    hipLaunchKernelGGL(cyl_seed_kernel, dim3(ord ? verify->n_seed : S), dim3(kBlock), 0,
+                       ctx->stream, d_xyz, d_offsets, S, nullptr, vcnt, verify->d_frames,
+                       ord ? verify->d_seed_order : nullptr);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (verify->n_work > 0) {
@@ -1887,8 +1910,10 @@ int sloam_cyl_plan_exec_ex(sloam_cyl_plan* p, const float* d_xyz, const int32_t*
   }
   {
     SloamKernelScope ks(ctx, "cyl_seed");
-    hipLaunchKernelGGL(cyl_seed_kernel, dim3(p->n_clusters), dim3(kBlock), 0, ctx->stream, d_xyz,
-                       d_offsets, p->n_clusters, d_active, cnt, p->d_frames);
+    const bool ord = !p->counted && p->n_seed > 0;
+    hipLaunchKernelGGL(cyl_seed_kernel, dim3(ord ? p->n_seed : p->n_clusters), dim3(kBlock), 0,
+                       ctx->stream, d_xyz, d_offsets, p->n_clusters, d_active, cnt, p->d_frames,
+                       ord ? p->d_seed_order : nullptr);
   }
   SLOAM_HIP(ctx, hipGetLastError());
   if (p->n_work > 0) {
