@@ -1139,6 +1139,25 @@ int dump(const char* path) {
   return 0;
 }
 
+// the pose solves' arithmetic setting: FP64 by default, 0 or 1 accepted,
+// anything else refused without a change
+void pose_precision_setting() {
+  sloam::sloam s;
+  EXPECT_EQ(s.posePrecision(), 1.0);
+  s.setPosePrecision(0);
+  EXPECT_EQ(s.posePrecision(), 0.0);
+  bool threw = false;
+  try {
+    s.setPosePrecision(2);
+  } catch (const std::invalid_argument&) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  EXPECT_EQ(s.posePrecision(), 0.0);
+  s.setPosePrecision(1);
+  EXPECT_EQ(s.posePrecision(), 1.0);
+}
+
 struct Case {
   const char* name;
   bool gpu;
@@ -1240,6 +1259,7 @@ int main(int argc, char** argv) {
       {"MapManager", false, map_manager},
       {"PCD.ReadWrite", false, pcd_io},
       {"ROSMsg.WireFormat", false, ros_messages},
+      {"Sloam.PosePrecision", false, pose_precision_setting},
       {"PlaneTest.Initalizes", true, plane_initializes},
       {"PlaneTest.DistanceToFeature", true, plane_distance_to_feature},
       {"PlaneTest.TranslateModel", true, plane_translate_model},
