@@ -157,9 +157,12 @@ __device__ inline void gsum(double (&v)[K], double* lds) {
   __syncthreads();
 }
 
-// info: {valid, kept, binned, 0}; model: plane[4], centroid[3], unused
+// info: {valid, kept, binned, 0}; model: plane[4], centroid[3], unused.
+// offsets (nullable): the cells' exclusive prefix of counts; without it
+// (few cells) each workgroup adds the counts of the cells before its own.
 __global__ __launch_bounds__(kGB) void ground_fit_kernel(
-    const float4* __restrict__ pts, const int32_t* __restrict__ offsets,
+    const float4* __restrict__ pts, const int32_t* __restrict__ counts,
+    const int32_t* __restrict__ offsets,
     const int32_t* __restrict__ by_idx, const int32_t* __restrict__ by_z, GroundCfgDev cfg,
     int4* __restrict__ info, double* __restrict__ model, float4* __restrict__ features,
     int32_t* __restrict__ kept_index, int32_t* __restrict__ kept_offsets) {
@@ -167,8 +170,15 @@ __global__ __launch_bounds__(kGB) void ground_fit_kernel(
   __shared__ double red[(kGB / 64) * 6];
   __shared__ float cen[3];
   const int c = blockIdx.x;
-  const int off = offsets[c];
-  const int s = offsets[c + 1] - off;
+  int off, s;
+  if (offsets) {
+    off = offsets[c];
+    s = offsets[c + 1] - off;
+  } else {
+    off = 0;
+    for (int i = 0; i < c; ++i) off += counts[i];
+    s = counts[c];
+  }
   int k = s;
   const int32_t* list = by_idx + off;
   if (cfg.retain_num < (double)s) {
@@ -721,10 +731,14 @@ int sloam_ground_planes(sloam_ctx* ctx, const float* d_points, int32_t capacity,
   }
   {
     SloamKernelScope ks(ctx, "ground_fit");
-    hipLaunchKernelGGL(ground_offsets_kernel, dim3(1), dim3(kGB), 0, ctx->stream, counts, n_cells,
-                       offs);
+    // a few hundred cells: each fit workgroup adds the counts before its cell
+    const bool prefix = n_cells > kGB;
+    if (prefix)
+      hipLaunchKernelGGL(ground_offsets_kernel, dim3(1), dim3(kGB), 0, ctx->stream, counts,
+                         n_cells, offs);
     hipLaunchKernelGGL(ground_fit_kernel, dim3(n_cells), dim3(kGB), 0, ctx->stream,
-                       reinterpret_cast<const float4*>(d_points), offs, by_idx, by_z, dc,
+                       reinterpret_cast<const float4*>(d_points), counts,
+                       prefix ? offs : nullptr, by_idx, by_z, dc,
                        reinterpret_cast<int4*>(d_cell_info), d_cell_model,
                        reinterpret_cast<float4*>(d_features), d_kept_index, d_kept_offsets);
   }

@@ -21,6 +21,10 @@ PARAMS = {
     # params/sloam.yaml: 2 x 18 bins, 5..25 m, retain 0.05, 5 features
     "sloam_yaml": dict(radii_bins=2, theta_bins=18, min_dist=5.0, max_dist=25.0,
                        retain_thresh=0.05, num_features=5),
+    # more cells than a workgroup has threads: the cells' offsets come from a
+    # prefix kernel instead of each fit workgroup's own sum
+    "many_cells": dict(radii_bins=16, theta_bins=36, min_dist=0.0, max_dist=30.0,
+                       retain_thresh=0.05, num_features=5),
 }
 
 
